@@ -1,0 +1,156 @@
+// epi_dev.h -- device-side data layout and the scalar laws of the epi5_MT hot
+// path, for CDNA4 (gfx950).
+//
+// Layout: one Tis descriptor per tissue in device memory; kernels take the
+// descriptor array and use blockIdx.y as the tissue index, so a single
+// launch advances every tissue of a batch.  Cells are SoA, neighbour lists
+// are fixed-stride rows (stride = ns entries per cell), grids are stored
+// z-major / x-fastest with a padded x pitch.
+//
+// Every FP64 expression keeps the operand order of the reference so that a
+// build without FP contraction (-ffp-contract=off) reproduces it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace epi {
+
+// cell states, epi5_MT/define.h:36-47
+enum : uint8_t { ALIVE = 0, DEAD = 1, DISA = 2, UNUSED = 3, FIX = 4, BLANK = 5,
+                 DER = 6, MUSUME = 7, AIR = 8, MEMB = 9 };
+
+// status codes, mirrored in include/epi_amd.h
+enum { OK = 0, ERR_CAPACITY = 1, ERR_STRONG_INTERACTION = 2, ERR_NO_DERMIS = 3,
+       ERR_OUT_OF_GRID = 4, ERR_REMOVE_ALIAS = 5, ERR_RNG_EXHAUSTED = 6,
+       ERR_INVALID = 7, ERR_HIP = 8, ERR_PAIR_REMOVED = 9 };
+
+// ---- constants: define.h:69-197 and the physics files -------------------
+constexpr double DT_Cell = 0.01;           // define.h:86
+constexpr double DT_Ca = 0.01;             // define.h:89
+constexpr double R_max = 1.4;              // define.h:116
+constexpr double R_der = 1.4;              // define.h:119
+constexpr double COMPRESS_FACTOR = 4;      // define.h:125
+constexpr double THRESH_SP = 3.0;          // define.h:132
+constexpr double LJ_THRESH = 1.2;          // define.h:135
+constexpr int MEMB_ADJ = 6;                // define.h:147-153
+constexpr double THRESH_DEAD = 22.0;       // define.h:156
+constexpr double ca2p_init = 0.122;        // define.h:165
+constexpr double gj_init = 0.99;           // define.h:174
+constexpr double FAC_MAP = 2.0;            // define.h:184
+constexpr int SW_THRESH = 20;              // define.h:187
+constexpr int MAX_CONN = 400;              // define.h:80, cell_connection.cpp:30
+constexpr int N3 = 200;                    // cell_connection.cpp:28
+constexpr double AREA_GRID = 2.0 + 1e-7;   // cell_connection.cpp:16-19
+constexpr int CA_ITR = (int)(10.0 / DT_Ca);  // ca2p.cpp:14
+
+// cell_interaction.cpp:21-45
+constexpr double eps_m = 0.01;
+constexpr double P_MEMB = 1. / COMPRESS_FACTOR;
+constexpr double DER_DER_CONST = 0.2;
+constexpr double K_TOTAL = 3.0;
+constexpr double K_DESMOSOME = K_TOTAL * 0.01;
+constexpr double Kspring = 25.0;
+constexpr double Kspring_d = 5.0;
+constexpr double KBEND = 0.5;
+constexpr double delta_R = 0.4 * R_der;
+constexpr double para_ljp2 = 0.005;
+constexpr double Kspring_division = 5.0;
+
+// force laws (cell_interaction.cpp:68-76)
+enum : int { LAW_NONE = 0, LAW_OTHER, LAW_DER_DER, LAW_FIXMU, LAW_FIX_MEMB,
+             LAW_MU_MEMB, LAW_ALAIRDE, LAW_MEMB_MEMB, LAW_PAIR };
+
+// ---- per-tissue device status (changes on device) ---------------------
+struct Status {
+  int n;                  // live cells
+  int n_old;              // cells before the last state renewal
+  int err;                // first error code
+  int sw;                 // cornifications since the last Ca solve (uint8 wrap)
+  int ndiv;               // divisions in the last renewal
+  int nrm;                // removals in the last renewal
+  int maxlen;             // longest neighbour list of the last build
+  int pad0;
+  long long rng_pos;      // replay cursor
+  long long next_uid;
+  unsigned long long zzmax_bits;  // calc_zzmax result (non-negative double bits)
+  double zzmax;
+};
+
+// ---- per-tissue descriptor (constant between allocations) ---------------
+struct Tis {
+  // geometry
+  int nmemb, nder, cap, ns;   // ns = neighbour row stride
+  int NX, NY, NZ, XP;         // XP = padded x pitch of grid rows
+  long long PL;               // plane pitch (XP*(NY+1))
+  long long gsize;            // XP*(NY+1)*(NZ+1)
+  double LX, LY, LZ, dx, dy, dz, inv_dx, inv_dy, inv_dz;
+  int ANX, ANY, ANZ, bcap;
+  int irx, iry, irz, pad1;
+  Status* st;
+  // cells
+  uint8_t* state; uint8_t* is_malignant; uint8_t* touch_air;
+  int* fix_origin; int* rest_div_times; int* pair; int* dermis; int* lat;  // lat: 3/cell
+  double* pos[2][3];          // [buffer][axis]
+  double *radius, *div_age_thresh;
+  double* ca2p[2]; double* ip3[2];
+  double *ca2p_avg, *ex_inert, *agek, *ageb, *ex_fat, *in_fat, *spr_nat_len, *diff_u;
+  long long* uid; long long* old_uid; int* src_row;
+  // neighbour rows (two buffers: current and previous build)
+  int* nlen[2]; int* col[2]; double* gj[2];
+  int* lo_sorted;             // part of a row below the cell index, ascending
+  int* nlo;                   // entries of the row below the cell index
+  int* btmp; int* bcnt;       // upper partners gathered during the sweep
+  int* memb6;                 // triangular adjacency, 6 per membrane cell
+  // buckets
+  int* bucket_of; int* bucket_n; int* bucket_cells;
+  // state renewal
+  int* div_slot; double* div_dir;  // per division: parent, direction
+  int* div_parent; int* rm_list;
+  // grids
+  double* atp[2]; double* bf[2];
+  int* cmap1; uint8_t* cmap2; unsigned* vox;
+  // replay stream
+  const double* rng; long long rng_n;
+};
+
+__device__ __forceinline__ void set_err(Status* st, int e) { atomicCAS(&st->err, 0, e); }
+
+// ---- geometry helpers: utils.h:25-53 ----------------------------------
+__device__ __forceinline__ double p_diff_x(const Tis& t, double x1, double x2) {
+  const double diff = x1 - x2;
+  if (diff > 0.5 * t.LX) return diff - t.LX;
+  if (diff <= -0.5 * t.LX) return diff + t.LX;
+  return diff;
+}
+__device__ __forceinline__ double p_diff_y(const Tis& t, double y1, double y2) {
+  const double diff = y1 - y2;
+  if (diff > 0.5 * t.LY) return diff - t.LY;
+  if (diff <= -0.5 * t.LY) return diff + t.LY;
+  return diff;
+}
+__device__ __forceinline__ double min0(double a) { return a > 0 ? a : 0; }
+
+#define POW6(x) ((x) * (x) * (x) * (x) * (x) * (x))
+#define POW3(x) ((x) * (x) * (x))
+
+struct P3 { double x, y, z; };
+
+__device__ __forceinline__ P3 ld_pos(const Tis& t, int pb, int c) {
+  return P3{t.pos[pb][0][c], t.pos[pb][1][c], t.pos[pb][2][c]};
+}
+
+__device__ __forceinline__ double dist_sq(const Tis& t, P3 a, P3 b) {
+  const double diffx = p_diff_x(t, a.x, b.x);
+  const double diffy = p_diff_y(t, a.y, b.y);
+  const double diffz = a.z - b.z;
+  return diffx * diffx + diffy * diffy + diffz * diffz;
+}
+
+// grid index of (x, y, z) lattice point in the device layout
+__device__ __forceinline__ long long gidx(const Tis& t, int j, int k, int l) {
+  return (long long)l * t.PL + (long long)k * t.XP + j;
+}
+
+__device__ __forceinline__ bool is_afm(uint8_t s) { return s == ALIVE || s == FIX || s == MUSUME; }
+
+}  // namespace epi

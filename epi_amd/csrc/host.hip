@@ -1,0 +1,975 @@
+// host.hip -- context, memory, the C ABI (include/epi_amd.h) and the step
+// driver of proc.cpp:131-173.
+//
+// A context owns one device arena holding every tissue; per-tissue
+// descriptors (Tis) live in device memory once, so each kernel launch covers
+// all tissues (blockIdx.y).  Double buffers are selected by parities kept on
+// the host and passed as kernel arguments; all tissues of a context step in
+// lock-step, so the parities are shared.  One status read-back per step
+// feeds the Ca gating (proc.cpp:158-172) and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/epi_amd.h"
+#include "epi_dev.h"
+#include "launch.h"
+
+using namespace epi;
+
+namespace {
+
+struct TH {
+  epi_geometry g{};
+  int nmemb = 0, nder = 0, cap = 0, ns = 0, bcap = 0;
+  int NX = 0, NY = 0, NZ = 0, XP = 0;
+  long long PL = 0, gsize = 0;
+  int nbuckets = 0;
+  int n_est = 0;
+  double* rng_d = nullptr;
+  long long rng_n = 0;
+  bool loaded = false;
+};
+
+}  // namespace
+
+struct epi_ctx {
+  int T = 0;
+  std::vector<TH> th;
+  std::vector<Tis> tis;
+  Tis* d_tis = nullptr;
+  Status* d_st = nullptr;
+  Status* h_st = nullptr;
+  double* h_zz = nullptr;
+  int* d_tlist = nullptr;
+  int* h_tlist = nullptr;
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  hipStream_t stream = nullptr;
+  int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
+  bool pos_pending = false;
+  std::vector<long long> ca_solves;
+  std::string err;
+};
+
+namespace {
+
+int fail(epi_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                               \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ctx, EPI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  bool dry;
+  template <class T>
+  T* take(size_t count) {
+    off = align_up(off, 256);
+    T* p = dry ? nullptr : reinterpret_cast<T*>(base + off);
+    off += sizeof(T) * (count ? count : 1);
+    return p;
+  }
+};
+
+void carve_tissue(Carver& cv, TH& h, Tis& t) {
+  const size_t cap = h.cap, ns = h.ns;
+  t.state = cv.take<uint8_t>(cap);
+  t.is_malignant = cv.take<uint8_t>(cap);
+  t.touch_air = cv.take<uint8_t>(cap);
+  t.fix_origin = cv.take<int>(cap);
+  t.rest_div_times = cv.take<int>(cap);
+  t.pair = cv.take<int>(cap);
+  t.dermis = cv.take<int>(cap);
+  t.lat = cv.take<int>(3 * cap);
+  for (int b = 0; b < 2; b++)
+    for (int a = 0; a < 3; a++) t.pos[b][a] = cv.take<double>(cap);
+  t.radius = cv.take<double>(cap);
+  t.div_age_thresh = cv.take<double>(cap);
+  for (int b = 0; b < 2; b++) { t.ca2p[b] = cv.take<double>(cap); t.ip3[b] = cv.take<double>(cap); }
+  t.ca2p_avg = cv.take<double>(cap);
+  t.ex_inert = cv.take<double>(cap);
+  t.agek = cv.take<double>(cap);
+  t.ageb = cv.take<double>(cap);
+  t.ex_fat = cv.take<double>(cap);
+  t.in_fat = cv.take<double>(cap);
+  t.spr_nat_len = cv.take<double>(cap);
+  t.diff_u = cv.take<double>(cap);
+  t.uid = cv.take<long long>(cap);
+  t.old_uid = cv.take<long long>(cap);
+  t.src_row = cv.take<int>(cap);
+  for (int b = 0; b < 2; b++) {
+    t.nlen[b] = cv.take<int>(cap);
+    t.col[b] = cv.take<int>(cap * ns);
+    t.gj[b] = cv.take<double>(cap * ns);
+  }
+  t.lo_sorted = cv.take<int>(cap * ns);
+  t.nlo = cv.take<int>(cap);
+  t.btmp = cv.take<int>(cap * ns);
+  t.bcnt = cv.take<int>(cap);
+  t.memb6 = cv.take<int>((size_t)h.nmemb * 6);
+  t.bucket_of = cv.take<int>(cap);
+  t.bucket_n = cv.take<int>(h.nbuckets);
+  t.bucket_cells = cv.take<int>((size_t)h.nbuckets * h.bcap);
+  t.div_slot = cv.take<int>(cap);
+  t.div_dir = cv.take<double>(3 * cap);
+  t.div_parent = cv.take<int>(cap);
+  t.rm_list = cv.take<int>(cap);
+  for (int b = 0; b < 2; b++) { t.atp[b] = cv.take<double>(h.gsize); t.bf[b] = cv.take<double>(h.gsize); }
+  t.cmap1 = cv.take<int>(h.gsize);
+  t.cmap2 = cv.take<uint8_t>(h.gsize);
+  t.vox = cv.take<unsigned>(h.gsize);
+}
+
+int check_tissue(epi_ctx* c, int t) {
+  if (!c) return EPI_ERR_INVALID;
+  if (t < 0 || t >= c->T) return fail(c, EPI_ERR_INVALID, "tissue index out of range");
+  return EPI_OK;
+}
+
+int sync_status(epi_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(Status) * c->T, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < c->T; t++) c->th[t].n_est = c->h_st[t].n;
+  for (int t = 0; t < c->T; t++)
+    if (c->h_st[t].err) {
+      char buf[128];
+      snprintf(buf, sizeof buf, "tissue %d: device error code %d", t, c->h_st[t].err);
+      return fail(c, c->h_st[t].err, buf);
+    }
+  return EPI_OK;
+}
+
+int put_status(epi_ctx* c, int t) {
+  HIPCHK(c, hipMemcpyAsync(c->d_st + t, c->h_st + t, sizeof(Status), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EPI_OK;
+}
+
+// launch bounds over all tissues
+int max_n(const epi_ctx* c) {
+  int m = 1;
+  for (auto& h : c->th) m = std::max(m, h.n_est + 64);
+  return m;
+}
+int max_memb(const epi_ctx* c) {
+  int m = 0;
+  for (auto& h : c->th) m = std::max(m, h.nmemb);
+  return m;
+}
+int max_nb(const epi_ctx* c) {
+  int m = 1;
+  for (auto& h : c->th) m = std::max(m, h.nbuckets);
+  return m;
+}
+long long max_gsize(const epi_ctx* c) {
+  long long m = 1;
+  for (auto& h : c->th) m = std::max(m, h.gsize);
+  return m;
+}
+int max_halo(const epi_ctx* c) {
+  int m = 1;
+  for (auto& h : c->th) m = std::max(m, (h.NX + 1 + h.NY + 1) * h.NZ);
+  return m;
+}
+int max_nx(const epi_ctx* c) {
+  int m = 1;
+  for (auto& h : c->th) m = std::max(m, h.NX);
+  return m;
+}
+int max_ny(const epi_ctx* c) {
+  int m = 1;
+  for (auto& h : c->th) m = std::max(m, h.NY);
+  return m;
+}
+// z planes the stencils will likely touch: from the last known zzmax
+int izb_est(const epi_ctx* c, bool ca) {
+  int m = 1;
+  for (int t = 0; t < c->T; t++) {
+    const Tis& ti = c->tis[t];
+    double zz = c->h_st ? c->h_st[t].zzmax : 0.0;
+    int v = ca ? (int)((zz + FAC_MAP * R_max) / ti.dz) : (int)((zz + FAC_MAP * R_max) * ti.inv_dz);
+    m = std::max(m, std::min(v + 2, c->th[t].NZ + 1));
+  }
+  return m;
+}
+
+long long dev_index(const TH& h, int j, int k, int l) { return (long long)l * h.PL + (long long)k * h.XP + j; }
+
+// [x][y][z] (z fastest) <-> device layout
+void to_device_layout(const TH& h, const double* src, std::vector<double>& dst) {
+  dst.assign(h.gsize, 0.0);
+  for (int j = 0; j <= h.NX; j++)
+    for (int k = 0; k <= h.NY; k++)
+      for (int l = 0; l <= h.NZ; l++)
+        dst[dev_index(h, j, k, l)] = src[((size_t)j * (h.NY + 1) + k) * (h.NZ + 1) + l];
+}
+template <class T>
+void from_device_layout(const TH& h, const std::vector<T>& src, T* dst) {
+  for (int j = 0; j <= h.NX; j++)
+    for (int k = 0; k <= h.NY; k++)
+      for (int l = 0; l <= h.NZ; l++)
+        dst[((size_t)j * (h.NY + 1) + k) * (h.NZ + 1) + l] = src[dev_index(h, j, k, l)];
+}
+
+template <class T>
+int upload(epi_ctx* c, T* dst, const T* src, size_t n) {
+  if (!src || !n) return EPI_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EPI_OK;
+}
+template <class T>
+int download(epi_ctx* c, T* dst, const T* src, size_t n) {
+  if (!dst || !n) return EPI_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EPI_OK;
+}
+
+#define RCHK(x)                  \
+  do {                           \
+    int r_ = (x);                \
+    if (r_ != EPI_OK) return r_; \
+  } while (0)
+
+int set_zz(epi_ctx* c, const double* zzmax) {
+  if (!zzmax) return EPI_OK;
+  for (int t = 0; t < c->T; t++) {
+    c->h_zz[t] = zzmax[t];
+    HIPCHK(c, hipMemcpyAsync(&c->d_st[t].zzmax_bits, &c->h_zz[t], 8, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < c->T; t++) c->h_st[t].zzmax = zzmax[t];
+  return EPI_OK;
+}
+
+// calc_ca2p (ca2p.cpp:380-402) for the tissues in tlist (nullptr = all)
+int run_ca(epi_ctx* c, const int* d_tlist, int ntis) {
+  const int mn = max_n(c);
+  const int izb = izb_est(c, true);
+  long long box = 1;
+  for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * std::min(izb, h.NZ + 1));
+  launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
+  for (int s = 0; s < CA_ITR; s++) {
+    launch_ca_substep(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c), izb);
+    c->cp ^= 1;
+    c->ap ^= 1;
+  }
+  launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
+  HIPCHK(c, hipGetLastError());
+  return EPI_OK;
+}
+
+void memb_init_host(int nmx, int nmemb, std::vector<int>& m) {   // cellmanager.cpp:66-113
+  m.assign((size_t)nmemb * 6, 0);
+  for (int j = 0; j < nmemb; j++) {
+    int jj = j % nmx;
+    m[6 * j + 3] = (jj == 0) ? j + nmx - 1 : j - 1;
+    m[6 * j + 0] = (jj == nmx - 1) ? j - (nmx - 1) : j + 1;
+  }
+  for (int j = 0; j < nmemb; j++) {
+    int kk = j / nmx;
+    int top = (j - nmx + nmemb) % nmemb;
+    int bot = (j + nmx) % nmemb;
+    if (kk % 2 == 0) {
+      m[6 * j + 1] = top;
+      m[6 * j + 2] = m[6 * top + 3];
+      m[6 * j + 4] = m[6 * bot + 3];
+      m[6 * j + 5] = bot;
+    } else {
+      m[6 * j + 1] = m[6 * top + 0];
+      m[6 * j + 2] = top;
+      m[6 * j + 4] = bot;
+      m[6 * j + 5] = m[6 * bot + 0];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
+  if (!out || ntissues <= 0 || !geos) return EPI_ERR_INVALID;
+  *out = nullptr;
+  epi_ctx* c = new epi_ctx();
+  c->T = ntissues;
+  c->th.resize(ntissues);
+  c->tis.resize(ntissues);
+  c->ca_solves.assign(ntissues, 0);
+  for (int t = 0; t < ntissues; t++) {
+    const epi_geometry& g = geos[t];
+    TH& h = c->th[t];
+    Tis& ti = c->tis[t];
+    if (g.nmx < 3 || g.nmy < 4 || (g.nmy % 2) || g.nx < 2 || g.ny < 2 || g.nz < 2 || g.max_cells <= g.nmx * g.nmy ||
+        !(g.lx > 0) || !(g.ly > 0) || !(g.lz > 0)) {
+      delete c;
+      return EPI_ERR_INVALID;
+    }
+    h.g = g;
+    h.nmemb = g.nmx * g.nmy;
+    h.cap = g.max_cells;
+    h.ns = g.row_stride > 0 ? std::min(g.row_stride, MAX_CONN) : MAX_CONN;
+    h.bcap = g.bucket_cap > 0 ? std::min(g.bucket_cap, N3) : N3;
+    h.NX = g.nx; h.NY = g.ny; h.NZ = g.nz;
+    h.XP = (int)align_up(g.nx + 1, 16);
+    h.PL = (long long)h.XP * (g.ny + 1);
+    h.gsize = h.PL * (g.nz + 1);
+    memset(&ti, 0, sizeof(Tis));
+    ti.nmemb = h.nmemb; ti.nder = 0; ti.cap = h.cap; ti.ns = h.ns;
+    ti.NX = g.nx; ti.NY = g.ny; ti.NZ = g.nz; ti.XP = h.XP; ti.PL = h.PL; ti.gsize = h.gsize;
+    ti.LX = g.lx; ti.LY = g.ly; ti.LZ = g.lz;
+    // define.h:111-113
+    ti.dx = g.lx / g.nx; ti.inv_dx = g.nx / g.lx;
+    ti.dy = g.ly / g.ny; ti.inv_dy = g.ny / g.ly;
+    ti.dz = g.lz / g.nz; ti.inv_dz = g.nz / g.lz;
+    // cell_connection.cpp:22-26
+    ti.ANX = (int)((double)g.lx / 2.0 + 0.5);
+    ti.ANY = (int)((double)g.ly / 2.0 + 0.5);
+    ti.ANZ = (int)((double)g.lz / 2.0);
+    if (ti.ANX < 5 || ti.ANY < 5 || ti.ANZ < 5) { delete c; return EPI_ERR_INVALID; }
+    h.nbuckets = ti.ANX * ti.ANY * ti.ANZ;
+    ti.bcap = h.bcap;
+    // map.cpp:27-31
+    const double Rad = 1.4;
+    ti.irx = (int)(Rad * g.nx / g.lx);
+    ti.iry = (int)(Rad * g.ny / g.ly);
+    ti.irz = (int)(Rad * g.nz / g.lz);
+  }
+  // arena
+  Carver dry{nullptr, 0, true};
+  for (int t = 0; t < ntissues; t++) carve_tissue(dry, c->th[t], c->tis[t]);
+  c->arena_bytes = align_up(dry.off, 256);
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return EPI_ERR_HIP; }
+  if (hipMalloc(&c->arena, c->arena_bytes) != hipSuccess ||
+      hipMalloc(&c->d_tis, sizeof(Tis) * ntissues) != hipSuccess ||
+      hipMalloc(&c->d_st, sizeof(Status) * ntissues) != hipSuccess ||
+      hipMalloc(&c->d_tlist, sizeof(int) * ntissues) != hipSuccess ||
+      hipHostMalloc(&c->h_st, sizeof(Status) * ntissues, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->h_zz, sizeof(double) * ntissues, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->h_tlist, sizeof(int) * ntissues, hipHostMallocDefault) != hipSuccess) {
+    epi_destroy(c);
+    return EPI_ERR_HIP;
+  }
+  if (hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->d_st, 0, sizeof(Status) * ntissues, c->stream) != hipSuccess) {
+    epi_destroy(c);
+    return EPI_ERR_HIP;
+  }
+  memset(c->h_st, 0, sizeof(Status) * ntissues);
+  Carver real{c->arena, 0, false};
+  for (int t = 0; t < ntissues; t++) {
+    carve_tissue(real, c->th[t], c->tis[t]);
+    c->tis[t].st = c->d_st + t;
+  }
+  if (hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * ntissues, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    epi_destroy(c);
+    return EPI_ERR_HIP;
+  }
+  // cmap1 starts empty (-1)
+  std::vector<int> neg;
+  for (int t = 0; t < ntissues; t++) {
+    neg.assign(c->th[t].gsize, -1);
+    if (hipMemcpy(c->tis[t].cmap1, neg.data(), sizeof(int) * neg.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      epi_destroy(c);
+      return EPI_ERR_HIP;
+    }
+  }
+  *out = c;
+  return EPI_OK;
+}
+
+void epi_destroy(epi_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& h : c->th)
+    if (h.rng_d) (void)hipFree(h.rng_d);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->d_tis) (void)hipFree(c->d_tis);
+  if (c->d_st) (void)hipFree(c->d_st);
+  if (c->d_tlist) (void)hipFree(c->d_tlist);
+  if (c->h_st) (void)hipHostFree(c->h_st);
+  if (c->h_zz) (void)hipHostFree(c->h_zz);
+  if (c->h_tlist) (void)hipHostFree(c->h_tlist);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* epi_last_error(const epi_ctx* c) { return c ? c->err.c_str() : "null context"; }
+int32_t epi_num_tissues(const epi_ctx* c) { return c ? c->T : 0; }
+
+int epi_load_cells(epi_ctx* c, int32_t t, const epi_cell_input* in) {
+  RCHK(check_tissue(c, t));
+  TH& h = c->th[t];
+  Tis& ti = c->tis[t];
+  if (!in || in->n <= 0 || in->n > h.cap) return fail(c, EPI_ERR_CAPACITY, "cell count exceeds capacity");
+  if (in->nmemb != h.nmemb) return fail(c, EPI_ERR_INVALID, "nmemb must equal nmx*nmy (cellmanager.cpp:318-321)");
+  const int n = in->n;
+  // phase order MEMB, DER, rest (cellmanager.cpp:266-282)
+  for (int i = 0; i < n; i++) {
+    const uint8_t s = in->state[i];
+    const bool want_memb = i < in->nmemb, want_der = !want_memb && i < in->nmemb + in->nder;
+    if ((s == MEMB) != want_memb || (s == DER) != want_der) return fail(c, EPI_ERR_INVALID, "rows must be MEMB, then DER, then the rest");
+  }
+  std::vector<uint8_t> st(n), mal(n);
+  std::vector<int> fo(n), rdt(n), pair(n, -1), derm(n, -1), src(n, -1), nl(n, 0);
+  std::vector<double> dat(n), ca(n), ip(n);
+  std::vector<long long> uid(n);
+  for (int i = 0; i < n; i++) {
+    st[i] = in->state[i];
+    mal[i] = in->is_malignant ? in->is_malignant[i] : 0;
+    fo[i] = (st[i] == DER || st[i] == MEMB) ? -1 : in->fix_origin[i];           // cell.cpp:67
+    rdt[i] = st[i] == FIX ? 15 : in->rest_div_times[i];                         // cell.cpp:18-20
+    dat[i] = st[i] == FIX ? 6.0 : st[i] == MUSUME ? 6.0 : 0.0;                    // cell.cpp:11-15
+    uid[i] = i;
+    if (in->pair && in->pair[i] >= 0) {
+      if (in->pair[i] >= n) return fail(c, EPI_ERR_INVALID, "pair out of range");
+      pair[i] = in->pair[i];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    if (pair[i] >= 0 && pair[pair[i]] != i) return fail(c, EPI_ERR_INVALID, "pairs must be mutual");
+  std::vector<int> m6;
+  memb_init_host(h.g.nmx, h.nmemb, m6);
+  // the gather in k_forces_memb needs a symmetric adjacency of distinct cells
+  for (int j = 0; j < h.nmemb; j++)
+    for (int a = 0; a < 6; a++) {
+      const int q = m6[6 * j + a];
+      int cnt = 0, back = 0;
+      for (int b = 0; b < 6; b++) { cnt += m6[6 * j + b] == q; back += m6[6 * q + b] == j; }
+      if (cnt != 1 || back != 1 || q == j) return fail(c, EPI_ERR_INVALID, "membrane adjacency is not symmetric");
+    }
+  h.nder = in->nder;
+  ti.nder = in->nder;
+  HIPCHK(c, hipMemcpy(c->d_tis + t, &ti, sizeof(Tis), hipMemcpyHostToDevice));
+  RCHK(upload(c, ti.state, st.data(), n));
+  RCHK(upload(c, ti.is_malignant, mal.data(), n));
+  RCHK(upload(c, ti.fix_origin, fo.data(), n));
+  RCHK(upload(c, ti.rest_div_times, rdt.data(), n));
+  RCHK(upload(c, ti.pair, pair.data(), n));
+  RCHK(upload(c, ti.dermis, derm.data(), n));
+  RCHK(upload(c, ti.src_row, src.data(), n));
+  RCHK(upload(c, ti.nlen[0], nl.data(), n));
+  RCHK(upload(c, ti.nlen[1], nl.data(), n));
+  RCHK(upload(c, ti.uid, uid.data(), n));
+  RCHK(upload(c, ti.old_uid, uid.data(), n));
+  for (int b = 0; b < 2; b++) {
+    RCHK(upload(c, ti.pos[b][0], in->x, n));
+    RCHK(upload(c, ti.pos[b][1], in->y, n));
+    RCHK(upload(c, ti.pos[b][2], in->z, n));
+    RCHK(upload(c, ti.ca2p[b], in->ca2p, n));
+    RCHK(upload(c, ti.ip3[b], in->ip3, n));
+  }
+  RCHK(upload(c, ti.radius, in->radius, n));
+  RCHK(upload(c, ti.div_age_thresh, dat.data(), n));
+  RCHK(upload(c, ti.ca2p_avg, in->ca2p_avg, n));
+  RCHK(upload(c, ti.ex_inert, in->ex_inert, n));
+  RCHK(upload(c, ti.agek, in->agek, n));
+  RCHK(upload(c, ti.ageb, in->ageb, n));
+  RCHK(upload(c, ti.ex_fat, in->ex_fat, n));
+  RCHK(upload(c, ti.in_fat, in->in_fat, n));
+  RCHK(upload(c, ti.spr_nat_len, in->spr_nat_len, n));
+  RCHK(upload(c, ti.memb6, m6.data(), m6.size()));
+  Status s{};
+  s.n = n;
+  s.next_uid = n;
+  c->h_st[t] = s;
+  RCHK(put_status(c, t));
+  h.n_est = n;
+  h.loaded = true;
+  return EPI_OK;
+}
+
+int epi_value_init(epi_ctx* c, int32_t t) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  const int n = c->h_st[t].n;
+  std::vector<uint8_t> st(n);
+  RCHK(download(c, st.data(), ti.state, n));
+  std::vector<double> ca(n, ca2p_init), avg(n, ca2p_init), exi(n, 0.97), ip(n, 0.0);
+  for (int i = h.nmemb + h.nder; i < n; i++)
+    if (st[i] == DEAD) { avg[i] = 0; ca[i] = 0; }
+  for (int b = 0; b < 2; b++) {
+    RCHK(upload(c, ti.ca2p[b], ca.data(), n));
+    RCHK(upload(c, ti.ip3[b], ip.data(), n));
+  }
+  RCHK(upload(c, ti.ca2p_avg, avg.data(), n));
+  RCHK(upload(c, ti.ex_inert, exi.data(), n));
+  return EPI_OK;
+}
+
+int epi_set_cell_values(epi_ctx* c, int32_t t, const double* ca2p, const double* ip3, const double* ex_inert,
+                        const double* ca2p_avg, const double* gj) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  const int n = c->h_st[t].n;
+  for (int b = 0; b < 2; b++) {
+    RCHK(upload(c, ti.ca2p[b], ca2p, n));
+    RCHK(upload(c, ti.ip3[b], ip3, n));
+  }
+  RCHK(upload(c, ti.ex_inert, ex_inert, n));
+  RCHK(upload(c, ti.ca2p_avg, ca2p_avg, n));
+  if (gj) {
+    std::vector<int> nl(n);
+    RCHK(download(c, nl.data(), ti.nlen[c->cb], n));
+    std::vector<double> g((size_t)n * h.ns, 0.0);
+    size_t p = 0;
+    for (int i = 0; i < n; i++)
+      for (int e = 0; e < nl[i]; e++) g[(size_t)i * h.ns + e] = gj[p++];
+    RCHK(upload(c, ti.gj[c->cb], g.data(), g.size()));
+  }
+  return EPI_OK;
+}
+
+int epi_set_fields(epi_ctx* c, int32_t t, const double* atp, const double* atp_next, const double* b,
+                   const double* b_next) {
+  RCHK(check_tissue(c, t));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  std::vector<double> tmp;
+  const double* srcs[4] = {atp, atp_next, b, b_next};
+  double* dsts[4] = {ti.atp[c->ap], ti.atp[1 - c->ap], ti.bf[c->bp], ti.bf[1 - c->bp]};
+  for (int i = 0; i < 4; i++) {
+    if (!srcs[i]) continue;
+    to_device_layout(h, srcs[i], tmp);
+    RCHK(upload(c, dsts[i], tmp.data(), tmp.size()));
+  }
+  return EPI_OK;
+}
+
+int epi_set_rng(epi_ctx* c, int32_t t, const double* draws, int64_t n) {
+  RCHK(check_tissue(c, t));
+  if (n < 0 || (n > 0 && !draws)) return fail(c, EPI_ERR_INVALID, "bad stream");
+  RCHK(sync_status(c));
+  TH& h = c->th[t];
+  if (h.rng_d) { (void)hipFree(h.rng_d); h.rng_d = nullptr; }
+  if (n > 0) {
+    HIPCHK(c, hipMalloc(&h.rng_d, sizeof(double) * n));
+    RCHK(upload(c, h.rng_d, draws, n));
+  }
+  h.rng_n = n;
+  c->tis[t].rng = h.rng_d;
+  c->tis[t].rng_n = n;
+  HIPCHK(c, hipMemcpy(c->d_tis + t, &c->tis[t], sizeof(Tis), hipMemcpyHostToDevice));
+  c->h_st[t].rng_pos = 0;
+  return put_status(c, t);
+}
+
+int epi_rng_mt19937_64(uint64_t seed, double* out, int64_t n) {
+  if (n < 0 || (n > 0 && !out)) return EPI_ERR_INVALID;
+  std::mt19937_64 mt(seed);
+  std::uniform_real_distribution<double> dist(0.0, 1.0);
+  for (int64_t i = 0; i < n; i++) out[i] = dist(mt);
+  return EPI_OK;
+}
+
+int epi_cell_interaction(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), max_n(c));
+  HIPCHK(c, hipGetLastError());
+  c->pos_pending = true;
+  return EPI_OK;
+}
+
+int epi_pos_copy(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  if (c->pos_pending) {
+    c->pb ^= 1;
+    c->pos_pending = false;
+  }
+  return EPI_OK;
+}
+
+int epi_cell_state_renew(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  if (c->pos_pending) return fail(c, EPI_ERR_INVALID, "pos_copy must follow cell_interaction");
+  launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, max_n(c));
+  HIPCHK(c, hipGetLastError());
+  return EPI_OK;
+}
+
+int epi_cell_pos_periodic_fix(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  launch_periodic_fix(c->stream, c->d_tis, c->T, c->pb, max_n(c));
+  HIPCHK(c, hipGetLastError());
+  return EPI_OK;
+}
+
+int epi_connect_cell(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  const int nb = 1 - c->cb;
+  launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, max_n(c), max_nb(c));
+  HIPCHK(c, hipGetLastError());
+  c->cb = nb;
+  return EPI_OK;
+}
+
+int epi_calc_zzmax(epi_ctx* c, double* zzmax) {
+  if (!c) return EPI_ERR_INVALID;
+  launch_zzmax(c->stream, c->d_tis, c->T, c->pb, max_n(c));
+  HIPCHK(c, hipGetLastError());
+  RCHK(sync_status(c));
+  for (int t = 0; t < c->T; t++) {
+    double v;
+    unsigned long long b = c->h_st[t].zzmax_bits;
+    memcpy(&v, &b, 8);
+    c->h_st[t].zzmax = v;
+    if (zzmax) zzmax[t] = v;
+  }
+  return EPI_OK;
+}
+
+int epi_setup_map_lat(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  launch_maps(c->stream, c->d_tis, c->T, c->pb, max_n(c), max_gsize(c), max_halo(c));
+  HIPCHK(c, hipGetLastError());
+  return EPI_OK;
+}
+
+int epi_calc_ext_stim(epi_ctx* c, const double* zzmax) {
+  if (!c) return EPI_ERR_INVALID;
+  RCHK(set_zz(c, zzmax));
+  launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false));
+  HIPCHK(c, hipGetLastError());
+  c->bp ^= 1;
+  return EPI_OK;
+}
+
+int epi_initialize_sc(epi_ctx* c, const double* zzmax) {
+  if (!c) return EPI_ERR_INVALID;
+  RCHK(set_zz(c, zzmax));
+  launch_initialize_sc(c->stream, c->d_tis, nullptr, c->T, c->pb, max_n(c));
+  HIPCHK(c, hipGetLastError());
+  return EPI_OK;
+}
+
+int epi_calc_ca2p(epi_ctx* c, const double* zzmax) {
+  if (!c) return EPI_ERR_INVALID;
+  RCHK(set_zz(c, zzmax));
+  RCHK(run_ca(c, nullptr, c->T));
+  return sync_status(c);
+}
+
+int epi_should_calc_ca(epi_ctx* c, int32_t* flags) {
+  if (!c) return EPI_ERR_INVALID;
+  RCHK(sync_status(c));
+  for (int t = 0; t < c->T; t++) flags[t] = (c->h_st[t].sw & 255) >= SW_THRESH;
+  return EPI_OK;
+}
+
+int epi_ca_calc_condition_reset(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  RCHK(sync_status(c));
+  for (int t = 0; t < c->T; t++) {
+    c->h_st[t].sw = 0;
+    HIPCHK(c, hipMemcpyAsync(&c->d_st[t].sw, &c->h_st[t].sw, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EPI_OK;
+}
+
+int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
+  if (!c) return EPI_ERR_INVALID;
+  // a zzmax whose calc_ca2p bound is iz_bound
+  std::vector<double> zz(c->T);
+  for (int t = 0; t < c->T; t++) zz[t] = iz_bound * c->tis[t].dz - FAC_MAP * R_max + 0.5 * c->tis[t].dz;
+  RCHK(set_zz(c, zz.data()));
+  for (int t = 0; t < c->T; t++)
+    if ((int)((zz[t] + FAC_MAP * R_max) / c->tis[t].dz) != iz_bound) return fail(c, EPI_ERR_INVALID, "iz_bound");
+  const int mn = max_n(c);
+  long long box = 1;
+  for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * (iz_bound + 1));
+  launch_ca_touch(c->stream, c->d_tis, nullptr, c->T, c->cb, mn, box);
+  launch_ca_substep(c->stream, c->d_tis, nullptr, c->T, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c),
+                    iz_bound + 1);
+  c->cp ^= 1;
+  c->ap ^= 1;
+  launch_ca_diffu_zero(c->stream, c->d_tis, nullptr, c->T, mn);
+  HIPCHK(c, hipGetLastError());
+  return sync_status(c);
+}
+
+int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
+  if (!c || !drv) return EPI_ERR_INVALID;
+  for (int64_t it = 0; it < nsteps; it++) {
+    const int mn = max_n(c);
+    // cell_dynamics (proc.cpp:22-32)
+    launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), mn);
+    c->pb ^= 1;
+    launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
+    launch_periodic_fix(c->stream, c->d_tis, c->T, c->pb, mn);
+    const int nb = 1 - c->cb;
+    launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, mn, max_nb(c));
+    c->cb = nb;
+    // proc.cpp:154-157
+    launch_zzmax(c->stream, c->d_tis, c->T, c->pb, mn);
+    launch_maps(c->stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c));
+    launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false));
+    c->bp ^= 1;
+    HIPCHK(c, hipGetLastError());
+    RCHK(sync_status(c));
+    int nsc = 0, nca = 0;
+    std::vector<int> ca_list;
+    for (int t = 0; t < c->T; t++) {
+      double v;
+      unsigned long long b = c->h_st[t].zzmax_bits;
+      memcpy(&v, &b, 8);
+      c->h_st[t].zzmax = v;
+      epi_driver& d = drv[t];
+      // proc.cpp:158-163
+      if ((double)d.step * DT_Cell > 6000.0 && d.force_cornif) {
+        d.force_cornif = 0;
+        c->h_tlist[nsc++] = t;
+        d.num_sc = 1;
+      }
+    }
+    if (nsc) {
+      HIPCHK(c, hipMemcpyAsync(c->d_tlist, c->h_tlist, sizeof(int) * nsc, hipMemcpyHostToDevice, c->stream));
+      launch_initialize_sc(c->stream, c->d_tis, c->d_tlist, nsc, c->pb, mn);
+    }
+    for (int t = 0; t < c->T; t++) {
+      epi_driver& d = drv[t];
+      const bool forced = d.ca_every > 0 && ((d.step + 1) % d.ca_every) == 0;
+      if ((c->h_st[t].sw & 255) >= SW_THRESH || d.num_sc > 0 || forced) {
+        ca_list.push_back(t);
+        if (d.num_sc > 0) d.num_sc--;
+        c->ca_solves[t]++;
+      }
+      d.step++;
+    }
+    nca = (int)ca_list.size();
+    if (nca) {
+      if (nsc) HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (int i = 0; i < nca; i++) c->h_tlist[i] = ca_list[i];
+      HIPCHK(c, hipMemcpyAsync(c->d_tlist, c->h_tlist, sizeof(int) * nca, hipMemcpyHostToDevice, c->stream));
+      RCHK(run_ca(c, c->d_tlist, nca));
+      RCHK(sync_status(c));
+    }
+  }
+  return EPI_OK;
+}
+
+int epi_synchronize(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  return sync_status(c);
+}
+
+int epi_get_status(epi_ctx* c, int32_t t, epi_status* out) {
+  RCHK(check_tissue(c, t));
+  int rc = sync_status(c);
+  const Status& s = c->h_st[t];
+  out->n = s.n; out->err = s.err; out->sw = s.sw & 255; out->ndiv = s.ndiv; out->nrm = s.nrm;
+  out->maxlen = s.maxlen; out->rng_pos = s.rng_pos; out->next_uid = s.next_uid;
+  unsigned long long b = s.zzmax_bits;
+  memcpy(&out->zzmax, &b, 8);
+  out->ca_solves = c->ca_solves[t];
+  return rc;
+}
+
+int epi_get_cells(epi_ctx* c, int32_t t, epi_cells* o) {
+  RCHK(check_tissue(c, t));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->h_st, c->d_st, sizeof(Status) * c->T, hipMemcpyDeviceToHost));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  const int n = c->h_st[t].n;
+  o->n = n; o->nmemb = h.nmemb; o->nder = h.nder; o->sw = c->h_st[t].sw & 255;
+  const int p1 = c->pos_pending ? 1 - c->pb : c->pb;
+  RCHK(download(c, o->state, ti.state, n));
+  RCHK(download(c, o->fix_origin, ti.fix_origin, n));
+  RCHK(download(c, o->x, ti.pos[c->pb][0], n));
+  RCHK(download(c, o->y, ti.pos[c->pb][1], n));
+  RCHK(download(c, o->z, ti.pos[c->pb][2], n));
+  RCHK(download(c, o->xn, ti.pos[p1][0], n));
+  RCHK(download(c, o->yn, ti.pos[p1][1], n));
+  RCHK(download(c, o->zn, ti.pos[p1][2], n));
+  RCHK(download(c, o->radius, ti.radius, n));
+  RCHK(download(c, o->ca2p, ti.ca2p[c->cp], n));
+  RCHK(download(c, o->ca2p_next, ti.ca2p[1 - c->cp], n));
+  RCHK(download(c, o->ip3, ti.ip3[c->cp], n));
+  RCHK(download(c, o->ip3_next, ti.ip3[1 - c->cp], n));
+  RCHK(download(c, o->ca2p_avg, ti.ca2p_avg, n));
+  RCHK(download(c, o->ex_inert, ti.ex_inert, n));
+  RCHK(download(c, o->agek, ti.agek, n));
+  RCHK(download(c, o->ageb, ti.ageb, n));
+  RCHK(download(c, o->ex_fat, ti.ex_fat, n));
+  RCHK(download(c, o->in_fat, ti.in_fat, n));
+  RCHK(download(c, o->spr_nat_len, ti.spr_nat_len, n));
+  RCHK(download(c, o->diff_u, ti.diff_u, n));
+  RCHK(download(c, o->rest_div_times, ti.rest_div_times, n));
+  RCHK(download(c, o->pair, ti.pair, n));
+  RCHK(download(c, o->dermis, ti.dermis, n));
+  RCHK(download(c, o->lat, ti.lat, 3 * (size_t)n));
+  RCHK(download(c, o->is_malignant, ti.is_malignant, n));
+  if (o->uid) {
+    std::vector<long long> u(n);
+    RCHK(download(c, u.data(), ti.uid, n));
+    for (int i = 0; i < n; i++) o->uid[i] = u[i];
+  }
+  if (o->memb_adj) RCHK(download(c, o->memb_adj, ti.memb6, 6 * (size_t)h.nmemb));
+  if (o->conn_ptr || o->conn || o->gj) {
+    std::vector<int> nl(n);
+    RCHK(download(c, nl.data(), ti.nlen[c->cb], n));
+    long long nnz = 0;
+    for (int i = 0; i < n; i++) nnz += nl[i];
+    if (o->conn_ptr) {
+      int acc = 0;
+      for (int i = 0; i < n; i++) { o->conn_ptr[i] = acc; acc += nl[i]; }
+      o->conn_ptr[n] = acc;
+    }
+    if (o->conn || o->gj) {
+      if (nnz > o->conn_cap) return fail(c, EPI_ERR_CAPACITY, "conn_cap too small");
+      std::vector<int> col((size_t)n * h.ns);
+      std::vector<double> g((size_t)n * h.ns);
+      RCHK(download(c, col.data(), ti.col[c->cb], col.size()));
+      RCHK(download(c, g.data(), ti.gj[c->cb], g.size()));
+      size_t p = 0;
+      for (int i = 0; i < n; i++)
+        for (int e = 0; e < nl[i]; e++, p++) {
+          if (o->conn) o->conn[p] = col[(size_t)i * h.ns + e];
+          if (o->gj) o->gj[p] = g[(size_t)i * h.ns + e];
+        }
+    }
+  }
+  return EPI_OK;
+}
+
+int epi_get_fields(epi_ctx* c, int32_t t, double* atp, double* atp_next, double* b, double* b_next, int32_t* cmap1,
+                   uint8_t* cmap2) {
+  RCHK(check_tissue(c, t));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  std::vector<double> tmp(h.gsize);
+  double* dsts[4] = {atp, atp_next, b, b_next};
+  const double* srcs[4] = {ti.atp[c->ap], ti.atp[1 - c->ap], ti.bf[c->bp], ti.bf[1 - c->bp]};
+  for (int i = 0; i < 4; i++) {
+    if (!dsts[i]) continue;
+    RCHK(download(c, tmp.data(), srcs[i], tmp.size()));
+    from_device_layout(h, tmp, dsts[i]);
+  }
+  if (cmap1) {
+    std::vector<int> m(h.gsize);
+    RCHK(download(c, m.data(), ti.cmap1, m.size()));
+    from_device_layout(h, m, cmap1);
+  }
+  if (cmap2) {
+    std::vector<uint8_t> m(h.gsize);
+    RCHK(download(c, m.data(), ti.cmap2, m.size()));
+    from_device_layout(h, m, cmap2);
+  }
+  return EPI_OK;
+}
+
+int epi_set_positions(epi_ctx* c, int32_t t, const double* x, const double* y, const double* z) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  const Tis& ti = c->tis[t];
+  const int n = c->h_st[t].n;
+  for (int b = 0; b < 2; b++) {
+    RCHK(upload(c, ti.pos[b][0], x, n));
+    RCHK(upload(c, ti.pos[b][1], y, n));
+    RCHK(upload(c, ti.pos[b][2], z, n));
+  }
+  c->pos_pending = false;
+  return EPI_OK;
+}
+
+int epi_set_state(epi_ctx* c, int32_t t, const uint8_t* state, const double* agek) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  const Tis& ti = c->tis[t];
+  const int n = c->h_st[t].n;
+  RCHK(upload(c, ti.state, state, n));
+  RCHK(upload(c, ti.agek, agek, n));
+  return EPI_OK;
+}
+
+int epi_set_lists(epi_ctx* c, int32_t t, const int32_t* ptr, const int32_t* conn, const double* gj) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  const int n = c->h_st[t].n;
+  std::vector<int> col((size_t)n * h.ns, 0), los((size_t)n * h.ns, 0), nl(n), nlo(n, 0), srow(n);
+  std::vector<double> g((size_t)n * h.ns, gj_init);
+  std::vector<long long> u(n);
+  RCHK(download(c, u.data(), ti.uid, n));
+  for (int i = 0; i < n; i++) {
+    const int len = ptr[i + 1] - ptr[i];
+    if (len > h.ns || len >= MAX_CONN) return fail(c, EPI_ERR_CAPACITY, "list too long");
+    nl[i] = len;
+    srow[i] = i;
+    int k = i < h.nmemb ? MEMB_ADJ : 0;
+    int lo = 0;
+    for (int e = 0; e < len; e++) {
+      const int q = conn[ptr[i] + e];
+      col[(size_t)i * h.ns + e] = q;
+      if (gj) g[(size_t)i * h.ns + e] = gj[ptr[i] + e];
+    }
+    if (i >= h.nmemb) {
+      while (lo < len && col[(size_t)i * h.ns + lo] < i) lo++;
+      k = lo;
+    }
+    for (int e = k; e < len; e++)
+      if (col[(size_t)i * h.ns + e] <= i || (e > k && col[(size_t)i * h.ns + e] < col[(size_t)i * h.ns + e - 1]))
+        return fail(c, EPI_ERR_INVALID, "list is not in canonical order");
+    nlo[i] = lo;
+    std::vector<int> s(col.begin() + (size_t)i * h.ns, col.begin() + (size_t)i * h.ns + lo);
+    std::sort(s.begin(), s.end());
+    std::copy(s.begin(), s.end(), los.begin() + (size_t)i * h.ns);
+  }
+  RCHK(upload(c, ti.col[c->cb], col.data(), col.size()));
+  RCHK(upload(c, ti.gj[c->cb], g.data(), g.size()));
+  RCHK(upload(c, ti.nlen[c->cb], nl.data(), n));
+  RCHK(upload(c, ti.nlo, nlo.data(), n));
+  RCHK(upload(c, ti.lo_sorted, los.data(), los.size()));
+  RCHK(upload(c, ti.src_row, srow.data(), n));
+  RCHK(upload(c, ti.old_uid, u.data(), n));
+  return EPI_OK;
+}
+
+int epi_set_maps(epi_ctx* c, int32_t t, const int32_t* cmap1, const uint8_t* cmap2, const int32_t* lat) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  const Tis& ti = c->tis[t];
+  const TH& h = c->th[t];
+  if (cmap1) {
+    std::vector<int> m(h.gsize, -1);
+    for (int j = 0; j <= h.NX; j++)
+      for (int k = 0; k <= h.NY; k++)
+        for (int l = 0; l <= h.NZ; l++) m[dev_index(h, j, k, l)] = cmap1[((size_t)j * (h.NY + 1) + k) * (h.NZ + 1) + l];
+    RCHK(upload(c, ti.cmap1, m.data(), m.size()));
+  }
+  if (cmap2) {
+    std::vector<uint8_t> m(h.gsize, 0);
+    for (int j = 0; j <= h.NX; j++)
+      for (int k = 0; k <= h.NY; k++)
+        for (int l = 0; l <= h.NZ; l++) m[dev_index(h, j, k, l)] = cmap2[((size_t)j * (h.NY + 1) + k) * (h.NZ + 1) + l];
+    RCHK(upload(c, ti.cmap2, m.data(), m.size()));
+  }
+  if (lat) RCHK(upload(c, ti.lat, lat, 3 * (size_t)c->h_st[t].n));
+  return EPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,427 @@
+// k_field.hip -- setup_map_lat (map.cpp:18-167), calc_ext_stim
+// (ext_stim.cpp:18-69) and calc_ca2p (ca2p.cpp:14-402) on CDNA4.
+//
+// Grids are z-major with x fastest (x pitch padded to 16 doubles), so a
+// wavefront reads 64 consecutive x points of one (y, z) row.  The stencils
+// map threads to (x, y) columns and march z in chunks of ZB planes; the
+// number of active planes (iz_bound) is derived on the device from zzmax and
+// the z chunks are grid-strided, so no host round trip is needed.
+#include "epi_dev.h"
+#include "launch.h"
+
+namespace epi {
+
+constexpr int ZB = 4;   // z planes per thread per chunk
+
+// ---- voxel maps ---------------------------------------------------------
+__global__ void __launch_bounds__(256) k_map_clear(const Tis* __restrict__ T) {
+  const Tis& t = T[blockIdx.y];
+  const long long g = t.gsize;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < g;
+       v += (long long)gridDim.x * blockDim.x) {
+    t.cmap1[v] = -1;
+    t.cmap2[v] = 0;
+  }
+}
+
+__device__ __forceinline__ int lat_wrap(long v, int N, Status* st) {
+  // precalc_lat_* hold i%N for i in [0, 3N) (utils.cpp:18-30)
+  const long i = v + N;
+  if (i < 0 || i >= 3L * N) { set_err(st, ERR_OUT_OF_GRID); return (int)(((i % N) + N) % N); }
+  return (int)(i % N);
+}
+
+// set_lattice (:18-22) and the sphere stamping (:39-155); the last writer of
+// the serial loop is the highest cell index, hence atomicMax on cmap1
+__global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, int pb) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  const int n = st->n;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const P3 p = ld_pos(t, pb, c);
+    const int lx = lat_wrap((long)(int)(p.x * t.inv_dx), t.NX, st);
+    const int ly = lat_wrap((long)(int)(p.y * t.inv_dy), t.NY, st);
+    const int lz = lat_wrap((long)(int)(p.z * t.inv_dz), t.NZ, st);
+    t.lat[3 * c] = lx; t.lat[3 * c + 1] = ly; t.lat[3 * c + 2] = lz;
+    const uint8_t s = t.state[c];
+    const bool afm = is_afm(s);
+    const int irx = afm ? 2 * t.irx : t.irx, iry = afm ? 2 * t.iry : t.iry, irz = afm ? 2 * t.irz : t.irz;
+    const double r = t.radius[c];
+    const double normal_radSq = r * r;
+    const double crad = FAC_MAP * r;
+    const double cradSq = crad * crad;
+    const int z_b = lz - irz, z_u = lz + irz;
+    const int zmin = z_b >= 1 ? z_b : 1;
+    const int zmax = z_u < t.NZ ? z_u : t.NZ - 1;
+    for (int k = lx - irx; k <= lx + irx; k++) {
+      const double mx = k * t.dx;
+      const int ipx = lat_wrap(k, t.NX, st);
+      const double diffx = p_diff_x(t, mx, p.x);
+      const double diffxSq = diffx * diffx;
+      for (int l = ly - iry; l <= ly + iry; l++) {
+        const double my = l * t.dy;
+        const int ipy = lat_wrap(l, t.NY, st);
+        const double diffy = p_diff_y(t, my, p.y);
+        const double tmps = diffxSq + diffy * diffy;
+        for (int m = zmin; m <= zmax; m++) {
+          const double mz = m * t.dz;
+          const double diffz = mz - p.z;
+          const double distSq = tmps + diffz * diffz;
+          const long long v = gidx(t, ipx, ipy, m);
+          if (!afm) {
+            if (distSq < normal_radSq) atomicMax(&t.cmap1[v], c);
+          } else if (distSq < cradSq) {
+            t.cmap2[v] = 1;
+            if (distSq < normal_radSq) atomicMax(&t.cmap1[v], c);
+          }
+        }
+      }
+    }
+  }
+}
+
+// periodic planes x = NX and y = NY for l < NZ (:156-166)
+__global__ void __launch_bounds__(256) k_map_halo(const Tis* __restrict__ T) {
+  const Tis& t = T[blockIdx.y];
+  const int NX = t.NX, NY = t.NY;
+  const int per_l = (NX + 1) + (NY + 1);
+  const long long tot = (long long)per_l * t.NZ;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+    const int l = (int)(i / per_l), r = (int)(i % per_l);
+    long long dst, src;
+    if (r <= NX) {   // y = NY row
+      const int j = r;
+      dst = gidx(t, j, NY, l);
+      src = gidx(t, j == NX ? 0 : j, 0, l);
+    } else {         // x = NX column
+      const int k = r - (NX + 1);
+      dst = gidx(t, NX, k, l);
+      src = gidx(t, 0, k == NY ? 0 : k, l);
+    }
+    t.cmap1[dst] = t.cmap1[src];
+    t.cmap2[dst] = t.cmap2[src];
+  }
+}
+
+// ---- ext_stim B: calc_ext_stim (ext_stim.cpp:18-69) ------------------------
+__device__ __forceinline__ double fB(double age, double B, bool cornif) {   // :9-16
+  const double kb = 0.025, DUR_ALIVE = 0.5, DUR_DEAD = 2.0;
+  return (cornif && age > THRESH_DEAD - DUR_ALIVE && age <= THRESH_DEAD + DUR_DEAD ? 1 : 0) - kb * B;
+}
+
+// threads: x fastest (64 per wave), 4 y rows per block; blockIdx.z packs
+// (y block, z chunk); z chunks are grid-strided up to iz_bound
+__global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, int bp, int nky, int nzc) {
+  const Tis& t = T[blockIdx.y];
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int k = (blockIdx.z % nky) * 4 + (threadIdx.x >> 6);
+  const int zc = blockIdx.z / nky;
+  const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
+  const int iz_bound = (int)((zzmax + FAC_MAP * R_max) * t.inv_dz);
+  if (iz_bound > NZ || iz_bound < 0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.z == 0) set_err(t.st, ERR_OUT_OF_GRID);
+    return;
+  }
+  if (j >= NX || k >= NY) return;
+  const double* __restrict__ carr = t.bf[bp];
+  double* __restrict__ narr = t.bf[1 - bp];
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  const double DB = 0.0009;
+  const double inv_dz = t.inv_dz;
+  for (int l0 = zc * ZB; l0 < iz_bound; l0 += nzc * ZB) {
+    const int l1 = min(l0 + ZB, iz_bound);
+    for (int l = l0; l < l1; l++) {
+      const int prev_z = l == 0 ? 1 : l - 1, next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+      const long long v = gidx(t, j, k, l);
+      double dum_age = 0;
+      bool flg = false;
+      const int cc = t.cmap1[v];
+      if (cc >= 0) {
+        const uint8_t s = t.state[cc];
+        if (s == DEAD || s == ALIVE) { dum_age = t.agek[cc]; flg = true; }
+      }
+      const double cext = carr[v];
+      const long long vxm = gidx(t, prev_x, k, l), vym = gidx(t, j, prev_y, l);
+      const long long vzm = gidx(t, j, k, prev_z), vzp = gidx(t, j, k, next_z);
+      const long long vyp = gidx(t, j, next_y, l), vxp = gidx(t, next_x, k, l);
+      const double nv = cext + DT_Ca * (DB *
+          ((double)cm2[vxm] * (carr[vxm] - cext)
+           + (double)cm2[vym] * (carr[vym] - cext)
+           + (double)cm2[vzm] * (carr[vzm] - cext)
+           + (double)cm2[vzp] * (carr[vzp] - cext)
+           + (double)cm2[vyp] * (carr[vyp] - cext)
+           + (double)cm2[vxp] * (carr[vxp] - cext))
+          * inv_dz * inv_dz + fB(dum_age, cext, flg));
+      narr[v] = nv;
+      // periodic copies (:63-66)
+      if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+      if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+      if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+    }
+  }
+}
+
+// ---- calc_ca2p ------------------------------------------------------------
+constexpr double Kpp = 0.3;
+constexpr double dp_c = 0.1;
+
+__device__ __forceinline__ double ca2p_reaction_factor(double u, double ER3, double p, double B) {   // :36-87
+  const double kbc = 0.4 * 1.2, Hb = 0.01, Cout = 1.0;
+  const double kg = 0.1, gamma = 2.0;
+  const double mu0 = 0.567, mu1 = 0.1, kmu = 0.05;
+  const double para_b = 0.11, para_bb = 0.89, para_k1 = 0.7;
+  const double k_flux = 8.1, beta_zero = 0.02, CA_OUT = 1.0;
+  const double leak_from_storage = CA_OUT * beta_zero;
+  const double er1 = (mu0 + mu1 * p / (p + kmu));
+  const double er2 = (para_b + para_bb * u / (para_k1 + u));
+  const double ca2p_from_storage = k_flux * er1 * er2 * ER3;
+  const double into = gamma * u / (kg + u);
+  const double by_ext = kbc * B * B * Cout / (Hb + B * B);
+  return ca2p_from_storage - into + leak_from_storage + by_ext;
+}
+
+__device__ __forceinline__ double grid_avg8(const Tis& t, const double* g, int ix, int iy, int iz) {   // utils.h:18-23
+  return 0.125 * (g[gidx(t, ix, iy, iz)] + g[gidx(t, ix + 1, iy, iz)] + g[gidx(t, ix, iy + 1, iz)]
+                  + g[gidx(t, ix, iy, iz + 1)] + g[gidx(t, ix + 1, iy + 1, iz)] + g[gidx(t, ix + 1, iy, iz + 1)]
+                  + g[gidx(t, ix, iy + 1, iz + 1)] + g[gidx(t, ix + 1, iy + 1, iz + 1)]);
+}
+
+__device__ __forceinline__ int ca_iz_bound(const Tis& t) {   // :386
+  const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
+  return (int)((zzmax + FAC_MAP * R_max) / t.dz);
+}
+
+// init_ca2p_avg (:20-27) and the per-cell "touches AIR" flag used by
+// init_ca2p_map (:262-268)
+__global__ void __launch_bounds__(256) k_ca_begin(const Tis* __restrict__ T, const int* tlist, int cb, int zero_avg) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int* row = t.col[cb] + (size_t)c * t.ns;
+    const int len = t.nlen[cb][c];
+    uint8_t air = 0;
+    for (int e = 0; e < len; e++)
+      if (t.state[row[e]] == AIR) { air = 1; break; }
+    t.touch_air[c] = air;
+    if (zero_avg && c >= start && is_afm(t.state[c])) t.ca2p_avg[c] = 0;
+  }
+}
+
+// init_ca2p_map (:238-278): per voxel of the active box the AFM owner
+// (+1, 0 = the dummy zero) and the air flag in bit 31
+__global__ void __launch_bounds__(256) k_ca_vox(const Tis* __restrict__ T, const int* tlist) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int izb = ca_iz_bound(t);
+  if (izb > t.NZ || izb < 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) set_err(t.st, ERR_OUT_OF_GRID);
+    return;
+  }
+  const long long tot = (long long)t.NX * t.NY * izb;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(i % t.NX);
+    const long long r = i / t.NX;
+    const int k = (int)(r % t.NY), l = (int)(r / t.NY);
+    const long long v = gidx(t, j, k, l);
+    const int c = t.cmap1[v];
+    unsigned w = 0;
+    if (c >= 0) w = (is_afm(t.state[c]) ? (unsigned)(c + 1) : 0u) | ((unsigned)t.touch_air[c] << 31);
+    t.vox[v] = w;
+  }
+}
+
+// dead_IP3_calc (:156-173) and supra_calc (:182-222), one thread per cell
+__global__ void __launch_bounds__(256) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp, int ap, int bp) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int n = t.st->n;
+  const double* __restrict__ ca = t.ca2p[cp];
+  double* __restrict__ can = t.ca2p[1 - cp];
+  const double* __restrict__ ip3 = t.ip3[cp];
+  double* __restrict__ ip3n = t.ip3[1 - cp];
+  for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint8_t s = t.state[c];
+    const int* row = t.col[cb] + (size_t)c * t.ns;
+    const int len = t.nlen[cb][c];
+    if (s == DEAD) {
+      int count = 0;
+      double tmp = 0;
+      const double pc = ip3[c];
+      for (int e = 0; e < len; e++) {
+        const int o = row[e];
+        if (t.state[o] == ALIVE) { tmp += ip3[o]; count++; }
+      }
+      tmp = DT_Ca * (dp_c * (tmp - count * pc) - Kpp * pc);
+      ip3n[c] = pc + tmp;
+    } else if (is_afm(s)) {
+      const double ca2p_du = 0.01;
+      const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
+      const bool is_alive = s == ALIVE;
+      const double agek = t.agek[c];
+      // calc_th, calc_Kpa, calc_IAG (:90-123)
+      const double _th = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
+      const double _Kpa = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
+      const double IAGv = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
+      const double u = ca[c], p = ip3[c];
+      double tmp_diffu = 0, tmp_IP3 = 0;
+      double* g = t.gj[cb] + (size_t)c * t.ns;
+      for (int e = 0; e < len; e++) {
+        const int o = row[e];
+        const uint8_t so = t.state[o];
+        if (so == ALIVE || so == DEAD || so == FIX || so == MUSUME) {
+          const double gv = g[e];
+          const double uo = ca[o];
+          tmp_diffu += gv * (uo - u);
+          tmp_IP3 += gv * (ip3[o] - p);
+          if (so == ALIVE) {
+            // fw (:138-144)
+            const double diff = fabs(uo - u);
+            const double fwv = (1. - gv) + (-1. + tanh((0.1 - diff) / 0.1)) / 2.;
+            g[e] = gv + DT_Ca * fwv;
+          }
+        }
+      }
+      const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
+      const double Aavg = grid_avg8(t, t.atp[ap], ix, iy, iz);
+      const double diff_u = ca2p_reaction_factor(u, t.ex_inert[c], p, Bavg) + ca2p_du * IAGv * tmp_diffu;
+      t.diff_u[c] = diff_u;
+      const double H0 = 0.5;
+      ip3n[c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
+      const double cs = u + DT_Ca * diff_u;
+      can[c] = cs;
+      t.ca2p_avg[c] = t.ca2p_avg[c] + cs;
+      const double para_k2 = 0.7;
+      const double exi = t.ex_inert[c];
+      t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);
+    }
+  }
+}
+
+// ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound
+__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int nky, int nzc) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int k = (blockIdx.z % nky) * 4 + (threadIdx.x >> 6);
+  const int zc = blockIdx.z / nky;
+  const int izb = ca_iz_bound(t);
+  if (izb > NZ || izb < 0) return;
+  if (j > NX || k > NY) return;
+  const double* __restrict__ carr = t.atp[ap];
+  double* __restrict__ narr = t.atp[1 - ap];
+  const double Da = 1.0, AIR_STIM = 0.1, STIM11 = 0.002, Kaa = 0.5;
+  const double inv_dx = t.inv_dx;
+  if (j < NX && k < NY) {
+    const uint8_t* __restrict__ cm2 = t.cmap2;
+    const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+    const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+    for (int l0 = zc * ZB; l0 < izb; l0 += nzc * ZB) {
+      const int l1 = min(l0 + ZB, izb);
+      for (int l = l0; l < l1; l++) {
+        const int prev_z = l == 0 ? 1 : l - 1, next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+        const long long v = gidx(t, j, k, l);
+        const double catp = carr[v];
+        const unsigned w = t.vox[v];
+        const unsigned cell1 = w & 0x7fffffffu;
+        const double diffu = cell1 ? t.diff_u[cell1 - 1] : 0.0;
+        const double air = (double)(w >> 31);
+        const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
+        const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
+        const long long vzm = gidx(t, j, k, prev_z), vzp = gidx(t, j, k, next_z);
+        const double fa = STIM11 * min0(diffu) - catp * Kaa;
+        const double nv = catp + DT_Ca * (Da * ((double)cm2[vxm] * (carr[vxm] - catp)
+                                                + (double)cm2[vxp] * (carr[vxp] - catp)
+                                                + (double)cm2[vym] * (carr[vym] - catp)
+                                                + (double)cm2[vyp] * (carr[vyp] - catp)
+                                                + (double)cm2[vzm] * (carr[vzm] - catp)
+                                                + (double)cm2[vzp] * (carr[vzp] - catp))
+                                          * inv_dx * inv_dx + fa + air * AIR_STIM);
+        narr[v] = nv;
+        if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+        if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+        if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+      }
+    }
+  }
+  // plane l = iz_bound: copies of values left in the next buffer (:334-337)
+  if (zc == 0) {
+    if (k == 0) narr[gidx(t, j, NY, izb)] = narr[gidx(t, j == NX ? 0 : j, 0, izb)];
+    if (j == 0 && k < NY) narr[gidx(t, NX, k, izb)] = narr[gidx(t, 0, k, izb)];
+  }
+}
+
+// set_cell_ca2p (:355-367) and the diff_u reset of update_values (:346-348);
+// ca_calc_condition_reset (cellmanager.h:101)
+__global__ void __launch_bounds__(256) k_ca_end(const Tis* __restrict__ T, const int* tlist, int full) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int n = t.st->n;
+  for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    if (!full) {
+    } else if (is_afm(t.state[c])) {
+      t.ca2p_avg[c] = t.ca2p_avg[c] / CA_ITR;
+    } else {
+      t.ca2p_avg[c] = 0;
+      t.ca2p[0][c] = 0;
+      t.ca2p[1][c] = 0;
+    }
+    t.diff_u[c] = 0;
+  }
+}
+
+__global__ void k_sw_reset(const Tis* __restrict__ T, const int* tlist) {
+  const Tis& t = T[tlist ? tlist[blockIdx.x] : blockIdx.x];
+  if (threadIdx.x == 0) t.st->sw = 0;
+}
+
+void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo) {
+  dim3 gg((unsigned)((max_gsize + 255) / 256 < 65535 ? (max_gsize + 255) / 256 : 65535), ntis);
+  hipLaunchKernelGGL(k_map_clear, gg, dim3(256), 0, s, dT);
+  hipLaunchKernelGGL(k_map_stamp, dim3((max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
+  hipLaunchKernelGGL(k_map_halo, dim3((max_halo + 255) / 256, ntis), dim3(256), 0, s, dT);
+}
+
+void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est) {
+  const int nky = (max_ny + 3) / 4;
+  const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
+  dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
+  hipLaunchKernelGGL(k_ext_stim, g, dim3(256), 0, s, dT, bp, nky, nzc);
+}
+
+static void ca_maps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box,
+                    int zero_avg) {
+  hipLaunchKernelGGL(k_ca_begin, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, zero_avg);
+  unsigned gb = (unsigned)((max_box + 255) / 256);
+  if (gb < 1) gb = 1;
+  if (gb > 65535) gb = 65535;
+  hipLaunchKernelGGL(k_ca_vox, dim3(gb, ntis), dim3(256), 0, s, dT, tlist);
+}
+
+void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box) {
+  ca_maps(s, dT, tlist, ntis, cb, max_n, max_box, 1);
+}
+
+void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box) {
+  ca_maps(s, dT, tlist, ntis, cb, max_n, max_box, 0);
+}
+
+void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n) {
+  hipLaunchKernelGGL(k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 0);
+}
+
+void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
+                       int max_n, int max_nx, int max_ny, int izb_est) {
+  hipLaunchKernelGGL(k_ca_cells, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, cp, ap, bp);
+  const int nky = (max_ny + 1 + 3) / 4;
+  const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
+  dim3 g((max_nx + 1 + 63) / 64, ntis, nky * nzc);
+  hipLaunchKernelGGL(k_atp, g, dim3(256), 0, s, dT, tlist, ap, nky, nzc);
+}
+
+void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n) {
+  hipLaunchKernelGGL(k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 1);
+  hipLaunchKernelGGL(k_sw_reset, dim3(ntis), dim3(64), 0, s, dT, tlist);
+}
+
+}  // namespace epi

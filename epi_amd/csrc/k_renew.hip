@@ -1,0 +1,408 @@
+// k_renew.hip -- cell_state_renew (cell_state_renew.cpp:400-429) on CDNA4.
+//
+// The reference walks the non-membrane cells in one loop; each cell only
+// changes itself and the daughter it appends, but the replayed random draws
+// are consumed in loop order and a division consumes extra draws.  So:
+//   k_renew_scan   one workgroup per tissue walks the cells in 1024-cell
+//                  chunks, compacts the division-ready cells and the removal
+//                  queue in index order with wave ballots, and decides the
+//                  divisions 64 ready cells at a time (a ballot finds the
+//                  first division of a batch; draws after it shift by its
+//                  direction draws);
+//   k_renew_apply  every cell applies its own rule in parallel; daughters are
+//                  written at n_old + division rank (parent order);
+//   k_remove       remove_exec's swap-with-tail, one workgroup per tissue;
+//   k_pair_disperse pair_disperse from the higher-index partner.
+#include "epi_dev.h"
+#include "launch.h"
+
+namespace epi {
+
+// cell_state_renew.cpp:17-119
+constexpr double stoch_div_time_ratio = 0.25;
+constexpr double S2 = 0.1;
+constexpr double eps_kb = 0.12;
+constexpr double accel_div = 1.0;
+constexpr double alpha_b = 5.0;
+constexpr double S0 = 0.1 * 0.2;
+constexpr double eps_ks = 0.10 * 0.5;
+constexpr double accel_diff = 1.0;
+constexpr double alpha_k = 2.0;
+constexpr double eps_kk = 0.10 * 0.5;
+constexpr double S1 = 0.1;
+constexpr double ubar = 0.45;
+constexpr double delta_lipid = 0.1;
+constexpr double delta_sig_r1 = 0.1;
+constexpr double lipid_rel = 0.05 * 4.0;
+constexpr double lipid_c = 0.6;
+constexpr double stoch_corr_coef = 1;
+constexpr double agki_max = 6.0;
+constexpr double delta_L = 0.01 * R_max;      // :231
+constexpr double ADHE_CONST = 31.3;           // :331
+constexpr unsigned DISA_conn_num_thresh = 11;  // :332
+constexpr double PI_D = 3.14159265358979323846;
+
+__device__ __forceinline__ double weighted_eps_kb(uint8_t malig) { return malig ? accel_div * eps_kb : eps_kb; }
+
+// is_divide_ready without the draw (:147-155), with the MUSUME/FIX guards of
+// _MUSUME_state_renew (:282-298) and _FIX_state_renew (:308-318)
+__device__ __forceinline__ bool needs_draw(const Tis& t, int c) {
+  const uint8_t s = t.state[c];
+  if (s == FIX) {
+    if (t.dermis[c] < 0) return false;
+  } else if (s == MUSUME) {
+    if (t.dermis[c] < 0 && t.pair[c] < 0) return false;
+    if (!(t.rest_div_times[c] > 0)) return false;
+  } else {
+    return false;
+  }
+  return t.pair[c] < 0 && (t.ageb[c] >= t.div_age_thresh[c] * (1.0 - stoch_div_time_ratio));
+}
+
+__device__ __forceinline__ bool to_remove(const Tis& t, int cb, int c) {
+  const uint8_t s = t.state[c];
+  return (s == DEAD || s == AIR) && t.agek[c] >= ADHE_CONST &&
+         (unsigned)t.nlen[cb][c] <= DISA_conn_num_thresh;
+}
+
+__global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, int pb, int cb) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  __shared__ int s_ready[1024];
+  __shared__ int s_wcnt[16], s_wrm[16];
+  __shared__ int s_nready, s_nrm_base;
+  __shared__ long long s_cursor;
+  __shared__ int s_ndiv;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int start = t.nmemb + t.nder;
+  const int n_old = st->n;
+  if (tid == 0) { s_cursor = st->rng_pos; s_ndiv = 0; s_nrm_base = 0; st->n_old = n_old; }
+  __syncthreads();
+  for (int base = start; base < n_old; base += 1024) {
+    const int c = base + tid;
+    const bool valid = c < n_old;
+    const bool rd = valid && needs_draw(t, c);
+    const bool rm = valid && to_remove(t, cb, c);
+    const unsigned long long mr = __ballot(rd), mm = __ballot(rm);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (lane == 0) { s_wcnt[wv] = __popcll(mr); s_wrm[wv] = __popcll(mm); }
+    __syncthreads();
+    int off_r = 0, off_m = 0, tot_r = 0, tot_m = 0;
+    for (int w = 0; w < 16; w++) {
+      if (w < wv) { off_r += s_wcnt[w]; off_m += s_wrm[w]; }
+      tot_r += s_wcnt[w]; tot_m += s_wrm[w];
+    }
+    if (rd) s_ready[off_r + __popcll(mr & below)] = c;
+    if (rm) t.rm_list[s_nrm_base + off_m + __popcll(mm & below)] = c;
+    __syncthreads();
+    if (wv == 0) {
+      // decide divisions in index order, 64 ready cells per round
+      long long cur = s_cursor;
+      int ndiv = s_ndiv;
+      int i = 0;
+      while (i < tot_r) {
+        const int idx = i + lane;
+        const bool act = idx < tot_r;
+        const int cell = act ? s_ready[idx] : 0;
+        bool dv = false;
+        if (act) {
+          const long long dpos = cur + lane;
+          if (dpos >= t.rng_n) {
+            set_err(st, ERR_RNG_EXHAUSTED);
+          } else {
+            const double u = t.rng[dpos];
+            dv = u * (t.div_age_thresh[cell] * stoch_div_time_ratio) <=
+                 stoch_corr_coef * DT_Cell * weighted_eps_kb(t.is_malignant[cell]) * S2;
+          }
+        }
+        const unsigned long long md = __ballot(dv);
+        const int k = min(64, tot_r - i);
+        if (md == 0ull) {
+          if (act) t.div_slot[cell] = -1;
+          cur += k;
+          i += k;
+        } else {
+          const int f = __ffsll((long long)md) - 1;
+          if (act && lane < f) t.div_slot[cell] = -1;
+          int att = 0;
+          if (lane == f) {
+            // div_direction (:192-217) with draws after the decision draw
+            const int der = t.dermis[cell];
+            const P3 me = ld_pos(t, pb, cell), dm = ld_pos(t, pb, der);
+            const double nvx = p_diff_x(t, me.x, dm.x);
+            const double nvy = p_diff_y(t, me.y, dm.y);
+            const double nvz = me.z - dm.z;
+            const double norm = sqrt(nvx * nvx + nvy * nvy + nvz * nvz);
+            const double nx = nvx / norm, ny = nvy / norm, nz = nvz / norm;
+            double ox, oy, oz, sum;
+            long long dp = cur + f + 1;
+            do {
+              if (dp + 1 >= t.rng_n) { set_err(st, ERR_RNG_EXHAUSTED); ox = 1; oy = 0; oz = 0; att++; break; }
+              const double rand_theta = PI_D * t.rng[dp];
+              const double cr1 = cos(rand_theta);
+              const double sr1 = sin(rand_theta);
+              const double rand_phi = 2 * PI_D * t.rng[dp + 1];
+              const double cr2 = cos(rand_phi);
+              const double sr2 = sin(rand_phi);
+              const double rvx = sr1 * cr2;
+              const double rvy = sr1 * sr2;
+              const double rvz = cr1;
+              ox = ny * rvz - nz * rvy;
+              oy = nz * rvx - nx * rvz;
+              oz = nx * rvy - ny * rvx;
+              dp += 2;
+              att++;
+            } while ((sum = ox * ox + oy * oy + oz * oz) < 1.0e-14);
+            sum = sqrt(sum);
+            t.div_slot[cell] = ndiv;
+            t.div_parent[ndiv] = cell;
+            t.div_dir[3 * ndiv + 0] = ox / sum;
+            t.div_dir[3 * ndiv + 1] = oy / sum;
+            t.div_dir[3 * ndiv + 2] = oz / sum;
+          }
+          att = __shfl(att, f, 64);
+          ndiv++;
+          cur += f + 1 + 2 * att;
+          i += f + 1;
+        }
+      }
+      if (lane == 0) { s_cursor = cur; s_ndiv = ndiv; s_nrm_base += tot_m; }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st->rng_pos = s_cursor;
+    st->ndiv = s_ndiv;
+    st->nrm = s_nrm_base;
+  }
+}
+
+// ageing and lipid laws (:31-119)
+__device__ __forceinline__ double ageb_const(double ca2p_avg, uint8_t malig) {
+  return weighted_eps_kb(malig) * (S2 + alpha_b * min0(ca2p_avg - ca2p_init));
+}
+__device__ __forceinline__ double agek_ALIVE_const(double ca2p_avg, uint8_t malig) {
+  return (malig ? accel_diff * eps_ks : eps_ks) * (S0 + alpha_k * min0(ca2p_avg - ca2p_init));
+}
+__device__ __forceinline__ double k_lipid_release(double ca2p_avg, double agek) {
+  return 0.25 * lipid_rel * (1 + tanh((ca2p_avg - ubar) / delta_sig_r1)) *
+         (1 + tanh((agek - THRESH_SP) / delta_lipid));
+}
+__device__ __forceinline__ double k_lipid(double ca2p_avg, double agek) {
+  return 0.25 * lipid_c * (1 + tanh((ubar - ca2p_avg) / delta_sig_r1)) *
+         (1 + tanh((agek - THRESH_SP) / delta_lipid));
+}
+
+// cell_divide (:227-270) for division rank k of parent c
+__device__ void divide(const Tis& t, int pb, int cpar, int c, int k, int n_old, long long next_uid) {
+  const int d = n_old + k;
+  if (d >= t.cap) { set_err(t.st, ERR_CAPACITY); return; }
+  const P3 p = ld_pos(t, pb, c);
+  const double divx = t.div_dir[3 * k], divy = t.div_dir[3 * k + 1], divz = t.div_dir[3 * k + 2];
+  const double cav = t.ca2p[cpar][c];
+  const double ipv = t.ip3[cpar][c];
+  const int rdt = t.rest_div_times[c];
+  const bool mus = t.state[c] == MUSUME;
+  // daughter: CellManager::create(MUSUME, ...) (cellmanager.cpp:366-390)
+  t.state[d] = MUSUME;
+  t.fix_origin[d] = t.fix_origin[c];
+  t.radius[d] = t.radius[c];
+  t.div_age_thresh[d] = agki_max;
+  t.ca2p[0][d] = cav; t.ca2p[1][d] = cav;
+  t.ca2p_avg[d] = cav;
+  t.ip3[0][d] = ipv; t.ip3[1][d] = ipv;
+  t.ex_inert[d] = t.ex_inert[c];
+  t.agek[d] = 0; t.ageb[d] = 0; t.ex_fat[d] = 0; t.in_fat[d] = 0;
+  t.spr_nat_len[d] = delta_L;
+  t.rest_div_times[d] = mus ? rdt - 1 : rdt;
+  t.is_malignant[d] = t.is_malignant[c];
+  t.pair[d] = c;
+  t.dermis[d] = -1;
+  t.diff_u[d] = 0;
+  t.uid[d] = next_uid + k;
+  t.src_row[d] = -1;
+  t.touch_air[d] = 0;
+  t.lat[3 * d] = 0; t.lat[3 * d + 1] = 0; t.lat[3 * d + 2] = 0;
+  t.pos[pb][0][d] = p.x - divx * 0.5 * delta_L;
+  t.pos[pb][1][d] = p.y - divy * 0.5 * delta_L;
+  t.pos[pb][2][d] = p.z - divz * 0.5 * delta_L;
+  // parent
+  t.pair[c] = d;
+  t.spr_nat_len[c] = delta_L;
+  t.ageb[c] = 0;
+  if (mus) t.rest_div_times[c] = rdt - 1;
+  t.pos[pb][0][c] = p.x + divx * 0.5 * delta_L;
+  t.pos[pb][1][c] = p.y + divy * 0.5 * delta_L;
+  t.pos[pb][2][c] = p.z + divz * 0.5 * delta_L;
+}
+
+__global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, int pb, int cb, int cpar) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  const int n_old = st->n_old;
+  const long long next_uid = st->next_uid;
+  int ncorn = 0;
+  for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n_old;
+       c += gridDim.x * blockDim.x) {
+    switch (t.state[c]) {
+      case ALIVE: {   // _ALIVE_state_renew :347-358
+        const double agek = t.agek[c];
+        if (agek >= THRESH_DEAD) {
+          t.state[c] = DEAD;   // cornificate (cellmanager.cpp:48-53)
+          ncorn++;
+        } else {
+          const double cav = t.ca2p_avg[c];
+          const double in_fat = t.in_fat[c];
+          const double tmp = k_lipid_release(cav, agek) * in_fat;
+          t.in_fat[c] = in_fat + DT_Cell * (k_lipid(cav, agek) * (1.0 - in_fat) - tmp);
+          t.ex_fat[c] = t.ex_fat[c] + DT_Cell * tmp;
+          t.agek[c] = agek + DT_Cell * agek_ALIVE_const(cav, t.is_malignant[c]);
+        }
+        break;
+      }
+      case DEAD: case AIR:   // _DEAD_AIR_state_renew :330-341
+        if (!to_remove(t, cb, c)) t.agek[c] = t.agek[c] + DT_Cell * (eps_kk * S1);
+        break;
+      case FIX: {   // _FIX_state_renew :308-324
+        if (t.dermis[c] < 0) { set_err(st, ERR_NO_DERMIS); break; }
+        const bool rd = needs_draw(t, c);
+        const int k = rd ? t.div_slot[c] : -1;
+        if (k >= 0) divide(t, pb, cpar, c, k, n_old, next_uid);
+        else t.ageb[c] = t.ageb[c] + DT_Cell * ageb_const(t.ca2p_avg[c], t.is_malignant[c]);
+        break;
+      }
+      case MUSUME: {   // _MUSUME_state_renew :282-302 (SYSTEM == WHOLE)
+        if (t.dermis[c] < 0 && t.pair[c] < 0) { t.state[c] = ALIVE; break; }
+        const bool rd = needs_draw(t, c);
+        const int k = rd ? t.div_slot[c] : -1;
+        if (k >= 0) divide(t, pb, cpar, c, k, n_old, next_uid);
+        else t.ageb[c] = t.ageb[c] + DT_Cell * ageb_const(t.ca2p_avg[c], t.is_malignant[c]);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  if (ncorn) atomicAdd(&st->sw, ncorn);
+}
+
+// CellManager::remove_exec (cellmanager.cpp:338-352): queued slot q_i takes
+// the cell at the current tail N-1-i.  A queued slot at or past the current
+// end means a queued cell had already been moved (the reference deletes a
+// live cell there): ERR_REMOVE_ALIAS.
+__global__ void __launch_bounds__(256) k_remove(const Tis* __restrict__ T, int pb) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  const int ndiv = st->ndiv, nrm = st->nrm;
+  const int N = st->n_old + ndiv;
+  for (int i = threadIdx.x; i < nrm; i += blockDim.x) {
+    const int q = t.rm_list[i];
+    if (q >= N - i) { set_err(st, ERR_REMOVE_ALIAS); continue; }
+    if (t.pair[q] >= 0) set_err(st, ERR_PAIR_REMOVED);
+    const int s = N - 1 - i;
+    if (s == q) continue;
+    t.state[q] = t.state[s]; t.is_malignant[q] = t.is_malignant[s]; t.touch_air[q] = t.touch_air[s];
+    t.fix_origin[q] = t.fix_origin[s]; t.rest_div_times[q] = t.rest_div_times[s];
+    t.pair[q] = t.pair[s]; t.dermis[q] = t.dermis[s];
+    for (int a = 0; a < 3; a++) { t.lat[3 * q + a] = t.lat[3 * s + a]; t.pos[pb][a][q] = t.pos[pb][a][s]; }
+    t.radius[q] = t.radius[s]; t.div_age_thresh[q] = t.div_age_thresh[s];
+    t.ca2p[0][q] = t.ca2p[0][s]; t.ca2p[1][q] = t.ca2p[1][s];
+    t.ip3[0][q] = t.ip3[0][s]; t.ip3[1][q] = t.ip3[1][s];
+    t.ca2p_avg[q] = t.ca2p_avg[s]; t.ex_inert[q] = t.ex_inert[s]; t.agek[q] = t.agek[s];
+    t.ageb[q] = t.ageb[s]; t.ex_fat[q] = t.ex_fat[s]; t.in_fat[q] = t.in_fat[s];
+    t.spr_nat_len[q] = t.spr_nat_len[s]; t.diff_u[q] = t.diff_u[s];
+    t.uid[q] = t.uid[s]; t.src_row[q] = t.src_row[s];
+  }
+  __syncthreads();
+  // pair pointers follow the moved cells (Cell* identity in the reference)
+  for (int i = threadIdx.x; i < nrm; i += blockDim.x) {
+    const int q = t.rm_list[i];
+    const int s = N - 1 - i;
+    if (q >= N - i || s == q) continue;
+    const int p = t.pair[q];
+    if (p < 0) continue;
+    int np = p;
+    if (p >= N - nrm) {
+      const int j = N - 1 - p;
+      np = t.rm_list[j];
+      if (np == p || np >= N - j) { set_err(st, ERR_PAIR_REMOVED); continue; }
+    } else {
+      t.pair[p] = q;
+    }
+    t.pair[q] = np;
+  }
+  if (threadIdx.x == 0) {
+    st->n = N - nrm;
+    st->next_uid += ndiv;
+  }
+}
+
+// pair_disperse (:364-385), from the higher-index partner (:422-428)
+__global__ void __launch_bounds__(256) k_pair_disperse(const Tis* __restrict__ T, int pb) {
+  const Tis& t = T[blockIdx.y];
+  const int n = t.st->n;
+  for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int p = t.pair[c];
+    if (p < 0 || !(c > p)) continue;
+    const double eps_L = 0.14, unpair_dist_coef = 0.9;
+    const double rad_sum = t.radius[c] + t.radius[p];
+    const double unpair_th = unpair_dist_coef * rad_sum;
+    const double snl = t.spr_nat_len[c];
+    if (snl < 2.0 * t.radius[c]) {
+      const double v = snl + DT_Cell * eps_L;
+      t.spr_nat_len[c] = v;
+      t.spr_nat_len[p] = v;
+    } else if (dist_sq(t, ld_pos(t, pb, c), ld_pos(t, pb, p)) > unpair_th * unpair_th) {
+      t.spr_nat_len[c] = 0;
+      t.spr_nat_len[p] = 0;
+      t.pair[p] = -1;
+      t.pair[c] = -1;
+    }
+  }
+}
+
+// initialize_sc (:435-443)
+__global__ void __launch_bounds__(256) k_initialize_sc(const Tis* __restrict__ T, const int* tlist, int pb) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int n = t.st->n;
+  const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
+  for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    if (t.state[c] == ALIVE && zzmax - t.pos[pb][2][c] < 8 * R_max) {
+      t.agek[c] = t.agek[c] > THRESH_DEAD ? t.agek[c] : THRESH_DEAD;
+      t.state[c] = AIR;
+    }
+  }
+}
+
+// cell_pos_periodic_fix (cellmanager.cpp:395-414)
+__global__ void __launch_bounds__(256) k_periodic_fix(const Tis* __restrict__ T, int pb) {
+  const Tis& t = T[blockIdx.y];
+  const int n = t.st->n;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const double x = t.pos[pb][0][c], y = t.pos[pb][1][c];
+    if (x > t.LX) t.pos[pb][0][c] = x - t.LX;
+    else if (x < 0) t.pos[pb][0][c] = x + t.LX;
+    if (y > t.LY) t.pos[pb][1][c] = y - t.LY;
+    else if (y < 0) t.pos[pb][1][c] = y + t.LY;
+  }
+}
+
+void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n) {
+  hipLaunchKernelGGL(k_renew_scan, dim3(1, ntis), dim3(1024), 0, s, dT, pb, cb);
+  dim3 g((max_n + 255) / 256, ntis);
+  hipLaunchKernelGGL(k_renew_apply, g, dim3(256), 0, s, dT, pb, cb, cpar);
+  hipLaunchKernelGGL(k_remove, dim3(1, ntis), dim3(256), 0, s, dT, pb);
+  hipLaunchKernelGGL(k_pair_disperse, g, dim3(256), 0, s, dT, pb);
+}
+
+void launch_initialize_sc(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int pb, int max_n) {
+  dim3 g((max_n + 255) / 256, ntis);
+  hipLaunchKernelGGL(k_initialize_sc, g, dim3(256), 0, s, dT, tlist, pb);
+}
+
+void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {
+  dim3 g((max_n + 255) / 256, ntis);
+  hipLaunchKernelGGL(k_periodic_fix, g, dim3(256), 0, s, dT, pb);
+}
+
+}  // namespace epi

@@ -1,0 +1,381 @@
+"""ctypes binding of the HIP library (include/epi_amd.h).
+
+The package has no CPU fallback: if libepi_amd.so is missing the import of
+this module fails with the build command to run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libepi_amd.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
+
+ERRORS = {0: "OK", 1: "CAPACITY", 2: "STRONG_INTERACTION", 3: "NO_DERMIS",
+          4: "OUT_OF_GRID", 5: "REMOVE_ALIAS", 6: "RNG_EXHAUSTED", 7: "INVALID",
+          8: "HIP", 9: "PAIR_REMOVED"}
+
+
+class EpiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+class Geometry(C.Structure):
+    _fields_ = [("nmx", C.c_int32), ("nmy", C.c_int32), ("nx", C.c_int32),
+                ("ny", C.c_int32), ("nz", C.c_int32), ("lx", C.c_double),
+                ("ly", C.c_double), ("lz", C.c_double), ("max_cells", C.c_int32),
+                ("row_stride", C.c_int32), ("bucket_cap", C.c_int32)]
+
+
+class CellInput(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nmemb", C.c_int32), ("nder", C.c_int32),
+                ("state", _u8p), ("fix_origin", _ip),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("radius", _dp), ("ca2p", _dp),
+                ("ca2p_avg", _dp), ("ip3", _dp), ("ex_inert", _dp),
+                ("agek", _dp), ("ageb", _dp), ("ex_fat", _dp), ("in_fat", _dp),
+                ("spr_nat_len", _dp), ("rest_div_times", _ip), ("is_malignant", _u8p),
+                ("pair", _ip)]
+
+
+class Cells(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nmemb", C.c_int32), ("nder", C.c_int32),
+                ("sw", C.c_int32),
+                ("state", _u8p), ("fix_origin", _ip),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("xn", _dp), ("yn", _dp), ("zn", _dp),
+                ("radius", _dp),
+                ("ca2p", _dp), ("ca2p_next", _dp), ("ip3", _dp), ("ip3_next", _dp),
+                ("ca2p_avg", _dp), ("ex_inert", _dp), ("agek", _dp), ("ageb", _dp),
+                ("ex_fat", _dp), ("in_fat", _dp), ("spr_nat_len", _dp), ("diff_u", _dp),
+                ("rest_div_times", _ip), ("pair", _ip), ("dermis", _ip), ("lat", _ip),
+                ("is_malignant", _u8p), ("uid", _i64p),
+                ("conn_ptr", _ip), ("conn", _ip), ("gj", _dp), ("conn_cap", C.c_int32),
+                ("memb_adj", _ip)]
+
+
+class Driver(C.Structure):
+    _fields_ = [("step", C.c_int64), ("force_cornif", C.c_int32), ("num_sc", C.c_int32),
+                ("ca_every", C.c_int32), ("clean_every", C.c_int32)]
+
+
+class Status(C.Structure):
+    _fields_ = [("n", C.c_int32), ("err", C.c_int32), ("sw", C.c_int32), ("ndiv", C.c_int32),
+                ("nrm", C.c_int32), ("maxlen", C.c_int32), ("rng_pos", C.c_int64),
+                ("next_uid", C.c_int64), ("zzmax", C.c_double), ("ca_solves", C.c_int64)]
+
+
+CELL_DTYPES = {
+    "state": np.uint8, "fix_origin": np.int32, "x": np.float64, "y": np.float64,
+    "z": np.float64, "xn": np.float64, "yn": np.float64, "zn": np.float64,
+    "radius": np.float64, "ca2p": np.float64, "ca2p_next": np.float64,
+    "ip3": np.float64, "ip3_next": np.float64, "ca2p_avg": np.float64,
+    "ex_inert": np.float64, "agek": np.float64, "ageb": np.float64,
+    "ex_fat": np.float64, "in_fat": np.float64, "spr_nat_len": np.float64,
+    "diff_u": np.float64, "rest_div_times": np.int32, "pair": np.int32,
+    "dermis": np.int32, "is_malignant": np.uint8, "uid": np.int64,
+}
+
+SYMBOLS = [
+    "epi_create", "epi_destroy", "epi_last_error", "epi_num_tissues", "epi_load_cells",
+    "epi_value_init", "epi_set_cell_values", "epi_set_fields", "epi_set_rng",
+    "epi_rng_mt19937_64", "epi_cell_interaction", "epi_pos_copy", "epi_cell_state_renew",
+    "epi_cell_pos_periodic_fix", "epi_connect_cell", "epi_calc_zzmax", "epi_setup_map_lat",
+    "epi_calc_ext_stim", "epi_initialize_sc", "epi_calc_ca2p", "epi_should_calc_ca",
+    "epi_ca_calc_condition_reset", "epi_ca_substep", "epi_step", "epi_synchronize",
+    "epi_get_status", "epi_get_cells", "epi_get_fields", "epi_set_positions",
+    "epi_set_state", "epi_set_lists", "epi_set_maps",
+]
+
+_lib = None
+
+
+def load_library():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python epi_amd/build.py` "
+                          "(or __graft_entry__.build()) first; there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.epi_create.argtypes = [C.c_int32, C.POINTER(Geometry), C.POINTER(vp)]
+    L.epi_destroy.argtypes = [vp]
+    L.epi_destroy.restype = None
+    L.epi_last_error.argtypes = [vp]
+    L.epi_last_error.restype = C.c_char_p
+    L.epi_num_tissues.argtypes = [vp]
+    L.epi_load_cells.argtypes = [vp, C.c_int32, C.POINTER(CellInput)]
+    L.epi_value_init.argtypes = [vp, C.c_int32]
+    L.epi_set_cell_values.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]
+    L.epi_set_fields.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp]
+    L.epi_set_rng.argtypes = [vp, C.c_int32, _dp, C.c_int64]
+    L.epi_rng_mt19937_64.argtypes = [C.c_uint64, _dp, C.c_int64]
+    for f in ("epi_cell_interaction", "epi_pos_copy", "epi_cell_state_renew",
+              "epi_cell_pos_periodic_fix", "epi_connect_cell", "epi_setup_map_lat",
+              "epi_ca_calc_condition_reset", "epi_synchronize"):
+        getattr(L, f).argtypes = [vp]
+    L.epi_calc_zzmax.argtypes = [vp, _dp]
+    for f in ("epi_calc_ext_stim", "epi_initialize_sc", "epi_calc_ca2p"):
+        getattr(L, f).argtypes = [vp, _dp]
+    L.epi_should_calc_ca.argtypes = [vp, _ip]
+    L.epi_ca_substep.argtypes = [vp, C.c_int32]
+    L.epi_step.argtypes = [vp, C.c_int64, C.POINTER(Driver)]
+    L.epi_get_status.argtypes = [vp, C.c_int32, C.POINTER(Status)]
+    L.epi_get_cells.argtypes = [vp, C.c_int32, C.POINTER(Cells)]
+    L.epi_get_fields.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp, _ip, _u8p]
+    L.epi_set_positions.argtypes = [vp, C.c_int32, _dp, _dp, _dp]
+    L.epi_set_state.argtypes = [vp, C.c_int32, _u8p, _dp]
+    L.epi_set_lists.argtypes = [vp, C.c_int32, _ip, _ip, _dp]
+    L.epi_set_maps.argtypes = [vp, C.c_int32, _ip, _u8p, _ip]
+    _lib = L
+    return L
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def mt19937_64_uniform(seed, n):
+    """std::mt19937_64(seed) through uniform_real_distribution<double>(0,1)."""
+    out = np.empty(n, dtype=np.float64)
+    rc = load_library().epi_rng_mt19937_64(C.c_uint64(seed), _ptr(out, _dp), n)
+    if rc:
+        raise EpiError(rc, "epi_rng_mt19937_64")
+    return out
+
+
+def _geo(g, row_stride=0, bucket_cap=0):
+    return Geometry(g.nmx, g.nmy, g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.max_cells,
+                    row_stride, bucket_cap)
+
+
+class TissueSet:
+    """One context: ``len(geometries)`` tissues advanced together."""
+
+    def __init__(self, geometries, row_stride=0, bucket_cap=0):
+        self.L = load_library()
+        self.geos = list(geometries)
+        self.T = len(self.geos)
+        arr = (Geometry * self.T)(*[_geo(g, row_stride, bucket_cap) for g in self.geos])
+        h = C.c_void_p()
+        rc = self.L.epi_create(self.T, arr, C.byref(h))
+        if rc:
+            raise EpiError(rc, "epi_create")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.epi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what=""):
+        if rc:
+            raise EpiError(rc, f"{what}: {self.L.epi_last_error(self.h).decode()}")
+
+    # --- load (cman_init) ---
+    def load(self, t, spec, value_init=False):
+        c = spec["cells"]
+        n = c["x"].size
+        inp = CellInput()
+        inp.n, inp.nmemb, inp.nder = n, spec["nmemb"], spec["nder"]
+        conv = {}
+        for k, dt, pt in (("state", np.uint8, _u8p), ("fix_origin", np.int32, _ip),
+                          ("x", np.float64, _dp), ("y", np.float64, _dp), ("z", np.float64, _dp),
+                          ("radius", np.float64, _dp), ("ca2p", np.float64, _dp),
+                          ("ca2p_avg", np.float64, _dp), ("ip3", np.float64, _dp),
+                          ("ex_inert", np.float64, _dp), ("agek", np.float64, _dp),
+                          ("ageb", np.float64, _dp), ("ex_fat", np.float64, _dp),
+                          ("in_fat", np.float64, _dp), ("spr_nat_len", np.float64, _dp),
+                          ("rest_div_times", np.int32, _ip), ("is_malignant", np.uint8, _u8p),
+                          ("pair", np.int32, _ip)):
+            conv[k] = np.ascontiguousarray(c[k], dtype=dt)
+            setattr(inp, k, _ptr(conv[k], pt))
+        self._keep[("in", t)] = conv
+        self._chk(self.L.epi_load_cells(self.h, t, C.byref(inp)), "load_cells")
+        self._chk(self.L.epi_connect_cell(self.h), "connect_cell")
+        vals = spec.get("values")
+        if value_init or vals is None:
+            self._chk(self.L.epi_value_init(self.h, t), "value_init")
+        else:
+            self.set_cell_values(t, **vals)
+        f = spec.get("fields")
+        if f is not None:
+            self.set_fields(t, f["atp"], f["atp"], f["b"], f["b"])
+
+    def set_cell_values(self, t, ca2p=None, ip3=None, ex_inert=None, ca2p_avg=None, gj=None):
+        cv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        ca2p, ip3, ex_inert, ca2p_avg, gj = map(cv, (ca2p, ip3, ex_inert, ca2p_avg, gj))
+        self._chk(self.L.epi_set_cell_values(self.h, t, _ptr(ca2p, _dp), _ptr(ip3, _dp),
+                                             _ptr(ex_inert, _dp), _ptr(ca2p_avg, _dp),
+                                             _ptr(gj, _dp)), "set_cell_values")
+
+    def set_fields(self, t, atp=None, atp_next=None, b=None, b_next=None):
+        cv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+        a, an, bb, bn = map(cv, (atp, atp_next, b, b_next))
+        self._chk(self.L.epi_set_fields(self.h, t, _ptr(a, _dp), _ptr(an, _dp), _ptr(bb, _dp),
+                                        _ptr(bn, _dp)), "set_fields")
+
+    def set_rng(self, t, draws):
+        d = np.ascontiguousarray(draws, dtype=np.float64)
+        self._chk(self.L.epi_set_rng(self.h, t, _ptr(d, _dp), d.size), "set_rng")
+
+    # --- reference call surface (every tissue) ---
+    def cell_interaction(self): self._chk(self.L.epi_cell_interaction(self.h), "cell_interaction")
+    def pos_copy(self): self._chk(self.L.epi_pos_copy(self.h), "pos_copy")
+    def cell_state_renew(self): self._chk(self.L.epi_cell_state_renew(self.h), "cell_state_renew")
+    def cell_pos_periodic_fix(self): self._chk(self.L.epi_cell_pos_periodic_fix(self.h), "periodic_fix")
+    def connect_cell(self): self._chk(self.L.epi_connect_cell(self.h), "connect_cell")
+    def setup_map_lat(self): self._chk(self.L.epi_setup_map_lat(self.h), "setup_map_lat")
+    def ca_calc_condition_reset(self): self._chk(self.L.epi_ca_calc_condition_reset(self.h), "reset")
+    def synchronize(self): self._chk(self.L.epi_synchronize(self.h), "synchronize")
+
+    def _zz(self, zzmax):
+        if zzmax is None:
+            return None
+        z = np.ascontiguousarray(np.broadcast_to(np.asarray(zzmax, dtype=np.float64), (self.T,)))
+        self._keep["zz"] = z
+        return _ptr(z, _dp)
+
+    def calc_zzmax(self):
+        out = np.zeros(self.T)
+        self._chk(self.L.epi_calc_zzmax(self.h, _ptr(out, _dp)), "calc_zzmax")
+        return out
+
+    def calc_ext_stim(self, zzmax): self._chk(self.L.epi_calc_ext_stim(self.h, self._zz(zzmax)), "ext_stim")
+    def initialize_sc(self, zzmax): self._chk(self.L.epi_initialize_sc(self.h, self._zz(zzmax)), "initialize_sc")
+    def calc_ca2p(self, zzmax): self._chk(self.L.epi_calc_ca2p(self.h, self._zz(zzmax)), "calc_ca2p")
+    def ca_substep(self, iz_bound): self._chk(self.L.epi_ca_substep(self.h, iz_bound), "ca_substep")
+
+    def should_calc_ca(self):
+        f = np.zeros(self.T, dtype=np.int32)
+        self._chk(self.L.epi_should_calc_ca(self.h, _ptr(f, _ip)), "should_calc_ca")
+        return f.astype(bool)
+
+    def step(self, drivers, nsteps=1):
+        arr = (Driver * self.T)(*drivers)
+        rc = self.L.epi_step(self.h, nsteps, arr)
+        for i in range(self.T):
+            drivers[i].step, drivers[i].force_cornif = arr[i].step, arr[i].force_cornif
+            drivers[i].num_sc = arr[i].num_sc
+        self._chk(rc, "step")
+
+    # --- state exchange ---
+    def status(self, t):
+        s = Status()
+        rc = self.L.epi_get_status(self.h, t, C.byref(s))
+        return s, rc
+
+    def get_cells(self, t):
+        s, _ = self.status(t)
+        n = s.n
+        ptrs = np.zeros(n + 1, dtype=np.int32)
+        probe = Cells()
+        probe.conn_ptr = _ptr(ptrs, _ip)
+        self._chk(self.L.epi_get_cells(self.h, t, C.byref(probe)), "get_cells")
+        nnz = int(ptrs[n])
+        out = {k: np.zeros(n, dtype=dt) for k, dt in CELL_DTYPES.items()}
+        out["lat"] = np.zeros(3 * n, dtype=np.int32)
+        out["conn_ptr"] = np.zeros(n + 1, dtype=np.int32)
+        out["conn"] = np.zeros(max(nnz, 1), dtype=np.int32)
+        out["gj"] = np.zeros(max(nnz, 1), dtype=np.float64)
+        out["memb_adj"] = np.zeros(6 * probe.nmemb, dtype=np.int32)
+        cs = Cells()
+        for k, dt in CELL_DTYPES.items():
+            pt = {np.uint8: _u8p, np.int32: _ip, np.float64: _dp, np.int64: _i64p}[dt]
+            setattr(cs, k, _ptr(out[k], pt))
+        cs.lat = _ptr(out["lat"], _ip)
+        cs.conn_ptr = _ptr(out["conn_ptr"], _ip)
+        cs.conn = _ptr(out["conn"], _ip)
+        cs.gj = _ptr(out["gj"], _dp)
+        cs.conn_cap = out["conn"].size
+        cs.memb_adj = _ptr(out["memb_adj"], _ip)
+        self._chk(self.L.epi_get_cells(self.h, t, C.byref(cs)), "get_cells")
+        out["conn"] = out["conn"][:nnz]
+        out["gj"] = out["gj"][:nnz]
+        out["n"], out["nmemb"], out["nder"], out["sw"] = cs.n, cs.nmemb, cs.nder, cs.sw
+        return out
+
+    def get_fields(self, t):
+        g = self.geos[t]
+        shape = (g.nx + 1, g.ny + 1, g.nz + 1)
+        sz = int(np.prod(shape))
+        o = {k: np.zeros(sz, dtype=np.float64) for k in ("atp", "atp_next", "b", "b_next")}
+        o["cmap1"] = np.zeros(sz, dtype=np.int32)
+        o["cmap2"] = np.zeros(sz, dtype=np.uint8)
+        self._chk(self.L.epi_get_fields(self.h, t, _ptr(o["atp"], _dp), _ptr(o["atp_next"], _dp),
+                                        _ptr(o["b"], _dp), _ptr(o["b_next"], _dp),
+                                        _ptr(o["cmap1"], _ip), _ptr(o["cmap2"], _u8p)), "get_fields")
+        return {k: v.reshape(shape) for k, v in o.items()}
+
+    def set_positions(self, t, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        self._chk(self.L.epi_set_positions(self.h, t, _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp)), "set_positions")
+
+    def set_state(self, t, state=None, agek=None):
+        s = None if state is None else np.ascontiguousarray(state, dtype=np.uint8)
+        a = None if agek is None else np.ascontiguousarray(agek, dtype=np.float64)
+        self._chk(self.L.epi_set_state(self.h, t, _ptr(s, _u8p), _ptr(a, _dp)), "set_state")
+
+    def set_lists(self, t, conn_ptr, conn, gj=None):
+        cp = np.ascontiguousarray(conn_ptr, dtype=np.int32)
+        cn = np.ascontiguousarray(conn, dtype=np.int32)
+        g = None if gj is None else np.ascontiguousarray(gj, dtype=np.float64)
+        self._chk(self.L.epi_set_lists(self.h, t, _ptr(cp, _ip), _ptr(cn, _ip), _ptr(g, _dp)), "set_lists")
+
+    def set_maps(self, t, cmap1=None, cmap2=None, lat=None):
+        c1 = None if cmap1 is None else np.ascontiguousarray(cmap1, dtype=np.int32).ravel()
+        c2 = None if cmap2 is None else np.ascontiguousarray(cmap2, dtype=np.uint8).ravel()
+        la = None if lat is None else np.ascontiguousarray(lat, dtype=np.int32).ravel()
+        self._chk(self.L.epi_set_maps(self.h, t, _ptr(c1, _ip), _ptr(c2, _u8p), _ptr(la, _ip)), "set_maps")
+
+
+class Tissue:
+    """Single-tissue view with the same methods as the oracle wrapper."""
+
+    def __init__(self, geometry, **kw):
+        self.set = TissueSet([geometry], **kw)
+        self.geo = geometry
+
+    def load(self, spec, value_init=False): self.set.load(0, spec, value_init)
+    def set_rng(self, draws): self.set.set_rng(0, draws)
+    def set_cell_values(self, **kw): self.set.set_cell_values(0, **kw)
+    def set_fields(self, *a, **kw): self.set.set_fields(0, *a, **kw)
+    def cell_interaction(self): self.set.cell_interaction()
+    def pos_copy(self): self.set.pos_copy()
+    def cell_state_renew(self): self.set.cell_state_renew()
+    def cell_pos_periodic_fix(self): self.set.cell_pos_periodic_fix()
+    def connect_cell(self): self.set.connect_cell()
+    def calc_zzmax(self): return float(self.set.calc_zzmax()[0])
+    def setup_map_lat(self): self.set.setup_map_lat()
+    def calc_ext_stim(self, zzmax): self.set.calc_ext_stim(zzmax)
+    def initialize_sc(self, zzmax): self.set.initialize_sc(zzmax)
+    def calc_ca2p(self, zzmax): self.set.calc_ca2p(zzmax)
+    def ca_substep(self, iz_bound): self.set.ca_substep(iz_bound)
+    def should_calc_ca(self): return bool(self.set.should_calc_ca()[0])
+    def ca_calc_condition_reset(self): self.set.ca_calc_condition_reset()
+    def step(self, driver, nsteps=1): self.set.step([driver], nsteps)
+    def status(self): return self.set.status(0)[0]
+    def rng_cursor(self): return int(self.status().rng_pos)
+    def error(self): return int(self.set.status(0)[0].err)
+    def get_cells(self): return self.set.get_cells(0)
+    def get_fields(self): return self.set.get_fields(0)
+    def set_positions(self, x, y, z): self.set.set_positions(0, x, y, z)
+    def set_state(self, state=None, agek=None): self.set.set_state(0, state, agek)
+    def set_lists(self, conn_ptr, conn, gj=None): self.set.set_lists(0, conn_ptr, conn, gj)
+    def set_maps(self, cmap1=None, cmap2=None, lat=None): self.set.set_maps(0, cmap1, cmap2, lat)
+    def close(self): self.set.close()

@@ -1,0 +1,164 @@
+/*
+ * epi_amd.h -- C ABI of the MI355X implementation of the epi5_MT per-step
+ * hot path (apqw/epi).  Plain pointers and sizes only.
+ *
+ * One context holds one or more tissues (a batch) bound to one HIP stream;
+ * every call of the reference call surface advances all tissues of the
+ * context in the same kernel launches.  A context is not thread-safe;
+ * different contexts may be used from different threads.  All entry points
+ * return 0 (EPI_OK) or an EPI_ERR_* code; epi_last_error() gives text.
+ * Host buffers belong to the caller, device memory to the library.
+ *
+ * Arrays of cells are in cell-index order; fields are (nx+1)*(ny+1)*(nz+1)
+ * doubles in the reference's [x][y][z] order (z fastest, Field.h:9-53).
+ */
+#ifndef EPI_AMD_H
+#define EPI_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    EPI_OK = 0,
+    EPI_ERR_CAPACITY = 1,            /* bucket >= 200, list >= 400, cells > capacity */
+    EPI_ERR_STRONG_INTERACTION = 2,  /* |coef| > 1000 (cell_interaction.cpp:51) or membrane > 100 */
+    EPI_ERR_NO_DERMIS = 3,           /* FIX without a membrane contact (cell_state_renew.cpp:309) */
+    EPI_ERR_OUT_OF_GRID = 4,         /* cell outside the bucket or voxel grid */
+    EPI_ERR_REMOVE_ALIAS = 5,        /* remove_exec would delete a moved cell (cellmanager.cpp:338-352) */
+    EPI_ERR_RNG_EXHAUSTED = 6,       /* replay stream shorter than the draws needed */
+    EPI_ERR_INVALID = 7,             /* bad argument or inconsistent input */
+    EPI_ERR_HIP = 8,                 /* HIP runtime failure */
+    EPI_ERR_PAIR_REMOVED = 9         /* a removed cell still had a division partner */
+};
+
+/* cell states (define.h:36-47) */
+enum { EPI_ALIVE = 0, EPI_DEAD = 1, EPI_DISA = 2, EPI_UNUSED = 3, EPI_FIX = 4,
+       EPI_BLANK = 5, EPI_DER = 6, EPI_MUSUME = 7, EPI_AIR = 8, EPI_MEMB = 9 };
+
+typedef struct epi_ctx epi_ctx;
+
+/* Runtime form of define.h:72-113 (NMX, NMY_tri, NX/NY/NZ, LX/LY/LZ,
+ * MAX_CELL_NUM).  row_stride caps a neighbour list (<= 400, the reference's
+ * MAX_CONNECT_CELL_NUM); bucket_cap caps a bucket (<= 200, N3). 0 = default. */
+typedef struct epi_geometry {
+    int32_t nmx, nmy;
+    int32_t nx, ny, nz;
+    double lx, ly, lz;
+    int32_t max_cells;
+    int32_t row_stride;
+    int32_t bucket_cap;
+} epi_geometry;
+
+/* Input rows of the deck (cellmanager.cpp:243-302): MEMB rows first, then
+ * DER, then the rest; pair[i] is the partner row or -1. */
+typedef struct epi_cell_input {
+    int32_t n, nmemb, nder;
+    const uint8_t* state; const int32_t* fix_origin;
+    const double *x, *y, *z, *radius, *ca2p, *ca2p_avg, *ip3, *ex_inert;
+    const double *agek, *ageb, *ex_fat, *in_fat, *spr_nat_len;
+    const int32_t* rest_div_times; const uint8_t* is_malignant; const int32_t* pair;
+} epi_cell_input;
+
+/* Full per-cell state, download and kernel-level upload.  *_next are the
+ * second buffers of the reference's dual_double / SwapData.  Lists are CSR
+ * (conn_ptr[n+1], conn, gj per entry).  Null pointers are skipped. */
+typedef struct epi_cells {
+    int32_t n, nmemb, nder, sw;
+    uint8_t* state; int32_t* fix_origin;
+    double *x, *y, *z, *xn, *yn, *zn;
+    double* radius;
+    double *ca2p, *ca2p_next, *ip3, *ip3_next;
+    double *ca2p_avg, *ex_inert, *agek, *ageb, *ex_fat, *in_fat, *spr_nat_len, *diff_u;
+    int32_t *rest_div_times, *pair, *dermis, *lat;
+    uint8_t* is_malignant;
+    int64_t* uid;
+    int32_t* conn_ptr; int32_t* conn; double* gj; int32_t conn_cap;
+    int32_t* memb_adj;
+} epi_cells;
+
+/* Step driver of proc.cpp:131-173 per tissue, plus a forced Ca cadence:
+ * ca_every > 0 also runs calc_ca2p when (step+1) % ca_every == 0. */
+typedef struct epi_driver {
+    int64_t step;
+    int32_t force_cornif;
+    int32_t num_sc;
+    int32_t ca_every;
+    int32_t clean_every;   /* accepted for parity with the oracle; stale gj
+                              entries are never kept here, so it is a no-op */
+} epi_driver;
+
+typedef struct epi_status {
+    int32_t n, err, sw, ndiv, nrm, maxlen;
+    int64_t rng_pos, next_uid;
+    double zzmax;
+    int64_t ca_solves;   /* calc_ca2p calls run by epi_step */
+} epi_status;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* new context with ntissues tissues on the current HIP device */
+int epi_create(int32_t ntissues, const epi_geometry* geometries, epi_ctx** out);
+void epi_destroy(epi_ctx* ctx);
+const char* epi_last_error(const epi_ctx* ctx);
+int32_t epi_num_tissues(const epi_ctx* ctx);
+
+/* ---- initialisation: cman_init (cell_init.cpp:132-142) ------------------ */
+/* CellManager::_load_from_file + _memb_init (cellmanager.cpp:198-323, 66-113) */
+int epi_load_cells(epi_ctx* ctx, int32_t tissue, const epi_cell_input* in);
+/* _cman_value_init (cell_init.cpp:18-40) */
+int epi_value_init(epi_ctx* ctx, int32_t tissue);
+/* load_uvp / load_w (cell_init.cpp:47-109); gj in CSR order of the current lists */
+int epi_set_cell_values(epi_ctx* ctx, int32_t tissue, const double* ca2p, const double* ip3,
+                        const double* ex_inert, const double* ca2p_avg, const double* gj);
+/* Field::read_binary into ATP/ext_stim buffers (proc.cpp:107-110) */
+int epi_set_fields(epi_ctx* ctx, int32_t tissue, const double* atp, const double* atp_next,
+                   const double* b, const double* b_next);
+/* replay stream of genrand_real (Random_gen.cpp:14-54), consumed in the
+ * reference's serial order; the library keeps its own device copy */
+int epi_set_rng(epi_ctx* ctx, int32_t tissue, const double* draws, int64_t n);
+/* std::mt19937_64(seed) through uniform_real_distribution<double>(0,1) */
+int epi_rng_mt19937_64(uint64_t seed, double* out, int64_t n);
+
+/* ---- the reference call surface (all tissues of the context) ----------- */
+int epi_cell_interaction(epi_ctx* ctx);            /* cell_interaction.h:5, cell_interaction.cpp:648 */
+int epi_pos_copy(epi_ctx* ctx);                    /* cellmanager.h:117, cellmanager.cpp:19 */
+int epi_cell_state_renew(epi_ctx* ctx);            /* cell_state_renew.h:3, cell_state_renew.cpp:400 */
+int epi_cell_pos_periodic_fix(epi_ctx* ctx);       /* cellmanager.h:111, cellmanager.cpp:395 */
+int epi_connect_cell(epi_ctx* ctx);                /* cell_connection.h:3, cell_connection.cpp:191 */
+int epi_calc_zzmax(epi_ctx* ctx, double* zzmax);   /* proc.cpp:34; zzmax[ntissues] */
+int epi_setup_map_lat(epi_ctx* ctx);               /* map.h:5-7, map.cpp:24 */
+int epi_calc_ext_stim(epi_ctx* ctx, const double* zzmax); /* ext_stim.h:5, ext_stim.cpp:18 */
+int epi_initialize_sc(epi_ctx* ctx, const double* zzmax); /* cell_state_renew.h:4, :435 */
+int epi_calc_ca2p(epi_ctx* ctx, const double* zzmax);     /* ca2p.h:7, ca2p.cpp:380 */
+int epi_should_calc_ca(epi_ctx* ctx, int32_t* flags);     /* cellmanager.h:97 */
+int epi_ca_calc_condition_reset(epi_ctx* ctx);            /* cellmanager.h:101 */
+/* one sub-step of calc_ca2p's loop (ca2p.cpp:392-399) with a given iz_bound,
+ * for kernel-level checks; maps from the last setup_map_lat */
+int epi_ca_substep(epi_ctx* ctx, int32_t iz_bound);
+
+/* ---- the driver ---------------------------------------------------------- */
+/* nsteps iterations of proc.cpp:153-172 for every tissue; drivers[ntissues]
+ * hold each tissue's loop state and are updated */
+int epi_step(epi_ctx* ctx, int64_t nsteps, epi_driver* drivers);
+int epi_synchronize(epi_ctx* ctx);
+
+/* ---- state exchange ------------------------------------------------------ */
+int epi_get_status(epi_ctx* ctx, int32_t tissue, epi_status* out);
+/* fills the arrays of *out (n entries; conn/gj up to conn_cap) */
+int epi_get_cells(epi_ctx* ctx, int32_t tissue, epi_cells* out);
+int epi_get_fields(epi_ctx* ctx, int32_t tissue, double* atp, double* atp_next, double* b,
+                   double* b_next, int32_t* cmap1, uint8_t* cmap2);
+/* kernel-level inputs: positions (both buffers), state/agek, lists, maps */
+int epi_set_positions(epi_ctx* ctx, int32_t tissue, const double* x, const double* y, const double* z);
+int epi_set_state(epi_ctx* ctx, int32_t tissue, const uint8_t* state, const double* agek);
+int epi_set_lists(epi_ctx* ctx, int32_t tissue, const int32_t* conn_ptr, const int32_t* conn,
+                  const double* gj);
+int epi_set_maps(epi_ctx* ctx, int32_t tissue, const int32_t* cmap1, const uint8_t* cmap2,
+                 const int32_t* lat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
