@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Throughput of the epi5_MT per-step hot path on one MI355X.
+
+Workload (default): the plain FP64 large single tissue -- 128x128 membrane
+sheet, 64 FIX, 20,000 free cells in 8 layers (top layer DEAD), FP64 grids
+128x128x256, division and removal active, seed 2 -- stepped through
+proc.cpp:153-172 with the reference's sw-gated Ca solve plus a forced solve
+every 500 steps.  A step is one iteration of that loop for the tissue.
+
+Prints one JSON line:
+  value           cell-steps/s (sum over timed steps of the live cell count
+                  divided by the wall time of the timed region)
+  grid_cell_steps_per_s  active grid points swept (one ext_stim sweep per
+                  step, 1000 ATP sweeps per Ca solve) per second
+  roofline        the dominant kernel's algorithmic bytes per launch over its
+                  mean duration (HIP events on the library's stream during
+                  the timed region) against 8 TB/s
+  cpu_baseline    the serial CPU restatement (oracle/) on a bounded sample
+                  of the same workload, extrapolated to the timed region
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM_GBS = 8000.0
+# algorithmic bytes per unit of work (grid point per sweep, cell per step)
+BYTES_PER_POINT = {"k_atp": 22.0, "k_ext_stim": 21.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", choices=["large", "small"], default="large")
+    ap.add_argument("--ca-every", type=int, default=500)
+    ap.add_argument("--roofline-kernel", default="k_atp")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="approximate CPU time budget of the cpu_baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-all", action="store_true",
+                    help="time every kernel with events and print a breakdown to stderr")
+    return ap.parse_args()
+
+
+def make_spec(name):
+    from epi_amd import configs
+    return configs.config_large(2) if name == "large" else configs.config_small(1)
+
+
+def n_draws_needed(spec, steps):
+    st = spec["cells"]["state"]
+    ready = int(np.count_nonzero((st == 4) | (st == 7)))
+    return int((steps + 64) * (ready + 32) + 100000)
+
+
+def cpu_baseline(spec, seed, gpu_cell_steps, gpu_ca_solves, timed_steps, budget_s):
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    t = O.OracleTissue(spec["geometry"])
+    t.load(spec)
+    t.set_rng(O.mt19937_64_uniform(seed, n_draws_needed(spec, 400)))
+    drv = O.Driver(0, 0, 0, 0, 0)
+    # mechanics + B sample
+    t0 = time.perf_counter()
+    t.step(drv)
+    t.step(drv)
+    per = (time.perf_counter() - t0) / 2
+    k = max(3, min(300, int(0.5 * budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    ran = 0
+    for _ in range(k):
+        ran += t.step(drv)
+    t_mech = time.perf_counter() - t0
+    zz = t.calc_zzmax()
+    # Ca sample: a fraction of one solve, scaled to 1000 sub-steps
+    t0 = time.perf_counter()
+    t.calc_ca2p_n(zz, 10)
+    per_sub = (time.perf_counter() - t0) / 10
+    n_sub = max(10, min(1000, int(0.5 * budget_s / max(per_sub, 1e-6))))
+    t0 = time.perf_counter()
+    t.calc_ca2p_n(zz, n_sub)
+    t_sub = (time.perf_counter() - t0) / n_sub
+    t_step = t_mech / k
+    est = timed_steps * t_step + gpu_ca_solves * 1000 * t_sub
+    return {
+        "value": gpu_cell_steps / est,
+        "unit": "cell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"serial CPU restatement on the same tissue: {k} steps at {t_step*1e3:.2f} ms/step "
+                   f"(Ca solves in sample: {ran}) and {n_sub} Ca sub-steps at {t_sub*1e3:.3f} ms; "
+                   f"extrapolated to {timed_steps} steps + {gpu_ca_solves} Ca solves"),
+        "seconds_per_step": t_step,
+        "seconds_per_ca_substep": t_sub,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(tissues shard across GPUs with no per-step exchange; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+    from epi_amd import native
+    spec = make_spec(args.config)
+    seed = spec["seed"]
+    geo = spec["geometry"]
+    ts = native.TissueSet([geo])
+    ts.load(0, spec)
+    ts.set_rng(0, native.mt19937_64_uniform(seed, n_draws_needed(spec, args.steps + args.warmup)))
+    drv = [native.Driver(0, 0, 0, args.ca_every, 0)]
+    if args.warmup:
+        ts.step(drv, args.warmup)
+    ts.synchronize()
+    s0, _ = ts.status(0)
+    ts.profile_reset()
+    ts.profile_enable(None if args.profile_all else [args.roofline_kernel])
+    t0 = time.perf_counter()
+    ts.step(drv, args.steps)
+    ts.synchronize()
+    t1 = time.perf_counter()
+    s1, _ = ts.status(0)
+    prof = ts.profile()
+    ts.profile_enable([])
+    wall = t1 - t0
+    cell_steps = s1.cell_steps - s0.cell_steps
+    grid_pts = (s1.grid_points_b - s0.grid_points_b) + (s1.grid_points_ca - s0.grid_points_ca)
+    ca_solves = s1.ca_solves - s0.ca_solves
+    value = cell_steps / wall
+    # roofline of the dominant kernel
+    rk = args.roofline_kernel
+    roof = None
+    if rk in prof:
+        ms, cnt = prof[rk]
+        avg_s = ms / cnt / 1e3
+        if rk == "k_atp":
+            units = (s1.grid_points_ca - s0.grid_points_ca) / cnt
+        else:
+            units = (s1.grid_points_b - s0.grid_points_b) / cnt
+        alg = BYTES_PER_POINT[rk] * units
+        achieved = alg / avg_s / 1e9
+        traffic = None
+        pmc = os.path.join(HERE, "profiles", f"{rk}_traffic.json")
+        if os.path.exists(pmc):
+            with open(pmc) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": rk,
+                "avg_us": round(avg_s * 1e6, 3), "launches": cnt,
+                "algorithmic_bytes_per_launch": round(alg, 1)}
+    if args.profile_all:
+        tot = sum(v[0] for v in prof.values())
+        for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+            print(f"  {k:18s} {ms:10.2f} ms {cnt:8d} launches {1e3*ms/cnt:9.2f} us/launch "
+                  f"{100*ms/max(tot,1e-9):5.1f}%", file=sys.stderr)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(spec, seed, cell_steps, ca_solves, args.steps, args.cpu_seconds)
+    out = {
+        "metric": "cell-steps/s + grid-cell-steps/s (FP64, 1e-10 vs ref); batched & adjoint vs plain",
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 {args.config} tissue ({geo.nmx}x{geo.nmy} sheet, "
+                               f"{geo.nx}x{geo.ny}x{geo.nz} grid, seed {seed}), Ca every {args.ca_every} + sw gating",
+                   "cells_start": int(spec["cells"]["x"].size), "cells_end": int(s1.n),
+                   "ca_solves": int(ca_solves)},
+        "grid_cell_steps_per_s": round(grid_pts / wall, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+    ts.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -250,14 +250,17 @@ def config_large(seed=2):
                        dead_top=True, n_hot_alive=60)
 
 
-def config_tiny(seed=7, n_free=160, side=16, grid=16, n_layers=2, dead_top=False):
-    """Small tissue for fast parity tests (same generator)."""
+def config_tiny(seed=7, n_free=160, side=16, grid=16, n_layers=2, dead_top=False, nz=None):
+    """Small tissue for fast parity tests (same generator); nz > grid makes
+    the box taller (dz = dx)."""
     lx = 0.8 * side
     if not check_bucket_span(lx):
         lx = 2.0 * math.ceil(0.4 * side)
+    nz = nz or grid
     return make_tissue(seed, nmx=side, nmy=side, lx=lx, ly=lx, nx=grid,
-                       ny=grid, nz=grid, n_fix=4, n_free=n_free,
-                       n_layers=n_layers, dead_top=dead_top, n_hot_alive=4)
+                       ny=grid, nz=nz, n_fix=4, n_free=n_free,
+                       n_layers=n_layers, dead_top=dead_top, n_hot_alive=4,
+                       lz=nz * (lx / grid))
 
 
 def batched_configs(n_tissues=256, base_seed=3000, scale=1.0):

@@ -23,7 +23,62 @@
 
 using namespace epi;
 
+// ---- per-kernel event timer ----------------------------------------------
+namespace epi {
 namespace {
+struct ProfRec { int k; hipEvent_t a, b; };
+struct Prof {
+  unsigned long long mask = 0;
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[K_COUNT] = {};
+  long long cnt[K_COUNT] = {};
+};
+Prof g_prof;
+hipEvent_t take_event() {
+  if (!g_prof.pool.empty()) {
+    hipEvent_t e = g_prof.pool.back();
+    g_prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+}  // namespace
+bool prof_on(int k) { return (g_prof.mask >> k) & 1ull; }
+hipEvent_t prof_start(hipStream_t s) {
+  hipEvent_t a = take_event();
+  (void)hipEventRecord(a, s);
+  return a;
+}
+void prof_stop(hipStream_t s, int k, hipEvent_t a) {
+  hipEvent_t b = take_event();
+  (void)hipEventRecord(b, s);
+  g_prof.pending.push_back({k, a, b});
+}
+}  // namespace epi
+
+namespace {
+
+const char* kKernelNames[K_COUNT] = {
+    "k_forces_memb", "k_forces_other", "k_renew_scan", "k_renew_apply", "k_remove", "k_pair_disperse",
+    "k_initialize_sc", "k_periodic_fix", "k_bucket_clear", "k_bucket_insert", "k_bucket_sort", "k_sweep",
+    "k_finish", "k_snapshot", "k_zz_set", "k_zzmax", "k_map_clear", "k_map_stamp", "k_map_halo",
+    "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset"};
+
+void prof_resolve() {
+  for (auto& r : g_prof.pending) {
+    float ms = 0;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      g_prof.ms[r.k] += ms;
+      g_prof.cnt[r.k]++;
+    }
+    g_prof.pool.push_back(r.a);
+    g_prof.pool.push_back(r.b);
+  }
+  g_prof.pending.clear();
+}
 
 struct TH {
   epi_geometry g{};
@@ -55,6 +110,8 @@ struct epi_ctx {
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
   std::vector<long long> ca_solves;
+  // throughput accounting of epi_step
+  std::vector<long long> steps, cell_steps, grid_b, grid_ca;
   std::string err;
 };
 
@@ -145,6 +202,7 @@ int check_tissue(epi_ctx* c, int t) {
 int sync_status(epi_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(Status) * c->T, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!g_prof.pending.empty()) prof_resolve();
   for (int t = 0; t < c->T; t++) c->th[t].n_est = c->h_st[t].n;
   for (int t = 0; t < c->T; t++)
     if (c->h_st[t].err) {
@@ -313,6 +371,10 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
   c->th.resize(ntissues);
   c->tis.resize(ntissues);
   c->ca_solves.assign(ntissues, 0);
+  c->steps.assign(ntissues, 0);
+  c->cell_steps.assign(ntissues, 0);
+  c->grid_b.assign(ntissues, 0);
+  c->grid_ca.assign(ntissues, 0);
   for (int t = 0; t < ntissues; t++) {
     const epi_geometry& g = geos[t];
     TH& h = c->th[t];
@@ -736,6 +798,10 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
       unsigned long long b = c->h_st[t].zzmax_bits;
       memcpy(&v, &b, 8);
       c->h_st[t].zzmax = v;
+      const Tis& ti = c->tis[t];
+      c->steps[t]++;
+      c->cell_steps[t] += c->h_st[t].n;
+      c->grid_b[t] += (long long)ti.NX * ti.NY * (int)((v + FAC_MAP * R_max) * ti.inv_dz);
       epi_driver& d = drv[t];
       // proc.cpp:158-163
       if ((double)d.step * DT_Cell > 6000.0 && d.force_cornif) {
@@ -755,6 +821,8 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
         ca_list.push_back(t);
         if (d.num_sc > 0) d.num_sc--;
         c->ca_solves[t]++;
+        const Tis& ti = c->tis[t];
+        c->grid_ca[t] += (long long)CA_ITR * ti.NX * ti.NY * (int)((c->h_st[t].zzmax + FAC_MAP * R_max) / ti.dz);
       }
       d.step++;
     }
@@ -784,8 +852,40 @@ int epi_get_status(epi_ctx* c, int32_t t, epi_status* out) {
   unsigned long long b = s.zzmax_bits;
   memcpy(&out->zzmax, &b, 8);
   out->ca_solves = c->ca_solves[t];
+  out->steps = c->steps[t];
+  out->cell_steps = c->cell_steps[t];
+  out->grid_points_b = c->grid_b[t];
+  out->grid_points_ca = c->grid_ca[t];
   return rc;
 }
+
+int epi_profile_enable(epi_ctx* c, uint64_t mask) {
+  if (!c) return EPI_ERR_INVALID;
+  RCHK(sync_status(c));
+  g_prof.mask = mask;
+  return EPI_OK;
+}
+
+int epi_profile_read(epi_ctx* c, int32_t kernel, const char** name, double* total_ms, int64_t* count) {
+  if (!c) return EPI_ERR_INVALID;
+  if (kernel < 0 || kernel >= K_COUNT) return fail(c, EPI_ERR_INVALID, "kernel id");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  prof_resolve();
+  if (name) *name = kKernelNames[kernel];
+  if (total_ms) *total_ms = g_prof.ms[kernel];
+  if (count) *count = g_prof.cnt[kernel];
+  return EPI_OK;
+}
+
+int epi_profile_reset(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  prof_resolve();
+  for (int k = 0; k < K_COUNT; k++) { g_prof.ms[k] = 0; g_prof.cnt[k] = 0; }
+  return EPI_OK;
+}
+
+int32_t epi_profile_kernel_count(void) { return K_COUNT; }
 
 int epi_get_cells(epi_ctx* c, int32_t t, epi_cells* o) {
   RCHK(check_tissue(c, t));

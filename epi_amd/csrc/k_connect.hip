@@ -124,6 +124,7 @@ __global__ void __launch_bounds__(256) k_finish(const Tis* __restrict__ T, int p
   Status* st = t.st;
   const int n = st->n;
   const int ob = 1 - nb;
+  int mylen = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     const bool memb = c < t.nmemb;
     const int base = memb ? MEMB_ADJ : 0;
@@ -136,6 +137,7 @@ __global__ void __launch_bounds__(256) k_finish(const Tis* __restrict__ T, int p
       t.nlen[nb][c] = 0;
       continue;
     }
+    mylen = max(mylen, len);
     int* const row = t.col[nb] + (size_t)c * t.ns;
     if (memb)
       for (int i = 0; i < MEMB_ADJ; i++) row[i] = t.memb6[6 * c + i];
@@ -157,18 +159,23 @@ __global__ void __launch_bounds__(256) k_finish(const Tis* __restrict__ T, int p
       ls[j + 1] = v;
     }
     t.nlen[nb][c] = len;
-    atomicMax(&st->maxlen, len);
-    // gj_refresh: a directed edge keeps its value while it stays in the list
+    // gj_refresh: a directed edge keeps its value while it stays in the list.
+    // Rows change little between builds, so the search starts after the
+    // previous match and wraps around once.
     double* const g = t.gj[nb] + (size_t)c * t.ns;
     const int src = t.src_row[c];
     const int* orow = src >= 0 ? t.col[ob] + (size_t)src * t.ns : nullptr;
     const double* og = src >= 0 ? t.gj[ob] + (size_t)src * t.ns : nullptr;
     const int olen = src >= 0 ? t.nlen[ob][src] : 0;
+    int hint = 0;
     for (int e = 0; e < len; e++) {
       const long long u = t.uid[row[e]];
       double v = gj_init;
-      for (int f = 0; f < olen; f++)
-        if (t.old_uid[orow[f]] == u) { v = og[f]; break; }
+      for (int f = 0; f < olen; f++) {
+        int q = hint + f;
+        if (q >= olen) q -= olen;
+        if (t.old_uid[orow[q]] == u) { v = og[q]; hint = q + 1; break; }
+      }
       g[e] = v;
     }
     // set_dermis / find_dermis (:158-185)
@@ -189,6 +196,8 @@ __global__ void __launch_bounds__(256) k_finish(const Tis* __restrict__ T, int p
       }
     }
   }
+  for (int o = 32; o > 0; o >>= 1) mylen = max(mylen, __shfl_down(mylen, o, 64));
+  if ((threadIdx.x & 63) == 0 && mylen > 0) atomicMax(&st->maxlen, mylen);
 }
 
 // after a build: every cell's row is its own, uids snapshot the numbering
@@ -230,22 +239,22 @@ __global__ void k_zz_set(const Tis* __restrict__ T, int use_value, double v) {
 
 void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets) {
   dim3 gb((max_buckets + 255) / 256, ntis), gc((max_n + 255) / 256, ntis);
-  hipLaunchKernelGGL(k_bucket_clear, gb, dim3(256), 0, s, dT);
-  hipLaunchKernelGGL(k_bucket_insert, gc, dim3(256), 0, s, dT, pb);
-  hipLaunchKernelGGL(k_bucket_sort, gb, dim3(256), 0, s, dT);
-  hipLaunchKernelGGL(k_sweep, gc, dim3(256), 0, s, dT, pb, nb);
-  hipLaunchKernelGGL(k_finish, gc, dim3(256), 0, s, dT, pb, nb);
-  hipLaunchKernelGGL(k_snapshot, gc, dim3(256), 0, s, dT);
+  EPI_LAUNCH(K_BUCKET_CLEAR, k_bucket_clear, gb, dim3(256), 0, s, dT);
+  EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb);
+  EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gb, dim3(256), 0, s, dT);
+  EPI_LAUNCH(K_SWEEP, k_sweep, gc, dim3(256), 0, s, dT, pb, nb);
+  EPI_LAUNCH(K_FINISH, k_finish, gc, dim3(256), 0, s, dT, pb, nb);
+  EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT);
 }
 
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {
-  hipLaunchKernelGGL(k_zz_set, dim3(ntis), dim3(64), 0, s, dT, 0, 0.0);
+  EPI_LAUNCH(K_ZZ_SET, k_zz_set, dim3(ntis), dim3(64), 0, s, dT, 0, 0.0);
   dim3 gc((max_n + 255) / 256, ntis);
-  hipLaunchKernelGGL(k_zzmax, gc, dim3(256), 0, s, dT, pb);
+  EPI_LAUNCH(K_ZZMAX, k_zzmax, gc, dim3(256), 0, s, dT, pb);
 }
 
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v) {
-  hipLaunchKernelGGL(k_zz_set, dim3(ntis), dim3(64), 0, s, dT, 1, v);
+  EPI_LAUNCH(K_ZZ_SET, k_zz_set, dim3(ntis), dim3(64), 0, s, dT, 1, v);
 }
 
 }  // namespace epi

@@ -377,25 +377,25 @@ __global__ void k_sw_reset(const Tis* __restrict__ T, const int* tlist) {
 
 void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo) {
   dim3 gg((unsigned)((max_gsize + 255) / 256 < 65535 ? (max_gsize + 255) / 256 : 65535), ntis);
-  hipLaunchKernelGGL(k_map_clear, gg, dim3(256), 0, s, dT);
-  hipLaunchKernelGGL(k_map_stamp, dim3((max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
-  hipLaunchKernelGGL(k_map_halo, dim3((max_halo + 255) / 256, ntis), dim3(256), 0, s, dT);
+  EPI_LAUNCH(K_MAP_CLEAR, k_map_clear, gg, dim3(256), 0, s, dT);
+  EPI_LAUNCH(K_MAP_STAMP, k_map_stamp, dim3((max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
+  EPI_LAUNCH(K_MAP_HALO, k_map_halo, dim3((max_halo + 255) / 256, ntis), dim3(256), 0, s, dT);
 }
 
 void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est) {
   const int nky = (max_ny + 3) / 4;
   const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
   dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
-  hipLaunchKernelGGL(k_ext_stim, g, dim3(256), 0, s, dT, bp, nky, nzc);
+  EPI_LAUNCH(K_EXT_STIM, k_ext_stim, g, dim3(256), 0, s, dT, bp, nky, nzc);
 }
 
 static void ca_maps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box,
                     int zero_avg) {
-  hipLaunchKernelGGL(k_ca_begin, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, zero_avg);
+  EPI_LAUNCH(K_CA_BEGIN, k_ca_begin, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, zero_avg);
   unsigned gb = (unsigned)((max_box + 255) / 256);
   if (gb < 1) gb = 1;
   if (gb > 65535) gb = 65535;
-  hipLaunchKernelGGL(k_ca_vox, dim3(gb, ntis), dim3(256), 0, s, dT, tlist);
+  EPI_LAUNCH(K_CA_VOX, k_ca_vox, dim3(gb, ntis), dim3(256), 0, s, dT, tlist);
 }
 
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box) {
@@ -407,21 +407,21 @@ void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, i
 }
 
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n) {
-  hipLaunchKernelGGL(k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 0);
+  EPI_LAUNCH(K_CA_END, k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 0);
 }
 
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
                        int max_n, int max_nx, int max_ny, int izb_est) {
-  hipLaunchKernelGGL(k_ca_cells, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, cp, ap, bp);
+  EPI_LAUNCH(K_CA_CELLS, k_ca_cells, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, cp, ap, bp);
   const int nky = (max_ny + 1 + 3) / 4;
   const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
   dim3 g((max_nx + 1 + 63) / 64, ntis, nky * nzc);
-  hipLaunchKernelGGL(k_atp, g, dim3(256), 0, s, dT, tlist, ap, nky, nzc);
+  EPI_LAUNCH(K_ATP, k_atp, g, dim3(256), 0, s, dT, tlist, ap, nky, nzc);
 }
 
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n) {
-  hipLaunchKernelGGL(k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 1);
-  hipLaunchKernelGGL(k_sw_reset, dim3(ntis), dim3(64), 0, s, dT, tlist);
+  EPI_LAUNCH(K_CA_END, k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 1);
+  EPI_LAUNCH(K_SW_RESET, k_sw_reset, dim3(ntis), dim3(64), 0, s, dT, tlist);
 }
 
 }  // namespace epi

@@ -357,11 +357,11 @@ __global__ void __launch_bounds__(256) k_forces_other(const Tis* __restrict__ T,
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other) {
   if (max_memb > 0) {
     dim3 g((max_memb + 255) / 256, ntis);
-    hipLaunchKernelGGL(k_forces_memb, g, dim3(256), 0, s, dT, pb, cb);
+    EPI_LAUNCH(K_FORCES_MEMB, k_forces_memb, g, dim3(256), 0, s, dT, pb, cb);
   }
   if (max_other > 0) {
     dim3 g((max_other + 255) / 256, ntis);
-    hipLaunchKernelGGL(k_forces_other, g, dim3(256), 0, s, dT, pb, cb);
+    EPI_LAUNCH(K_FORCES_OTHER, k_forces_other, g, dim3(256), 0, s, dT, pb, cb);
   }
 }
 

@@ -388,21 +388,21 @@ __global__ void __launch_bounds__(256) k_periodic_fix(const Tis* __restrict__ T,
 }
 
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n) {
-  hipLaunchKernelGGL(k_renew_scan, dim3(1, ntis), dim3(1024), 0, s, dT, pb, cb);
+  EPI_LAUNCH(K_RENEW_SCAN, k_renew_scan, dim3(1, ntis), dim3(1024), 0, s, dT, pb, cb);
   dim3 g((max_n + 255) / 256, ntis);
-  hipLaunchKernelGGL(k_renew_apply, g, dim3(256), 0, s, dT, pb, cb, cpar);
-  hipLaunchKernelGGL(k_remove, dim3(1, ntis), dim3(256), 0, s, dT, pb);
-  hipLaunchKernelGGL(k_pair_disperse, g, dim3(256), 0, s, dT, pb);
+  EPI_LAUNCH(K_RENEW_APPLY, k_renew_apply, g, dim3(256), 0, s, dT, pb, cb, cpar);
+  EPI_LAUNCH(K_REMOVE, k_remove, dim3(1, ntis), dim3(256), 0, s, dT, pb);
+  EPI_LAUNCH(K_PAIR_DISPERSE, k_pair_disperse, g, dim3(256), 0, s, dT, pb);
 }
 
 void launch_initialize_sc(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int pb, int max_n) {
   dim3 g((max_n + 255) / 256, ntis);
-  hipLaunchKernelGGL(k_initialize_sc, g, dim3(256), 0, s, dT, tlist, pb);
+  EPI_LAUNCH(K_INIT_SC, k_initialize_sc, g, dim3(256), 0, s, dT, tlist, pb);
 }
 
 void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {
   dim3 g((max_n + 255) / 256, ntis);
-  hipLaunchKernelGGL(k_periodic_fix, g, dim3(256), 0, s, dT, pb);
+  EPI_LAUNCH(K_PERIODIC_FIX, k_periodic_fix, g, dim3(256), 0, s, dT, pb);
 }
 
 }  // namespace epi

@@ -1,9 +1,32 @@
-// launch.h -- host-side launchers of the kernel families (one per .hip file).
+// launch.h -- host-side launchers of the kernel families (one per .hip file)
+// and the optional per-kernel event timer.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace epi {
 struct Tis;
+
+enum KernelId {
+  K_FORCES_MEMB, K_FORCES_OTHER, K_RENEW_SCAN, K_RENEW_APPLY, K_REMOVE, K_PAIR_DISPERSE,
+  K_INIT_SC, K_PERIODIC_FIX, K_BUCKET_CLEAR, K_BUCKET_INSERT, K_BUCKET_SORT, K_SWEEP,
+  K_FINISH, K_SNAPSHOT, K_ZZ_SET, K_ZZMAX, K_MAP_CLEAR, K_MAP_STAMP, K_MAP_HALO,
+  K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET, K_COUNT
+};
+
+// Event timer: when the bit of a kernel id is set, its launches are
+// bracketed by HIP events on the launch stream (see host.hip).
+bool prof_on(int k);
+hipEvent_t prof_start(hipStream_t s);
+void prof_stop(hipStream_t s, int k, hipEvent_t a);
+
+#define EPI_LAUNCH(kid, kernel, grid, block, shmem, stream, ...)                \
+  do {                                                                          \
+    hipEvent_t a_ = nullptr;                                                    \
+    const bool on_ = ::epi::prof_on(kid);                                       \
+    if (on_) a_ = ::epi::prof_start(stream);                                    \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);        \
+    if (on_) ::epi::prof_stop(stream, kid, a_);                                 \
+  } while (0)
 
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other);
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n);

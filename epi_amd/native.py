@@ -69,7 +69,9 @@ class Driver(C.Structure):
 class Status(C.Structure):
     _fields_ = [("n", C.c_int32), ("err", C.c_int32), ("sw", C.c_int32), ("ndiv", C.c_int32),
                 ("nrm", C.c_int32), ("maxlen", C.c_int32), ("rng_pos", C.c_int64),
-                ("next_uid", C.c_int64), ("zzmax", C.c_double), ("ca_solves", C.c_int64)]
+                ("next_uid", C.c_int64), ("zzmax", C.c_double), ("ca_solves", C.c_int64),
+                ("steps", C.c_int64), ("cell_steps", C.c_int64), ("grid_points_b", C.c_int64),
+                ("grid_points_ca", C.c_int64)]
 
 
 CELL_DTYPES = {
@@ -91,7 +93,8 @@ SYMBOLS = [
     "epi_calc_ext_stim", "epi_initialize_sc", "epi_calc_ca2p", "epi_should_calc_ca",
     "epi_ca_calc_condition_reset", "epi_ca_substep", "epi_step", "epi_synchronize",
     "epi_get_status", "epi_get_cells", "epi_get_fields", "epi_set_positions",
-    "epi_set_state", "epi_set_lists", "epi_set_maps",
+    "epi_set_state", "epi_set_lists", "epi_set_maps", "epi_profile_enable",
+    "epi_profile_read", "epi_profile_reset", "epi_profile_kernel_count",
 ]
 
 _lib = None
@@ -135,6 +138,10 @@ def load_library():
     L.epi_set_state.argtypes = [vp, C.c_int32, _u8p, _dp]
     L.epi_set_lists.argtypes = [vp, C.c_int32, _ip, _ip, _dp]
     L.epi_set_maps.argtypes = [vp, C.c_int32, _ip, _u8p, _ip]
+    L.epi_profile_enable.argtypes = [vp, C.c_uint64]
+    L.epi_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), _dp, _i64p]
+    L.epi_profile_reset.argtypes = [vp]
+    L.epi_profile_kernel_count.argtypes = []
     _lib = L
     return L
 
@@ -273,6 +280,34 @@ class TissueSet:
             drivers[i].step, drivers[i].force_cornif = arr[i].step, arr[i].force_cornif
             drivers[i].num_sc = arr[i].num_sc
         self._chk(rc, "step")
+
+    # --- kernel timing ---
+    def profile_enable(self, names=None):
+        """Time the named kernels (all when names is None) with HIP events."""
+        n = self.L.epi_profile_kernel_count()
+        mask = 0
+        for k in range(n):
+            if names is None or self.kernel_name(k) in names:
+                mask |= 1 << k
+        self._chk(self.L.epi_profile_enable(self.h, mask), "profile_enable")
+
+    def kernel_name(self, k):
+        nm = C.c_char_p()
+        self._chk(self.L.epi_profile_read(self.h, k, C.byref(nm), None, None), "profile_read")
+        return nm.value.decode()
+
+    def profile(self):
+        """{kernel: (total_ms, launches)} for kernels with launches."""
+        out = {}
+        for k in range(self.L.epi_profile_kernel_count()):
+            nm, ms, cnt = C.c_char_p(), C.c_double(), C.c_int64()
+            self._chk(self.L.epi_profile_read(self.h, k, C.byref(nm), C.byref(ms), C.byref(cnt)), "profile_read")
+            if cnt.value:
+                out[nm.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def profile_reset(self):
+        self._chk(self.L.epi_profile_reset(self.h), "profile_reset")
 
     # --- state exchange ---
     def status(self, t):

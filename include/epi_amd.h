@@ -94,7 +94,11 @@ typedef struct epi_status {
     int32_t n, err, sw, ndiv, nrm, maxlen;
     int64_t rng_pos, next_uid;
     double zzmax;
-    int64_t ca_solves;   /* calc_ca2p calls run by epi_step */
+    int64_t ca_solves;      /* calc_ca2p calls run by epi_step */
+    int64_t steps;          /* steps run by epi_step */
+    int64_t cell_steps;     /* sum over those steps of the live cell count */
+    int64_t grid_points_b;  /* sum of nx*ny*iz_bound over ext_stim sweeps */
+    int64_t grid_points_ca; /* sum of nx*ny*iz_bound over ATP sub-steps */
 } epi_status;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -157,6 +161,15 @@ int epi_set_lists(epi_ctx* ctx, int32_t tissue, const int32_t* conn_ptr, const i
                   const double* gj);
 int epi_set_maps(epi_ctx* ctx, int32_t tissue, const int32_t* cmap1, const uint8_t* cmap2,
                  const int32_t* lat);
+
+/* ---- kernel timing -------------------------------------------------------
+ * No counterpart in the reference (its only timer is proc.cpp:128-137).
+ * Bit k of mask brackets every launch of kernel k with HIP events on the
+ * context's stream; totals accumulate until epi_profile_reset. */
+int epi_profile_enable(epi_ctx* ctx, uint64_t mask);
+int epi_profile_read(epi_ctx* ctx, int32_t kernel, const char** name, double* total_ms, int64_t* count);
+int epi_profile_reset(epi_ctx* ctx);
+int32_t epi_profile_kernel_count(void);
 
 #ifdef __cplusplus
 }

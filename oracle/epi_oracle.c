@@ -1329,7 +1329,16 @@ void eo_ca_substep(eo_tissue* t, int iz_bound) {
     free(m.diffu_cell); free(m.air);
 }
 
+static void calc_ca2p_n(eo_tissue* t, double zzmax, unsigned n_sub);
+
 void eo_calc_ca2p(eo_tissue* t, double zzmax) {
+    calc_ca2p_n(t, zzmax, (int)(Ca_avg_time / DT_Ca));
+}
+
+/* calc_ca2p with a given number of sub-steps (timing samples only) */
+void eo_calc_ca2p_n(eo_tissue* t, double zzmax, int n_sub) { calc_ca2p_n(t, zzmax, (unsigned)n_sub); }
+
+static void calc_ca2p_n(eo_tissue* t, double zzmax, unsigned n_sub) {
     (void)g_maps_cache;
     const unsigned Ca_ITR = (int)(Ca_avg_time / DT_Ca);
     /* init_ca2p_avg :20-27 */
@@ -1340,7 +1349,7 @@ void eo_calc_ca2p(eo_tissue* t, double zzmax) {
     m.diffu_cell = (int32_t*)malloc(sizeof(int32_t) * t->gsz);
     m.air = (uint8_t*)malloc(t->gsz);
     init_ca2p_map(t, &m);
-    for (unsigned cstp = 0; cstp < Ca_ITR; cstp++) {
+    for (unsigned cstp = 0; cstp < n_sub; cstp++) {
         dead_IP3_calc(t);
         supra_calc(t);
         ATP_refresh(t, &m);

@@ -84,6 +84,8 @@ void eo_clean_up(eo_tissue* t);                /* cellmanager.cpp:543 */
 
 /* Sub-steps of calc_ca2p exposed for kernel-level tests. */
 void eo_ca_substep(eo_tissue* t, int iz_bound); /* ca2p.cpp:392-399 body */
+/* calc_ca2p with n_sub sub-steps instead of Ca_ITR (timing samples) */
+void eo_calc_ca2p_n(eo_tissue* t, double zzmax, int n_sub);
 
 /* One iteration of the proc loop body (proc.cpp:153-172), plus an optional
  * forced Ca cadence: ca_every > 0 also fires calc_ca2p when

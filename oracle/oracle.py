@@ -94,6 +94,7 @@ def lib():
         for f in ("eo_calc_ext_stim", "eo_initialize_sc", "eo_calc_ca2p"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_double]
         L.eo_ca_substep.argtypes = [C.c_void_p, C.c_int]
+        L.eo_calc_ca2p_n.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.eo_should_calc_ca.argtypes = [C.c_void_p]
         L.eo_step.argtypes = [C.c_void_p, C.POINTER(Driver)]
         L.eo_get_cells.argtypes = [C.c_void_p, C.POINTER(Cells)]
@@ -213,6 +214,7 @@ class OracleTissue:
     def initialize_sc(self, zzmax): self.L.eo_initialize_sc(self.h, zzmax)
     def calc_ca2p(self, zzmax): self.L.eo_calc_ca2p(self.h, zzmax)
     def ca_substep(self, iz_bound): self.L.eo_ca_substep(self.h, iz_bound)
+    def calc_ca2p_n(self, zzmax, n_sub): self.L.eo_calc_ca2p_n(self.h, zzmax, n_sub)
     def should_calc_ca(self): return bool(self.L.eo_should_calc_ca(self.h))
     def ca_calc_condition_reset(self): self.L.eo_ca_calc_condition_reset(self.h)
     def clean_up(self): self.L.eo_clean_up(self.h)

@@ -1,0 +1,147 @@
+"""HIP path against the oracle on the same seeded inputs (MI355X only).
+
+Bar: integer, index and event state bit-exact; positions bit-exact while no
+transcendental enters them; every FP64 field within 1e-10 relative."""
+import numpy as np
+import pytest
+
+from epi_amd import configs
+from parity import compare_cells, compare_fields, worst
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-10
+EXACT_FP = ("x", "y", "z", "xn", "yn", "zn", "radius", "spr_nat_len")
+
+
+def pair(O, spec, draws=400000):
+    from epi_amd import native
+    g = spec["geometry"]
+    ot = O.OracleTissue(g)
+    ot.load(spec)
+    gt = native.Tissue(g)
+    gt.load(spec)
+    rng = O.mt19937_64_uniform(spec["seed"] or 1, draws)
+    ot.set_rng(rng)
+    gt.set_rng(rng)
+    return ot, gt
+
+
+def check(gt, ot, exact_positions=True, tag=""):
+    assert ot.error() == 0, (tag, "oracle error", ot.error())
+    assert gt.error() == 0, (tag, "device error", gt.error())
+    d = compare_cells(gt.get_cells(), ot.get_cells())
+    assert "n" not in d, (tag, d)
+    for k, v in d.items():
+        if isinstance(v, int):
+            assert v == 0, (tag, k, v)
+        elif k in EXACT_FP and exact_positions:
+            assert v == 0.0, (tag, k, v)
+        else:
+            assert v <= FP_TOL, (tag, k, v)
+    f = compare_fields(gt.get_fields(), ot.get_fields())
+    for k, v in f.items():
+        assert (v == 0) if k.startswith("cmap") else (v <= FP_TOL), (tag, k, v)
+    assert gt.rng_cursor() == ot.rng_cursor(), tag
+    return d
+
+
+def test_stage_by_stage(oracle_mod):
+    spec = configs.config_tiny(7, n_free=200, nz=24)
+    ot, gt = pair(oracle_mod, spec)
+    check(gt, ot, tag="load")
+    for s in range(4):
+        for name in ("cell_interaction", "pos_copy", "cell_state_renew", "cell_pos_periodic_fix",
+                     "connect_cell"):
+            getattr(ot, name)()
+            getattr(gt, name)()
+            check(gt, ot, tag=f"{s}:{name}")
+        zo, zg = ot.calc_zzmax(), gt.calc_zzmax()
+        assert zo == zg
+        ot.setup_map_lat()
+        gt.setup_map_lat()
+        check(gt, ot, tag=f"{s}:maps")
+        ot.calc_ext_stim(zo)
+        gt.calc_ext_stim(zg)
+        check(gt, ot, tag=f"{s}:ext_stim")
+    ot.ca_substep(10)
+    gt.ca_substep(10)
+    check(gt, ot, tag="ca_substep")
+    ot.calc_ca2p(zo)
+    gt.calc_ca2p(zg)
+    check(gt, ot, tag="calc_ca2p")
+
+
+@pytest.mark.parametrize("seed", [3, 8])
+def test_mechanics_only_is_bit_exact(oracle_mod, seed):
+    spec = configs.config_tiny(seed, n_free=250, nz=24)
+    spec["cells"]["agek"] = np.minimum(spec["cells"]["agek"], 10.0)   # no cornification, no Ca
+    ot, gt = pair(oracle_mod, spec)
+    from epi_amd import native
+    do, dg = oracle_mod.Driver(0, 0, 0, 0, 0), native.Driver(0, 0, 0, 0, 0)
+    for k in range(3):
+        gt.step(dg, 40)
+        for _ in range(40):
+            ot.step(do)
+        d = check(gt, ot, tag=f"chunk {k}")
+        assert d["x"] == 0.0 and d["gj"] == 0.0 and d["agek"] == 0.0
+
+
+def test_driver_small_config(oracle_mod):
+    """Plain FP64 small config (64x64 sheet, 2,000 free cells, 64^3 grid,
+    seed 1) for 300 steps with divisions and a Ca solve every 100 steps."""
+    spec = configs.config_small(1)
+    ot, gt = pair(oracle_mod, spec, draws=1_000_000)
+    from epi_amd import native
+    do, dg = oracle_mod.Driver(0, 0, 0, 100, 0), native.Driver(0, 0, 0, 100, 0)
+    for k in range(3):
+        gt.step(dg, 100)
+        for _ in range(100):
+            ot.step(do)
+        check(gt, ot, tag=f"chunk {k}")
+    assert gt.status().ca_solves >= 3
+    assert gt.get_cells()["n"] > spec["cells"]["x"].size       # divisions happened
+
+
+def test_driver_with_removals(oracle_mod):
+    spec = configs.config_tiny(21, n_free=300, side=16, grid=16, n_layers=3, dead_top=True, nz=40)
+    c = spec["cells"]
+    dead = np.nonzero(c["state"] == 1)[0]
+    c["agek"][dead] = np.linspace(31.28, 31.32, dead.size)
+    ot, gt = pair(oracle_mod, spec)
+    from epi_amd import native
+    do, dg = oracle_mod.Driver(0, 0, 0, 0, 0), native.Driver(0, 0, 0, 0, 0)
+    gt.step(dg, 150)
+    for _ in range(150):
+        ot.step(do)
+    assert ot.error() == 0 and gt.error() == 0
+    check(gt, ot, exact_positions=False, tag="removals")
+    assert gt.get_cells()["n"] < c["x"].size
+
+
+def test_forced_cornification(oracle_mod):
+    """proc.cpp:158-163: past T_TURNOVER with force_cornif, initialize_sc turns
+    the top ALIVE cells into AIR and a Ca solve runs (num_sc)."""
+    spec = configs.config_tiny(5, n_free=200, nz=24)
+    ot, gt = pair(oracle_mod, spec)
+    from epi_amd import native
+    do, dg = oracle_mod.Driver(600001, 1, 0, 0, 0), native.Driver(600001, 1, 0, 0, 0)
+    gt.step(dg, 3)
+    for _ in range(3):
+        ot.step(do)
+    check(gt, ot, exact_positions=False, tag="forced")
+    assert np.count_nonzero(gt.get_cells()["state"] == 8) > 0
+    assert gt.status().ca_solves == 1 and dg.force_cornif == 0
+
+
+def test_large_config_first_steps(oracle_mod):
+    """Plain FP64 large single tissue (128x128 sheet, 20,000 free cells,
+    128x128x256 grid, seed 2): the first steps against the oracle."""
+    spec = configs.config_large(2)
+    ot, gt = pair(oracle_mod, spec, draws=2_000_000)
+    from epi_amd import native
+    do, dg = oracle_mod.Driver(0, 0, 0, 0, 0), native.Driver(0, 0, 0, 0, 0)
+    gt.step(dg, 25)
+    for _ in range(25):
+        ot.step(do)
+    check(gt, ot, tag="large")
