@@ -69,7 +69,7 @@ struct Status {
   int ndiv;               // divisions in the last renewal
   int nrm;                // removals in the last renewal
   int maxlen;             // longest neighbour list of the last build
-  int pad0;
+  int stamp_z;            // highest z plane ever stamped into the voxel maps
   long long rng_pos;      // replay cursor
   long long next_uid;
   unsigned long long zzmax_bits;  // calc_zzmax result (non-negative double bits)
@@ -97,15 +97,15 @@ struct Tis {
   long long* uid; long long* old_uid; int* src_row;
   // neighbour rows (two buffers: current and previous build)
   int* nlen[2]; int* col[2]; double* gj[2];
-  int* lo_sorted;             // part of a row below the cell index, ascending
   int* nlo;                   // entries of the row below the cell index
   int* btmp; int* bcnt;       // upper partners gathered during the sweep
   int* memb6;                 // triangular adjacency, 6 per membrane cell
   // buckets
-  int* bucket_of; int* bucket_n; int* bucket_cells;
+  int* bucket_of; int* bslot; int* bucket_n; int* bucket_cells;
   // state renewal
   int* div_slot; double* div_dir;  // per division: parent, direction
   int* div_parent; int* rm_list;
+  uint8_t* rflag; int* chunk_any;   // renewal flags and per-1024-cell chunk marks
   // grids
   double* atp[2]; double* bf[2];
   int* cmap1; uint8_t* cmap2; unsigned* vox;

@@ -62,8 +62,8 @@ void prof_stop(hipStream_t s, int k, hipEvent_t a) {
 namespace {
 
 const char* kKernelNames[K_COUNT] = {
-    "k_forces_memb", "k_forces_other", "k_renew_scan", "k_renew_apply", "k_remove", "k_pair_disperse",
-    "k_initialize_sc", "k_periodic_fix", "k_bucket_clear", "k_bucket_insert", "k_bucket_sort", "k_sweep",
+    "k_forces", "k_forces_long", "k_renew_flags", "k_renew_scan", "k_renew_apply", "k_remove", "k_pair_disperse",
+    "k_initialize_sc", "k_periodic_fix", "k_bucket_insert", "k_bucket_sort", "k_sweep",
     "k_finish", "k_snapshot", "k_zz_set", "k_zzmax", "k_map_clear", "k_map_stamp", "k_map_halo",
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset"};
 
@@ -175,18 +175,20 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
     t.col[b] = cv.take<int>(cap * ns);
     t.gj[b] = cv.take<double>(cap * ns);
   }
-  t.lo_sorted = cv.take<int>(cap * ns);
   t.nlo = cv.take<int>(cap);
   t.btmp = cv.take<int>(cap * ns);
   t.bcnt = cv.take<int>(cap);
   t.memb6 = cv.take<int>((size_t)h.nmemb * 6);
   t.bucket_of = cv.take<int>(cap);
+  t.bslot = cv.take<int>(cap);
   t.bucket_n = cv.take<int>(h.nbuckets);
   t.bucket_cells = cv.take<int>((size_t)h.nbuckets * h.bcap);
   t.div_slot = cv.take<int>(cap);
   t.div_dir = cv.take<double>(3 * cap);
   t.div_parent = cv.take<int>(cap);
   t.rm_list = cv.take<int>(cap);
+  t.rflag = cv.take<uint8_t>(cap);
+  t.chunk_any = cv.take<int>(cap / 1024 + 2);
   for (int b = 0; b < 2; b++) { t.atp[b] = cv.take<double>(h.gsize); t.bf[b] = cv.take<double>(h.gsize); }
   t.cmap1 = cv.take<int>(h.gsize);
   t.cmap2 = cv.take<uint8_t>(h.gsize);
@@ -552,6 +554,7 @@ int epi_load_cells(epi_ctx* c, int32_t t, const epi_cell_input* in) {
   Status s{};
   s.n = n;
   s.next_uid = n;
+  s.stamp_z = h.NZ;   // the first map build clears every plane
   c->h_st[t] = s;
   RCHK(put_status(c, t));
   h.n_est = n;
@@ -681,7 +684,7 @@ int epi_cell_pos_periodic_fix(epi_ctx* c) {
 int epi_connect_cell(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
   const int nb = 1 - c->cb;
-  launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, max_n(c), max_nb(c));
+  launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, max_n(c), max_nb(c), 0);
   HIPCHK(c, hipGetLastError());
   c->cb = nb;
   return EPI_OK;
@@ -780,12 +783,12 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), mn);
     c->pb ^= 1;
     launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
-    launch_periodic_fix(c->stream, c->d_tis, c->T, c->pb, mn);
+    // periodic fix fused into the bucket insertion; calc_zzmax fused into
+    // the snapshot after the build
     const int nb = 1 - c->cb;
-    launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, mn, max_nb(c));
+    launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, mn, max_nb(c), 1);
     c->cb = nb;
-    // proc.cpp:154-157
-    launch_zzmax(c->stream, c->d_tis, c->T, c->pb, mn);
+    // proc.cpp:155-157
     launch_maps(c->stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c));
     launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false));
     c->bp ^= 1;
@@ -1011,7 +1014,7 @@ int epi_set_lists(epi_ctx* c, int32_t t, const int32_t* ptr, const int32_t* conn
   const Tis& ti = c->tis[t];
   const TH& h = c->th[t];
   const int n = c->h_st[t].n;
-  std::vector<int> col((size_t)n * h.ns, 0), los((size_t)n * h.ns, 0), nl(n), nlo(n, 0), srow(n);
+  std::vector<int> col((size_t)n * h.ns, 0), nl(n), nlo(n, 0), srow(n);
   std::vector<double> g((size_t)n * h.ns, gj_init);
   std::vector<long long> u(n);
   RCHK(download(c, u.data(), ti.uid, n));
@@ -1035,15 +1038,11 @@ int epi_set_lists(epi_ctx* c, int32_t t, const int32_t* ptr, const int32_t* conn
       if (col[(size_t)i * h.ns + e] <= i || (e > k && col[(size_t)i * h.ns + e] < col[(size_t)i * h.ns + e - 1]))
         return fail(c, EPI_ERR_INVALID, "list is not in canonical order");
     nlo[i] = lo;
-    std::vector<int> s(col.begin() + (size_t)i * h.ns, col.begin() + (size_t)i * h.ns + lo);
-    std::sort(s.begin(), s.end());
-    std::copy(s.begin(), s.end(), los.begin() + (size_t)i * h.ns);
   }
   RCHK(upload(c, ti.col[c->cb], col.data(), col.size()));
   RCHK(upload(c, ti.gj[c->cb], g.data(), g.size()));
   RCHK(upload(c, ti.nlen[c->cb], nl.data(), n));
   RCHK(upload(c, ti.nlo, nlo.data(), n));
-  RCHK(upload(c, ti.lo_sorted, los.data(), los.size()));
   RCHK(upload(c, ti.src_row, srow.data(), n));
   RCHK(upload(c, ti.old_uid, u.data(), n));
   return EPI_OK;

@@ -10,51 +10,59 @@
 // k_sweep          runs connect_proc (:70-129) for every non-membrane cell:
 //                  the lower part of its own row is written directly, the
 //                  upper parts of its partners are appended with atomics,
-// k_finish         sorts the upper parts, keeps a sorted copy of the lower
-//                  part (the force gather needs it), carries gap-junction
+// k_finish         sorts the upper parts, carries gap-junction
 //                  values over from the previous rows by cell uid
 //                  (gj_refresh :133-152) and picks the dermis (:158-185).
 #include "epi_dev.h"
 #include "launch.h"
+#include "epi_group.h"
 
 namespace epi {
 
-__global__ void __launch_bounds__(256) k_bucket_clear(const Tis* __restrict__ T) {
-  const Tis& t = T[blockIdx.y];
-  const int nb = t.ANX * t.ANY * t.ANZ;
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) t.bucket_n[b] = 0;
-}
-
-// grid_init (:38-62)
-__global__ void __launch_bounds__(256) k_bucket_insert(const Tis* __restrict__ T, int pb) {
+// grid_init (:38-62); with fix != 0 it first applies cell_pos_periodic_fix
+// (cellmanager.cpp:395-414) to the cell, which touches only that cell
+__global__ void __launch_bounds__(256) k_bucket_insert(const Tis* __restrict__ T, int pb, int fix) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
   const int n = st->n;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const double x = t.pos[pb][0][c], y = t.pos[pb][1][c], z = t.pos[pb][2][c];
+    double x = t.pos[pb][0][c], y = t.pos[pb][1][c];
+    const double z = t.pos[pb][2][c];
+    if (fix) {
+      if (x > t.LX) { x = x - t.LX; t.pos[pb][0][c] = x; }
+      else if (x < 0) { x = x + t.LX; t.pos[pb][0][c] = x; }
+      if (y > t.LY) { y = y - t.LY; t.pos[pb][1][c] = y; }
+      else if (y < 0) { y = y + t.LY; t.pos[pb][1][c] = y; }
+    }
     const long aix = (long)((0.5 * t.LX - p_diff_x(t, 0.5 * t.LX, x)) / AREA_GRID);
     const long aiy = (long)((0.5 * t.LY - p_diff_y(t, 0.5 * t.LY, y)) / AREA_GRID);
     const long aiz = (long)((min0(z)) / AREA_GRID);
     t.bcnt[c] = 0;
     t.nlo[c] = 0;
+    t.bslot[c] = -1;
     if (aix >= t.ANX || aiy >= t.ANY || aiz >= t.ANZ || aix < 0 || aiy < 0 || aiz < 0) {
       set_err(st, ERR_OUT_OF_GRID);
       t.bucket_of[c] = -1;
       continue;
     }
     const int b = (int)((aix * t.ANY + aiy) * t.ANZ + aiz);
+    t.bucket_of[c] = b;
     const int slot = atomicAdd(&t.bucket_n[b], 1);
     // the reference asserts fewer than N3 = 200 cells per bucket (:59)
     if (slot + 1 >= N3 || slot >= t.bcap) { set_err(st, ERR_CAPACITY); continue; }
     t.bucket_cells[(size_t)b * t.bcap + slot] = c;
-    t.bucket_of[c] = b;
+    t.bslot[c] = slot;
   }
 }
 
+// the cell that took slot 0 of a bucket orders that bucket by cell index
+// (the serial insertion order of grid_init)
 __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) {
   const Tis& t = T[blockIdx.y];
-  const int nb = t.ANX * t.ANY * t.ANZ;
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+  const int n = t.st->n;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    if (t.bslot[c] != 0) continue;
+    const int b = t.bucket_of[c];
     const int m = min(t.bucket_n[b], t.bcap);
     if (m < 2) continue;
     int* a = t.bucket_cells + (size_t)b * t.bcap;
@@ -68,64 +76,106 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
 }
 
 // connect_proc (:70-129): the 5x5x5 sweep, periodic in x, y and z, buckets
-// in ascending (j, k, l) with cells in ascending index inside a bucket
+// in ascending (j, k, l) with cells in ascending index inside a bucket.
+// A group of 32 lanes takes one cell; lane g < 25 sweeps the column
+// (j, k) = (g / 5, g % 5) over its 5 z buckets.  Lanes count their partners,
+// an exclusive scan over the group gives each lane its place in the row, and
+// a second pass writes the entries -- the row comes out in sweep order.
+constexpr int SWEEP_G = 32;
+
+__device__ __forceinline__ bool sweep_hit(const Tis& t, int pb, const P3& pc, double rc, int o) {
+  const P3 po = ld_pos(t, pb, o);
+  const double diffx = p_diff_x(t, pc.x, po.x);
+  const double diffy = p_diff_y(t, pc.y, po.y);
+  const double diffz = pc.z - po.z;
+  const double rad_sum = rc + t.radius[o];
+  return diffx * diffx + diffy * diffy + diffz * diffz <= LJ_THRESH * LJ_THRESH * rad_sum * rad_sum;
+}
+
 __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb, int nb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
   const int n = st->n;
   const int ANX = t.ANX, ANY = t.ANY, ANZ = t.ANZ;
+  const int lane = threadIdx.x & (SWEEP_G - 1);
+  const int groups = gridDim.x * (blockDim.x / SWEEP_G);
   int* const col = t.col[nb];
-  for (int c = t.nmemb + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+  for (int c = t.nmemb + blockIdx.x * (blockDim.x / SWEEP_G) + threadIdx.x / SWEEP_G; c < n; c += groups) {
     const P3 pc = ld_pos(t, pb, c);
     const double rc = t.radius[c];
     const long anx = (long)(pc.x / AREA_GRID);
     const long any = (long)(pc.y / AREA_GRID);
     const long anz = (long)(pc.z / AREA_GRID);
-    if (anx >= ANX || any >= ANY || anz >= ANZ || anx < 0 || any < 0 || anz < 0) {
-      set_err(st, ERR_OUT_OF_GRID);
+    const bool bad = anx >= ANX || any >= ANY || anz >= ANZ || anx < 0 || any < 0 || anz < 0;
+    if (bad) {
+      if (lane == 0) set_err(st, ERR_OUT_OF_GRID);
       continue;
     }
-    int* const row = col + (size_t)c * t.ns;
-    int na = 0;
-    for (long j = anx - 2 + ANX; j <= anx + 2 + ANX; j++) {
-      const long aix = j % ANX;
-      for (long k = any - 2 + ANY; k <= any + 2 + ANY; k++) {
-        const long aiy = k % ANY;
-        const long bxy = (aix * ANY + aiy) * ANZ;
-        for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
-          const long b = bxy + l % ANZ;
-          const int sz = min(t.bucket_n[b], t.bcap);
-          const int* cells = t.bucket_cells + (size_t)b * t.bcap;
-          for (int m = 0; m < sz; ++m) {
-            const int o = cells[m];
-            if (c <= o) break;   // ascending within the bucket
-            const P3 po = ld_pos(t, pb, o);
-            const double diffx = p_diff_x(t, pc.x, po.x);
-            const double diffy = p_diff_y(t, pc.y, po.y);
-            const double diffz = pc.z - po.z;
-            const double rad_sum = rc + t.radius[o];
-            if (diffx * diffx + diffy * diffy + diffz * diffz <= LJ_THRESH * LJ_THRESH * rad_sum * rad_sum) {
-              if (na < t.ns) row[na] = o;
-              na++;
-              const int slot = atomicAdd(&t.bcnt[o], 1);
-              if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
-            }
+    const bool act = lane < 25;
+    const long aix = (anx - 2 + lane / 5 + ANX) % ANX;
+    const long aiy = (any - 2 + lane % 5 + ANY) % ANY;
+    const long bxy = (aix * ANY + aiy) * ANZ;
+    int cnt = 0;
+    if (act) {
+      for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
+        const long b = bxy + l % ANZ;
+        const int sz = min(t.bucket_n[b], t.bcap);
+        const int* cells = t.bucket_cells + (size_t)b * t.bcap;
+        for (int m = 0; m < sz; ++m) {
+          const int o = cells[m];
+          if (c <= o) break;   // ascending within the bucket
+          cnt += sweep_hit(t, pb, pc, rc, o);
+        }
+      }
+    }
+    int total;
+    int off = group_exclusive_scan<SWEEP_G>(cnt, lane, &total);
+    if (cnt) {
+      int* const row = col + (size_t)c * t.ns;
+      for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
+        const long b = bxy + l % ANZ;
+        const int sz = min(t.bucket_n[b], t.bcap);
+        const int* cells = t.bucket_cells + (size_t)b * t.bcap;
+        for (int m = 0; m < sz; ++m) {
+          const int o = cells[m];
+          if (c <= o) break;
+          if (sweep_hit(t, pb, pc, rc, o)) {
+            if (off < t.ns) row[off] = o;
+            off++;
+            const int slot = atomicAdd(&t.bcnt[o], 1);
+            if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
           }
         }
       }
     }
-    t.nlo[c] = na;
+    if (lane == 0) t.nlo[c] = total;
   }
 }
 
-// Row assembly, gj carry-over and dermis.
-__global__ void __launch_bounds__(256) k_finish(const Tis* __restrict__ T, int pb, int nb) {
+// Row assembly, gj carry-over and dermis.  A group of KG lanes owns one
+// cell: the upper partners are placed by rank (they are distinct), the
+// previous row's uids are staged in LDS and each lane resolves the gap
+// junction of its entries there (gj_refresh :133-152: a directed edge keeps
+// its value while it stays in the list, a new edge starts at gj_init), and
+// the leader picks the dermis in list order (find_dermis :158-172).
+constexpr int KG = 16;
+constexpr int KCAP = 128;
+constexpr int K_BLOCK = 128;
+constexpr int K_GROUPS = K_BLOCK / KG;
+
+__global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, int pb, int nb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
   const int n = st->n;
   const int ob = 1 - nb;
+  __shared__ long long s_ou[K_GROUPS][KCAP];
+  __shared__ double s_d[K_GROUPS][KCAP];
+  const int lane = threadIdx.x & (KG - 1);
+  const int grp = threadIdx.x / KG;
   int mylen = 0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->zzmax_bits = 0ull;   // reduced by k_snapshot
+  const int stride = gridDim.x * K_GROUPS;
+  for (int c = blockIdx.x * K_GROUPS + grp; c < n; c += stride) {
     const bool memb = c < t.nmemb;
     const int base = memb ? MEMB_ADJ : 0;
     const int na = memb ? 0 : t.nlo[c];
@@ -133,81 +183,106 @@ __global__ void __launch_bounds__(256) k_finish(const Tis* __restrict__ T, int p
     const int len = base + na + nbh;
     // the reference asserts a list shorter than N2 = 400 (:122)
     if (len >= MAX_CONN || len > t.ns) {
-      set_err(st, ERR_CAPACITY);
-      t.nlen[nb][c] = 0;
+      if (lane == 0) { set_err(st, ERR_CAPACITY); t.nlen[nb][c] = 0; }
       continue;
     }
     mylen = max(mylen, len);
     int* const row = t.col[nb] + (size_t)c * t.ns;
-    if (memb)
-      for (int i = 0; i < MEMB_ADJ; i++) row[i] = t.memb6[6 * c + i];
-    // upper partners, ascending
+    if (memb && lane < MEMB_ADJ) row[lane] = t.memb6[6 * c + lane];
+    // upper partners, ascending, by rank
     const int* bt = t.btmp + (size_t)c * t.ns;
-    int* const up = row + base + na;
-    for (int i = 0; i < nbh; i++) {
+    for (int i = lane; i < nbh; i += KG) {
       const int v = bt[i];
-      int j = i - 1;
-      while (j >= 0 && up[j] > v) { up[j + 1] = up[j]; j--; }
-      up[j + 1] = v;
+      int r = 0;
+      for (int k = 0; k < nbh; k++) r += bt[k] < v;
+      row[base + na + r] = v;
     }
-    // sorted copy of the lower part
-    int* const ls = t.lo_sorted + (size_t)c * t.ns;
-    for (int i = 0; i < na; i++) {
-      const int v = row[i];
-      int j = i - 1;
-      while (j >= 0 && ls[j] > v) { ls[j + 1] = ls[j]; j--; }
-      ls[j + 1] = v;
-    }
-    t.nlen[nb][c] = len;
-    // gj_refresh: a directed edge keeps its value while it stays in the list.
-    // Rows change little between builds, so the search starts after the
-    // previous match and wraps around once.
-    double* const g = t.gj[nb] + (size_t)c * t.ns;
+    if (lane == 0) t.nlen[nb][c] = len;
+    // previous row of this cell
     const int src = t.src_row[c];
+    const int olen = src >= 0 ? t.nlen[ob][src] : 0;
     const int* orow = src >= 0 ? t.col[ob] + (size_t)src * t.ns : nullptr;
     const double* og = src >= 0 ? t.gj[ob] + (size_t)src * t.ns : nullptr;
-    const int olen = src >= 0 ? t.nlen[ob][src] : 0;
-    int hint = 0;
-    for (int e = 0; e < len; e++) {
+    const bool staged = olen <= KCAP;
+    if (staged)
+      for (int f = lane; f < olen; f += KG) s_ou[grp][f] = t.old_uid[orow[f]];
+    __threadfence_block();
+    wave_sync();
+    double* const g = t.gj[nb] + (size_t)c * t.ns;
+    for (int e = lane; e < len; e += KG) {
       const long long u = t.uid[row[e]];
       double v = gj_init;
       for (int f = 0; f < olen; f++) {
-        int q = hint + f;
+        int q = e + f;
         if (q >= olen) q -= olen;
-        if (t.old_uid[orow[q]] == u) { v = og[q]; hint = q + 1; break; }
+        const long long w = staged ? s_ou[grp][q] : t.old_uid[orow[q]];
+        if (w == u) { v = og[q]; break; }
       }
       g[e] = v;
     }
     // set_dermis / find_dermis (:158-185)
-    if (!memb && c >= t.nmemb + t.nder) {
-      const uint8_t s = t.state[c];
-      if (s == FIX || s == MUSUME) {
+    const uint8_t s = t.state[c];
+    if (!memb && c >= t.nmemb + t.nder && (s == FIX || s == MUSUME)) {
+      const P3 pc = ld_pos(t, pb, c);
+      const bool fits = len <= KCAP;
+      for (int e = lane; e < len && fits; e += KG) {
+        const int o = row[e];
+        s_d[grp][e] = t.state[o] == MEMB ? dist_sq(t, pc, ld_pos(t, pb, o)) : -1.0;
+      }
+      wave_sync();
+      if (lane == 0) {
         double d1Sq = t.LX * t.LX;
         int dermis = -1;
-        const P3 pc = ld_pos(t, pb, c);
         for (int e = 0; e < len; e++) {
           const int o = row[e];
-          if (t.state[o] == MEMB) {
-            const double distanceSq = dist_sq(t, pc, ld_pos(t, pb, o));
-            if (distanceSq < d1Sq) { d1Sq = distanceSq; dermis = o; }
+          double distanceSq;
+          if (fits) {
+            distanceSq = s_d[grp][e];
+            if (distanceSq < 0) continue;
+          } else {
+            if (t.state[o] != MEMB) continue;
+            distanceSq = dist_sq(t, pc, ld_pos(t, pb, o));
           }
+          if (distanceSq < d1Sq) { d1Sq = distanceSq; dermis = o; }
         }
         t.dermis[c] = dermis;
       }
     }
+    wave_sync();
   }
   for (int o = 32; o > 0; o >>= 1) mylen = max(mylen, __shfl_down(mylen, o, 64));
   if ((threadIdx.x & 63) == 0 && mylen > 0) atomicMax(&st->maxlen, mylen);
 }
 
-// after a build: every cell's row is its own, uids snapshot the numbering
-__global__ void __launch_bounds__(256) k_snapshot(const Tis* __restrict__ T) {
+// after a build: every cell's row is its own, uids snapshot the numbering,
+// the buckets are emptied for the next build, and calc_zzmax (proc.cpp:34-43)
+// is reduced (z values that matter are positive, where the IEEE bit order is
+// the numeric order)
+__global__ void __launch_bounds__(256) k_snapshot(const Tis* __restrict__ T, int pb) {
   const Tis& t = T[blockIdx.y];
-  const int n = t.st->n;
+  Status* st = t.st;
+  const int n = st->n;
+  const int start = t.nmemb + t.nder;
+  double m = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     t.src_row[c] = c;
     t.old_uid[c] = t.uid[c];
+    const int b = t.bucket_of[c];
+    if (b >= 0) t.bucket_n[b] = 0;
+    if (c >= start) {
+      const uint8_t s = t.state[c];
+      if (s == DEAD || s == ALIVE || s == MUSUME || s == FIX) {
+        const double z = t.pos[pb][2][c];
+        if (m < z) m = z;
+      }
+    }
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_down(m, o, 64);
+    if (m < v) m = v;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0)
+    atomicMax(&st->zzmax_bits, (unsigned long long)__double_as_longlong(m));
 }
 
 // calc_zzmax (proc.cpp:34-43): z values that matter are positive, where the
@@ -237,14 +312,16 @@ __global__ void k_zz_set(const Tis* __restrict__ T, int use_value, double v) {
   if (threadIdx.x == 0) t.st->zzmax_bits = use_value ? (unsigned long long)__double_as_longlong(v) : 0ull;
 }
 
-void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets) {
-  dim3 gb((max_buckets + 255) / 256, ntis), gc((max_n + 255) / 256, ntis);
-  EPI_LAUNCH(K_BUCKET_CLEAR, k_bucket_clear, gb, dim3(256), 0, s, dT);
-  EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb);
-  EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gb, dim3(256), 0, s, dT);
-  EPI_LAUNCH(K_SWEEP, k_sweep, gc, dim3(256), 0, s, dT, pb, nb);
-  EPI_LAUNCH(K_FINISH, k_finish, gc, dim3(256), 0, s, dT, pb, nb);
-  EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT);
+void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix) {
+  (void)max_buckets;
+  dim3 gc((max_n + 255) / 256, ntis);
+  EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb, fix);
+  EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gc, dim3(256), 0, s, dT);
+  dim3 gs((max_n + 7) / 8, ntis);
+  EPI_LAUNCH(K_SWEEP, k_sweep, gs, dim3(256), 0, s, dT, pb, nb);
+  dim3 gk((max_n + K_GROUPS - 1) / K_GROUPS, ntis);
+  EPI_LAUNCH(K_FINISH, k_finish, gk, dim3(K_BLOCK), 0, s, dT, pb, nb);
+  EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT, pb);
 }
 
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {

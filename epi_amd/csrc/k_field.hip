@@ -8,15 +8,19 @@
 // the z chunks are grid-strided, so no host round trip is needed.
 #include "epi_dev.h"
 #include "launch.h"
+#include "epi_group.h"
 
 namespace epi {
 
 constexpr int ZB = 4;   // z planes per thread per chunk
 
 // ---- voxel maps ---------------------------------------------------------
+// Only planes 0..stamp_z can hold stamps (stamp_z never decreases), so only
+// they are cleared; the rest of the maps stays empty from the first build.
 __global__ void __launch_bounds__(256) k_map_clear(const Tis* __restrict__ T) {
   const Tis& t = T[blockIdx.y];
-  const long long g = t.gsize;
+  const int planes = min(t.st->stamp_z, t.NZ) + 1;
+  const long long g = (long long)planes * t.PL;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < g;
        v += (long long)gridDim.x * blockDim.x) {
     t.cmap1[v] = -1;
@@ -32,11 +36,14 @@ __device__ __forceinline__ int lat_wrap(long v, int N, Status* st) {
 }
 
 // set_lattice (:18-22) and the sphere stamping (:39-155); the last writer of
-// the serial loop is the highest cell index, hence atomicMax on cmap1
+// the serial loop is the highest cell index, hence atomicMax on cmap1.  One
+// lane per cell: lanes of a wave then stamp different cells, which keeps the
+// atomics of one instruction on different cache lines.
 __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, int pb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
   const int n = st->n;
+  int ztop = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     const P3 p = ld_pos(t, pb, c);
     const int lx = lat_wrap((long)(int)(p.x * t.inv_dx), t.NX, st);
@@ -53,6 +60,7 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
     const int z_b = lz - irz, z_u = lz + irz;
     const int zmin = z_b >= 1 ? z_b : 1;
     const int zmax = z_u < t.NZ ? z_u : t.NZ - 1;
+    ztop = max(ztop, zmax);
     for (int k = lx - irx; k <= lx + irx; k++) {
       const double mx = k * t.dx;
       const int ipx = lat_wrap(k, t.NX, st);
@@ -78,6 +86,8 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
       }
     }
   }
+  for (int o = 32; o > 0; o >>= 1) ztop = max(ztop, __shfl_down(ztop, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&st->stamp_z, ztop);
 }
 
 // periodic planes x = NX and y = NY for l < NZ (:156-166)
@@ -85,7 +95,7 @@ __global__ void __launch_bounds__(256) k_map_halo(const Tis* __restrict__ T) {
   const Tis& t = T[blockIdx.y];
   const int NX = t.NX, NY = t.NY;
   const int per_l = (NX + 1) + (NY + 1);
-  const long long tot = (long long)per_l * t.NZ;
+  const long long tot = (long long)per_l * min(t.st->stamp_z + 1, t.NZ);
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
     const int l = (int)(i / per_l), r = (int)(i % per_l);
     long long dst, src;
@@ -233,69 +243,168 @@ __global__ void __launch_bounds__(256) k_ca_vox(const Tis* __restrict__ T, const
   }
 }
 
-// dead_IP3_calc (:156-173) and supra_calc (:182-222), one thread per cell
-__global__ void __launch_bounds__(256) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp, int ap, int bp) {
+// dead_IP3_calc (:156-173) and supra_calc (:182-222).  A group of CG lanes
+// owns one cell: lanes evaluate the gap-junction terms of the list entries
+// (and update gj toward ALIVE neighbours) and load the 16 grid corners of
+// the two grid_avg8; the leader sums them in list / corner order.
+constexpr int CG = 16;
+constexpr int CCAP = 64;
+constexpr int C_BLOCK = 128;
+constexpr int C_GROUPS = C_BLOCK / CG;
+
+__device__ __forceinline__ double fw_term(double diff, double w) {   // fw (:138-144)
+  return (1. - w) + (-1. + tanh((0.1 - diff) / 0.1)) / 2.;
+}
+
+// the serial form of one cell, for lists longer than the LDS slots
+__device__ void ca_cell_serial(const Tis& t, int c, int cb, int cp, int ap, int bp) {
+  const double* ca = t.ca2p[cp];
+  const double* ip3 = t.ip3[cp];
+  const uint8_t s = t.state[c];
+  const int* row = t.col[cb] + (size_t)c * t.ns;
+  const int len = t.nlen[cb][c];
+  if (s == DEAD) {
+    int count = 0;
+    double tmp = 0;
+    const double pc = ip3[c];
+    for (int e = 0; e < len; e++) {
+      const int o = row[e];
+      if (t.state[o] == ALIVE) { tmp += ip3[o]; count++; }
+    }
+    tmp = DT_Ca * (dp_c * (tmp - count * pc) - Kpp * pc);
+    t.ip3[1 - cp][c] = pc + tmp;
+    return;
+  }
+  const double ca2p_du = 0.01;
+  const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
+  const bool is_alive = s == ALIVE;
+  const double agek = t.agek[c];
+  const double _th = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
+  const double _Kpa = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
+  const double IAGv = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
+  const double u = ca[c], p = ip3[c];
+  double tmp_diffu = 0, tmp_IP3 = 0;
+  double* g = t.gj[cb] + (size_t)c * t.ns;
+  for (int e = 0; e < len; e++) {
+    const int o = row[e];
+    const uint8_t so = t.state[o];
+    if (so == ALIVE || so == DEAD || so == FIX || so == MUSUME) {
+      const double gv = g[e];
+      const double uo = ca[o];
+      tmp_diffu += gv * (uo - u);
+      tmp_IP3 += gv * (ip3[o] - p);
+      if (so == ALIVE) g[e] = gv + DT_Ca * fw_term(fabs(uo - u), gv);
+    }
+  }
+  const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
+  const double Aavg = grid_avg8(t, t.atp[ap], ix, iy, iz);
+  const double diff_u = ca2p_reaction_factor(u, t.ex_inert[c], p, Bavg) + ca2p_du * IAGv * tmp_diffu;
+  t.diff_u[c] = diff_u;
+  const double H0 = 0.5;
+  t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
+  const double cs = u + DT_Ca * diff_u;
+  t.ca2p[1 - cp][c] = cs;
+  t.ca2p_avg[c] = t.ca2p_avg[c] + cs;
+  const double para_k2 = 0.7;
+  const double exi = t.ex_inert[c];
+  t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);
+}
+
+__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp, int ap, int bp) {
   const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
   const int n = t.st->n;
+  __shared__ double s_t1[C_GROUPS][CCAP];
+  __shared__ double s_t2[C_GROUPS][CCAP];
+  __shared__ uint8_t s_f[C_GROUPS][CCAP];
+  __shared__ double s_g[C_GROUPS][16];
+  const int lane = threadIdx.x & (CG - 1);
+  const int grp = threadIdx.x / CG;
   const double* __restrict__ ca = t.ca2p[cp];
-  double* __restrict__ can = t.ca2p[1 - cp];
   const double* __restrict__ ip3 = t.ip3[cp];
-  double* __restrict__ ip3n = t.ip3[1 - cp];
-  for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+  const int stride = gridDim.x * C_GROUPS;
+  for (int c = t.nmemb + t.nder + blockIdx.x * C_GROUPS + grp; c < n; c += stride) {
     const uint8_t s = t.state[c];
-    const int* row = t.col[cb] + (size_t)c * t.ns;
+    const bool dead = s == DEAD;
+    if (!dead && !is_afm(s)) continue;
     const int len = t.nlen[cb][c];
-    if (s == DEAD) {
-      int count = 0;
-      double tmp = 0;
-      const double pc = ip3[c];
-      for (int e = 0; e < len; e++) {
+    if (len > CCAP) {
+      if (lane == 0) ca_cell_serial(t, c, cb, cp, ap, bp);
+      continue;
+    }
+    const int* row = t.col[cb] + (size_t)c * t.ns;
+    if (dead) {
+      for (int e = lane; e < len; e += CG) {
         const int o = row[e];
-        if (t.state[o] == ALIVE) { tmp += ip3[o]; count++; }
+        const bool al = t.state[o] == ALIVE;
+        s_f[grp][e] = al;
+        s_t1[grp][e] = al ? ip3[o] : 0.0;
       }
-      tmp = DT_Ca * (dp_c * (tmp - count * pc) - Kpp * pc);
-      ip3n[c] = pc + tmp;
-    } else if (is_afm(s)) {
+      wave_sync();
+      if (lane == 0) {
+        int count = 0;
+        double tmp = 0;
+        const double pc = ip3[c];
+        for (int e = 0; e < len; e++)
+          if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
+        tmp = DT_Ca * (dp_c * (tmp - count * pc) - Kpp * pc);
+        t.ip3[1 - cp][c] = pc + tmp;
+      }
+      wave_sync();
+      continue;
+    }
+    const double u = ca[c], p = ip3[c];
+    double* g = t.gj[cb] + (size_t)c * t.ns;
+    for (int e = lane; e < len; e += CG) {
+      const int o = row[e];
+      const uint8_t so = t.state[o];
+      uint8_t f = 0;
+      double a1 = 0, a2 = 0;
+      if (so == ALIVE || so == DEAD || so == FIX || so == MUSUME) {
+        const double gv = g[e];
+        const double uo = ca[o];
+        a1 = gv * (uo - u);
+        a2 = gv * (ip3[o] - p);
+        f = 1;
+        if (so == ALIVE) g[e] = gv + DT_Ca * fw_term(fabs(uo - u), gv);
+      }
+      s_f[grp][e] = f;
+      s_t1[grp][e] = a1;
+      s_t2[grp][e] = a2;
+    }
+    {
+      // corners of grid_avg8 (utils.h:18-23): lanes 0-7 B, 8-15 ATP
+      const int k = lane & 7;
+      const int ix = t.lat[3 * c] + ((k == 1 || k == 4 || k == 5 || k == 7) ? 1 : 0);
+      const int iy = t.lat[3 * c + 1] + ((k == 2 || k == 4 || k == 6 || k == 7) ? 1 : 0);
+      const int iz = t.lat[3 * c + 2] + ((k == 3 || k == 5 || k == 6 || k == 7) ? 1 : 0);
+      s_g[grp][lane] = (lane < 8 ? t.bf[bp] : t.atp[ap])[gidx(t, ix, iy, iz)];
+    }
+    wave_sync();
+    if (lane == 0) {
       const double ca2p_du = 0.01;
-      const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
       const bool is_alive = s == ALIVE;
       const double agek = t.agek[c];
-      // calc_th, calc_Kpa, calc_IAG (:90-123)
       const double _th = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
       const double _Kpa = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
       const double IAGv = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
-      const double u = ca[c], p = ip3[c];
       double tmp_diffu = 0, tmp_IP3 = 0;
-      double* g = t.gj[cb] + (size_t)c * t.ns;
-      for (int e = 0; e < len; e++) {
-        const int o = row[e];
-        const uint8_t so = t.state[o];
-        if (so == ALIVE || so == DEAD || so == FIX || so == MUSUME) {
-          const double gv = g[e];
-          const double uo = ca[o];
-          tmp_diffu += gv * (uo - u);
-          tmp_IP3 += gv * (ip3[o] - p);
-          if (so == ALIVE) {
-            // fw (:138-144)
-            const double diff = fabs(uo - u);
-            const double fwv = (1. - gv) + (-1. + tanh((0.1 - diff) / 0.1)) / 2.;
-            g[e] = gv + DT_Ca * fwv;
-          }
-        }
-      }
-      const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
-      const double Aavg = grid_avg8(t, t.atp[ap], ix, iy, iz);
+      for (int e = 0; e < len; e++)
+        if (s_f[grp][e]) { tmp_diffu += s_t1[grp][e]; tmp_IP3 += s_t2[grp][e]; }
+      const double* q = s_g[grp];
+      const double Bavg = 0.125 * (q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7]);
+      const double Aavg = 0.125 * (q[8] + q[9] + q[10] + q[11] + q[12] + q[13] + q[14] + q[15]);
       const double diff_u = ca2p_reaction_factor(u, t.ex_inert[c], p, Bavg) + ca2p_du * IAGv * tmp_diffu;
       t.diff_u[c] = diff_u;
       const double H0 = 0.5;
-      ip3n[c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
+      t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
       const double cs = u + DT_Ca * diff_u;
-      can[c] = cs;
+      t.ca2p[1 - cp][c] = cs;
       t.ca2p_avg[c] = t.ca2p_avg[c] + cs;
       const double para_k2 = 0.7;
       const double exi = t.ex_inert[c];
       t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);
     }
+    wave_sync();
   }
 }
 
@@ -412,7 +521,8 @@ void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int nt
 
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
                        int max_n, int max_nx, int max_ny, int izb_est) {
-  EPI_LAUNCH(K_CA_CELLS, k_ca_cells, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, cb, cp, ap, bp);
+  EPI_LAUNCH(K_CA_CELLS, k_ca_cells, dim3((max_n + C_GROUPS - 1) / C_GROUPS, ntis), dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap,
+             bp);
   const int nky = (max_ny + 1 + 3) / 4;
   const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
   dim3 g((max_nx + 1 + 63) / 64, ntis, nky * nzc);

@@ -9,6 +9,7 @@
 // FP64 result is the serial one bit for bit.
 #include "epi_dev.h"
 #include "launch.h"
+#include "epi_group.h"
 
 namespace epi {
 
@@ -212,60 +213,45 @@ __device__ P3 tri_bend_2(const Tis& t, P3 r1v, P3 a1v, P3 b1v, P3 c1v) {   // :4
                        v_div(v_cross(vpsub(t, a1v, b1v), tri_rho(norm2, norm1)), v_norm(norm2_pre))));
 }
 
-// ---- membrane particles: bend, wall, membrane-membrane, others on them ----
-__global__ void __launch_bounds__(256) k_forces_memb(const Tis* __restrict__ T, int pb, int cb) {
-  const Tis& t = T[blockIdx.y];
-  Status* st = t.st;
-  const int nmemb = t.nmemb;
-  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < nmemb; X += gridDim.x * blockDim.x) {
-    const CV cx = load_cv(t, pb, X);
-    P3 acc = cx.p;
+// ---- serial form: one lane does a whole cell --------------------------------
+// Used for rows longer than the LDS slots of the cooperative kernel.
+__device__ void forces_serial(const Tis& t, int pb, int cb, int X, Status* st) {
+  const CV cx = load_cv(t, pb, X);
+  P3 acc = cx.p;
+  const int* row = t.col[cb] + (size_t)X * t.ns;
+  const int len = t.nlen[cb][X];
+  if (X < t.nmemb) {
     const int* md = t.memb6 + 6 * X;
-    int m[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) m[i] = md[i];
-    // tri_memb_bend (:488-499)
-    {
-      P3 mp[6];
-#pragma unroll
-      for (int i = 0; i < 6; i++) mp[i] = ld_pos(t, pb, m[i]);
-      P3 dr = P3{0, 0, 0};
-      for (int i = 0; i < 6; i++) {
-        const int i1 = (i + 1) % 6;
-        const int c1 = t.memb6[6 * m[i1] + i];
-        dr = v_add(dr, tri_bend_1(t, cx.p, mp[i], mp[i1], ld_pos(t, pb, c1)));
-      }
-      for (int i = 0; i < 6; i++)
-        dr = v_add(dr, tri_bend_2(t, cx.p, mp[i], mp[(i + 1) % 6], mp[(i + 2) % 6]));
-      acc.x += DT_Cell * KBEND * dr.x;
-      acc.y += DT_Cell * KBEND * dr.y;
-      acc.z += DT_Cell * KBEND * dr.z;
+    P3 dr = P3{0, 0, 0};
+    for (int i = 0; i < 6; i++) {
+      const int i1 = (i + 1) % 6;
+      dr = v_add(dr, tri_bend_1(t, cx.p, ld_pos(t, pb, md[i]), ld_pos(t, pb, md[i1]),
+                                ld_pos(t, pb, t.memb6[6 * md[i1] + i])));
     }
-    // _MEMB_interaction (:529-547): wall, then own slots toward lower indices
+    for (int i = 0; i < 6; i++)
+      dr = v_add(dr, tri_bend_2(t, cx.p, ld_pos(t, pb, md[i]), ld_pos(t, pb, md[(i + 1) % 6]),
+                                ld_pos(t, pb, md[(i + 2) % 6])));
+    acc.x += DT_Cell * KBEND * dr.x;
+    acc.y += DT_Cell * KBEND * dr.y;
+    acc.z += DT_Cell * KBEND * dr.z;
     if (cx.p.z < cx.r) acc.z += wall_dz(cx);
     int hi[6], nhi = 0;
     for (int i = 0; i < 6; i++) {
-      const int q = m[i];
+      const int q = md[i];
       if (t.state[q] != MEMB) continue;
       if (X > q) {
-        const CV cq = load_cv(t, pb, q);
-        const P3 d = delta(t, LAW_MEMB_MEMB, cx, cq, st);
+        const P3 d = delta(t, LAW_MEMB_MEMB, cx, load_cv(t, pb, q), st);
         acc.x += d.x; acc.y += d.y; acc.z += d.z;
       } else if (q > X) {
-        int k = nhi++;          // insertion into ascending order
+        int k = nhi++;
         while (k > 0 && hi[k - 1] > q) { hi[k] = hi[k - 1]; k--; }
         hi[k] = q;
       }
     }
-    // higher-index membrane particles process X in their own iterations
     for (int i = 0; i < nhi; i++) {
-      const CV ch = load_cv(t, pb, hi[i]);
-      const P3 d = delta(t, LAW_MEMB_MEMB, ch, cx, st);
+      const P3 d = delta(t, LAW_MEMB_MEMB, load_cv(t, pb, hi[i]), cx, st);
       acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
     }
-    // non-membrane cells (all of higher index) process X in ascending order
-    const int* row = t.col[cb] + (size_t)X * t.ns;
-    const int len = t.nlen[cb][X];
     for (int i = MEMB_ADJ; i < len; i++) {
       const int p = row[i];
       const CV cp = load_cv(t, pb, p);
@@ -275,36 +261,22 @@ __global__ void __launch_bounds__(256) k_forces_memb(const Tis* __restrict__ T, 
         acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
       }
     }
-    t.pos[1 - pb][0][X] = acc.x;
-    t.pos[1 - pb][1][X] = acc.y;
-    t.pos[1 - pb][2][X] = acc.z;
-  }
-}
-
-// ---- every other cell -----------------------------------------------------
-__global__ void __launch_bounds__(256) k_forces_other(const Tis* __restrict__ T, int pb, int cb) {
-  const Tis& t = T[blockIdx.y];
-  Status* st = t.st;
-  const int n = st->n;
-  for (int X = t.nmemb + blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
-    const CV cx = load_cv(t, pb, X);
-    P3 acc = cx.p;
-    const int* row = t.col[cb] + (size_t)X * t.ns;
-    const int* los = t.lo_sorted + (size_t)X * t.ns;
-    const int len = t.nlen[cb][X];
+  } else {
     const int nlo = t.nlo[X];
-    // 1. lower-index non-membrane handlers that process X, ascending
-    for (int i = 0; i < nlo; i++) {
-      const int p = los[i];
-      if (t.state[p] == MEMB) continue;
-      const CV cp = load_cv(t, pb, p);
-      const int law = proc_law(cp, p, cx, X);
+    // lower-index handlers in ascending order (selection over the lower part)
+    int last = -1;
+    for (int r = 0; r < nlo; r++) {
+      int best = 0x7fffffff;
+      for (int i = 0; i < nlo; i++) { const int v = row[i]; if (v > last && v < best) best = v; }
+      last = best;
+      if (t.state[best] == MEMB) continue;
+      const CV cp = load_cv(t, pb, best);
+      const int law = proc_law(cp, best, cx, X);
       if (law != LAW_NONE) {
         const P3 d = delta(t, law, cp, cx, st);
         acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
       }
     }
-    // 2. X's own handler: wall (DER), its list in order, then its pair
     if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(cx);
     for (int i = 0; i < len; i++) {
       const int q = row[i];
@@ -316,18 +288,14 @@ __global__ void __launch_bounds__(256) k_forces_other(const Tis* __restrict__ T,
       }
     }
     if (cx.pair >= 0 && X > cx.pair) {
-      const CV cq = load_cv(t, pb, cx.pair);
-      const P3 d = delta(t, LAW_PAIR, cx, cq, st);
+      const P3 d = delta(t, LAW_PAIR, cx, load_cv(t, pb, cx.pair), st);
       acc.x += d.x; acc.y += d.y; acc.z += d.z;
     }
-    // 3. higher-index handlers in ascending order; a higher-index pair partner
-    //    applies the spring after its list (:678)
     int P = cx.pair > X ? cx.pair : -1;
     for (int i = nlo; i < len; i++) {
       const int p = row[i];
       if (P >= 0 && P < p) {
-        const CV cq = load_cv(t, pb, P);
-        const P3 d = delta(t, LAW_PAIR, cq, cx, st);
+        const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
         acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
         P = -1;
       }
@@ -344,25 +312,177 @@ __global__ void __launch_bounds__(256) k_forces_other(const Tis* __restrict__ T,
       }
     }
     if (P >= 0) {
-      const CV cq = load_cv(t, pb, P);
-      const P3 d = delta(t, LAW_PAIR, cq, cx, st);
+      const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
       acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
     }
-    t.pos[1 - pb][0][X] = acc.x;
-    t.pos[1 - pb][1][X] = acc.y;
-    t.pos[1 - pb][2][X] = acc.z;
+  }
+  t.pos[1 - pb][0][X] = acc.x;
+  t.pos[1 - pb][1][X] = acc.y;
+  t.pos[1 - pb][2][X] = acc.z;
+}
+
+// ---- cooperative form --------------------------------------------------------
+// A group of FG lanes owns one cell.  Every list entry e is handled by lane
+// e % FG, which evaluates both displacements the pair can contribute to the
+// cell: "own" (the cell's handler processes the entry, added) and "oth" (the
+// entry's handler processes the cell, subtracted).  The group leader then
+// adds them in the serial order of cell_interaction (:650-679): bending,
+// wall, lower-index handlers ascending, the cell's own list, its pair,
+// higher-index handlers ascending with a higher-index partner's spring
+// after that partner's list.  Lane 0..11 evaluate the 12 dihedral terms of a
+// membrane particle.
+constexpr int FG = 16;
+constexpr int FCAP = 48;
+constexpr int F_BLOCK = 128;
+constexpr int F_GROUPS = F_BLOCK / FG;
+
+__global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, int pb, int cb) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  __shared__ P3 s_own[F_GROUPS][FCAP];
+  __shared__ P3 s_oth[F_GROUPS][FCAP];
+  __shared__ int s_idx[F_GROUPS][FCAP];
+  __shared__ uint8_t s_flag[F_GROUPS][FCAP];
+  __shared__ uint8_t s_perm[F_GROUPS][FCAP];
+  __shared__ P3 s_bend[F_GROUPS][12];
+  const int lane = threadIdx.x & (FG - 1);
+  const int grp = threadIdx.x / FG;
+  const int n = st->n;
+  const int stride = gridDim.x * F_GROUPS;
+  for (int X = blockIdx.x * F_GROUPS + grp; X < n; X += stride) {
+    const int len = t.nlen[cb][X];
+    if (len > FCAP) continue;   // k_forces_long
+    const CV cx = load_cv(t, pb, X);
+    const bool memb = X < t.nmemb;
+    const int nA = memb ? 0 : t.nlo[X];
+    const int* row = t.col[cb] + (size_t)X * t.ns;
+    for (int e = lane; e < len; e += FG) {
+      const int q = row[e];
+      const CV cq = load_cv(t, pb, q);
+      uint8_t f = 0;
+      P3 own = P3{0, 0, 0}, oth = P3{0, 0, 0};
+      if (memb) {
+        if (e < MEMB_ADJ) {
+          if (cq.s == MEMB) {
+            if (X > q) { own = delta(t, LAW_MEMB_MEMB, cx, cq, st); f |= 1; }
+            else if (q > X) { oth = delta(t, LAW_MEMB_MEMB, cq, cx, st); f |= 2; }
+          }
+        } else {
+          const int law2 = proc_law(cq, q, cx, X);
+          if (law2 != LAW_NONE) { oth = delta(t, law2, cq, cx, st); f |= 2; }
+        }
+      } else {
+        const int law = proc_law(cx, X, cq, q);
+        if (law != LAW_NONE) { own = delta(t, law, cx, cq, st); f |= 1; }
+        if (e >= nA || cq.s != MEMB) {
+          const int law2 = proc_law(cq, q, cx, X);
+          if (law2 != LAW_NONE) { oth = delta(t, law2, cq, cx, st); f |= 2; }
+        }
+      }
+      s_idx[grp][e] = q;
+      s_flag[grp][e] = f;
+      s_own[grp][e] = own;
+      s_oth[grp][e] = oth;
+    }
+    if (memb && lane < 12) {
+      const int* md = t.memb6 + 6 * X;
+      const int i = lane < 6 ? lane : lane - 6;
+      const P3 a = ld_pos(t, pb, md[i]);
+      const P3 b = ld_pos(t, pb, md[(i + 1) % 6]);
+      if (lane < 6)
+        s_bend[grp][lane] = tri_bend_1(t, cx.p, a, b, ld_pos(t, pb, t.memb6[6 * md[(i + 1) % 6] + i]));
+      else
+        s_bend[grp][lane] = tri_bend_2(t, cx.p, a, b, ld_pos(t, pb, md[(i + 2) % 6]));
+    }
+    wave_sync();
+    // rank of each lower-part entry by cell index
+    for (int e = lane; e < nA; e += FG) {
+      const int v = s_idx[grp][e];
+      int r = 0;
+      for (int k = 0; k < nA; k++) r += s_idx[grp][k] < v;
+      s_perm[grp][r] = (uint8_t)e;
+    }
+    wave_sync();
+    if (lane == 0) {
+      P3 acc = cx.p;
+      if (memb) {
+        P3 dr = P3{0, 0, 0};
+        for (int i = 0; i < 12; i++) dr = v_add(dr, s_bend[grp][i]);
+        acc.x += DT_Cell * KBEND * dr.x;
+        acc.y += DT_Cell * KBEND * dr.y;
+        acc.z += DT_Cell * KBEND * dr.z;
+        if (cx.p.z < cx.r) acc.z += wall_dz(cx);
+        int hi[6], nhi = 0;
+        for (int e = 0; e < MEMB_ADJ; e++) {
+          const uint8_t f = s_flag[grp][e];
+          if (f & 1) { const P3 d = s_own[grp][e]; acc.x += d.x; acc.y += d.y; acc.z += d.z; }
+          if (f & 2) {
+            int k = nhi++;
+            const int q = s_idx[grp][e];
+            while (k > 0 && s_idx[grp][hi[k - 1]] > q) { hi[k] = hi[k - 1]; k--; }
+            hi[k] = e;
+          }
+        }
+        for (int i = 0; i < nhi; i++) {
+          const P3 d = s_oth[grp][hi[i]];
+          acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
+        }
+        for (int e = MEMB_ADJ; e < len; e++)
+          if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
+      } else {
+        for (int r = 0; r < nA; r++) {
+          const int e = s_perm[grp][r];
+          if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
+        }
+        if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(cx);
+        for (int e = 0; e < len; e++)
+          if (s_flag[grp][e] & 1) { const P3 d = s_own[grp][e]; acc.x += d.x; acc.y += d.y; acc.z += d.z; }
+        if (cx.pair >= 0 && X > cx.pair) {
+          const P3 d = delta(t, LAW_PAIR, cx, load_cv(t, pb, cx.pair), st);
+          acc.x += d.x; acc.y += d.y; acc.z += d.z;
+        }
+        int P = cx.pair > X ? cx.pair : -1;
+        for (int e = nA; e < len; e++) {
+          const int p = s_idx[grp][e];
+          if (P >= 0 && P < p) {
+            const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
+            acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
+            P = -1;
+          }
+          if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
+          if (p == P) {
+            const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
+            acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
+            P = -1;
+          }
+        }
+        if (P >= 0) {
+          const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
+          acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
+        }
+      }
+      t.pos[1 - pb][0][X] = acc.x;
+      t.pos[1 - pb][1][X] = acc.y;
+      t.pos[1 - pb][2][X] = acc.z;
+    }
+    wave_sync();
   }
 }
 
+// cells whose rows exceed the LDS slots of k_forces, one lane per cell
+__global__ void __launch_bounds__(256) k_forces_long(const Tis* __restrict__ T, int pb, int cb) {
+  const Tis& t = T[blockIdx.y];
+  const int n = t.st->n;
+  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x)
+    if (t.nlen[cb][X] > FCAP) forces_serial(t, pb, cb, X, t.st);
+}
+
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other) {
-  if (max_memb > 0) {
-    dim3 g((max_memb + 255) / 256, ntis);
-    EPI_LAUNCH(K_FORCES_MEMB, k_forces_memb, g, dim3(256), 0, s, dT, pb, cb);
-  }
-  if (max_other > 0) {
-    dim3 g((max_other + 255) / 256, ntis);
-    EPI_LAUNCH(K_FORCES_OTHER, k_forces_other, g, dim3(256), 0, s, dT, pb, cb);
-  }
+  (void)max_memb;
+  dim3 g((max_other + F_GROUPS - 1) / F_GROUPS, ntis);
+  EPI_LAUNCH(K_FORCES, k_forces, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
+  dim3 gl((max_other + 255) / 256, ntis);
+  EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb);
 }
 
 }  // namespace epi

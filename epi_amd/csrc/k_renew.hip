@@ -65,39 +65,74 @@ __device__ __forceinline__ bool to_remove(const Tis& t, int cb, int c) {
          (unsigned)t.nlen[cb][c] <= DISA_conn_num_thresh;
 }
 
+// Flags of every non-membrane cell (bit 0: needs a draw, bit 1: removal)
+// and, per 1024-cell chunk, whether anything needs the ordered walk.
+constexpr int RCHUNK = 1024;
+
+__global__ void __launch_bounds__(256) k_renew_flags(const Tis* __restrict__ T, int cb) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  const int start = t.nmemb + t.nder;
+  const int n = st->n;
+  // blocks are 256 cells and chunks 1024, so a wave never spans two chunks
+  for (int c = start + blockIdx.x * blockDim.x + threadIdx.x; c - (int)(threadIdx.x & 63) < n;
+       c += gridDim.x * blockDim.x) {
+    uint8_t f = 0;
+    if (c < n) {
+      f = (needs_draw(t, c) ? 1 : 0) | (to_remove(t, cb, c) ? 2 : 0);
+      t.rflag[c] = f;
+    }
+    const unsigned long long any = __ballot(f != 0);
+    if (any && (threadIdx.x & 63) == 0) atomicOr(&t.chunk_any[(c - start) / RCHUNK], 1);
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, int pb, int cb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
-  __shared__ int s_ready[1024];
+  __shared__ int s_ready[RCHUNK];
+  __shared__ double s_thr[RCHUNK], s_rhs[RCHUNK];
+  __shared__ double s_rng[2 * RCHUNK];
   __shared__ int s_wcnt[16], s_wrm[16];
-  __shared__ int s_nready, s_nrm_base;
+  __shared__ int s_nrm_base, s_ndiv;
   __shared__ long long s_cursor;
-  __shared__ int s_ndiv;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int start = t.nmemb + t.nder;
   const int n_old = st->n;
   if (tid == 0) { s_cursor = st->rng_pos; s_ndiv = 0; s_nrm_base = 0; st->n_old = n_old; }
   __syncthreads();
-  for (int base = start; base < n_old; base += 1024) {
+  for (int base = start; base < n_old; base += RCHUNK) {
+    const int chunk = (base - start) / RCHUNK;
+    if (!t.chunk_any[chunk]) continue;           // uniform for the block
     const int c = base + tid;
-    const bool valid = c < n_old;
-    const bool rd = valid && needs_draw(t, c);
-    const bool rm = valid && to_remove(t, cb, c);
+    const uint8_t f = c < n_old ? t.rflag[c] : 0;
+    const bool rd = f & 1, rm = f & 2;
     const unsigned long long mr = __ballot(rd), mm = __ballot(rm);
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     if (lane == 0) { s_wcnt[wv] = __popcll(mr); s_wrm[wv] = __popcll(mm); }
+    // replay window for this chunk's decisions
+    const long long cur0 = s_cursor;
+    for (int i = tid; i < 2 * RCHUNK; i += RCHUNK) {
+      const long long q = cur0 + i;
+      s_rng[i] = q < t.rng_n ? t.rng[q] : 2.0;
+    }
     __syncthreads();
     int off_r = 0, off_m = 0, tot_r = 0, tot_m = 0;
     for (int w = 0; w < 16; w++) {
       if (w < wv) { off_r += s_wcnt[w]; off_m += s_wrm[w]; }
       tot_r += s_wcnt[w]; tot_m += s_wrm[w];
     }
-    if (rd) s_ready[off_r + __popcll(mr & below)] = c;
+    if (rd) {
+      const int k = off_r + __popcll(mr & below);
+      s_ready[k] = c;
+      s_thr[k] = t.div_age_thresh[c] * stoch_div_time_ratio;
+      s_rhs[k] = stoch_corr_coef * DT_Cell * weighted_eps_kb(t.is_malignant[c]) * S2;
+    }
     if (rm) t.rm_list[s_nrm_base + off_m + __popcll(mm & below)] = c;
     __syncthreads();
     if (wv == 0) {
       // decide divisions in index order, 64 ready cells per round
-      long long cur = s_cursor;
+      long long cur = cur0;
       int ndiv = s_ndiv;
       int i = 0;
       while (i < tot_r) {
@@ -110,9 +145,9 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
           if (dpos >= t.rng_n) {
             set_err(st, ERR_RNG_EXHAUSTED);
           } else {
-            const double u = t.rng[dpos];
-            dv = u * (t.div_age_thresh[cell] * stoch_div_time_ratio) <=
-                 stoch_corr_coef * DT_Cell * weighted_eps_kb(t.is_malignant[cell]) * S2;
+            const long long wi = dpos - cur0;
+            const double u = wi < 2 * RCHUNK ? s_rng[wi] : t.rng[dpos];
+            dv = u * s_thr[idx] <= s_rhs[idx];
           }
         }
         const unsigned long long md = __ballot(dv);
@@ -122,10 +157,10 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
           cur += k;
           i += k;
         } else {
-          const int f = __ffsll((long long)md) - 1;
-          if (act && lane < f) t.div_slot[cell] = -1;
+          const int f0 = __ffsll((long long)md) - 1;
+          if (act && lane < f0) t.div_slot[cell] = -1;
           int att = 0;
-          if (lane == f) {
+          if (lane == f0) {
             // div_direction (:192-217) with draws after the decision draw
             const int der = t.dermis[cell];
             const P3 me = ld_pos(t, pb, cell), dm = ld_pos(t, pb, der);
@@ -135,7 +170,7 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
             const double norm = sqrt(nvx * nvx + nvy * nvy + nvz * nvz);
             const double nx = nvx / norm, ny = nvy / norm, nz = nvz / norm;
             double ox, oy, oz, sum;
-            long long dp = cur + f + 1;
+            long long dp = cur + f0 + 1;
             do {
               if (dp + 1 >= t.rng_n) { set_err(st, ERR_RNG_EXHAUSTED); ox = 1; oy = 0; oz = 0; att++; break; }
               const double rand_theta = PI_D * t.rng[dp];
@@ -160,15 +195,16 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
             t.div_dir[3 * ndiv + 1] = oy / sum;
             t.div_dir[3 * ndiv + 2] = oz / sum;
           }
-          att = __shfl(att, f, 64);
+          att = __shfl(att, f0, 64);
           ndiv++;
-          cur += f + 1 + 2 * att;
-          i += f + 1;
+          cur += f0 + 1 + 2 * att;
+          i += f0 + 1;
         }
       }
       if (lane == 0) { s_cursor = cur; s_ndiv = ndiv; s_nrm_base += tot_m; }
     }
     __syncthreads();
+    if (tid == 0) t.chunk_any[chunk] = 0;
   }
   if (tid == 0) {
     st->rng_pos = s_cursor;
@@ -388,8 +424,9 @@ __global__ void __launch_bounds__(256) k_periodic_fix(const Tis* __restrict__ T,
 }
 
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n) {
-  EPI_LAUNCH(K_RENEW_SCAN, k_renew_scan, dim3(1, ntis), dim3(1024), 0, s, dT, pb, cb);
   dim3 g((max_n + 255) / 256, ntis);
+  EPI_LAUNCH(K_RENEW_FLAGS, k_renew_flags, g, dim3(256), 0, s, dT, cb);
+  EPI_LAUNCH(K_RENEW_SCAN, k_renew_scan, dim3(1, ntis), dim3(1024), 0, s, dT, pb, cb);
   EPI_LAUNCH(K_RENEW_APPLY, k_renew_apply, g, dim3(256), 0, s, dT, pb, cb, cpar);
   EPI_LAUNCH(K_REMOVE, k_remove, dim3(1, ntis), dim3(256), 0, s, dT, pb);
   EPI_LAUNCH(K_PAIR_DISPERSE, k_pair_disperse, g, dim3(256), 0, s, dT, pb);

@@ -7,8 +7,8 @@ namespace epi {
 struct Tis;
 
 enum KernelId {
-  K_FORCES_MEMB, K_FORCES_OTHER, K_RENEW_SCAN, K_RENEW_APPLY, K_REMOVE, K_PAIR_DISPERSE,
-  K_INIT_SC, K_PERIODIC_FIX, K_BUCKET_CLEAR, K_BUCKET_INSERT, K_BUCKET_SORT, K_SWEEP,
+  K_FORCES, K_FORCES_LONG, K_RENEW_FLAGS, K_RENEW_SCAN, K_RENEW_APPLY, K_REMOVE, K_PAIR_DISPERSE,
+  K_INIT_SC, K_PERIODIC_FIX, K_BUCKET_INSERT, K_BUCKET_SORT, K_SWEEP,
   K_FINISH, K_SNAPSHOT, K_ZZ_SET, K_ZZMAX, K_MAP_CLEAR, K_MAP_STAMP, K_MAP_HALO,
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET, K_COUNT
 };
@@ -32,7 +32,7 @@ void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int m
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n);
 void launch_initialize_sc(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int pb, int max_n);
 void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
-void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets);
+void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix);
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v);
 void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo);
