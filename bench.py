@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes instead of running the autotuner")
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel with events and print a breakdown to stderr")
     return ap.parse_args()
@@ -116,9 +118,22 @@ def main():
     spec = make_spec(args.config)
     seed = spec["seed"]
     geo = spec["geometry"]
+    draws = native.mt19937_64_uniform(seed, n_draws_needed(spec, args.steps + args.warmup))
+    tuning, tune_s = None, 0.0
+    if not args.no_tune:
+        # launch shapes are chosen on a throwaway copy of the workload
+        from epi_amd import autotune
+        tt = native.TissueSet([geo])
+        tt.load(0, spec)
+        tt.set_rng(0, draws)
+        tuning, rep = autotune.tune(tt, [native.Driver(0, 0, 0, 0, 0)], log=sys.stderr)
+        tune_s = rep["seconds"]
+        tt.close()
     ts = native.TissueSet([geo])
     ts.load(0, spec)
-    ts.set_rng(0, native.mt19937_64_uniform(seed, n_draws_needed(spec, args.steps + args.warmup)))
+    ts.set_rng(0, draws)
+    if tuning:
+        ts.set_tuning(tuning)
     drv = [native.Driver(0, 0, 0, args.ca_every, 0)]
     if args.warmup:
         ts.step(drv, args.warmup)
@@ -183,7 +198,8 @@ def main():
         "config": {"workload": f"plain FP64 {args.config} tissue ({geo.nmx}x{geo.nmy} sheet, "
                                f"{geo.nx}x{geo.ny}x{geo.nz} grid, seed {seed}), Ca every {args.ca_every} + sw gating",
                    "cells_start": int(spec["cells"]["x"].size), "cells_end": int(s1.n),
-                   "ca_solves": int(ca_solves)},
+                   "ca_solves": int(ca_solves), "launch_shapes": ts.get_tuning(),
+                   "autotune_seconds": round(tune_s, 2)},
         "grid_cell_steps_per_s": round(grid_pts / wall, 1),
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -1,0 +1,89 @@
+"""Launch-shape autotuner for the grid kernels of the epi5_MT hot path.
+
+The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
+
+  k_ext_stim   z planes per thread (1, 2, 4, 8) x y rows per block (1, 2, 4)
+  k_atp        the same
+  k_ca_cells   lanes per cell (4, 8, 16)
+
+A launch shape changes timing only: every grid point is computed on its own
+and the Ca cell kernel sums each list in list order whatever its group size,
+so the tuner can time candidates on a throwaway copy of the workload (which
+it advances) and hand the winners to the real run.  Each candidate is timed
+with HIP events on the library's stream (TissueSet.profile).
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import time
+
+STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8), (1, 2, 4)))
+CA_GROUPS = (4, 8, 16)
+DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16}
+
+
+def _avg_us(ts, kernel):
+    prof = ts.profile()
+    if kernel not in prof:
+        return float("inf")
+    ms, cnt = prof[kernel]
+    return 1e3 * ms / cnt
+
+
+def _time(ts, kernel, run):
+    ts.synchronize()
+    ts.profile_reset()
+    ts.profile_enable([kernel])
+    run()
+    ts.synchronize()
+    us = _avg_us(ts, kernel)
+    ts.profile_enable([])
+    return us
+
+
+def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
+    """Choose launch shapes on `ts`, a loaded TissueSet that may be advanced.
+
+    Returns (tuning dict, report dict with every candidate's mean launch time
+    in microseconds)."""
+    t0 = time.perf_counter()
+    if warm_steps:
+        ts.step(drivers, warm_steps)
+    zz = ts.calc_zzmax()
+    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}}
+    best = dict(DEFAULT)
+    ts.set_tuning(best)
+
+    def run_b():
+        for _ in range(b_reps):
+            ts.calc_ext_stim(zz)
+
+    for zb, rows in STENCIL_SHAPES:
+        ts.set_tuning({"b_zb": zb, "b_rows": rows})
+        ts.calc_ext_stim(zz)
+        report["k_ext_stim"][f"zb{zb}_rows{rows}"] = _time(ts, "k_ext_stim", run_b)
+    key = min(report["k_ext_stim"], key=report["k_ext_stim"].get)
+    best["b_zb"], best["b_rows"] = (int(v) for v in key[2:].split("_rows"))
+    ts.set_tuning(best)
+
+    def run_ca():
+        ts.calc_ca2p_n(zz, ca_sub)
+
+    for zb, rows in STENCIL_SHAPES:
+        ts.set_tuning({"atp_zb": zb, "atp_rows": rows})
+        report["k_atp"][f"zb{zb}_rows{rows}"] = _time(ts, "k_atp", run_ca)
+    key = min(report["k_atp"], key=report["k_atp"].get)
+    best["atp_zb"], best["atp_rows"] = (int(v) for v in key[2:].split("_rows"))
+    ts.set_tuning(best)
+
+    for g in CA_GROUPS:
+        ts.set_tuning({"ca_group": g})
+        report["k_ca_cells"][f"group{g}"] = _time(ts, "k_ca_cells", run_ca)
+    key = min(report["k_ca_cells"], key=report["k_ca_cells"].get)
+    best["ca_group"] = int(key[5:])
+    ts.set_tuning(best)
+    report["seconds"] = time.perf_counter() - t0
+    if log is not None:
+        print("autotune " + json.dumps({"choice": best, "us": report}), file=log, flush=True)
+    return best, report

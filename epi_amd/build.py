@@ -45,7 +45,24 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_driver()
     return OUT
+
+
+DRIVER_SRC = os.path.join(HERE, "host", "proc.cpp")
+DRIVER = os.path.join(HERE, "host", "epi_proc")
+
+
+def build_driver():
+    """The C++ driver of proc.cpp's loop over the facade (host code only)."""
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
+           f"-I{os.path.join(os.path.dirname(HERE), 'include')}", DRIVER_SRC, "-o", DRIVER + ".tmp",
+           f"-L{HERE}", "-l:libepi_amd.so", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -94,6 +94,8 @@ struct Tis {
   double *radius, *div_age_thresh;
   double* ca2p[2]; double* ip3[2];
   double *ca2p_avg, *ex_inert, *agek, *ageb, *ex_fat, *in_fat, *spr_nat_len, *diff_u;
+  double *ca_th, *ca_kpa, *ca_iag;   // per-solve constants of the Ca step (agek and state are frozen)
+  uint8_t* ca_kind;                  // per list entry during a solve: 0 skip, 1 coupled, 3 coupled ALIVE
   long long* uid; long long* old_uid; int* src_row;
   // neighbour rows (two buffers: current and previous build)
   int* nlen[2]; int* col[2]; double* gj[2];

@@ -109,6 +109,7 @@ struct epi_ctx {
   hipStream_t stream = nullptr;
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
+  Tuning tu;
   std::vector<long long> ca_solves;
   // throughput accounting of epi_step
   std::vector<long long> steps, cell_steps, grid_b, grid_ca;
@@ -167,6 +168,10 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.in_fat = cv.take<double>(cap);
   t.spr_nat_len = cv.take<double>(cap);
   t.diff_u = cv.take<double>(cap);
+  t.ca_th = cv.take<double>(cap);
+  t.ca_kpa = cv.take<double>(cap);
+  t.ca_iag = cv.take<double>(cap);
+  t.ca_kind = cv.take<uint8_t>(cap * ns);
   t.uid = cv.take<long long>(cap);
   t.old_uid = cv.take<long long>(cap);
   t.src_row = cv.take<int>(cap);
@@ -320,14 +325,15 @@ int set_zz(epi_ctx* c, const double* zzmax) {
 }
 
 // calc_ca2p (ca2p.cpp:380-402) for the tissues in tlist (nullptr = all)
-int run_ca(epi_ctx* c, const int* d_tlist, int ntis) {
+int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
   const int mn = max_n(c);
   const int izb = izb_est(c, true);
   long long box = 1;
   for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * std::min(izb, h.NZ + 1));
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
-  for (int s = 0; s < CA_ITR; s++) {
-    launch_ca_substep(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c), izb);
+  for (int s = 0; s < n_sub; s++) {
+    launch_ca_substep(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c), izb,
+                      c->tu);
     c->cp ^= 1;
     c->ap ^= 1;
   }
@@ -715,7 +721,7 @@ int epi_setup_map_lat(epi_ctx* c) {
 int epi_calc_ext_stim(epi_ctx* c, const double* zzmax) {
   if (!c) return EPI_ERR_INVALID;
   RCHK(set_zz(c, zzmax));
-  launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false));
+  launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
   HIPCHK(c, hipGetLastError());
   c->bp ^= 1;
   return EPI_OK;
@@ -733,6 +739,13 @@ int epi_calc_ca2p(epi_ctx* c, const double* zzmax) {
   if (!c) return EPI_ERR_INVALID;
   RCHK(set_zz(c, zzmax));
   RCHK(run_ca(c, nullptr, c->T));
+  return sync_status(c);
+}
+
+int epi_calc_ca2p_n(epi_ctx* c, const double* zzmax, int32_t n_sub) {
+  if (!c || n_sub < 0) return EPI_ERR_INVALID;
+  RCHK(set_zz(c, zzmax));
+  RCHK(run_ca(c, nullptr, c->T, n_sub));
   return sync_status(c);
 }
 
@@ -767,7 +780,7 @@ int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
   for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * (iz_bound + 1));
   launch_ca_touch(c->stream, c->d_tis, nullptr, c->T, c->cb, mn, box);
   launch_ca_substep(c->stream, c->d_tis, nullptr, c->T, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c),
-                    iz_bound + 1);
+                    iz_bound + 1, c->tu);
   c->cp ^= 1;
   c->ap ^= 1;
   launch_ca_diffu_zero(c->stream, c->d_tis, nullptr, c->T, mn);
@@ -790,7 +803,7 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     c->cb = nb;
     // proc.cpp:155-157
     launch_maps(c->stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c));
-    launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false));
+    launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
     c->bp ^= 1;
     HIPCHK(c, hipGetLastError());
     RCHK(sync_status(c));
@@ -889,6 +902,42 @@ int epi_profile_reset(epi_ctx* c) {
 }
 
 int32_t epi_profile_kernel_count(void) { return K_COUNT; }
+
+namespace {
+int* tuning_slot(Tuning& tu, int key) {
+  switch (key) {
+    case EPI_TUNE_B_ZB: return &tu.b_zb;
+    case EPI_TUNE_B_ROWS: return &tu.b_rows;
+    case EPI_TUNE_ATP_ZB: return &tu.a_zb;
+    case EPI_TUNE_ATP_ROWS: return &tu.a_rows;
+    case EPI_TUNE_CA_GROUP: return &tu.ca_group;
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
+  if (!c) return EPI_ERR_INVALID;
+  int* slot = tuning_slot(c->tu, key);
+  if (!slot) return fail(c, EPI_ERR_INVALID, "unknown tuning key");
+  bool ok = false;
+  switch (key) {
+    case EPI_TUNE_B_ZB: case EPI_TUNE_ATP_ZB: ok = value == 1 || value == 2 || value == 4 || value == 8; break;
+    case EPI_TUNE_B_ROWS: case EPI_TUNE_ATP_ROWS: ok = value == 1 || value == 2 || value == 4; break;
+    case EPI_TUNE_CA_GROUP: ok = value == 4 || value == 8 || value == 16; break;
+  }
+  if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
+  *slot = value;
+  return EPI_OK;
+}
+
+int epi_get_tuning(epi_ctx* c, int32_t key, int32_t* value) {
+  if (!c || !value) return EPI_ERR_INVALID;
+  int* slot = tuning_slot(c->tu, key);
+  if (!slot) return fail(c, EPI_ERR_INVALID, "unknown tuning key");
+  *value = *slot;
+  return EPI_OK;
+}
 
 int epi_get_cells(epi_ctx* c, int32_t t, epi_cells* o) {
   RCHK(check_tissue(c, t));

@@ -78,10 +78,13 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
 // connect_proc (:70-129): the 5x5x5 sweep, periodic in x, y and z, buckets
 // in ascending (j, k, l) with cells in ascending index inside a bucket.
 // A group of 32 lanes takes one cell; lane g < 25 sweeps the column
-// (j, k) = (g / 5, g % 5) over its 5 z buckets.  Lanes count their partners,
-// an exclusive scan over the group gives each lane its place in the row, and
-// a second pass writes the entries -- the row comes out in sweep order.
+// (j, k) = (g / 5, g % 5) over its 5 z buckets.  Lanes count their partners
+// (keeping the first SWEEP_H of them in LDS), an exclusive scan over the
+// group gives each lane its place in the row, and the lane writes its
+// partners there -- the row comes out in sweep order.  A lane with more than
+// SWEEP_H partners sweeps its column a second time.
 constexpr int SWEEP_G = 32;
+constexpr int SWEEP_H = 8;
 
 __device__ __forceinline__ bool sweep_hit(const Tis& t, int pb, const P3& pc, double rc, int o) {
   const P3 po = ld_pos(t, pb, o);
@@ -99,6 +102,8 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
   const int ANX = t.ANX, ANY = t.ANY, ANZ = t.ANZ;
   const int lane = threadIdx.x & (SWEEP_G - 1);
   const int groups = gridDim.x * (blockDim.x / SWEEP_G);
+  __shared__ int s_hit[256][SWEEP_H];
+  int* const hits = s_hit[threadIdx.x];
   int* const col = t.col[nb];
   for (int c = t.nmemb + blockIdx.x * (blockDim.x / SWEEP_G) + threadIdx.x / SWEEP_G; c < n; c += groups) {
     const P3 pc = ld_pos(t, pb, c);
@@ -117,14 +122,19 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
     const long bxy = (aix * ANY + aiy) * ANZ;
     int cnt = 0;
     if (act) {
-      for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
-        const long b = bxy + l % ANZ;
-        const int sz = min(t.bucket_n[b], t.bcap);
-        const int* cells = t.bucket_cells + (size_t)b * t.bcap;
-        for (int m = 0; m < sz; ++m) {
+      int sz[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) sz[i] = min(t.bucket_n[bxy + (anz - 2 + i + ANZ) % ANZ], t.bcap);
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        const int* cells = t.bucket_cells + (size_t)(bxy + (anz - 2 + i + ANZ) % ANZ) * t.bcap;
+        for (int m = 0; m < sz[i]; ++m) {
           const int o = cells[m];
           if (c <= o) break;   // ascending within the bucket
-          cnt += sweep_hit(t, pb, pc, rc, o);
+          if (sweep_hit(t, pb, pc, rc, o)) {
+            if (cnt < SWEEP_H) hits[cnt] = o;
+            cnt++;
+          }
         }
       }
     }
@@ -132,18 +142,27 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
     int off = group_exclusive_scan<SWEEP_G>(cnt, lane, &total);
     if (cnt) {
       int* const row = col + (size_t)c * t.ns;
-      for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
-        const long b = bxy + l % ANZ;
-        const int sz = min(t.bucket_n[b], t.bcap);
-        const int* cells = t.bucket_cells + (size_t)b * t.bcap;
-        for (int m = 0; m < sz; ++m) {
-          const int o = cells[m];
-          if (c <= o) break;
-          if (sweep_hit(t, pb, pc, rc, o)) {
-            if (off < t.ns) row[off] = o;
-            off++;
-            const int slot = atomicAdd(&t.bcnt[o], 1);
-            if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
+      if (cnt <= SWEEP_H) {
+        for (int i = 0; i < cnt; i++, off++) {
+          const int o = hits[i];
+          if (off < t.ns) row[off] = o;
+          const int slot = atomicAdd(&t.bcnt[o], 1);
+          if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
+        }
+      } else {
+        for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
+          const long b = bxy + l % ANZ;
+          const int sz = min(t.bucket_n[b], t.bcap);
+          const int* cells = t.bucket_cells + (size_t)b * t.bcap;
+          for (int m = 0; m < sz; ++m) {
+            const int o = cells[m];
+            if (c <= o) break;
+            if (sweep_hit(t, pb, pc, rc, o)) {
+              if (off < t.ns) row[off] = o;
+              off++;
+              const int slot = atomicAdd(&t.bcnt[o], 1);
+              if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
+            }
           }
         }
       }
@@ -153,11 +172,16 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
 }
 
 // Row assembly, gj carry-over and dermis.  A group of KG lanes owns one
-// cell: the upper partners are placed by rank (they are distinct), the
-// previous row's uids are staged in LDS and each lane resolves the gap
-// junction of its entries there (gj_refresh :133-152: a directed edge keeps
-// its value while it stays in the list, a new edge starts at gj_init), and
-// the leader picks the dermis in list order (find_dermis :158-172).
+// cell.  Rows of at most KCAP entries are assembled in LDS: the lower part
+// (written by k_sweep in sweep order) and the upper partners (gathered in
+// k_sweep in arbitrary order, placed here by rank since they are distinct).
+// The previous row's uids are staged in LDS too and each lane resolves the
+// gap junction of its entries there (gj_refresh :133-152: a directed edge
+// keeps its value while it stays in the list, a new edge starts at gj_init).
+// The dermis (find_dermis :158-172: the first MEMB entry of minimal distance
+// in list order, strictly below LX^2) is a lexicographic (distance, entry)
+// minimum over the group.  Longer rows take the same steps through global
+// memory.
 constexpr int KG = 16;
 constexpr int KCAP = 128;
 constexpr int K_BLOCK = 128;
@@ -168,8 +192,9 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
   Status* st = t.st;
   const int n = st->n;
   const int ob = 1 - nb;
+  __shared__ int s_row[K_GROUPS][KCAP];
+  __shared__ int s_bt[K_GROUPS][KCAP];
   __shared__ long long s_ou[K_GROUPS][KCAP];
-  __shared__ double s_d[K_GROUPS][KCAP];
   const int lane = threadIdx.x & (KG - 1);
   const int grp = threadIdx.x / KG;
   int mylen = 0;
@@ -180,6 +205,7 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
     const int base = memb ? MEMB_ADJ : 0;
     const int na = memb ? 0 : t.nlo[c];
     const int nbh = t.bcnt[c];
+    const int src = t.src_row[c];
     const int len = base + na + nbh;
     // the reference asserts a list shorter than N2 = 400 (:122)
     if (len >= MAX_CONN || len > t.ns) {
@@ -187,30 +213,41 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
       continue;
     }
     mylen = max(mylen, len);
-    int* const row = t.col[nb] + (size_t)c * t.ns;
-    if (memb && lane < MEMB_ADJ) row[lane] = t.memb6[6 * c + lane];
-    // upper partners, ascending, by rank
-    const int* bt = t.btmp + (size_t)c * t.ns;
-    for (int i = lane; i < nbh; i += KG) {
-      const int v = bt[i];
-      int r = 0;
-      for (int k = 0; k < nbh; k++) r += bt[k] < v;
-      row[base + na + r] = v;
-    }
-    if (lane == 0) t.nlen[nb][c] = len;
-    // previous row of this cell
-    const int src = t.src_row[c];
     const int olen = src >= 0 ? t.nlen[ob][src] : 0;
     const int* orow = src >= 0 ? t.col[ob] + (size_t)src * t.ns : nullptr;
     const double* og = src >= 0 ? t.gj[ob] + (size_t)src * t.ns : nullptr;
+    int* const row = t.col[nb] + (size_t)c * t.ns;
+    const int* bt = t.btmp + (size_t)c * t.ns;
+    const bool fits = len <= KCAP;
     const bool staged = olen <= KCAP;
+    int* const r = fits ? s_row[grp] : row;
+    if (lane == 0) t.nlen[nb][c] = len;
+    if (memb && lane < MEMB_ADJ) r[lane] = t.memb6[6 * c + lane];
+    if (fits) {
+      for (int i = lane; i < na; i += KG) r[i] = row[i];
+      for (int i = lane; i < nbh; i += KG) s_bt[grp][i] = bt[i];
+    }
     if (staged)
       for (int f = lane; f < olen; f += KG) s_ou[grp][f] = t.old_uid[orow[f]];
-    __threadfence_block();
     wave_sync();
+    const int* const b = fits ? s_bt[grp] : bt;
+    for (int i = lane; i < nbh; i += KG) {
+      const int v = b[i];
+      int rk = 0;
+      for (int k = 0; k < nbh; k++) rk += b[k] < v;
+      r[base + na + rk] = v;
+    }
+    wave_sync();
+    const uint8_t s = t.state[c];
+    const bool want_dermis = !memb && c >= t.nmemb + t.nder && (s == FIX || s == MUSUME);
+    const P3 pc = ld_pos(t, pb, c);
+    double bd = t.LX * t.LX;
+    int be = 0x7fffffff;
     double* const g = t.gj[nb] + (size_t)c * t.ns;
     for (int e = lane; e < len; e += KG) {
-      const long long u = t.uid[row[e]];
+      const int o = r[e];
+      if (fits && (e < base || e >= base + na)) row[e] = o;
+      const long long u = t.uid[o];
       double v = gj_init;
       for (int f = 0; f < olen; f++) {
         int q = e + f;
@@ -219,39 +256,27 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
         if (w == u) { v = og[q]; break; }
       }
       g[e] = v;
+      if (want_dermis && t.state[o] == MEMB) {
+        const double d = dist_sq(t, pc, ld_pos(t, pb, o));
+        if (d < bd) { bd = d; be = e; }   // entries of a lane ascend
+      }
     }
-    // set_dermis / find_dermis (:158-185)
-    const uint8_t s = t.state[c];
-    if (!memb && c >= t.nmemb + t.nder && (s == FIX || s == MUSUME)) {
-      const P3 pc = ld_pos(t, pb, c);
-      const bool fits = len <= KCAP;
-      for (int e = lane; e < len && fits; e += KG) {
-        const int o = row[e];
-        s_d[grp][e] = t.state[o] == MEMB ? dist_sq(t, pc, ld_pos(t, pb, o)) : -1.0;
+    if (want_dermis) {
+#pragma unroll
+      for (int o = 1; o < KG; o <<= 1) {
+        const double d2 = __shfl_xor(bd, o, KG);
+        const int e2 = __shfl_xor(be, o, KG);
+        if (d2 < bd || (d2 == bd && e2 < be)) { bd = d2; be = e2; }
       }
-      wave_sync();
-      if (lane == 0) {
-        double d1Sq = t.LX * t.LX;
-        int dermis = -1;
-        for (int e = 0; e < len; e++) {
-          const int o = row[e];
-          double distanceSq;
-          if (fits) {
-            distanceSq = s_d[grp][e];
-            if (distanceSq < 0) continue;
-          } else {
-            if (t.state[o] != MEMB) continue;
-            distanceSq = dist_sq(t, pc, ld_pos(t, pb, o));
-          }
-          if (distanceSq < d1Sq) { d1Sq = distanceSq; dermis = o; }
-        }
-        t.dermis[c] = dermis;
-      }
+      if (lane == 0) t.dermis[c] = be == 0x7fffffff ? -1 : r[be];
     }
     wave_sync();
   }
+  // maxlen only grows: a wave that cannot raise it skips the atomic, which
+  // would otherwise serialise every wave of the launch on one address
   for (int o = 32; o > 0; o >>= 1) mylen = max(mylen, __shfl_down(mylen, o, 64));
-  if ((threadIdx.x & 63) == 0 && mylen > 0) atomicMax(&st->maxlen, mylen);
+  if ((threadIdx.x & 63) == 0 && mylen > __hip_atomic_load(&st->maxlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&st->maxlen, mylen);
 }
 
 // after a build: every cell's row is its own, uids snapshot the numbering,
@@ -281,8 +306,11 @@ __global__ void __launch_bounds__(256) k_snapshot(const Tis* __restrict__ T, int
     const double v = __shfl_down(m, o, 64);
     if (m < v) m = v;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0)
-    atomicMax(&st->zzmax_bits, (unsigned long long)__double_as_longlong(m));
+  // the value was cleared by k_finish and only grows during this launch
+  const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+  if ((threadIdx.x & 63) == 0 && m > 0 &&
+      mb > __hip_atomic_load(&st->zzmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&st->zzmax_bits, mb);
 }
 
 // calc_zzmax (proc.cpp:34-43): z values that matter are positive, where the
@@ -303,8 +331,11 @@ __global__ void __launch_bounds__(256) k_zzmax(const Tis* __restrict__ T, int pb
     const double v = __shfl_down(m, o, 64);
     if (m < v) m = v;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0)
-    atomicMax(&st->zzmax_bits, (unsigned long long)__double_as_longlong(m));
+  // the value was cleared by k_finish and only grows during this launch
+  const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+  if ((threadIdx.x & 63) == 0 && m > 0 &&
+      mb > __hip_atomic_load(&st->zzmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&st->zzmax_bits, mb);
 }
 
 __global__ void k_zz_set(const Tis* __restrict__ T, int use_value, double v) {

@@ -3,16 +3,18 @@
 //
 // Grids are z-major with x fastest (x pitch padded to 16 doubles), so a
 // wavefront reads 64 consecutive x points of one (y, z) row.  The stencils
-// map threads to (x, y) columns and march z in chunks of ZB planes; the
-// number of active planes (iz_bound) is derived on the device from zzmax and
-// the z chunks are grid-strided, so no host round trip is needed.
+// map threads to (x, y) columns and march z in chunks of ZB planes (a
+// template parameter; the y rows per block are a launch parameter, both set
+// by the autotuner); the number of active planes (iz_bound) is derived on the
+// device from zzmax and the z chunks are grid-strided, so no host round trip
+// is needed.
+#include <type_traits>
+
 #include "epi_dev.h"
 #include "launch.h"
 #include "epi_group.h"
 
 namespace epi {
-
-constexpr int ZB = 4;   // z planes per thread per chunk
 
 // ---- voxel maps ---------------------------------------------------------
 // Only planes 0..stamp_z can hold stamps (stamp_z never decreases), so only
@@ -37,8 +39,7 @@ __device__ __forceinline__ int lat_wrap(long v, int N, Status* st) {
 
 // set_lattice (:18-22) and the sphere stamping (:39-155); the last writer of
 // the serial loop is the highest cell index, hence atomicMax on cmap1.  One
-// lane per cell: lanes of a wave then stamp different cells, which keeps the
-// atomics of one instruction on different cache lines.
+// lane per cell.
 __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, int pb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
@@ -87,7 +88,8 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
     }
   }
   for (int o = 32; o > 0; o >>= 1) ztop = max(ztop, __shfl_down(ztop, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&st->stamp_z, ztop);
+  if ((threadIdx.x & 63) == 0 && ztop > __hip_atomic_load(&st->stamp_z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(&st->stamp_z, ztop);
 }
 
 // periodic planes x = NX and y = NY for l < NZ (:156-166)
@@ -119,13 +121,16 @@ __device__ __forceinline__ double fB(double age, double B, bool cornif) {   // :
   return (cornif && age > THRESH_DEAD - DUR_ALIVE && age <= THRESH_DEAD + DUR_DEAD ? 1 : 0) - kb * B;
 }
 
-// threads: x fastest (64 per wave), 4 y rows per block; blockIdx.z packs
-// (y block, z chunk); z chunks are grid-strided up to iz_bound
+// threads: x fastest (64 per wave), blockDim.x / 64 y rows per block;
+// blockIdx.z packs (y block, z chunk); z chunks are grid-strided up to
+// iz_bound
+template <int ZB>
 __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, int bp, int nky, int nzc) {
   const Tis& t = T[blockIdx.y];
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const int rows = blockDim.x >> 6;
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int k = (blockIdx.z % nky) * 4 + (threadIdx.x >> 6);
+  const int k = (blockIdx.z % nky) * rows + (threadIdx.x >> 6);
   const int zc = blockIdx.z / nky;
   const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
   const int iz_bound = (int)((zzmax + FAC_MAP * R_max) * t.inv_dz);
@@ -141,27 +146,37 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, int
   const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
   const double DB = 0.0009;
   const double inv_dz = t.inv_dz;
+  const long long cxy = gidx(t, j, k, 0);
   for (int l0 = zc * ZB; l0 < iz_bound; l0 += nzc * ZB) {
     const int l1 = min(l0 + ZB, iz_bound);
+    // the z column of (j, k) rolls through registers; mirror z boundary
+    // prev_z(0) = 1, next_z(NZ - 1) = NZ - 1 (utils.cpp:32-72)
+    int pz = l0 == 0 ? 1 : l0 - 1;
+    double czm = carr[cxy + pz * t.PL];
+    double mzm = (double)cm2[cxy + pz * t.PL];
+    double cc = carr[cxy + l0 * t.PL];
+    double mc = (double)cm2[cxy + l0 * t.PL];
     for (int l = l0; l < l1; l++) {
-      const int prev_z = l == 0 ? 1 : l - 1, next_z = l + 1 == NZ ? NZ - 1 : l + 1;
-      const long long v = gidx(t, j, k, l);
+      const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+      const long long v = cxy + l * t.PL;
+      const long long vzp = cxy + next_z * t.PL;
+      const double czp = carr[vzp];
+      const double mzp = (double)cm2[vzp];
       double dum_age = 0;
       bool flg = false;
-      const int cc = t.cmap1[v];
-      if (cc >= 0) {
-        const uint8_t s = t.state[cc];
-        if (s == DEAD || s == ALIVE) { dum_age = t.agek[cc]; flg = true; }
+      const int cell = t.cmap1[v];
+      if (cell >= 0) {
+        const uint8_t s = t.state[cell];
+        if (s == DEAD || s == ALIVE) { dum_age = t.agek[cell]; flg = true; }
       }
-      const double cext = carr[v];
+      const double cext = cc;
       const long long vxm = gidx(t, prev_x, k, l), vym = gidx(t, j, prev_y, l);
-      const long long vzm = gidx(t, j, k, prev_z), vzp = gidx(t, j, k, next_z);
       const long long vyp = gidx(t, j, next_y, l), vxp = gidx(t, next_x, k, l);
       const double nv = cext + DT_Ca * (DB *
           ((double)cm2[vxm] * (carr[vxm] - cext)
            + (double)cm2[vym] * (carr[vym] - cext)
-           + (double)cm2[vzm] * (carr[vzm] - cext)
-           + (double)cm2[vzp] * (carr[vzp] - cext)
+           + mzm * (czm - cext)
+           + mzp * (czp - cext)
            + (double)cm2[vyp] * (carr[vyp] - cext)
            + (double)cm2[vxp] * (carr[vxp] - cext))
           * inv_dz * inv_dz + fB(dum_age, cext, flg));
@@ -170,6 +185,9 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, int
       if (k == 0) narr[gidx(t, j, NY, l)] = nv;
       if (j == 0) narr[gidx(t, NX, k, l)] = nv;
       if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+      // plane l + 1 sees plane l below it; at l = 0 the plane below 1 is 0
+      czm = cc; mzm = mc;
+      cc = czp; mc = mzp;
     }
   }
 }
@@ -217,7 +235,26 @@ __global__ void __launch_bounds__(256) k_ca_begin(const Tis* __restrict__ T, con
     for (int e = 0; e < len; e++)
       if (t.state[row[e]] == AIR) { air = 1; break; }
     t.touch_air[c] = air;
-    if (zero_avg && c >= start && is_afm(t.state[c])) t.ca2p_avg[c] = 0;
+    const uint8_t s = t.state[c];
+    if (c >= start && (s == DEAD || is_afm(s))) {
+      // which entries take part in the gap-junction sums (supra_calc
+      // :201-209; dead_IP3_calc :160-165 counts the ALIVE ones)
+      uint8_t* kind = t.ca_kind + (size_t)c * t.ns;
+      for (int e = 0; e < len; e++) {
+        const uint8_t so = t.state[row[e]];
+        kind[e] = (so == ALIVE) ? 3 : (so == DEAD || so == FIX || so == MUSUME) ? 1 : 0;
+      }
+    }
+    if (c >= start && is_afm(s)) {
+      if (zero_avg) t.ca2p_avg[c] = 0;
+      // calc_th, calc_Kpa, calc_IAG (ca2p.cpp:90-126): agek and state do
+      // not change during a solve
+      const bool is_alive = s == ALIVE;
+      const double agek = t.agek[c];
+      t.ca_th[c] = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
+      t.ca_kpa[c] = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
+      t.ca_iag[c] = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
+    }
   }
 }
 
@@ -244,73 +281,21 @@ __global__ void __launch_bounds__(256) k_ca_vox(const Tis* __restrict__ T, const
 }
 
 // dead_IP3_calc (:156-173) and supra_calc (:182-222).  A group of CG lanes
-// owns one cell: lanes evaluate the gap-junction terms of the list entries
-// (and update gj toward ALIVE neighbours) and load the 16 grid corners of
-// the two grid_avg8; the leader sums them in list / corner order.
-constexpr int CG = 16;
-constexpr int CCAP = 64;
+// (4, 8 or 16, set by the autotuner) owns one cell: lanes evaluate the
+// gap-junction terms of the list entries (and update gj toward ALIVE
+// neighbours) 4 * CG entries at a time and load the 16 grid corners of the two
+// grid_avg8; the leader sums them in list / corner order.
 constexpr int C_BLOCK = 128;
-constexpr int C_GROUPS = C_BLOCK / CG;
 
 __device__ __forceinline__ double fw_term(double diff, double w) {   // fw (:138-144)
   return (1. - w) + (-1. + tanh((0.1 - diff) / 0.1)) / 2.;
 }
 
 // the serial form of one cell, for lists longer than the LDS slots
-__device__ void ca_cell_serial(const Tis& t, int c, int cb, int cp, int ap, int bp) {
-  const double* ca = t.ca2p[cp];
-  const double* ip3 = t.ip3[cp];
-  const uint8_t s = t.state[c];
-  const int* row = t.col[cb] + (size_t)c * t.ns;
-  const int len = t.nlen[cb][c];
-  if (s == DEAD) {
-    int count = 0;
-    double tmp = 0;
-    const double pc = ip3[c];
-    for (int e = 0; e < len; e++) {
-      const int o = row[e];
-      if (t.state[o] == ALIVE) { tmp += ip3[o]; count++; }
-    }
-    tmp = DT_Ca * (dp_c * (tmp - count * pc) - Kpp * pc);
-    t.ip3[1 - cp][c] = pc + tmp;
-    return;
-  }
-  const double ca2p_du = 0.01;
-  const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
-  const bool is_alive = s == ALIVE;
-  const double agek = t.agek[c];
-  const double _th = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
-  const double _Kpa = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
-  const double IAGv = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
-  const double u = ca[c], p = ip3[c];
-  double tmp_diffu = 0, tmp_IP3 = 0;
-  double* g = t.gj[cb] + (size_t)c * t.ns;
-  for (int e = 0; e < len; e++) {
-    const int o = row[e];
-    const uint8_t so = t.state[o];
-    if (so == ALIVE || so == DEAD || so == FIX || so == MUSUME) {
-      const double gv = g[e];
-      const double uo = ca[o];
-      tmp_diffu += gv * (uo - u);
-      tmp_IP3 += gv * (ip3[o] - p);
-      if (so == ALIVE) g[e] = gv + DT_Ca * fw_term(fabs(uo - u), gv);
-    }
-  }
-  const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
-  const double Aavg = grid_avg8(t, t.atp[ap], ix, iy, iz);
-  const double diff_u = ca2p_reaction_factor(u, t.ex_inert[c], p, Bavg) + ca2p_du * IAGv * tmp_diffu;
-  t.diff_u[c] = diff_u;
-  const double H0 = 0.5;
-  t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
-  const double cs = u + DT_Ca * diff_u;
-  t.ca2p[1 - cp][c] = cs;
-  t.ca2p_avg[c] = t.ca2p_avg[c] + cs;
-  const double para_k2 = 0.7;
-  const double exi = t.ex_inert[c];
-  t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);
-}
-
+template <int CG>
 __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp, int ap, int bp) {
+  constexpr int C_GROUPS = C_BLOCK / CG;
+  constexpr int CCAP = 4 * CG;   // list entries per chunk; LDS per block is the same for every CG
   const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
   const int n = t.st->n;
   __shared__ double s_t1[C_GROUPS][CCAP];
@@ -327,142 +312,152 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
     const bool dead = s == DEAD;
     if (!dead && !is_afm(s)) continue;
     const int len = t.nlen[cb][c];
-    if (len > CCAP) {
-      if (lane == 0) ca_cell_serial(t, c, cb, cp, ap, bp);
-      continue;
-    }
     const int* row = t.col[cb] + (size_t)c * t.ns;
+    const uint8_t* kind = t.ca_kind + (size_t)c * t.ns;
+    const double u = ca[c], p = ip3[c];
     if (dead) {
-      for (int e = lane; e < len; e += CG) {
-        const int o = row[e];
-        const bool al = t.state[o] == ALIVE;
-        s_f[grp][e] = al;
-        s_t1[grp][e] = al ? ip3[o] : 0.0;
+      int count = 0;
+      double tmp = 0;
+      for (int e0 = 0; e0 < len; e0 += CCAP) {
+        const int m = min(CCAP, len - e0);
+        for (int e = lane; e < m; e += CG) {
+          const bool al = kind[e0 + e] == 3;
+          s_f[grp][e] = al;
+          s_t1[grp][e] = al ? ip3[row[e0 + e]] : 0.0;
+        }
+        wave_sync();
+        if (lane == 0)
+          for (int e = 0; e < m; e++)
+            if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
+        wave_sync();
       }
-      wave_sync();
-      if (lane == 0) {
-        int count = 0;
-        double tmp = 0;
-        const double pc = ip3[c];
-        for (int e = 0; e < len; e++)
-          if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
-        tmp = DT_Ca * (dp_c * (tmp - count * pc) - Kpp * pc);
-        t.ip3[1 - cp][c] = pc + tmp;
-      }
-      wave_sync();
+      if (lane == 0) t.ip3[1 - cp][c] = p + DT_Ca * (dp_c * (tmp - count * p) - Kpp * p);
       continue;
     }
-    const double u = ca[c], p = ip3[c];
-    double* g = t.gj[cb] + (size_t)c * t.ns;
-    for (int e = lane; e < len; e += CG) {
-      const int o = row[e];
-      const uint8_t so = t.state[o];
-      uint8_t f = 0;
-      double a1 = 0, a2 = 0;
-      if (so == ALIVE || so == DEAD || so == FIX || so == MUSUME) {
-        const double gv = g[e];
-        const double uo = ca[o];
-        a1 = gv * (uo - u);
-        a2 = gv * (ip3[o] - p);
-        f = 1;
-        if (so == ALIVE) g[e] = gv + DT_Ca * fw_term(fabs(uo - u), gv);
-      }
-      s_f[grp][e] = f;
-      s_t1[grp][e] = a1;
-      s_t2[grp][e] = a2;
+    // the leader's per-cell values are fetched now, beside the lanes' loads
+    double exi = 0, avg = 0, _th = 0, _Kpa = 0, IAGv = 0;
+    if (lane == 0) {
+      exi = t.ex_inert[c]; avg = t.ca2p_avg[c];
+      _th = t.ca_th[c]; _Kpa = t.ca_kpa[c]; IAGv = t.ca_iag[c];
     }
-    {
-      // corners of grid_avg8 (utils.h:18-23): lanes 0-7 B, 8-15 ATP
-      const int k = lane & 7;
+    // corners of grid_avg8 (utils.h:18-23): slots 0-7 B, 8-15 ATP
+    for (int q = lane; q < 16; q += CG) {
+      const int k = q & 7;
       const int ix = t.lat[3 * c] + ((k == 1 || k == 4 || k == 5 || k == 7) ? 1 : 0);
       const int iy = t.lat[3 * c + 1] + ((k == 2 || k == 4 || k == 6 || k == 7) ? 1 : 0);
       const int iz = t.lat[3 * c + 2] + ((k == 3 || k == 5 || k == 6 || k == 7) ? 1 : 0);
-      s_g[grp][lane] = (lane < 8 ? t.bf[bp] : t.atp[ap])[gidx(t, ix, iy, iz)];
+      s_g[grp][q] = (q < 8 ? t.bf[bp] : t.atp[ap])[gidx(t, ix, iy, iz)];
     }
-    wave_sync();
+    double* g = t.gj[cb] + (size_t)c * t.ns;
+    double tmp_diffu = 0, tmp_IP3 = 0;
+    for (int e0 = 0; e0 < len; e0 += CCAP) {
+      const int m = min(CCAP, len - e0);
+      for (int e = lane; e < m; e += CG) {
+        const int o = row[e0 + e];
+        const uint8_t kd = kind[e0 + e];
+        const double gv = g[e0 + e];
+        double a1 = 0, a2 = 0;
+        if (kd) {
+          const double uo = ca[o];
+          a1 = gv * (uo - u);
+          a2 = gv * (ip3[o] - p);
+          if (kd == 3) g[e0 + e] = gv + DT_Ca * fw_term(fabs(uo - u), gv);
+        }
+        s_f[grp][e] = kd;
+        s_t1[grp][e] = a1;
+        s_t2[grp][e] = a2;
+      }
+      wave_sync();
+      if (lane == 0)
+        for (int e = 0; e < m; e++)
+          if (s_f[grp][e]) { tmp_diffu += s_t1[grp][e]; tmp_IP3 += s_t2[grp][e]; }
+      wave_sync();
+    }
     if (lane == 0) {
       const double ca2p_du = 0.01;
-      const bool is_alive = s == ALIVE;
-      const double agek = t.agek[c];
-      const double _th = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
-      const double _Kpa = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
-      const double IAGv = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
-      double tmp_diffu = 0, tmp_IP3 = 0;
-      for (int e = 0; e < len; e++)
-        if (s_f[grp][e]) { tmp_diffu += s_t1[grp][e]; tmp_IP3 += s_t2[grp][e]; }
       const double* q = s_g[grp];
       const double Bavg = 0.125 * (q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7]);
       const double Aavg = 0.125 * (q[8] + q[9] + q[10] + q[11] + q[12] + q[13] + q[14] + q[15]);
-      const double diff_u = ca2p_reaction_factor(u, t.ex_inert[c], p, Bavg) + ca2p_du * IAGv * tmp_diffu;
+      const double diff_u = ca2p_reaction_factor(u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
       t.diff_u[c] = diff_u;
       const double H0 = 0.5;
       t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
       const double cs = u + DT_Ca * diff_u;
       t.ca2p[1 - cp][c] = cs;
-      t.ca2p_avg[c] = t.ca2p_avg[c] + cs;
+      t.ca2p_avg[c] = avg + cs;
       const double para_k2 = 0.7;
-      const double exi = t.ex_inert[c];
       t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);
     }
     wave_sync();
   }
 }
 
-// ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound
+// ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound.
+// Threads cover [0, NX) x [0, NY); the z column rolls through registers.
+template <int ZB>
 __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int nky, int nzc) {
   const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const int rows = blockDim.x >> 6;
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int k = (blockIdx.z % nky) * 4 + (threadIdx.x >> 6);
+  const int k = (blockIdx.z % nky) * rows + (threadIdx.x >> 6);
   const int zc = blockIdx.z / nky;
   const int izb = ca_iz_bound(t);
   if (izb > NZ || izb < 0) return;
-  if (j > NX || k > NY) return;
+  if (j >= NX || k >= NY) return;
   const double* __restrict__ carr = t.atp[ap];
   double* __restrict__ narr = t.atp[1 - ap];
   const double Da = 1.0, AIR_STIM = 0.1, STIM11 = 0.002, Kaa = 0.5;
   const double inv_dx = t.inv_dx;
-  if (j < NX && k < NY) {
-    const uint8_t* __restrict__ cm2 = t.cmap2;
-    const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
-    const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
-    for (int l0 = zc * ZB; l0 < izb; l0 += nzc * ZB) {
-      const int l1 = min(l0 + ZB, izb);
-      for (int l = l0; l < l1; l++) {
-        const int prev_z = l == 0 ? 1 : l - 1, next_z = l + 1 == NZ ? NZ - 1 : l + 1;
-        const long long v = gidx(t, j, k, l);
-        const double catp = carr[v];
-        const unsigned w = t.vox[v];
-        const unsigned cell1 = w & 0x7fffffffu;
-        const double diffu = cell1 ? t.diff_u[cell1 - 1] : 0.0;
-        const double air = (double)(w >> 31);
-        const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
-        const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
-        const long long vzm = gidx(t, j, k, prev_z), vzp = gidx(t, j, k, next_z);
-        const double fa = STIM11 * min0(diffu) - catp * Kaa;
-        const double nv = catp + DT_Ca * (Da * ((double)cm2[vxm] * (carr[vxm] - catp)
-                                                + (double)cm2[vxp] * (carr[vxp] - catp)
-                                                + (double)cm2[vym] * (carr[vym] - catp)
-                                                + (double)cm2[vyp] * (carr[vyp] - catp)
-                                                + (double)cm2[vzm] * (carr[vzm] - catp)
-                                                + (double)cm2[vzp] * (carr[vzp] - catp))
-                                          * inv_dx * inv_dx + fa + air * AIR_STIM);
-        narr[v] = nv;
-        if (k == 0) narr[gidx(t, j, NY, l)] = nv;
-        if (j == 0) narr[gidx(t, NX, k, l)] = nv;
-        if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
-      }
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  const long long cxy = gidx(t, j, k, 0);
+  for (int l0 = zc * ZB; l0 < izb; l0 += nzc * ZB) {
+    const int l1 = min(l0 + ZB, izb);
+    const int pz = l0 == 0 ? 1 : l0 - 1;   // mirror z boundary (utils.cpp:32-72)
+    double czm = carr[cxy + pz * t.PL];
+    double mzm = (double)cm2[cxy + pz * t.PL];
+    double cc = carr[cxy + l0 * t.PL];
+    double mc = (double)cm2[cxy + l0 * t.PL];
+    for (int l = l0; l < l1; l++) {
+      const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+      const long long v = cxy + l * t.PL;
+      const long long vzp = cxy + next_z * t.PL;
+      const double czp = carr[vzp];
+      const double mzp = (double)cm2[vzp];
+      const double catp = cc;
+      const unsigned w = t.vox[v];
+      const unsigned cell1 = w & 0x7fffffffu;
+      const double diffu = cell1 ? t.diff_u[cell1 - 1] : 0.0;
+      const double air = (double)(w >> 31);
+      const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
+      const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
+      const double fa = STIM11 * min0(diffu) - catp * Kaa;
+      const double nv = catp + DT_Ca * (Da * ((double)cm2[vxm] * (carr[vxm] - catp)
+                                              + (double)cm2[vxp] * (carr[vxp] - catp)
+                                              + (double)cm2[vym] * (carr[vym] - catp)
+                                              + (double)cm2[vyp] * (carr[vyp] - catp)
+                                              + mzm * (czm - catp)
+                                              + mzp * (czp - catp))
+                                        * inv_dx * inv_dx + fa + air * AIR_STIM);
+      narr[v] = nv;
+      if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+      if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+      if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+      czm = cc; mzm = mc;
+      cc = czp; mc = mzp;
     }
   }
-  // plane l = iz_bound: copies of values left in the next buffer (:334-337)
+  // plane l = iz_bound: copies of values left in the next buffer (:334-337);
+  // the corner (NX, NY) ends as a copy of (0, 0)
   if (zc == 0) {
-    if (k == 0) narr[gidx(t, j, NY, izb)] = narr[gidx(t, j == NX ? 0 : j, 0, izb)];
-    if (j == 0 && k < NY) narr[gidx(t, NX, k, izb)] = narr[gidx(t, 0, k, izb)];
+    if (k == 0) narr[gidx(t, j, NY, izb)] = narr[gidx(t, j, 0, izb)];
+    if (j == 0) narr[gidx(t, NX, k, izb)] = narr[gidx(t, 0, k, izb)];
+    if (j == 0 && k == 0) narr[gidx(t, NX, NY, izb)] = narr[gidx(t, 0, 0, izb)];
   }
 }
 
-// set_cell_ca2p (:355-367) and the diff_u reset of update_values (:346-348);
-// ca_calc_condition_reset (cellmanager.h:101)
 __global__ void __launch_bounds__(256) k_ca_end(const Tis* __restrict__ T, const int* tlist, int full) {
   const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
   const int n = t.st->n;
@@ -491,11 +486,28 @@ void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long
   EPI_LAUNCH(K_MAP_HALO, k_map_halo, dim3((max_halo + 255) / 256, ntis), dim3(256), 0, s, dT);
 }
 
-void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est) {
-  const int nky = (max_ny + 3) / 4;
-  const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
-  dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
-  EPI_LAUNCH(K_EXT_STIM, k_ext_stim, g, dim3(256), 0, s, dT, bp, nky, nzc);
+namespace {
+template <class F>
+void by_zb(int zb, F f) {
+  switch (zb) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 8: f(std::integral_constant<int, 8>()); break;
+    default: f(std::integral_constant<int, 4>()); break;
+  }
+}
+}  // namespace
+
+void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est,
+                     const Tuning& tu) {
+  const int rows = tu.b_rows;
+  const int nky = (max_ny + rows - 1) / rows;
+  by_zb(tu.b_zb, [&](auto zbc) {
+    constexpr int zb = decltype(zbc)::value;
+    const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
+    dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
+    EPI_LAUNCH(K_EXT_STIM, k_ext_stim<zb>, g, dim3(64 * rows), 0, s, dT, bp, nky, nzc);
+  });
 }
 
 static void ca_maps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box,
@@ -520,13 +532,23 @@ void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int nt
 }
 
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est) {
-  EPI_LAUNCH(K_CA_CELLS, k_ca_cells, dim3((max_n + C_GROUPS - 1) / C_GROUPS, ntis), dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap,
-             bp);
-  const int nky = (max_ny + 1 + 3) / 4;
-  const int nzc = izb_est > 0 ? (izb_est + ZB - 1) / ZB : 1;
-  dim3 g((max_nx + 1 + 63) / 64, ntis, nky * nzc);
-  EPI_LAUNCH(K_ATP, k_atp, g, dim3(256), 0, s, dT, tlist, ap, nky, nzc);
+                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
+  const int cg = tu.ca_group;
+  const dim3 gc((max_n + C_BLOCK / cg - 1) / (C_BLOCK / cg), ntis);
+  if (cg == 4)
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp);
+  else if (cg == 8)
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp);
+  else
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp);
+  const int rows = tu.a_rows;
+  const int nky = (max_ny + rows - 1) / rows;
+  by_zb(tu.a_zb, [&](auto zbc) {
+    constexpr int zb = decltype(zbc)::value;
+    const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
+    dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
+    EPI_LAUNCH(K_ATP, k_atp<zb>, g, dim3(64 * rows), 0, s, dT, tlist, ap, nky, nzc);
+  });
 }
 
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n) {

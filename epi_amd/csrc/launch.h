@@ -13,6 +13,13 @@ enum KernelId {
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET, K_COUNT
 };
 
+// Launch shapes that the autotuner may change (values checked by the host).
+struct Tuning {
+  int b_zb = 4, b_rows = 4;   // k_ext_stim: z planes per chunk, y rows per block
+  int a_zb = 4, a_rows = 4;   // k_atp: the same
+  int ca_group = 16;          // k_ca_cells: lanes per cell
+};
+
 // Event timer: when the bit of a kernel id is set, its launches are
 // bracketed by HIP events on the launch stream (see host.hip).
 bool prof_on(int k);
@@ -36,10 +43,11 @@ void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int 
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v);
 void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo);
-void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est);
+void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est,
+                     const Tuning& tu);
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est);
+                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
 void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);

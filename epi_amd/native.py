@@ -95,7 +95,11 @@ SYMBOLS = [
     "epi_get_status", "epi_get_cells", "epi_get_fields", "epi_set_positions",
     "epi_set_state", "epi_set_lists", "epi_set_maps", "epi_profile_enable",
     "epi_profile_read", "epi_profile_reset", "epi_profile_kernel_count",
+    "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n",
 ]
+
+# launch-shape keys of epi_set_tuning (include/epi_amd.h)
+TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4}
 
 _lib = None
 
@@ -142,6 +146,9 @@ def load_library():
     L.epi_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), _dp, _i64p]
     L.epi_profile_reset.argtypes = [vp]
     L.epi_profile_kernel_count.argtypes = []
+    L.epi_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+    L.epi_calc_ca2p_n.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
+    L.epi_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -267,6 +274,8 @@ class TissueSet:
     def initialize_sc(self, zzmax): self._chk(self.L.epi_initialize_sc(self.h, self._zz(zzmax)), "initialize_sc")
     def calc_ca2p(self, zzmax): self._chk(self.L.epi_calc_ca2p(self.h, self._zz(zzmax)), "calc_ca2p")
     def ca_substep(self, iz_bound): self._chk(self.L.epi_ca_substep(self.h, iz_bound), "ca_substep")
+    def calc_ca2p_n(self, zzmax, n_sub):
+        self._chk(self.L.epi_calc_ca2p_n(self.h, self._zz(zzmax), int(n_sub)), "calc_ca2p_n")
 
     def should_calc_ca(self):
         f = np.zeros(self.T, dtype=np.int32)
@@ -308,6 +317,20 @@ class TissueSet:
 
     def profile_reset(self):
         self._chk(self.L.epi_profile_reset(self.h), "profile_reset")
+
+    # --- launch shapes ---
+    def set_tuning(self, tuning):
+        """Apply a {key: value} dict of launch shapes (keys of TUNING_KEYS)."""
+        for k, v in tuning.items():
+            self._chk(self.L.epi_set_tuning(self.h, TUNING_KEYS[k], int(v)), f"set_tuning {k}={v}")
+
+    def get_tuning(self):
+        out = {}
+        for k, key in TUNING_KEYS.items():
+            v = C.c_int32()
+            self._chk(self.L.epi_get_tuning(self.h, key, C.byref(v)), "get_tuning")
+            out[k] = v.value
+        return out
 
     # --- state exchange ---
     def status(self, t):

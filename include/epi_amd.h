@@ -136,6 +136,10 @@ int epi_setup_map_lat(epi_ctx* ctx);               /* map.h:5-7, map.cpp:24 */
 int epi_calc_ext_stim(epi_ctx* ctx, const double* zzmax); /* ext_stim.h:5, ext_stim.cpp:18 */
 int epi_initialize_sc(epi_ctx* ctx, const double* zzmax); /* cell_state_renew.h:4, :435 */
 int epi_calc_ca2p(epi_ctx* ctx, const double* zzmax);     /* ca2p.h:7, ca2p.cpp:380 */
+/* calc_ca2p with n_sub sub-steps in place of Ca_ITR (the averages are still
+ * divided by Ca_ITR).  No counterpart in the reference: timing and tuning
+ * samples only. */
+int epi_calc_ca2p_n(epi_ctx* ctx, const double* zzmax, int32_t n_sub);
 int epi_should_calc_ca(epi_ctx* ctx, int32_t* flags);     /* cellmanager.h:97 */
 int epi_ca_calc_condition_reset(epi_ctx* ctx);            /* cellmanager.h:101 */
 /* one sub-step of calc_ca2p's loop (ca2p.cpp:392-399) with a given iz_bound,
@@ -170,6 +174,19 @@ int epi_profile_enable(epi_ctx* ctx, uint64_t mask);
 int epi_profile_read(epi_ctx* ctx, int32_t kernel, const char** name, double* total_ms, int64_t* count);
 int epi_profile_reset(epi_ctx* ctx);
 int32_t epi_profile_kernel_count(void);
+
+/* ---- launch shapes -------------------------------------------------------
+ * No counterpart in the reference.  Launch-shape knobs of the grid kernels,
+ * chosen per workload by epi_amd/autotune.py; they change timing only, never
+ * results.  Values: *_ZB in {1, 2, 4, 8} (z planes per thread per chunk),
+ * *_ROWS in {1, 2, 4} (y rows of 64 x points per block), CA_GROUP in {4, 8, 16} (lanes per
+ * cell in the Ca cell kernel).  EPI_ERR_INVALID for other keys or values. */
+enum {
+    EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
+    EPI_TUNE_CA_GROUP = 4, EPI_TUNE_COUNT = 5
+};
+int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
+int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,98 @@
+"""The C++ facade (include/epi_amd.hpp) and the driver loop written against it
+(epi_amd/host/proc.cpp, the reference's proc.cpp:131-173 call for call).
+
+CPU: the header compiles on its own, the deck format round-trips, the driver
+parses its arguments.  GPU: the driver, fed a deck through cman_init,
+reaches the same state as the Python driver of the same library through
+epi_step (bit-exact), and that state agrees with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from epi_amd import configs, deck
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "epi_amd", "host", "epi_proc")
+
+
+def test_header_compiles_standalone(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "epi_amd.hpp"\n'
+                   "int main() { epi_amd::CellManager* m = nullptr; (void)m; return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                        f"-I{os.path.join(ROOT, 'include')}", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_deck_round_trip(tmp_path):
+    spec = configs.config_tiny(7, 160)
+    p = tmp_path / "deck.txt"
+    deck.write_deck(spec, p)
+    back = deck.spec_from_columns(deck.parse_deck(p), spec["geometry"])
+    assert back["nmemb"] == spec["nmemb"] and back["nder"] == spec["nder"]
+    for k in ("x", "y", "z", "radius", "agek", "ageb", "ex_fat", "in_fat", "spr_nat_len"):
+        assert np.array_equal(back["cells"][k], spec["cells"][k]), k
+    for k in ("state", "pair", "rest_div_times", "fix_origin"):
+        assert np.array_equal(back["cells"][k], spec["cells"][k]), k
+    # the loader's flag: stem_orig_id < MALIG_NUM = 0 (cellmanager.cpp:292)
+    assert np.array_equal(back["cells"]["is_malignant"], (spec["cells"]["fix_origin"] < 0).astype(np.uint8))
+
+
+def test_driver_usage():
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    r = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def _geometry_args(g):
+    return ["--geometry", str(g.nmx), str(g.nmy), str(g.nx), str(g.ny), str(g.nz), repr(g.lx),
+            repr(g.ly), repr(g.lz), str(g.max_cells)]
+
+
+@pytest.mark.gpu
+def test_driver_matches_python_driver_and_oracle(tmp_path, oracle_mod):
+    from epi_amd import native
+    assert os.path.exists(DRIVER), "epi_amd/host/epi_proc is not built"
+    spec0 = configs.config_tiny(7, 160)
+    g = spec0["geometry"]
+    p = tmp_path / "deck.txt"
+    deck.write_deck(spec0, p)
+    steps, ca_every, seed = 40, 20, 11
+    dump = tmp_path / "dump.bin"
+    print(f"facade driver: tiny deck, {steps} steps, Ca every {ca_every}, seed {seed}", flush=True)
+    r = subprocess.run([DRIVER, str(p), str(steps), "--seed", str(seed), "--ca-every", str(ca_every),
+                        "--dump", str(dump), *_geometry_args(g)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(dump, dtype=np.float64)
+    n = int(raw[0])
+    x, y, z, ca2p, agek = raw[1:].reshape(5, n)
+
+    spec = deck.spec_from_columns(deck.parse_deck(p), g)
+    draws = native.mt19937_64_uniform(seed, 200000)
+    t = native.Tissue(g)
+    t.load(spec, value_init=True)
+    t.set_rng(draws)
+    t.step(native.Driver(0, 0, 0, ca_every, 0), steps)
+    assert t.error() == 0
+    c = t.get_cells()
+    assert c["x"].size == n
+    for k, v in (("x", x), ("y", y), ("z", z), ("ca2p", ca2p), ("agek", agek)):
+        assert np.array_equal(c[k], v), k
+
+    o = oracle_mod.OracleTissue(g)
+    o.load(spec, value_init=True)
+    o.set_rng(draws)
+    drv = oracle_mod.Driver(0, 0, 0, ca_every, 0)
+    for _ in range(steps):
+        o.step(drv)
+    assert o.error() == 0
+    oc = o.get_cells()
+    assert oc["x"].size == n
+    for k, v in (("x", x), ("y", y), ("z", z)):
+        assert np.array_equal(oc[k], v), k
+    # Ca values carry ulp differences of tanh between glibc and the GPU
+    rel = np.max(np.abs(oc["ca2p"] - ca2p) / np.maximum(np.abs(oc["ca2p"]), 1e-300))
+    assert rel <= 1e-10, rel
