@@ -41,7 +41,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", choices=["large", "small"], default="large")
+    ap.add_argument("--config", choices=["large", "small", "batched"], default="large")
+    ap.add_argument("--tissues", type=int, default=256, help="batched: number of tissues")
+    ap.add_argument("--loop-sample", type=int, default=16,
+                    help="batched: tissues timed one context at a time for the per-instance loop")
     ap.add_argument("--ca-every", type=int, default=500)
     ap.add_argument("--roofline-kernel", default="k_atp")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -107,6 +110,119 @@ def cpu_baseline(spec, seed, gpu_cell_steps, gpu_ca_solves, timed_steps, budget_
     }
 
 
+def run_batched(args):
+    """256 independent tissues (uneven sheets, cell counts and grids) in one
+    context: every kernel launch covers all of them (blockIdx.y = tissue).
+    Beside it, the plain per-instance loop: a sample of the same tissues,
+    each in its own context, run one after another over the same steps."""
+    from epi_amd import autotune, configs, native
+    specs = configs.batched_configs(args.tissues)
+    geos = [s["geometry"] for s in specs]
+    total = args.steps + args.warmup
+    draws = [native.mt19937_64_uniform(s["seed"], n_draws_needed(s, total)) for s in specs]
+    drivers = lambda k: [native.Driver(0, 0, 0, args.ca_every, 0) for _ in range(k)]
+
+    def make(idx):
+        ts = native.TissueSet([geos[i] for i in idx])
+        for t, i in enumerate(idx):
+            ts.load(t, specs[i])
+            ts.set_rng(t, draws[i])
+        return ts
+
+    tuning, tune_s = None, 0.0
+    if not args.no_tune:
+        tt = make(range(len(specs)))
+        tuning, rep = autotune.tune(tt, drivers(len(specs)), warm_steps=1, b_reps=5, ca_sub=10,
+                                    log=sys.stderr)
+        tune_s = rep["seconds"]
+        tt.close()
+    ts = make(range(len(specs)))
+    if tuning:
+        ts.set_tuning(tuning)
+    drv = drivers(len(specs))
+    if args.warmup:
+        ts.step(drv, args.warmup)
+    ts.synchronize()
+    s0 = [ts.status(t)[0] for t in range(len(specs))]
+    ts.profile_reset()
+    ts.profile_enable([args.roofline_kernel])
+    t0 = time.perf_counter()
+    ts.step(drv, args.steps)
+    ts.synchronize()
+    wall = time.perf_counter() - t0
+    s1 = [ts.status(t)[0] for t in range(len(specs))]
+    prof = ts.profile()
+    ts.profile_enable([])
+    cell_steps = sum(b.cell_steps - a.cell_steps for a, b in zip(s0, s1))
+    grid_pts = sum((b.grid_points_b - a.grid_points_b) + (b.grid_points_ca - a.grid_points_ca)
+                   for a, b in zip(s0, s1))
+    ca_solves = sum(b.ca_solves - a.ca_solves for a, b in zip(s0, s1))
+    errs = [b.err for b in s1 if b.err]
+    roof = None
+    if args.roofline_kernel in prof:
+        ms, cnt = prof[args.roofline_kernel]
+        avg_s = ms / cnt / 1e3
+        key = "grid_points_ca" if args.roofline_kernel == "k_atp" else "grid_points_b"
+        units = sum(getattr(b, key) - getattr(a, key) for a, b in zip(s0, s1)) / cnt
+        alg = BYTES_PER_POINT[args.roofline_kernel] * units
+        roof = {"bound": "hbm", "achieved": round(alg / avg_s / 1e9, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(alg / avg_s / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": args.roofline_kernel, "avg_us": round(avg_s * 1e6, 3), "launches": cnt,
+                "algorithmic_bytes_per_launch": round(alg, 1)}
+    ts.close()
+    # per-instance loop over an evenly spaced sample of the same tissues
+    idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
+                     for i in range(args.loop_sample)))
+    loop_cells, loop_wall = 0, 0.0
+    for i in idx:
+        one = make([i])
+        if tuning:
+            one.set_tuning(tuning)
+        d = drivers(1)
+        if args.warmup:
+            one.step(d, args.warmup)
+        one.synchronize()
+        a = one.status(0)[0]
+        t0 = time.perf_counter()
+        one.step(d, args.steps)
+        one.synchronize()
+        loop_wall += time.perf_counter() - t0
+        loop_cells += one.status(0)[0].cell_steps - a.cell_steps
+        one.close()
+    loop_value = loop_cells / loop_wall
+    value = cell_steps / wall
+    out = {
+        "metric": "cell-steps/s + grid-cell-steps/s (FP64, 1e-10 vs ref); batched & adjoint vs plain",
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched: {len(specs)} tissues in one context (sheet side U[32,96], "
+                               f"free cells U[500,8000] capped to the box, grid edge U[32,96], nz = 2 edge, "
+                               f"seeds 3000+t), Ca every {args.ca_every} + sw gating",
+                   "cells_start": int(sum(s["cells"]["x"].size for s in specs)),
+                   "grid_points": int(sum((g.nx + 1) * (g.ny + 1) * (g.nz + 1) for g in geos)),
+                   "ca_solves": int(ca_solves), "errors": errs, "launch_shapes": tuning,
+                   "autotune_seconds": round(tune_s, 2)},
+        "grid_cell_steps_per_s": round(grid_pts / wall, 1),
+        "roofline": roof,
+        "per_instance_loop": {"value": round(loop_value, 1), "unit": "cell-steps/s",
+                              "tissues_timed": len(idx), "steps": args.steps,
+                              "sample": "each sampled tissue in its own context, one after another, "
+                                        "same steps and Ca cadence"},
+        "batched_vs_loop": round(value / loop_value, 2),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -114,6 +230,8 @@ def main():
               "(tissues shard across GPUs with no per-step exchange; see DESIGN.md)",
               file=sys.stderr)
         return 2
+    if args.config == "batched":
+        return run_batched(args)
     from epi_amd import native
     spec = make_spec(args.config)
     seed = spec["seed"]

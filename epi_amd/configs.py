@@ -146,10 +146,16 @@ def make_tissue(seed, nmx, nmy, lx, ly, nx, ny, nz, n_fix, n_free, n_layers,
     n_layers = max(n_layers, int(math.ceil((n_free + n_fix) / sites)))
     per_layer = [n_free // n_layers] * n_layers
     per_layer[0] += n_free - sum(per_layer)
-    if per_layer[0] + n_fix > sites:
-        shift = per_layer[0] + n_fix - sites
-        per_layer[0] -= shift
-        per_layer[1] += shift
+    # layer 0 shares its sites with the FIX cells; the excess moves up
+    extra = max(0, per_layer[0] + n_fix - sites)
+    per_layer[0] -= extra
+    for li in range(1, n_layers):
+        if not extra:
+            break
+        move = min(extra, sites - per_layer[li])
+        per_layer[li] += move
+        extra -= move
+    assert extra == 0 and all(0 < p <= sites for p in per_layer), "tissue does not fit its layers"
     z0 = R_MEMB + R_MEMB + R_MAX
     layers = []
     fix_part = None
@@ -263,11 +269,23 @@ def config_tiny(seed=7, n_free=160, side=16, grid=16, n_layers=2, dead_top=False
                        lz=nz * (lx / grid))
 
 
+def batched_capacity(side_n, lz, dz, n_fix):
+    """Free cells a box holds at the layer pitch of make_tissue while the top
+    layer keeps calc_ca2p's iz_bound = (zzmax + 2.8) / dz below nz (with a
+    margin of 2 for the jitter and the first steps' motion)."""
+    z0 = R_MEMB + R_MEMB + R_MAX
+    layers = int((lz - 2.8 - 2.0 * dz - 2.0 - z0) / 2.2) + 1
+    return side_n * side_n * max(layers, 1) - n_fix
+
+
 def batched_configs(n_tissues=256, base_seed=3000, scale=1.0):
     """Batched config: per tissue t (seed 3000+t) a sheet side ~U[32,96]
     (even), free cells ~U[500,8000], 16 FIX, grid edge e ~U[32,96],
-    nz = 2e, dx = L/e.  L = 2*ceil(0.4*side) keeps every x below ANX.
-    ``scale`` < 1 shrinks the cell counts for quick tests."""
+    nz = 2e, dx = dy = dz = L/e (so LZ = 2L).  L = 2*ceil(0.4*side) keeps
+    every x below ANX.  The free-cell count is capped at what the box holds
+    at layer pitch 2.0 x 2.2 (batched_capacity): a side-32 sheet (L = 26)
+    holds about 3,000.  ``scale`` < 1 shrinks the cell counts for quick
+    tests."""
     out = []
     for t in range(n_tissues):
         r = np.random.default_rng(base_seed + t)
@@ -275,9 +293,10 @@ def batched_configs(n_tissues=256, base_seed=3000, scale=1.0):
         n_free = int(r.integers(500, 8001) * scale)
         edge = int(r.integers(32, 97))
         lx = 2.0 * math.ceil(0.4 * side)
-        n_layers = max(2, int(math.ceil(n_free / (side * side * 0.3))))
+        lz = 2 * edge * (lx / edge)
+        side_n = max(2, int(lx / 2.0))
+        n_free = max(64, min(n_free, batched_capacity(side_n, lz, lx / edge, 16)))
         out.append(make_tissue(base_seed + t, nmx=side, nmy=side, lx=lx, ly=lx,
                                nx=edge, ny=edge, nz=2 * edge, n_fix=16,
-                               n_free=max(n_free, 64), n_layers=n_layers,
-                               n_hot_alive=8, lz=2 * edge * (lx / edge)))
+                               n_free=n_free, n_layers=2, n_hot_alive=8, lz=lz))
     return out

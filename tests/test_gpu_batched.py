@@ -1,0 +1,92 @@
+"""Batching: tissues of different sheets, cell counts and grids advanced in
+one context (every launch covers all of them) end bit-identical to each
+tissue run alone, and each agrees with the oracle (MI355X only)."""
+import numpy as np
+import pytest
+
+from epi_amd import configs
+from parity import compare_cells, compare_fields
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-10
+
+
+def batch_specs():
+    return [
+        configs.config_tiny(7, n_free=160),
+        configs.config_tiny(8, n_free=260, side=20, grid=20, nz=28),
+        configs.config_tiny(9, n_free=120, side=16, grid=12, nz=20, n_layers=2),
+        configs.config_tiny(10, n_free=300, side=24, grid=18, nz=30, dead_top=True, n_layers=3),
+    ]
+
+
+def _single(native, spec, draws, steps, ca_every, tuning=None):
+    t = native.Tissue(spec["geometry"])
+    t.load(spec)
+    t.set_rng(draws)
+    if tuning:
+        t.set.set_tuning(tuning)
+    t.step(native.Driver(0, 0, 0, ca_every, 0), steps)
+    return t
+
+
+@pytest.mark.parametrize("tuning", [None, {"b_zb": 2, "b_rows": 1, "atp_zb": 8, "atp_rows": 2, "ca_group": 4}])
+def test_batch_equals_single_runs(oracle_mod, tuning):
+    from epi_amd import native
+    specs = batch_specs()
+    steps, ca_every = 40, 20
+    draws = [native.mt19937_64_uniform(100 + i, 300000) for i in range(len(specs))]
+    print(f"batched: {len(specs)} tissues, {steps} steps, Ca every {ca_every}, tuning {tuning}", flush=True)
+    ts = native.TissueSet([s["geometry"] for s in specs])
+    for t, s in enumerate(specs):
+        ts.load(t, s)
+        ts.set_rng(t, draws[t])
+    if tuning:
+        ts.set_tuning(tuning)
+    ts.step([native.Driver(0, 0, 0, ca_every, 0) for _ in specs], steps)
+    for t, s in enumerate(specs):
+        st = ts.status(t)[0]
+        assert st.err == 0, (t, st.err)
+        one = _single(native, s, draws[t], steps, ca_every)
+        a, b = ts.get_cells(t), one.get_cells()
+        d = compare_cells(a, b)
+        assert "n" not in d, (t, d)
+        for k, v in d.items():
+            assert v == 0, (t, k, v)          # bit-identical, floats included
+        fa, fb = ts.get_fields(t), one.get_fields()
+        for k in fa:
+            assert np.array_equal(fa[k], fb[k]), (t, k)
+        assert st.rng_pos == one.rng_cursor()
+        assert st.ca_solves == one.status().ca_solves
+
+
+def test_batch_against_oracle(oracle_mod):
+    from epi_amd import native
+    specs = batch_specs()
+    steps, ca_every = 30, 15
+    draws = [native.mt19937_64_uniform(200 + i, 300000) for i in range(len(specs))]
+    ts = native.TissueSet([s["geometry"] for s in specs])
+    for t, s in enumerate(specs):
+        ts.load(t, s)
+        ts.set_rng(t, draws[t])
+    ts.step([native.Driver(0, 0, 0, ca_every, 0) for _ in specs], steps)
+    for t, s in enumerate(specs):
+        o = oracle_mod.OracleTissue(s["geometry"])
+        o.load(s)
+        o.set_rng(draws[t])
+        drv = oracle_mod.Driver(0, 0, 0, ca_every, 0)
+        for _ in range(steps):
+            o.step(drv)
+        assert o.error() == 0
+        d = compare_cells(ts.get_cells(t), o.get_cells())
+        assert "n" not in d, (t, d)
+        for k, v in d.items():
+            if isinstance(v, int):
+                assert v == 0, (t, k, v)
+            else:
+                assert v <= FP_TOL, (t, k, v)
+        f = compare_fields(ts.get_fields(t), o.get_fields())
+        for k, v in f.items():
+            assert (v == 0) if k.startswith("cmap") else (v <= FP_TOL), (t, k, v)
+        assert ts.status(t)[0].rng_pos == o.rng_cursor()
