@@ -86,6 +86,7 @@ struct Tis {
   double LX, LY, LZ, dx, dy, dz, inv_dx, inv_dy, inv_dz;
   int ANX, ANY, ANZ, bcap;
   int irx, iry, irz, pad1;
+  double ca_du, ca_dp;        // ca2p_du (ca2p.cpp:183) and dp (ca2p.cpp:17)
   Status* st;
   // cells
   uint8_t* state; uint8_t* is_malignant; uint8_t* touch_air;

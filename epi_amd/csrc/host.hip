@@ -20,6 +20,7 @@
 #include "../../include/epi_amd.h"
 #include "epi_dev.h"
 #include "launch.h"
+#include "rd_dev.h"
 
 using namespace epi;
 
@@ -110,6 +111,18 @@ struct epi_ctx {
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
   Tuning tu;
+  // frozen reaction-diffusion window (epi_rd_*)
+  struct Rd {
+    bool active = false;
+    std::vector<RdTis> h;
+    RdTis* d = nullptr;
+    char* arena = nullptr;      // snapshot and adjoint arrays
+    char* work = nullptr;       // segment slots and checkpoints
+    size_t work_bytes = 0;
+    double* d_gj0 = nullptr;
+    std::vector<int> izb;
+    std::vector<double> zz;
+  } rd;
   std::vector<long long> ca_solves;
   // throughput accounting of epi_step
   std::vector<long long> steps, cell_steps, grid_b, grid_ca;
@@ -421,6 +434,8 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
     ti.irx = (int)(Rad * g.nx / g.lx);
     ti.iry = (int)(Rad * g.ny / g.ly);
     ti.irz = (int)(Rad * g.nz / g.lz);
+    ti.ca_du = 0.01;
+    ti.ca_dp = 0.1;
   }
   // arena
   Carver dry{nullptr, 0, true};
@@ -469,6 +484,10 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
 void epi_destroy(epi_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->rd.arena) (void)hipFree(c->rd.arena);
+  if (c->rd.work) (void)hipFree(c->rd.work);
+  if (c->rd.d) (void)hipFree(c->rd.d);
+  if (c->rd.d_gj0) (void)hipFree(c->rd.d_gj0);
   for (auto& h : c->th)
     if (h.rng_d) (void)hipFree(h.rng_d);
   if (c->arena) (void)hipFree(c->arena);
@@ -1121,3 +1140,230 @@ int epi_set_maps(epi_ctx* c, int32_t t, const int32_t* cmap1, const uint8_t* cma
 }
 
 }  // extern "C"
+
+// ---- frozen reaction-diffusion window and its adjoint ----------------------
+namespace {
+
+long long rd_items(const epi_ctx* c) {
+  long long m = 1;
+  for (int t = 0; t < c->T; t++)
+    m = std::max({m, (long long)c->th[t].n_est * c->th[t].ns, c->th[t].gsize});
+  return m;
+}
+
+long long rd_max_box(const epi_ctx* c) {
+  long long m = 1;
+  for (auto& r : c->rd.h) m = std::max(m, r.box);
+  return m;
+}
+
+// du/dp into the device descriptors, kpa (or its snapshot) and gj0 staged
+int rd_params(epi_ctx* c, const epi_rd_params* prm) {
+  std::vector<double> g0(c->T);
+  for (int t = 0; t < c->T; t++) {
+    c->tis[t].ca_du = prm[t].du;
+    c->tis[t].ca_dp = prm[t].dp;
+    g0[t] = prm[t].gj0;
+    const RdTis& r = c->rd.h[t];
+    const size_t n = (size_t)c->h_st[t].n;
+    if (prm[t].kpa)
+      HIPCHK(c, hipMemcpyAsync(r.kpa_in, prm[t].kpa, 8 * n, hipMemcpyHostToDevice, c->stream));
+    else
+      HIPCHK(c, hipMemcpyAsync(r.kpa_in, r.kpa0, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * c->T, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->rd.d_gj0, g0.data(), 8 * c->T, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EPI_OK;
+}
+
+int rd_substep(epi_ctx* c) {
+  int izb = 1;
+  for (int v : c->rd.izb) izb = std::max(izb, v + 1);
+  launch_ca_substep(c->stream, c->d_tis, nullptr, c->T, c->cb, c->cp, c->ap, c->bp, max_n(c), max_nx(c), max_ny(c),
+                    izb, c->tu);
+  c->cp ^= 1;
+  c->ap ^= 1;
+  return EPI_OK;
+}
+
+}  // namespace
+
+int epi_rd_begin(epi_ctx* c, const double* zzmax) {
+  if (!c || !zzmax) return EPI_ERR_INVALID;
+  if (c->rd.active) return fail(c, EPI_ERR_INVALID, "a reaction-diffusion window is already open");
+  RCHK(sync_status(c));
+  auto& rd = c->rd;
+  rd.izb.assign(c->T, 0);
+  rd.zz.assign(zzmax, zzmax + c->T);
+  for (int t = 0; t < c->T; t++) {
+    rd.izb[t] = (int)((zzmax[t] + FAC_MAP * R_max) / c->tis[t].dz);   // ca2p.cpp:386
+    if (rd.izb[t] < 0 || rd.izb[t] > c->tis[t].NZ) return fail(c, EPI_ERR_OUT_OF_GRID, "iz_bound outside the grid");
+  }
+  RCHK(set_zz(c, zzmax));
+  // per-solve maps: entry kinds, th/Kpa/IAG, voxel owners (init_ca2p_map)
+  long long box = 1;
+  for (int t = 0; t < c->T; t++) box = std::max(box, (long long)c->th[t].NX * c->th[t].NY * (rd.izb[t] + 1));
+  launch_ca_touch(c->stream, c->d_tis, nullptr, c->T, c->cb, max_n(c), box);
+  // arena: snapshot + adjoint arrays per tissue
+  rd.h.assign(c->T, RdTis{});
+  Carver dry{nullptr, 0, true};
+  auto carve = [&](Carver& cv) {
+    for (int t = 0; t < c->T; t++) {
+      RdTis& r = rd.h[t];
+      const size_t cap = c->th[t].cap, ns = c->th[t].ns, gs = c->th[t].gsize;
+      r.izb = rd.izb[t];
+      r.box = (long long)(rd.izb[t] + 1) * c->th[t].PL;
+      for (int b = 0; b < 2; b++) {
+        r.u0[b] = cv.take<double>(cap); r.p0[b] = cv.take<double>(cap); r.A0[b] = cv.take<double>(gs);
+        r.lu[b] = cv.take<double>(cap); r.lp[b] = cv.take<double>(cap); r.lh[b] = cv.take<double>(cap);
+        r.lg[b] = cv.take<double>(cap * ns); r.lA[b] = cv.take<double>((size_t)r.box);
+      }
+      r.h0 = cv.take<double>(cap); r.avg0 = cv.take<double>(cap); r.kpa0 = cv.take<double>(cap);
+      r.g0 = cv.take<double>(cap * ns);
+      r.accA = cv.take<double>(cap); r.gkpa = cv.take<double>(cap); r.target = cv.take<double>(cap);
+      r.kpa_in = cv.take<double>(cap); r.scal = cv.take<double>(4);
+      r.nafm = 0;
+    }
+  };
+  carve(dry);
+  if (hipMalloc(&rd.arena, align_up(dry.off, 256)) != hipSuccess ||
+      hipMalloc(&rd.d, sizeof(RdTis) * c->T) != hipSuccess || hipMalloc(&rd.d_gj0, 8 * c->T) != hipSuccess)
+    return fail(c, EPI_ERR_HIP, "window allocation failed");
+  Carver real{rd.arena, 0, false};
+  carve(real);
+  HIPCHK(c, hipMemcpyAsync(rd.d, rd.h.data(), sizeof(RdTis) * c->T, hipMemcpyHostToDevice, c->stream));
+  launch_rd_snapshot(c->stream, c->d_tis, rd.d, c->T, c->cb, c->cp, c->ap, rd_items(c));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(rd.h.data(), rd.d, sizeof(RdTis) * c->T, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  rd.active = true;
+  return EPI_OK;
+}
+
+int epi_rd_forward(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub) {
+  if (!c || !prm || n_sub < 0) return EPI_ERR_INVALID;
+  if (!c->rd.active) return fail(c, EPI_ERR_INVALID, "no reaction-diffusion window is open");
+  RCHK(rd_params(c, prm));
+  launch_rd_restore(c->stream, c->d_tis, c->rd.d, c->rd.d_gj0, c->T, c->cb, c->cp, c->ap, 1, rd_items(c));
+  for (int s = 0; s < n_sub; s++) RCHK(rd_substep(c));
+  HIPCHK(c, hipGetLastError());
+  return sync_status(c);
+}
+
+int epi_rd_get_ca2p(epi_ctx* c, int32_t t, double* out) {
+  RCHK(check_tissue(c, t));
+  if (!out) return EPI_ERR_INVALID;
+  HIPCHK(c, hipMemcpy(out, c->tis[t].ca2p[c->cp], 8 * (size_t)c->h_st[t].n, hipMemcpyDeviceToHost));
+  return EPI_OK;
+}
+
+int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t every, const double* const* targets,
+                   double* loss, epi_rd_grad* grads) {
+  if (!c || !prm || !targets || n_sub < 1 || every < 1) return EPI_ERR_INVALID;
+  auto& rd = c->rd;
+  if (!rd.active) return fail(c, EPI_ERR_INVALID, "no reaction-diffusion window is open");
+  const int K = std::min(every, n_sub);
+  const int nck = (n_sub + K - 1) / K;
+  // segment slots (K states) and checkpoints (nck, both buffers)
+  size_t need = 0;
+  {
+    Carver dry{nullptr, 0, true};
+    for (int t = 0; t < c->T; t++) {
+      const size_t cap = c->th[t].cap, ns = c->th[t].ns, box = (size_t)rd.h[t].box;
+      dry.take<double>(K * cap); dry.take<double>(K * cap); dry.take<double>(K * cap);
+      dry.take<double>(K * cap * ns); dry.take<double>(K * box);
+      dry.take<double>(2 * nck * cap); dry.take<double>(2 * nck * cap); dry.take<double>(nck * cap);
+      dry.take<double>(nck * cap * ns); dry.take<double>(2 * nck * box);
+    }
+    need = align_up(dry.off, 256);
+  }
+  if (need > rd.work_bytes) {
+    if (rd.work) (void)hipFree(rd.work);
+    rd.work = nullptr;
+    rd.work_bytes = 0;
+    if (hipMalloc(&rd.work, need) != hipSuccess) return fail(c, EPI_ERR_HIP, "window work allocation failed");
+    rd.work_bytes = need;
+  }
+  Carver cv{rd.work, 0, false};
+  for (int t = 0; t < c->T; t++) {
+    RdTis& r = rd.h[t];
+    const size_t cap = c->th[t].cap, ns = c->th[t].ns, box = (size_t)r.box;
+    r.su = cv.take<double>(K * cap); r.sp = cv.take<double>(K * cap); r.sh = cv.take<double>(K * cap);
+    r.sg = cv.take<double>(K * cap * ns); r.sA = cv.take<double>(K * box);
+    r.cu = cv.take<double>(2 * nck * cap); r.cp_ = cv.take<double>(2 * nck * cap); r.ch = cv.take<double>(nck * cap);
+    r.cg = cv.take<double>(nck * cap * ns); r.cA = cv.take<double>(2 * nck * box);
+    HIPCHK(c, hipMemcpyAsync(r.target, targets[t], 8 * (size_t)c->h_st[t].n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(r.scal, 0, 4 * sizeof(double), c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(rd.d, rd.h.data(), sizeof(RdTis) * c->T, hipMemcpyHostToDevice, c->stream));
+  RCHK(rd_params(c, prm));
+  const long long items = rd_items(c);
+  launch_rd_restore(c->stream, c->d_tis, rd.d, rd.d_gj0, c->T, c->cb, c->cp, c->ap, 1, items);
+  // forward, checkpoint every K sub-steps
+  std::vector<int> ck_cp(nck), ck_ap(nck);
+  for (int s = 0; s < n_sub; s++) {
+    if (s % K == 0) {
+      ck_cp[s / K] = c->cp;
+      ck_ap[s / K] = c->ap;
+      launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s / K, 2, c->cb, c->cp, c->ap, items);
+    }
+    RCHK(rd_substep(c));
+  }
+  int q = 0;
+  launch_rd_seed(c->stream, c->d_tis, rd.d, c->T, c->cp, c->cb, q, items);
+  const int mn = max_n(c);
+  const long long mb = rd_max_box(c);
+  for (int k = nck - 1; k >= 0; k--) {
+    c->cp = ck_cp[k];
+    c->ap = ck_ap[k];
+    launch_rd_state(c->stream, c->d_tis, rd.d, c->T, k, 3, c->cb, c->cp, c->ap, items);
+    const int s0 = k * K, s1 = std::min(n_sub, s0 + K);
+    for (int s = s0; s < s1; s++) {
+      launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s - s0, 0, c->cb, c->cp, c->ap, items);
+      RCHK(rd_substep(c));
+    }
+    for (int s = s1 - 1; s >= s0; s--) {
+      launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, mn, mb);
+      q ^= 1;
+    }
+  }
+  launch_rd_gj0(c->stream, c->d_tis, rd.d, c->T, c->cb, q, items);
+  HIPCHK(c, hipGetLastError());
+  RCHK(sync_status(c));
+  for (int t = 0; t < c->T; t++) {
+    double sc[4];
+    HIPCHK(c, hipMemcpy(sc, rd.h[t].scal, sizeof(sc), hipMemcpyDeviceToHost));
+    if (loss) loss[t] = sc[0];
+    if (grads) {
+      grads[t].du = sc[1];
+      grads[t].dp = sc[2];
+      grads[t].gj0 = sc[3];
+      if (grads[t].kpa)
+        HIPCHK(c, hipMemcpy(grads[t].kpa, rd.h[t].gkpa, 8 * (size_t)c->h_st[t].n, hipMemcpyDeviceToHost));
+    }
+  }
+  return EPI_OK;
+}
+
+int epi_rd_end(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  auto& rd = c->rd;
+  if (!rd.active) return EPI_OK;
+  for (int t = 0; t < c->T; t++) { c->tis[t].ca_du = 0.01; c->tis[t].ca_dp = 0.1; }
+  HIPCHK(c, hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * c->T, hipMemcpyHostToDevice, c->stream));
+  std::vector<double> g0(c->T, gj_init);
+  HIPCHK(c, hipMemcpyAsync(rd.d_gj0, g0.data(), 8 * c->T, hipMemcpyHostToDevice, c->stream));
+  const long long items = rd_items(c);
+  launch_rd_restore(c->stream, c->d_tis, rd.d, rd.d_gj0, c->T, c->cb, c->cp, c->ap, 0, items);
+  launch_rd_put_gj(c->stream, c->d_tis, rd.d, c->T, c->cb, items);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(rd.arena); rd.arena = nullptr;
+  if (rd.work) (void)hipFree(rd.work);
+  rd.work = nullptr; rd.work_bytes = 0;
+  (void)hipFree(rd.d); rd.d = nullptr;
+  (void)hipFree(rd.d_gj0); rd.d_gj0 = nullptr;
+  rd.active = false;
+  return EPI_OK;
+}

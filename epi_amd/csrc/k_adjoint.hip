@@ -1,0 +1,386 @@
+// k_adjoint.hip -- the frozen reaction-diffusion window (the calc_ca2p
+// sub-step, ca2p.cpp:392-399, with positions, lists, states and voxel maps
+// frozen) and its reverse pass.
+//
+// Forward: the window restores a snapshot of the tissue and runs the plain
+// sub-step kernels (k_ca_cells, k_atp) with the runtime coefficients in Tis.
+// Reverse: for the loss L = mean over AFM cells of (ca2p_T - target)^2 each
+// sub-step is differentiated with respect to its state (ca2p, IP3, ex_inert,
+// per-edge gj, ATP) and the parameters.  Values a sub-step never writes
+// (ca2p of DEAD cells, ATP planes from iz_bound up) are constants.  The
+// reverse pass scatters with FP64 atomics, so gradients are reproducible to
+// rounding, not bit for bit.
+#include "epi_dev.h"
+#include "launch.h"
+#include "ca_laws.h"
+#include "rd_dev.h"
+
+namespace epi {
+
+namespace {
+constexpr double STIM11 = 0.002, Kaa = 0.5, Da = 1.0, H0 = 0.5;
+
+__device__ __forceinline__ long long box_end(const Tis& t, const RdTis& r) { return (long long)r.izb * t.PL; }
+
+// grid point that a grid_avg8 corner reads: copy planes x = NX, y = NY hold
+// the values of x = 0, y = 0; planes from iz_bound up are constants (-1)
+__device__ __forceinline__ long long corner_var(const Tis& t, const RdTis& r, int j, int k, int l) {
+  if (l >= r.izb) return -1;
+  if (j == t.NX) j = 0;
+  if (k == t.NY) k = 0;
+  return gidx(t, j, k, l);
+}
+}  // namespace
+
+// restore the snapshot with the window's starting conductance (and kpa)
+__global__ void __launch_bounds__(256) k_rd_restore(const Tis* __restrict__ T, RdTis* __restrict__ R,
+                                                     const double* __restrict__ gj0, int cb, int cp, int ap,
+                                                     int use_kpa) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const double g0 = gj0[blockIdx.y];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns; i += stride) {
+    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
+    if (e < t.nlen[cb][c]) t.gj[cb][i] = g0;
+  }
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    t.ca2p[cp][c] = r.u0[0][c]; t.ca2p[1 - cp][c] = r.u0[1][c];
+    t.ip3[cp][c] = r.p0[0][c]; t.ip3[1 - cp][c] = r.p0[1][c];
+    t.ex_inert[c] = r.h0[c]; t.ca2p_avg[c] = r.avg0[c]; t.diff_u[c] = 0;
+    t.ca_kpa[c] = use_kpa ? r.kpa_in[c] : r.kpa0[c];
+  }
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < t.gsize; v += stride) {
+    t.atp[ap][v] = r.A0[0][v]; t.atp[1 - ap][v] = r.A0[1][v];
+  }
+}
+
+// take the snapshot (the reverse of k_rd_restore; gj rows are kept as they are)
+__global__ void __launch_bounds__(256) k_rd_snapshot(const Tis* __restrict__ T, RdTis* __restrict__ R, int cb,
+                                                      int cp, int ap) {
+  const Tis& t = T[blockIdx.y];
+  RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  int nafm = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns; i += stride) {
+    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
+    if (e < t.nlen[cb][c]) r.g0[i] = t.gj[cb][i];
+  }
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    r.u0[0][c] = t.ca2p[cp][c]; r.u0[1][c] = t.ca2p[1 - cp][c];
+    r.p0[0][c] = t.ip3[cp][c]; r.p0[1][c] = t.ip3[1 - cp][c];
+    r.h0[c] = t.ex_inert[c]; r.avg0[c] = t.ca2p_avg[c]; r.kpa0[c] = t.ca_kpa[c];
+    nafm += (c >= start && is_afm(t.state[c])) ? 1 : 0;
+  }
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < t.gsize; v += stride) {
+    r.A0[0][v] = t.atp[ap][v]; r.A0[1][v] = t.atp[1 - ap][v];
+  }
+  for (int o = 32; o > 0; o >>= 1) nafm += __shfl_down(nafm, o, 64);
+  if ((threadIdx.x & 63) == 0 && nafm) atomicAdd(&r.nafm, nafm);
+}
+
+// put back the gj rows of the snapshot (end of a window)
+__global__ void __launch_bounds__(256) k_rd_put_gj(const Tis* __restrict__ T, RdTis* __restrict__ R, int cb) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
+    if (e < t.nlen[cb][c]) t.gj[cb][i] = r.g0[i];
+  }
+}
+
+// copy the state between the tissue and a segment slot (dir 0: save,
+// 1: restore) or a checkpoint (both buffers; dir 2: save, 3: restore)
+__global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
+                                                   int dir, int cb, int cp, int ap) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const long long cap = t.cap, ns = t.ns;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const bool ck = dir >= 2, save = (dir & 1) == 0;
+  double* gs = (ck ? r.cg : r.sg) + slot * cap * ns;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * ns; i += stride) {
+    const int c = (int)(i / ns), e = (int)(i % ns);
+    if (e < t.nlen[cb][c]) {
+      if (save) gs[i] = t.gj[cb][i];
+      else t.gj[cb][i] = gs[i];
+    }
+  }
+  const int nb = ck ? 2 : 1;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    double* hs = (ck ? r.ch : r.sh) + slot * cap;
+    if (save) hs[c] = t.ex_inert[c];
+    else t.ex_inert[c] = hs[c];
+    for (int b = 0; b < nb; b++) {
+      const int buf = b ? 1 - cp : cp;
+      double* us = (ck ? r.cu + (2 * slot + b) * cap : r.su + slot * cap);
+      double* ps = (ck ? r.cp_ + (2 * slot + b) * cap : r.sp + slot * cap);
+      if (save) { us[c] = t.ca2p[buf][c]; ps[c] = t.ip3[buf][c]; }
+      else { t.ca2p[buf][c] = us[c]; t.ip3[buf][c] = ps[c]; }
+    }
+  }
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
+    for (int b = 0; b < nb; b++) {
+      const int buf = b ? 1 - ap : ap;
+      double* as = (ck ? r.cA + (2 * slot + b) * r.box : r.sA + slot * r.box);
+      if (save) as[v] = t.atp[buf][v];
+      else t.atp[buf][v] = as[v];
+    }
+  }
+}
+
+// the loss and the seed of the reverse pass
+__global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTis* __restrict__ R, int cp, int cb,
+                                                  int q) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  double part = 0;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    double s = 0;
+    if (c >= start && is_afm(t.state[c])) {
+      const double d = t.ca2p[cp][c] - r.target[c];
+      part += d * d;
+      s = 2.0 * d / r.nafm;
+    }
+    r.lu[q][c] = s; r.lp[q][c] = 0; r.lh[q][c] = 0; r.gkpa[c] = 0;
+  }
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns; i += stride) {
+    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
+    if (e < t.nlen[cb][c]) r.lg[q][i] = 0;
+  }
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) r.lA[q][v] = 0;
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+  if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[0], part / r.nafm);
+}
+
+// clear the accumulators of the state before a sub-step; gather the ATP
+// adjoints of each AFM cell's voxels (its fa source term)
+__global__ void __launch_bounds__(256) k_rd_zero(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    r.lu[1 - q][c] = 0; r.lp[1 - q][c] = 0; r.lh[1 - q][c] = 0; r.accA[c] = 0;
+  }
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) r.lA[1 - q][v] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_rd_acc(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const long long end = box_end(t, r);
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+       v += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(v % t.XP), k = (int)((v / t.XP) % (t.NY + 1));
+    if (j >= t.NX || k >= t.NY) continue;
+    const unsigned w = t.vox[v] & 0x7fffffffu;
+    if (w) atomicAdd(&r.accA[w - 1], r.lA[q][v]);
+  }
+}
+
+// reverse of dead_IP3_calc and supra_calc for one sub-step: one thread per
+// cell, the state before the sub-step from slot `slot`
+__global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
+                                                   int cb, int bp, int q) {
+  const Tis& t = T[blockIdx.y];
+  RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  const long long cap = t.cap, ns = t.ns;
+  const double* u = r.su + slot * cap;
+  const double* p = r.sp + slot * cap;
+  const double* h = r.sh + slot * cap;
+  const double* g = r.sg + slot * cap * ns;
+  const double* A = r.sA + slot * r.box;
+  const double *lu = r.lu[q], *lp = r.lp[q], *lh = r.lh[q], *lg = r.lg[q];
+  double *nu = r.lu[1 - q], *np = r.lp[1 - q], *nh = r.lh[1 - q], *ng = r.lg[1 - q], *nA = r.lA[1 - q];
+  const double du = t.ca_du, dp = t.ca_dp;
+  double pdu = 0, pdp = 0;
+  for (int c = start + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint8_t s = t.state[c];
+    const int* row = t.col[cb] + (size_t)c * ns;
+    const uint8_t* kind = t.ca_kind + (size_t)c * ns;
+    const int len = t.nlen[cb][c];
+    if (s == DEAD) {
+      int cnt = 0;
+      double sum = 0;
+      for (int e = 0; e < len; e++)
+        if (kind[e] == 3) { sum += p[row[e]]; cnt++; }
+      atomicAdd(&np[c], lp[c] * (1.0 - DT_Ca * dp * cnt - DT_Ca * Kpp));
+      for (int e = 0; e < len; e++)
+        if (kind[e] == 3) atomicAdd(&np[row[e]], lp[c] * DT_Ca * dp);
+      pdp += lp[c] * DT_Ca * (sum - cnt * p[c]);
+      continue;
+    }
+    if (!is_afm(s)) continue;
+    const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
+    const double* gr = g + (size_t)c * ns;
+    const double uc = u[c], pc = p[c], hc = h[c];
+    double su = 0, sp = 0, gsum = 0;
+    for (int e = 0; e < len; e++) {
+      if (!kind[e]) continue;
+      const int o = row[e];
+      su += gr[e] * (u[o] - uc);
+      sp += gr[e] * (p[o] - pc);
+      gsum += gr[e];
+    }
+    const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
+    double Aavg = 0;
+    {
+      // A holds planes 0..izb of the grid in the device layout
+      const long long b0 = gidx(t, ix, iy, iz);
+      Aavg = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
+                      + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
+    }
+    const double iag = t.ca_iag[c], kpa = t.ca_kpa[c], th = t.ca_th[c];
+    const double diffu = ca2p_reaction_factor(uc, hc, pc, Bavg) + du * iag * su;
+    const double kflux = 8.1, mu0 = 0.567, mu1 = 0.1, kmu = 0.05, pb = 0.11, pbb = 0.89, pk1 = 0.7;
+    const double kg = 0.1, gam = 2.0, k2 = 0.7;
+    const double e1 = mu0 + mu1 * pc / (pc + kmu), e2 = pb + pbb * uc / (pk1 + uc);
+    const double dRdu = kflux * e1 * (pbb * pk1 / ((pk1 + uc) * (pk1 + uc))) * hc - gam * kg / ((kg + uc) * (kg + uc));
+    const double dRdh = kflux * e1 * e2;
+    const double dRdp = kflux * (mu1 * kmu / ((pc + kmu) * (pc + kmu))) * e2 * hc;
+    const double abar = DT_Ca * lu[c] + (diffu > 0 ? r.accA[c] * DT_Ca * STIM11 : 0.0);
+    const double qq = k2 * k2 + uc * uc;
+    double own_u = lu[c] + abar * (dRdu - du * iag * gsum) + lh[c] * DT_Ca * (-2.0 * uc * k2 * k2 / (qq * qq)) / th;
+    for (int e = 0; e < len; e++) {
+      if (!kind[e]) continue;
+      const int o = row[e];
+      double to_u = abar * du * iag * gr[e];
+      double lge = lg[(size_t)c * ns + e];
+      const double lg_in = lge;
+      if (kind[e] == 3) {
+        const double d = u[o] - uc;
+        const double tz = tanh((0.1 - fabs(d)) / 0.1);
+        const double fwd = -5.0 * (1.0 - tz * tz);
+        const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
+        to_u += lg_in * DT_Ca * fwd * sg;
+        own_u -= lg_in * DT_Ca * fwd * sg;
+        lge = lg_in * (1.0 - DT_Ca);
+      }
+      ng[(size_t)c * ns + e] = lge + abar * du * iag * (u[o] - uc) + lp[c] * DT_Ca * dp * iag * (p[o] - pc);
+      if (is_afm(t.state[o])) atomicAdd(&nu[o], to_u);
+      atomicAdd(&np[o], lp[c] * DT_Ca * dp * iag * gr[e]);
+    }
+    atomicAdd(&nu[c], own_u);
+    atomicAdd(&np[c], lp[c] * (1.0 - DT_Ca * Kpp - DT_Ca * dp * iag * gsum) + abar * dRdp);
+    atomicAdd(&nh[c], lh[c] * (1.0 - DT_Ca / th) + abar * dRdh);
+    const double wA = lp[c] * DT_Ca * kpa * H0 / ((H0 + Aavg) * (H0 + Aavg)) * 0.125;
+    for (int q8 = 0; q8 < 8; q8++) {
+      const long long vv = corner_var(t, r, ix + (q8 & 1), iy + ((q8 >> 1) & 1), iz + ((q8 >> 2) & 1));
+      if (vv >= 0) atomicAdd(&nA[vv], wA);
+    }
+    pdu += abar * iag * su;
+    pdp += lp[c] * DT_Ca * iag * sp;
+    r.gkpa[c] += lp[c] * DT_Ca * Aavg / (H0 + Aavg);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    pdu += __shfl_down(pdu, o, 64);
+    pdp += __shfl_down(pdp, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (pdu != 0) atomicAdd(&r.scal[1], pdu);
+    if (pdp != 0) atomicAdd(&r.scal[2], pdp);
+  }
+}
+
+// reverse of ATP_refresh: each point of the box gathers from the points
+// whose update read it (periodic x, y; mirror z: prev_z(0) = 1,
+// next_z(NZ - 1) = NZ - 1)
+__global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const long long end = box_end(t, r);
+  const double* lA = r.lA[q];
+  double* nA = r.lA[1 - q];
+  const uint8_t* m = t.cmap2;
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ, izb = r.izb;
+  const double cD = DT_Ca * Da * t.inv_dx * t.inv_dx;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+       v += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(v % t.XP), k = (int)((v / t.XP) % (NY + 1)), l = (int)(v / t.PL);
+    if (j >= NX || k >= NY) continue;
+    const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
+    const int py = k == 0 ? NY - 1 : k - 1, ny = k == NY - 1 ? 0 : k + 1;
+    const int pz = l == 0 ? 1 : l - 1, nz = l + 1 == NZ ? NZ - 1 : l + 1;
+    const double msum = (double)m[gidx(t, px, k, l)] + m[gidx(t, nx, k, l)] + m[gidx(t, j, py, l)]
+                        + m[gidx(t, j, ny, l)] + m[gidx(t, j, k, pz)] + m[gidx(t, j, k, nz)];
+    const double mv = (double)m[v];
+    double acc = lA[v] * (1.0 - cD * msum - DT_Ca * Kaa);
+    // in-plane readers of v: its four periodic neighbours
+    acc += (lA[gidx(t, px, k, l)] + lA[gidx(t, nx, k, l)] + lA[gidx(t, j, py, l)] + lA[gidx(t, j, ny, l)]) * cD * mv;
+    // z readers: w = l - 1 reads v as next_z, w = l + 1 reads v as prev_z;
+    // w = 0 reads plane 1 twice; w = NZ - 1 reads itself as next_z
+    if (l >= 1) acc += lA[gidx(t, j, k, l - 1)] * cD * mv;
+    if (l + 1 < izb) acc += lA[gidx(t, j, k, l + 1)] * cD * mv;
+    if (l == 1) acc += lA[gidx(t, j, k, 0)] * cD * mv;
+    if (l == NZ - 1) acc += lA[v] * cD * mv;
+    nA[v] += acc;
+  }
+}
+
+// dL/dgj0: the adjoints of every coupled edge at the window's start
+__global__ void __launch_bounds__(256) k_rd_gj0(const Tis* __restrict__ T, RdTis* __restrict__ R, int cb, int q) {
+  const Tis& t = T[blockIdx.y];
+  RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  double part = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
+    if (c >= start && e < t.nlen[cb][c] && is_afm(t.state[c]) && t.ca_kind[i]) part += r.lg[q][i];
+  }
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+  if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[3], part);
+}
+
+// ---- launchers -------------------------------------------------------------
+namespace {
+dim3 cells_grid(long long items, int ntis) {
+  long long b = (items + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return dim3((unsigned)b, ntis);
+}
+}  // namespace
+
+void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items) {
+  hipLaunchKernelGGL(k_rd_snapshot, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, cp, ap);
+}
+void launch_rd_restore(hipStream_t s, const Tis* dT, RdTis* dR, const double* gj0, int ntis, int cb, int cp, int ap,
+                       int use_kpa, long long items) {
+  hipLaunchKernelGGL(k_rd_restore, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, gj0, cb, cp, ap, use_kpa);
+}
+void launch_rd_put_gj(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, long long items) {
+  hipLaunchKernelGGL(k_rd_put_gj, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb);
+}
+void launch_rd_state(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int dir, int cb, int cp, int ap,
+                     long long items) {
+  hipLaunchKernelGGL(k_rd_state, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, slot, dir, cb, cp, ap);
+}
+void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, int cb, int q, long long items) {
+  hipLaunchKernelGGL(k_rd_seed, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cp, cb, q);
+}
+void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
+                                int max_n, long long max_box) {
+  hipLaunchKernelGGL(k_rd_zero, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q);
+  hipLaunchKernelGGL(k_rd_acc, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
+  hipLaunchKernelGGL(k_rd_cells, cells_grid(max_n, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q);
+  hipLaunchKernelGGL(k_rd_grid, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
+}
+void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items) {
+  hipLaunchKernelGGL(k_rd_gj0, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, q);
+}
+
+}  // namespace epi

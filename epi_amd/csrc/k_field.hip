@@ -13,6 +13,7 @@
 #include "epi_dev.h"
 #include "launch.h"
 #include "epi_group.h"
+#include "ca_laws.h"
 
 namespace epi {
 
@@ -193,35 +194,6 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, int
 }
 
 // ---- calc_ca2p ------------------------------------------------------------
-constexpr double Kpp = 0.3;
-constexpr double dp_c = 0.1;
-
-__device__ __forceinline__ double ca2p_reaction_factor(double u, double ER3, double p, double B) {   // :36-87
-  const double kbc = 0.4 * 1.2, Hb = 0.01, Cout = 1.0;
-  const double kg = 0.1, gamma = 2.0;
-  const double mu0 = 0.567, mu1 = 0.1, kmu = 0.05;
-  const double para_b = 0.11, para_bb = 0.89, para_k1 = 0.7;
-  const double k_flux = 8.1, beta_zero = 0.02, CA_OUT = 1.0;
-  const double leak_from_storage = CA_OUT * beta_zero;
-  const double er1 = (mu0 + mu1 * p / (p + kmu));
-  const double er2 = (para_b + para_bb * u / (para_k1 + u));
-  const double ca2p_from_storage = k_flux * er1 * er2 * ER3;
-  const double into = gamma * u / (kg + u);
-  const double by_ext = kbc * B * B * Cout / (Hb + B * B);
-  return ca2p_from_storage - into + leak_from_storage + by_ext;
-}
-
-__device__ __forceinline__ double grid_avg8(const Tis& t, const double* g, int ix, int iy, int iz) {   // utils.h:18-23
-  return 0.125 * (g[gidx(t, ix, iy, iz)] + g[gidx(t, ix + 1, iy, iz)] + g[gidx(t, ix, iy + 1, iz)]
-                  + g[gidx(t, ix, iy, iz + 1)] + g[gidx(t, ix + 1, iy + 1, iz)] + g[gidx(t, ix + 1, iy, iz + 1)]
-                  + g[gidx(t, ix, iy + 1, iz + 1)] + g[gidx(t, ix + 1, iy + 1, iz + 1)]);
-}
-
-__device__ __forceinline__ int ca_iz_bound(const Tis& t) {   // :386
-  const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
-  return (int)((zzmax + FAC_MAP * R_max) / t.dz);
-}
-
 // init_ca2p_avg (:20-27) and the per-cell "touches AIR" flag used by
 // init_ca2p_map (:262-268)
 __global__ void __launch_bounds__(256) k_ca_begin(const Tis* __restrict__ T, const int* tlist, int cb, int zero_avg) {
@@ -287,9 +259,6 @@ __global__ void __launch_bounds__(256) k_ca_vox(const Tis* __restrict__ T, const
 // grid_avg8; the leader sums them in list / corner order.
 constexpr int C_BLOCK = 128;
 
-__device__ __forceinline__ double fw_term(double diff, double w) {   // fw (:138-144)
-  return (1. - w) + (-1. + tanh((0.1 - diff) / 0.1)) / 2.;
-}
 
 // the serial form of one cell, for lists longer than the LDS slots
 template <int CG>
@@ -331,7 +300,7 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
             if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
         wave_sync();
       }
-      if (lane == 0) t.ip3[1 - cp][c] = p + DT_Ca * (dp_c * (tmp - count * p) - Kpp * p);
+      if (lane == 0) t.ip3[1 - cp][c] = p + DT_Ca * (t.ca_dp * (tmp - count * p) - Kpp * p);
       continue;
     }
     // the leader's per-cell values are fetched now, beside the lanes' loads
@@ -374,14 +343,14 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
       wave_sync();
     }
     if (lane == 0) {
-      const double ca2p_du = 0.01;
+      const double ca2p_du = t.ca_du;
       const double* q = s_g[grp];
       const double Bavg = 0.125 * (q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7]);
       const double Aavg = 0.125 * (q[8] + q[9] + q[10] + q[11] + q[12] + q[13] + q[14] + q[15]);
       const double diff_u = ca2p_reaction_factor(u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
       t.diff_u[c] = diff_u;
       const double H0 = 0.5;
-      t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + dp_c * IAGv * tmp_IP3);
+      t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + t.ca_dp * IAGv * tmp_IP3);
       const double cs = u + DT_Ca * diff_u;
       t.ca2p[1 - cp][c] = cs;
       t.ca2p_avg[c] = avg + cs;

@@ -51,4 +51,16 @@ void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis,
 void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
+// frozen reaction-diffusion window (k_adjoint.hip)
+struct RdTis;
+void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items);
+void launch_rd_restore(hipStream_t s, const Tis* dT, RdTis* dR, const double* gj0, int ntis, int cb, int cp, int ap,
+                       int use_kpa, long long items);
+void launch_rd_put_gj(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, long long items);
+void launch_rd_state(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int dir, int cb, int cp, int ap,
+                     long long items);
+void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, int cb, int q, long long items);
+void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
+                                int max_n, long long max_box);
+void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items);
 }  // namespace epi

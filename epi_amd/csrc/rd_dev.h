@@ -1,0 +1,33 @@
+// rd_dev.h -- device records of the frozen reaction-diffusion window and
+// its adjoint (k_adjoint.hip, host.hip).
+//
+// A window is the sub-step of calc_ca2p (ca2p.cpp:392-399) run n_sub times
+// from a snapshot of the tissue, with positions, lists, states and voxel maps
+// frozen and four parameter groups made runtime values: the Ca2+ and IP3
+// gap-junction coefficients (Tis::ca_du, Tis::ca_dp), the starting
+// conductance of every edge, and the per-cell IP3 production Tis::ca_kpa.
+#pragma once
+#include "epi_dev.h"
+
+namespace epi {
+
+struct RdTis {
+  int nafm;                    // AFM cells of the tissue (loss normaliser)
+  int izb;                     // iz_bound of the window
+  long long box;               // grid entries of planes 0..izb (saved per state)
+  // snapshot: [0] = the current buffer, [1] = the other one
+  double *u0[2], *p0[2], *A0[2];
+  double *h0, *avg0, *kpa0, *g0;
+  // saved states of one segment (slot-major) and the checkpoints
+  double *su, *sp, *sh, *sg, *sA;
+  double *cu, *cp_, *ch, *cg, *cA;   // checkpoint-major, two buffers each for u, p, A
+  // adjoint: [q] = the state after the sub-step, [1-q] = before it
+  double *lu[2], *lp[2], *lh[2], *lg[2], *lA[2];
+  double *accA;                // per-cell sum of the ATP adjoints of its voxels
+  double *gkpa;                // dL/dkpa per cell
+  double* scal;                // [0] loss, [1] dL/ddu, [2] dL/ddp, [3] dL/dgj0
+  double* target;              // per-cell target ca2p
+  double* kpa_in;              // per-cell kpa for the next run (set by the host)
+};
+
+}  // namespace epi

@@ -96,10 +96,19 @@ SYMBOLS = [
     "epi_set_state", "epi_set_lists", "epi_set_maps", "epi_profile_enable",
     "epi_profile_read", "epi_profile_reset", "epi_profile_kernel_count",
     "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n",
+    "epi_rd_begin", "epi_rd_forward", "epi_rd_get_ca2p", "epi_rd_adjoint", "epi_rd_end",
 ]
 
 # launch-shape keys of epi_set_tuning (include/epi_amd.h)
 TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4}
+
+class RdParams(C.Structure):
+    _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", C.POINTER(C.c_double))]
+
+
+class RdGrad(C.Structure):
+    _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", C.POINTER(C.c_double))]
+
 
 _lib = None
 
@@ -148,6 +157,12 @@ def load_library():
     L.epi_profile_kernel_count.argtypes = []
     L.epi_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
     L.epi_calc_ca2p_n.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
+    L.epi_rd_begin.argtypes = [vp, C.POINTER(C.c_double)]
+    L.epi_rd_forward.argtypes = [vp, C.POINTER(RdParams), C.c_int32]
+    L.epi_rd_get_ca2p.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.epi_rd_adjoint.argtypes = [vp, C.POINTER(RdParams), C.c_int32, C.c_int32,
+                                 C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_double), C.POINTER(RdGrad)]
+    L.epi_rd_end.argtypes = [vp]
     L.epi_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     _lib = L
     return L
@@ -317,6 +332,46 @@ class TissueSet:
 
     def profile_reset(self):
         self._chk(self.L.epi_profile_reset(self.h), "profile_reset")
+
+    # --- frozen reaction-diffusion window ---
+    def _rd_params(self, params):
+        arr = (RdParams * self.T)()
+        keep = []
+        for t, p in enumerate(params):
+            k = p.get("kpa")
+            if k is not None:
+                k = np.ascontiguousarray(k, dtype=np.float64)
+                keep.append(k)
+            arr[t] = RdParams(p.get("du", 0.01), p.get("dp", 0.1), p.get("gj0", 0.99),
+                              None if k is None else _ptr(k, _dp))
+        return arr, keep
+
+    def rd_begin(self, zzmax):
+        self._chk(self.L.epi_rd_begin(self.h, self._zz(zzmax)), "rd_begin")
+
+    def rd_forward(self, params, n_sub):
+        arr, keep = self._rd_params(params)
+        self._chk(self.L.epi_rd_forward(self.h, arr, int(n_sub)), "rd_forward")
+
+    def rd_get_ca2p(self, t):
+        out = np.zeros(self.status(t)[0].n)
+        self._chk(self.L.epi_rd_get_ca2p(self.h, t, _ptr(out, _dp)), "rd_get_ca2p")
+        return out
+
+    def rd_adjoint(self, params, n_sub, checkpoint_every, targets):
+        arr, keep = self._rd_params(params)
+        tg = [np.ascontiguousarray(x, dtype=np.float64) for x in targets]
+        tp = (C.POINTER(C.c_double) * self.T)(*[_ptr(x, _dp) for x in tg])
+        loss = np.zeros(self.T)
+        gk = [np.zeros(self.status(t)[0].n) for t in range(self.T)]
+        grads = (RdGrad * self.T)(*[RdGrad(0, 0, 0, _ptr(g, _dp)) for g in gk])
+        self._chk(self.L.epi_rd_adjoint(self.h, arr, int(n_sub), int(checkpoint_every), tp, _ptr(loss, _dp), grads),
+                  "rd_adjoint")
+        return loss, [{"du": grads[t].du, "dp": grads[t].dp, "gj0": grads[t].gj0, "kpa": gk[t]}
+                      for t in range(self.T)]
+
+    def rd_end(self):
+        self._chk(self.L.epi_rd_end(self.h), "rd_end")
 
     # --- launch shapes ---
     def set_tuning(self, tuning):

@@ -175,6 +175,36 @@ int epi_profile_read(epi_ctx* ctx, int32_t kernel, const char** name, double* to
 int epi_profile_reset(epi_ctx* ctx);
 int32_t epi_profile_kernel_count(void);
 
+/* ---- frozen reaction-diffusion window and its adjoint --------------------
+ * No counterpart in the reference (it has no gradients).  A window is the
+ * sub-step of calc_ca2p (ca2p.cpp:392-399) run n_sub times from a snapshot
+ * of every tissue, with positions, lists, states and voxel maps frozen, and
+ * with the parameters below as runtime values.  With du = 0.01, dp = 0.1,
+ * kpa = NULL (calc_Kpa) and gj0 = 0.99 on fresh edges a window is
+ * calc_ca2p's loop.  Division and removal are outside a window (they are not
+ * differentiable).  epi_rd_end restores the snapshot. */
+typedef struct epi_rd_params {
+    double du;          /* ca2p_du, the Ca2+ gap-junction coefficient (ca2p.cpp:183) */
+    double dp;          /* dp, the IP3 coefficient (ca2p.cpp:17; supra_calc and dead_IP3_calc) */
+    double gj0;         /* starting conductance of every directed edge (gj_init) */
+    const double* kpa;  /* per-cell IP3 production (calc_Kpa, ca2p.cpp:101-110); NULL = from agek */
+} epi_rd_params;
+typedef struct epi_rd_grad {
+    double du, dp, gj0;
+    double* kpa;        /* n doubles filled when non-NULL */
+} epi_rd_grad;
+/* freeze the current state of every tissue; zzmax[t] sets iz_bound as in calc_ca2p */
+int epi_rd_begin(epi_ctx* ctx, const double* zzmax);
+/* n_sub sub-steps from the snapshot with params[t] per tissue */
+int epi_rd_forward(epi_ctx* ctx, const epi_rd_params* params, int32_t n_sub);
+/* ca2p of every cell after the last forward (n doubles) */
+int epi_rd_get_ca2p(epi_ctx* ctx, int32_t tissue, double* out);
+/* loss[t] = mean over AFM cells of (ca2p after n_sub - targets[t])^2 and
+ * its gradient; recompute from checkpoints every checkpoint_every sub-steps */
+int epi_rd_adjoint(epi_ctx* ctx, const epi_rd_params* params, int32_t n_sub, int32_t checkpoint_every,
+                   const double* const* targets, double* loss, epi_rd_grad* grads);
+int epi_rd_end(epi_ctx* ctx);
+
 /* ---- launch shapes -------------------------------------------------------
  * No counterpart in the reference.  Launch-shape knobs of the grid kernels,
  * chosen per workload by epi_amd/autotune.py; they change timing only, never

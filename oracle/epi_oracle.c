@@ -1523,3 +1523,334 @@ int eo_set_lists(eo_tissue* t, const int32_t* conn_ptr, const int32_t* conn, con
 }
 
 double eo_p_diff_x(const eo_tissue* t, double a, double b) { return p_diff_x(t, a, b); }
+
+/* ==== frozen reaction-diffusion window and its adjoint ====================
+ * The sub-step of calc_ca2p (ca2p.cpp:392-399: dead_IP3_calc, supra_calc,
+ * ATP_refresh, update_values) run n_sub times with positions, lists, states
+ * and voxel maps frozen at the tissue's current state, and with four
+ * parameter groups made runtime values:
+ *   du   the Ca2+ gap-junction coefficient ca2p_du (0.01, ca2p.cpp:183)
+ *   dp   the IP3 coefficient dp (0.1, ca2p.cpp:17), in supra_calc and
+ *        dead_IP3_calc
+ *   gj0  the starting conductance of every directed edge (gj_init = 0.99)
+ *   kpa  per-cell IP3 production rate (calc_Kpa, ca2p.cpp:101-110)
+ * With du = 0.01, dp = 0.1, kpa = calc_Kpa and gj0 = 0.99 on fresh edges a
+ * window is calc_ca2p's loop.  The adjoint is the reverse pass of the same
+ * arithmetic for the loss L = mean over AFM cells of (ca2p_T - target)^2.
+ * Values the window never writes (ca2p of DEAD cells, grid planes from
+ * iz_bound up, both alternating between the two buffers) are constants. */
+
+typedef struct {
+    int n, start, NX, NY, NZ, izb;
+    size_t gsz;
+    int32_t* ptr; int32_t* col; uint8_t* kind;    /* CSR lists, 0 / 1 coupled / 3 coupled ALIVE */
+    double *th, *iag, *kpa0;
+    int32_t* owner; uint8_t* air;                 /* voxel -> AFM cell, air flag */
+    double *u0[2], *p0[2], *h0, *A0[2];
+    const double* B;
+} rd_win;
+
+static void rd_free(rd_win* w) {
+    free(w->ptr); free(w->col); free(w->kind); free(w->th); free(w->iag); free(w->kpa0);
+    free(w->owner); free(w->air);
+    for (int b = 0; b < 2; b++) { free(w->u0[b]); free(w->p0[b]); free(w->A0[b]); }
+    free(w->h0);
+}
+
+static int rd_setup(eo_tissue* t, double zzmax, rd_win* w) {
+    memset(w, 0, sizeof(*w));
+    w->n = t->n; w->start = t->nmemb + t->nder;
+    w->NX = t->NX; w->NY = t->NY; w->NZ = t->NZ;
+    w->izb = ca_iz_bound(t, zzmax);
+    if (w->izb > t->NZ || w->izb < 0) return EO_ERR_OUT_OF_GRID;
+    w->gsz = t->gsz;
+    const int n = t->n;
+    w->ptr = xcalloc(n + 1, 4);
+    for (int c = 0; c < n; c++) w->ptr[c + 1] = w->ptr[c] + t->nconn[c];
+    w->col = xcalloc(w->ptr[n], 4); w->kind = xcalloc(w->ptr[n], 1);
+    for (int c = 0; c < n; c++) {
+        int32_t* row = conn_row(t, c);
+        for (int e = 0; e < t->nconn[c]; e++) {
+            int o = row[e], so = t->state[o];
+            w->col[w->ptr[c] + e] = o;
+            w->kind[w->ptr[c] + e] = so == EO_ALIVE ? 3 : (so == EO_DEAD || so == EO_FIX || so == EO_MUSUME) ? 1 : 0;
+        }
+    }
+    w->th = xcalloc(n, 8); w->iag = xcalloc(n, 8); w->kpa0 = xcalloc(n, 8);
+    for (int c = w->start; c < n; c++) {
+        if (!is_afm(t->state[c])) continue;
+        int al = t->state[c] == EO_ALIVE;
+        w->th[c] = calc_th(al, t->agek[c]); w->iag[c] = calc_IAG(al, t->agek[c]); w->kpa0[c] = calc_Kpa(al, t->agek[c]);
+    }
+    ca_maps m;
+    m.iz_bound = w->izb;
+    w->owner = m.diffu_cell = (int32_t*)malloc(sizeof(int32_t) * t->gsz);
+    w->air = m.air = (uint8_t*)malloc(t->gsz);
+    init_ca2p_map(t, &m);
+    for (int b = 0; b < 2; b++) {
+        w->u0[b] = xcalloc(n, 8); w->p0[b] = xcalloc(n, 8); w->A0[b] = xcalloc(t->gsz, 8);
+        memcpy(w->u0[b], t->ca_buf[b ? 1 - t->ca_first : t->ca_first], 8 * (size_t)n);
+        memcpy(w->p0[b], t->ip3_buf[b ? 1 - t->ca_first : t->ca_first], 8 * (size_t)n);
+        memcpy(w->A0[b], t->atp[b ? 1 - t->atp_first : t->atp_first], 8 * t->gsz);
+    }
+    w->h0 = xcalloc(n, 8);
+    memcpy(w->h0, t->ex_inert, 8 * (size_t)n);
+    w->B = t->bst[t->b_first];
+    return EO_OK;
+}
+
+typedef struct { double *u[2], *p[2], *h, *g, *A[2], *du; int cur; } rd_state;
+
+static void rd_state_init(const rd_win* w, rd_state* s, double gj0) {
+    for (int b = 0; b < 2; b++) {
+        s->u[b] = xcalloc(w->n, 8); s->p[b] = xcalloc(w->n, 8); s->A[b] = xcalloc(w->gsz, 8);
+        memcpy(s->u[b], w->u0[b], 8 * (size_t)w->n);
+        memcpy(s->p[b], w->p0[b], 8 * (size_t)w->n);
+        memcpy(s->A[b], w->A0[b], 8 * w->gsz);
+    }
+    s->h = xcalloc(w->n, 8); memcpy(s->h, w->h0, 8 * (size_t)w->n);
+    s->g = xcalloc(w->ptr[w->n], 8);
+    for (int e = 0; e < w->ptr[w->n]; e++) s->g[e] = gj0;
+    s->du = xcalloc(w->n, 8);
+    s->cur = 0;
+}
+static void rd_state_free(rd_state* s) {
+    for (int b = 0; b < 2; b++) { free(s->u[b]); free(s->p[b]); free(s->A[b]); }
+    free(s->h); free(s->g); free(s->du);
+}
+
+static inline double rd_kpa(const rd_win* w, const eo_rd_params* P, int c) { return P->kpa ? P->kpa[c] : w->kpa0[c]; }
+
+/* one sub-step in the order of ca2p.cpp:392-399 */
+static void rd_substep(eo_tissue* t, const rd_win* w, const eo_rd_params* P, rd_state* s) {
+    const double* u = s->u[s->cur]; double* un = s->u[1 - s->cur];
+    const double* p = s->p[s->cur]; double* pn = s->p[1 - s->cur];
+    const double* A = s->A[s->cur]; double* An = s->A[1 - s->cur];
+    for (int c = w->start; c < w->n; c++) {          /* dead_IP3_calc */
+        if (t->state[c] != EO_DEAD) continue;
+        int count = 0; double tmp = 0;
+        for (int e = w->ptr[c]; e < w->ptr[c + 1]; e++)
+            if (w->kind[e] == 3) { tmp += p[w->col[e]]; count++; }
+        tmp = DT_Ca * (P->dp * (tmp - count * p[c]) - Kpp * p[c]);
+        pn[c] = p[c] + tmp;
+    }
+    for (int c = 0; c < w->n; c++) s->du[c] = 0;
+    for (int c = w->start; c < w->n; c++) {          /* supra_calc */
+        if (!is_afm(t->state[c])) continue;
+        const int ix = t->lat[3 * c], iy = t->lat[3 * c + 1], iz = t->lat[3 * c + 2];
+        double su = 0, sp = 0;
+        for (int e = w->ptr[c]; e < w->ptr[c + 1]; e++) {
+            if (!w->kind[e]) continue;
+            const int o = w->col[e];
+            su += s->g[e] * (u[o] - u[c]);
+            sp += s->g[e] * (p[o] - p[c]);
+            if (w->kind[e] == 3) s->g[e] += DT_Ca * fw(fabs(u[o] - u[c]), s->g[e]);
+        }
+        s->du[c] = ca2p_reaction_factor(u[c], s->h[c], p[c], grid_avg8(t, w->B, ix, iy, iz)) + P->du * w->iag[c] * su;
+        pn[c] = p[c] + DT_Ca * (IP3_default_diff(rd_kpa(w, P, c), grid_avg8(t, A, ix, iy, iz), p[c]) + P->dp * w->iag[c] * sp);
+        un[c] = u[c] + DT_Ca * s->du[c];
+        s->h[c] += DT_Ca * ex_inert_diff(u[c], s->h[c], w->th[c]);
+    }
+    /* ATP_refresh */
+    const double Da = 1.0, AIR_STIM = 0.1;
+    const uint8_t* cm2 = t->cmap2;
+    const double inv_dx = t->inv_dx;
+    for (int j = 0; j < w->NX; j++) {
+        const int px = t->px_prev[j], nx = t->px_next[j];
+        for (int k = 0; k < w->NY; k++) {
+            const int py = t->py_prev[k], ny = t->py_next[k];
+            for (int l = 0; l < w->izb; l++) {
+                const int pz = t->pz_prev[l], nz = t->pz_next[l];
+                const size_t v = IDX3(t, j, k, l);
+                const double a = A[v];
+                const int dc = w->owner[v];
+                const double diffu = dc >= 0 ? s->du[dc] : 0.0;
+                An[v] = a + DT_Ca * (Da * ((double)cm2[IDX3(t, px, k, l)] * (A[IDX3(t, px, k, l)] - a)
+                                           + (double)cm2[IDX3(t, nx, k, l)] * (A[IDX3(t, nx, k, l)] - a)
+                                           + (double)cm2[IDX3(t, j, py, l)] * (A[IDX3(t, j, py, l)] - a)
+                                           + (double)cm2[IDX3(t, j, ny, l)] * (A[IDX3(t, j, ny, l)] - a)
+                                           + (double)cm2[IDX3(t, j, k, pz)] * (A[IDX3(t, j, k, pz)] - a)
+                                           + (double)cm2[IDX3(t, j, k, nz)] * (A[IDX3(t, j, k, nz)] - a))
+                                     * inv_dx * inv_dx + fa(diffu, a) + (double)w->air[v] * AIR_STIM);
+            }
+        }
+    }
+    for (int l = 0; l <= w->izb; l++) {
+        for (int j = 0; j <= w->NX; j++) An[IDX3(t, j, w->NY, l)] = An[IDX3(t, j, 0, l)];
+        for (int k = 0; k <= w->NY; k++) An[IDX3(t, w->NX, k, l)] = An[IDX3(t, 0, k, l)];
+    }
+    s->cur = 1 - s->cur;
+}
+
+int eo_rd_forward(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, double* ca2p_out) {
+    rd_win w; rd_state s;
+    int rc = rd_setup(t, zzmax, &w);
+    if (rc) { rd_free(&w); return rc; }
+    rd_state_init(&w, &s, P->gj0);
+    for (int i = 0; i < n_sub; i++) rd_substep(t, &w, P, &s);
+    if (ca2p_out) memcpy(ca2p_out, s.u[s.cur], 8 * (size_t)w.n);
+    rd_state_free(&s); rd_free(&w);
+    return EO_OK;
+}
+
+/* map a grid corner read by grid_avg8 to the variable it reads: the copy
+ * planes x = NX and y = NY hold the values of x = 0 and y = 0 written by the
+ * last ATP_refresh; planes from iz_bound up are constants (-1) */
+static inline long rd_corner_var(const eo_tissue* t, const rd_win* w, int j, int k, int l) {
+    if (l >= w->izb) return -1;
+    if (j == w->NX) j = 0;
+    if (k == w->NY) k = 0;
+    return (long)IDX3(t, j, k, l);
+}
+
+int eo_rd_adjoint(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, const double* target,
+                  double* loss, eo_rd_grad* G) {
+    rd_win w; rd_state s;
+    int rc = rd_setup(t, zzmax, &w);
+    if (rc) { rd_free(&w); return rc; }
+    const int n = w.n, E = w.ptr[n];
+    const size_t gsz = w.gsz;
+    rd_state_init(&w, &s, P->gj0);
+    /* forward, keeping every state */
+    double** Us = malloc(sizeof(double*) * (n_sub + 1));
+    double** Ps = malloc(sizeof(double*) * (n_sub + 1));
+    double** Hs = malloc(sizeof(double*) * (n_sub + 1));
+    double** Gs = malloc(sizeof(double*) * (n_sub + 1));
+    double** As = malloc(sizeof(double*) * (n_sub + 1));
+    for (int i = 0; i <= n_sub; i++) {
+        Us[i] = xcalloc(n, 8); Ps[i] = xcalloc(n, 8); Hs[i] = xcalloc(n, 8); Gs[i] = xcalloc(E ? E : 1, 8);
+        As[i] = xcalloc(gsz, 8);
+        memcpy(Us[i], s.u[s.cur], 8 * (size_t)n); memcpy(Ps[i], s.p[s.cur], 8 * (size_t)n);
+        memcpy(Hs[i], s.h, 8 * (size_t)n); memcpy(Gs[i], s.g, 8 * (size_t)E); memcpy(As[i], s.A[s.cur], 8 * gsz);
+        if (i < n_sub) rd_substep(t, &w, P, &s);
+    }
+    /* loss and its seed */
+    int nafm = 0;
+    for (int c = w.start; c < n; c++) nafm += is_afm(t->state[c]);
+    double L = 0;
+    double *lu = xcalloc(n, 8), *lp = xcalloc(n, 8), *lh = xcalloc(n, 8), *lg = xcalloc(E ? E : 1, 8),
+           *lA = xcalloc(gsz, 8);
+    double *nu = xcalloc(n, 8), *np_ = xcalloc(n, 8), *nh = xcalloc(n, 8), *ng = xcalloc(E ? E : 1, 8),
+           *nA = xcalloc(gsz, 8), *accA = xcalloc(n, 8);
+    for (int c = w.start; c < n; c++)
+        if (is_afm(t->state[c])) {
+            const double d = Us[n_sub][c] - target[c];
+            L += d * d;
+            lu[c] = 2.0 * d / nafm;
+        }
+    *loss = L / nafm;
+    G->du = G->dp = G->gj0 = 0;
+    if (G->kpa) for (int c = 0; c < n; c++) G->kpa[c] = 0;
+    const double Da = 1.0, Kaa = 0.5, STIM11 = 0.002, H0 = 0.5;
+    const double kflux = 8.1, mu0 = 0.567, mu1 = 0.1, kmu = 0.05, pb = 0.11, pbb = 0.89, pk1 = 0.7;
+    const double kg = 0.1, gam = 2.0, k2 = 0.7;
+    const double cD = DT_Ca * Da * t->inv_dx * t->inv_dx;
+    const uint8_t* cm2 = t->cmap2;
+    for (int i = n_sub - 1; i >= 0; i--) {
+        const double *u = Us[i], *p = Ps[i], *h = Hs[i], *g = Gs[i], *A = As[i];
+        memset(nu, 0, 8 * (size_t)n); memset(np_, 0, 8 * (size_t)n); memset(nh, 0, 8 * (size_t)n);
+        memset(ng, 0, 8 * (size_t)E); memset(nA, 0, 8 * gsz); memset(accA, 0, 8 * (size_t)n);
+        /* voxel adjoints gathered per owner cell (for the fa source term) */
+        for (int j = 0; j < w.NX; j++)
+            for (int k = 0; k < w.NY; k++)
+                for (int l = 0; l < w.izb; l++) {
+                    const size_t v = IDX3(t, j, k, l);
+                    if (w.owner[v] >= 0) accA[w.owner[v]] += lA[v];
+                }
+        for (int c = w.start; c < n; c++) {
+            const int st = t->state[c];
+            if (st == EO_DEAD) {
+                int cnt = 0; double sum = 0;
+                for (int e = w.ptr[c]; e < w.ptr[c + 1]; e++)
+                    if (w.kind[e] == 3) { sum += p[w.col[e]]; cnt++; }
+                np_[c] += lp[c] * (1.0 - DT_Ca * P->dp * cnt - DT_Ca * Kpp);
+                for (int e = w.ptr[c]; e < w.ptr[c + 1]; e++)
+                    if (w.kind[e] == 3) np_[w.col[e]] += lp[c] * DT_Ca * P->dp;
+                G->dp += lp[c] * DT_Ca * (sum - cnt * p[c]);
+                continue;
+            }
+            if (!is_afm(st)) continue;
+            const int ix = t->lat[3 * c], iy = t->lat[3 * c + 1], iz = t->lat[3 * c + 2];
+            double su = 0, sp = 0, gsum = 0;
+            for (int e = w.ptr[c]; e < w.ptr[c + 1]; e++) {
+                if (!w.kind[e]) continue;
+                const int o = w.col[e];
+                su += g[e] * (u[o] - u[c]); sp += g[e] * (p[o] - p[c]); gsum += g[e];
+            }
+            const double Bavg = grid_avg8(t, w.B, ix, iy, iz), Aavg = grid_avg8(t, A, ix, iy, iz);
+            const double iag = w.iag[c], kpa = rd_kpa(&w, P, c);
+            const double diffu = ca2p_reaction_factor(u[c], h[c], p[c], Bavg) + P->du * iag * su;
+            const double e1 = mu0 + mu1 * p[c] / (p[c] + kmu), e2 = pb + pbb * u[c] / (pk1 + u[c]);
+            const double dRdu = kflux * e1 * (pbb * pk1 / ((pk1 + u[c]) * (pk1 + u[c]))) * h[c]
+                                - gam * kg / ((kg + u[c]) * (kg + u[c]));
+            const double dRdh = kflux * e1 * e2;
+            const double dRdp = kflux * (mu1 * kmu / ((p[c] + kmu) * (p[c] + kmu))) * e2 * h[c];
+            const double abar = DT_Ca * lu[c] + (diffu > 0 ? accA[c] * DT_Ca * STIM11 : 0.0);
+            const double q = k2 * k2 + u[c] * u[c];
+            nu[c] += lu[c] + abar * (dRdu - P->du * iag * gsum) + lh[c] * DT_Ca * (-2.0 * u[c] * k2 * k2 / (q * q)) / w.th[c];
+            np_[c] += lp[c] * (1.0 - DT_Ca * Kpp - DT_Ca * P->dp * iag * gsum) + abar * dRdp;
+            nh[c] += lh[c] * (1.0 - DT_Ca / w.th[c]) + abar * dRdh;
+            for (int e = w.ptr[c]; e < w.ptr[c + 1]; e++) {
+                if (!w.kind[e]) continue;
+                const int o = w.col[e];
+                double to_u = abar * P->du * iag * g[e];
+                double lge = lg[e];
+                if (w.kind[e] == 3) {
+                    const double d = u[o] - u[c];
+                    const double th_ = tanh((0.1 - fabs(d)) / 0.1);
+                    const double fwd = -5.0 * (1.0 - th_ * th_);
+                    const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
+                    to_u += lg[e] * DT_Ca * fwd * sg;
+                    nu[c] -= lg[e] * DT_Ca * fwd * sg;
+                    lge = lg[e] * (1.0 - DT_Ca);
+                }
+                ng[e] = lge + abar * P->du * iag * (u[o] - u[c]) + lp[c] * DT_Ca * P->dp * iag * (p[o] - p[c]);
+                if (is_afm(t->state[o])) nu[o] += to_u;
+                np_[o] += lp[c] * DT_Ca * P->dp * iag * g[e];
+            }
+            /* grid_avg8 of ATP in the IP3 production term */
+            const double wA = lp[c] * DT_Ca * kpa * H0 / ((H0 + Aavg) * (H0 + Aavg)) * 0.125;
+            for (int q8 = 0; q8 < 8; q8++) {
+                const long vv = rd_corner_var(t, &w, ix + (q8 & 1), iy + ((q8 >> 1) & 1), iz + ((q8 >> 2) & 1));
+                if (vv >= 0) nA[vv] += wA;
+            }
+            G->du += abar * iag * su;
+            G->dp += lp[c] * DT_Ca * iag * sp;
+            if (G->kpa) G->kpa[c] += lp[c] * DT_Ca * Aavg / (H0 + Aavg);
+        }
+        /* ATP_refresh: every updated point w reads its six neighbours */
+        for (int j = 0; j < w.NX; j++) {
+            const int px = t->px_prev[j], nx = t->px_next[j];
+            for (int k = 0; k < w.NY; k++) {
+                const int py = t->py_prev[k], ny = t->py_next[k];
+                for (int l = 0; l < w.izb; l++) {
+                    const size_t v = IDX3(t, j, k, l);
+                    const int pz = t->pz_prev[l], nz = t->pz_next[l];
+                    const size_t nb[6] = {IDX3(t, px, k, l), IDX3(t, nx, k, l), IDX3(t, j, py, l),
+                                          IDX3(t, j, ny, l), IDX3(t, j, k, pz), IDX3(t, j, k, nz)};
+                    const int nbl[6] = {l, l, l, l, pz, nz};
+                    double msum = 0;
+                    for (int q6 = 0; q6 < 6; q6++) msum += (double)cm2[nb[q6]];
+                    nA[v] += lA[v] * (1.0 - cD * msum - DT_Ca * Kaa);
+                    for (int q6 = 0; q6 < 6; q6++)
+                        if (nbl[q6] < w.izb) nA[nb[q6]] += lA[v] * cD * (double)cm2[nb[q6]];
+                }
+            }
+        }
+        double* tmp;
+        tmp = lu; lu = nu; nu = tmp;
+        tmp = lp; lp = np_; np_ = tmp;
+        tmp = lh; lh = nh; nh = tmp;
+        tmp = lg; lg = ng; ng = tmp;
+        tmp = lA; lA = nA; nA = tmp;
+    }
+    for (int c = w.start; c < n; c++)
+        if (is_afm(t->state[c]))
+            for (int e = w.ptr[c]; e < w.ptr[c + 1]; e++)
+                if (w.kind[e]) G->gj0 += lg[e];
+    for (int i = 0; i <= n_sub; i++) { free(Us[i]); free(Ps[i]); free(Hs[i]); free(Gs[i]); free(As[i]); }
+    free(Us); free(Ps); free(Hs); free(Gs); free(As);
+    free(lu); free(lp); free(lh); free(lg); free(lA); free(nu); free(np_); free(nh); free(ng); free(nA); free(accA);
+    rd_state_free(&s); rd_free(&w);
+    return EO_OK;
+}

@@ -138,6 +138,23 @@ int eo_set_state(eo_tissue* t, const uint8_t* state, const double* agek);
 int eo_set_lists(eo_tissue* t, const int32_t* conn_ptr, const int32_t* conn,
                  const double* gj);
 
+/* ---- frozen reaction-diffusion window (the calc_ca2p sub-step with
+ * positions, lists, states and maps frozen) and its adjoint ---------------- */
+typedef struct eo_rd_params {
+    double du;          /* ca2p_du (ca2p.cpp:183), 0.01 in the reference */
+    double dp;          /* dp (ca2p.cpp:17), 0.1 */
+    double gj0;         /* starting gj of every directed edge, gj_init 0.99 */
+    const double* kpa;  /* per-cell IP3 production (calc_Kpa); NULL = calc_Kpa(agek) */
+} eo_rd_params;
+typedef struct eo_rd_grad { double du, dp, gj0; double* kpa; /* n, may be NULL */ } eo_rd_grad;
+/* n_sub sub-steps from the tissue's current state (left unchanged); the
+ * final ca2p of every cell into ca2p_out (n) */
+int eo_rd_forward(eo_tissue* t, double zzmax, const eo_rd_params* p, int n_sub, double* ca2p_out);
+/* loss = mean over AFM cells of (ca2p after n_sub - target)^2 and its
+ * gradient with respect to the parameters */
+int eo_rd_adjoint(eo_tissue* t, double zzmax, const eo_rd_params* p, int n_sub, const double* target,
+                  double* loss, eo_rd_grad* grad);
+
 /* helpers used by the pinning tests */
 double eo_p_diff_x(const eo_tissue* t, double a, double b);
 double eo_coef(eo_tissue* t, int law, int c1, int c2);

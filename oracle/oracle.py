@@ -65,6 +65,14 @@ def build():
 _lib = None
 
 
+class RdParams(C.Structure):
+    _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", _dp)]
+
+
+class RdGrad(C.Structure):
+    _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", _dp)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -97,6 +105,9 @@ def lib():
         L.eo_calc_ca2p_n.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.eo_should_calc_ca.argtypes = [C.c_void_p]
         L.eo_step.argtypes = [C.c_void_p, C.POINTER(Driver)]
+        L.eo_rd_forward.argtypes = [C.c_void_p, C.c_double, C.POINTER(RdParams), C.c_int, _dp]
+        L.eo_rd_adjoint.argtypes = [C.c_void_p, C.c_double, C.POINTER(RdParams), C.c_int, _dp, _dp,
+                                    C.POINTER(RdGrad)]
         L.eo_get_cells.argtypes = [C.c_void_p, C.POINTER(Cells)]
         L.eo_set_cell_values.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.eo_get_fields.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, _u8p]
@@ -221,6 +232,34 @@ class OracleTissue:
 
     def step(self, driver):
         return int(self.L.eo_step(self.h, C.byref(driver)))
+
+    # --- frozen reaction-diffusion window ---
+    def _rd_params(self, du, dp, gj0, kpa):
+        k = None if kpa is None else np.ascontiguousarray(kpa, dtype=np.float64)
+        p = RdParams(du, dp, gj0, None if k is None else k.ctypes.data_as(_dp))
+        return p, k
+
+    def rd_forward(self, zzmax, n_sub, du=0.01, dp=0.1, gj0=0.99, kpa=None):
+        p, keep = self._rd_params(du, dp, gj0, kpa)
+        n = self.get_cells()["x"].size
+        out = np.zeros(n)
+        rc = self.L.eo_rd_forward(self.h, zzmax, C.byref(p), n_sub, out.ctypes.data_as(_dp))
+        if rc:
+            raise RuntimeError(f"eo_rd_forward {rc}")
+        return out
+
+    def rd_adjoint(self, zzmax, n_sub, target, du=0.01, dp=0.1, gj0=0.99, kpa=None):
+        p, keep = self._rd_params(du, dp, gj0, kpa)
+        n = self.get_cells()["x"].size
+        tgt = np.ascontiguousarray(target, dtype=np.float64)
+        gk = np.zeros(n)
+        g = RdGrad(0, 0, 0, gk.ctypes.data_as(_dp))
+        loss = C.c_double()
+        rc = self.L.eo_rd_adjoint(self.h, zzmax, C.byref(p), n_sub, tgt.ctypes.data_as(_dp), C.byref(loss),
+                                  C.byref(g))
+        if rc:
+            raise RuntimeError(f"eo_rd_adjoint {rc}")
+        return loss.value, {"du": g.du, "dp": g.dp, "gj0": g.gj0, "kpa": gk}
 
     # --- state exchange ---
     def get_cells(self):
