@@ -44,14 +44,17 @@ constexpr double AREA_GRID = 2.0 + 1e-7;   // cell_connection.cpp:16-19
 constexpr int CA_ITR = (int)(10.0 / DT_Ca);  // ca2p.cpp:14
 
 // cell_interaction.cpp:21-45
-constexpr double eps_m = 0.01;
+// eps_m, K_TOTAL, Kspring and KBEND are per-tissue values (Tis::m_eps,
+// m_kadh, m_kspring, m_kbend) so that a mechanics window can vary them;
+// these are the reference's values
+constexpr double eps_m_ref = 0.01;
 constexpr double P_MEMB = 1. / COMPRESS_FACTOR;
 constexpr double DER_DER_CONST = 0.2;
-constexpr double K_TOTAL = 3.0;
-constexpr double K_DESMOSOME = K_TOTAL * 0.01;
-constexpr double Kspring = 25.0;
+constexpr double K_TOTAL_ref = 3.0;
+constexpr double K_DESMOSOME_RATIO = 0.01;
+constexpr double Kspring_ref = 25.0;
 constexpr double Kspring_d = 5.0;
-constexpr double KBEND = 0.5;
+constexpr double KBEND_ref = 0.5;
 constexpr double delta_R = 0.4 * R_der;
 constexpr double para_ljp2 = 0.005;
 constexpr double Kspring_division = 5.0;
@@ -87,6 +90,7 @@ struct Tis {
   int ANX, ANY, ANZ, bcap;
   int irx, iry, irz, pad1;
   double ca_du, ca_dp;        // ca2p_du (ca2p.cpp:183) and dp (ca2p.cpp:17)
+  double m_eps, m_kadh, m_kspring, m_kbend;   // cell_interaction.cpp:21-28
   Status* st;
   // cells
   uint8_t* state; uint8_t* is_malignant; uint8_t* touch_air;

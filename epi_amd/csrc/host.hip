@@ -21,6 +21,7 @@
 #include "epi_dev.h"
 #include "launch.h"
 #include "rd_dev.h"
+#include "mech_dev.h"
 
 using namespace epi;
 
@@ -123,6 +124,15 @@ struct epi_ctx {
     std::vector<int> izb;
     std::vector<double> zz;
   } rd;
+  // frozen mechanics window (epi_mech_*)
+  struct Mech {
+    bool active = false;
+    std::vector<MechTis> h;
+    MechTis* d = nullptr;
+    char* arena = nullptr;
+    char* work = nullptr;
+    size_t work_bytes = 0;
+  } mech;
   std::vector<long long> ca_solves;
   // throughput accounting of epi_step
   std::vector<long long> steps, cell_steps, grid_b, grid_ca;
@@ -436,6 +446,10 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
     ti.irz = (int)(Rad * g.nz / g.lz);
     ti.ca_du = 0.01;
     ti.ca_dp = 0.1;
+    ti.m_eps = eps_m_ref;
+    ti.m_kadh = K_TOTAL_ref;
+    ti.m_kspring = Kspring_ref;
+    ti.m_kbend = KBEND_ref;
   }
   // arena
   Carver dry{nullptr, 0, true};
@@ -485,6 +499,9 @@ void epi_destroy(epi_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->rd.arena) (void)hipFree(c->rd.arena);
+  if (c->mech.arena) (void)hipFree(c->mech.arena);
+  if (c->mech.work) (void)hipFree(c->mech.work);
+  if (c->mech.d) (void)hipFree(c->mech.d);
   if (c->rd.work) (void)hipFree(c->rd.work);
   if (c->rd.d) (void)hipFree(c->rd.d);
   if (c->rd.d_gj0) (void)hipFree(c->rd.d_gj0);
@@ -1365,5 +1382,158 @@ int epi_rd_end(epi_ctx* c) {
   (void)hipFree(rd.d); rd.d = nullptr;
   (void)hipFree(rd.d_gj0); rd.d_gj0 = nullptr;
   rd.active = false;
+  return EPI_OK;
+}
+
+// ---- frozen mechanics window and its adjoint --------------------------------
+namespace {
+int mech_params(epi_ctx* c, const epi_mech_params* prm) {
+  for (int t = 0; t < c->T; t++) {
+    c->tis[t].m_eps = prm[t].eps;
+    c->tis[t].m_kadh = prm[t].k_adh;
+    c->tis[t].m_kspring = prm[t].k_spring;
+    c->tis[t].m_kbend = prm[t].k_bend;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * c->T, hipMemcpyHostToDevice, c->stream));
+  return EPI_OK;
+}
+int mech_step(epi_ctx* c) {
+  launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), max_n(c));
+  c->pb ^= 1;
+  return EPI_OK;
+}
+}  // namespace
+
+int epi_mech_begin(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  auto& m = c->mech;
+  if (m.active) return fail(c, EPI_ERR_INVALID, "a mechanics window is already open");
+  if (c->pos_pending) return fail(c, EPI_ERR_INVALID, "pos_copy pending: open the window between steps");
+  RCHK(sync_status(c));
+  m.h.assign(c->T, MechTis{});
+  Carver dry{nullptr, 0, true};
+  auto carve = [&](Carver& cv) {
+    for (int t = 0; t < c->T; t++) {
+      MechTis& r = m.h[t];
+      const size_t cap = c->th[t].cap;
+      r.x0[0] = cv.take<double>(3 * cap); r.x0[1] = cv.take<double>(3 * cap);
+      r.lam[0] = cv.take<double>(3 * cap); r.lam[1] = cv.take<double>(3 * cap);
+      r.target = cv.take<double>(3 * cap);
+      r.scal = cv.take<double>(5);
+    }
+  };
+  carve(dry);
+  if (hipMalloc(&m.arena, align_up(dry.off, 256)) != hipSuccess || hipMalloc(&m.d, sizeof(MechTis) * c->T) != hipSuccess)
+    return fail(c, EPI_ERR_HIP, "window allocation failed");
+  Carver real{m.arena, 0, false};
+  carve(real);
+  HIPCHK(c, hipMemcpyAsync(m.d, m.h.data(), sizeof(MechTis) * c->T, hipMemcpyHostToDevice, c->stream));
+  launch_mech_state(c->stream, c->d_tis, m.d, c->T, 0, 5, c->pb, max_n(c));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  m.active = true;
+  return EPI_OK;
+}
+
+int epi_mech_forward(epi_ctx* c, const epi_mech_params* prm, int32_t n_steps) {
+  if (!c || !prm || n_steps < 0) return EPI_ERR_INVALID;
+  if (!c->mech.active) return fail(c, EPI_ERR_INVALID, "no mechanics window is open");
+  RCHK(mech_params(c, prm));
+  launch_mech_state(c->stream, c->d_tis, c->mech.d, c->T, 0, 4, c->pb, max_n(c));
+  for (int s = 0; s < n_steps; s++) RCHK(mech_step(c));
+  HIPCHK(c, hipGetLastError());
+  return sync_status(c);
+}
+
+int epi_mech_get_positions(epi_ctx* c, int32_t t, double* xyz) {
+  RCHK(check_tissue(c, t));
+  if (!xyz) return EPI_ERR_INVALID;
+  const size_t n = (size_t)c->h_st[t].n;
+  for (int a = 0; a < 3; a++)
+    HIPCHK(c, hipMemcpy(xyz + a * n, c->tis[t].pos[c->pb][a], 8 * n, hipMemcpyDeviceToHost));
+  return EPI_OK;
+}
+
+int epi_mech_adjoint(epi_ctx* c, const epi_mech_params* prm, int32_t n_steps, int32_t every,
+                     const double* const* targets, double* loss, epi_mech_grad* grads) {
+  if (!c || !prm || !targets || n_steps < 1 || every < 1) return EPI_ERR_INVALID;
+  auto& m = c->mech;
+  if (!m.active) return fail(c, EPI_ERR_INVALID, "no mechanics window is open");
+  const int K = std::min(every, n_steps);
+  const int nck = (n_steps + K - 1) / K;
+  size_t need = 0;
+  for (int t = 0; t < c->T; t++) need += align_up(8 * 3 * (size_t)c->th[t].cap * (K + nck), 256) + 512;
+  if (need > m.work_bytes) {
+    if (m.work) (void)hipFree(m.work);
+    m.work = nullptr;
+    m.work_bytes = 0;
+    if (hipMalloc(&m.work, need) != hipSuccess) return fail(c, EPI_ERR_HIP, "window work allocation failed");
+    m.work_bytes = need;
+  }
+  Carver cv{m.work, 0, false};
+  for (int t = 0; t < c->T; t++) {
+    MechTis& r = m.h[t];
+    const size_t cap = c->th[t].cap, n = (size_t)c->h_st[t].n;
+    r.slot = cv.take<double>(3 * cap * K);
+    r.ck = cv.take<double>(3 * cap * nck);
+    for (int a = 0; a < 3; a++)
+      HIPCHK(c, hipMemcpyAsync(r.target + a * cap, targets[t] + a * n, 8 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(r.scal, 0, 5 * sizeof(double), c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(m.d, m.h.data(), sizeof(MechTis) * c->T, hipMemcpyHostToDevice, c->stream));
+  RCHK(mech_params(c, prm));
+  const int mn = max_n(c);
+  launch_mech_state(c->stream, c->d_tis, m.d, c->T, 0, 4, c->pb, mn);
+  std::vector<int> ck_pb(nck);
+  for (int s = 0; s < n_steps; s++) {
+    if (s % K == 0) {
+      ck_pb[s / K] = c->pb;
+      launch_mech_state(c->stream, c->d_tis, m.d, c->T, s / K, 2, c->pb, mn);
+    }
+    RCHK(mech_step(c));
+  }
+  int q = 0;
+  launch_mech_seed(c->stream, c->d_tis, m.d, c->T, c->pb, q, mn);
+  for (int k = nck - 1; k >= 0; k--) {
+    c->pb = ck_pb[k];
+    launch_mech_state(c->stream, c->d_tis, m.d, c->T, k, 3, c->pb, mn);
+    const int s0 = k * K, s1 = std::min(n_steps, s0 + K);
+    for (int s = s0; s < s1; s++) {
+      launch_mech_state(c->stream, c->d_tis, m.d, c->T, s - s0, 0, c->pb, mn);
+      RCHK(mech_step(c));
+    }
+    for (int s = s1 - 1; s >= s0; s--) {
+      launch_mech_backward_step(c->stream, c->d_tis, m.d, c->T, s - s0, c->pb, c->cb, q, mn);
+      q ^= 1;
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  RCHK(sync_status(c));
+  for (int t = 0; t < c->T; t++) {
+    double sc[5];
+    HIPCHK(c, hipMemcpy(sc, m.h[t].scal, sizeof(sc), hipMemcpyDeviceToHost));
+    if (loss) loss[t] = sc[0];
+    if (grads) { grads[t].eps = sc[1]; grads[t].k_adh = sc[2]; grads[t].k_spring = sc[3]; grads[t].k_bend = sc[4]; }
+  }
+  return EPI_OK;
+}
+
+int epi_mech_end(epi_ctx* c) {
+  if (!c) return EPI_ERR_INVALID;
+  auto& m = c->mech;
+  if (!m.active) return EPI_OK;
+  for (int t = 0; t < c->T; t++) {
+    c->tis[t].m_eps = eps_m_ref; c->tis[t].m_kadh = K_TOTAL_ref;
+    c->tis[t].m_kspring = Kspring_ref; c->tis[t].m_kbend = KBEND_ref;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * c->T, hipMemcpyHostToDevice, c->stream));
+  launch_mech_state(c->stream, c->d_tis, m.d, c->T, 0, 4, c->pb, max_n(c));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(m.arena); m.arena = nullptr;
+  if (m.work) (void)hipFree(m.work);
+  m.work = nullptr; m.work_bytes = 0;
+  (void)hipFree(m.d); m.d = nullptr;
+  m.active = false;
   return EPI_OK;
 }

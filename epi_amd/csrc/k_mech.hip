@@ -10,6 +10,7 @@
 #include "epi_dev.h"
 #include "launch.h"
 #include "epi_group.h"
+#include "mech_dev.h"
 
 namespace epi {
 
@@ -61,7 +62,7 @@ __device__ __forceinline__ double ljmain(const Tis& t, const CV& a, const CV& b)
   const double distSq = dist_sq(t, a.p, b.p);
   const double rad_sum = a.r + b.r;
   const double LJ6 = POW6(rad_sum) / POW3(distSq);
-  return 4.0 * eps_m * LJ6 * (LJ6 - 1) / distSq;
+  return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / distSq;
 }
 __device__ __forceinline__ bool is_near(const Tis& t, const CV& a, const CV& b) {
   return dist_sq(t, a.p, b.p) < (a.r + b.r) * (a.r + b.r);
@@ -86,31 +87,31 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
         const double dist2 = dist + delta_R;
         const double rad_sum = a.r + b.r;
         const double LJ6 = POW6(rad_sum) / POW6(dist2);
-        return 4.0 * eps_m * LJ6 * (LJ6 - 1) / (dist * dist2) + para_ljp2;
+        return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / (dist * dist2) + para_ljp2;
       } else if (is_near(t, a, b)) {
         return para_ljp2;
       } else {
         const double dist2 = dist + delta_R;
         const double rad_sum = a.r + b.r;
         const double LJ6 = POW6(rad_sum) / POW6(dist2);
-        return 4.0 * eps_m * LJ6 * (LJ6 - 1) / (dist * dist) + para_ljp2;
+        return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / (dist * dist) + para_ljp2;
       }
     }
     case LAW_ALAIRDE:
       if (is_near(t, a, b)) return ljmain(t, a, b);
       else {
-        double spf = K_DESMOSOME;
-        if (a.agek > THRESH_SP && b.agek > THRESH_SP) spf = K_TOTAL;
+        double spf = (t.m_kadh * K_DESMOSOME_RATIO);
+        if (a.agek > THRESH_SP && b.agek > THRESH_SP) spf = t.m_kadh;
         return adhesion(t, a, b, spf);
       }
     case LAW_FIXMU:
       if (is_near(t, a, b)) return ljmain(t, a, b);
-      return adhesion(t, a, b, K_DESMOSOME);
+      return adhesion(t, a, b, (t.m_kadh * K_DESMOSOME_RATIO));
     case LAW_MU_MEMB: {
       if (is_near(t, a, b)) return ljmain(t, a, b);
       // spring_force_to_memb, set at the top of _MUSUME_interaction (:613-620)
       double spring = 0;
-      if (a.pair >= 0 && t.state[a.pair] == FIX) spring = Kspring;
+      if (a.pair >= 0 && t.state[a.pair] == FIX) spring = t.m_kspring;
       else if (a.rdt > 0) spring = Kspring_d;
       return adhesion(t, a, b, spring);
     }
@@ -119,7 +120,7 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
       else {
         const double distlj = sqrt(dist_sq(t, a.p, b.p));
         const double LJ2 = distlj / (a.r + b.r);
-        return -(Kspring / distlj) * (LJ2 - 1.0);
+        return -(t.m_kspring / distlj) * (LJ2 - 1.0);
       }
     case LAW_MEMB_MEMB: {
       const double rad_sum = a.r + b.r;
@@ -129,7 +130,7 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
       double tmp;
       if (distSq < (cr_dist_sq = rad_sum_sq * P_MEMB * P_MEMB)) {
         const double LJ6 = POW3(cr_dist_sq) / POW3(distSq);
-        tmp = 4.0 * eps_m * LJ6 * (LJ6 - 1) / distSq;
+        tmp = 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / distSq;
       } else if (distSq < rad_sum_sq) {
         const double distlj = sqrt(distSq);
         const double cr_dist = rad_sum * P_MEMB;
@@ -140,7 +141,7 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
         const double cr_dist = rad_sum * P_MEMB;
         const double LJ6 = POW6(cr_dist) / POW6(lambda_dist - distlj);
         tmp = -(DER_DER_CONST / rad_sum) * ((1.0 - P_MEMB) / P_MEMB)
-              - 4 * eps_m * (LJ6 * (LJ6 - 1.0)) / ((lambda_dist - distlj) * distlj);
+              - 4 * t.m_eps * (LJ6 * (LJ6 - 1.0)) / ((lambda_dist - distlj) * distlj);
       }
       if (fabs(tmp) > 100) set_err(st, ERR_STRONG_INTERACTION);
       return tmp;
@@ -164,10 +165,10 @@ __device__ __forceinline__ P3 delta(const Tis& t, int law, const CV& a, const CV
 }
 
 // wall_interaction (:287-294): the z increment
-__device__ __forceinline__ double wall_dz(const CV& c) {
+__device__ __forceinline__ double wall_dz(const Tis& t, const CV& c) {
   const double distlj = 2.0 * c.p.z;
   const double LJ6 = POW6(c.r) / POW6(c.p.z);
-  const double ljm = 4.0 * eps_m * LJ6 * (LJ6 - 1.0) / (distlj * distlj);
+  const double ljm = 4.0 * t.m_eps * LJ6 * (LJ6 - 1.0) / (distlj * distlj);
   return DT_Cell * ljm * 2.0 * c.p.z;
 }
 
@@ -231,10 +232,10 @@ __device__ void forces_serial(const Tis& t, int pb, int cb, int X, Status* st) {
     for (int i = 0; i < 6; i++)
       dr = v_add(dr, tri_bend_2(t, cx.p, ld_pos(t, pb, md[i]), ld_pos(t, pb, md[(i + 1) % 6]),
                                 ld_pos(t, pb, md[(i + 2) % 6])));
-    acc.x += DT_Cell * KBEND * dr.x;
-    acc.y += DT_Cell * KBEND * dr.y;
-    acc.z += DT_Cell * KBEND * dr.z;
-    if (cx.p.z < cx.r) acc.z += wall_dz(cx);
+    acc.x += DT_Cell * t.m_kbend * dr.x;
+    acc.y += DT_Cell * t.m_kbend * dr.y;
+    acc.z += DT_Cell * t.m_kbend * dr.z;
+    if (cx.p.z < cx.r) acc.z += wall_dz(t, cx);
     int hi[6], nhi = 0;
     for (int i = 0; i < 6; i++) {
       const int q = md[i];
@@ -277,7 +278,7 @@ __device__ void forces_serial(const Tis& t, int pb, int cb, int X, Status* st) {
         acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
       }
     }
-    if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(cx);
+    if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(t, cx);
     for (int i = 0; i < len; i++) {
       const int q = row[i];
       const CV cq = load_cv(t, pb, q);
@@ -408,10 +409,10 @@ __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, i
       if (memb) {
         P3 dr = P3{0, 0, 0};
         for (int i = 0; i < 12; i++) dr = v_add(dr, s_bend[grp][i]);
-        acc.x += DT_Cell * KBEND * dr.x;
-        acc.y += DT_Cell * KBEND * dr.y;
-        acc.z += DT_Cell * KBEND * dr.z;
-        if (cx.p.z < cx.r) acc.z += wall_dz(cx);
+        acc.x += DT_Cell * t.m_kbend * dr.x;
+        acc.y += DT_Cell * t.m_kbend * dr.y;
+        acc.z += DT_Cell * t.m_kbend * dr.z;
+        if (cx.p.z < cx.r) acc.z += wall_dz(t, cx);
         int hi[6], nhi = 0;
         for (int e = 0; e < MEMB_ADJ; e++) {
           const uint8_t f = s_flag[grp][e];
@@ -434,7 +435,7 @@ __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, i
           const int e = s_perm[grp][r];
           if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
         }
-        if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(cx);
+        if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(t, cx);
         for (int e = 0; e < len; e++)
           if (s_flag[grp][e] & 1) { const P3 d = s_own[grp][e]; acc.x += d.x; acc.y += d.y; acc.z += d.z; }
         if (cx.pair >= 0 && X > cx.pair) {
@@ -483,6 +484,389 @@ void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int m
   EPI_LAUNCH(K_FORCES, k_forces, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
   dim3 gl((max_other + 255) / 256, ntis);
   EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb);
+}
+
+// ==== frozen mechanics window: the reverse pass ==============================
+// The window is the plain k_forces step repeated with lists, states, pairs
+// and dermis frozen.  Which interactions apply depends only on that frozen
+// data, so the reverse pass enumerates them from each handler cell (its own
+// list, its pair, the membrane slots), evaluates the derivative of the
+// active branch of each law at the step's positions, and scatters
+// lam_c1 - lam_c2 through the pair's Jacobian with FP64 atomics.
+
+namespace {
+struct LawD { double coef, ds, deps, dadh, dspr; };
+
+__device__ LawD ljmain_d(double s, double R, double eps) {
+  const double L = POW6(R) / POW3(s);
+  return LawD{4.0 * eps * L * (L - 1) / s, 4.0 * eps * (4.0 * L - 7.0 * L * L) / (s * s), 4.0 * L * (L - 1) / s, 0, 0};
+}
+__device__ LawD adhesion_d(double s, double R, double K) {
+  LawD r{0, 0, 0, 0, 0};
+  const double d = sqrt(s);
+  const double x = d / R - 1;
+  if (x + 1 > LJ_THRESH) return r;
+  const double c2 = (LJ_THRESH - 1.0) * (LJ_THRESH - 1.0);
+  const double g = x * (1 - x * x / c2);
+  r.coef = -(K / d) * g;
+  const double dg = (1 - 3 * x * x / c2) / R;
+  r.ds = (K * g / (d * d) - (K / d) * dg) / (2 * d);
+  return r;
+}
+
+// coef of `law` (c1 = a, c2 = b) and its derivatives with respect to the
+// squared distance and to eps_m, K_TOTAL, Kspring
+__device__ LawD coef_law_d(const Tis& t, int law, const CV& a, const CV& b) {
+  const double s = dist_sq(t, a.p, b.p);
+  const double R = a.r + b.r;
+  const double eps = t.m_eps;
+  LawD r{0, 0, 0, 0, 0};
+  switch (law) {
+    case LAW_OTHER:
+      return ljmain_d(s, R, eps);
+    case LAW_DER_DER: {
+      const double d = sqrt(s), D = d + delta_R;
+      const double L = POW6(R) / POW6(D);
+      const double dL = -6.0 * L / D;
+      double g, dgd;
+      if (D < R) {
+        g = L * (L - 1) / (d * D);
+        dgd = (2 * L - 1) * dL / (d * D) - L * (L - 1) * (D + d) / ((d * D) * (d * D));
+      } else if (s < R * R) {
+        return r;
+      } else {
+        g = L * (L - 1) / (d * d);
+        dgd = (2 * L - 1) * dL / (d * d) - 2 * L * (L - 1) / (d * d * d);
+      }
+      return LawD{4.0 * eps * g + para_ljp2, 4.0 * eps * dgd / (2 * d), 4.0 * g, 0, 0};
+    }
+    case LAW_ALAIRDE:
+      if (s < R * R) return ljmain_d(s, R, eps);
+      {
+        const bool strong = a.agek > THRESH_SP && b.agek > THRESH_SP;
+        r = adhesion_d(s, R, strong ? t.m_kadh : t.m_kadh * K_DESMOSOME_RATIO);
+        r.dadh = r.coef / t.m_kadh;
+        return r;
+      }
+    case LAW_FIXMU:
+      if (s < R * R) return ljmain_d(s, R, eps);
+      r = adhesion_d(s, R, t.m_kadh * K_DESMOSOME_RATIO);
+      r.dadh = r.coef / t.m_kadh;
+      return r;
+    case LAW_MU_MEMB: {
+      if (s < R * R) return ljmain_d(s, R, eps);
+      const bool fixpair = a.pair >= 0 && t.state[a.pair] == FIX;
+      const double spring = fixpair ? t.m_kspring : (a.rdt > 0 ? Kspring_d : 0.0);
+      r = adhesion_d(s, R, spring);
+      if (fixpair) r.dspr = r.coef / t.m_kspring;
+      return r;
+    }
+    case LAW_FIX_MEMB:
+      if (s < R * R) return ljmain_d(s, R, eps);
+      {
+        const double d = sqrt(s);
+        return LawD{-(t.m_kspring / d) * (d / R - 1.0), -t.m_kspring / (2 * d * d * d), 0, 0,
+                    -(1.0 / d) * (d / R - 1.0)};
+      }
+    case LAW_MEMB_MEMB: {
+      const double cr = R * P_MEMB, cr2 = cr * cr;
+      if (s < cr2) {
+        const double L = POW3(cr2) / POW3(s);
+        return LawD{4.0 * eps * L * (L - 1) / s, 4.0 * eps * (4.0 * L - 7.0 * L * L) / (s * s),
+                    4.0 * L * (L - 1) / s, 0, 0};
+      } else if (s < R * R) {
+        const double d = sqrt(s);
+        return LawD{-(DER_DER_CONST / d) * (d / cr - 1.0), -DER_DER_CONST / (2 * d * d * d), 0, 0, 0};
+      } else {
+        const double d = sqrt(s);
+        const double lam = (1.0 + P_MEMB) * R;
+        const double m = lam - d;
+        const double L = POW6(cr) / POW6(m);
+        const double dL = 6.0 * L / m;
+        const double q = L * (L - 1.0) / (m * d);
+        const double dq = (2 * L - 1) * dL / (m * d) + L * (L - 1.0) * (d - m) / ((m * d) * (m * d));
+        return LawD{-(DER_DER_CONST / R) * ((1.0 - P_MEMB) / P_MEMB) - 4 * eps * q, -4 * eps * dq / (2 * d), -4 * q,
+                    0, 0};
+      }
+    }
+    case LAW_PAIR: {
+      const double d = sqrt(s);
+      return LawD{-(Kspring_division * (d - a.snl)) / d, -Kspring_division * a.snl / (2 * d * d * d), 0, 0, 0};
+    }
+    default:
+      return r;
+  }
+}
+
+__device__ __forceinline__ void atomic_add3(double* lam, long long cap, int c, P3 g) {
+  atomicAdd(&lam[c], g.x);
+  atomicAdd(&lam[cap + c], g.y);
+  atomicAdd(&lam[2 * cap + c], g.z);
+}
+__device__ __forceinline__ P3 ld3(const double* lam, long long cap, int c) {
+  return P3{lam[c], lam[cap + c], lam[2 * cap + c]};
+}
+
+struct ParAcc { double eps, adh, spr; };
+
+__device__ __forceinline__ void pair_bwd(const Tis& t, int law, const CV& a, int ia, const CV& b, int ib,
+                                         const double* lam, double* nl, long long cap, ParAcc& pa) {
+  const LawD ld = coef_law_d(t, law, a, b);
+  const P3 r{p_diff_x(t, a.p.x, b.p.x), p_diff_y(t, a.p.y, b.p.y), a.p.z - b.p.z};
+  const P3 la = ld3(lam, cap, ia), lb = ld3(lam, cap, ib);
+  const P3 av{la.x - lb.x, la.y - lb.y, la.z - lb.z};
+  const double ar = v_dot(av, r);
+  const double k2 = 2 * ld.ds * ar;
+  const P3 g{DT_Cell * (ld.coef * av.x + k2 * r.x), DT_Cell * (ld.coef * av.y + k2 * r.y),
+             DT_Cell * (ld.coef * av.z + k2 * r.z)};
+  atomic_add3(nl, cap, ia, g);
+  atomic_add3(nl, cap, ib, P3{-g.x, -g.y, -g.z});
+  pa.eps += DT_Cell * ld.deps * ar;
+  pa.adh += DT_Cell * ld.dadh * ar;
+  pa.spr += DT_Cell * ld.dspr * ar;
+}
+
+// adjoint of n = N / |N|
+__device__ __forceinline__ P3 unit_bar(P3 N, P3 nb) {
+  const double nn = v_norm(N);
+  const P3 n = v_div(N, nn);
+  return v_div(v_sub(nb, v_scale(v_dot(nb, n), n)), nn);
+}
+__device__ __forceinline__ double bend_factor_d(double c) {
+  if (c >= 0) return -1.0;
+  return -2.0 / ((1.0 + c) * (1.0 + c) * (1.0 + c));
+}
+// y = scale f(d) (v x (n2 - d n1)), d = n1.n2
+__device__ __forceinline__ void bend_pair_bar(P3 n1, P3 n2, P3 v, double scale, P3 ob, P3& vb, P3& n1b, P3& n2b,
+                                              double& db) {
+  const double d = v_dot(n1, n2);
+  const P3 rho = v_sub(n2, v_scale(d, n1));
+  const P3 oy = v_scale(scale * bend_factor(d), ob);
+  db += scale * bend_factor_d(d) * v_dot(ob, v_cross(v, rho));
+  vb = v_add(vb, v_cross(rho, oy));
+  const P3 rb = v_cross(oy, v);
+  n2b = v_add(n2b, rb);
+  db += -v_dot(rb, n1);
+  n1b = v_add(n1b, v_scale(-d, rb));
+}
+// reverse of tri_bend_1(r, a, b, c); bar[0..3] get the adjoints of r, a, b, c
+__device__ void tri_bend_1_bar(const Tis& t, P3 rv, P3 avp, P3 bv, P3 cv, P3 ob, P3* bar) {
+  const P3 u = vpsub(t, rv, bv), v = vpsub(t, avp, bv), w = vpsub(t, cv, bv);
+  P3 N1 = v_cross(u, v), N2 = v_cross(v, w);
+  P3 t1 = N1, t2 = N2;
+  const P3 n1 = v_normalize(t1), n2 = v_normalize(t2);
+  const double div = v_norm(t1);
+  P3 vb{0, 0, 0}, n1b{0, 0, 0}, n2b{0, 0, 0};
+  double db = 0;
+  bend_pair_bar(n1, n2, v, 1.0 / div, ob, vb, n1b, n2b, db);
+  n1b = v_add(n1b, v_scale(db, n2));
+  n2b = v_add(n2b, v_scale(db, n1));
+  const P3 N1b = unit_bar(N1, n1b), N2b = unit_bar(N2, n2b);
+  const P3 ub = v_cross(v, N1b);
+  vb = v_add(vb, v_add(v_cross(N1b, u), v_cross(w, N2b)));
+  const P3 wb = v_cross(N2b, v);
+  bar[0] = v_add(bar[0], ub);
+  bar[1] = v_add(bar[1], vb);
+  bar[3] = v_add(bar[3], wb);
+  bar[2] = v_sub(bar[2], v_add(ub, v_add(vb, wb)));
+}
+// reverse of tri_bend_2(r, a, b, c)
+__device__ void tri_bend_2_bar(const Tis& t, P3 rv, P3 avp, P3 bv, P3 cv, P3 ob, P3* bar) {
+  const P3 p = vpsub(t, rv, cv), q = vpsub(t, bv, cv);
+  const P3 u = vpsub(t, rv, bv), v = vpsub(t, avp, bv);
+  P3 N1 = v_cross(p, q), N2 = v_cross(u, v);
+  P3 t1 = N1, t2 = N2;
+  const P3 n1 = v_normalize(t1), n2 = v_normalize(t2);
+  const double div1 = v_norm(t1), div2 = v_norm(t2);
+  P3 qb{0, 0, 0}, vb{0, 0, 0}, n1b{0, 0, 0}, n2b{0, 0, 0};
+  double db = 0;
+  bend_pair_bar(n1, n2, q, 1.0 / div1, ob, qb, n1b, n2b, db);
+  bend_pair_bar(n2, n1, v, 1.0 / div2, ob, vb, n2b, n1b, db);
+  n1b = v_add(n1b, v_scale(db, n2));
+  n2b = v_add(n2b, v_scale(db, n1));
+  const P3 N1b = unit_bar(N1, n1b), N2b = unit_bar(N2, n2b);
+  const P3 pb = v_cross(q, N1b);
+  qb = v_add(qb, v_cross(N1b, p));
+  const P3 ub = v_cross(v, N2b);
+  vb = v_add(vb, v_cross(N2b, u));
+  bar[0] = v_add(bar[0], v_add(pb, ub));
+  bar[1] = v_add(bar[1], vb);
+  bar[2] = v_add(bar[2], v_sub(qb, v_add(ub, vb)));
+  bar[3] = v_sub(bar[3], v_add(pb, qb));
+}
+}  // namespace
+
+// positions between the tissue's current buffer and the window's storage:
+// dir 0 save to slot, 1 restore from slot, 2 save checkpoint, 3 restore
+// checkpoint, 4 restore the snapshot (both buffers), 5 take the snapshot
+__global__ void __launch_bounds__(256) k_mech_state(const Tis* __restrict__ T, const MechTis* __restrict__ M,
+                                                     int slot, int dir, int pb) {
+  const Tis& t = T[blockIdx.y];
+  const MechTis& m = M[blockIdx.y];
+  const int n = t.st->n;
+  const long long cap = t.cap;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    for (int a = 0; a < 3; a++) {
+      if (dir <= 3) {
+        double* st = (dir < 2 ? m.slot : m.ck) + (long long)slot * 3 * cap + a * cap;
+        if (dir == 0 || dir == 2) st[c] = t.pos[pb][a][c];
+        else t.pos[pb][a][c] = st[c];
+      } else if (dir == 4) {
+        t.pos[pb][a][c] = m.x0[0][a * cap + c];
+        t.pos[1 - pb][a][c] = m.x0[1][a * cap + c];
+      } else {
+        m.x0[0][a * cap + c] = t.pos[pb][a][c];
+        m.x0[1][a * cap + c] = t.pos[1 - pb][a][c];
+      }
+    }
+  }
+}
+
+// loss = mean over cells of |x - target|^2 (minimum image in x, y) and the seed
+__global__ void __launch_bounds__(256) k_mech_seed(const Tis* __restrict__ T, MechTis* __restrict__ M, int pb,
+                                                    int q) {
+  const Tis& t = T[blockIdx.y];
+  MechTis& m = M[blockIdx.y];
+  const int n = t.st->n;
+  const long long cap = t.cap;
+  double part = 0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const double dx = p_diff_x(t, t.pos[pb][0][c], m.target[c]);
+    const double dy = p_diff_y(t, t.pos[pb][1][c], m.target[cap + c]);
+    const double dz = t.pos[pb][2][c] - m.target[2 * cap + c];
+    part += dx * dx + dy * dy + dz * dz;
+    m.lam[q][c] = 2 * dx / n;
+    m.lam[q][cap + c] = 2 * dy / n;
+    m.lam[q][2 * cap + c] = 2 * dz / n;
+  }
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+  if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&m.scal[0], part / n);
+}
+
+// restore the state of slot `slot` and start the next adjoint at the identity
+__global__ void __launch_bounds__(256) k_mech_bwd_begin(const Tis* __restrict__ T, MechTis* __restrict__ M, int slot,
+                                                         int pb, int q) {
+  const Tis& t = T[blockIdx.y];
+  MechTis& m = M[blockIdx.y];
+  const int n = t.st->n;
+  const long long cap = t.cap;
+  const double* st = m.slot + (long long)slot * 3 * cap;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    for (int a = 0; a < 3; a++) {
+      t.pos[pb][a][c] = st[a * cap + c];
+      m.lam[1 - q][a * cap + c] = m.lam[q][a * cap + c];
+    }
+}
+
+// every interaction a cell's handler applies (cell_interaction.cpp:529-641)
+__global__ void __launch_bounds__(256) k_mech_bwd_pairs(const Tis* __restrict__ T, MechTis* __restrict__ M, int pb,
+                                                         int cb, int q) {
+  const Tis& t = T[blockIdx.y];
+  MechTis& m = M[blockIdx.y];
+  const int n = t.st->n;
+  const long long cap = t.cap;
+  const double* lam = m.lam[q];
+  double* nl = m.lam[1 - q];
+  ParAcc pa{0, 0, 0};
+  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
+    const CV cx = load_cv(t, pb, X);
+    const int* row = t.col[cb] + (size_t)X * t.ns;
+    const int len = t.nlen[cb][X];
+    if (X < t.nmemb) {
+      for (int i = 0; i < MEMB_ADJ; i++) {
+        const int o = t.memb6[6 * X + i];
+        if (t.state[o] == MEMB && X > o) pair_bwd(t, LAW_MEMB_MEMB, cx, X, load_cv(t, pb, o), o, lam, nl, cap, pa);
+      }
+      continue;
+    }
+    for (int e = 0; e < len; e++) {
+      const int o = row[e];
+      const CV co = load_cv(t, pb, o);
+      const int law = proc_law(cx, X, co, o);
+      if (law != LAW_NONE) pair_bwd(t, law, cx, X, co, o, lam, nl, cap, pa);
+    }
+    if (cx.pair >= 0 && X > cx.pair) pair_bwd(t, LAW_PAIR, cx, X, load_cv(t, pb, cx.pair), cx.pair, lam, nl, cap, pa);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    pa.eps += __shfl_down(pa.eps, o, 64);
+    pa.adh += __shfl_down(pa.adh, o, 64);
+    pa.spr += __shfl_down(pa.spr, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (pa.eps != 0) atomicAdd(&m.scal[1], pa.eps);
+    if (pa.adh != 0) atomicAdd(&m.scal[2], pa.adh);
+    if (pa.spr != 0) atomicAdd(&m.scal[3], pa.spr);
+  }
+}
+
+// the wall (membrane and DER cells, :287-294) and the bending of every
+// membrane particle (:488-499)
+__global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T, MechTis* __restrict__ M, int pb,
+                                                        int q) {
+  const Tis& t = T[blockIdx.y];
+  MechTis& m = M[blockIdx.y];
+  const int n = t.st->n;
+  const long long cap = t.cap;
+  const double* lam = m.lam[q];
+  double* nl = m.lam[1 - q];
+  double peps = 0, pbend = 0;
+  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
+    const bool memb = X < t.nmemb;
+    if (!memb && t.state[X] != DER) continue;
+    const CV cx = load_cv(t, pb, X);
+    if (cx.p.z < cx.r) {
+      const double z = cx.p.z;
+      const double L = POW6(cx.r) / POW6(z);
+      atomicAdd(&nl[2 * cap + X], lam[2 * cap + X] * DT_Cell * 2.0 * t.m_eps * (7.0 * L - 13.0 * L * L) / (z * z));
+      peps += lam[2 * cap + X] * DT_Cell * 2.0 * L * (L - 1.0) / z;
+    }
+    if (!memb) continue;
+    const P3 lm = ld3(lam, cap, X);
+    const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
+    const int* md = t.memb6 + 6 * X;
+    P3 dr{0, 0, 0};
+    for (int i = 0; i < 6; i++) {
+      const int i1 = (i + 1) % 6;
+      const int idx[4] = {X, md[i], md[i1], t.memb6[6 * md[i1] + i]};
+      const P3 pr = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
+      dr = v_add(dr, tri_bend_1(t, cx.p, pr, pbv, pc));
+      P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      tri_bend_1_bar(t, cx.p, pr, pbv, pc, ob, bar);
+      for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
+    }
+    for (int i = 0; i < 6; i++) {
+      const int idx[4] = {X, md[i], md[(i + 1) % 6], md[(i + 2) % 6]};
+      const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
+      dr = v_add(dr, tri_bend_2(t, cx.p, pa, pbv, pc));
+      P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
+      for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
+    }
+    pbend += DT_Cell * v_dot(lm, dr);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    peps += __shfl_down(peps, o, 64);
+    pbend += __shfl_down(pbend, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (peps != 0) atomicAdd(&m.scal[1], peps);
+    if (pbend != 0) atomicAdd(&m.scal[4], pbend);
+  }
+}
+
+void launch_mech_state(hipStream_t s, const Tis* dT, const MechTis* dM, int ntis, int slot, int dir, int pb,
+                       int max_n) {
+  hipLaunchKernelGGL(k_mech_state, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, dM, slot, dir, pb);
+}
+void launch_mech_seed(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int pb, int q, int max_n) {
+  hipLaunchKernelGGL(k_mech_seed, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, dM, pb, q);
+}
+void launch_mech_backward_step(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int slot, int pb, int cb, int q,
+                               int max_n) {
+  const dim3 g((max_n + 255) / 256, ntis);
+  hipLaunchKernelGGL(k_mech_bwd_begin, g, dim3(256), 0, s, dT, dM, slot, pb, q);
+  hipLaunchKernelGGL(k_mech_bwd_pairs, g, dim3(256), 0, s, dT, dM, pb, cb, q);
+  hipLaunchKernelGGL(k_mech_bwd_memb, g, dim3(256), 0, s, dT, dM, pb, q);
 }
 
 }  // namespace epi

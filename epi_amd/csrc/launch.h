@@ -62,5 +62,12 @@ void launch_rd_state(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot
 void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, int cb, int q, long long items);
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
                                 int max_n, long long max_box);
+// frozen mechanics window (k_mech.hip)
+struct MechTis;
+void launch_mech_state(hipStream_t s, const Tis* dT, const MechTis* dM, int ntis, int slot, int dir, int pb,
+                       int max_n);
+void launch_mech_seed(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int pb, int q, int max_n);
+void launch_mech_backward_step(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int slot, int pb, int cb, int q,
+                               int max_n);
 void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items);
 }  // namespace epi

@@ -97,13 +97,20 @@ SYMBOLS = [
     "epi_profile_read", "epi_profile_reset", "epi_profile_kernel_count",
     "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n",
     "epi_rd_begin", "epi_rd_forward", "epi_rd_get_ca2p", "epi_rd_adjoint", "epi_rd_end",
+    "epi_mech_begin", "epi_mech_forward", "epi_mech_get_positions", "epi_mech_adjoint", "epi_mech_end",
 ]
+
+MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
 
 # launch-shape keys of epi_set_tuning (include/epi_amd.h)
 TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4}
 
 class RdParams(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", C.POINTER(C.c_double))]
+
+
+class MechParams(C.Structure):
+    _fields_ = [("eps", C.c_double), ("k_adh", C.c_double), ("k_spring", C.c_double), ("k_bend", C.c_double)]
 
 
 class RdGrad(C.Structure):
@@ -163,6 +170,12 @@ def load_library():
     L.epi_rd_adjoint.argtypes = [vp, C.POINTER(RdParams), C.c_int32, C.c_int32,
                                  C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_double), C.POINTER(RdGrad)]
     L.epi_rd_end.argtypes = [vp]
+    L.epi_mech_begin.argtypes = [vp]
+    L.epi_mech_forward.argtypes = [vp, C.POINTER(MechParams), C.c_int32]
+    L.epi_mech_get_positions.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.epi_mech_adjoint.argtypes = [vp, C.POINTER(MechParams), C.c_int32, C.c_int32,
+                                   C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_double), C.POINTER(MechParams)]
+    L.epi_mech_end.argtypes = [vp]
     L.epi_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     _lib = L
     return L
@@ -332,6 +345,39 @@ class TissueSet:
 
     def profile_reset(self):
         self._chk(self.L.epi_profile_reset(self.h), "profile_reset")
+
+    # --- frozen mechanics window ---
+    def _mech_params(self, params):
+        arr = (MechParams * self.T)()
+        for t, p in enumerate(params):
+            q = dict(MECH_DEFAULT)
+            q.update(p)
+            arr[t] = MechParams(q["eps"], q["k_adh"], q["k_spring"], q["k_bend"])
+        return arr
+
+    def mech_begin(self):
+        self._chk(self.L.epi_mech_begin(self.h), "mech_begin")
+
+    def mech_forward(self, params, n_steps):
+        self._chk(self.L.epi_mech_forward(self.h, self._mech_params(params), int(n_steps)), "mech_forward")
+
+    def mech_get_positions(self, t):
+        n = self.status(t)[0].n
+        out = np.zeros(3 * n)
+        self._chk(self.L.epi_mech_get_positions(self.h, t, _ptr(out, _dp)), "mech_get_positions")
+        return out.reshape(3, n)
+
+    def mech_adjoint(self, params, n_steps, checkpoint_every, targets):
+        tg = [np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1)) for x in targets]
+        tp = (C.POINTER(C.c_double) * self.T)(*[_ptr(x, _dp) for x in tg])
+        loss = np.zeros(self.T)
+        grads = (MechParams * self.T)()
+        self._chk(self.L.epi_mech_adjoint(self.h, self._mech_params(params), int(n_steps), int(checkpoint_every), tp,
+                                          _ptr(loss, _dp), grads), "mech_adjoint")
+        return loss, [{"eps": g.eps, "k_adh": g.k_adh, "k_spring": g.k_spring, "k_bend": g.k_bend} for g in grads]
+
+    def mech_end(self):
+        self._chk(self.L.epi_mech_end(self.h), "mech_end")
 
     # --- frozen reaction-diffusion window ---
     def _rd_params(self, params):

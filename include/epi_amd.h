@@ -175,6 +175,33 @@ int epi_profile_read(epi_ctx* ctx, int32_t kernel, const char** name, double* to
 int epi_profile_reset(epi_ctx* ctx);
 int32_t epi_profile_kernel_count(void);
 
+/* ---- frozen mechanics window and its adjoint ------------------------------
+ * No counterpart in the reference.  A window is cell_interaction + pos_copy
+ * (cell_interaction.cpp:648-685, cellmanager.cpp:19-26) run n_steps times
+ * from a snapshot of every tissue's positions, with lists, states, pairs and
+ * dermis frozen and no periodic fix, and with the parameters below as
+ * runtime values (the reference's values give the plain step).  The
+ * Ca-dependent parts of the mechanics (the agek switch between K_DESMOSOME
+ * and K_TOTAL, fed by Ca through the ageing rate) are held at the window's
+ * start.  epi_mech_end restores the snapshot. */
+typedef struct epi_mech_params {
+    double eps;       /* eps_m of the LJ laws and the wall (cell_interaction.cpp:21), 0.01 */
+    double k_adh;     /* K_TOTAL of adhesion; K_DESMOSOME = 0.01 K_TOTAL (:24-25), 3.0 */
+    double k_spring;  /* Kspring of the FIX and FIX-paired MUSUME anchors (:26), 25.0 */
+    double k_bend;    /* KBEND of the membrane bending (:28), 0.5 */
+} epi_mech_params;
+typedef epi_mech_params epi_mech_grad;
+int epi_mech_begin(epi_ctx* ctx);
+int epi_mech_forward(epi_ctx* ctx, const epi_mech_params* params, int32_t n_steps);
+/* positions after the last forward: x[n], y[n], z[n] */
+int epi_mech_get_positions(epi_ctx* ctx, int32_t tissue, double* xyz);
+/* loss[t] = mean over cells of |x_T - target|^2 (minimum image in x, y),
+ * targets[t] = x[n], y[n], z[n]; recompute from checkpoints every
+ * checkpoint_every steps */
+int epi_mech_adjoint(epi_ctx* ctx, const epi_mech_params* params, int32_t n_steps, int32_t checkpoint_every,
+                     const double* const* targets, double* loss, epi_mech_grad* grads);
+int epi_mech_end(epi_ctx* ctx);
+
 /* ---- frozen reaction-diffusion window and its adjoint --------------------
  * No counterpart in the reference (it has no gradients).  A window is the
  * sub-step of calc_ca2p (ca2p.cpp:392-399) run n_sub times from a snapshot

@@ -41,15 +41,17 @@
 #define AREA_GRID (AREA_GRID_ORIGINAL + 1e-7) /* cell_connection.cpp:19 */
 
 /* cell_interaction.cpp:21-45 */
-#define eps_m 0.01
+/* eps_m, K_TOTAL, Kspring and KBEND are runtime values (g_mp) so that a
+ * mechanics window can vary them; these are their reference values */
+#define eps_m_ref 0.01
 #define P_MEMB (1. / COMPRESS_FACTOR)
 #define DER_DER_CONST 0.2
-#define K_TOTAL 3.0
+#define K_TOTAL_ref 3.0
 #define K_DESMOSOME_RATIO 0.01
-#define K_DESMOSOME (K_TOTAL * K_DESMOSOME_RATIO)
-#define Kspring 25.0
+#define K_DESMOSOME (g_mp.k_adh * K_DESMOSOME_RATIO)
+#define Kspring_ref 25.0
 #define Kspring_d 5.0
-#define KBEND 0.5
+#define KBEND_ref 0.5
 #define delta_R (0.4 * R_der)
 #define para_ljp2 0.005
 #define Kspring_division 5.0
@@ -67,6 +69,12 @@
 /* force laws (cell_interaction.cpp:68-76) */
 enum { LAW_NONE = 0, LAW_OTHER, LAW_DER_DER, LAW_FIXMU, LAW_FIX_MEMB,
        LAW_MU_MEMB, LAW_ALAIRDE, LAW_MEMB_MEMB, LAW_PAIR };
+
+static eo_mech_params g_mp = {eps_m_ref, K_TOTAL_ref, Kspring_ref, KBEND_ref};
+#define eps_m (g_mp.eps)
+#define K_TOTAL (g_mp.k_adh)
+#define Kspring (g_mp.k_spring)
+#define KBEND (g_mp.k_bend)
 
 typedef struct { int64_t uid; double v; uint8_t checked; } gj_ent;
 typedef struct { gj_ent* e; int n, cap; } gj_map;
@@ -600,7 +608,10 @@ static double coef_law(eo_tissue* t, int law, int c1, int c2) {
 double eo_coef(eo_tissue* t, int law, int c1, int c2) { return coef_law(t, law, c1, c2); }
 
 /* cell_interaction_apply, cell_interaction.cpp:48-66 */
+static int g_tape_on;
+static void tape_push(int law, int c1, int c2);
 static void apply(eo_tissue* t, int law, int c1, int c2) {
+    if (g_tape_on) tape_push(law, c1, c2);
     const double coef = coef_law(t, law, c1, c2);
     if (fabs(coef) > 1000) set_err(t, EO_ERR_STRONG_INTERACTION);
     const double dumx = DT_Cell * coef * p_diff_x(t, t->x1[c1], t->x1[c2]);
@@ -1853,4 +1864,359 @@ int eo_rd_adjoint(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, 
     free(lu); free(lp); free(lh); free(lg); free(lA); free(nu); free(np_); free(nh); free(ng); free(nA); free(accA);
     rd_state_free(&s); rd_free(&w);
     return EO_OK;
+}
+
+/* ==== frozen mechanics window and its adjoint =============================
+ * cell_interaction (cell_interaction.cpp:648-685) + pos_copy repeated
+ * n_steps times with lists, states, pairs and dermis frozen and without the
+ * periodic fix (a smooth function of the positions), the parameters eps_m,
+ * K_TOTAL (K_DESMOSOME = 0.01 K_TOTAL), Kspring and KBEND at runtime.  Which
+ * interactions apply depends only on the frozen data, so each step applies
+ * the same list of (law, c1, c2); the reverse pass walks it backwards with
+ * the derivative of the active branch of each law. */
+
+typedef struct { int law, c1, c2; } mech_app;
+static mech_app* g_tape = NULL;
+static int g_tape_n = 0, g_tape_cap = 0;
+
+static void tape_push(int law, int c1, int c2) {
+    if (g_tape_n == g_tape_cap) {
+        g_tape_cap = g_tape_cap ? 2 * g_tape_cap : 1024;
+        g_tape = realloc(g_tape, sizeof(mech_app) * g_tape_cap);
+    }
+    g_tape[g_tape_n].law = law; g_tape[g_tape_n].c1 = c1; g_tape[g_tape_n].c2 = c2;
+    g_tape_n++;
+}
+
+/* coef of a law at the current positions and its partial derivatives with
+ * respect to the squared distance s and to eps_m, K_TOTAL and Kspring */
+typedef struct { double coef, ds, deps, dadh, dspr; } law_d;
+
+static law_d ljmain_d(double s, double R, double eps) {
+    law_d r = {0, 0, 0, 0, 0};
+    const double L = POW6(R) / POW3(s);
+    r.coef = 4.0 * eps * L * (L - 1) / s;
+    r.ds = 4.0 * eps * (4.0 * L - 7.0 * L * L) / (s * s);
+    r.deps = 4.0 * L * (L - 1) / s;
+    return r;
+}
+/* adhesion with spring constant K: h(d) = -(K/d) x (1 - x^2/0.04), x = d/R - 1 */
+static law_d adhesion_d(double s, double R, double K) {
+    law_d r = {0, 0, 0, 0, 0};
+    const double d = sqrt(s);
+    const double x = d / R - 1;
+    if (x + 1 > LJ_THRESH) return r;
+    const double c2 = (LJ_THRESH - 1.0) * (LJ_THRESH - 1.0);
+    const double g = x * (1 - x * x / c2);
+    r.coef = -(K / d) * g;
+    const double dg = (1 - 3 * x * x / c2) / R;
+    const double dd = K * g / (d * d) - (K / d) * dg;
+    r.ds = dd / (2 * d);
+    return r;
+}
+
+static law_d coef_law_d(eo_tissue* t, int law, int c1, int c2) {
+    const double s = cell_dist_sq(t, c1, c2);
+    const double R = t->radius[c1] + t->radius[c2];
+    const double eps = g_mp.eps;
+    law_d r = {0, 0, 0, 0, 0};
+    switch (law) {
+    case LAW_OTHER:
+        return ljmain_d(s, R, eps);
+    case LAW_DER_DER: {
+        const double d = sqrt(s), D = d + delta_R;
+        const double L = POW6(R) / POW6(D);
+        const double dL = -6.0 * L / D;
+        double g, dgd;
+        if (D < R) {
+            g = L * (L - 1) / (d * D);
+            dgd = (2 * L - 1) * dL / (d * D) - L * (L - 1) * (D + d) / ((d * D) * (d * D));
+        } else if (s < R * R) {
+            return r;   /* para_ljp2 */
+        } else {
+            g = L * (L - 1) / (d * d);
+            dgd = (2 * L - 1) * dL / (d * d) - 2 * L * (L - 1) / (d * d * d);
+        }
+        r.coef = 4.0 * eps * g + para_ljp2;
+        r.ds = 4.0 * eps * dgd / (2 * d);
+        r.deps = 4.0 * g;
+        return r;
+    }
+    case LAW_ALAIRDE:
+        if (s < R * R) return ljmain_d(s, R, eps);
+        else {
+            const int strong = t->agek[c1] > THRESH_SP && t->agek[c2] > THRESH_SP;
+            const double K = strong ? g_mp.k_adh : g_mp.k_adh * K_DESMOSOME_RATIO;
+            r = adhesion_d(s, R, K);
+            r.dadh = r.coef / g_mp.k_adh;
+            return r;
+        }
+    case LAW_FIXMU:
+        if (s < R * R) return ljmain_d(s, R, eps);
+        r = adhesion_d(s, R, g_mp.k_adh * K_DESMOSOME_RATIO);
+        r.dadh = r.coef / g_mp.k_adh;
+        return r;
+    case LAW_MU_MEMB:
+        if (s < R * R) return ljmain_d(s, R, eps);
+        r = adhesion_d(s, R, t->spring_force_to_memb[c1]);
+        {
+            const int pr = t->pair[c1];
+            if (pr >= 0 && t->state[pr] == EO_FIX && t->spring_force_to_memb[c1] != 0)
+                r.dspr = r.coef / g_mp.k_spring;
+        }
+        return r;
+    case LAW_FIX_MEMB:
+        if (s < R * R) return ljmain_d(s, R, eps);
+        else {
+            const double d = sqrt(s);
+            r.coef = -(g_mp.k_spring / d) * (d / R - 1.0);
+            r.ds = -g_mp.k_spring / (2 * d * d * d);
+            r.dspr = -(1.0 / d) * (d / R - 1.0);
+            return r;
+        }
+    case LAW_MEMB_MEMB: {
+        const double cr = R * P_MEMB, cr2 = cr * cr;
+        if (s < cr2) {
+            const double L = POW3(cr2) / POW3(s);
+            r.coef = 4.0 * eps * L * (L - 1) / s;
+            r.ds = 4.0 * eps * (4.0 * L - 7.0 * L * L) / (s * s);
+            r.deps = 4.0 * L * (L - 1) / s;
+        } else if (s < R * R) {
+            const double d = sqrt(s);
+            r.coef = -(DER_DER_CONST / d) * (d / cr - 1.0);
+            r.ds = -DER_DER_CONST / (2 * d * d * d);
+        } else {
+            const double d = sqrt(s);
+            const double lam = (1.0 + P_MEMB) * R;
+            const double m = lam - d;
+            const double L = POW6(cr) / POW6(m);
+            const double dL = 6.0 * L / m;
+            const double q = L * (L - 1.0) / (m * d);
+            const double dq = (2 * L - 1) * dL / (m * d) + L * (L - 1.0) * (d - m) / ((m * d) * (m * d));
+            r.coef = -(DER_DER_CONST / R) * ((1.0 - P_MEMB) / P_MEMB) - 4 * eps * q;
+            r.ds = -4 * eps * dq / (2 * d);
+            r.deps = -4 * q;
+        }
+        return r;
+    }
+    case LAW_PAIR: {
+        const double d = sqrt(s);
+        r.coef = -(Kspring_division * (d - t->spr_nat_len[c1])) / d;
+        r.ds = -Kspring_division * t->spr_nat_len[c1] / (2 * d * d * d);
+        return r;
+    }
+    default:
+        return r;
+    }
+}
+
+/* reverse of tri_bend_1(r, a, b, c) for the output adjoint o: the result is
+ * f(n1.n2) (v x rho(n1, n2)) / |n1|, n1 = unit(u x v), n2 = unit(v x w),
+ * u = r-b, v = a-b, w = c-b (minimum image); the divisor |n1| is 1 up to
+ * rounding and is held constant */
+static v3 v3_unit_bar(v3 N, v3 n_adj) {   /* adjoint of n = N/|N| */
+    const double nn = v3_norm(N);
+    const v3 n = v3_div(N, nn);
+    return v3_div(v3_sub(n_adj, v3_scale(v3_dot(n_adj, n), n)), nn);
+}
+static inline double bend_factor_d(double c) {
+    if (c >= 0) return -1.0;
+    return -2.0 / ((1.0 + c) * (1.0 + c) * (1.0 + c));
+}
+static void bend_pair_bar(v3 n1, v3 n2, v3 v, double scale, v3 o_adj, v3* v_adj, v3* n1_adj, v3* n2_adj, double* d_adj) {
+    /* y = scale * f(d) * (v x rho), rho = n2 - d n1 (d given separately) */
+    const double d = v3_dot(n1, n2);
+    const v3 rho = v3_sub(n2, v3_scale(d, n1));
+    const v3 oy = v3_scale(scale * bend_factor(d), o_adj);
+    *d_adj += scale * bend_factor_d(d) * v3_dot(o_adj, v3_cross(v, rho));
+    *v_adj = v3_add(*v_adj, v3_cross(rho, oy));
+    const v3 r_adj = v3_cross(oy, v);
+    *n2_adj = v3_add(*n2_adj, r_adj);
+    *d_adj += -v3_dot(r_adj, n1);
+    *n1_adj = v3_add(*n1_adj, v3_scale(-d, r_adj));
+}
+static void tri_bend_1_bar(const eo_tissue* t, int r1, int a1, int b1, int c1, v3 o_adj, v3* bar) {
+    /* bar[0..3] accumulate the adjoints of r1, a1, b1, c1 */
+    v3 r1v = ctov(t, r1), a1v = ctov(t, a1), b1v = ctov(t, b1), c1v = ctov(t, c1);
+    const v3 u = vpsub(t, r1v, b1v), v = vpsub(t, a1v, b1v), w = vpsub(t, c1v, b1v);
+    v3 N1 = v3_cross(u, v), N2 = v3_cross(v, w);
+    v3 t1 = N1, t2 = N2;
+    const v3 n1 = v3_normalize(&t1), n2 = v3_normalize(&t2);
+    const double div = v3_norm(t1);
+    v3 v_adj = {0, 0, 0}, n1_adj = {0, 0, 0}, n2_adj = {0, 0, 0};
+    double d_adj = 0;
+    bend_pair_bar(n1, n2, v, 1.0 / div, o_adj, &v_adj, &n1_adj, &n2_adj, &d_adj);
+    n1_adj = v3_add(n1_adj, v3_scale(d_adj, n2));
+    n2_adj = v3_add(n2_adj, v3_scale(d_adj, n1));
+    const v3 N1_adj = v3_unit_bar(N1, n1_adj), N2_adj = v3_unit_bar(N2, n2_adj);
+    const v3 u_adj = v3_cross(v, N1_adj);
+    v_adj = v3_add(v_adj, v3_add(v3_cross(N1_adj, u), v3_cross(w, N2_adj)));
+    const v3 w_adj = v3_cross(N2_adj, v);
+    bar[0] = v3_add(bar[0], u_adj);
+    bar[1] = v3_add(bar[1], v_adj);
+    bar[3] = v3_add(bar[3], w_adj);
+    bar[2] = v3_sub(bar[2], v3_add(u_adj, v3_add(v_adj, w_adj)));
+}
+/* reverse of tri_bend_2(r, a, b, c): n1 = unit((r-c) x (b-c)),
+ * n2 = unit((r-b) x (a-b)), result f(n1.n2) [(b-c) x rho(n1,n2)/|n1| +
+ * (a-b) x rho(n2,n1)/|n2|] */
+static void tri_bend_2_bar(const eo_tissue* t, int r1, int a1, int b1, int c1, v3 o_adj, v3* bar) {
+    v3 r1v = ctov(t, r1), a1v = ctov(t, a1), b1v = ctov(t, b1), c1v = ctov(t, c1);
+    const v3 p = vpsub(t, r1v, c1v), q = vpsub(t, b1v, c1v);       /* N1 = p x q */
+    const v3 u = vpsub(t, r1v, b1v), v = vpsub(t, a1v, b1v);       /* N2 = u x v */
+    v3 N1 = v3_cross(p, q), N2 = v3_cross(u, v);
+    v3 t1 = N1, t2 = N2;
+    const v3 n1 = v3_normalize(&t1), n2 = v3_normalize(&t2);
+    const double div1 = v3_norm(t1), div2 = v3_norm(t2);
+    /* f(d) is shared: d = n1.n2; the two terms are f(d) q x rho(n1,n2)/div1
+     * and f(d) v x rho(n2,n1)/div2; bend_pair_bar handles f(d) (x) rho */
+    v3 q_adj = {0, 0, 0}, v_adj = {0, 0, 0}, n1_adj = {0, 0, 0}, n2_adj = {0, 0, 0};
+    double d_adj = 0;
+    bend_pair_bar(n1, n2, q, 1.0 / div1, o_adj, &q_adj, &n1_adj, &n2_adj, &d_adj);
+    bend_pair_bar(n2, n1, v, 1.0 / div2, o_adj, &v_adj, &n2_adj, &n1_adj, &d_adj);
+    n1_adj = v3_add(n1_adj, v3_scale(d_adj, n2));
+    n2_adj = v3_add(n2_adj, v3_scale(d_adj, n1));
+    const v3 N1_adj = v3_unit_bar(N1, n1_adj), N2_adj = v3_unit_bar(N2, n2_adj);
+    const v3 p_adj = v3_cross(q, N1_adj);
+    q_adj = v3_add(q_adj, v3_cross(N1_adj, p));
+    const v3 u_adj = v3_cross(v, N2_adj);
+    v_adj = v3_add(v_adj, v3_cross(N2_adj, u));
+    /* p = r-c, q = b-c, u = r-b, v = a-b */
+    bar[0] = v3_add(bar[0], v3_add(p_adj, u_adj));
+    bar[1] = v3_add(bar[1], v_adj);
+    bar[2] = v3_add(bar[2], v3_sub(q_adj, v3_add(u_adj, v_adj)));
+    bar[3] = v3_sub(bar[3], v3_add(p_adj, q_adj));
+}
+
+static int mech_window(eo_tissue* t, const eo_mech_params* p, int n_steps, double** keep) {
+    const eo_mech_params saved = g_mp;
+    g_mp = *p;
+    for (int s = 0; s < n_steps; s++) {
+        if (keep) {
+            memcpy(keep[s], t->x1, 8 * (size_t)t->n);
+            memcpy(keep[s] + t->n, t->y1, 8 * (size_t)t->n);
+            memcpy(keep[s] + 2 * (size_t)t->n, t->z1, 8 * (size_t)t->n);
+        }
+        eo_cell_interaction(t);
+        eo_pos_copy(t);
+    }
+    g_mp = saved;
+    return t->err;
+}
+
+int eo_mech_forward(eo_tissue* t, const eo_mech_params* p, int n_steps, double* xyz_out) {
+    const size_t n = (size_t)t->n;
+    double* save = malloc(8 * 6 * n);
+    memcpy(save, t->x1, 8 * n); memcpy(save + n, t->y1, 8 * n); memcpy(save + 2 * n, t->z1, 8 * n);
+    memcpy(save + 3 * n, t->x2, 8 * n); memcpy(save + 4 * n, t->y2, 8 * n); memcpy(save + 5 * n, t->z2, 8 * n);
+    int rc = mech_window(t, p, n_steps, NULL);
+    if (xyz_out) { memcpy(xyz_out, t->x1, 8 * n); memcpy(xyz_out + n, t->y1, 8 * n); memcpy(xyz_out + 2 * n, t->z1, 8 * n); }
+    memcpy(t->x1, save, 8 * n); memcpy(t->y1, save + n, 8 * n); memcpy(t->z1, save + 2 * n, 8 * n);
+    memcpy(t->x2, save + 3 * n, 8 * n); memcpy(t->y2, save + 4 * n, 8 * n); memcpy(t->z2, save + 5 * n, 8 * n);
+    free(save);
+    return rc;
+}
+
+int eo_mech_adjoint(eo_tissue* t, const eo_mech_params* p, int n_steps, const double* target, double* loss,
+                    eo_mech_grad* G) {
+    const int n = t->n;
+    double* save = malloc(8 * 6 * (size_t)n);
+    memcpy(save, t->x1, 8 * (size_t)n); memcpy(save + n, t->y1, 8 * (size_t)n); memcpy(save + 2 * n, t->z1, 8 * (size_t)n);
+    memcpy(save + 3 * n, t->x2, 8 * (size_t)n); memcpy(save + 4 * n, t->y2, 8 * (size_t)n); memcpy(save + 5 * n, t->z2, 8 * (size_t)n);
+    double** keep = malloc(sizeof(double*) * (n_steps + 1));
+    for (int s = 0; s <= n_steps; s++) keep[s] = malloc(8 * 3 * (size_t)n);
+    /* the interactions of one step (the same every step) */
+    g_tape_n = 0;
+    g_tape_on = 1;
+    const eo_mech_params saved = g_mp;
+    g_mp = *p;
+    int rc = mech_window(t, p, n_steps, keep);
+    g_tape_on = 0;
+    memcpy(keep[n_steps], t->x1, 8 * (size_t)n); memcpy(keep[n_steps] + n, t->y1, 8 * (size_t)n);
+    memcpy(keep[n_steps] + 2 * n, t->z1, 8 * (size_t)n);
+    const int napp = g_tape_n / (n_steps > 0 ? n_steps : 1);
+    /* loss and seed */
+    double* lam = calloc(3 * (size_t)n, 8);
+    double L = 0;
+    for (int c = 0; c < n; c++) {
+        const double dx = p_diff_x(t, keep[n_steps][c], target[c]);
+        const double dy = p_diff_y(t, keep[n_steps][n + c], target[n + c]);
+        const double dz = keep[n_steps][2 * n + c] - target[2 * n + c];
+        L += dx * dx + dy * dy + dz * dz;
+        lam[c] = 2 * dx / n; lam[n + c] = 2 * dy / n; lam[2 * n + c] = 2 * dz / n;
+    }
+    *loss = L / n;
+    G->eps = G->k_adh = G->k_spring = G->k_bend = 0;
+    double* nl = calloc(3 * (size_t)n, 8);
+    v3* b_adj = calloc((size_t)n, sizeof(v3));
+    for (int s = n_steps - 1; s >= 0; s--) {
+        memcpy(t->x1, keep[s], 8 * (size_t)n); memcpy(t->y1, keep[s] + n, 8 * (size_t)n);
+        memcpy(t->z1, keep[s] + 2 * (size_t)n, 8 * (size_t)n);
+        memcpy(nl, lam, 8 * 3 * (size_t)n);   /* the identity part of x' = x + ... */
+        /* pairs */
+        for (int k = 0; k < napp; k++) {
+            const mech_app a = g_tape[k];
+            if (a.law == LAW_MU_MEMB) {   /* spring_force_to_memb as set in the step */
+                int pr = t->pair[a.c1];
+                t->spring_force_to_memb[a.c1] = (pr >= 0 && t->state[pr] == EO_FIX) ? Kspring
+                                                : (t->rest_div_times[a.c1] > 0 ? Kspring_d : 0);
+            }
+            const law_d ld = coef_law_d(t, a.law, a.c1, a.c2);
+            const double rx = p_diff_x(t, t->x1[a.c1], t->x1[a.c2]);
+            const double ry = p_diff_y(t, t->y1[a.c1], t->y1[a.c2]);
+            const double rz = t->z1[a.c1] - t->z1[a.c2];
+            const double ax = lam[a.c1] - lam[a.c2], ay = lam[n + a.c1] - lam[n + a.c2],
+                         az = lam[2 * n + a.c1] - lam[2 * n + a.c2];
+            const double ar = ax * rx + ay * ry + az * rz;
+            const double gx = DT_Cell * (ld.coef * ax + 2 * ld.ds * ar * rx);
+            const double gy = DT_Cell * (ld.coef * ay + 2 * ld.ds * ar * ry);
+            const double gz = DT_Cell * (ld.coef * az + 2 * ld.ds * ar * rz);
+            nl[a.c1] += gx; nl[n + a.c1] += gy; nl[2 * n + a.c1] += gz;
+            nl[a.c2] -= gx; nl[n + a.c2] -= gy; nl[2 * n + a.c2] -= gz;
+            G->eps += DT_Cell * ld.deps * ar;
+            G->k_adh += DT_Cell * ld.dadh * ar;
+            G->k_spring += DT_Cell * ld.dspr * ar;
+        }
+        /* wall: z' = z + DT 2 eps L(L-1)/z, L = r^6/z^6, for z < r */
+        for (int c = 0; c < n; c++) {
+            const int st = t->state[c];
+            if (!(c < t->nmemb || st == EO_DER)) continue;
+            if (!collide_with_wall(t, c)) continue;
+            const double z = t->z1[c];
+            const double Lw = POW6(t->radius[c]) / POW6(z);
+            nl[2 * n + c] += lam[2 * n + c] * DT_Cell * 2.0 * g_mp.eps * (7.0 * Lw - 13.0 * Lw * Lw) / (z * z);
+            G->eps += lam[2 * n + c] * DT_Cell * 2.0 * Lw * (Lw - 1.0) / z;
+        }
+        /* bending of every membrane particle */
+        for (int m = 0; m < n; m++) b_adj[m] = v3_make(0, 0, 0);
+        for (int m = 0; m < t->nmemb; m++) {
+            const v3 lm = v3_make(lam[m], lam[n + m], lam[2 * n + m]);
+            const v3 dr = tri_memb_dr(t, m);
+            G->k_bend += DT_Cell * v3_dot(lm, dr);
+            const v3 ob = v3_scale(DT_Cell * g_mp.k_bend, lm);
+            const int32_t* md = &t->memb_adj[6 * m];
+            for (int i = 0; i < 6; i++) {
+                const int nb = md[(i + 1) % 6];
+                const int idx[4] = {m, md[i], nb, t->memb_adj[6 * nb + i]};
+                v3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                tri_bend_1_bar(t, idx[0], idx[1], idx[2], idx[3], ob, bar);
+                for (int q = 0; q < 4; q++) b_adj[idx[q]] = v3_add(b_adj[idx[q]], bar[q]);
+            }
+            for (int i = 0; i < 6; i++) {
+                const int idx[4] = {m, md[i], md[(i + 1) % 6], md[(i + 2) % 6]};
+                v3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                tri_bend_2_bar(t, idx[0], idx[1], idx[2], idx[3], ob, bar);
+                for (int q = 0; q < 4; q++) b_adj[idx[q]] = v3_add(b_adj[idx[q]], bar[q]);
+            }
+        }
+        for (int m = 0; m < n; m++) { nl[m] += b_adj[m].x; nl[n + m] += b_adj[m].y; nl[2 * n + m] += b_adj[m].z; }
+        double* tmp = lam; lam = nl; nl = tmp;
+    }
+    g_mp = saved;
+    memcpy(t->x1, save, 8 * (size_t)n); memcpy(t->y1, save + n, 8 * (size_t)n); memcpy(t->z1, save + 2 * n, 8 * (size_t)n);
+    memcpy(t->x2, save + 3 * n, 8 * (size_t)n); memcpy(t->y2, save + 4 * n, 8 * (size_t)n);
+    memcpy(t->z2, save + 5 * n, 8 * (size_t)n);
+    for (int s = 0; s <= n_steps; s++) free(keep[s]);
+    free(keep); free(save); free(lam); free(nl); free(b_adj);
+    return rc;
 }

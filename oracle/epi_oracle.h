@@ -138,6 +138,23 @@ int eo_set_state(eo_tissue* t, const uint8_t* state, const double* agek);
 int eo_set_lists(eo_tissue* t, const int32_t* conn_ptr, const int32_t* conn,
                  const double* gj);
 
+/* ---- frozen mechanics window (cell_interaction + pos_copy with lists,
+ * states, pairs and dermis frozen, no periodic fix) and its adjoint ------- */
+typedef struct eo_mech_params {
+    double eps;       /* eps_m of the LJ laws and the wall (cell_interaction.cpp:21), 0.01 */
+    double k_adh;     /* K_TOTAL of adhesion, K_DESMOSOME = 0.01 K_TOTAL (:24-25), 3.0 */
+    double k_spring;  /* Kspring of the FIX and FIX-paired MUSUME anchors (:26), 25.0 */
+    double k_bend;    /* KBEND of the membrane bending (:28), 0.5 */
+} eo_mech_params;
+typedef eo_mech_params eo_mech_grad;
+/* n_steps steps from the current positions (left unchanged); positions
+ * after the window into xyz_out (x[n], y[n], z[n]) */
+int eo_mech_forward(eo_tissue* t, const eo_mech_params* p, int n_steps, double* xyz_out);
+/* loss = mean over cells of |x_T - target|^2 (minimum image in x, y) and
+ * its gradient with respect to the parameters */
+int eo_mech_adjoint(eo_tissue* t, const eo_mech_params* p, int n_steps, const double* target_xyz,
+                    double* loss, eo_mech_grad* grad);
+
 /* ---- frozen reaction-diffusion window (the calc_ca2p sub-step with
  * positions, lists, states and maps frozen) and its adjoint ---------------- */
 typedef struct eo_rd_params {

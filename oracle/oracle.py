@@ -69,6 +69,13 @@ class RdParams(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", _dp)]
 
 
+class MechParams(C.Structure):
+    _fields_ = [("eps", C.c_double), ("k_adh", C.c_double), ("k_spring", C.c_double), ("k_bend", C.c_double)]
+
+
+MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
+
+
 class RdGrad(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", _dp)]
 
@@ -105,6 +112,8 @@ def lib():
         L.eo_calc_ca2p_n.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.eo_should_calc_ca.argtypes = [C.c_void_p]
         L.eo_step.argtypes = [C.c_void_p, C.POINTER(Driver)]
+        L.eo_mech_forward.argtypes = [C.c_void_p, C.POINTER(MechParams), C.c_int, _dp]
+        L.eo_mech_adjoint.argtypes = [C.c_void_p, C.POINTER(MechParams), C.c_int, _dp, _dp, C.POINTER(MechParams)]
         L.eo_rd_forward.argtypes = [C.c_void_p, C.c_double, C.POINTER(RdParams), C.c_int, _dp]
         L.eo_rd_adjoint.argtypes = [C.c_void_p, C.c_double, C.POINTER(RdParams), C.c_int, _dp, _dp,
                                     C.POINTER(RdGrad)]
@@ -232,6 +241,29 @@ class OracleTissue:
 
     def step(self, driver):
         return int(self.L.eo_step(self.h, C.byref(driver)))
+
+    # --- frozen mechanics window ---
+    def mech_forward(self, n_steps, **params):
+        p = dict(MECH_DEFAULT)
+        p.update(params)
+        n = self.get_cells()["x"].size
+        out = np.zeros(3 * n)
+        rc = self.L.eo_mech_forward(self.h, C.byref(MechParams(**p)), n_steps, out.ctypes.data_as(_dp))
+        if rc:
+            raise RuntimeError(f"eo_mech_forward {rc}")
+        return out.reshape(3, n)
+
+    def mech_adjoint(self, n_steps, target, **params):
+        p = dict(MECH_DEFAULT)
+        p.update(params)
+        tgt = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(-1))
+        g = MechParams()
+        loss = C.c_double()
+        rc = self.L.eo_mech_adjoint(self.h, C.byref(MechParams(**p)), n_steps, tgt.ctypes.data_as(_dp),
+                                    C.byref(loss), C.byref(g))
+        if rc:
+            raise RuntimeError(f"eo_mech_adjoint {rc}")
+        return loss.value, {"eps": g.eps, "k_adh": g.k_adh, "k_spring": g.k_spring, "k_bend": g.k_bend}
 
     # --- frozen reaction-diffusion window ---
     def _rd_params(self, du, dp, gj0, kpa):
