@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", choices=["large", "small", "batched"], default="large")
+    ap.add_argument("--config", choices=["large", "small", "batched", "adjoint-rd", "adjoint-mech"],
+                    default="large")
     ap.add_argument("--tissues", type=int, default=256, help="batched: number of tissues")
     ap.add_argument("--loop-sample", type=int, default=16,
                     help="batched: tissues timed one context at a time for the per-instance loop")
@@ -223,6 +224,96 @@ def run_batched(args):
     return 0
 
 
+def _timed(ts, fn):
+    ts.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    ts.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def run_adjoint_rd(args):
+    """Reaction-diffusion window of 200 calc_ca2p sub-steps on a 64x64x128
+    grid with 5,000 frozen free cells; loss = mean squared AFM Ca2+ against a
+    target from parameters perturbed by 5%; checkpoint every 20 sub-steps."""
+    from epi_amd import configs, native
+    spec = configs.config_adjoint_rd(4)
+    g = spec["geometry"]
+    ts = native.TissueSet([g])
+    ts.load(0, spec)
+    ts.set_rng(0, native.mt19937_64_uniform(4, 400000))
+    ts.step([native.Driver(0, 0, 0, 0, 0)], 2)
+    zz = ts.calc_zzmax()
+    c = ts.get_cells(0)
+    st, agek = c["state"], c["agek"]
+    alive = st == 0
+    kpa = np.where(alive, 4.0 + (6.0 - 4.0) * 0.5 * (1.0 + np.tanh((3.0 - agek) / 1.0)), 6.0)
+    n_sub, every = 200, 20
+    ts.rd_begin(zz)
+    rng = np.random.default_rng(4)
+    pert = dict(du=0.01 * 1.05, dp=0.1 * 0.95, gj0=0.99 * 0.95, kpa=kpa * (1 + 0.05 * rng.choice([-1, 1], kpa.size)))
+    ts.rd_forward([pert], n_sub)
+    target = ts.rd_get_ca2p(0)
+    base = dict(du=0.01, dp=0.1, gj0=0.99, kpa=kpa)
+    ts.rd_forward([base], n_sub)            # warm
+    reps = 3
+    t_fwd = min(_timed(ts, lambda: ts.rd_forward([base], n_sub))[0] for _ in range(reps))
+    ts.rd_adjoint([base], n_sub, every, [target])   # warm
+    t_adj, (loss, grads) = min((_timed(ts, lambda: ts.rd_adjoint([base], n_sub, every, [target]))
+                                for _ in range(reps)), key=lambda r: r[0])
+    ts.rd_end()
+    gk = grads[0]["kpa"]
+    out = {"metric": "adjoint cost / forward cost (reaction-diffusion window)", "value": round(t_adj / t_fwd, 3),
+           "unit": "x", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "64x64 sheet + 16 FIX + 5,000 free cells frozen, grid 64x64x128, 200 Ca sub-steps, "
+                                  "checkpoint every 20, seed 4", "cells": int(st.size), "iz_bound": int((zz[0] + 2.8) / (g.lz / g.nz))},
+           "forward_ms": round(1e3 * t_fwd, 3), "adjoint_ms": round(1e3 * t_adj, 3),
+           "forward_substeps_per_s": round(n_sub / t_fwd, 1), "loss": float(loss[0]),
+           "grad": {"du": grads[0]["du"], "dp": grads[0]["dp"], "gj0": grads[0]["gj0"],
+                    "kpa_l2": float(np.linalg.norm(gk))}}
+    print(json.dumps(out), flush=True)
+    ts.close()
+    return 0
+
+
+def run_adjoint_mech(args):
+    """Mechanics windows of 100 steps on 64 tissues of 2,000-6,000 cells
+    (48^3 grids) in one context; loss = mean squared deviation of positions
+    from a target made with parameters perturbed by 5%; checkpoint every 10."""
+    from epi_amd import configs, native
+    specs = configs.adjoint_mech_configs(64)
+    ts = native.TissueSet([s["geometry"] for s in specs])
+    for t, s in enumerate(specs):
+        ts.load(t, s)
+        ts.set_rng(t, native.mt19937_64_uniform(s["seed"], 300000))
+    ts.step([native.Driver(0, 0, 0, 0, 0) for _ in specs], 2)
+    n_steps, every = 100, 10
+    ts.mech_begin()
+    pert = [dict(eps=0.0105, k_adh=2.85, k_spring=26.25, k_bend=0.475) for _ in specs]
+    ts.mech_forward(pert, n_steps)
+    targets = [ts.mech_get_positions(t) for t in range(len(specs))]
+    base = [dict() for _ in specs]
+    ts.mech_forward(base, n_steps)   # warm
+    reps = 3
+    t_fwd = min(_timed(ts, lambda: ts.mech_forward(base, n_steps))[0] for _ in range(reps))
+    ts.mech_adjoint(base, n_steps, every, targets)   # warm
+    t_adj, (loss, grads) = min((_timed(ts, lambda: ts.mech_adjoint(base, n_steps, every, targets))
+                                for _ in range(reps)), key=lambda r: r[0])
+    ts.mech_end()
+    cells = sum(s["cells"]["x"].size for s in specs)
+    out = {"metric": "adjoint cost / forward cost (batched mechanics windows)", "value": round(t_adj / t_fwd, 3),
+           "unit": "x", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "64 tissues of 2,000-6,000 cells (40x40 sheet, grid 48^3), 100-step windows, "
+                                  "checkpoint every 10, seeds 5000+t", "cells": int(cells)},
+           "forward_ms": round(1e3 * t_fwd, 3), "adjoint_ms": round(1e3 * t_adj, 3),
+           "forward_cell_steps_per_s": round(cells * n_steps / t_fwd, 1),
+           "loss_mean": float(np.mean(loss)),
+           "grad_mean": {k: float(np.mean([g[k] for g in grads])) for k in grads[0]}}
+    print(json.dumps(out), flush=True)
+    ts.close()
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -232,6 +323,10 @@ def main():
         return 2
     if args.config == "batched":
         return run_batched(args)
+    if args.config == "adjoint-rd":
+        return run_adjoint_rd(args)
+    if args.config == "adjoint-mech":
+        return run_adjoint_mech(args)
     from epi_amd import native
     spec = make_spec(args.config)
     seed = spec["seed"]

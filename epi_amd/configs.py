@@ -300,3 +300,26 @@ def batched_configs(n_tissues=256, base_seed=3000, scale=1.0):
                                nx=edge, ny=edge, nz=2 * edge, n_fix=16,
                                n_free=n_free, n_layers=2, n_hot_alive=8, lz=lz))
     return out
+
+
+def config_adjoint_rd(seed=4):
+    """Adjoint reaction-diffusion workload: a 64x64 sheet (pitch 0.8), 16 FIX
+    and 5,000 free cells (positions frozen during the window), FP64 grid
+    64x64x128 (dx = dz = 0.8), seed 4."""
+    return make_tissue(seed, nmx=64, nmy=64, lx=51.2, ly=51.2, nx=64, ny=64, nz=128,
+                       n_fix=16, n_free=5000, n_layers=3, n_hot_alive=0, lz=102.4)
+
+
+def adjoint_mech_configs(n_tissues=64, base_seed=5000):
+    """Batched mechanics adjoint workload: per tissue t (seed 5000+t) a
+    40x40 sheet (L = 32), 16 FIX and free cells bringing the tissue to
+    U[2000, 6000] cells, FP64 grid 48^3 (dx = L/48, dz = LZ/48 with
+    LZ = 64 so that the stack fits the box)."""
+    out = []
+    for t in range(n_tissues):
+        r = np.random.default_rng(base_seed + t)
+        n_total = int(r.integers(2000, 6001))
+        n_free = n_total - 40 * 40 - 16
+        out.append(make_tissue(base_seed + t, nmx=40, nmy=40, lx=32.0, ly=32.0, nx=48, ny=48, nz=48,
+                               n_fix=16, n_free=n_free, n_layers=2, n_hot_alive=0, lz=64.0))
+    return out

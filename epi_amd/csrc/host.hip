@@ -1217,6 +1217,18 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
     rd.izb[t] = (int)((zzmax[t] + FAC_MAP * R_max) / c->tis[t].dz);   // ca2p.cpp:386
     if (rd.izb[t] < 0 || rd.izb[t] > c->tis[t].NZ) return fail(c, EPI_ERR_OUT_OF_GRID, "iz_bound outside the grid");
   }
+  // every AFM cell's grid_avg8 corners must lie in planes 0..iz_bound (the
+  // planes a saved state holds)
+  for (int t = 0; t < c->T; t++) {
+    const int n = c->h_st[t].n;
+    std::vector<uint8_t> st(n);
+    std::vector<int> lat(3 * (size_t)n);
+    HIPCHK(c, hipMemcpy(st.data(), c->tis[t].state, n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(lat.data(), c->tis[t].lat, 12 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int i = c->tis[t].nmemb + c->tis[t].nder; i < n; i++)
+      if ((st[i] == ALIVE || st[i] == FIX || st[i] == MUSUME) && lat[3 * i + 2] + 1 > rd.izb[t])
+        return fail(c, EPI_ERR_INVALID, "zzmax lies below a cell of the window");
+  }
   RCHK(set_zz(c, zzmax));
   // per-solve maps: entry kinds, th/Kpa/IAG, voxel owners (init_ca2p_map)
   long long box = 1;
