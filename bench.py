@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes instead of running the autotuner")
+    ap.add_argument("--copy-calibration", type=int, metavar="MIB", default=0,
+                    help="only run a plain device copy of MIB MiB five times (the known-size kernel that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes)")
+    ap.add_argument("--traffic-from", nargs=5, metavar=("KERNEL", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE output directories of this bench "
+                         "and of --copy-calibration into bytes per launch of KERNEL (JSON on stdout)")
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel with events and print a breakdown to stderr")
     return ap.parse_args()
@@ -224,6 +230,51 @@ def run_batched(args):
     return 0
 
 
+def copy_calibration(mib):
+    """Five device copies of mib MiB (FP64): FETCH_SIZE and WRITE_SIZE of one
+    such dispatch should read mib MiB each."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x5", flush=True)
+    return 0
+
+
+def _pmc_per_dispatch(d, counter, match):
+    import csv
+    import glob
+    vals = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter or match not in row.get("Kernel_Name", ""):
+                    continue
+                key = (f, int(row.get("Dispatch_Id", 0)))
+                vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+    return [v for _, v in sorted(vals.items(), key=lambda kv: kv[0][1])]
+
+
+def traffic_summary(kernel, fdir, wdir, cfdir, cwdir, copy_mib=512):
+    """HBM bytes per launch of `kernel` from the FETCH_SIZE / WRITE_SIZE runs
+    (KiB per dispatch), each scaled by what the counter reads for a copy of
+    copy_mib MiB (the largest dispatches of the calibration run)."""
+    import statistics
+    copy_bytes = copy_mib * 1024 * 1024
+    cf = statistics.median(sorted(_pmc_per_dispatch(cfdir, "FETCH_SIZE", ""))[-5:]) * 1024
+    cw = statistics.median(sorted(_pmc_per_dispatch(cwdir, "WRITE_SIZE", ""))[-5:]) * 1024
+    fetch = _pmc_per_dispatch(fdir, "FETCH_SIZE", kernel)
+    write = _pmc_per_dispatch(wdir, "WRITE_SIZE", kernel)
+    n = min(len(fetch), len(write), 1000)
+    fk, wk = statistics.mean(fetch[-n:]), statistics.mean(write[-n:])
+    return {"kernel": kernel, "dispatches": n, "fetch_kib_per_dispatch": fk, "write_kib_per_dispatch": wk,
+            "copy_fetch_factor": copy_bytes / cf, "copy_write_factor": copy_bytes / cw,
+            "bytes_per_launch": fk * 1024 * copy_bytes / cf + wk * 1024 * copy_bytes / cw}
+
+
 def _timed(ts, fn):
     ts.synchronize()
     t0 = time.perf_counter()
@@ -321,6 +372,11 @@ def main():
               "(tissues shard across GPUs with no per-step exchange; see DESIGN.md)",
               file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
+    if args.traffic_from:
+        print(json.dumps(traffic_summary(*args.traffic_from)))
+        return 0
     if args.config == "batched":
         return run_batched(args)
     if args.config == "adjoint-rd":
