@@ -32,8 +32,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic bytes per unit of work (grid point per sweep, cell per step)
+# algorithmic bytes per unit of work: per grid point of a stencil sweep, and
+# per AFM cell of a Ca sub-step 96 + 24 bytes per list entry (k_ca_cells)
 BYTES_PER_POINT = {"k_atp": 22.0, "k_ext_stim": 21.0}
+CA_CELL_BYTES, CA_ENTRY_BYTES = 96.0, 24.0
+MODELED = ("k_ca_cells", "k_atp", "k_ext_stim")
 
 
 def parse():
@@ -41,18 +44,22 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", choices=["large", "small", "batched", "adjoint-rd", "adjoint-mech"],
+    ap.add_argument("--config", choices=["large", "small", "batched", "adjoint-rd", "adjoint-mech", "ca-solve"],
                     default="large")
     ap.add_argument("--tissues", type=int, default=256, help="batched: number of tissues")
     ap.add_argument("--loop-sample", type=int, default=16,
                     help="batched: tissues timed one context at a time for the per-instance loop")
     ap.add_argument("--ca-every", type=int, default=500)
-    ap.add_argument("--roofline-kernel", default="k_atp")
+    ap.add_argument("--roofline-kernel", default="auto",
+                    help="kernel whose roofline is reported (auto: the dominant one)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="approximate CPU time budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes instead of running the autotuner")
+    ap.add_argument("--tuning", default=None,
+                    help="JSON dict of launch shapes to use instead of running the autotuner "
+                         "(profiling runs pin the shapes the timed run chose)")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB", default=0,
                     help="only run a plain device copy of MIB MiB five times (the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -62,6 +69,58 @@ def parse():
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel with events and print a breakdown to stderr")
     return ap.parse_args()
+
+
+def roofline(ts, prof, s0, s1, want="auto"):
+    """Roofline of the dominant kernel: the kernel with the largest total time
+    if it has a byte model, else the costliest modeled one.  Algorithmic bytes
+    per launch over the mean launch duration (dispatch-recorded HIP events on
+    the library's stream during the timed region)."""
+    tot = sum(v[0] for v in prof.values()) or 1.0
+    order = sorted(prof.items(), key=lambda kv: -kv[1][0])
+    kernels = {k: {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt, "share": round(ms / tot, 4)}
+               for k, (ms, cnt) in order[:8]}
+    rk = want
+    if rk == "auto":
+        top = order[0][0] if order else None
+        rk = top if top in MODELED else next((k for k, _ in order if k in MODELED), None)
+    if rk not in prof:
+        return None, kernels
+    ms, cnt = prof[rk]
+    avg_s = ms / cnt / 1e3
+    if rk == "k_ca_cells":
+        # every Ca sub-step of the timed region covers the solving tissues;
+        # weight each tissue's cell bytes by its share of the sub-steps
+        alg, units, entries = 0.0, 0, 0.0
+        subs = [b.grid_points_ca - a.grid_points_ca for a, b in zip(s0, s1)]
+        for t in range(ts.T):
+            if not subs[t]:
+                continue
+            c = ts.get_cells(t)
+            st, lens = c["state"], np.diff(c["conn_ptr"])
+            afm = (st == 0) | (st == 4) | (st == 7)
+            afm[:c["nmemb"] + c["nder"]] = False
+            w = (s1[t].ca_solves - s0[t].ca_solves) * 1000 / cnt
+            units += int(afm.sum() * w)
+            entries += float(lens[afm].sum()) * w
+        alg = CA_CELL_BYTES * units + CA_ENTRY_BYTES * entries
+        unit = f"{units} AFM cells, mean list {entries / max(units, 1):.2f}"
+    else:
+        key = "grid_points_ca" if rk == "k_atp" else "grid_points_b"
+        units = sum(getattr(b, key) - getattr(a, key) for a, b in zip(s0, s1)) / cnt
+        alg = BYTES_PER_POINT[rk] * units
+        unit = f"{units:.0f} grid points"
+    achieved = alg / avg_s / 1e9
+    traffic = None
+    pmc = os.path.join(HERE, "profiles", f"{rk}_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": rk,
+            "avg_us": round(avg_s * 1e6, 3), "launches": cnt, "units_per_launch": unit,
+            "algorithmic_bytes_per_launch": round(alg, 1)}
+    return roof, kernels
 
 
 def make_spec(name):
@@ -152,7 +211,7 @@ def run_batched(args):
     ts.synchronize()
     s0 = [ts.status(t)[0] for t in range(len(specs))]
     ts.profile_reset()
-    ts.profile_enable([args.roofline_kernel])
+    ts.profile_enable(None)
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -165,17 +224,7 @@ def run_batched(args):
                    for a, b in zip(s0, s1))
     ca_solves = sum(b.ca_solves - a.ca_solves for a, b in zip(s0, s1))
     errs = [b.err for b in s1 if b.err]
-    roof = None
-    if args.roofline_kernel in prof:
-        ms, cnt = prof[args.roofline_kernel]
-        avg_s = ms / cnt / 1e3
-        key = "grid_points_ca" if args.roofline_kernel == "k_atp" else "grid_points_b"
-        units = sum(getattr(b, key) - getattr(a, key) for a, b in zip(s0, s1)) / cnt
-        alg = BYTES_PER_POINT[args.roofline_kernel] * units
-        roof = {"bound": "hbm", "achieved": round(alg / avg_s / 1e9, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(alg / avg_s / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
-                "kernel": args.roofline_kernel, "avg_us": round(avg_s * 1e6, 3), "launches": cnt,
-                "algorithmic_bytes_per_launch": round(alg, 1)}
+    roof, kernels = roofline(ts, prof, s0, s1, args.roofline_kernel)
     ts.close()
     # per-instance loop over an evenly spaced sample of the same tissues
     idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
@@ -220,6 +269,7 @@ def run_batched(args):
                    "autotune_seconds": round(tune_s, 2)},
         "grid_cell_steps_per_s": round(grid_pts / wall, 1),
         "roofline": roof,
+        "kernels": kernels,
         "per_instance_loop": {"value": round(loop_value, 1), "unit": "cell-steps/s",
                               "tissues_timed": len(idx), "steps": args.steps,
                               "sample": "each sampled tissue in its own context, one after another, "
@@ -281,6 +331,36 @@ def _timed(ts, fn):
     out = fn()
     ts.synchronize()
     return time.perf_counter() - t0, out
+
+
+def run_ca_solve(args):
+    """Whole calc_ca2p solves (1000 sub-steps) on the large tissue after a few
+    steps: wall time per sub-step beside the kernels' event times."""
+    from epi_amd import native
+    spec = make_spec("large")
+    ts = native.TissueSet([spec["geometry"]])
+    ts.load(0, spec)
+    ts.set_rng(0, native.mt19937_64_uniform(spec["seed"], n_draws_needed(spec, 64)))
+    if not args.no_tune:
+        from epi_amd import autotune
+        tuning, _ = autotune.tune(ts, [native.Driver(0, 0, 0, 0, 0)], log=sys.stderr)
+    ts.step([native.Driver(0, 0, 0, 0, 0)], 5)
+    zz = ts.calc_zzmax()
+    ts.calc_ca2p(zz)
+    solves = 3
+    t_wall, _ = _timed(ts, lambda: [ts.calc_ca2p(zz) for _ in range(solves)])
+    ts.profile_reset()
+    ts.profile_enable(["k_ca_cells", "k_atp"])
+    ts.calc_ca2p(zz)
+    ts.synchronize()
+    prof = ts.profile()
+    ts.profile_enable([])
+    out = {"metric": "calc_ca2p sub-step wall time", "value": round(1e6 * t_wall / (solves * 1000), 3), "unit": "us",
+           "higher_is_better": False, "launch_shapes": ts.get_tuning(),
+           "kernel_us": {k: round(1e3 * v[0] / v[1], 3) for k, v in prof.items()}}
+    print(json.dumps(out), flush=True)
+    ts.close()
+    return 0
 
 
 def run_adjoint_rd(args):
@@ -379,6 +459,8 @@ def main():
         return 0
     if args.config == "batched":
         return run_batched(args)
+    if args.config == "ca-solve":
+        return run_ca_solve(args)
     if args.config == "adjoint-rd":
         return run_adjoint_rd(args)
     if args.config == "adjoint-mech":
@@ -389,7 +471,9 @@ def main():
     geo = spec["geometry"]
     draws = native.mt19937_64_uniform(seed, n_draws_needed(spec, args.steps + args.warmup))
     tuning, tune_s = None, 0.0
-    if not args.no_tune:
+    if args.tuning:
+        tuning = json.loads(args.tuning)
+    elif not args.no_tune:
         # launch shapes are chosen on a throwaway copy of the workload
         from epi_amd import autotune
         tt = native.TissueSet([geo])
@@ -409,7 +493,7 @@ def main():
     ts.synchronize()
     s0, _ = ts.status(0)
     ts.profile_reset()
-    ts.profile_enable(None if args.profile_all else [args.roofline_kernel])
+    ts.profile_enable(None)   # every kernel, with dispatch-recorded events
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -422,27 +506,7 @@ def main():
     grid_pts = (s1.grid_points_b - s0.grid_points_b) + (s1.grid_points_ca - s0.grid_points_ca)
     ca_solves = s1.ca_solves - s0.ca_solves
     value = cell_steps / wall
-    # roofline of the dominant kernel
-    rk = args.roofline_kernel
-    roof = None
-    if rk in prof:
-        ms, cnt = prof[rk]
-        avg_s = ms / cnt / 1e3
-        if rk == "k_atp":
-            units = (s1.grid_points_ca - s0.grid_points_ca) / cnt
-        else:
-            units = (s1.grid_points_b - s0.grid_points_b) / cnt
-        alg = BYTES_PER_POINT[rk] * units
-        achieved = alg / avg_s / 1e9
-        traffic = None
-        pmc = os.path.join(HERE, "profiles", f"{rk}_traffic.json")
-        if os.path.exists(pmc):
-            with open(pmc) as f:
-                traffic = json.load(f).get("bytes_per_launch")
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": rk,
-                "avg_us": round(avg_s * 1e6, 3), "launches": cnt,
-                "algorithmic_bytes_per_launch": round(alg, 1)}
+    roof, kernels = roofline(ts, prof, [s0], [s1], args.roofline_kernel)
     if args.profile_all:
         tot = sum(v[0] for v in prof.values())
         for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
@@ -472,6 +536,7 @@ def main():
         "grid_cell_steps_per_s": round(grid_pts / wall, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "kernels": kernels,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

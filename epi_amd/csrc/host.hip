@@ -49,16 +49,11 @@ hipEvent_t take_event() {
 }
 }  // namespace
 bool prof_on(int k) { return (g_prof.mask >> k) & 1ull; }
-hipEvent_t prof_start(hipStream_t s) {
-  hipEvent_t a = take_event();
-  (void)hipEventRecord(a, s);
-  return a;
+void prof_events(hipEvent_t* a, hipEvent_t* b) {
+  *a = take_event();
+  *b = take_event();
 }
-void prof_stop(hipStream_t s, int k, hipEvent_t a) {
-  hipEvent_t b = take_event();
-  (void)hipEventRecord(b, s);
-  g_prof.pending.push_back({k, a, b});
-}
+void prof_add(int k, hipEvent_t a, hipEvent_t b) { g_prof.pending.push_back({k, a, b}); }
 }  // namespace epi
 
 namespace {
@@ -1161,10 +1156,11 @@ int epi_set_maps(epi_ctx* c, int32_t t, const int32_t* cmap1, const uint8_t* cma
 // ---- frozen reaction-diffusion window and its adjoint ----------------------
 namespace {
 
+// threads for the window's copy kernels: 16 lanes per cell for the rows,
+// one per cell, one per grid point
 long long rd_items(const epi_ctx* c) {
   long long m = 1;
-  for (int t = 0; t < c->T; t++)
-    m = std::max({m, (long long)c->th[t].n_est * c->th[t].ns, c->th[t].gsize});
+  for (int t = 0; t < c->T; t++) m = std::max({m, (long long)(c->th[t].n_est + 64) * 16, c->th[t].gsize});
   return m;
 }
 

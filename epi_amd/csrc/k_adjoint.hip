@@ -19,6 +19,15 @@ namespace epi {
 
 namespace {
 constexpr double STIM11 = 0.002, Kaa = 0.5, Da = 1.0, H0 = 0.5;
+constexpr int RL = 16;   // lanes per cell for row-wise copies
+
+// every (cell, list entry) of the rows of buffer cb, RL lanes per cell
+#define RD_FOR_ROW_ENTRIES(t, cb, n, c, i)                                                      \
+  for (int c = (int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) / RL); c < (n);          \
+       c += (int)((long long)gridDim.x * blockDim.x / RL))                                       \
+    for (long long i = (long long)c * (t).ns + (threadIdx.x & (RL - 1)),                         \
+                   i##_end = (long long)c * (t).ns + (t).nlen[cb][c];                            \
+         i < i##_end; i += RL)
 
 __device__ __forceinline__ long long box_end(const Tis& t, const RdTis& r) { return (long long)r.izb * t.PL; }
 
@@ -41,10 +50,7 @@ __global__ void __launch_bounds__(256) k_rd_restore(const Tis* __restrict__ T, R
   const int n = t.st->n;
   const double g0 = gj0[blockIdx.y];
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns; i += stride) {
-    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
-    if (e < t.nlen[cb][c]) t.gj[cb][i] = g0;
-  }
+  RD_FOR_ROW_ENTRIES(t, cb, n, c, i) t.gj[cb][i] = g0;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
     t.ca2p[cp][c] = r.u0[0][c]; t.ca2p[1 - cp][c] = r.u0[1][c];
     t.ip3[cp][c] = r.p0[0][c]; t.ip3[1 - cp][c] = r.p0[1][c];
@@ -65,10 +71,7 @@ __global__ void __launch_bounds__(256) k_rd_snapshot(const Tis* __restrict__ T, 
   const int start = t.nmemb + t.nder;
   const long long stride = (long long)gridDim.x * blockDim.x;
   int nafm = 0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns; i += stride) {
-    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
-    if (e < t.nlen[cb][c]) r.g0[i] = t.gj[cb][i];
-  }
+  RD_FOR_ROW_ENTRIES(t, cb, n, c, i) r.g0[i] = t.gj[cb][i];
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
     r.u0[0][c] = t.ca2p[cp][c]; r.u0[1][c] = t.ca2p[1 - cp][c];
     r.p0[0][c] = t.ip3[cp][c]; r.p0[1][c] = t.ip3[1 - cp][c];
@@ -87,11 +90,7 @@ __global__ void __launch_bounds__(256) k_rd_put_gj(const Tis* __restrict__ T, Rd
   const Tis& t = T[blockIdx.y];
   const RdTis& r = R[blockIdx.y];
   const int n = t.st->n;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
-    if (e < t.nlen[cb][c]) t.gj[cb][i] = r.g0[i];
-  }
+  RD_FOR_ROW_ENTRIES(t, cb, n, c, i) t.gj[cb][i] = r.g0[i];
 }
 
 // copy the state between the tissue and a segment slot (dir 0: save,
@@ -105,12 +104,9 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
   const long long stride = (long long)gridDim.x * blockDim.x;
   const bool ck = dir >= 2, save = (dir & 1) == 0;
   double* gs = (ck ? r.cg : r.sg) + slot * cap * ns;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * ns; i += stride) {
-    const int c = (int)(i / ns), e = (int)(i % ns);
-    if (e < t.nlen[cb][c]) {
-      if (save) gs[i] = t.gj[cb][i];
-      else t.gj[cb][i] = gs[i];
-    }
+  RD_FOR_ROW_ENTRIES(t, cb, n, c, i) {
+    if (save) gs[i] = t.gj[cb][i];
+    else t.gj[cb][i] = gs[i];
   }
   const int nb = ck ? 2 : 1;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
@@ -153,10 +149,7 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
     }
     r.lu[q][c] = s; r.lp[q][c] = 0; r.lh[q][c] = 0; r.gkpa[c] = 0;
   }
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns; i += stride) {
-    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
-    if (e < t.nlen[cb][c]) r.lg[q][i] = 0;
-  }
+  RD_FOR_ROW_ENTRIES(t, cb, n, c, i) r.lg[q][i] = 0;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) r.lA[q][v] = 0;
   for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
   if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[0], part / r.nafm);
@@ -188,8 +181,19 @@ __global__ void __launch_bounds__(256) k_rd_acc(const Tis* __restrict__ T, RdTis
   }
 }
 
-// reverse of dead_IP3_calc and supra_calc for one sub-step: one thread per
-// cell, the state before the sub-step from slot `slot`
+// reverse of dead_IP3_calc and supra_calc for one sub-step, the state before
+// the sub-step from slot `slot`.  A group of RG lanes owns one cell: lanes
+// take the list entries, their partial sums are reduced over the group (a
+// gradient need not follow the forward's summation order), and each lane
+// scatters its entries' contributions.
+constexpr int RG = 8;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < RG; o <<= 1) v += __shfl_xor(v, o, RG);
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
                                                    int cb, int bp, int q) {
   const Tis& t = T[blockIdx.y];
@@ -197,6 +201,8 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
   const int n = t.st->n;
   const int start = t.nmemb + t.nder;
   const long long cap = t.cap, ns = t.ns;
+  const int lane = threadIdx.x & (RG - 1);
+  const int groups = gridDim.x * (blockDim.x / RG);
   const double* u = r.su + slot * cap;
   const double* p = r.sp + slot * cap;
   const double* h = r.sh + slot * cap;
@@ -206,20 +212,27 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
   double *nu = r.lu[1 - q], *np = r.lp[1 - q], *nh = r.lh[1 - q], *ng = r.lg[1 - q], *nA = r.lA[1 - q];
   const double du = t.ca_du, dp = t.ca_dp;
   double pdu = 0, pdp = 0;
-  for (int c = start + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+  for (int c = start + blockIdx.x * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
     const uint8_t s = t.state[c];
     const int* row = t.col[cb] + (size_t)c * ns;
     const uint8_t* kind = t.ca_kind + (size_t)c * ns;
     const int len = t.nlen[cb][c];
+    const double lpc = lp[c];
     if (s == DEAD) {
-      int cnt = 0;
-      double sum = 0;
-      for (int e = 0; e < len; e++)
-        if (kind[e] == 3) { sum += p[row[e]]; cnt++; }
-      atomicAdd(&np[c], lp[c] * (1.0 - DT_Ca * dp * cnt - DT_Ca * Kpp));
-      for (int e = 0; e < len; e++)
-        if (kind[e] == 3) atomicAdd(&np[row[e]], lp[c] * DT_Ca * dp);
-      pdp += lp[c] * DT_Ca * (sum - cnt * p[c]);
+      double cnt = 0, sum = 0;
+      for (int e = lane; e < len; e += RG)
+        if (kind[e] == 3) {
+          const int o = row[e];
+          sum += p[o];
+          cnt += 1;
+          atomicAdd(&np[o], lpc * DT_Ca * dp);
+        }
+      cnt = group_sum(cnt);
+      sum = group_sum(sum);
+      if (lane == 0) {
+        atomicAdd(&np[c], lpc * (1.0 - DT_Ca * dp * cnt - DT_Ca * Kpp));
+        pdp += lpc * DT_Ca * (sum - cnt * p[c]);
+      }
       continue;
     }
     if (!is_afm(s)) continue;
@@ -227,38 +240,30 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
     const double* gr = g + (size_t)c * ns;
     const double uc = u[c], pc = p[c], hc = h[c];
     double su = 0, sp = 0, gsum = 0;
-    for (int e = 0; e < len; e++) {
+    for (int e = lane; e < len; e += RG) {
       if (!kind[e]) continue;
       const int o = row[e];
       su += gr[e] * (u[o] - uc);
       sp += gr[e] * (p[o] - pc);
       gsum += gr[e];
     }
+    su = group_sum(su);
+    sp = group_sum(sp);
+    gsum = group_sum(gsum);
     const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
-    double Aavg = 0;
-    {
-      // A holds planes 0..izb of the grid in the device layout
-      const long long b0 = gidx(t, ix, iy, iz);
-      Aavg = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
-                      + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
-    }
+    const long long b0 = gidx(t, ix, iy, iz);
+    const double Aavg = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
+                                 + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
     const double iag = t.ca_iag[c], kpa = t.ca_kpa[c], th = t.ca_th[c];
     const double diffu = ca2p_reaction_factor(uc, hc, pc, Bavg) + du * iag * su;
-    const double kflux = 8.1, mu0 = 0.567, mu1 = 0.1, kmu = 0.05, pb = 0.11, pbb = 0.89, pk1 = 0.7;
-    const double kg = 0.1, gam = 2.0, k2 = 0.7;
-    const double e1 = mu0 + mu1 * pc / (pc + kmu), e2 = pb + pbb * uc / (pk1 + uc);
-    const double dRdu = kflux * e1 * (pbb * pk1 / ((pk1 + uc) * (pk1 + uc))) * hc - gam * kg / ((kg + uc) * (kg + uc));
-    const double dRdh = kflux * e1 * e2;
-    const double dRdp = kflux * (mu1 * kmu / ((pc + kmu) * (pc + kmu))) * e2 * hc;
     const double abar = DT_Ca * lu[c] + (diffu > 0 ? r.accA[c] * DT_Ca * STIM11 : 0.0);
-    const double qq = k2 * k2 + uc * uc;
-    double own_u = lu[c] + abar * (dRdu - du * iag * gsum) + lh[c] * DT_Ca * (-2.0 * uc * k2 * k2 / (qq * qq)) / th;
-    for (int e = 0; e < len; e++) {
+    double own_u = 0;
+    for (int e = lane; e < len; e += RG) {
       if (!kind[e]) continue;
       const int o = row[e];
       double to_u = abar * du * iag * gr[e];
-      double lge = lg[(size_t)c * ns + e];
-      const double lg_in = lge;
+      const double lg_in = lg[(size_t)c * ns + e];
+      double lge = lg_in;
       if (kind[e] == 3) {
         const double d = u[o] - uc;
         const double tz = tanh((0.1 - fabs(d)) / 0.1);
@@ -268,21 +273,32 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
         own_u -= lg_in * DT_Ca * fwd * sg;
         lge = lg_in * (1.0 - DT_Ca);
       }
-      ng[(size_t)c * ns + e] = lge + abar * du * iag * (u[o] - uc) + lp[c] * DT_Ca * dp * iag * (p[o] - pc);
+      ng[(size_t)c * ns + e] = lge + abar * du * iag * (u[o] - uc) + lpc * DT_Ca * dp * iag * (p[o] - pc);
       if (is_afm(t.state[o])) atomicAdd(&nu[o], to_u);
-      atomicAdd(&np[o], lp[c] * DT_Ca * dp * iag * gr[e]);
+      atomicAdd(&np[o], lpc * DT_Ca * dp * iag * gr[e]);
     }
-    atomicAdd(&nu[c], own_u);
-    atomicAdd(&np[c], lp[c] * (1.0 - DT_Ca * Kpp - DT_Ca * dp * iag * gsum) + abar * dRdp);
-    atomicAdd(&nh[c], lh[c] * (1.0 - DT_Ca / th) + abar * dRdh);
-    const double wA = lp[c] * DT_Ca * kpa * H0 / ((H0 + Aavg) * (H0 + Aavg)) * 0.125;
-    for (int q8 = 0; q8 < 8; q8++) {
-      const long long vv = corner_var(t, r, ix + (q8 & 1), iy + ((q8 >> 1) & 1), iz + ((q8 >> 2) & 1));
+    own_u = group_sum(own_u);
+    if (lane < 8) {   // the eight grid_avg8 corners of the IP3 production term
+      const double wA = lpc * DT_Ca * kpa * H0 / ((H0 + Aavg) * (H0 + Aavg)) * 0.125;
+      const long long vv = corner_var(t, r, ix + (lane & 1), iy + ((lane >> 1) & 1), iz + ((lane >> 2) & 1));
       if (vv >= 0) atomicAdd(&nA[vv], wA);
     }
-    pdu += abar * iag * su;
-    pdp += lp[c] * DT_Ca * iag * sp;
-    r.gkpa[c] += lp[c] * DT_Ca * Aavg / (H0 + Aavg);
+    if (lane == 0) {
+      const double kflux = 8.1, mu0 = 0.567, mu1 = 0.1, kmu = 0.05, pb = 0.11, pbb = 0.89, pk1 = 0.7;
+      const double kg = 0.1, gam = 2.0, k2 = 0.7;
+      const double e1 = mu0 + mu1 * pc / (pc + kmu), e2 = pb + pbb * uc / (pk1 + uc);
+      const double dRdu = kflux * e1 * (pbb * pk1 / ((pk1 + uc) * (pk1 + uc))) * hc - gam * kg / ((kg + uc) * (kg + uc));
+      const double dRdh = kflux * e1 * e2;
+      const double dRdp = kflux * (mu1 * kmu / ((pc + kmu) * (pc + kmu))) * e2 * hc;
+      const double qq = k2 * k2 + uc * uc;
+      atomicAdd(&nu[c], own_u + lu[c] + abar * (dRdu - du * iag * gsum)
+                            + lh[c] * DT_Ca * (-2.0 * uc * k2 * k2 / (qq * qq)) / th);
+      atomicAdd(&np[c], lpc * (1.0 - DT_Ca * Kpp - DT_Ca * dp * iag * gsum) + abar * dRdp);
+      atomicAdd(&nh[c], lh[c] * (1.0 - DT_Ca / th) + abar * dRdh);
+      pdu += abar * iag * su;
+      pdp += lpc * DT_Ca * iag * sp;
+      r.gkpa[c] += lpc * DT_Ca * Aavg / (H0 + Aavg);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     pdu += __shfl_down(pdu, o, 64);
@@ -336,11 +352,8 @@ __global__ void __launch_bounds__(256) k_rd_gj0(const Tis* __restrict__ T, RdTis
   const int n = t.st->n;
   const int start = t.nmemb + t.nder;
   double part = 0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * t.ns;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i / t.ns), e = (int)(i % t.ns);
-    if (c >= start && e < t.nlen[cb][c] && is_afm(t.state[c]) && t.ca_kind[i]) part += r.lg[q][i];
-  }
+  RD_FOR_ROW_ENTRIES(t, cb, n, c, i)
+    if (c >= start && is_afm(t.state[c]) && t.ca_kind[i]) part += r.lg[q][i];
   for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
   if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[3], part);
 }
@@ -376,7 +389,7 @@ void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int nti
                                 int max_n, long long max_box) {
   hipLaunchKernelGGL(k_rd_zero, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q);
   hipLaunchKernelGGL(k_rd_acc, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
-  hipLaunchKernelGGL(k_rd_cells, cells_grid(max_n, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q);
+  hipLaunchKernelGGL(k_rd_cells, cells_grid((long long)max_n * RG, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q);
   hipLaunchKernelGGL(k_rd_grid, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
 }
 void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items) {

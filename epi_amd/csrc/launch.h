@@ -2,6 +2,7 @@
 // and the optional per-kernel event timer.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 namespace epi {
 struct Tis;
@@ -20,19 +21,24 @@ struct Tuning {
   int ca_group = 16;          // k_ca_cells: lanes per cell
 };
 
-// Event timer: when the bit of a kernel id is set, its launches are
-// bracketed by HIP events on the launch stream (see host.hip).
+// Event timer: when the bit of a kernel id is set, its launches carry a
+// start and a stop event that the dispatch itself records
+// (hipExtLaunchKernelGGL), so the elapsed time is the kernel's own duration
+// without the cost of separate event packets (see host.hip).
 bool prof_on(int k);
-hipEvent_t prof_start(hipStream_t s);
-void prof_stop(hipStream_t s, int k, hipEvent_t a);
+void prof_events(hipEvent_t* a, hipEvent_t* b);
+void prof_add(int k, hipEvent_t a, hipEvent_t b);
 
-#define EPI_LAUNCH(kid, kernel, grid, block, shmem, stream, ...)                \
-  do {                                                                          \
-    hipEvent_t a_ = nullptr;                                                    \
-    const bool on_ = ::epi::prof_on(kid);                                       \
-    if (on_) a_ = ::epi::prof_start(stream);                                    \
-    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);        \
-    if (on_) ::epi::prof_stop(stream, kid, a_);                                 \
+#define EPI_LAUNCH(kid, kernel, grid, block, shmem, stream, ...)                              \
+  do {                                                                                        \
+    if (::epi::prof_on(kid)) {                                                                \
+      hipEvent_t a_ = nullptr, b_ = nullptr;                                                  \
+      ::epi::prof_events(&a_, &b_);                                                           \
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, a_, b_, 0, __VA_ARGS__);      \
+      ::epi::prof_add(kid, a_, b_);                                                           \
+    } else {                                                                                  \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                    \
+    }                                                                                         \
   } while (0)
 
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other);
