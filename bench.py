@@ -71,21 +71,33 @@ def parse():
     return ap.parse_args()
 
 
-def roofline(ts, prof, s0, s1, want="auto"):
-    """Roofline of the dominant kernel: the kernel with the largest total time
-    if it has a byte model, else the costliest modeled one.  Algorithmic bytes
-    per launch over the mean launch duration (dispatch-recorded HIP events on
-    the library's stream during the timed region)."""
+def kernel_mix(ts, native, steps):
+    """An untimed pass of `steps` steps (one Ca period, so exactly one solve)
+    with every kernel timed: the share of each kernel and the dominant one
+    (the kernel with the largest total time if it has a byte model, else the
+    costliest modeled one).  The timed region then times only that kernel."""
+    ts.profile_reset()
+    ts.profile_enable(None)
+    ts.step([native.Driver(0, 0, 0, steps, 0) for _ in range(ts.T)], steps)
+    ts.synchronize()
+    prof = ts.profile()
+    ts.profile_enable([])
+    ts.profile_reset()
     tot = sum(v[0] for v in prof.values()) or 1.0
     order = sorted(prof.items(), key=lambda kv: -kv[1][0])
-    kernels = {k: {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt, "share": round(ms / tot, 4)}
-               for k, (ms, cnt) in order[:8]}
-    rk = want
-    if rk == "auto":
-        top = order[0][0] if order else None
-        rk = top if top in MODELED else next((k for k, _ in order if k in MODELED), None)
+    mix = {k: {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt, "share": round(ms / tot, 4)}
+           for k, (ms, cnt) in order[:8]}
+    top = order[0][0] if order else None
+    dominant = top if top in MODELED else next((k for k, _ in order if k in MODELED), None)
+    return mix, dominant
+
+
+def roofline(ts, prof, s0, s1, rk):
+    """Algorithmic bytes per launch of kernel rk over its mean launch
+    duration (dispatch-recorded HIP events on the library's stream during the
+    timed region)."""
     if rk not in prof:
-        return None, kernels
+        return None
     ms, cnt = prof[rk]
     avg_s = ms / cnt / 1e3
     if rk == "k_ca_cells":
@@ -120,7 +132,7 @@ def roofline(ts, prof, s0, s1, want="auto"):
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": rk,
             "avg_us": round(avg_s * 1e6, 3), "launches": cnt, "units_per_launch": unit,
             "algorithmic_bytes_per_launch": round(alg, 1)}
-    return roof, kernels
+    return roof
 
 
 def make_spec(name):
@@ -184,7 +196,7 @@ def run_batched(args):
     from epi_amd import autotune, configs, native
     specs = configs.batched_configs(args.tissues)
     geos = [s["geometry"] for s in specs]
-    total = args.steps + args.warmup
+    total = args.steps + args.warmup + args.ca_every
     draws = [native.mt19937_64_uniform(s["seed"], n_draws_needed(s, total)) for s in specs]
     drivers = lambda k: [native.Driver(0, 0, 0, args.ca_every, 0) for _ in range(k)]
 
@@ -208,10 +220,12 @@ def run_batched(args):
     drv = drivers(len(specs))
     if args.warmup:
         ts.step(drv, args.warmup)
+    kernels, dominant = kernel_mix(ts, native, args.ca_every)
+    rk = dominant if args.roofline_kernel == "auto" else args.roofline_kernel
     ts.synchronize()
     s0 = [ts.status(t)[0] for t in range(len(specs))]
     ts.profile_reset()
-    ts.profile_enable(None)
+    ts.profile_enable([rk])
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -224,7 +238,7 @@ def run_batched(args):
                    for a, b in zip(s0, s1))
     ca_solves = sum(b.ca_solves - a.ca_solves for a, b in zip(s0, s1))
     errs = [b.err for b in s1 if b.err]
-    roof, kernels = roofline(ts, prof, s0, s1, args.roofline_kernel)
+    roof = roofline(ts, prof, s0, s1, rk)
     ts.close()
     # per-instance loop over an evenly spaced sample of the same tissues
     idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
@@ -269,7 +283,7 @@ def run_batched(args):
                    "autotune_seconds": round(tune_s, 2)},
         "grid_cell_steps_per_s": round(grid_pts / wall, 1),
         "roofline": roof,
-        "kernels": kernels,
+        "kernel_mix": kernels,
         "per_instance_loop": {"value": round(loop_value, 1), "unit": "cell-steps/s",
                               "tissues_timed": len(idx), "steps": args.steps,
                               "sample": "each sampled tissue in its own context, one after another, "
@@ -363,6 +377,21 @@ def run_ca_solve(args):
     return 0
 
 
+def window_mix(ts, fn):
+    """Per-kernel time of one untimed call of fn with every kernel timed
+    (dispatch-recorded events): {kernel: {"ms", "launches"}}."""
+    ts.synchronize()
+    ts.profile_reset()
+    ts.profile_enable(None)
+    fn()
+    ts.synchronize()
+    prof = ts.profile()
+    ts.profile_enable([])
+    ts.profile_reset()
+    return {k: {"ms": round(ms, 4), "launches": cnt}
+            for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+
+
 def run_adjoint_rd(args):
     """Reaction-diffusion window of 200 calc_ca2p sub-steps on a 64x64x128
     grid with 5,000 frozen free cells; loss = mean squared AFM Ca2+ against a
@@ -392,6 +421,8 @@ def run_adjoint_rd(args):
     ts.rd_adjoint([base], n_sub, every, [target])   # warm
     t_adj, (loss, grads) = min((_timed(ts, lambda: ts.rd_adjoint([base], n_sub, every, [target]))
                                 for _ in range(reps)), key=lambda r: r[0])
+    mix = {"forward": window_mix(ts, lambda: ts.rd_forward([base], n_sub)),
+           "adjoint": window_mix(ts, lambda: ts.rd_adjoint([base], n_sub, every, [target]))}
     ts.rd_end()
     gk = grads[0]["kpa"]
     out = {"metric": "adjoint cost / forward cost (reaction-diffusion window)", "value": round(t_adj / t_fwd, 3),
@@ -401,7 +432,8 @@ def run_adjoint_rd(args):
            "forward_ms": round(1e3 * t_fwd, 3), "adjoint_ms": round(1e3 * t_adj, 3),
            "forward_substeps_per_s": round(n_sub / t_fwd, 1), "loss": float(loss[0]),
            "grad": {"du": grads[0]["du"], "dp": grads[0]["dp"], "gj0": grads[0]["gj0"],
-                    "kpa_l2": float(np.linalg.norm(gk))}}
+                    "kpa_l2": float(np.linalg.norm(gk))},
+           "kernel_mix": mix}
     print(json.dumps(out), flush=True)
     ts.close()
     return 0
@@ -430,6 +462,8 @@ def run_adjoint_mech(args):
     ts.mech_adjoint(base, n_steps, every, targets)   # warm
     t_adj, (loss, grads) = min((_timed(ts, lambda: ts.mech_adjoint(base, n_steps, every, targets))
                                 for _ in range(reps)), key=lambda r: r[0])
+    mix = {"forward": window_mix(ts, lambda: ts.mech_forward(base, n_steps)),
+           "adjoint": window_mix(ts, lambda: ts.mech_adjoint(base, n_steps, every, targets))}
     ts.mech_end()
     cells = sum(s["cells"]["x"].size for s in specs)
     out = {"metric": "adjoint cost / forward cost (batched mechanics windows)", "value": round(t_adj / t_fwd, 3),
@@ -439,7 +473,8 @@ def run_adjoint_mech(args):
            "forward_ms": round(1e3 * t_fwd, 3), "adjoint_ms": round(1e3 * t_adj, 3),
            "forward_cell_steps_per_s": round(cells * n_steps / t_fwd, 1),
            "loss_mean": float(np.mean(loss)),
-           "grad_mean": {k: float(np.mean([g[k] for g in grads])) for k in grads[0]}}
+           "grad_mean": {k: float(np.mean([g[k] for g in grads])) for k in grads[0]},
+           "kernel_mix": mix}
     print(json.dumps(out), flush=True)
     ts.close()
     return 0
@@ -469,7 +504,7 @@ def main():
     spec = make_spec(args.config)
     seed = spec["seed"]
     geo = spec["geometry"]
-    draws = native.mt19937_64_uniform(seed, n_draws_needed(spec, args.steps + args.warmup))
+    draws = native.mt19937_64_uniform(seed, n_draws_needed(spec, args.steps + args.warmup + args.ca_every))
     tuning, tune_s = None, 0.0
     if args.tuning:
         tuning = json.loads(args.tuning)
@@ -490,10 +525,12 @@ def main():
     drv = [native.Driver(0, 0, 0, args.ca_every, 0)]
     if args.warmup:
         ts.step(drv, args.warmup)
+    kernels, dominant = kernel_mix(ts, native, args.ca_every)
+    rk = dominant if args.roofline_kernel == "auto" else args.roofline_kernel
     ts.synchronize()
     s0, _ = ts.status(0)
     ts.profile_reset()
-    ts.profile_enable(None)   # every kernel, with dispatch-recorded events
+    ts.profile_enable(None if args.profile_all else [rk])
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -506,7 +543,7 @@ def main():
     grid_pts = (s1.grid_points_b - s0.grid_points_b) + (s1.grid_points_ca - s0.grid_points_ca)
     ca_solves = s1.ca_solves - s0.ca_solves
     value = cell_steps / wall
-    roof, kernels = roofline(ts, prof, [s0], [s1], args.roofline_kernel)
+    roof = roofline(ts, prof, [s0], [s1], rk)
     if args.profile_all:
         tot = sum(v[0] for v in prof.values())
         for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
@@ -536,7 +573,7 @@ def main():
         "grid_cell_steps_per_s": round(grid_pts / wall, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels": kernels,
+        "kernel_mix": kernels,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -62,7 +62,10 @@ const char* kKernelNames[K_COUNT] = {
     "k_forces", "k_forces_long", "k_renew_flags", "k_renew_scan", "k_renew_apply", "k_remove", "k_pair_disperse",
     "k_initialize_sc", "k_periodic_fix", "k_bucket_insert", "k_bucket_sort", "k_sweep",
     "k_finish", "k_snapshot", "k_zz_set", "k_zzmax", "k_map_clear", "k_map_stamp", "k_map_halo",
-    "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset"};
+    "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
+    "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed", "k_rd_acc",
+    "k_rd_cells", "k_rd_grid", "k_rd_gj0", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
+    "k_mech_bwd_pairs", "k_mech_bwd_memb"};
 
 void prof_resolve() {
   for (auto& r : g_prof.pending) {
@@ -1325,10 +1328,14 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
   RCHK(rd_params(c, prm));
   const long long items = rd_items(c);
   launch_rd_restore(c->stream, c->d_tis, rd.d, rd.d_gj0, c->T, c->cb, c->cp, c->ap, 1, items);
-  // forward, checkpoint every K sub-steps
+  // forward, checkpoint every K sub-steps; the last segment saves its states
+  // straight into the slots, so the reverse pass starts without a recompute
   std::vector<int> ck_cp(nck), ck_ap(nck);
+  const int last0 = (nck - 1) * K;
   for (int s = 0; s < n_sub; s++) {
-    if (s % K == 0) {
+    if (s >= last0) {
+      launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s - last0, 0, c->cb, c->cp, c->ap, items);
+    } else if (s % K == 0) {
       ck_cp[s / K] = c->cp;
       ck_ap[s / K] = c->ap;
       launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s / K, 2, c->cb, c->cp, c->ap, items);
@@ -1340,13 +1347,15 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
   const int mn = max_n(c);
   const long long mb = rd_max_box(c);
   for (int k = nck - 1; k >= 0; k--) {
-    c->cp = ck_cp[k];
-    c->ap = ck_ap[k];
-    launch_rd_state(c->stream, c->d_tis, rd.d, c->T, k, 3, c->cb, c->cp, c->ap, items);
     const int s0 = k * K, s1 = std::min(n_sub, s0 + K);
-    for (int s = s0; s < s1; s++) {
-      launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s - s0, 0, c->cb, c->cp, c->ap, items);
-      RCHK(rd_substep(c));
+    if (k < nck - 1) {
+      c->cp = ck_cp[k];
+      c->ap = ck_ap[k];
+      launch_rd_state(c->stream, c->d_tis, rd.d, c->T, k, 3, c->cb, c->cp, c->ap, items);
+      for (int s = s0; s < s1; s++) {
+        launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s - s0, 0, c->cb, c->cp, c->ap, items);
+        RCHK(rd_substep(c));
+      }
     }
     for (int s = s1 - 1; s >= s0; s--) {
       launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, mn, mb);
@@ -1492,9 +1501,14 @@ int epi_mech_adjoint(epi_ctx* c, const epi_mech_params* prm, int32_t n_steps, in
   RCHK(mech_params(c, prm));
   const int mn = max_n(c);
   launch_mech_state(c->stream, c->d_tis, m.d, c->T, 0, 4, c->pb, mn);
+  // checkpoint every K steps; the last segment saves its states straight
+  // into the slots, so the reverse pass starts without a recompute
   std::vector<int> ck_pb(nck);
+  const int last0 = (nck - 1) * K;
   for (int s = 0; s < n_steps; s++) {
-    if (s % K == 0) {
+    if (s >= last0) {
+      launch_mech_state(c->stream, c->d_tis, m.d, c->T, s - last0, 0, c->pb, mn);
+    } else if (s % K == 0) {
       ck_pb[s / K] = c->pb;
       launch_mech_state(c->stream, c->d_tis, m.d, c->T, s / K, 2, c->pb, mn);
     }
@@ -1503,12 +1517,14 @@ int epi_mech_adjoint(epi_ctx* c, const epi_mech_params* prm, int32_t n_steps, in
   int q = 0;
   launch_mech_seed(c->stream, c->d_tis, m.d, c->T, c->pb, q, mn);
   for (int k = nck - 1; k >= 0; k--) {
-    c->pb = ck_pb[k];
-    launch_mech_state(c->stream, c->d_tis, m.d, c->T, k, 3, c->pb, mn);
     const int s0 = k * K, s1 = std::min(n_steps, s0 + K);
-    for (int s = s0; s < s1; s++) {
-      launch_mech_state(c->stream, c->d_tis, m.d, c->T, s - s0, 0, c->pb, mn);
-      RCHK(mech_step(c));
+    if (k < nck - 1) {
+      c->pb = ck_pb[k];
+      launch_mech_state(c->stream, c->d_tis, m.d, c->T, k, 3, c->pb, mn);
+      for (int s = s0; s < s1; s++) {
+        launch_mech_state(c->stream, c->d_tis, m.d, c->T, s - s0, 0, c->pb, mn);
+        RCHK(mech_step(c));
+      }
     }
     for (int s = s1 - 1; s >= s0; s--) {
       launch_mech_backward_step(c->stream, c->d_tis, m.d, c->T, s - s0, c->pb, c->cb, q, mn);

@@ -131,7 +131,10 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
   }
 }
 
-// the loss and the seed of the reverse pass
+// the loss and the seed of the reverse pass; both parities of the cell
+// adjoints and the per-cell ATP sums start at zero (the sub-steps keep them
+// so: k_rd_acc clears the ATP adjoint it scatters into, k_rd_grid the cell
+// adjoints the next sub-step scatters into)
 __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTis* __restrict__ R, int cp, int cb,
                                                   int q) {
   const Tis& t = T[blockIdx.y];
@@ -148,6 +151,7 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
       s = 2.0 * d / r.nafm;
     }
     r.lu[q][c] = s; r.lp[q][c] = 0; r.lh[q][c] = 0; r.gkpa[c] = 0;
+    r.lu[1 - q][c] = 0; r.lp[1 - q][c] = 0; r.lh[1 - q][c] = 0; r.accA[c] = 0;
   }
   RD_FOR_ROW_ENTRIES(t, cb, n, c, i) r.lg[q][i] = 0;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) r.lA[q][v] = 0;
@@ -155,25 +159,18 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
   if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[0], part / r.nafm);
 }
 
-// clear the accumulators of the state before a sub-step; gather the ATP
-// adjoints of each AFM cell's voxels (its fa source term)
-__global__ void __launch_bounds__(256) k_rd_zero(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
-  const Tis& t = T[blockIdx.y];
-  const RdTis& r = R[blockIdx.y];
-  const int n = t.st->n;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
-    r.lu[1 - q][c] = 0; r.lp[1 - q][c] = 0; r.lh[1 - q][c] = 0; r.accA[c] = 0;
-  }
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) r.lA[1 - q][v] = 0;
-}
-
+// first kernel of a reverse sub-step: clear the ATP adjoint of the state
+// before the sub-step and gather the ATP adjoints of each cell's voxels
+// (the fa source term of its AFM owner)
 __global__ void __launch_bounds__(256) k_rd_acc(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
   const Tis& t = T[blockIdx.y];
   const RdTis& r = R[blockIdx.y];
   const long long end = box_end(t, r);
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+  double* nA = r.lA[1 - q];
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box;
        v += (long long)gridDim.x * blockDim.x) {
+    nA[v] = 0;
+    if (v >= end) continue;
     const int j = (int)(v % t.XP), k = (int)((v / t.XP) % (t.NY + 1));
     if (j >= t.NX || k >= t.NY) continue;
     const unsigned w = t.vox[v] & 0x7fffffffu;
@@ -343,6 +340,12 @@ __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTi
     if (l == NZ - 1) acc += lA[v] * cD * mv;
     nA[v] += acc;
   }
+  // the cell adjoints read by this sub-step's k_rd_cells are dead now: clear
+  // them for the next sub-step, which scatters into them
+  const int n = t.st->n;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    r.lu[q][c] = 0; r.lp[q][c] = 0; r.lh[q][c] = 0; r.accA[c] = 0;
+  }
 }
 
 // dL/dgj0: the adjoints of every coupled edge at the window's start
@@ -369,31 +372,30 @@ dim3 cells_grid(long long items, int ntis) {
 }  // namespace
 
 void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items) {
-  hipLaunchKernelGGL(k_rd_snapshot, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, cp, ap);
+  EPI_LAUNCH(K_RD_SNAPSHOT, k_rd_snapshot, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, cp, ap);
 }
 void launch_rd_restore(hipStream_t s, const Tis* dT, RdTis* dR, const double* gj0, int ntis, int cb, int cp, int ap,
                        int use_kpa, long long items) {
-  hipLaunchKernelGGL(k_rd_restore, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, gj0, cb, cp, ap, use_kpa);
+  EPI_LAUNCH(K_RD_RESTORE, k_rd_restore, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, gj0, cb, cp, ap, use_kpa);
 }
 void launch_rd_put_gj(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, long long items) {
-  hipLaunchKernelGGL(k_rd_put_gj, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb);
+  EPI_LAUNCH(K_RD_PUT_GJ, k_rd_put_gj, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb);
 }
 void launch_rd_state(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int dir, int cb, int cp, int ap,
                      long long items) {
-  hipLaunchKernelGGL(k_rd_state, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, slot, dir, cb, cp, ap);
+  EPI_LAUNCH(K_RD_STATE, k_rd_state, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, slot, dir, cb, cp, ap);
 }
 void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, int cb, int q, long long items) {
-  hipLaunchKernelGGL(k_rd_seed, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cp, cb, q);
+  EPI_LAUNCH(K_RD_SEED, k_rd_seed, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cp, cb, q);
 }
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
                                 int max_n, long long max_box) {
-  hipLaunchKernelGGL(k_rd_zero, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q);
-  hipLaunchKernelGGL(k_rd_acc, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
-  hipLaunchKernelGGL(k_rd_cells, cells_grid((long long)max_n * RG, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q);
-  hipLaunchKernelGGL(k_rd_grid, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
+  EPI_LAUNCH(K_RD_ACC, k_rd_acc, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
+  EPI_LAUNCH(K_RD_CELLS, k_rd_cells, cells_grid((long long)max_n * RG, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q);
+  EPI_LAUNCH(K_RD_GRID, k_rd_grid, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q);
 }
 void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items) {
-  hipLaunchKernelGGL(k_rd_gj0, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, q);
+  EPI_LAUNCH(K_RD_GJ0, k_rd_gj0, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, q);
 }
 
 }  // namespace epi

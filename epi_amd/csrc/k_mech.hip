@@ -856,17 +856,17 @@ __global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T
 
 void launch_mech_state(hipStream_t s, const Tis* dT, const MechTis* dM, int ntis, int slot, int dir, int pb,
                        int max_n) {
-  hipLaunchKernelGGL(k_mech_state, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, dM, slot, dir, pb);
+  EPI_LAUNCH(K_MECH_STATE, k_mech_state, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, dM, slot, dir, pb);
 }
 void launch_mech_seed(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int pb, int q, int max_n) {
-  hipLaunchKernelGGL(k_mech_seed, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, dM, pb, q);
+  EPI_LAUNCH(K_MECH_SEED, k_mech_seed, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, dM, pb, q);
 }
 void launch_mech_backward_step(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int slot, int pb, int cb, int q,
                                int max_n) {
   const dim3 g((max_n + 255) / 256, ntis);
-  hipLaunchKernelGGL(k_mech_bwd_begin, g, dim3(256), 0, s, dT, dM, slot, pb, q);
-  hipLaunchKernelGGL(k_mech_bwd_pairs, g, dim3(256), 0, s, dT, dM, pb, cb, q);
-  hipLaunchKernelGGL(k_mech_bwd_memb, g, dim3(256), 0, s, dT, dM, pb, q);
+  EPI_LAUNCH(K_MECH_BWD_BEGIN, k_mech_bwd_begin, g, dim3(256), 0, s, dT, dM, slot, pb, q);
+  EPI_LAUNCH(K_MECH_BWD_PAIRS, k_mech_bwd_pairs, g, dim3(256), 0, s, dT, dM, pb, cb, q);
+  EPI_LAUNCH(K_MECH_BWD_MEMB, k_mech_bwd_memb, g, dim3(256), 0, s, dT, dM, pb, q);
 }
 
 }  // namespace epi

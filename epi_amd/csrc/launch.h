@@ -11,7 +11,9 @@ enum KernelId {
   K_FORCES, K_FORCES_LONG, K_RENEW_FLAGS, K_RENEW_SCAN, K_RENEW_APPLY, K_REMOVE, K_PAIR_DISPERSE,
   K_INIT_SC, K_PERIODIC_FIX, K_BUCKET_INSERT, K_BUCKET_SORT, K_SWEEP,
   K_FINISH, K_SNAPSHOT, K_ZZ_SET, K_ZZMAX, K_MAP_CLEAR, K_MAP_STAMP, K_MAP_HALO,
-  K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET, K_COUNT
+  K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
+  K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_ACC, K_RD_CELLS, K_RD_GRID,
+  K_RD_GJ0, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB, K_COUNT
 };
 
 // Launch shapes that the autotuner may change (values checked by the host).
