@@ -5,6 +5,7 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
   k_ext_stim   z planes per thread (1, 2, 4, 8) x y rows per block (1, 2, 4)
   k_atp        the same
   k_ca_cells   lanes per cell (4, 8, 16)
+  k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
 
 A launch shape changes timing only: every grid point is computed on its own
 and the Ca cell kernel sums each list in list order whatever its group size,
@@ -20,7 +21,7 @@ import time
 
 STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
-DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16}
+DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0}
 
 
 def _avg_us(ts, kernel):
@@ -32,12 +33,14 @@ def _avg_us(ts, kernel):
 
 
 def _time(ts, kernel, run):
+    """Mean launch time of `kernel` (or the sum over a list of kernels)."""
+    kernels = [kernel] if isinstance(kernel, str) else list(kernel)
     ts.synchronize()
     ts.profile_reset()
-    ts.profile_enable([kernel])
+    ts.profile_enable(kernels)
     run()
     ts.synchronize()
-    us = _avg_us(ts, kernel)
+    us = sum(_avg_us(ts, k) for k in kernels)
     ts.profile_enable([])
     return us
 
@@ -51,7 +54,7 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
     if warm_steps:
         ts.step(drivers, warm_steps)
     zz = ts.calc_zzmax()
-    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}}
+    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}, "xcd": {}}
     best = dict(DEFAULT)
     ts.set_tuning(best)
 
@@ -82,6 +85,13 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
         report["k_ca_cells"][f"group{g}"] = _time(ts, "k_ca_cells", run_ca)
     key = min(report["k_ca_cells"], key=report["k_ca_cells"].get)
     best["ca_group"] = int(key[5:])
+    ts.set_tuning(best)
+
+    report["xcd"] = {}
+    for x in (0, 1):
+        ts.set_tuning({"xcd": x})
+        report["xcd"][f"xcd{x}"] = _time(ts, ("k_ca_cells", "k_atp"), run_ca)
+    best["xcd"] = int(min(report["xcd"], key=report["xcd"].get)[3:])
     ts.set_tuning(best)
     report["seconds"] = time.perf_counter() - t0
     if log is not None:

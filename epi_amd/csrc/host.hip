@@ -945,6 +945,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_ATP_ZB: return &tu.a_zb;
     case EPI_TUNE_ATP_ROWS: return &tu.a_rows;
     case EPI_TUNE_CA_GROUP: return &tu.ca_group;
+    case EPI_TUNE_XCD: return &tu.xcd;
     default: return nullptr;
   }
 }
@@ -959,6 +960,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_B_ZB: case EPI_TUNE_ATP_ZB: ok = value == 1 || value == 2 || value == 4 || value == 8; break;
     case EPI_TUNE_B_ROWS: case EPI_TUNE_ATP_ROWS: ok = value == 1 || value == 2 || value == 4; break;
     case EPI_TUNE_CA_GROUP: ok = value == 4 || value == 8 || value == 16; break;
+    case EPI_TUNE_XCD: ok = value == 0 || value == 1; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

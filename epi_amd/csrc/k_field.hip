@@ -252,20 +252,35 @@ __global__ void __launch_bounds__(256) k_ca_vox(const Tis* __restrict__ T, const
   }
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id.  With `on`, the
+// linear id is renumbered so that each XCD runs one contiguous run of logical
+// ids (a bijection for any grid size): neighbouring cells and grid rows then
+// share an L2.  Results do not depend on it.
+__device__ __forceinline__ unsigned xcd_order(unsigned L, unsigned G, int on) {
+  if (!on) return L;
+  const unsigned q = G >> 3, r = G & 7, x = L & 7;
+  return x * q + min(x, r) + (L >> 3);
+}
+
 // dead_IP3_calc (:156-173) and supra_calc (:182-222).  A group of CG lanes
 // (4, 8 or 16, set by the autotuner) owns one cell: lanes evaluate the
 // gap-junction terms of the list entries (and update gj toward ALIVE
 // neighbours) 4 * CG entries at a time and load the 16 grid corners of the two
-// grid_avg8; the leader sums them in list / corner order.
+// grid_avg8; the leader sums them in list / corner order.  Every lane issues
+// all loads of its entries of a chunk (list, kind, gj, then the neighbours'
+// Ca2+ and IP3) before it uses any of them.
 constexpr int C_BLOCK = 128;
 
-
-// the serial form of one cell, for lists longer than the LDS slots
 template <int CG>
-__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp, int ap, int bp) {
+__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
+                                                      int ap, int bp, int xcd) {
   constexpr int C_GROUPS = C_BLOCK / CG;
-  constexpr int CCAP = 4 * CG;   // list entries per chunk; LDS per block is the same for every CG
-  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  constexpr int EPL = 4;            // entries per lane per chunk
+  constexpr int CCAP = EPL * CG;    // list entries per chunk; LDS per block is the same for every CG
+  constexpr int NQ = 16 / CG;       // grid corners per lane
+  const unsigned lid = xcd_order(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, xcd);
+  const unsigned bx = lid % gridDim.x, by = lid / gridDim.x;
+  const Tis& t = T[tlist ? tlist[by] : by];
   const int n = t.st->n;
   __shared__ double s_t1[C_GROUPS][CCAP];
   __shared__ double s_t2[C_GROUPS][CCAP];
@@ -276,23 +291,34 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
   const double* __restrict__ ca = t.ca2p[cp];
   const double* __restrict__ ip3 = t.ip3[cp];
   const int stride = gridDim.x * C_GROUPS;
-  for (int c = t.nmemb + t.nder + blockIdx.x * C_GROUPS + grp; c < n; c += stride) {
+  for (int c = t.nmemb + t.nder + bx * C_GROUPS + grp; c < n; c += stride) {
     const uint8_t s = t.state[c];
+    const int len = t.nlen[cb][c];
+    const double u = ca[c], p = ip3[c];
     const bool dead = s == DEAD;
     if (!dead && !is_afm(s)) continue;
-    const int len = t.nlen[cb][c];
     const int* row = t.col[cb] + (size_t)c * t.ns;
     const uint8_t* kind = t.ca_kind + (size_t)c * t.ns;
-    const double u = ca[c], p = ip3[c];
     if (dead) {
       int count = 0;
       double tmp = 0;
       for (int e0 = 0; e0 < len; e0 += CCAP) {
         const int m = min(CCAP, len - e0);
-        for (int e = lane; e < m; e += CG) {
-          const bool al = kind[e0 + e] == 3;
-          s_f[grp][e] = al;
-          s_t1[grp][e] = al ? ip3[row[e0 + e]] : 0.0;
+        int o[EPL];
+        bool al[EPL];
+        double po[EPL];
+#pragma unroll
+        for (int i = 0; i < EPL; i++) {
+          const int e = lane + i * CG;
+          al[i] = e < m && kind[e0 + e] == 3;
+          o[i] = e < m ? row[e0 + e] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < EPL; i++) po[i] = al[i] ? ip3[o[i]] : 0.0;
+#pragma unroll
+        for (int i = 0; i < EPL; i++) {
+          const int e = lane + i * CG;
+          if (e < m) { s_f[grp][e] = al[i]; s_t1[grp][e] = po[i]; }
         }
         wave_sync();
         if (lane == 0)
@@ -303,38 +329,60 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
       if (lane == 0) t.ip3[1 - cp][c] = p + DT_Ca * (t.ca_dp * (tmp - count * p) - Kpp * p);
       continue;
     }
-    // the leader's per-cell values are fetched now, beside the lanes' loads
+    // the leader's per-cell values and the corners of grid_avg8 (utils.h:18-23;
+    // slots 0-7 B, 8-15 ATP) are fetched now, beside the list loads
     double exi = 0, avg = 0, _th = 0, _Kpa = 0, IAGv = 0;
     if (lane == 0) {
       exi = t.ex_inert[c]; avg = t.ca2p_avg[c];
       _th = t.ca_th[c]; _Kpa = t.ca_kpa[c]; IAGv = t.ca_iag[c];
     }
-    // corners of grid_avg8 (utils.h:18-23): slots 0-7 B, 8-15 ATP
-    for (int q = lane; q < 16; q += CG) {
-      const int k = q & 7;
-      const int ix = t.lat[3 * c] + ((k == 1 || k == 4 || k == 5 || k == 7) ? 1 : 0);
-      const int iy = t.lat[3 * c + 1] + ((k == 2 || k == 4 || k == 6 || k == 7) ? 1 : 0);
-      const int iz = t.lat[3 * c + 2] + ((k == 3 || k == 5 || k == 6 || k == 7) ? 1 : 0);
-      s_g[grp][q] = (q < 8 ? t.bf[bp] : t.atp[ap])[gidx(t, ix, iy, iz)];
+    double cq[NQ];
+    {
+      const int lx = t.lat[3 * c], ly = t.lat[3 * c + 1], lz = t.lat[3 * c + 2];
+#pragma unroll
+      for (int j = 0; j < NQ; j++) {
+        const int q = lane + j * CG;
+        const int k = q & 7;
+        const int ix = lx + ((k == 1 || k == 4 || k == 5 || k == 7) ? 1 : 0);
+        const int iy = ly + ((k == 2 || k == 4 || k == 6 || k == 7) ? 1 : 0);
+        const int iz = lz + ((k == 3 || k == 5 || k == 6 || k == 7) ? 1 : 0);
+        cq[j] = (q < 8 ? t.bf[bp] : t.atp[ap])[gidx(t, ix, iy, iz)];
+      }
     }
     double* g = t.gj[cb] + (size_t)c * t.ns;
     double tmp_diffu = 0, tmp_IP3 = 0;
     for (int e0 = 0; e0 < len; e0 += CCAP) {
       const int m = min(CCAP, len - e0);
-      for (int e = lane; e < m; e += CG) {
-        const int o = row[e0 + e];
-        const uint8_t kd = kind[e0 + e];
-        const double gv = g[e0 + e];
-        double a1 = 0, a2 = 0;
-        if (kd) {
-          const double uo = ca[o];
-          a1 = gv * (uo - u);
-          a2 = gv * (ip3[o] - p);
-          if (kd == 3) g[e0 + e] = gv + DT_Ca * fw_term(fabs(uo - u), gv);
+      int o[EPL];
+      uint8_t kd[EPL];
+      double gv[EPL], uo[EPL], po[EPL];
+#pragma unroll
+      for (int i = 0; i < EPL; i++) {
+        const int e = lane + i * CG;
+        const bool in = e < m;
+        o[i] = in ? row[e0 + e] : 0;
+        kd[i] = in ? kind[e0 + e] : 0;
+        gv[i] = in ? g[e0 + e] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < EPL; i++) {
+        uo[i] = kd[i] ? ca[o[i]] : 0.0;
+        po[i] = kd[i] ? ip3[o[i]] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < EPL; i++) {
+        const int e = lane + i * CG;
+        if (e < m) {
+          double a1 = 0, a2 = 0;
+          if (kd[i]) {
+            a1 = gv[i] * (uo[i] - u);
+            a2 = gv[i] * (po[i] - p);
+            if (kd[i] == 3) g[e0 + e] = gv[i] + DT_Ca * fw_term(fabs(uo[i] - u), gv[i]);
+          }
+          s_f[grp][e] = kd[i];
+          s_t1[grp][e] = a1;
+          s_t2[grp][e] = a2;
         }
-        s_f[grp][e] = kd;
-        s_t1[grp][e] = a1;
-        s_t2[grp][e] = a2;
       }
       wave_sync();
       if (lane == 0)
@@ -342,6 +390,9 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
           if (s_f[grp][e]) { tmp_diffu += s_t1[grp][e]; tmp_IP3 += s_t2[grp][e]; }
       wave_sync();
     }
+#pragma unroll
+    for (int j = 0; j < NQ; j++) s_g[grp][lane + j * CG] = cq[j];
+    wave_sync();
     if (lane == 0) {
       const double ca2p_du = t.ca_du;
       const double* q = s_g[grp];
@@ -364,13 +415,17 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
 // ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound.
 // Threads cover [0, NX) x [0, NY); the z column rolls through registers.
 template <int ZB>
-__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int nky, int nzc) {
-  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int nky, int nzc,
+                                             int xcd) {
+  const unsigned gxy = gridDim.x * gridDim.y;
+  const unsigned lid = xcd_order((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, gxy * gridDim.z, xcd);
+  const unsigned bx = lid % gridDim.x, by = (lid / gridDim.x) % gridDim.y, bz = lid / gxy;
+  const Tis& t = T[tlist ? tlist[by] : by];
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
   const int rows = blockDim.x >> 6;
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int k = (blockIdx.z % nky) * rows + (threadIdx.x >> 6);
-  const int zc = blockIdx.z / nky;
+  const int j = bx * 64 + (threadIdx.x & 63);
+  const int k = (bz % nky) * rows + (threadIdx.x >> 6);
+  const int zc = bz / nky;
   const int izb = ca_iz_bound(t);
   if (izb > NZ || izb < 0) return;
   if (j >= NX || k >= NY) return;
@@ -505,18 +560,18 @@ void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis,
   const int cg = tu.ca_group;
   const dim3 gc((max_n + C_BLOCK / cg - 1) / (C_BLOCK / cg), ntis);
   if (cg == 4)
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
   else if (cg == 8)
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
   else
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
   const int rows = tu.a_rows;
   const int nky = (max_ny + rows - 1) / rows;
   by_zb(tu.a_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
     const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
     dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
-    EPI_LAUNCH(K_ATP, k_atp<zb>, g, dim3(64 * rows), 0, s, dT, tlist, ap, nky, nzc);
+    EPI_LAUNCH(K_ATP, k_atp<zb>, g, dim3(64 * rows), 0, s, dT, tlist, ap, nky, nzc, tu.xcd);
   });
 }
 

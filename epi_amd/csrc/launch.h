@@ -21,6 +21,7 @@ struct Tuning {
   int b_zb = 4, b_rows = 4;   // k_ext_stim: z planes per chunk, y rows per block
   int a_zb = 4, a_rows = 4;   // k_atp: the same
   int ca_group = 16;          // k_ca_cells: lanes per cell
+  int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
 };
 
 // Event timer: when the bit of a kernel id is set, its launches carry a

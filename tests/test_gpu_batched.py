@@ -31,7 +31,7 @@ def _single(native, spec, draws, steps, ca_every, tuning=None):
     return t
 
 
-@pytest.mark.parametrize("tuning", [None, {"b_zb": 2, "b_rows": 1, "atp_zb": 8, "atp_rows": 2, "ca_group": 4}])
+@pytest.mark.parametrize("tuning", [None, {"b_zb": 2, "b_rows": 1, "atp_zb": 8, "atp_rows": 2, "ca_group": 4, "xcd": 1}])
 def test_batch_equals_single_runs(oracle_mod, tuning):
     from epi_amd import native
     specs = batch_specs()
@@ -108,3 +108,35 @@ def test_batch_against_oracle(oracle_mod):
         for k, v in f.items():
             assert (v == 0) if k.startswith("cmap") else (v <= FP_TOL), (t, k, v)
         assert ts.status(t)[0].rng_pos == o.rng_cursor()
+
+
+def test_launch_shapes_do_not_change_results():
+    """Every launch shape the autotuner may pick gives bit-identical tissues."""
+    from epi_amd import autotune, native
+    specs = batch_specs()[:2]
+    steps, ca_every = 30, 15
+    draws = [native.mt19937_64_uniform(300 + i, 300000) for i in range(len(specs))]
+
+    def run(tuning):
+        ts = native.TissueSet([s["geometry"] for s in specs])
+        for t, s in enumerate(specs):
+            ts.load(t, s)
+            ts.set_rng(t, draws[t])
+        ts.set_tuning(tuning)
+        ts.step([native.Driver(0, 0, 0, ca_every, 0) for _ in specs], steps)
+        out = [(ts.get_cells(t), ts.get_fields(t)) for t in range(len(specs))]
+        ts.close()
+        return out
+
+    ref = run(autotune.DEFAULT)
+    shapes = [dict(autotune.DEFAULT, ca_group=g, xcd=x, atp_zb=zb, atp_rows=rows, b_zb=zb, b_rows=rows)
+              for g, x, (zb, rows) in [(4, 1, (1, 4)), (8, 0, (2, 2)), (8, 1, (8, 1)), (16, 1, (4, 4))]]
+    for tu in shapes:
+        print(f"shape {tu}", flush=True)
+        got = run(tu)
+        for t in range(len(specs)):
+            (ca, fa), (cb, fb) = ref[t], got[t]
+            d = compare_cells(ca, cb)
+            assert "n" not in d and all(v == 0 for v in d.values()), (tu, t, d)
+            for k in fa:
+                assert np.array_equal(fa[k], fb[k]), (tu, t, k)
