@@ -71,7 +71,17 @@ def test_batch_equals_single_runs(oracle_mod, tuning):
                     nb = int(a["conn"][e])
                     info.append((int(c), int(e - ptr[c]), int(a["state"][c]), nb, int(a["state"][nb]),
                                  float(a["gj"][e]), float(b["gj"][e]), float(r["gj"][e])))
-            raise AssertionError((t, bad, "cell, slot, state, nb, nb state, batched, single, oracle", info))
+            # the whole row of the first differing cell in each run, and
+            # whether a second single run in a fresh context repeats it
+            rows = {}
+            if info:
+                c0 = info[0][0]
+                for name, cells in (("batched", a), ("single", b), ("oracle", r)):
+                    p0, p1 = cells["conn_ptr"][c0], cells["conn_ptr"][c0 + 1]
+                    rows[name] = (cells["conn"][p0:p1].tolist(), cells["gj"][p0:p1].tolist())
+            again = _single(native, s, draws[t], steps, ca_every).get_cells()
+            rows["single_again_equal_to_batched"] = {k: v for k, v in compare_cells(a, again).items() if v != 0}
+            raise AssertionError((t, bad, "cell, slot, state, nb, nb state, batched, single, oracle", info, rows))
         fa, fb = ts.get_fields(t), one.get_fields()
         for k in fa:
             assert np.array_equal(fa[k], fb[k]), (t, k)
