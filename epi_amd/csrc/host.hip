@@ -63,7 +63,7 @@ const char* kKernelNames[K_COUNT] = {
     "k_initialize_sc", "k_periodic_fix", "k_bucket_insert", "k_bucket_sort", "k_sweep",
     "k_finish", "k_snapshot", "k_zz_set", "k_zzmax", "k_map_clear", "k_map_stamp", "k_map_halo",
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
-    "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed", "k_rd_acc",
+    "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_cells", "k_rd_grid", "k_rd_gj0", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
     "k_mech_bwd_pairs", "k_mech_bwd_memb"};
 
@@ -1247,11 +1247,12 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
       for (int b = 0; b < 2; b++) {
         r.u0[b] = cv.take<double>(cap); r.p0[b] = cv.take<double>(cap); r.A0[b] = cv.take<double>(gs);
         r.lu[b] = cv.take<double>(cap); r.lp[b] = cv.take<double>(cap); r.lh[b] = cv.take<double>(cap);
-        r.lg[b] = cv.take<double>(cap * ns); r.lA[b] = cv.take<double>((size_t)r.box);
+        r.lg[b] = cv.take<double>(cap * ns); r.accA[b] = cv.take<double>(cap);
       }
       r.h0 = cv.take<double>(cap); r.avg0 = cv.take<double>(cap); r.kpa0 = cv.take<double>(cap);
       r.g0 = cv.take<double>(cap * ns);
-      r.accA = cv.take<double>(cap); r.gkpa = cv.take<double>(cap); r.target = cv.take<double>(cap);
+      for (int b = 0; b < 3; b++) r.lA[b] = cv.take<double>((size_t)r.box);
+      r.gkpa = cv.take<double>(cap); r.target = cv.take<double>(cap);
       r.kpa_in = cv.take<double>(cap); r.scal = cv.take<double>(4);
       r.nafm = 0;
     }
@@ -1344,7 +1345,7 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
     }
     RCHK(rd_substep(c));
   }
-  int q = 0;
+  int q = 0, qa = 0;
   launch_rd_seed(c->stream, c->d_tis, rd.d, c->T, c->cp, c->cb, q, items);
   const int mn = max_n(c);
   const long long mb = rd_max_box(c);
@@ -1360,8 +1361,9 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
       }
     }
     for (int s = s1 - 1; s >= s0; s--) {
-      launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, mn, mb);
+      launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, qa, mn, mb);
       q ^= 1;
+      qa = qa == 2 ? 0 : qa + 1;
     }
   }
   launch_rd_gj0(c->stream, c->d_tis, rd.d, c->T, c->cb, q, items);

@@ -131,10 +131,9 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
   }
 }
 
-// the loss and the seed of the reverse pass; both parities of the cell
-// adjoints and the per-cell ATP sums start at zero (the sub-steps keep them
-// so: k_rd_acc clears the ATP adjoint it scatters into, k_rd_grid the cell
-// adjoints the next sub-step scatters into)
+// the loss and the seed of the reverse pass.  Every adjoint buffer starts at
+// zero; the sub-steps keep them so (k_rd_grid clears the cell adjoints and
+// the ATP buffer the next sub-step scatters into).
 __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTis* __restrict__ R, int cp, int cb,
                                                   int q) {
   const Tis& t = T[blockIdx.y];
@@ -151,39 +150,26 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
       s = 2.0 * d / r.nafm;
     }
     r.lu[q][c] = s; r.lp[q][c] = 0; r.lh[q][c] = 0; r.gkpa[c] = 0;
-    r.lu[1 - q][c] = 0; r.lp[1 - q][c] = 0; r.lh[1 - q][c] = 0; r.accA[c] = 0;
+    r.lu[1 - q][c] = 0; r.lp[1 - q][c] = 0; r.lh[1 - q][c] = 0;
+    r.accA[0][c] = 0; r.accA[1][c] = 0;
   }
   RD_FOR_ROW_ENTRIES(t, cb, n, c, i) r.lg[q][i] = 0;
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) r.lA[q][v] = 0;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
+    r.lA[0][v] = 0; r.lA[1][v] = 0; r.lA[2][v] = 0;
+  }
   for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
   if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[0], part / r.nafm);
-}
-
-// first kernel of a reverse sub-step: clear the ATP adjoint of the state
-// before the sub-step and gather the ATP adjoints of each cell's voxels
-// (the fa source term of its AFM owner)
-__global__ void __launch_bounds__(256) k_rd_acc(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
-  const Tis& t = T[blockIdx.y];
-  const RdTis& r = R[blockIdx.y];
-  const long long end = box_end(t, r);
-  double* nA = r.lA[1 - q];
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box;
-       v += (long long)gridDim.x * blockDim.x) {
-    nA[v] = 0;
-    if (v >= end) continue;
-    const int j = (int)(v % t.XP), k = (int)((v / t.XP) % (t.NY + 1));
-    if (j >= t.NX || k >= t.NY) continue;
-    const unsigned w = t.vox[v] & 0x7fffffffu;
-    if (w) atomicAdd(&r.accA[w - 1], r.lA[q][v]);
-  }
 }
 
 // reverse of dead_IP3_calc and supra_calc for one sub-step, the state before
 // the sub-step from slot `slot`.  A group of RG lanes owns one cell: lanes
 // take the list entries, their partial sums are reduced over the group (a
 // gradient need not follow the forward's summation order), and each lane
-// scatters its entries' contributions.
-constexpr int RG = 8;
+// scatters its entries' contributions.  A lane keeps its first REP entries
+// (and their neighbour values) in registers for both passes, every load of
+// them issued up front; entries past RG * REP take the same steps from memory.
+constexpr int RG = 16;
+constexpr int REP = 2;
 
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -191,8 +177,37 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+namespace {
+// the contribution of one list entry of an AFM cell to the adjoints before
+// the sub-step (a lane's share of the second pass)
+struct RdEntry {
+  double abar_du_iag;   // abar * du * IAG
+  double lp_dp_iag;     // lp * DT_Ca * dp * IAG
+  double uc, pc;
+};
+
+__device__ __forceinline__ void rd_entry_bwd(const Tis& t, const RdEntry& k, int kd, int o, double gv, double uo,
+                                             double po, double lg_in, uint8_t so, double* ng_e, double* nu,
+                                             double* np, double& own_u) {
+  double to_u = k.abar_du_iag * gv;
+  double lge = lg_in;
+  if (kd == 3) {
+    const double d = uo - k.uc;
+    const double tz = tanh((0.1 - fabs(d)) / 0.1);
+    const double fwd = -5.0 * (1.0 - tz * tz);
+    const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
+    to_u += lg_in * DT_Ca * fwd * sg;
+    own_u -= lg_in * DT_Ca * fwd * sg;
+    lge = lg_in * (1.0 - DT_Ca);
+  }
+  *ng_e = lge + k.abar_du_iag * (uo - k.uc) + k.lp_dp_iag * (po - k.pc);
+  if (is_afm(so)) atomicAdd(&nu[o], to_u);
+  atomicAdd(&np[o], k.lp_dp_iag * gv);
+}
+}  // namespace
+
 __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
-                                                   int cb, int bp, int q) {
+                                                   int cb, int bp, int q, int qa) {
   const Tis& t = T[blockIdx.y];
   RdTis& r = R[blockIdx.y];
   const int n = t.st->n;
@@ -206,15 +221,18 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
   const double* g = r.sg + slot * cap * ns;
   const double* A = r.sA + slot * r.box;
   const double *lu = r.lu[q], *lp = r.lp[q], *lh = r.lh[q], *lg = r.lg[q];
-  double *nu = r.lu[1 - q], *np = r.lp[1 - q], *nh = r.lh[1 - q], *ng = r.lg[1 - q], *nA = r.lA[1 - q];
+  double *nu = r.lu[1 - q], *np = r.lp[1 - q], *nh = r.lh[1 - q], *ng = r.lg[1 - q];
+  double* nA = r.lA[qa == 2 ? 0 : qa + 1];
+  const double* accA = r.accA[q];
   const double du = t.ca_du, dp = t.ca_dp;
   double pdu = 0, pdp = 0;
   for (int c = start + blockIdx.x * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
     const uint8_t s = t.state[c];
-    const int* row = t.col[cb] + (size_t)c * ns;
-    const uint8_t* kind = t.ca_kind + (size_t)c * ns;
     const int len = t.nlen[cb][c];
     const double lpc = lp[c];
+    const double uc = u[c], pc = p[c];
+    const int* row = t.col[cb] + (size_t)c * ns;
+    const uint8_t* kind = t.ca_kind + (size_t)c * ns;
     if (s == DEAD) {
       double cnt = 0, sum = 0;
       for (int e = lane; e < len; e += RG)
@@ -228,20 +246,50 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
       sum = group_sum(sum);
       if (lane == 0) {
         atomicAdd(&np[c], lpc * (1.0 - DT_Ca * dp * cnt - DT_Ca * Kpp));
-        pdp += lpc * DT_Ca * (sum - cnt * p[c]);
+        pdp += lpc * DT_Ca * (sum - cnt * pc);
       }
       continue;
     }
     if (!is_afm(s)) continue;
-    const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
     const double* gr = g + (size_t)c * ns;
-    const double uc = u[c], pc = p[c], hc = h[c];
+    const double* lgr = lg + (size_t)c * ns;
+    // the lane's first REP entries: list, kind, gj, edge adjoint, then the
+    // neighbours' state, Ca2+ and IP3, all in flight together
+    const int m0 = min(len, RG * REP);
+    int o[REP];
+    uint8_t kd[REP], so[REP];
+    double gv[REP], lgv[REP], uo[REP], po[REP];
+#pragma unroll
+    for (int i = 0; i < REP; i++) {
+      const int e = lane + i * RG;
+      const bool in = e < m0;
+      o[i] = in ? row[e] : 0;
+      kd[i] = in ? kind[e] : 0;
+      gv[i] = in ? gr[e] : 0.0;
+      lgv[i] = in ? lgr[e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < REP; i++) {
+      uo[i] = kd[i] ? u[o[i]] : 0.0;
+      po[i] = kd[i] ? p[o[i]] : 0.0;
+      so[i] = kd[i] ? t.state[o[i]] : (uint8_t)0;
+    }
+    const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
+    const double hc = h[c], luc = lu[c], lhc = lh[c], accc = accA[c];
+    const double iag = t.ca_iag[c], kpa = t.ca_kpa[c], th = t.ca_th[c];
     double su = 0, sp = 0, gsum = 0;
-    for (int e = lane; e < len; e += RG) {
+#pragma unroll
+    for (int i = 0; i < REP; i++)
+      if (kd[i]) {
+        su += gv[i] * (uo[i] - uc);
+        sp += gv[i] * (po[i] - pc);
+        gsum += gv[i];
+      }
+    for (int e = RG * REP + lane; e < len; e += RG) {
       if (!kind[e]) continue;
-      const int o = row[e];
-      su += gr[e] * (u[o] - uc);
-      sp += gr[e] * (p[o] - pc);
+      const int oe = row[e];
+      su += gr[e] * (u[oe] - uc);
+      sp += gr[e] * (p[oe] - pc);
       gsum += gr[e];
     }
     su = group_sum(su);
@@ -251,28 +299,19 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
     const long long b0 = gidx(t, ix, iy, iz);
     const double Aavg = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
                                  + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
-    const double iag = t.ca_iag[c], kpa = t.ca_kpa[c], th = t.ca_th[c];
     const double diffu = ca2p_reaction_factor(uc, hc, pc, Bavg) + du * iag * su;
-    const double abar = DT_Ca * lu[c] + (diffu > 0 ? r.accA[c] * DT_Ca * STIM11 : 0.0);
+    const double abar = DT_Ca * luc + (diffu > 0 ? accc * DT_Ca * STIM11 : 0.0);
+    const RdEntry k{abar * du * iag, lpc * DT_Ca * dp * iag, uc, pc};
     double own_u = 0;
-    for (int e = lane; e < len; e += RG) {
-      if (!kind[e]) continue;
-      const int o = row[e];
-      double to_u = abar * du * iag * gr[e];
-      const double lg_in = lg[(size_t)c * ns + e];
-      double lge = lg_in;
-      if (kind[e] == 3) {
-        const double d = u[o] - uc;
-        const double tz = tanh((0.1 - fabs(d)) / 0.1);
-        const double fwd = -5.0 * (1.0 - tz * tz);
-        const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
-        to_u += lg_in * DT_Ca * fwd * sg;
-        own_u -= lg_in * DT_Ca * fwd * sg;
-        lge = lg_in * (1.0 - DT_Ca);
-      }
-      ng[(size_t)c * ns + e] = lge + abar * du * iag * (u[o] - uc) + lpc * DT_Ca * dp * iag * (p[o] - pc);
-      if (is_afm(t.state[o])) atomicAdd(&nu[o], to_u);
-      atomicAdd(&np[o], lpc * DT_Ca * dp * iag * gr[e]);
+#pragma unroll
+    for (int i = 0; i < REP; i++)
+      if (kd[i]) rd_entry_bwd(t, k, kd[i], o[i], gv[i], uo[i], po[i], lgv[i], so[i], &ng[(size_t)c * ns + lane + i * RG],
+                              nu, np, own_u);
+    for (int e = RG * REP + lane; e < len; e += RG) {
+      const int kde = kind[e];
+      if (!kde) continue;
+      const int oe = row[e];
+      rd_entry_bwd(t, k, kde, oe, gr[e], u[oe], p[oe], lgr[e], t.state[oe], &ng[(size_t)c * ns + e], nu, np, own_u);
     }
     own_u = group_sum(own_u);
     if (lane < 8) {   // the eight grid_avg8 corners of the IP3 production term
@@ -288,10 +327,10 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
       const double dRdh = kflux * e1 * e2;
       const double dRdp = kflux * (mu1 * kmu / ((pc + kmu) * (pc + kmu))) * e2 * hc;
       const double qq = k2 * k2 + uc * uc;
-      atomicAdd(&nu[c], own_u + lu[c] + abar * (dRdu - du * iag * gsum)
-                            + lh[c] * DT_Ca * (-2.0 * uc * k2 * k2 / (qq * qq)) / th);
+      atomicAdd(&nu[c], own_u + luc + abar * (dRdu - du * iag * gsum)
+                            + lhc * DT_Ca * (-2.0 * uc * k2 * k2 / (qq * qq)) / th);
       atomicAdd(&np[c], lpc * (1.0 - DT_Ca * Kpp - DT_Ca * dp * iag * gsum) + abar * dRdp);
-      atomicAdd(&nh[c], lh[c] * (1.0 - DT_Ca / th) + abar * dRdh);
+      atomicAdd(&nh[c], lhc * (1.0 - DT_Ca / th) + abar * dRdh);
       pdu += abar * iag * su;
       pdp += lpc * DT_Ca * iag * sp;
       r.gkpa[c] += lpc * DT_Ca * Aavg / (H0 + Aavg);
@@ -309,18 +348,26 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
 
 // reverse of ATP_refresh: each point of the box gathers from the points
 // whose update read it (periodic x, y; mirror z: prev_z(0) = 1,
-// next_z(NZ - 1) = NZ - 1)
-__global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTis* __restrict__ R, int q) {
+// next_z(NZ - 1) = NZ - 1).  The sum completes the ATP adjoint of the state
+// before the sub-step (k_rd_cells has scattered the grid_avg8 terms into it),
+// so each point also adds it to its AFM owner's sum for the next sub-step's
+// fa term.  The dead ATP buffer and the cell adjoints this sub-step has
+// consumed are cleared for the next one.
+__global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTis* __restrict__ R, int q, int qa) {
   const Tis& t = T[blockIdx.y];
   const RdTis& r = R[blockIdx.y];
   const long long end = box_end(t, r);
-  const double* lA = r.lA[q];
-  double* nA = r.lA[1 - q];
+  const double* lA = r.lA[qa];
+  double* nA = r.lA[qa == 2 ? 0 : qa + 1];
+  double* dead = r.lA[qa == 0 ? 2 : qa - 1];
+  double* acc_next = r.accA[1 - q];
   const uint8_t* m = t.cmap2;
   const int NX = t.NX, NY = t.NY, NZ = t.NZ, izb = r.izb;
   const double cD = DT_Ca * Da * t.inv_dx * t.inv_dx;
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < end;
-       v += (long long)gridDim.x * blockDim.x) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
+    dead[v] = 0;
+    if (v >= end) continue;
     const int j = (int)(v % t.XP), k = (int)((v / t.XP) % (NY + 1)), l = (int)(v / t.PL);
     if (j >= NX || k >= NY) continue;
     const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
@@ -338,13 +385,14 @@ __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTi
     if (l + 1 < izb) acc += lA[gidx(t, j, k, l + 1)] * cD * mv;
     if (l == 1) acc += lA[gidx(t, j, k, 0)] * cD * mv;
     if (l == NZ - 1) acc += lA[v] * cD * mv;
-    nA[v] += acc;
+    const double fin = nA[v] + acc;
+    nA[v] = fin;
+    const unsigned w = t.vox[v] & 0x7fffffffu;
+    if (w && fin != 0) atomicAdd(&acc_next[w - 1], fin);
   }
-  // the cell adjoints read by this sub-step's k_rd_cells are dead now: clear
-  // them for the next sub-step, which scatters into them
   const int n = t.st->n;
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-    r.lu[q][c] = 0; r.lp[q][c] = 0; r.lh[q][c] = 0; r.accA[c] = 0;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+    r.lu[q][c] = 0; r.lp[q][c] = 0; r.lh[q][c] = 0; r.accA[q][c] = 0;
   }
 }
 
@@ -389,10 +437,11 @@ void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, i
   EPI_LAUNCH(K_RD_SEED, k_rd_seed, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cp, cb, q);
 }
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
-                                int max_n, long long max_box) {
-  EPI_LAUNCH(K_RD_ACC, k_rd_acc, cells_grid(max_box, ntis), dim3(256), 0, s, dT, dR, q);
-  EPI_LAUNCH(K_RD_CELLS, k_rd_cells, cells_grid((long long)max_n * RG, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q);
-  EPI_LAUNCH(K_RD_GRID, k_rd_grid, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q);
+                                int qa, int max_n, long long max_box) {
+  EPI_LAUNCH(K_RD_CELLS, k_rd_cells, cells_grid((long long)max_n * RG, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q,
+             qa);
+  EPI_LAUNCH(K_RD_GRID, k_rd_grid, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q,
+             qa);
 }
 void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items) {
   EPI_LAUNCH(K_RD_GJ0, k_rd_gj0, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, q);

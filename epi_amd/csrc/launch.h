@@ -12,7 +12,7 @@ enum KernelId {
   K_INIT_SC, K_PERIODIC_FIX, K_BUCKET_INSERT, K_BUCKET_SORT, K_SWEEP,
   K_FINISH, K_SNAPSHOT, K_ZZ_SET, K_ZZMAX, K_MAP_CLEAR, K_MAP_STAMP, K_MAP_HALO,
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
-  K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_ACC, K_RD_CELLS, K_RD_GRID,
+  K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_CELLS, K_RD_GRID,
   K_RD_GJ0, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB, K_COUNT
 };
 
@@ -70,7 +70,7 @@ void launch_rd_state(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot
                      long long items);
 void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, int cb, int q, long long items);
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
-                                int max_n, long long max_box);
+                                int qa, int max_n, long long max_box);
 // frozen mechanics window (k_mech.hip)
 struct MechTis;
 void launch_mech_state(hipStream_t s, const Tis* dT, const MechTis* dM, int ntis, int slot, int dir, int pb,

@@ -22,8 +22,11 @@ struct RdTis {
   double *su, *sp, *sh, *sg, *sA;
   double *cu, *cp_, *ch, *cg, *cA;   // checkpoint-major, two buffers each for u, p, A
   // adjoint: [q] = the state after the sub-step, [1-q] = before it
-  double *lu[2], *lp[2], *lh[2], *lg[2], *lA[2];
-  double *accA;                // per-cell sum of the ATP adjoints of its voxels
+  double *lu[2], *lp[2], *lh[2], *lg[2];
+  // ATP adjoint in three rotating buffers: [a] after the sub-step, [a+1]
+  // before it, [a+2] dead (cleared for the next sub-step), a = step mod 3
+  double* lA[3];
+  double* accA[2];             // per-cell sums of the ATP adjoints of its voxels, by parity q
   double *gkpa;                // dL/dkpa per cell
   double* scal;                // [0] loss, [1] dL/ddu, [2] dL/ddp, [3] dL/dgj0
   double* target;              // per-cell target ca2p
