@@ -92,6 +92,14 @@ def kernel_mix(ts, native, steps):
     return mix, dominant
 
 
+def timed_kernels(rk):
+    """Kernels that carry events in the timed region.  A dispatch-recorded
+    event pair brackets its kernel only when the kernel before it on the
+    stream carries events too, so the two kernels of a Ca sub-step are timed
+    together."""
+    return ["k_ca_cells", "k_atp"] if rk in ("k_ca_cells", "k_atp") else [rk]
+
+
 def roofline(ts, prof, s0, s1, rk):
     """Algorithmic bytes per launch of kernel rk over its mean launch
     duration (dispatch-recorded HIP events on the library's stream during the
@@ -225,7 +233,7 @@ def run_batched(args):
     ts.synchronize()
     s0 = [ts.status(t)[0] for t in range(len(specs))]
     ts.profile_reset()
-    ts.profile_enable([rk])
+    ts.profile_enable(timed_kernels(rk))
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -418,9 +426,16 @@ def run_adjoint_rd(args):
     ts.rd_forward([base], n_sub)            # warm
     reps = 3
     t_fwd = min(_timed(ts, lambda: ts.rd_forward([base], n_sub))[0] for _ in range(reps))
-    ts.rd_adjoint([base], n_sub, every, [target])   # warm
-    t_adj, (loss, grads) = min((_timed(ts, lambda: ts.rd_adjoint([base], n_sub, every, [target]))
-                                for _ in range(reps)), key=lambda r: r[0])
+    # lanes per cell of the reverse cell kernel: time both shapes, keep the faster
+    shapes = {}
+    for rg in (8, 16):
+        ts.set_tuning({"rd_group": rg})
+        ts.rd_adjoint([base], n_sub, every, [target])   # warm
+        shapes[rg] = min((_timed(ts, lambda: ts.rd_adjoint([base], n_sub, every, [target]))
+                          for _ in range(reps)), key=lambda r: r[0])
+    rg_best = min(shapes, key=lambda k: shapes[k][0])
+    ts.set_tuning({"rd_group": rg_best})
+    t_adj, (loss, grads) = shapes[rg_best]
     mix = {"forward": window_mix(ts, lambda: ts.rd_forward([base], n_sub)),
            "adjoint": window_mix(ts, lambda: ts.rd_adjoint([base], n_sub, every, [target]))}
     ts.rd_end()
@@ -430,6 +445,7 @@ def run_adjoint_rd(args):
            "config": {"workload": "64x64 sheet + 16 FIX + 5,000 free cells frozen, grid 64x64x128, 200 Ca sub-steps, "
                                   "checkpoint every 20, seed 4", "cells": int(st.size), "iz_bound": int((zz[0] + 2.8) / (g.lz / g.nz))},
            "forward_ms": round(1e3 * t_fwd, 3), "adjoint_ms": round(1e3 * t_adj, 3),
+           "adjoint_ms_by_rd_group": {str(k): round(1e3 * v[0], 3) for k, v in shapes.items()},
            "forward_substeps_per_s": round(n_sub / t_fwd, 1), "loss": float(loss[0]),
            "grad": {"du": grads[0]["du"], "dp": grads[0]["dp"], "gj0": grads[0]["gj0"],
                     "kpa_l2": float(np.linalg.norm(gk))},
@@ -530,7 +546,7 @@ def main():
     ts.synchronize()
     s0, _ = ts.status(0)
     ts.profile_reset()
-    ts.profile_enable(None if args.profile_all else [rk])
+    ts.profile_enable(None if args.profile_all else timed_kernels(rk))
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()

@@ -100,7 +100,13 @@ struct Tis {
   double* ca2p[2]; double* ip3[2];
   double *ca2p_avg, *ex_inert, *agek, *ageb, *ex_fat, *in_fat, *spr_nat_len, *diff_u;
   double *ca_th, *ca_kpa, *ca_iag;   // per-solve constants of the Ca step (agek and state are frozen)
-  uint8_t* ca_kind;                  // per list entry during a solve: 0 skip, 1 coupled, 3 coupled ALIVE
+  // per list entry during a solve: neighbour << 2 | kind (0 skip, 1 coupled,
+  // 3 coupled ALIVE), one word for the index and the kind
+  int* ca_ent;
+  // (ca2p, IP3) of buffer b beside the two arrays during a solve, so that a
+  // list entry reads its neighbour's pair with one 16-byte gather; every
+  // kernel that writes ca2p or IP3 inside a solve or window writes it too
+  double2* ca_up[2];
   long long* uid; long long* old_uid; int* src_row;
   // neighbour rows (two buffers: current and previous build)
   int* nlen[2]; int* col[2]; double* gj[2];

@@ -192,7 +192,8 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.ca_th = cv.take<double>(cap);
   t.ca_kpa = cv.take<double>(cap);
   t.ca_iag = cv.take<double>(cap);
-  t.ca_kind = cv.take<uint8_t>(cap * ns);
+  t.ca_ent = cv.take<int>(cap * ns);
+  for (int b = 0; b < 2; b++) t.ca_up[b] = cv.take<double2>(cap);
   t.uid = cv.take<long long>(cap);
   t.old_uid = cv.take<long long>(cap);
   t.src_row = cv.take<int>(cap);
@@ -946,6 +947,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_ATP_ROWS: return &tu.a_rows;
     case EPI_TUNE_CA_GROUP: return &tu.ca_group;
     case EPI_TUNE_XCD: return &tu.xcd;
+    case EPI_TUNE_RD_GROUP: return &tu.rd_group;
     default: return nullptr;
   }
 }
@@ -961,6 +963,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_B_ROWS: case EPI_TUNE_ATP_ROWS: ok = value == 1 || value == 2 || value == 4; break;
     case EPI_TUNE_CA_GROUP: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_XCD: ok = value == 0 || value == 1; break;
+    case EPI_TUNE_RD_GROUP: ok = value == 8 || value == 16; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;
@@ -1217,6 +1220,10 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
   for (int t = 0; t < c->T; t++) {
     rd.izb[t] = (int)((zzmax[t] + FAC_MAP * R_max) / c->tis[t].dz);   // ca2p.cpp:386
     if (rd.izb[t] < 0 || rd.izb[t] > c->tis[t].NZ) return fail(c, EPI_ERR_OUT_OF_GRID, "iz_bound outside the grid");
+    // the reverse pass gathers a cell's voxels from its stamp box, which must
+    // not wrap onto itself in x or y
+    if (c->tis[t].NX < 4 * c->tis[t].irx + 1 || c->tis[t].NY < 4 * c->tis[t].iry + 1)
+      return fail(c, EPI_ERR_INVALID, "grid too small for a window (stamp box wraps onto itself)");
   }
   // every AFM cell's grid_avg8 corners must lie in planes 0..iz_bound (the
   // planes a saved state holds)
@@ -1247,7 +1254,7 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
       for (int b = 0; b < 2; b++) {
         r.u0[b] = cv.take<double>(cap); r.p0[b] = cv.take<double>(cap); r.A0[b] = cv.take<double>(gs);
         r.lu[b] = cv.take<double>(cap); r.lp[b] = cv.take<double>(cap); r.lh[b] = cv.take<double>(cap);
-        r.lg[b] = cv.take<double>(cap * ns); r.accA[b] = cv.take<double>(cap);
+        r.lg[b] = cv.take<double>(cap * ns);
       }
       r.h0 = cv.take<double>(cap); r.avg0 = cv.take<double>(cap); r.kpa0 = cv.take<double>(cap);
       r.g0 = cv.take<double>(cap * ns);
@@ -1361,7 +1368,7 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
       }
     }
     for (int s = s1 - 1; s >= s0; s--) {
-      launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, qa, mn, mb);
+      launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, qa, mn, mb, c->tu.rd_group);
       q ^= 1;
       qa = qa == 2 ? 0 : qa + 1;
     }

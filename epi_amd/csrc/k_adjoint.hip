@@ -54,6 +54,8 @@ __global__ void __launch_bounds__(256) k_rd_restore(const Tis* __restrict__ T, R
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
     t.ca2p[cp][c] = r.u0[0][c]; t.ca2p[1 - cp][c] = r.u0[1][c];
     t.ip3[cp][c] = r.p0[0][c]; t.ip3[1 - cp][c] = r.p0[1][c];
+    t.ca_up[cp][c] = make_double2(r.u0[0][c], r.p0[0][c]);
+    t.ca_up[1 - cp][c] = make_double2(r.u0[1][c], r.p0[1][c]);
     t.ex_inert[c] = r.h0[c]; t.ca2p_avg[c] = r.avg0[c]; t.diff_u[c] = 0;
     t.ca_kpa[c] = use_kpa ? r.kpa_in[c] : r.kpa0[c];
   }
@@ -118,7 +120,7 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
       double* us = (ck ? r.cu + (2 * slot + b) * cap : r.su + slot * cap);
       double* ps = (ck ? r.cp_ + (2 * slot + b) * cap : r.sp + slot * cap);
       if (save) { us[c] = t.ca2p[buf][c]; ps[c] = t.ip3[buf][c]; }
-      else { t.ca2p[buf][c] = us[c]; t.ip3[buf][c] = ps[c]; }
+      else { t.ca2p[buf][c] = us[c]; t.ip3[buf][c] = ps[c]; t.ca_up[buf][c] = make_double2(us[c], ps[c]); }
     }
   }
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
@@ -151,7 +153,6 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
     }
     r.lu[q][c] = s; r.lp[q][c] = 0; r.lh[q][c] = 0; r.gkpa[c] = 0;
     r.lu[1 - q][c] = 0; r.lp[1 - q][c] = 0; r.lh[1 - q][c] = 0;
-    r.accA[0][c] = 0; r.accA[1][c] = 0;
   }
   RD_FOR_ROW_ENTRIES(t, cb, n, c, i) r.lg[q][i] = 0;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
@@ -162,15 +163,16 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
 }
 
 // reverse of dead_IP3_calc and supra_calc for one sub-step, the state before
-// the sub-step from slot `slot`.  A group of RG lanes owns one cell: lanes
-// take the list entries, their partial sums are reduced over the group (a
-// gradient need not follow the forward's summation order), and each lane
-// scatters its entries' contributions.  A lane keeps its first REP entries
-// (and their neighbour values) in registers for both passes, every load of
-// them issued up front; entries past RG * REP take the same steps from memory.
-constexpr int RG = 16;
-constexpr int REP = 2;
-
+// the sub-step from slot `slot`.  A group of RG lanes (8 or 16, a launch
+// shape) owns one cell: lanes take the list entries, their partial sums are
+// reduced over the group (a gradient need not follow the forward's summation
+// order), and each lane scatters its entries' contributions.  A lane keeps
+// its first 32 / RG entries (and their neighbour values) in registers for
+// both passes, every load of them issued up front; entries past 32 take the
+// same steps from memory.  The ATP adjoint of the cell's fa source is
+// gathered from the voxels the cell owns, which lie in the box its sphere
+// stamp covers (lat +- 2 ir, z from 1, periodic x and y).
+template <int RG>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
   for (int o = 1; o < RG; o <<= 1) v += __shfl_xor(v, o, RG);
@@ -186,9 +188,9 @@ struct RdEntry {
   double uc, pc;
 };
 
-__device__ __forceinline__ void rd_entry_bwd(const Tis& t, const RdEntry& k, int kd, int o, double gv, double uo,
-                                             double po, double lg_in, uint8_t so, double* ng_e, double* nu,
-                                             double* np, double& own_u) {
+__device__ __forceinline__ void rd_entry_bwd(const RdEntry& k, int kd, int o, double gv, double uo, double po,
+                                             double lg_in, uint8_t so, double* ng_e, double* nu, double* np,
+                                             double& own_u) {
   double to_u = k.abar_du_iag * gv;
   double lge = lg_in;
   if (kd == 3) {
@@ -206,8 +208,10 @@ __device__ __forceinline__ void rd_entry_bwd(const Tis& t, const RdEntry& k, int
 }
 }  // namespace
 
+template <int RG>
 __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
                                                    int cb, int bp, int q, int qa) {
+  constexpr int REP = 32 / RG;
   const Tis& t = T[blockIdx.y];
   RdTis& r = R[blockIdx.y];
   const int n = t.st->n;
@@ -222,28 +226,30 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
   const double* A = r.sA + slot * r.box;
   const double *lu = r.lu[q], *lp = r.lp[q], *lh = r.lh[q], *lg = r.lg[q];
   double *nu = r.lu[1 - q], *np = r.lp[1 - q], *nh = r.lh[1 - q], *ng = r.lg[1 - q];
+  const double* lA = r.lA[qa];
   double* nA = r.lA[qa == 2 ? 0 : qa + 1];
-  const double* accA = r.accA[q];
   const double du = t.ca_du, dp = t.ca_dp;
+  // the stamp box of an AFM cell (k_map_stamp)
+  const int bx = 4 * t.irx + 1, by = 4 * t.iry + 1, bz = 4 * t.irz + 1;
+  const int bxy = bx * by, bvol = bxy * bz;
   double pdu = 0, pdp = 0;
   for (int c = start + blockIdx.x * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
     const uint8_t s = t.state[c];
     const int len = t.nlen[cb][c];
     const double lpc = lp[c];
     const double uc = u[c], pc = p[c];
-    const int* row = t.col[cb] + (size_t)c * ns;
-    const uint8_t* kind = t.ca_kind + (size_t)c * ns;
+    const int* ent = t.ca_ent + (size_t)c * ns;
     if (s == DEAD) {
       double cnt = 0, sum = 0;
       for (int e = lane; e < len; e += RG)
-        if (kind[e] == 3) {
-          const int o = row[e];
+        if ((ent[e] & 3) == 3) {
+          const int o = ent[e] >> 2;
           sum += p[o];
           cnt += 1;
           atomicAdd(&np[o], lpc * DT_Ca * dp);
         }
-      cnt = group_sum(cnt);
-      sum = group_sum(sum);
+      cnt = group_sum<RG>(cnt);
+      sum = group_sum<RG>(sum);
       if (lane == 0) {
         atomicAdd(&np[c], lpc * (1.0 - DT_Ca * dp * cnt - DT_Ca * Kpp));
         pdp += lpc * DT_Ca * (sum - cnt * pc);
@@ -253,8 +259,8 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
     if (!is_afm(s)) continue;
     const double* gr = g + (size_t)c * ns;
     const double* lgr = lg + (size_t)c * ns;
-    // the lane's first REP entries: list, kind, gj, edge adjoint, then the
-    // neighbours' state, Ca2+ and IP3, all in flight together
+    // the lane's first REP entries: packed neighbour and kind, gj, edge
+    // adjoint, then the neighbours' state, Ca2+ and IP3, all in flight together
     const int m0 = min(len, RG * REP);
     int o[REP];
     uint8_t kd[REP], so[REP];
@@ -263,8 +269,9 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
     for (int i = 0; i < REP; i++) {
       const int e = lane + i * RG;
       const bool in = e < m0;
-      o[i] = in ? row[e] : 0;
-      kd[i] = in ? kind[e] : 0;
+      const int en = in ? ent[e] : 0;
+      o[i] = en >> 2;
+      kd[i] = (uint8_t)(en & 3);
       gv[i] = in ? gr[e] : 0.0;
       lgv[i] = in ? lgr[e] : 0.0;
     }
@@ -275,8 +282,22 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
       so[i] = kd[i] ? t.state[o[i]] : (uint8_t)0;
     }
     const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
-    const double hc = h[c], luc = lu[c], lhc = lh[c], accc = accA[c];
+    const double hc = h[c], luc = lu[c], lhc = lh[c];
     const double iag = t.ca_iag[c], kpa = t.ca_kpa[c], th = t.ca_th[c];
+    // ATP adjoint of the voxels the cell owns (after the sub-step)
+    double acc_own = 0;
+    {
+      const unsigned me = (unsigned)c + 1;
+      const int z0 = max(iz - 2 * t.irz, 1);
+      for (int b = lane; b < bvol; b += RG) {
+        const int l = z0 + b / bxy;
+        if (l >= r.izb || l >= t.NZ || l > iz + 2 * t.irz) continue;
+        const int jx = (ix - 2 * t.irx + b % bx + t.NX) % t.NX;
+        const int ky = (iy - 2 * t.iry + (b / bx) % by + t.NY) % t.NY;
+        const long long v = gidx(t, jx, ky, l);
+        if ((t.vox[v] & 0x7fffffffu) == me) acc_own += lA[v];
+      }
+    }
     double su = 0, sp = 0, gsum = 0;
 #pragma unroll
     for (int i = 0; i < REP; i++)
@@ -286,34 +307,35 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
         gsum += gv[i];
       }
     for (int e = RG * REP + lane; e < len; e += RG) {
-      if (!kind[e]) continue;
-      const int oe = row[e];
+      if (!(ent[e] & 3)) continue;
+      const int oe = ent[e] >> 2;
       su += gr[e] * (u[oe] - uc);
       sp += gr[e] * (p[oe] - pc);
       gsum += gr[e];
     }
-    su = group_sum(su);
-    sp = group_sum(sp);
-    gsum = group_sum(gsum);
+    su = group_sum<RG>(su);
+    sp = group_sum<RG>(sp);
+    gsum = group_sum<RG>(gsum);
+    acc_own = group_sum<RG>(acc_own);
     const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
     const long long b0 = gidx(t, ix, iy, iz);
     const double Aavg = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
                                  + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
     const double diffu = ca2p_reaction_factor(uc, hc, pc, Bavg) + du * iag * su;
-    const double abar = DT_Ca * luc + (diffu > 0 ? accc * DT_Ca * STIM11 : 0.0);
+    const double abar = DT_Ca * luc + (diffu > 0 ? acc_own * DT_Ca * STIM11 : 0.0);
     const RdEntry k{abar * du * iag, lpc * DT_Ca * dp * iag, uc, pc};
     double own_u = 0;
 #pragma unroll
     for (int i = 0; i < REP; i++)
-      if (kd[i]) rd_entry_bwd(t, k, kd[i], o[i], gv[i], uo[i], po[i], lgv[i], so[i], &ng[(size_t)c * ns + lane + i * RG],
+      if (kd[i]) rd_entry_bwd(k, kd[i], o[i], gv[i], uo[i], po[i], lgv[i], so[i], &ng[(size_t)c * ns + lane + i * RG],
                               nu, np, own_u);
     for (int e = RG * REP + lane; e < len; e += RG) {
-      const int kde = kind[e];
+      const int kde = ent[e] & 3;
       if (!kde) continue;
-      const int oe = row[e];
-      rd_entry_bwd(t, k, kde, oe, gr[e], u[oe], p[oe], lgr[e], t.state[oe], &ng[(size_t)c * ns + e], nu, np, own_u);
+      const int oe = ent[e] >> 2;
+      rd_entry_bwd(k, kde, oe, gr[e], u[oe], p[oe], lgr[e], t.state[oe], &ng[(size_t)c * ns + e], nu, np, own_u);
     }
-    own_u = group_sum(own_u);
+    own_u = group_sum<RG>(own_u);
     if (lane < 8) {   // the eight grid_avg8 corners of the IP3 production term
       const double wA = lpc * DT_Ca * kpa * H0 / ((H0 + Aavg) * (H0 + Aavg)) * 0.125;
       const long long vv = corner_var(t, r, ix + (lane & 1), iy + ((lane >> 1) & 1), iz + ((lane >> 2) & 1));
@@ -348,11 +370,9 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
 
 // reverse of ATP_refresh: each point of the box gathers from the points
 // whose update read it (periodic x, y; mirror z: prev_z(0) = 1,
-// next_z(NZ - 1) = NZ - 1).  The sum completes the ATP adjoint of the state
-// before the sub-step (k_rd_cells has scattered the grid_avg8 terms into it),
-// so each point also adds it to its AFM owner's sum for the next sub-step's
-// fa term.  The dead ATP buffer and the cell adjoints this sub-step has
-// consumed are cleared for the next one.
+// next_z(NZ - 1) = NZ - 1) and adds it to the grid_avg8 terms k_rd_cells has
+// scattered there.  The dead ATP buffer and the cell adjoints this sub-step
+// has consumed are cleared for the next one.
 __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTis* __restrict__ R, int q, int qa) {
   const Tis& t = T[blockIdx.y];
   const RdTis& r = R[blockIdx.y];
@@ -360,7 +380,6 @@ __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTi
   const double* lA = r.lA[qa];
   double* nA = r.lA[qa == 2 ? 0 : qa + 1];
   double* dead = r.lA[qa == 0 ? 2 : qa - 1];
-  double* acc_next = r.accA[1 - q];
   const uint8_t* m = t.cmap2;
   const int NX = t.NX, NY = t.NY, NZ = t.NZ, izb = r.izb;
   const double cD = DT_Ca * Da * t.inv_dx * t.inv_dx;
@@ -385,14 +404,11 @@ __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTi
     if (l + 1 < izb) acc += lA[gidx(t, j, k, l + 1)] * cD * mv;
     if (l == 1) acc += lA[gidx(t, j, k, 0)] * cD * mv;
     if (l == NZ - 1) acc += lA[v] * cD * mv;
-    const double fin = nA[v] + acc;
-    nA[v] = fin;
-    const unsigned w = t.vox[v] & 0x7fffffffu;
-    if (w && fin != 0) atomicAdd(&acc_next[w - 1], fin);
+    nA[v] += acc;
   }
   const int n = t.st->n;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
-    r.lu[q][c] = 0; r.lp[q][c] = 0; r.lh[q][c] = 0; r.accA[q][c] = 0;
+    r.lu[q][c] = 0; r.lp[q][c] = 0; r.lh[q][c] = 0;
   }
 }
 
@@ -404,7 +420,7 @@ __global__ void __launch_bounds__(256) k_rd_gj0(const Tis* __restrict__ T, RdTis
   const int start = t.nmemb + t.nder;
   double part = 0;
   RD_FOR_ROW_ENTRIES(t, cb, n, c, i)
-    if (c >= start && is_afm(t.state[c]) && t.ca_kind[i]) part += r.lg[q][i];
+    if (c >= start && is_afm(t.state[c]) && (t.ca_ent[i] & 3)) part += r.lg[q][i];
   for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
   if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[3], part);
 }
@@ -437,9 +453,13 @@ void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, i
   EPI_LAUNCH(K_RD_SEED, k_rd_seed, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cp, cb, q);
 }
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
-                                int qa, int max_n, long long max_box) {
-  EPI_LAUNCH(K_RD_CELLS, k_rd_cells, cells_grid((long long)max_n * RG, ntis), dim3(256), 0, s, dT, dR, slot, cb, bp, q,
-             qa);
+                                int qa, int max_n, long long max_box, int rg) {
+  if (rg == 16)
+    EPI_LAUNCH(K_RD_CELLS, k_rd_cells<16>, cells_grid((long long)max_n * 16, ntis), dim3(256), 0, s, dT, dR, slot, cb,
+               bp, q, qa);
+  else
+    EPI_LAUNCH(K_RD_CELLS, k_rd_cells<8>, cells_grid((long long)max_n * 8, ntis), dim3(256), 0, s, dT, dR, slot, cb,
+               bp, q, qa);
   EPI_LAUNCH(K_RD_GRID, k_rd_grid, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q,
              qa);
 }

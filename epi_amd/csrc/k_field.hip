@@ -208,13 +208,16 @@ __global__ void __launch_bounds__(256) k_ca_begin(const Tis* __restrict__ T, con
       if (t.state[row[e]] == AIR) { air = 1; break; }
     t.touch_air[c] = air;
     const uint8_t s = t.state[c];
+    t.ca_up[0][c] = make_double2(t.ca2p[0][c], t.ip3[0][c]);
+    t.ca_up[1][c] = make_double2(t.ca2p[1][c], t.ip3[1][c]);
     if (c >= start && (s == DEAD || is_afm(s))) {
       // which entries take part in the gap-junction sums (supra_calc
       // :201-209; dead_IP3_calc :160-165 counts the ALIVE ones)
-      uint8_t* kind = t.ca_kind + (size_t)c * t.ns;
+      int* ent = t.ca_ent + (size_t)c * t.ns;
       for (int e = 0; e < len; e++) {
-        const uint8_t so = t.state[row[e]];
-        kind[e] = (so == ALIVE) ? 3 : (so == DEAD || so == FIX || so == MUSUME) ? 1 : 0;
+        const int o = row[e];
+        const uint8_t so = t.state[o];
+        ent[e] = (o << 2) | ((so == ALIVE) ? 3 : (so == DEAD || so == FIX || so == MUSUME) ? 1 : 0);
       }
     }
     if (c >= start && is_afm(s)) {
@@ -288,37 +291,35 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
   __shared__ double s_g[C_GROUPS][16];
   const int lane = threadIdx.x & (CG - 1);
   const int grp = threadIdx.x / CG;
-  const double* __restrict__ ca = t.ca2p[cp];
-  const double* __restrict__ ip3 = t.ip3[cp];
+  const double2* __restrict__ up = t.ca_up[cp];
+  double2* __restrict__ up_next = t.ca_up[1 - cp];
   const int stride = gridDim.x * C_GROUPS;
   for (int c = t.nmemb + t.nder + bx * C_GROUPS + grp; c < n; c += stride) {
     const uint8_t s = t.state[c];
     const int len = t.nlen[cb][c];
-    const double u = ca[c], p = ip3[c];
+    const double2 uc = up[c];
+    const double u = uc.x, p = uc.y;
     const bool dead = s == DEAD;
     if (!dead && !is_afm(s)) continue;
-    const int* row = t.col[cb] + (size_t)c * t.ns;
-    const uint8_t* kind = t.ca_kind + (size_t)c * t.ns;
+    const int* ent = t.ca_ent + (size_t)c * t.ns;
     if (dead) {
       int count = 0;
       double tmp = 0;
       for (int e0 = 0; e0 < len; e0 += CCAP) {
         const int m = min(CCAP, len - e0);
-        int o[EPL];
-        bool al[EPL];
+        int en[EPL];
         double po[EPL];
 #pragma unroll
         for (int i = 0; i < EPL; i++) {
           const int e = lane + i * CG;
-          al[i] = e < m && kind[e0 + e] == 3;
-          o[i] = e < m ? row[e0 + e] : 0;
+          en[i] = e < m ? ent[e0 + e] : 0;
         }
 #pragma unroll
-        for (int i = 0; i < EPL; i++) po[i] = al[i] ? ip3[o[i]] : 0.0;
+        for (int i = 0; i < EPL; i++) po[i] = (en[i] & 3) == 3 ? up[en[i] >> 2].y : 0.0;
 #pragma unroll
         for (int i = 0; i < EPL; i++) {
           const int e = lane + i * CG;
-          if (e < m) { s_f[grp][e] = al[i]; s_t1[grp][e] = po[i]; }
+          if (e < m) { s_f[grp][e] = (en[i] & 3) == 3; s_t1[grp][e] = po[i]; }
         }
         wave_sync();
         if (lane == 0)
@@ -326,7 +327,11 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
             if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
         wave_sync();
       }
-      if (lane == 0) t.ip3[1 - cp][c] = p + DT_Ca * (t.ca_dp * (tmp - count * p) - Kpp * p);
+      if (lane == 0) {
+        const double np = p + DT_Ca * (t.ca_dp * (tmp - count * p) - Kpp * p);
+        t.ip3[1 - cp][c] = np;
+        reinterpret_cast<double*>(up_next + c)[1] = np;   // ca2p of a DEAD cell stays as it is
+      }
       continue;
     }
     // the leader's per-cell values and the corners of grid_avg8 (utils.h:18-23;
@@ -353,33 +358,30 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
     double tmp_diffu = 0, tmp_IP3 = 0;
     for (int e0 = 0; e0 < len; e0 += CCAP) {
       const int m = min(CCAP, len - e0);
-      int o[EPL];
-      uint8_t kd[EPL];
-      double gv[EPL], uo[EPL], po[EPL];
+      int en[EPL];
+      double gv[EPL];
+      double2 w[EPL];
 #pragma unroll
       for (int i = 0; i < EPL; i++) {
         const int e = lane + i * CG;
         const bool in = e < m;
-        o[i] = in ? row[e0 + e] : 0;
-        kd[i] = in ? kind[e0 + e] : 0;
+        en[i] = in ? ent[e0 + e] : 0;
         gv[i] = in ? g[e0 + e] : 0.0;
       }
 #pragma unroll
-      for (int i = 0; i < EPL; i++) {
-        uo[i] = kd[i] ? ca[o[i]] : 0.0;
-        po[i] = kd[i] ? ip3[o[i]] : 0.0;
-      }
+      for (int i = 0; i < EPL; i++) w[i] = (en[i] & 3) ? up[en[i] >> 2] : make_double2(0.0, 0.0);
 #pragma unroll
       for (int i = 0; i < EPL; i++) {
         const int e = lane + i * CG;
         if (e < m) {
+          const int kd = en[i] & 3;
           double a1 = 0, a2 = 0;
-          if (kd[i]) {
-            a1 = gv[i] * (uo[i] - u);
-            a2 = gv[i] * (po[i] - p);
-            if (kd[i] == 3) g[e0 + e] = gv[i] + DT_Ca * fw_term(fabs(uo[i] - u), gv[i]);
+          if (kd) {
+            a1 = gv[i] * (w[i].x - u);
+            a2 = gv[i] * (w[i].y - p);
+            if (kd == 3) g[e0 + e] = gv[i] + DT_Ca * fw_term(fabs(w[i].x - u), gv[i]);
           }
-          s_f[grp][e] = kd[i];
+          s_f[grp][e] = kd;
           s_t1[grp][e] = a1;
           s_t2[grp][e] = a2;
         }
@@ -401,9 +403,11 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
       const double diff_u = ca2p_reaction_factor(u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
       t.diff_u[c] = diff_u;
       const double H0 = 0.5;
-      t.ip3[1 - cp][c] = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + t.ca_dp * IAGv * tmp_IP3);
+      const double np = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + t.ca_dp * IAGv * tmp_IP3);
+      t.ip3[1 - cp][c] = np;
       const double cs = u + DT_Ca * diff_u;
       t.ca2p[1 - cp][c] = cs;
+      up_next[c] = make_double2(cs, np);
       t.ca2p_avg[c] = avg + cs;
       const double para_k2 = 0.7;
       t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);

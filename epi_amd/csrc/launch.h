@@ -22,6 +22,7 @@ struct Tuning {
   int a_zb = 4, a_rows = 4;   // k_atp: the same
   int ca_group = 16;          // k_ca_cells: lanes per cell
   int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
+  int rd_group = 8;           // k_rd_cells: lanes per cell
 };
 
 // Event timer: when the bit of a kernel id is set, its launches carry a
@@ -70,7 +71,7 @@ void launch_rd_state(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot
                      long long items);
 void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, int cb, int q, long long items);
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
-                                int qa, int max_n, long long max_box);
+                                int qa, int max_n, long long max_box, int rg);
 // frozen mechanics window (k_mech.hip)
 struct MechTis;
 void launch_mech_state(hipStream_t s, const Tis* dT, const MechTis* dM, int ntis, int slot, int dir, int pb,

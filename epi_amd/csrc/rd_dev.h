@@ -26,7 +26,6 @@ struct RdTis {
   // ATP adjoint in three rotating buffers: [a] after the sub-step, [a+1]
   // before it, [a+2] dead (cleared for the next sub-step), a = step mod 3
   double* lA[3];
-  double* accA[2];             // per-cell sums of the ATP adjoints of its voxels, by parity q
   double *gkpa;                // dL/dkpa per cell
   double* scal;                // [0] loss, [1] dL/ddu, [2] dL/ddp, [3] dL/dgj0
   double* target;              // per-cell target ca2p

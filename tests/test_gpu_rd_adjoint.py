@@ -116,10 +116,17 @@ def test_adjoint_against_oracle_and_differences(oracle_mod, n_sub, every):
     sel = np.abs(go["kpa"]) > 1e-3 * np.max(np.abs(go["kpa"]))
     assert np.max(np.abs(g["kpa"][sel] - go["kpa"][sel]) / np.abs(go["kpa"][sel])) <= 1e-8
     assert np.all(g["kpa"][~afm] == 0)
-    # checkpoint interval does not change the result
+    # neither the checkpoint interval nor the lanes per cell of the reverse
+    # kernel changes the result beyond the atomics' summation order
     _, g1 = t.set.rd_adjoint([base], n_sub, 1, [tgt])
     for k in ("du", "dp", "gj0"):
         assert g1[0][k] == pytest.approx(g[k], rel=1e-12), k
+    t.set.set_tuning({"rd_group": 16})
+    _, g16 = t.set.rd_adjoint([base], n_sub, every, [tgt])
+    t.set.set_tuning({"rd_group": 8})
+    for k in ("du", "dp", "gj0"):
+        assert g16[0][k] == pytest.approx(g[k], rel=1e-12), k
+    assert np.allclose(g16[0]["kpa"], g["kpa"], rtol=1e-12, atol=0)
 
     def L(**kw):
         p = dict(base)
