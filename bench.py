@@ -77,11 +77,13 @@ def kernel_mix(ts, native, steps):
     (the kernel with the largest total time if it has a byte model, else the
     costliest modeled one).  The timed region then times only that kernel."""
     ts.profile_reset()
+    ts.profile_sample(1)
     ts.profile_enable(None)
     ts.step([native.Driver(0, 0, 0, steps, 0) for _ in range(ts.T)], steps)
     ts.synchronize()
     prof = ts.profile()
     ts.profile_enable([])
+    ts.profile_sample(1)
     ts.profile_reset()
     tot = sum(v[0] for v in prof.values()) or 1.0
     order = sorted(prof.items(), key=lambda kv: -kv[1][0])
@@ -92,12 +94,18 @@ def kernel_mix(ts, native, steps):
     return mix, dominant
 
 
-def timed_kernels(rk):
-    """Kernels that carry events in the timed region.  A dispatch-recorded
-    event pair brackets its kernel only when the kernel before it on the
-    stream carries events too, so the two kernels of a Ca sub-step are timed
-    together."""
-    return ["k_ca_cells", "k_atp"] if rk in ("k_ca_cells", "k_atp") else [rk]
+# Events in the timed region: runs of SAMPLE_RUN consecutive launches, one
+# run every SAMPLE_PERIOD launches.  A dispatch-recorded event pair brackets
+# its kernel only when the launch before it carries events too, so the first
+# launch of a run is not counted; sampling keeps the timer's cost out of the
+# throughput.
+SAMPLE_PERIOD, SAMPLE_RUN = 100, 5
+
+
+def timed_region_profile(ts, rk, sample=True):
+    ts.profile_reset()
+    ts.profile_enable([rk])
+    ts.profile_sample(SAMPLE_PERIOD, SAMPLE_RUN) if sample else ts.profile_sample(1)
 
 
 def roofline(ts, prof, s0, s1, rk):
@@ -232,8 +240,7 @@ def run_batched(args):
     rk = dominant if args.roofline_kernel == "auto" else args.roofline_kernel
     ts.synchronize()
     s0 = [ts.status(t)[0] for t in range(len(specs))]
-    ts.profile_reset()
-    ts.profile_enable(timed_kernels(rk))
+    timed_region_profile(ts, rk)
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -241,6 +248,7 @@ def run_batched(args):
     s1 = [ts.status(t)[0] for t in range(len(specs))]
     prof = ts.profile()
     ts.profile_enable([])
+    ts.profile_sample(1)
     cell_steps = sum(b.cell_steps - a.cell_steps for a, b in zip(s0, s1))
     grid_pts = sum((b.grid_points_b - a.grid_points_b) + (b.grid_points_ca - a.grid_points_ca)
                    for a, b in zip(s0, s1))
@@ -372,11 +380,13 @@ def run_ca_solve(args):
     solves = 3
     t_wall, _ = _timed(ts, lambda: [ts.calc_ca2p(zz) for _ in range(solves)])
     ts.profile_reset()
+    ts.profile_sample(1)
     ts.profile_enable(["k_ca_cells", "k_atp"])
     ts.calc_ca2p(zz)
     ts.synchronize()
     prof = ts.profile()
     ts.profile_enable([])
+    ts.profile_sample(1)
     out = {"metric": "calc_ca2p sub-step wall time", "value": round(1e6 * t_wall / (solves * 1000), 3), "unit": "us",
            "higher_is_better": False, "launch_shapes": ts.get_tuning(),
            "kernel_us": {k: round(1e3 * v[0] / v[1], 3) for k, v in prof.items()}}
@@ -390,11 +400,13 @@ def window_mix(ts, fn):
     (dispatch-recorded events): {kernel: {"ms", "launches"}}."""
     ts.synchronize()
     ts.profile_reset()
+    ts.profile_sample(1)
     ts.profile_enable(None)
     fn()
     ts.synchronize()
     prof = ts.profile()
     ts.profile_enable([])
+    ts.profile_sample(1)
     ts.profile_reset()
     return {k: {"ms": round(ms, 4), "launches": cnt}
             for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0])}
@@ -545,8 +557,11 @@ def main():
     rk = dominant if args.roofline_kernel == "auto" else args.roofline_kernel
     ts.synchronize()
     s0, _ = ts.status(0)
-    ts.profile_reset()
-    ts.profile_enable(None if args.profile_all else timed_kernels(rk))
+    if args.profile_all:
+        ts.profile_reset()
+        ts.profile_enable(None)
+    else:
+        timed_region_profile(ts, rk)
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
     ts.synchronize()
@@ -554,6 +569,7 @@ def main():
     s1, _ = ts.status(0)
     prof = ts.profile()
     ts.profile_enable([])
+    ts.profile_sample(1)
     wall = t1 - t0
     cell_steps = s1.cell_steps - s0.cell_steps
     grid_pts = (s1.grid_points_b - s0.grid_points_b) + (s1.grid_points_ca - s0.grid_points_ca)

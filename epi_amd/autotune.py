@@ -37,6 +37,7 @@ def _time(ts, kernel, run):
     kernels = [kernel] if isinstance(kernel, str) else list(kernel)
     ts.synchronize()
     ts.profile_reset()
+    ts.profile_sample(1)
     ts.profile_enable(kernels)
     run()
     ts.synchronize()

@@ -28,9 +28,14 @@ using namespace epi;
 // ---- per-kernel event timer ----------------------------------------------
 namespace epi {
 namespace {
-struct ProfRec { int k; hipEvent_t a, b; };
+struct ProfRec { int k; hipEvent_t a, b; bool counted; };
 struct Prof {
   unsigned long long mask = 0;
+  // sampling: with period > 1, runs of `run` consecutive launches (of any
+  // kernel) carry events, one run starting every `period` launches; the first
+  // launch of a run only gives the second a bracketed predecessor and is not
+  // counted
+  long long period = 1, run = 1, launches = 0;
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
   double ms[K_COUNT] = {};
@@ -48,12 +53,23 @@ hipEvent_t take_event() {
   return e;
 }
 }  // namespace
-bool prof_on(int k) { return (g_prof.mask >> k) & 1ull; }
+bool prof_on(int k, bool* counted) {
+  if (!g_prof.mask) return false;
+  const bool want = (g_prof.mask >> k) & 1ull;
+  if (g_prof.period <= 1) {
+    *counted = want;
+    return want;
+  }
+  const long long ph = g_prof.launches++ % g_prof.period;
+  if (ph >= g_prof.run) return false;
+  *counted = want && ph > 0;
+  return true;
+}
 void prof_events(hipEvent_t* a, hipEvent_t* b) {
   *a = take_event();
   *b = take_event();
 }
-void prof_add(int k, hipEvent_t a, hipEvent_t b) { g_prof.pending.push_back({k, a, b}); }
+void prof_add(int k, hipEvent_t a, hipEvent_t b, bool counted) { g_prof.pending.push_back({k, a, b, counted}); }
 }  // namespace epi
 
 namespace {
@@ -64,13 +80,13 @@ const char* kKernelNames[K_COUNT] = {
     "k_finish", "k_snapshot", "k_zz_set", "k_zzmax", "k_map_clear", "k_map_stamp", "k_map_halo",
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
-    "k_rd_cells", "k_rd_grid", "k_rd_gj0", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
+    "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
     "k_mech_bwd_pairs", "k_mech_bwd_memb"};
 
 void prof_resolve() {
   for (auto& r : g_prof.pending) {
     float ms = 0;
-    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+    if (r.counted && hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
       g_prof.ms[r.k] += ms;
       g_prof.cnt[r.k]++;
     }
@@ -917,6 +933,17 @@ int epi_profile_enable(epi_ctx* c, uint64_t mask) {
   return EPI_OK;
 }
 
+int epi_profile_sample(epi_ctx* c, int32_t period, int32_t run) {
+  if (!c) return EPI_ERR_INVALID;
+  if (period < 1 || (period > 1 && (run < 2 || run > period)))
+    return fail(c, EPI_ERR_INVALID, "profile sampling needs period >= 1 and 2 <= run <= period");
+  RCHK(sync_status(c));
+  g_prof.period = period;
+  g_prof.run = period > 1 ? run : 1;
+  g_prof.launches = 0;
+  return EPI_OK;
+}
+
 int epi_profile_read(epi_ctx* c, int32_t kernel, const char** name, double* total_ms, int64_t* count) {
   if (!c) return EPI_ERR_INVALID;
   if (kernel < 0 || kernel >= K_COUNT) return fail(c, EPI_ERR_INVALID, "kernel id");
@@ -1261,6 +1288,9 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
       for (int b = 0; b < 3; b++) r.lA[b] = cv.take<double>((size_t)r.box);
       r.gkpa = cv.take<double>(cap); r.target = cv.take<double>(cap);
       r.kpa_in = cv.take<double>(cap); r.scal = cv.take<double>(4);
+      const Tis& ti = c->tis[t];
+      r.oc = (4 * ti.irx + 1) * (4 * ti.iry + 1) * (4 * ti.irz + 1);
+      r.own = cv.take<int>(cap * (size_t)r.oc); r.own_n = cv.take<int>(cap);
       r.nafm = 0;
     }
   };
@@ -1272,6 +1302,7 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
   carve(real);
   HIPCHK(c, hipMemcpyAsync(rd.d, rd.h.data(), sizeof(RdTis) * c->T, hipMemcpyHostToDevice, c->stream));
   launch_rd_snapshot(c->stream, c->d_tis, rd.d, c->T, c->cb, c->cp, c->ap, rd_items(c));
+  launch_rd_own(c->stream, c->d_tis, rd.d, c->T, max_n(c));
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(rd.h.data(), rd.d, sizeof(RdTis) * c->T, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -134,8 +134,8 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
 }
 
 // the loss and the seed of the reverse pass.  Every adjoint buffer starts at
-// zero; the sub-steps keep them so (k_rd_grid clears the cell adjoints and
-// the ATP buffer the next sub-step scatters into).
+// zero; the sub-steps keep them so (a cell's group clears the cell adjoints
+// it has read, the grid part the ATP buffer the next sub-step scatters into).
 __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTis* __restrict__ R, int cp, int cb,
                                                   int q) {
   const Tis& t = T[blockIdx.y];
@@ -170,8 +170,7 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
 // its first 32 / RG entries (and their neighbour values) in registers for
 // both passes, every load of them issued up front; entries past 32 take the
 // same steps from memory.  The ATP adjoint of the cell's fa source is
-// gathered from the voxels the cell owns, which lie in the box its sphere
-// stamp covers (lat +- 2 ir, z from 1, periodic x and y).
+// gathered from the voxels the cell owns (k_rd_own's list).
 template <int RG>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -209,16 +208,14 @@ __device__ __forceinline__ void rd_entry_bwd(const RdEntry& k, int kd, int o, do
 }  // namespace
 
 template <int RG>
-__global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
-                                                   int cb, int bp, int q, int qa) {
+__device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int bid, int nblk, int slot, int cb,
+                                              int bp, int q, int qa) {
   constexpr int REP = 32 / RG;
-  const Tis& t = T[blockIdx.y];
-  RdTis& r = R[blockIdx.y];
   const int n = t.st->n;
   const int start = t.nmemb + t.nder;
   const long long cap = t.cap, ns = t.ns;
   const int lane = threadIdx.x & (RG - 1);
-  const int groups = gridDim.x * (blockDim.x / RG);
+  const int groups = nblk * (blockDim.x / RG);
   const double* u = r.su + slot * cap;
   const double* p = r.sp + slot * cap;
   const double* h = r.sh + slot * cap;
@@ -229,11 +226,11 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
   const double* lA = r.lA[qa];
   double* nA = r.lA[qa == 2 ? 0 : qa + 1];
   const double du = t.ca_du, dp = t.ca_dp;
-  // the stamp box of an AFM cell (k_map_stamp)
-  const int bx = 4 * t.irx + 1, by = 4 * t.iry + 1, bz = 4 * t.irz + 1;
-  const int bxy = bx * by, bvol = bxy * bz;
   double pdu = 0, pdp = 0;
-  for (int c = start + blockIdx.x * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
+  double* const lu_w = r.lu[q];
+  double* const lp_w = r.lp[q];
+  double* const lh_w = r.lh[q];
+  for (int c = start + bid * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
     const uint8_t s = t.state[c];
     const int len = t.nlen[cb][c];
     const double lpc = lp[c];
@@ -253,6 +250,8 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
       if (lane == 0) {
         atomicAdd(&np[c], lpc * (1.0 - DT_Ca * dp * cnt - DT_Ca * Kpp));
         pdp += lpc * DT_Ca * (sum - cnt * pc);
+        // consumed: cleared for the sub-step after next, which scatters here
+        lu_w[c] = 0; lp_w[c] = 0; lh_w[c] = 0;
       }
       continue;
     }
@@ -287,16 +286,9 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
     // ATP adjoint of the voxels the cell owns (after the sub-step)
     double acc_own = 0;
     {
-      const unsigned me = (unsigned)c + 1;
-      const int z0 = max(iz - 2 * t.irz, 1);
-      for (int b = lane; b < bvol; b += RG) {
-        const int l = z0 + b / bxy;
-        if (l >= r.izb || l >= t.NZ || l > iz + 2 * t.irz) continue;
-        const int jx = (ix - 2 * t.irx + b % bx + t.NX) % t.NX;
-        const int ky = (iy - 2 * t.iry + (b / bx) % by + t.NY) % t.NY;
-        const long long v = gidx(t, jx, ky, l);
-        if ((t.vox[v] & 0x7fffffffu) == me) acc_own += lA[v];
-      }
+      const int no = r.own_n[c];
+      const int* ov = r.own + (size_t)c * r.oc;
+      for (int i = lane; i < no; i += RG) acc_own += lA[ov[i]];
     }
     double su = 0, sp = 0, gsum = 0;
 #pragma unroll
@@ -356,6 +348,7 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
       pdu += abar * iag * su;
       pdp += lpc * DT_Ca * iag * sp;
       r.gkpa[c] += lpc * DT_Ca * Aavg / (H0 + Aavg);
+      lu_w[c] = 0; lp_w[c] = 0; lh_w[c] = 0;
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -370,12 +363,11 @@ __global__ void __launch_bounds__(256) k_rd_cells(const Tis* __restrict__ T, RdT
 
 // reverse of ATP_refresh: each point of the box gathers from the points
 // whose update read it (periodic x, y; mirror z: prev_z(0) = 1,
-// next_z(NZ - 1) = NZ - 1) and adds it to the grid_avg8 terms k_rd_cells has
-// scattered there.  The dead ATP buffer and the cell adjoints this sub-step
-// has consumed are cleared for the next one.
-__global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTis* __restrict__ R, int q, int qa) {
-  const Tis& t = T[blockIdx.y];
-  const RdTis& r = R[blockIdx.y];
+// next_z(NZ - 1) = NZ - 1) and adds it to the ATP adjoint before the
+// sub-step, beside the grid_avg8 terms the cell part scatters there (both
+// with atomics, so the two parts run side by side).  The dead ATP buffer is
+// cleared for the next sub-step.
+__device__ __forceinline__ void rd_grid_part(const Tis& t, const RdTis& r, int bid, int nblk, int qa) {
   const long long end = box_end(t, r);
   const double* lA = r.lA[qa];
   double* nA = r.lA[qa == 2 ? 0 : qa + 1];
@@ -383,8 +375,8 @@ __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTi
   const uint8_t* m = t.cmap2;
   const int NX = t.NX, NY = t.NY, NZ = t.NZ, izb = r.izb;
   const double cD = DT_Ca * Da * t.inv_dx * t.inv_dx;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
+  const long long stride = (long long)nblk * blockDim.x;
+  for (long long v = (long long)bid * blockDim.x + threadIdx.x; v < r.box; v += stride) {
     dead[v] = 0;
     if (v >= end) continue;
     const int j = (int)(v % t.XP), k = (int)((v / t.XP) % (NY + 1)), l = (int)(v / t.PL);
@@ -404,12 +396,18 @@ __global__ void __launch_bounds__(256) k_rd_grid(const Tis* __restrict__ T, RdTi
     if (l + 1 < izb) acc += lA[gidx(t, j, k, l + 1)] * cD * mv;
     if (l == 1) acc += lA[gidx(t, j, k, 0)] * cD * mv;
     if (l == NZ - 1) acc += lA[v] * cD * mv;
-    nA[v] += acc;
+    if (acc != 0) atomicAdd(&nA[v], acc);
   }
-  const int n = t.st->n;
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
-    r.lu[q][c] = 0; r.lp[q][c] = 0; r.lh[q][c] = 0;
-  }
+}
+
+// one reverse sub-step: blocks [0, ncb) take the cells, the rest the box
+template <int RG>
+__global__ void __launch_bounds__(256) k_rd_step(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot, int cb,
+                                                  int bp, int q, int qa, int ncb) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  if ((int)blockIdx.x < ncb) rd_cells_part<RG>(t, r, blockIdx.x, ncb, slot, cb, bp, q, qa);
+  else rd_grid_part(t, r, blockIdx.x - ncb, gridDim.x - ncb, qa);
 }
 
 // dL/dgj0: the adjoints of every coupled edge at the window's start
@@ -425,6 +423,39 @@ __global__ void __launch_bounds__(256) k_rd_gj0(const Tis* __restrict__ T, RdTis
   if ((threadIdx.x & 63) == 0 && part != 0) atomicAdd(&r.scal[3], part);
 }
 
+// the voxels each AFM cell owns in the frozen voxel map of a window: they
+// lie in the box its sphere stamp covers (k_map_stamp: lat +- 2 ir, z from 1
+// to NZ - 1, periodic x and y) and below iz_bound.  One thread per cell,
+// stamp-box order; more than oc voxels is an error (the host sizes oc as the
+// whole box).
+__global__ void __launch_bounds__(256) k_rd_own(const Tis* __restrict__ T, RdTis* __restrict__ R) {
+  const Tis& t = T[blockIdx.y];
+  const RdTis& r = R[blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  const int bx = 4 * t.irx + 1, by = 4 * t.iry + 1;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    int m = 0;
+    if (c >= start && is_afm(t.state[c])) {
+      const unsigned me = (unsigned)c + 1;
+      const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
+      const int z0 = max(iz - 2 * t.irz, 1), z1 = min(min(iz + 2 * t.irz, t.NZ - 1), r.izb - 1);
+      int* ov = r.own + (size_t)c * r.oc;
+      for (int l = z0; l <= z1; l++)
+        for (int b = 0; b < bx * by; b++) {
+          const int jx = (ix - 2 * t.irx + b % bx + t.NX) % t.NX;
+          const int ky = (iy - 2 * t.iry + b / bx + t.NY) % t.NY;
+          const long long v = gidx(t, jx, ky, l);
+          if ((t.vox[v] & 0x7fffffffu) != me) continue;
+          if (m < r.oc) ov[m] = (int)v;
+          m++;
+        }
+      if (m > r.oc) { set_err(t.st, ERR_CAPACITY); m = r.oc; }
+    }
+    r.own_n[c] = m;
+  }
+}
+
 // ---- launchers -------------------------------------------------------------
 namespace {
 dim3 cells_grid(long long items, int ntis) {
@@ -437,6 +468,9 @@ dim3 cells_grid(long long items, int ntis) {
 
 void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items) {
   EPI_LAUNCH(K_RD_SNAPSHOT, k_rd_snapshot, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, cp, ap);
+}
+void launch_rd_own(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int max_n) {
+  EPI_LAUNCH(K_RD_OWN, k_rd_own, cells_grid(max_n, ntis), dim3(256), 0, s, dT, dR);
 }
 void launch_rd_restore(hipStream_t s, const Tis* dT, RdTis* dR, const double* gj0, int ntis, int cb, int cp, int ap,
                        int use_kpa, long long items) {
@@ -454,14 +488,12 @@ void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, i
 }
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
                                 int qa, int max_n, long long max_box, int rg) {
+  const int ncb = (int)cells_grid((long long)max_n * rg, ntis).x;
+  const dim3 g(ncb + cells_grid(max_box, ntis).x, ntis);
   if (rg == 16)
-    EPI_LAUNCH(K_RD_CELLS, k_rd_cells<16>, cells_grid((long long)max_n * 16, ntis), dim3(256), 0, s, dT, dR, slot, cb,
-               bp, q, qa);
+    EPI_LAUNCH(K_RD_STEP, k_rd_step<16>, g, dim3(256), 0, s, dT, dR, slot, cb, bp, q, qa, ncb);
   else
-    EPI_LAUNCH(K_RD_CELLS, k_rd_cells<8>, cells_grid((long long)max_n * 8, ntis), dim3(256), 0, s, dT, dR, slot, cb,
-               bp, q, qa);
-  EPI_LAUNCH(K_RD_GRID, k_rd_grid, cells_grid(std::max<long long>(max_n, max_box), ntis), dim3(256), 0, s, dT, dR, q,
-             qa);
+    EPI_LAUNCH(K_RD_STEP, k_rd_step<8>, g, dim3(256), 0, s, dT, dR, slot, cb, bp, q, qa, ncb);
 }
 void launch_rd_gj0(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int q, long long items) {
   EPI_LAUNCH(K_RD_GJ0, k_rd_gj0, cells_grid(items, ntis), dim3(256), 0, s, dT, dR, cb, q);

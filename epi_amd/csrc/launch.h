@@ -12,8 +12,8 @@ enum KernelId {
   K_INIT_SC, K_PERIODIC_FIX, K_BUCKET_INSERT, K_BUCKET_SORT, K_SWEEP,
   K_FINISH, K_SNAPSHOT, K_ZZ_SET, K_ZZMAX, K_MAP_CLEAR, K_MAP_STAMP, K_MAP_HALO,
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
-  K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_CELLS, K_RD_GRID,
-  K_RD_GJ0, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB, K_COUNT
+  K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
+  K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB, K_COUNT
 };
 
 // Launch shapes that the autotuner may change (values checked by the host).
@@ -29,17 +29,18 @@ struct Tuning {
 // start and a stop event that the dispatch itself records
 // (hipExtLaunchKernelGGL), so the elapsed time is the kernel's own duration
 // without the cost of separate event packets (see host.hip).
-bool prof_on(int k);
+bool prof_on(int k, bool* counted);
 void prof_events(hipEvent_t* a, hipEvent_t* b);
-void prof_add(int k, hipEvent_t a, hipEvent_t b);
+void prof_add(int k, hipEvent_t a, hipEvent_t b, bool counted);
 
 #define EPI_LAUNCH(kid, kernel, grid, block, shmem, stream, ...)                              \
   do {                                                                                        \
-    if (::epi::prof_on(kid)) {                                                                \
+    bool cnt_ = false;                                                                        \
+    if (::epi::prof_on(kid, &cnt_)) {                                                         \
       hipEvent_t a_ = nullptr, b_ = nullptr;                                                  \
       ::epi::prof_events(&a_, &b_);                                                           \
       hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, a_, b_, 0, __VA_ARGS__);      \
-      ::epi::prof_add(kid, a_, b_);                                                           \
+      ::epi::prof_add(kid, a_, b_, cnt_);                                                     \
     } else {                                                                                  \
       hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                    \
     }                                                                                         \
@@ -64,6 +65,7 @@ void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int
 // frozen reaction-diffusion window (k_adjoint.hip)
 struct RdTis;
 void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items);
+void launch_rd_own(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int max_n);
 void launch_rd_restore(hipStream_t s, const Tis* dT, RdTis* dR, const double* gj0, int ntis, int cb, int cp, int ap,
                        int use_kpa, long long items);
 void launch_rd_put_gj(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, long long items);

@@ -27,6 +27,11 @@ struct RdTis {
   // before it, [a+2] dead (cleared for the next sub-step), a = step mod 3
   double* lA[3];
   double *gkpa;                // dL/dkpa per cell
+  // the voxels each AFM cell owns in the window's frozen voxel map (its fa
+  // source), up to oc per cell, in stamp-box order
+  int* own;
+  int* own_n;
+  int oc;
   double* scal;                // [0] loss, [1] dL/ddu, [2] dL/ddp, [3] dL/dgj0
   double* target;              // per-cell target ca2p
   double* kpa_in;              // per-cell kpa for the next run (set by the host)

@@ -95,7 +95,7 @@ SYMBOLS = [
     "epi_get_status", "epi_get_cells", "epi_get_fields", "epi_set_positions",
     "epi_set_state", "epi_set_lists", "epi_set_maps", "epi_profile_enable",
     "epi_profile_read", "epi_profile_reset", "epi_profile_kernel_count",
-    "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n",
+    "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n", "epi_profile_sample",
     "epi_rd_begin", "epi_rd_forward", "epi_rd_get_ca2p", "epi_rd_adjoint", "epi_rd_end",
     "epi_mech_begin", "epi_mech_forward", "epi_mech_get_positions", "epi_mech_adjoint", "epi_mech_end",
 ]
@@ -159,6 +159,7 @@ def load_library():
     L.epi_set_lists.argtypes = [vp, C.c_int32, _ip, _ip, _dp]
     L.epi_set_maps.argtypes = [vp, C.c_int32, _ip, _u8p, _ip]
     L.epi_profile_enable.argtypes = [vp, C.c_uint64]
+    L.epi_profile_sample.argtypes = [vp, C.c_int32, C.c_int32]
     L.epi_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), _dp, _i64p]
     L.epi_profile_reset.argtypes = [vp]
     L.epi_profile_kernel_count.argtypes = []
@@ -327,6 +328,12 @@ class TissueSet:
             if names is None or self.kernel_name(k) in names:
                 mask |= 1 << k
         self._chk(self.L.epi_profile_enable(self.h, mask), "profile_enable")
+
+    def profile_sample(self, period=1, run=2):
+        """period 1: every launch of an enabled kernel is timed.  period > 1:
+        runs of `run` consecutive launches carry events, one run every
+        `period` launches; the first launch of a run is not counted."""
+        self._chk(self.L.epi_profile_sample(self.h, int(period), int(run)), "profile_sample")
 
     def kernel_name(self, k):
         nm = C.c_char_p()

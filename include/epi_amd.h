@@ -168,9 +168,17 @@ int epi_set_maps(epi_ctx* ctx, int32_t tissue, const int32_t* cmap1, const uint8
 
 /* ---- kernel timing -------------------------------------------------------
  * No counterpart in the reference (its only timer is proc.cpp:128-137).
- * Bit k of mask brackets every launch of kernel k with HIP events on the
- * context's stream; totals accumulate until epi_profile_reset. */
+ * Bit k of mask brackets launches of kernel k with HIP events recorded by
+ * the dispatch on the context's stream; totals accumulate until
+ * epi_profile_reset. */
 int epi_profile_enable(epi_ctx* ctx, uint64_t mask);
+/* period 1 (the default): every launch of an enabled kernel carries events.
+ * period > 1: runs of `run` consecutive launches (of any kernel) carry events,
+ * one run every `period` launches, and the enabled kernels' launches in a run
+ * are counted except the first launch of the run.  A dispatch-recorded event
+ * pair brackets its kernel only when the launch before it also carries
+ * events; sampling keeps that while timing a small share of launches. */
+int epi_profile_sample(epi_ctx* ctx, int32_t period, int32_t run);
 int epi_profile_read(epi_ctx* ctx, int32_t kernel, const char** name, double* total_ms, int64_t* count);
 int epi_profile_reset(epi_ctx* ctx);
 int32_t epi_profile_kernel_count(void);
