@@ -82,6 +82,7 @@ def kernel_mix(ts, native, steps):
     ts.step([native.Driver(0, 0, 0, steps, 0) for _ in range(ts.T)], steps)
     ts.synchronize()
     prof = ts.profile()
+    launched = ts.profile_launches()
     ts.profile_enable([])
     ts.profile_sample(1)
     ts.profile_reset()
@@ -108,14 +109,16 @@ def timed_region_profile(ts, rk, sample=True):
     ts.profile_sample(SAMPLE_PERIOD, SAMPLE_RUN) if sample else ts.profile_sample(1)
 
 
-def roofline(ts, prof, s0, s1, rk):
+def roofline(ts, prof, s0, s1, rk, launched):
     """Algorithmic bytes per launch of kernel rk over its mean launch
-    duration (dispatch-recorded HIP events on the library's stream during the
-    timed region)."""
-    if rk not in prof:
+    duration (dispatch-recorded HIP events on the library's stream, sampled
+    over the timed region).  `launched` counts every launch of rk in the
+    timed region, timed or not."""
+    if rk not in prof or not launched:
         return None
-    ms, cnt = prof[rk]
-    avg_s = ms / cnt / 1e3
+    ms, timed = prof[rk]
+    avg_s = ms / timed / 1e3
+    cnt = launched
     if rk == "k_ca_cells":
         # every Ca sub-step of the timed region covers the solving tissues;
         # weight each tissue's cell bytes by its share of the sub-steps
@@ -146,7 +149,7 @@ def roofline(ts, prof, s0, s1, rk):
             traffic = json.load(f).get("bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": rk,
-            "avg_us": round(avg_s * 1e6, 3), "launches": cnt, "units_per_launch": unit,
+            "avg_us": round(avg_s * 1e6, 3), "launches": cnt, "timed_launches": timed, "units_per_launch": unit,
             "algorithmic_bytes_per_launch": round(alg, 1)}
     return roof
 
@@ -247,6 +250,7 @@ def run_batched(args):
     wall = time.perf_counter() - t0
     s1 = [ts.status(t)[0] for t in range(len(specs))]
     prof = ts.profile()
+    launched = ts.profile_launches()
     ts.profile_enable([])
     ts.profile_sample(1)
     cell_steps = sum(b.cell_steps - a.cell_steps for a, b in zip(s0, s1))
@@ -254,7 +258,7 @@ def run_batched(args):
                    for a, b in zip(s0, s1))
     ca_solves = sum(b.ca_solves - a.ca_solves for a, b in zip(s0, s1))
     errs = [b.err for b in s1 if b.err]
-    roof = roofline(ts, prof, s0, s1, rk)
+    roof = roofline(ts, prof, s0, s1, rk, launched.get(rk, 0))
     ts.close()
     # per-instance loop over an evenly spaced sample of the same tissues
     idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
@@ -385,6 +389,7 @@ def run_ca_solve(args):
     ts.calc_ca2p(zz)
     ts.synchronize()
     prof = ts.profile()
+    launched = ts.profile_launches()
     ts.profile_enable([])
     ts.profile_sample(1)
     out = {"metric": "calc_ca2p sub-step wall time", "value": round(1e6 * t_wall / (solves * 1000), 3), "unit": "us",
@@ -405,6 +410,7 @@ def window_mix(ts, fn):
     fn()
     ts.synchronize()
     prof = ts.profile()
+    launched = ts.profile_launches()
     ts.profile_enable([])
     ts.profile_sample(1)
     ts.profile_reset()
@@ -568,6 +574,7 @@ def main():
     t1 = time.perf_counter()
     s1, _ = ts.status(0)
     prof = ts.profile()
+    launched = ts.profile_launches()
     ts.profile_enable([])
     ts.profile_sample(1)
     wall = t1 - t0
@@ -575,7 +582,7 @@ def main():
     grid_pts = (s1.grid_points_b - s0.grid_points_b) + (s1.grid_points_ca - s0.grid_points_ca)
     ca_solves = s1.ca_solves - s0.ca_solves
     value = cell_steps / wall
-    roof = roofline(ts, prof, [s0], [s1], rk)
+    roof = roofline(ts, prof, [s0], [s1], rk, launched.get(rk, 0))
     if args.profile_all:
         tot = sum(v[0] for v in prof.values())
         for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):

@@ -40,6 +40,7 @@ struct Prof {
   std::vector<hipEvent_t> pool;
   double ms[K_COUNT] = {};
   long long cnt[K_COUNT] = {};
+  long long launched[K_COUNT] = {};   // every launch of an enabled kernel, timed or not
 };
 Prof g_prof;
 hipEvent_t take_event() {
@@ -56,6 +57,7 @@ hipEvent_t take_event() {
 bool prof_on(int k, bool* counted) {
   if (!g_prof.mask) return false;
   const bool want = (g_prof.mask >> k) & 1ull;
+  if (want) g_prof.launched[k]++;
   if (g_prof.period <= 1) {
     *counted = want;
     return want;
@@ -955,11 +957,18 @@ int epi_profile_read(epi_ctx* c, int32_t kernel, const char** name, double* tota
   return EPI_OK;
 }
 
+int epi_profile_launches(epi_ctx* c, int32_t kernel, int64_t* launches) {
+  if (!c || !launches) return EPI_ERR_INVALID;
+  if (kernel < 0 || kernel >= K_COUNT) return fail(c, EPI_ERR_INVALID, "kernel id");
+  *launches = g_prof.launched[kernel];
+  return EPI_OK;
+}
+
 int epi_profile_reset(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   prof_resolve();
-  for (int k = 0; k < K_COUNT; k++) { g_prof.ms[k] = 0; g_prof.cnt[k] = 0; }
+  for (int k = 0; k < K_COUNT; k++) { g_prof.ms[k] = 0; g_prof.cnt[k] = 0; g_prof.launched[k] = 0; }
   return EPI_OK;
 }
 

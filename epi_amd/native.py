@@ -95,7 +95,7 @@ SYMBOLS = [
     "epi_get_status", "epi_get_cells", "epi_get_fields", "epi_set_positions",
     "epi_set_state", "epi_set_lists", "epi_set_maps", "epi_profile_enable",
     "epi_profile_read", "epi_profile_reset", "epi_profile_kernel_count",
-    "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n", "epi_profile_sample",
+    "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n", "epi_profile_sample", "epi_profile_launches",
     "epi_rd_begin", "epi_rd_forward", "epi_rd_get_ca2p", "epi_rd_adjoint", "epi_rd_end",
     "epi_mech_begin", "epi_mech_forward", "epi_mech_get_positions", "epi_mech_adjoint", "epi_mech_end",
 ]
@@ -160,6 +160,7 @@ def load_library():
     L.epi_set_maps.argtypes = [vp, C.c_int32, _ip, _u8p, _ip]
     L.epi_profile_enable.argtypes = [vp, C.c_uint64]
     L.epi_profile_sample.argtypes = [vp, C.c_int32, C.c_int32]
+    L.epi_profile_launches.argtypes = [vp, C.c_int32, _i64p]
     L.epi_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), _dp, _i64p]
     L.epi_profile_reset.argtypes = [vp]
     L.epi_profile_kernel_count.argtypes = []
@@ -348,6 +349,17 @@ class TissueSet:
             self._chk(self.L.epi_profile_read(self.h, k, C.byref(nm), C.byref(ms), C.byref(cnt)), "profile_read")
             if cnt.value:
                 out[nm.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def profile_launches(self):
+        """{kernel: launches} of the enabled kernels since the last reset,
+        timed or not (with sampling only some launches are timed)."""
+        out = {}
+        for k in range(self.L.epi_profile_kernel_count()):
+            n = C.c_int64()
+            self._chk(self.L.epi_profile_launches(self.h, k, C.byref(n)), "profile_launches")
+            if n.value:
+                out[self.kernel_name(k)] = n.value
         return out
 
     def profile_reset(self):

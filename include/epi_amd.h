@@ -180,6 +180,8 @@ int epi_profile_enable(epi_ctx* ctx, uint64_t mask);
  * events; sampling keeps that while timing a small share of launches. */
 int epi_profile_sample(epi_ctx* ctx, int32_t period, int32_t run);
 int epi_profile_read(epi_ctx* ctx, int32_t kernel, const char** name, double* total_ms, int64_t* count);
+/* every launch of an enabled kernel since the last reset, timed or not */
+int epi_profile_launches(epi_ctx* ctx, int32_t kernel, int64_t* launches);
 int epi_profile_reset(epi_ctx* ctx);
 int32_t epi_profile_kernel_count(void);
 
