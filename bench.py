@@ -36,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # per AFM cell of a Ca sub-step 96 + 24 bytes per list entry (k_ca_cells)
 BYTES_PER_POINT = {"k_atp": 22.0, "k_ext_stim": 21.0}
 CA_CELL_BYTES, CA_ENTRY_BYTES = 96.0, 24.0
-MODELED = ("k_ca_cells", "k_atp", "k_ext_stim")
+MODELED = ("k_ca_cells", "k_atp", "k_ext_stim", "k_ca_fused")
 
 
 def parse():
@@ -109,7 +109,7 @@ def timed_region_profile(ts, rk, sample=True):
     ts.profile_sample(SAMPLE_PERIOD, SAMPLE_RUN) if sample else ts.profile_sample(1)
 
 
-def roofline(ts, prof, s0, s1, rk, launched):
+def roofline(ts, prof, s0, s1, rk, launched, workload):
     """Algorithmic bytes per launch of kernel rk over its mean launch
     duration (dispatch-recorded HIP events on the library's stream, sampled
     over the timed region).  `launched` counts every launch of rk in the
@@ -119,23 +119,29 @@ def roofline(ts, prof, s0, s1, rk, launched):
     ms, timed = prof[rk]
     avg_s = ms / timed / 1e3
     cnt = launched
-    if rk == "k_ca_cells":
-        # every Ca sub-step of the timed region covers the solving tissues;
-        # weight each tissue's cell bytes by its share of the sub-steps
-        alg, units, entries = 0.0, 0, 0.0
-        subs = [b.grid_points_ca - a.grid_points_ca for a, b in zip(s0, s1)]
+    if rk in ("k_ca_cells", "k_ca_fused"):
+        # a launch advances the cells of one Ca sub-step of every solving
+        # tissue (a fused launch also the ATP grid of the sub-step before):
+        # weight each tissue by its share of the launches
+        per_solve = 1000 if rk == "k_ca_cells" else 999
+        alg, units, entries, points = 0.0, 0, 0.0, 0.0
         for t in range(ts.T):
-            if not subs[t]:
+            solves = s1[t].ca_solves - s0[t].ca_solves
+            if not solves:
                 continue
             c = ts.get_cells(t)
             st, lens = c["state"], np.diff(c["conn_ptr"])
             afm = (st == 0) | (st == 4) | (st == 7)
             afm[:c["nmemb"] + c["nder"]] = False
-            w = (s1[t].ca_solves - s0[t].ca_solves) * 1000 / cnt
+            w = solves * per_solve / cnt
             units += int(afm.sum() * w)
             entries += float(lens[afm].sum()) * w
+            points += (s1[t].grid_points_ca - s0[t].grid_points_ca) / (solves * 1000) * w
         alg = CA_CELL_BYTES * units + CA_ENTRY_BYTES * entries
         unit = f"{units} AFM cells, mean list {entries / max(units, 1):.2f}"
+        if rk == "k_ca_fused":
+            alg += BYTES_PER_POINT["k_atp"] * points
+            unit += f", {points:.0f} ATP grid points"
     else:
         key = "grid_points_ca" if rk == "k_atp" else "grid_points_b"
         units = sum(getattr(b, key) - getattr(a, key) for a, b in zip(s0, s1)) / cnt
@@ -143,7 +149,7 @@ def roofline(ts, prof, s0, s1, rk, launched):
         unit = f"{units:.0f} grid points"
     achieved = alg / avg_s / 1e9
     traffic = None
-    pmc = os.path.join(HERE, "profiles", f"{rk}_traffic.json")
+    pmc = os.path.join(HERE, "profiles", f"{rk}_{workload}_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -258,7 +264,7 @@ def run_batched(args):
                    for a, b in zip(s0, s1))
     ca_solves = sum(b.ca_solves - a.ca_solves for a, b in zip(s0, s1))
     errs = [b.err for b in s1 if b.err]
-    roof = roofline(ts, prof, s0, s1, rk, launched.get(rk, 0))
+    roof = roofline(ts, prof, s0, s1, rk, launched.get(rk, 0), "batched")
     ts.close()
     # per-instance loop over an evenly spaced sample of the same tissues
     idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
@@ -582,7 +588,7 @@ def main():
     grid_pts = (s1.grid_points_b - s0.grid_points_b) + (s1.grid_points_ca - s0.grid_points_ca)
     ca_solves = s1.ca_solves - s0.ca_solves
     value = cell_steps / wall
-    roof = roofline(ts, prof, [s0], [s1], rk, launched.get(rk, 0))
+    roof = roofline(ts, prof, [s0], [s1], rk, launched.get(rk, 0), args.config)
     if args.profile_all:
         tot = sum(v[0] for v in prof.values())
         for k, (ms, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):

@@ -6,6 +6,7 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
   k_atp        the same
   k_ca_cells   lanes per cell (4, 8, 16)
   k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
+  Ca solve     ATP of a sub-step and cells of the next in one launch (0, 1)
 
 A launch shape changes timing only: every grid point is computed on its own
 and the Ca cell kernel sums each list in list order whatever its group size,
@@ -21,7 +22,7 @@ import time
 
 STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
-DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0}
+DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0}
 
 
 def _avg_us(ts, kernel):
@@ -55,7 +56,7 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
     if warm_steps:
         ts.step(drivers, warm_steps)
     zz = ts.calc_zzmax()
-    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}, "xcd": {}}
+    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}, "xcd": {}, "ca_fuse": {}}
     best = dict(DEFAULT)
     ts.set_tuning(best)
 
@@ -93,6 +94,18 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
         ts.set_tuning({"xcd": x})
         report["xcd"][f"xcd{x}"] = _time(ts, ("k_ca_cells", "k_atp"), run_ca)
     best["xcd"] = int(min(report["xcd"], key=report["xcd"].get)[3:])
+    ts.set_tuning(best)
+
+    # per sub-step: the two kernels against one fused launch (z planes per
+    # thread of its grid part from the candidates above)
+    report["ca_fuse"] = {"unfused": report["xcd"][f"xcd{best['xcd']}"]}
+    for zb in (1, 2, 4):
+        ts.set_tuning({"ca_fuse": 1, "atp_zb": zb})
+        report["ca_fuse"][f"fused_zb{zb}"] = _time(ts, "k_ca_fused", run_ca)
+    key = min(report["ca_fuse"], key=report["ca_fuse"].get)
+    if key != "unfused":
+        best["ca_fuse"] = 1
+        best["atp_zb"] = int(key[8:])
     ts.set_tuning(best)
     report["seconds"] = time.perf_counter() - t0
     if log is not None:

@@ -107,6 +107,8 @@ struct Tis {
   // list entry reads its neighbour's pair with one 16-byte gather; every
   // kernel that writes ca2p or IP3 inside a solve or window writes it too
   double2* ca_up[2];
+  // diff_u of the Ca sub-steps at parity 1 (parity 0 writes diff_u itself)
+  double* ca_dfu;
   long long* uid; long long* old_uid; int* src_row;
   // neighbour rows (two buffers: current and previous build)
   int* nlen[2]; int* col[2]; double* gj[2];

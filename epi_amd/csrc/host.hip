@@ -83,7 +83,7 @@ const char* kKernelNames[K_COUNT] = {
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
-    "k_mech_bwd_pairs", "k_mech_bwd_memb"};
+    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused"};
 
 void prof_resolve() {
   for (auto& r : g_prof.pending) {
@@ -212,6 +212,7 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.ca_iag = cv.take<double>(cap);
   t.ca_ent = cv.take<int>(cap * ns);
   for (int b = 0; b < 2; b++) t.ca_up[b] = cv.take<double2>(cap);
+  t.ca_dfu = cv.take<double>(cap);
   t.uid = cv.take<long long>(cap);
   t.old_uid = cv.take<long long>(cap);
   t.src_row = cv.take<int>(cap);
@@ -371,12 +372,8 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
   long long box = 1;
   for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * std::min(izb, h.NZ + 1));
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
-  for (int s = 0; s < n_sub; s++) {
-    launch_ca_substep(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c), izb,
-                      c->tu);
-    c->cp ^= 1;
-    c->ap ^= 1;
-  }
+  launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mn, max_nx(c), max_ny(c),
+                     izb, c->tu);
   launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
   HIPCHK(c, hipGetLastError());
   return EPI_OK;
@@ -984,6 +981,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_CA_GROUP: return &tu.ca_group;
     case EPI_TUNE_XCD: return &tu.xcd;
     case EPI_TUNE_RD_GROUP: return &tu.rd_group;
+    case EPI_TUNE_CA_FUSE: return &tu.ca_fuse;
     default: return nullptr;
   }
 }
@@ -1000,6 +998,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_CA_GROUP: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_XCD: ok = value == 0 || value == 1; break;
     case EPI_TUNE_RD_GROUP: ok = value == 8 || value == 16; break;
+    case EPI_TUNE_CA_FUSE: ok = value == 0 || value == 1; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

@@ -85,7 +85,6 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
 // SWEEP_H partners sweeps its column a second time.
 constexpr int SWEEP_G = 32;
 constexpr int SWEEP_H = 8;
-constexpr int SWEEP_U = 4;
 
 __device__ __forceinline__ bool sweep_hit(const Tis& t, int pb, const P3& pc, double rc, int o) {
   const P3 po = ld_pos(t, pb, o);
@@ -123,41 +122,19 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
     const long bxy = (aix * ANY + aiy) * ANZ;
     int cnt = 0;
     if (act) {
-      // the column's 5 buckets as one sequence in sweep order, SWEEP_U
-      // candidates at a time with all their loads in flight; a bucket ends
-      // at its first candidate >= c (buckets hold ascending indices)
-      int sz[5], bk[5];
+      int sz[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) sz[i] = min(t.bucket_n[bxy + (anz - 2 + i + ANZ) % ANZ], t.bcap);
 #pragma unroll
       for (int i = 0; i < 5; i++) {
-        bk[i] = (int)(bxy + (anz - 2 + i + ANZ) % ANZ);
-        sz[i] = min(t.bucket_n[bk[i]], t.bcap);
-      }
-      int i = 0, m = 0;   // next candidate: bucket i, slot m
-      while (i < 5) {
-        int o[SWEEP_U];
-        bool live[SWEEP_U];
-        int ii = i, mm = m;
-#pragma unroll
-        for (int u = 0; u < SWEEP_U; u++) {
-          while (ii < 5 && mm >= sz[ii]) { ii++; mm = 0; }
-          live[u] = ii < 5;
-          o[u] = live[u] ? t.bucket_cells[(size_t)bk[ii] * t.bcap + mm] : 0;
-          mm++;
-        }
-        bool hit[SWEEP_U];
-#pragma unroll
-        for (int u = 0; u < SWEEP_U; u++) hit[u] = live[u] && o[u] < c && sweep_hit(t, pb, pc, rc, o[u]);
-        // consume in order: a candidate >= c closes its bucket
-#pragma unroll
-        for (int u = 0; u < SWEEP_U; u++) {
-          while (i < 5 && m >= sz[i]) { i++; m = 0; }
-          if (i >= 5) break;
-          if (c <= o[u]) { i++; m = 0; break; }
-          if (hit[u]) {
-            if (cnt < SWEEP_H) hits[cnt] = o[u];
+        const int* cells = t.bucket_cells + (size_t)(bxy + (anz - 2 + i + ANZ) % ANZ) * t.bcap;
+        for (int m = 0; m < sz[i]; ++m) {
+          const int o = cells[m];
+          if (c <= o) break;   // ascending within the bucket
+          if (sweep_hit(t, pb, pc, rc, o)) {
+            if (cnt < SWEEP_H) hits[cnt] = o;
             cnt++;
           }
-          m++;
         }
       }
     }

@@ -265,25 +265,64 @@ __device__ __forceinline__ unsigned xcd_order(unsigned L, unsigned G, int on) {
   return x * q + min(x, r) + (L >> 3);
 }
 
+// ATP_refresh's update of one point (ca2p.cpp:302-338) from its value, its
+// six neighbours (mirror z, periodic x and y) and their masks.  Every path
+// that advances an ATP point goes through this one expression.
+__device__ __forceinline__ double atp_new(double catp, double cxm, double cxp, double cym, double cyp, double czm,
+                                          double czp, double mxm, double mxp, double mym, double myp, double mzm,
+                                          double mzp, double diffu, double air, double inv_dx) {
+  const double Da = 1.0, AIR_STIM = 0.1, STIM11 = 0.002, Kaa = 0.5;
+  const double fa = STIM11 * min0(diffu) - catp * Kaa;
+  return catp + DT_Ca * (Da * (mxm * (cxm - catp) + mxp * (cxp - catp) + mym * (cym - catp) + myp * (cyp - catp)
+                               + mzm * (czm - catp) + mzp * (czp - catp))
+                         * inv_dx * inv_dx + fa + air * AIR_STIM);
+}
+
+// the new value of interior point (j, k, l), l < iz_bound, from buffer carr
+// and the sub-step's diff_u
+__device__ __forceinline__ double atp_point(const Tis& t, const double* __restrict__ carr,
+                                            const double* __restrict__ dfu, int j, int k, int l) {
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
+  const int py = k == 0 ? NY - 1 : k - 1, ny = k == NY - 1 ? 0 : k + 1;
+  const int pz = l == 0 ? 1 : l - 1, nz = l + 1 == NZ ? NZ - 1 : l + 1;
+  const long long v = gidx(t, j, k, l);
+  const long long vxm = gidx(t, px, k, l), vxp = gidx(t, nx, k, l);
+  const long long vym = gidx(t, j, py, l), vyp = gidx(t, j, ny, l);
+  const long long vzm = gidx(t, j, k, pz), vzp = gidx(t, j, k, nz);
+  const unsigned w = t.vox[v];
+  const unsigned cell1 = w & 0x7fffffffu;
+  const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
+  return atp_new(carr[v], carr[vxm], carr[vxp], carr[vym], carr[vyp], carr[vzm], carr[vzp], (double)cm2[vxm],
+                 (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], (double)cm2[vzm], (double)cm2[vzp], diffu,
+                 (double)(w >> 31), t.inv_dx);
+}
+
+// diff_u of a sub-step by Ca parity: the sub-step at parity cp writes it and
+// its ATP update reads it, so a fused launch (ATP of one sub-step beside the
+// cells of the next) reads one buffer while it writes the other
+__device__ __forceinline__ double* dfu_buf(const Tis& t, int cp) { return cp ? t.ca_dfu : t.diff_u; }
+
 // dead_IP3_calc (:156-173) and supra_calc (:182-222).  A group of CG lanes
 // (4, 8 or 16, set by the autotuner) owns one cell: lanes evaluate the
 // gap-junction terms of the list entries (and update gj toward ALIVE
 // neighbours) 4 * CG entries at a time and load the 16 grid corners of the two
 // grid_avg8; the leader sums them in list / corner order.  Every lane issues
-// all loads of its entries of a chunk (list, kind, gj, then the neighbours'
-// Ca2+ and IP3) before it uses any of them.
+// all loads of its entries of a chunk (list word, gj, then the neighbours'
+// Ca2+ and IP3 pair) before it uses any of them.  With carr_prev set, the ATP
+// corners are not read but computed from the previous sub-step's buffer
+// (atp_point), the ATP update of that sub-step running beside this one.
 constexpr int C_BLOCK = 128;
 
-template <int CG>
-__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
-                                                      int ap, int bp, int xcd) {
-  constexpr int C_GROUPS = C_BLOCK / CG;
+template <int CG, int NT>
+__device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigned nbx, int cb, int cp, int ap, int bp,
+                                              const double* __restrict__ carr_prev,
+                                              const double* __restrict__ dfu_prev) {
+  constexpr int C_GROUPS = NT / CG;
   constexpr int EPL = 4;            // entries per lane per chunk
-  constexpr int CCAP = EPL * CG;    // list entries per chunk; LDS per block is the same for every CG
+  constexpr int CCAP = EPL * CG;    // list entries per chunk
   constexpr int NQ = 16 / CG;       // grid corners per lane
-  const unsigned lid = xcd_order(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, xcd);
-  const unsigned bx = lid % gridDim.x, by = lid / gridDim.x;
-  const Tis& t = T[tlist ? tlist[by] : by];
   const int n = t.st->n;
   __shared__ double s_t1[C_GROUPS][CCAP];
   __shared__ double s_t2[C_GROUPS][CCAP];
@@ -293,7 +332,10 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
   const int grp = threadIdx.x / CG;
   const double2* __restrict__ up = t.ca_up[cp];
   double2* __restrict__ up_next = t.ca_up[1 - cp];
-  const int stride = gridDim.x * C_GROUPS;
+  double* __restrict__ dfu_out = dfu_buf(t, cp);
+  const int izb = carr_prev ? ca_iz_bound(t) : 0;
+  const bool recompute = carr_prev && izb <= t.NZ;
+  const int stride = nbx * C_GROUPS;
   for (int c = t.nmemb + t.nder + bx * C_GROUPS + grp; c < n; c += stride) {
     const uint8_t s = t.state[c];
     const int len = t.nlen[cb][c];
@@ -351,7 +393,18 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
         const int ix = lx + ((k == 1 || k == 4 || k == 5 || k == 7) ? 1 : 0);
         const int iy = ly + ((k == 2 || k == 4 || k == 6 || k == 7) ? 1 : 0);
         const int iz = lz + ((k == 3 || k == 5 || k == 6 || k == 7) ? 1 : 0);
-        cq[j] = (q < 8 ? t.bf[bp] : t.atp[ap])[gidx(t, ix, iy, iz)];
+        if (q < 8) {
+          cq[j] = t.bf[bp][gidx(t, ix, iy, iz)];
+        } else if (recompute && iz < izb) {
+          // the periodic copies x = NX, y = NY hold the values of x = 0, y = 0
+          cq[j] = atp_point(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz);
+        } else if (recompute) {
+          // plane iz_bound is never updated inside a solve; its copies are
+          // written beside this launch, so read the value they copy
+          cq[j] = t.atp[ap][gidx(t, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz)];
+        } else {
+          cq[j] = t.atp[ap][gidx(t, ix, iy, iz)];
+        }
       }
     }
     double* g = t.gj[cb] + (size_t)c * t.ns;
@@ -401,7 +454,7 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
       const double Bavg = 0.125 * (q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7]);
       const double Aavg = 0.125 * (q[8] + q[9] + q[10] + q[11] + q[12] + q[13] + q[14] + q[15]);
       const double diff_u = ca2p_reaction_factor(u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
-      t.diff_u[c] = diff_u;
+      dfu_out[c] = diff_u;
       const double H0 = 0.5;
       const double np = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + t.ca_dp * IAGv * tmp_IP3);
       t.ip3[1 - cp][c] = np;
@@ -416,15 +469,19 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
   }
 }
 
+template <int CG>
+__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
+                                                      int ap, int bp, int xcd) {
+  const unsigned lid = xcd_order(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, xcd);
+  const unsigned bx = lid % gridDim.x, by = lid / gridDim.x;
+  const Tis& t = T[tlist ? tlist[by] : by];
+  ca_cells_part<CG, C_BLOCK>(t, bx, gridDim.x, cb, cp, ap, bp, nullptr, nullptr);
+}
+
 // ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound.
 // Threads cover [0, NX) x [0, NY); the z column rolls through registers.
 template <int ZB>
-__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int nky, int nzc,
-                                             int xcd) {
-  const unsigned gxy = gridDim.x * gridDim.y;
-  const unsigned lid = xcd_order((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, gxy * gridDim.z, xcd);
-  const unsigned bx = lid % gridDim.x, by = (lid / gridDim.x) % gridDim.y, bz = lid / gxy;
-  const Tis& t = T[tlist ? tlist[by] : by];
+__device__ __forceinline__ void atp_part(const Tis& t, unsigned bx, unsigned bz, int nky, int nzc, int ap, int cp) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
   const int rows = blockDim.x >> 6;
   const int j = bx * 64 + (threadIdx.x & 63);
@@ -435,7 +492,7 @@ __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const in
   if (j >= NX || k >= NY) return;
   const double* __restrict__ carr = t.atp[ap];
   double* __restrict__ narr = t.atp[1 - ap];
-  const double Da = 1.0, AIR_STIM = 0.1, STIM11 = 0.002, Kaa = 0.5;
+  const double* __restrict__ dfu = dfu_buf(t, cp);
   const double inv_dx = t.inv_dx;
   const uint8_t* __restrict__ cm2 = t.cmap2;
   const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
@@ -454,21 +511,14 @@ __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const in
       const long long vzp = cxy + next_z * t.PL;
       const double czp = carr[vzp];
       const double mzp = (double)cm2[vzp];
-      const double catp = cc;
       const unsigned w = t.vox[v];
       const unsigned cell1 = w & 0x7fffffffu;
-      const double diffu = cell1 ? t.diff_u[cell1 - 1] : 0.0;
-      const double air = (double)(w >> 31);
+      const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
       const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
       const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
-      const double fa = STIM11 * min0(diffu) - catp * Kaa;
-      const double nv = catp + DT_Ca * (Da * ((double)cm2[vxm] * (carr[vxm] - catp)
-                                              + (double)cm2[vxp] * (carr[vxp] - catp)
-                                              + (double)cm2[vym] * (carr[vym] - catp)
-                                              + (double)cm2[vyp] * (carr[vyp] - catp)
-                                              + mzm * (czm - catp)
-                                              + mzp * (czp - catp))
-                                        * inv_dx * inv_dx + fa + air * AIR_STIM);
+      const double nv = atp_new(cc, carr[vxm], carr[vxp], carr[vym], carr[vyp], czm, czp, (double)cm2[vxm],
+                                (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], mzm, mzp, diffu,
+                                (double)(w >> 31), inv_dx);
       narr[v] = nv;
       if (k == 0) narr[gidx(t, j, NY, l)] = nv;
       if (j == 0) narr[gidx(t, NX, k, l)] = nv;
@@ -483,6 +533,33 @@ __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const in
     if (k == 0) narr[gidx(t, j, NY, izb)] = narr[gidx(t, j, 0, izb)];
     if (j == 0) narr[gidx(t, NX, k, izb)] = narr[gidx(t, 0, k, izb)];
     if (j == 0 && k == 0) narr[gidx(t, NX, NY, izb)] = narr[gidx(t, 0, 0, izb)];
+  }
+}
+
+template <int ZB>
+__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int cp, int nky,
+                                             int nzc, int xcd) {
+  const unsigned gxy = gridDim.x * gridDim.y;
+  const unsigned lid = xcd_order((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, gxy * gridDim.z, xcd);
+  const unsigned bx = lid % gridDim.x, by = (lid / gridDim.x) % gridDim.y, bz = lid / gxy;
+  const Tis& t = T[tlist ? tlist[by] : by];
+  atp_part<ZB>(t, bx, bz, nky, nzc, ap, cp);
+}
+
+// one launch for the ATP update of sub-step i (parities cp, ap) and the cells
+// of sub-step i + 1: blocks [0, ncb) take the cells, the rest the grid
+// (gx x-blocks per z-block, 4 rows of 64 points per block)
+constexpr int F_BLOCK = 256;
+
+template <int CG, int ZB>
+__global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
+                                                      int ap, int bp, int ncb, int gx, int nky, int nzc) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  if ((int)blockIdx.x < ncb) {
+    ca_cells_part<CG, F_BLOCK>(t, blockIdx.x, ncb, cb, 1 - cp, 1 - ap, bp, t.atp[ap], dfu_buf(t, cp));
+  } else {
+    const unsigned b = blockIdx.x - ncb;
+    atp_part<ZB>(t, b % gx, b / gx, nky, nzc, ap, cp);
   }
 }
 
@@ -559,8 +636,9 @@ void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int nt
   EPI_LAUNCH(K_CA_END, k_ca_end, dim3((max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist, 0);
 }
 
-void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
+namespace {
+void launch_cells(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp, int max_n,
+                  const Tuning& tu) {
   const int cg = tu.ca_group;
   const dim3 gc((max_n + C_BLOCK / cg - 1) / (C_BLOCK / cg), ntis);
   if (cg == 4)
@@ -569,14 +647,66 @@ void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis,
     EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
   else
     EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
+}
+
+void launch_atp(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cp, int ap, int max_nx, int max_ny,
+                int izb_est, const Tuning& tu) {
   const int rows = tu.a_rows;
   const int nky = (max_ny + rows - 1) / rows;
   by_zb(tu.a_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
     const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
     dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
-    EPI_LAUNCH(K_ATP, k_atp<zb>, g, dim3(64 * rows), 0, s, dT, tlist, ap, nky, nzc, tu.xcd);
+    EPI_LAUNCH(K_ATP, k_atp<zb>, g, dim3(64 * rows), 0, s, dT, tlist, ap, cp, nky, nzc, tu.xcd);
   });
+}
+
+template <int CG>
+void launch_fused_cg(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
+                     int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
+  const int ncb = (max_n + F_BLOCK / CG - 1) / (F_BLOCK / CG);
+  const int gx = (max_nx + 63) / 64, nky = (max_ny + 3) / 4;
+  by_zb(tu.a_zb, [&](auto zbc) {
+    constexpr int zb = decltype(zbc)::value;
+    const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
+    const dim3 g((unsigned)(ncb + gx * nky * nzc), ntis);
+    const auto kfn = k_ca_fused<CG, zb>;
+    EPI_LAUNCH(K_CA_FUSED, kfn, g, dim3(F_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, ncb, gx, nky, nzc);
+  });
+}
+}  // namespace
+
+void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
+                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
+  launch_cells(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, tu);
+  launch_atp(s, dT, tlist, ntis, cp, ap, max_nx, max_ny, izb_est, tu);
+}
+
+void launch_ca_substeps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int& cp, int& ap, int bp,
+                        int n_sub, int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
+  if (!tu.ca_fuse || n_sub < 2) {
+    for (int i = 0; i < n_sub; i++) {
+      launch_ca_substep(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+      cp ^= 1;
+      ap ^= 1;
+    }
+    return;
+  }
+  // cells of sub-step 0; then (ATP of i, cells of i + 1); then ATP of the last
+  launch_cells(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, tu);
+  for (int i = 0; i + 1 < n_sub; i++) {
+    if (tu.ca_group == 4)
+      launch_fused_cg<4>(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+    else if (tu.ca_group == 8)
+      launch_fused_cg<8>(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+    else
+      launch_fused_cg<16>(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+    cp ^= 1;
+    ap ^= 1;
+  }
+  launch_atp(s, dT, tlist, ntis, cp, ap, max_nx, max_ny, izb_est, tu);
+  cp ^= 1;
+  ap ^= 1;
 }
 
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n) {

@@ -13,7 +13,8 @@ enum KernelId {
   K_FINISH, K_SNAPSHOT, K_ZZ_SET, K_ZZMAX, K_MAP_CLEAR, K_MAP_STAMP, K_MAP_HALO,
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
   K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
-  K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB, K_COUNT
+  K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB,
+  K_CA_FUSED, K_COUNT
 };
 
 // Launch shapes that the autotuner may change (values checked by the host).
@@ -22,7 +23,8 @@ struct Tuning {
   int a_zb = 4, a_rows = 4;   // k_atp: the same
   int ca_group = 16;          // k_ca_cells: lanes per cell
   int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
-  int rd_group = 8;           // k_rd_cells: lanes per cell
+  int rd_group = 8;           // k_rd_step: lanes per cell
+  int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
 };
 
 // Event timer: when the bit of a kernel id is set, its launches carry a
@@ -59,6 +61,9 @@ void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx,
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
                        int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
+// n_sub sub-steps (fused when tu.ca_fuse), flipping cp and ap after each
+void launch_ca_substeps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int& cp, int& ap, int bp,
+                        int n_sub, int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
 void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);

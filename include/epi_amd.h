@@ -250,11 +250,14 @@ int epi_rd_end(epi_ctx* ctx);
  * cell in the Ca cell kernel), XCD in {0, 1} (1: the Ca cell and ATP kernels
  * number their workgroups so that each of the 8 XCDs runs one contiguous run
  * of cells / grid rows), RD_GROUP in {8, 16} (lanes per cell in the reverse
- * cell kernel of reaction-diffusion windows).  EPI_ERR_INVALID for other keys
- * or values. */
+ * cell kernel of reaction-diffusion windows), CA_FUSE in {0, 1} (1: a Ca solve
+ * launches the ATP update of a sub-step and the cells of the next one
+ * together, the cells computing their ATP corners themselves).
+ * EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
-    EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_COUNT = 7
+    EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
+    EPI_TUNE_COUNT = 8
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);
