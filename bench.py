@@ -391,7 +391,7 @@ def run_ca_solve(args):
     t_wall, _ = _timed(ts, lambda: [ts.calc_ca2p(zz) for _ in range(solves)])
     ts.profile_reset()
     ts.profile_sample(1)
-    ts.profile_enable(["k_ca_cells", "k_atp"])
+    ts.profile_enable(["k_ca_cells", "k_atp", "k_ca_fused"])
     ts.calc_ca2p(zz)
     ts.synchronize()
     prof = ts.profile()
