@@ -7,6 +7,7 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
   k_ca_cells   lanes per cell (4, 8, 16)
   k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
   Ca solve     ATP of a sub-step and cells of the next in one launch (0, 1)
+  k_map_stamp  lanes per cell (1, 4)
 
 A launch shape changes timing only: every grid point is computed on its own
 and the Ca cell kernel sums each list in list order whatever its group size,
@@ -22,7 +23,7 @@ import time
 
 STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
-DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0}
+DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0, "map_lanes": 1}
 
 
 def _avg_us(ts, kernel):
@@ -56,8 +57,18 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
     if warm_steps:
         ts.step(drivers, warm_steps)
     zz = ts.calc_zzmax()
-    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}, "xcd": {}, "ca_fuse": {}}
+    report = {"k_ext_stim": {}, "k_atp": {}, "k_ca_cells": {}, "xcd": {}, "ca_fuse": {}, "k_map_stamp": {}}
     best = dict(DEFAULT)
+    ts.set_tuning(best)
+
+    def run_map():
+        for _ in range(b_reps):
+            ts.setup_map_lat()
+
+    for ml in (1, 4):
+        ts.set_tuning({"map_lanes": ml})
+        report["k_map_stamp"][f"lanes{ml}"] = _time(ts, "k_map_stamp", run_map)
+    best["map_lanes"] = int(min(report["k_map_stamp"], key=report["k_map_stamp"].get)[5:])
     ts.set_tuning(best)
 
     def run_b():

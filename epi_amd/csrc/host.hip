@@ -432,7 +432,9 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
     h.g = g;
     h.nmemb = g.nmx * g.nmy;
     h.cap = g.max_cells;
-    h.ns = g.row_stride > 0 ? std::min(g.row_stride, MAX_CONN) : MAX_CONN;
+    // a row holds at least 64 slots: the Ca cell kernel reads its first list
+    // chunk (up to 64 entries) before it knows the length
+    h.ns = g.row_stride > 0 ? std::max(64, std::min(g.row_stride, MAX_CONN)) : MAX_CONN;
     h.bcap = g.bucket_cap > 0 ? std::min(g.bucket_cap, N3) : N3;
     h.NX = g.nx; h.NY = g.ny; h.NZ = g.nz;
     h.XP = (int)align_up(g.nx + 1, 16);
@@ -763,7 +765,7 @@ int epi_calc_zzmax(epi_ctx* c, double* zzmax) {
 
 int epi_setup_map_lat(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
-  launch_maps(c->stream, c->d_tis, c->T, c->pb, max_n(c), max_gsize(c), max_halo(c));
+  launch_maps(c->stream, c->d_tis, c->T, c->pb, max_n(c), max_gsize(c), max_halo(c), c->tu);
   HIPCHK(c, hipGetLastError());
   return EPI_OK;
 }
@@ -852,7 +854,7 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, mn, max_nb(c), 1);
     c->cb = nb;
     // proc.cpp:155-157
-    launch_maps(c->stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c));
+    launch_maps(c->stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
     launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
     c->bp ^= 1;
     HIPCHK(c, hipGetLastError());
@@ -982,6 +984,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_XCD: return &tu.xcd;
     case EPI_TUNE_RD_GROUP: return &tu.rd_group;
     case EPI_TUNE_CA_FUSE: return &tu.ca_fuse;
+    case EPI_TUNE_MAP_LANES: return &tu.map_lanes;
     default: return nullptr;
   }
 }
@@ -999,6 +1002,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_XCD: ok = value == 0 || value == 1; break;
     case EPI_TUNE_RD_GROUP: ok = value == 8 || value == 16; break;
     case EPI_TUNE_CA_FUSE: ok = value == 0 || value == 1; break;
+    case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

@@ -20,14 +20,19 @@ namespace epi {
 // ---- voxel maps ---------------------------------------------------------
 // Only planes 0..stamp_z can hold stamps (stamp_z never decreases), so only
 // they are cleared; the rest of the maps stays empty from the first build.
+// Four voxels per thread: a plane holds PL = XP (NY + 1) voxels with XP a
+// multiple of 16, and the maps start 256-byte aligned, so every quad is one
+// 16-byte and one 4-byte store.
 __global__ void __launch_bounds__(256) k_map_clear(const Tis* __restrict__ T) {
   const Tis& t = T[blockIdx.y];
   const int planes = min(t.st->stamp_z, t.NZ) + 1;
-  const long long g = (long long)planes * t.PL;
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < g;
-       v += (long long)gridDim.x * blockDim.x) {
-    t.cmap1[v] = -1;
-    t.cmap2[v] = 0;
+  const long long g4 = (long long)planes * t.PL / 4;
+  int4* __restrict__ m1 = reinterpret_cast<int4*>(t.cmap1);
+  unsigned* __restrict__ m2 = reinterpret_cast<unsigned*>(t.cmap2);
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g4;
+       q += (long long)gridDim.x * blockDim.x) {
+    m1[q] = make_int4(-1, -1, -1, -1);
+    m2[q] = 0u;
   }
 }
 
@@ -41,17 +46,22 @@ __device__ __forceinline__ int lat_wrap(long v, int N, Status* st) {
 // set_lattice (:18-22) and the sphere stamping (:39-155); the last writer of
 // the serial loop is the highest cell index, hence atomicMax on cmap1.  One
 // lane per cell.
+// MS lanes per cell (1 or 4, a launch shape) split the stamp box by z plane:
+// the lanes of a cell then touch voxels a plane apart, never one cache line.
+// The result does not depend on MS (a max and idempotent stores).
+template <int MS>
 __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, int pb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
   const int n = st->n;
+  const int zl = threadIdx.x & (MS - 1);
   int ztop = 0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+  for (int c = (blockIdx.x * blockDim.x + threadIdx.x) / MS; c < n; c += gridDim.x * blockDim.x / MS) {
     const P3 p = ld_pos(t, pb, c);
     const int lx = lat_wrap((long)(int)(p.x * t.inv_dx), t.NX, st);
     const int ly = lat_wrap((long)(int)(p.y * t.inv_dy), t.NY, st);
     const int lz = lat_wrap((long)(int)(p.z * t.inv_dz), t.NZ, st);
-    t.lat[3 * c] = lx; t.lat[3 * c + 1] = ly; t.lat[3 * c + 2] = lz;
+    if (zl == 0) { t.lat[3 * c] = lx; t.lat[3 * c + 1] = ly; t.lat[3 * c + 2] = lz; }
     const uint8_t s = t.state[c];
     const bool afm = is_afm(s);
     const int irx = afm ? 2 * t.irx : t.irx, iry = afm ? 2 * t.iry : t.iry, irz = afm ? 2 * t.irz : t.irz;
@@ -73,7 +83,7 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
         const int ipy = lat_wrap(l, t.NY, st);
         const double diffy = p_diff_y(t, my, p.y);
         const double tmps = diffxSq + diffy * diffy;
-        for (int m = zmin; m <= zmax; m++) {
+        for (int m = zmin + zl; m <= zmax; m += MS) {
           const double mz = m * t.dz;
           const double diffz = mz - p.z;
           const double distSq = tmps + diffz * diffz;
@@ -341,9 +351,19 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
     const int len = t.nlen[cb][c];
     const double2 uc = up[c];
     const double u = uc.x, p = uc.y;
+    // the first chunk's list words and gj do not wait for len: a row holds ns
+    // >= CCAP slots, and slots past len are masked once len has arrived
+    const int* ent = t.ca_ent + (size_t)c * t.ns;
+    double* g = t.gj[cb] + (size_t)c * t.ns;
+    int en0[EPL];
+    double gv0[EPL];
+#pragma unroll
+    for (int i = 0; i < EPL; i++) {
+      en0[i] = ent[lane + i * CG];
+      gv0[i] = g[lane + i * CG];
+    }
     const bool dead = s == DEAD;
     if (!dead && !is_afm(s)) continue;
-    const int* ent = t.ca_ent + (size_t)c * t.ns;
     if (dead) {
       int count = 0;
       double tmp = 0;
@@ -354,7 +374,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
 #pragma unroll
         for (int i = 0; i < EPL; i++) {
           const int e = lane + i * CG;
-          en[i] = e < m ? ent[e0 + e] : 0;
+          en[i] = e < m ? (e0 == 0 ? en0[i] : ent[e0 + e]) : 0;
         }
 #pragma unroll
         for (int i = 0; i < EPL; i++) po[i] = (en[i] & 3) == 3 ? up[en[i] >> 2].y : 0.0;
@@ -407,7 +427,6 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
         }
       }
     }
-    double* g = t.gj[cb] + (size_t)c * t.ns;
     double tmp_diffu = 0, tmp_IP3 = 0;
     for (int e0 = 0; e0 < len; e0 += CCAP) {
       const int m = min(CCAP, len - e0);
@@ -418,8 +437,8 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
       for (int i = 0; i < EPL; i++) {
         const int e = lane + i * CG;
         const bool in = e < m;
-        en[i] = in ? ent[e0 + e] : 0;
-        gv[i] = in ? g[e0 + e] : 0.0;
+        en[i] = in ? (e0 == 0 ? en0[i] : ent[e0 + e]) : 0;
+        gv[i] = in ? (e0 == 0 ? gv0[i] : g[e0 + e]) : 0.0;
       }
 #pragma unroll
       for (int i = 0; i < EPL; i++) w[i] = (en[i] & 3) ? up[en[i] >> 2] : make_double2(0.0, 0.0);
@@ -584,10 +603,15 @@ __global__ void k_sw_reset(const Tis* __restrict__ T, const int* tlist) {
   if (threadIdx.x == 0) t.st->sw = 0;
 }
 
-void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo) {
-  dim3 gg((unsigned)((max_gsize + 255) / 256 < 65535 ? (max_gsize + 255) / 256 : 65535), ntis);
+void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo,
+                 const Tuning& tu) {
+  const long long q = (max_gsize / 4 + 255) / 256;
+  dim3 gg((unsigned)(q < 1 ? 1 : q < 65535 ? q : 65535), ntis);
   EPI_LAUNCH(K_MAP_CLEAR, k_map_clear, gg, dim3(256), 0, s, dT);
-  EPI_LAUNCH(K_MAP_STAMP, k_map_stamp, dim3((max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
+  if (tu.map_lanes == 4)
+    EPI_LAUNCH(K_MAP_STAMP, k_map_stamp<4>, dim3((4 * max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
+  else
+    EPI_LAUNCH(K_MAP_STAMP, k_map_stamp<1>, dim3((max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
   EPI_LAUNCH(K_MAP_HALO, k_map_halo, dim3((max_halo + 255) / 256, ntis), dim3(256), 0, s, dT);
 }
 

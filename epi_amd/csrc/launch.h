@@ -25,6 +25,7 @@ struct Tuning {
   int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
   int rd_group = 8;           // k_rd_step: lanes per cell
   int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
+  int map_lanes = 1;          // k_map_stamp: lanes per cell (z planes split between them)
 };
 
 // Event timer: when the bit of a kernel id is set, its launches carry a
@@ -55,7 +56,8 @@ void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max
 void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix);
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v);
-void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo);
+void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo,
+                 const Tuning& tu);
 void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est,
                      const Tuning& tu);
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);

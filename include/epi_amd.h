@@ -41,8 +41,9 @@ enum { EPI_ALIVE = 0, EPI_DEAD = 1, EPI_DISA = 2, EPI_UNUSED = 3, EPI_FIX = 4,
 typedef struct epi_ctx epi_ctx;
 
 /* Runtime form of define.h:72-113 (NMX, NMY_tri, NX/NY/NZ, LX/LY/LZ,
- * MAX_CELL_NUM).  row_stride caps a neighbour list (<= 400, the reference's
- * MAX_CONNECT_CELL_NUM); bucket_cap caps a bucket (<= 200, N3). 0 = default. */
+ * MAX_CELL_NUM).  row_stride caps a neighbour list (64 to 400, the
+ * reference's MAX_CONNECT_CELL_NUM; smaller values are raised to 64);
+ * bucket_cap caps a bucket (<= 200, N3). 0 = default. */
 typedef struct epi_geometry {
     int32_t nmx, nmy;
     int32_t nx, ny, nz;
@@ -252,12 +253,13 @@ int epi_rd_end(epi_ctx* ctx);
  * of cells / grid rows), RD_GROUP in {8, 16} (lanes per cell in the reverse
  * cell kernel of reaction-diffusion windows), CA_FUSE in {0, 1} (1: a Ca solve
  * launches the ATP update of a sub-step and the cells of the next one
- * together, the cells computing their ATP corners themselves).
+ * together, the cells computing their ATP corners themselves), MAP_LANES in
+ * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes).
  * EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
     EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
-    EPI_TUNE_COUNT = 8
+    EPI_TUNE_MAP_LANES = 8, EPI_TUNE_COUNT = 9
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);
