@@ -6,7 +6,8 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
   k_atp        the same
   k_ca_cells   lanes per cell (4, 8, 16)
   k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
-  Ca solve     ATP of a sub-step and cells of the next in one launch (0, 1)
+  Ca solve     ATP of a sub-step and cells of the next in one launch (1; 2 =
+               only solves of a few tissues, when the whole set runs faster unfused)
   k_map_stamp  lanes per cell (1, 4)
 
 A launch shape changes timing only: every grid point is computed on its own
@@ -117,6 +118,10 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
     if key != "unfused":
         best["ca_fuse"] = 1
         best["atp_zb"] = int(key[8:])
+    else:
+        # the whole set solves faster unfused; a gated solve of a few
+        # tissues is latency-bound and still fuses (ca_fuse 2)
+        best["ca_fuse"] = 2
     ts.set_tuning(best)
     report["seconds"] = time.perf_counter() - t0
     if log is not None:

@@ -366,14 +366,33 @@ int set_zz(epi_ctx* c, const double* zzmax) {
 }
 
 // calc_ca2p (ca2p.cpp:380-402) for the tissues in tlist (nullptr = all)
+// (with d_tlist, c->h_tlist holds the same ntis ids).  Launch grids are
+// sized by the solving tissues only: a gated solve of one small tissue in a
+// large batch launches that tissue's blocks, not the batch's largest.
 int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
-  const int mn = max_n(c);
-  const int izb = izb_est(c, true);
+  int mn = 1, mx = 1, my = 1, izb = 1;
   long long box = 1;
-  for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * std::min(izb, h.NZ + 1));
+  for (int i = 0; i < ntis; i++) {
+    const int t = d_tlist ? c->h_tlist[i] : i;
+    const TH& h = c->th[t];
+    const double zz = c->h_st[t].zzmax;
+    const int iz = std::min((int)((zz + FAC_MAP * R_max) / c->tis[t].dz) + 2, h.NZ + 1);
+    mn = std::max(mn, h.n_est + 64);
+    mx = std::max(mx, h.NX);
+    my = std::max(my, h.NY);
+    izb = std::max(izb, iz);
+    box = std::max(box, (long long)h.NX * h.NY * iz);
+  }
+  // ca_fuse 2: fuse the solves of at most FUSE_AUTO_CELLS cells (a gated solve
+  // of a few tissues is latency-bound, where one launch per sub-step wins; a
+  // solve of a whole large batch is throughput-bound, where two leaner ones do)
+  constexpr long long FUSE_AUTO_CELLS = 131072;
+  long long cells = 0;
+  for (int i = 0; i < ntis; i++) cells += c->th[d_tlist ? c->h_tlist[i] : i].n_est;
+  Tuning tu = c->tu;
+  tu.ca_fuse = tu.ca_fuse == 1 || (tu.ca_fuse == 2 && cells <= FUSE_AUTO_CELLS);
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
-  launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mn, max_nx(c), max_ny(c),
-                     izb, c->tu);
+  launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mn, mx, my, izb, tu);
   launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
   HIPCHK(c, hipGetLastError());
   return EPI_OK;
@@ -1001,7 +1020,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_CA_GROUP: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_XCD: ok = value == 0 || value == 1; break;
     case EPI_TUNE_RD_GROUP: ok = value == 8 || value == 16; break;
-    case EPI_TUNE_CA_FUSE: ok = value == 0 || value == 1; break;
+    case EPI_TUNE_CA_FUSE: ok = value >= 0 && value <= 2; break;
     case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");

@@ -251,9 +251,10 @@ int epi_rd_end(epi_ctx* ctx);
  * cell in the Ca cell kernel), XCD in {0, 1} (1: the Ca cell and ATP kernels
  * number their workgroups so that each of the 8 XCDs runs one contiguous run
  * of cells / grid rows), RD_GROUP in {8, 16} (lanes per cell in the reverse
- * cell kernel of reaction-diffusion windows), CA_FUSE in {0, 1} (1: a Ca solve
- * launches the ATP update of a sub-step and the cells of the next one
- * together, the cells computing their ATP corners themselves), MAP_LANES in
+ * cell kernel of reaction-diffusion windows), CA_FUSE in {0, 1, 2} (1: a Ca
+ * solve launches the ATP update of a sub-step and the cells of the next one
+ * together, the cells computing their ATP corners themselves; 2: only for
+ * solves of at most 131072 cells in total), MAP_LANES in
  * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes).
  * EPI_ERR_INVALID for other keys or values. */
 enum {
