@@ -27,3 +27,16 @@ def oracle_mod():
 def deck_spec():
     from epi_amd import deck
     return deck.load_npz(os.path.join(GOLDEN, "deck_2layer_nfix16.npz"))
+
+
+# GPU tests run from the most basic to the most composite: single-tissue
+# parity first, then the windows and their adjoints, then batches.  Within a
+# module the file order is kept.
+_GPU_ORDER = ("test_gpu_parity", "test_gpu_rd_adjoint", "test_gpu_mech_adjoint", "test_gpu_batched")
+
+
+def pytest_collection_modifyitems(config, items):
+    def rank(item):
+        mod = item.module.__name__.rsplit(".", 1)[-1] if item.module else ""
+        return _GPU_ORDER.index(mod) if mod in _GPU_ORDER else -1
+    items.sort(key=rank)   # stable: non-GPU modules keep their places first
