@@ -362,6 +362,14 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
       en0[i] = ent[lane + i * CG];
       gv0[i] = g[lane + i * CG];
     }
+    // the lattice position and the leader's per-cell values are fetched now
+    // too, so the corner gathers start one round trip earlier
+    const int lx = t.lat[3 * c], ly = t.lat[3 * c + 1], lz = t.lat[3 * c + 2];
+    double exi = 0, avg = 0, _th = 0, _Kpa = 0, IAGv = 0;
+    if (lane == 0) {
+      exi = t.ex_inert[c]; avg = t.ca2p_avg[c];
+      _th = t.ca_th[c]; _Kpa = t.ca_kpa[c]; IAGv = t.ca_iag[c];
+    }
     const bool dead = s == DEAD;
     if (!dead && !is_afm(s)) continue;
     if (dead) {
@@ -384,9 +392,20 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           if (e < m) { s_f[grp][e] = (en[i] & 3) == 3; s_t1[grp][e] = po[i]; }
         }
         wave_sync();
-        if (lane == 0)
-          for (int e = 0; e < m; e++)
+        if (lane == 0) {
+          int e = 0;
+          for (; e + 4 <= m; e += 4) {
+            uint8_t f[4];
+            double a[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) { f[j] = s_f[grp][e + j]; a[j] = s_t1[grp][e + j]; }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+              if (f[j]) { tmp += a[j]; count++; }
+          }
+          for (; e < m; e++)
             if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
+        }
         wave_sync();
       }
       if (lane == 0) {
@@ -396,16 +415,10 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
       }
       continue;
     }
-    // the leader's per-cell values and the corners of grid_avg8 (utils.h:18-23;
-    // slots 0-7 B, 8-15 ATP) are fetched now, beside the list loads
-    double exi = 0, avg = 0, _th = 0, _Kpa = 0, IAGv = 0;
-    if (lane == 0) {
-      exi = t.ex_inert[c]; avg = t.ca2p_avg[c];
-      _th = t.ca_th[c]; _Kpa = t.ca_kpa[c]; IAGv = t.ca_iag[c];
-    }
+    // the corners of grid_avg8 (utils.h:18-23; slots 0-7 B, 8-15 ATP) are
+    // fetched now, beside the list loads
     double cq[NQ];
     {
-      const int lx = t.lat[3 * c], ly = t.lat[3 * c + 1], lz = t.lat[3 * c + 2];
 #pragma unroll
       for (int j = 0; j < NQ; j++) {
         const int q = lane + j * CG;
@@ -459,9 +472,22 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
         }
       }
       wave_sync();
-      if (lane == 0)
-        for (int e = 0; e < m; e++)
+      // the leader adds in list order; four entries' LDS reads are issued
+      // before their adds
+      if (lane == 0) {
+        int e = 0;
+        for (; e + 4 <= m; e += 4) {
+          uint8_t f[4];
+          double a[4], b[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) { f[j] = s_f[grp][e + j]; a[j] = s_t1[grp][e + j]; b[j] = s_t2[grp][e + j]; }
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (f[j]) { tmp_diffu += a[j]; tmp_IP3 += b[j]; }
+        }
+        for (; e < m; e++)
           if (s_f[grp][e]) { tmp_diffu += s_t1[grp][e]; tmp_IP3 += s_t2[grp][e]; }
+      }
       wave_sync();
     }
 #pragma unroll
