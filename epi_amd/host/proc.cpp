@@ -4,12 +4,15 @@
 //
 //   proc <deck> <steps> [--seed S] [--force-cornif] [--ca-every K]
 //        [--geometry nmx nmy nx ny nz lx ly lz max_cells] [--dump FILE]
-//        [--output FILE]
+//        [--output FILE] [--snapshots DIR [--cut N]]
 //
 // Defaults: define.h's geometry (NMX 100, NMY_tri 114, 100x100x200 grid on
 // 50x50x100), seed 1, the reference's Ca gating only.  --dump writes the
 // final n, then x, y, z, ca2p, agek as raw doubles; --output writes the
-// 17-column text of CellManager::output.
+// 17-column text of CellManager::output.  --snapshots writes that text to
+// DIR/<i / N> before every step i with i % N == 0 (output_cell_data,
+// proc.cpp:45-51, 139-141; N defaults to CUT = 1000): the numbered series the
+// reference's viewers read (epi_vis_lightweight/main.cpp:493-520).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -34,7 +37,8 @@ void cell_dynamics(CellManager& cellset) {   // proc.cpp:22-32
 int usage() {
   std::fprintf(stderr,
                "usage: proc <deck> <steps> [--seed S] [--force-cornif] [--ca-every K]\n"
-               "            [--geometry nmx nmy nx ny nz lx ly lz max_cells] [--dump FILE] [--output FILE]\n");
+               "            [--geometry nmx nmy nx ny nz lx ly lz max_cells] [--dump FILE] [--output FILE]\n"
+               "            [--snapshots DIR [--cut N]]\n");
   return 2;
 }
 
@@ -48,7 +52,8 @@ int main(int argc, char** argv) {
   unsigned long long seed = 1;
   bool force_cornif = false;
   long ca_every = 0;
-  std::string dump, output;
+  std::string dump, output, snapdir;
+  long cut = 1000;   // CUT (define.h)
   for (int i = 3; i < argc; i++) {
     const std::string a = argv[i];
     if (a == "--seed" && i + 1 < argc) seed = std::strtoull(argv[++i], nullptr, 10);
@@ -56,6 +61,8 @@ int main(int argc, char** argv) {
     else if (a == "--ca-every" && i + 1 < argc) ca_every = std::atol(argv[++i]);
     else if (a == "--dump" && i + 1 < argc) dump = argv[++i];
     else if (a == "--output" && i + 1 < argc) output = argv[++i];
+    else if (a == "--snapshots" && i + 1 < argc) snapdir = argv[++i];
+    else if (a == "--cut" && i + 1 < argc) cut = std::atol(argv[++i]);
     else if (a == "--geometry" && i + 9 < argc) {
       g.nmx = std::atoi(argv[++i]); g.nmy = std::atoi(argv[++i]);
       g.nx = std::atoi(argv[++i]); g.ny = std::atoi(argv[++i]); g.nz = std::atoi(argv[++i]);
@@ -85,6 +92,7 @@ int main(int argc, char** argv) {
         std::printf("loop:%ld elapsed[sec]:%lf\n", i, s);
         std::fflush(stdout);
       }
+      if (!snapdir.empty() && cut > 0 && i % cut == 0) cman.output(snapdir + "/" + std::to_string(i / cut));
       cell_dynamics(cman);
       zzmax = calc_zzmax(cman);
       setup_map_lat(cman, cell_map1, cell_map2);

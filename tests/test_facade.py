@@ -62,10 +62,23 @@ def test_driver_matches_python_driver_and_oracle(tmp_path, oracle_mod):
     deck.write_deck(spec0, p)
     steps, ca_every, seed = 40, 20, 11
     dump = tmp_path / "dump.bin"
+    snaps = tmp_path / "snaps"
+    snaps.mkdir()
     print(f"facade driver: tiny deck, {steps} steps, Ca every {ca_every}, seed {seed}", flush=True)
     r = subprocess.run([DRIVER, str(p), str(steps), "--seed", str(seed), "--ca-every", str(ca_every),
-                        "--dump", str(dump), *_geometry_args(g)], capture_output=True, text=True, timeout=120)
+                        "--dump", str(dump), "--snapshots", str(snaps), "--cut", "10", *_geometry_args(g)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    # the numbered series the reference's viewer reads: steps 0, 10, 20, 30,
+    # each line parsed by its sscanf layout (epi_vis_lightweight/main.cpp:521)
+    assert sorted(os.listdir(snaps)) == ["0", "1", "2", "3"]
+    rows = [ln.split() for ln in (snaps / "0").read_text().splitlines()]
+    assert rows and all(len(r_) == 17 for r_ in rows)
+    state = np.array([int(r_[1]) for r_ in rows])
+    xyz0 = np.array([[float(r_[6]), float(r_[7]), float(r_[8])] for r_ in rows])
+    assert np.array_equal(state, spec0["cells"]["state"])
+    # CellManager::output prints %1.14e (15 significant digits)
+    assert np.allclose(xyz0, np.stack([spec0["cells"][k] for k in ("x", "y", "z")], 1), rtol=1e-13, atol=0)
     raw = np.fromfile(dump, dtype=np.float64)
     n = int(raw[0])
     x, y, z, ca2p, agek = raw[1:].reshape(5, n)
