@@ -10,6 +10,9 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
                only solves of a few tissues, when the whole set runs faster unfused)
   k_map_stamp  lanes per cell (1, 4)
 
+With CA_FUSE 2 the fused small solves use CA_GROUP_SMALL lanes per cell
+(default 8, the best width on a latency-bound single-tissue solve).
+
 A launch shape changes timing only: every grid point is computed on its own
 and the Ca cell kernel sums each list in list order whatever its group size,
 so the tuner can time candidates on a throwaway copy of the workload (which

@@ -390,7 +390,13 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
   long long cells = 0;
   for (int i = 0; i < ntis; i++) cells += c->th[d_tlist ? c->h_tlist[i] : i].n_est;
   Tuning tu = c->tu;
-  tu.ca_fuse = tu.ca_fuse == 1 || (tu.ca_fuse == 2 && cells <= FUSE_AUTO_CELLS);
+  if (tu.ca_fuse == 2 && cells <= FUSE_AUTO_CELLS) {
+    // a small, latency-bound solve: fused, with its own lanes per cell
+    tu.ca_fuse = 1;
+    tu.ca_group = tu.ca_group_small;
+  } else {
+    tu.ca_fuse = tu.ca_fuse == 1;
+  }
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
   launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mn, mx, my, izb, tu);
   launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
@@ -1004,6 +1010,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_RD_GROUP: return &tu.rd_group;
     case EPI_TUNE_CA_FUSE: return &tu.ca_fuse;
     case EPI_TUNE_MAP_LANES: return &tu.map_lanes;
+    case EPI_TUNE_CA_GROUP_SMALL: return &tu.ca_group_small;
     default: return nullptr;
   }
 }
@@ -1022,6 +1029,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_RD_GROUP: ok = value == 8 || value == 16; break;
     case EPI_TUNE_CA_FUSE: ok = value >= 0 && value <= 2; break;
     case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4; break;
+    case EPI_TUNE_CA_GROUP_SMALL: ok = value == 4 || value == 8 || value == 16; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

@@ -26,6 +26,7 @@ struct Tuning {
   int rd_group = 8;           // k_rd_step: lanes per cell
   int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
                               // (0 never, 1 always, 2 for solves of at most 131072 cells)
+  int ca_group_small = 8;     // k_ca_cells lanes per cell in the solves that ca_fuse 2 fuses
   int map_lanes = 1;          // k_map_stamp: lanes per cell (z planes split between them)
 };
 

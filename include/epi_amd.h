@@ -255,12 +255,13 @@ int epi_rd_end(epi_ctx* ctx);
  * solve launches the ATP update of a sub-step and the cells of the next one
  * together, the cells computing their ATP corners themselves; 2: only for
  * solves of at most 131072 cells in total), MAP_LANES in
- * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes).
- * EPI_ERR_INVALID for other keys or values. */
+ * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes),
+ * CA_GROUP_SMALL in {4, 8, 16} (lanes per cell in the solves CA_FUSE 2
+ * fuses).  EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
     EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
-    EPI_TUNE_MAP_LANES = 8, EPI_TUNE_COUNT = 9
+    EPI_TUNE_MAP_LANES = 8, EPI_TUNE_CA_GROUP_SMALL = 9, EPI_TUNE_COUNT = 10
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);
