@@ -109,6 +109,10 @@ struct Tis {
   double2* ca_up[2];
   // diff_u of the Ca sub-steps at parity 1 (parity 0 writes diff_u itself)
   double* ca_dfu;
+  // per AFM cell and grid_avg8 ATP corner, during a solve: the corner's voxel
+  // word and the masks of its six stencil neighbours (bits 0-5; bit 31 set
+  // when a mask is not 0/1 and the corner reads the map instead)
+  uint2* ca_cq;
   long long* uid; long long* old_uid; int* src_row;
   // neighbour rows (two buffers: current and previous build)
   int* nlen[2]; int* col[2]; double* gj[2];

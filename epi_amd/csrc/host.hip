@@ -83,7 +83,7 @@ const char* kKernelNames[K_COUNT] = {
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
-    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused"};
+    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners"};
 
 void prof_resolve() {
   for (auto& r : g_prof.pending) {
@@ -213,6 +213,7 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.ca_ent = cv.take<int>(cap * ns);
   for (int b = 0; b < 2; b++) t.ca_up[b] = cv.take<double2>(cap);
   t.ca_dfu = cv.take<double>(cap);
+  t.ca_cq = cv.take<uint2>(cap * 8);
   t.uid = cv.take<long long>(cap);
   t.old_uid = cv.take<long long>(cap);
   t.src_row = cv.take<int>(cap);

@@ -314,6 +314,59 @@ __device__ __forceinline__ double atp_point(const Tis& t, const double* __restri
 // cells of the next) reads one buffer while it writes the other
 __device__ __forceinline__ double* dfu_buf(const Tis& t, int cp) { return cp ? t.ca_dfu : t.diff_u; }
 
+// The voxel map, the masks and iz_bound do not change during a solve, so
+// each AFM cell's eight ATP corners keep their voxel word and neighbour masks
+// for all of its sub-steps (used when the corners are computed, not read).
+// One thread per (cell, corner), after k_ca_vox.
+__global__ void __launch_bounds__(256) k_ca_corners(const Tis* __restrict__ T, const int* tlist) {
+  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+  const int n = t.st->n;
+  const int start = t.nmemb + t.nder;
+  const int izb = ca_iz_bound(t);
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < 8LL * n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i >> 3), k = (int)(i & 7);
+    if (c < start || !is_afm(t.state[c])) continue;
+    int j = t.lat[3 * c] + ((k == 1 || k == 4 || k == 5 || k == 7) ? 1 : 0);
+    int kk = t.lat[3 * c + 1] + ((k == 2 || k == 4 || k == 6 || k == 7) ? 1 : 0);
+    const int l = t.lat[3 * c + 2] + ((k == 3 || k == 5 || k == 6 || k == 7) ? 1 : 0);
+    if (j == NX) j = 0;
+    if (kk == NY) kk = 0;
+    uint2 pre = make_uint2(0u, 0x80000000u);
+    if (l < izb && izb <= NZ) {
+      const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
+      const int py = kk == 0 ? NY - 1 : kk - 1, ny = kk == NY - 1 ? 0 : kk + 1;
+      const int pz = l == 0 ? 1 : l - 1, nz = l + 1 == NZ ? NZ - 1 : l + 1;
+      const uint8_t m[6] = {t.cmap2[gidx(t, px, kk, l)], t.cmap2[gidx(t, nx, kk, l)], t.cmap2[gidx(t, j, py, l)],
+                            t.cmap2[gidx(t, j, ny, l)], t.cmap2[gidx(t, j, kk, pz)], t.cmap2[gidx(t, j, kk, nz)]};
+      unsigned bits = 0;
+      for (int b = 0; b < 6; b++) {
+        if (m[b] > 1) bits |= 0x80000000u;
+        bits |= (unsigned)(m[b] & 1) << b;
+      }
+      pre = make_uint2(t.vox[gidx(t, j, kk, l)], bits);
+    }
+    t.ca_cq[i] = pre;
+  }
+}
+
+// atp_point with the corner's voxel word and neighbour masks from k_ca_corners
+__device__ __forceinline__ double atp_point_pre(const Tis& t, const double* __restrict__ carr,
+                                                const double* __restrict__ dfu, int j, int k, int l, uint2 pre) {
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
+  const int py = k == 0 ? NY - 1 : k - 1, ny = k == NY - 1 ? 0 : k + 1;
+  const int pz = l == 0 ? 1 : l - 1, nz = l + 1 == NZ ? NZ - 1 : l + 1;
+  const unsigned cell1 = pre.x & 0x7fffffffu;
+  const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
+  const unsigned b = pre.y;
+  return atp_new(carr[gidx(t, j, k, l)], carr[gidx(t, px, k, l)], carr[gidx(t, nx, k, l)], carr[gidx(t, j, py, l)],
+                 carr[gidx(t, j, ny, l)], carr[gidx(t, j, k, pz)], carr[gidx(t, j, k, nz)], (double)(b & 1),
+                 (double)((b >> 1) & 1), (double)((b >> 2) & 1), (double)((b >> 3) & 1), (double)((b >> 4) & 1),
+                 (double)((b >> 5) & 1), diffu, (double)(pre.x >> 31), t.inv_dx);
+}
+
 // dead_IP3_calc (:156-173) and supra_calc (:182-222).  A group of CG lanes
 // (4, 8 or 16, set by the autotuner) owns one cell: lanes evaluate the
 // gap-junction terms of the list entries (and update gj toward ALIVE
@@ -430,7 +483,10 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           cq[j] = t.bf[bp][gidx(t, ix, iy, iz)];
         } else if (recompute && iz < izb) {
           // the periodic copies x = NX, y = NY hold the values of x = 0, y = 0
-          cq[j] = atp_point(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz);
+          const uint2 pre = t.ca_cq[8 * (size_t)c + k];
+          cq[j] = (pre.y & 0x80000000u)
+                      ? atp_point(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz)
+                      : atp_point_pre(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz, pre);
         } else if (recompute) {
           // plane iz_bound is never updated inside a solve; its copies are
           // written beside this launch, so read the value they copy
@@ -672,6 +728,7 @@ static void ca_maps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, in
   if (gb < 1) gb = 1;
   if (gb > 65535) gb = 65535;
   EPI_LAUNCH(K_CA_VOX, k_ca_vox, dim3(gb, ntis), dim3(256), 0, s, dT, tlist);
+  EPI_LAUNCH(K_CA_CORNERS, k_ca_corners, dim3((8 * max_n + 255) / 256, ntis), dim3(256), 0, s, dT, tlist);
 }
 
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box) {
