@@ -40,6 +40,9 @@ constexpr double FAC_MAP = 2.0;            // define.h:184
 constexpr int SW_THRESH = 20;              // define.h:187
 constexpr int MAX_CONN = 400;              // define.h:80, cell_connection.cpp:30
 constexpr int N3 = 200;                    // cell_connection.cpp:28
+// slots between two buckets: the capacity rounded up to a multiple of 4, so
+// the sweep can read a bucket's indices four at a time (16-byte aligned)
+__host__ __device__ constexpr int bucket_stride(int bcap) { return (bcap + 3) & ~3; }
 constexpr double AREA_GRID = 2.0 + 1e-7;   // cell_connection.cpp:16-19
 constexpr int CA_ITR = (int)(10.0 / DT_Ca);  // ca2p.cpp:14
 

@@ -229,7 +229,7 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.bucket_of = cv.take<int>(cap);
   t.bslot = cv.take<int>(cap);
   t.bucket_n = cv.take<int>(h.nbuckets);
-  t.bucket_cells = cv.take<int>((size_t)h.nbuckets * h.bcap);
+  t.bucket_cells = cv.take<int>((size_t)h.nbuckets * bucket_stride(h.bcap));
   t.div_slot = cv.take<int>(cap);
   t.div_dir = cv.take<double>(3 * cap);
   t.div_parent = cv.take<int>(cap);

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_bucket_insert(const Tis* __restrict__ T
     const int slot = atomicAdd(&t.bucket_n[b], 1);
     // the reference asserts fewer than N3 = 200 cells per bucket (:59)
     if (slot + 1 >= N3 || slot >= t.bcap) { set_err(st, ERR_CAPACITY); continue; }
-    t.bucket_cells[(size_t)b * t.bcap + slot] = c;
+    t.bucket_cells[(size_t)b * bucket_stride(t.bcap) + slot] = c;
     t.bslot[c] = slot;
   }
 }
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
     const int b = t.bucket_of[c];
     const int m = min(t.bucket_n[b], t.bcap);
     if (m < 2) continue;
-    int* a = t.bucket_cells + (size_t)b * t.bcap;
+    int* a = t.bucket_cells + (size_t)b * bucket_stride(t.bcap);
     for (int i = 1; i < m; i++) {
       const int v = a[i];
       int j = i - 1;
@@ -125,15 +125,28 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
       int sz[5];
 #pragma unroll
       for (int i = 0; i < 5; i++) sz[i] = min(t.bucket_n[bxy + (anz - 2 + i + ANZ) % ANZ], t.bcap);
+      // a bucket's indices four at a time (one 16-byte load: a bucket's
+      // slots start 16-byte aligned and its stride is a multiple of 4), then those
+      // candidates' positions together; the walk stops at the first index
+      // >= c (ascending within the bucket)
 #pragma unroll
       for (int i = 0; i < 5; i++) {
-        const int* cells = t.bucket_cells + (size_t)(bxy + (anz - 2 + i + ANZ) % ANZ) * t.bcap;
-        for (int m = 0; m < sz[i]; ++m) {
-          const int o = cells[m];
-          if (c <= o) break;   // ascending within the bucket
-          if (sweep_hit(t, pb, pc, rc, o)) {
-            if (cnt < SWEEP_H) hits[cnt] = o;
-            cnt++;
+        const int* cells = t.bucket_cells + (size_t)(bxy + (anz - 2 + i + ANZ) % ANZ) * bucket_stride(t.bcap);
+        bool done = false;
+        for (int m0 = 0; m0 < sz[i] && !done; m0 += 4) {
+          const int4 q = *reinterpret_cast<const int4*>(cells + m0);
+          const int o4[4] = {q.x, q.y, q.z, q.w};
+          bool hit[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) hit[u] = m0 + u < sz[i] && o4[u] < c && sweep_hit(t, pb, pc, rc, o4[u]);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (done || m0 + u >= sz[i]) continue;
+            if (c <= o4[u]) { done = true; continue; }
+            if (hit[u]) {
+              if (cnt < SWEEP_H) hits[cnt] = o4[u];
+              cnt++;
+            }
           }
         }
       }
@@ -153,7 +166,7 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
         for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
           const long b = bxy + l % ANZ;
           const int sz = min(t.bucket_n[b], t.bcap);
-          const int* cells = t.bucket_cells + (size_t)b * t.bcap;
+          const int* cells = t.bucket_cells + (size_t)b * bucket_stride(t.bcap);
           for (int m = 0; m < sz; ++m) {
             const int o = cells[m];
             if (c <= o) break;
