@@ -125,6 +125,11 @@ struct epi_ctx {
   char* arena = nullptr;
   size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
+  // a step builds the voxel maps on side_stream beside the connection build
+  // (both only need the positions that bucket insertion fixes); the fork and
+  // join events order it with the main stream
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
   Tuning tu;
@@ -498,6 +503,12 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
   for (int t = 0; t < ntissues; t++) carve_tissue(dry, c->th[t], c->tis[t]);
   c->arena_bytes = align_up(dry.off, 256);
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return EPI_ERR_HIP; }
+  if (hipStreamCreate(&c->side_stream) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    epi_destroy(c);
+    return EPI_ERR_HIP;
+  }
   if (hipMalloc(&c->arena, c->arena_bytes) != hipSuccess ||
       hipMalloc(&c->d_tis, sizeof(Tis) * ntissues) != hipSuccess ||
       hipMalloc(&c->d_st, sizeof(Status) * ntissues) != hipSuccess ||
@@ -557,6 +568,9 @@ void epi_destroy(epi_ctx* c) {
   if (c->h_zz) (void)hipHostFree(c->h_zz);
   if (c->h_tlist) (void)hipHostFree(c->h_tlist);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
 }
 
@@ -875,12 +889,18 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     c->pb ^= 1;
     launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
     // periodic fix fused into the bucket insertion; calc_zzmax fused into
-    // the snapshot after the build
+    // the snapshot after the build.  The voxel maps (proc.cpp:155-157) read
+    // only the fixed positions, states and radii, which the rest of the
+    // build leaves alone, so they run beside it on the side stream.
     const int nb = 1 - c->cb;
-    launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, mn, max_nb(c), 1);
+    launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
+    launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
+    launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
     c->cb = nb;
-    // proc.cpp:155-157
-    launch_maps(c->stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
     c->bp ^= 1;
     HIPCHK(c, hipGetLastError());

@@ -356,16 +356,25 @@ __global__ void k_zz_set(const Tis* __restrict__ T, int use_value, double v) {
   if (threadIdx.x == 0) t.st->zzmax_bits = use_value ? (unsigned long long)__double_as_longlong(v) : 0ull;
 }
 
-void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix) {
-  (void)max_buckets;
+void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, int fix) {
   dim3 gc((max_n + 255) / 256, ntis);
   EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb, fix);
+}
+
+void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n) {
+  dim3 gc((max_n + 255) / 256, ntis);
   EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gc, dim3(256), 0, s, dT);
   dim3 gs((max_n + 7) / 8, ntis);
   EPI_LAUNCH(K_SWEEP, k_sweep, gs, dim3(256), 0, s, dT, pb, nb);
   dim3 gk((max_n + K_GROUPS - 1) / K_GROUPS, ntis);
   EPI_LAUNCH(K_FINISH, k_finish, gk, dim3(K_BLOCK), 0, s, dT, pb, nb);
   EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT, pb);
+}
+
+void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix) {
+  (void)max_buckets;
+  launch_connect_insert(s, dT, ntis, pb, max_n, fix);
+  launch_connect_rest(s, dT, ntis, pb, nb, max_n);
 }
 
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {

@@ -55,6 +55,10 @@ void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int m
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n);
 void launch_initialize_sc(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int pb, int max_n);
 void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
+// the bucket insertion (with the periodic fix) and the rest of the build,
+// for callers that run the voxel maps beside the rest
+void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, int fix);
+void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n);
 void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix);
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v);
