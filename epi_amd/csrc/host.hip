@@ -376,7 +376,7 @@ int set_zz(epi_ctx* c, const double* zzmax) {
 // sized by the solving tissues only: a gated solve of one small tissue in a
 // large batch launches that tissue's blocks, not the batch's largest.
 int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
-  int mn = 1, mx = 1, my = 1, izb = 1;
+  int mn = 1, mcell = 1, mx = 1, my = 1, izb = 1;
   long long box = 1;
   for (int i = 0; i < ntis; i++) {
     const int t = d_tlist ? c->h_tlist[i] : i;
@@ -384,6 +384,7 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
     const double zz = c->h_st[t].zzmax;
     const int iz = std::min((int)((zz + FAC_MAP * R_max) / c->tis[t].dz) + 2, h.NZ + 1);
     mn = std::max(mn, h.n_est + 64);
+    mcell = std::max(mcell, h.n_est + 64 - h.nmemb - h.nder);   // cells the sub-step kernels visit
     mx = std::max(mx, h.NX);
     my = std::max(my, h.NY);
     izb = std::max(izb, iz);
@@ -404,7 +405,7 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
     tu.ca_fuse = tu.ca_fuse == 1;
   }
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
-  launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mn, mx, my, izb, tu);
+  launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mcell, mx, my, izb, tu);
   launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
   HIPCHK(c, hipGetLastError());
   return EPI_OK;

@@ -288,27 +288,6 @@ __device__ __forceinline__ double atp_new(double catp, double cxm, double cxp, d
                          * inv_dx * inv_dx + fa + air * AIR_STIM);
 }
 
-// the new value of interior point (j, k, l), l < iz_bound, from buffer carr
-// and the sub-step's diff_u
-__device__ __forceinline__ double atp_point(const Tis& t, const double* __restrict__ carr,
-                                            const double* __restrict__ dfu, int j, int k, int l) {
-  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
-  const uint8_t* __restrict__ cm2 = t.cmap2;
-  const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
-  const int py = k == 0 ? NY - 1 : k - 1, ny = k == NY - 1 ? 0 : k + 1;
-  const int pz = l == 0 ? 1 : l - 1, nz = l + 1 == NZ ? NZ - 1 : l + 1;
-  const long long v = gidx(t, j, k, l);
-  const long long vxm = gidx(t, px, k, l), vxp = gidx(t, nx, k, l);
-  const long long vym = gidx(t, j, py, l), vyp = gidx(t, j, ny, l);
-  const long long vzm = gidx(t, j, k, pz), vzp = gidx(t, j, k, nz);
-  const unsigned w = t.vox[v];
-  const unsigned cell1 = w & 0x7fffffffu;
-  const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
-  return atp_new(carr[v], carr[vxm], carr[vxp], carr[vym], carr[vyp], carr[vzm], carr[vzp], (double)cm2[vxm],
-                 (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], (double)cm2[vzm], (double)cm2[vzp], diffu,
-                 (double)(w >> 31), t.inv_dx);
-}
-
 // diff_u of a sub-step by Ca parity: the sub-step at parity cp writes it and
 // its ATP update reads it, so a fused launch (ATP of one sub-step beside the
 // cells of the next) reads one buffer while it writes the other
@@ -351,31 +330,52 @@ __global__ void __launch_bounds__(256) k_ca_corners(const Tis* __restrict__ T, c
   }
 }
 
-// atp_point with the corner's voxel word and neighbour masks from k_ca_corners
+// The new value of interior point (j, k, l), l < iz_bound, from buffer carr
+// and the sub-step's diff_u, with the point's voxel word and neighbour masks
+// from k_ca_corners.  The seven ATP loads do not depend on them; a point whose
+// top bit is set (a mask byte above 1) reads the word and masks itself.
 __device__ __forceinline__ double atp_point_pre(const Tis& t, const double* __restrict__ carr,
                                                 const double* __restrict__ dfu, int j, int k, int l, uint2 pre) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
   const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
   const int py = k == 0 ? NY - 1 : k - 1, ny = k == NY - 1 ? 0 : k + 1;
   const int pz = l == 0 ? 1 : l - 1, nz = l + 1 == NZ ? NZ - 1 : l + 1;
-  const unsigned cell1 = pre.x & 0x7fffffffu;
+  const long long v = gidx(t, j, k, l);
+  const long long vxm = gidx(t, px, k, l), vxp = gidx(t, nx, k, l);
+  const long long vym = gidx(t, j, py, l), vyp = gidx(t, j, ny, l);
+  const long long vzm = gidx(t, j, k, pz), vzp = gidx(t, j, k, nz);
+  const double c0 = carr[v], cxm = carr[vxm], cxp = carr[vxp], cym = carr[vym], cyp = carr[vyp], czm = carr[vzm],
+               czp = carr[vzp];
+  unsigned w;
+  double m[6];
+  if (pre.y & 0x80000000u) {
+    const uint8_t* __restrict__ cm2 = t.cmap2;
+    w = t.vox[v];
+    m[0] = cm2[vxm]; m[1] = cm2[vxp]; m[2] = cm2[vym]; m[3] = cm2[vyp]; m[4] = cm2[vzm]; m[5] = cm2[vzp];
+  } else {
+    w = pre.x;
+#pragma unroll
+    for (int b = 0; b < 6; b++) m[b] = (double)((pre.y >> b) & 1);
+  }
+  const unsigned cell1 = w & 0x7fffffffu;
   const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
-  const unsigned b = pre.y;
-  return atp_new(carr[gidx(t, j, k, l)], carr[gidx(t, px, k, l)], carr[gidx(t, nx, k, l)], carr[gidx(t, j, py, l)],
-                 carr[gidx(t, j, ny, l)], carr[gidx(t, j, k, pz)], carr[gidx(t, j, k, nz)], (double)(b & 1),
-                 (double)((b >> 1) & 1), (double)((b >> 2) & 1), (double)((b >> 3) & 1), (double)((b >> 4) & 1),
-                 (double)((b >> 5) & 1), diffu, (double)(pre.x >> 31), t.inv_dx);
+  return atp_new(c0, cxm, cxp, cym, cyp, czm, czp, m[0], m[1], m[2], m[3], m[4], m[5], diffu, (double)(w >> 31),
+                 t.inv_dx);
 }
 
 // dead_IP3_calc (:156-173) and supra_calc (:182-222).  A group of CG lanes
 // (4, 8 or 16, set by the autotuner) owns one cell: lanes evaluate the
 // gap-junction terms of the list entries (and update gj toward ALIVE
-// neighbours) 4 * CG entries at a time and load the 16 grid corners of the two
-// grid_avg8; the leader sums them in list / corner order.  Every lane issues
+// neighbours) 16 (CG 16: 32) entries at a time and load the 16 grid corners
+// of the two grid_avg8; the leader sums them in list / corner order.  Each
+// group takes one cell, so the grid must cover every cell past the membrane
+// and dermis (a launch that does not is reported as ERR_INVALID); a loop over
+// cells would keep loop-invariant constants of the math in registers for
+// the whole kernel and cost a wave per SIMD.  Every lane issues
 // all loads of its entries of a chunk (list word, gj, then the neighbours'
 // Ca2+ and IP3 pair) before it uses any of them.  With carr_prev set, the ATP
 // corners are not read but computed from the previous sub-step's buffer
-// (atp_point), the ATP update of that sub-step running beside this one.
+// (atp_point_pre), the ATP update of that sub-step running beside this one.
 constexpr int C_BLOCK = 128;
 
 template <int CG, int NT>
@@ -383,7 +383,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
                                               const double* __restrict__ carr_prev,
                                               const double* __restrict__ dfu_prev) {
   constexpr int C_GROUPS = NT / CG;
-  constexpr int EPL = 4;            // entries per lane per chunk
+  constexpr int EPL = CG >= 8 ? 2 : 4;   // entries per lane per chunk
   constexpr int CCAP = EPL * CG;    // list entries per chunk
   constexpr int NQ = 16 / CG;       // grid corners per lane
   const int n = t.st->n;
@@ -398,8 +398,11 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
   double* __restrict__ dfu_out = dfu_buf(t, cp);
   const int izb = carr_prev ? ca_iz_bound(t) : 0;
   const bool recompute = carr_prev && izb <= t.NZ;
-  const int stride = nbx * C_GROUPS;
-  for (int c = t.nmemb + t.nder + bx * C_GROUPS + grp; c < n; c += stride) {
+  const int start = t.nmemb + t.nder;
+  if (bx == 0 && threadIdx.x == 0 && start + (long long)nbx * C_GROUPS < n) set_err(t.st, ERR_INVALID);
+  const int c = start + bx * C_GROUPS + grp;
+  if (c >= n) return;
+  {
     const uint8_t s = t.state[c];
     const int len = t.nlen[cb][c];
     const double2 uc = up[c];
@@ -415,16 +418,18 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
       en0[i] = ent[lane + i * CG];
       gv0[i] = g[lane + i * CG];
     }
-    // the lattice position and the leader's per-cell values are fetched now
-    // too, so the corner gathers start one round trip earlier
+    // the lattice position is fetched now too, so the corner gathers start
+    // one round trip earlier
     const int lx = t.lat[3 * c], ly = t.lat[3 * c + 1], lz = t.lat[3 * c + 2];
-    double exi = 0, avg = 0, _th = 0, _Kpa = 0, IAGv = 0;
-    if (lane == 0) {
-      exi = t.ex_inert[c]; avg = t.ca2p_avg[c];
-      _th = t.ca_th[c]; _Kpa = t.ca_kpa[c]; IAGv = t.ca_iag[c];
+    // the corners' per-solve words depend on c only
+    uint2 pre[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      const int q = lane + j * CG;
+      pre[j] = recompute && q >= 8 ? t.ca_cq[8 * (size_t)c + (q & 7)] : make_uint2(0u, 0u);
     }
     const bool dead = s == DEAD;
-    if (!dead && !is_afm(s)) continue;
+    if (!dead && !is_afm(s)) return;
     if (dead) {
       int count = 0;
       double tmp = 0;
@@ -466,7 +471,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
         t.ip3[1 - cp][c] = np;
         reinterpret_cast<double*>(up_next + c)[1] = np;   // ca2p of a DEAD cell stays as it is
       }
-      continue;
+      return;
     }
     // the corners of grid_avg8 (utils.h:18-23; slots 0-7 B, 8-15 ATP) are
     // fetched now, beside the list loads
@@ -483,10 +488,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           cq[j] = t.bf[bp][gidx(t, ix, iy, iz)];
         } else if (recompute && iz < izb) {
           // the periodic copies x = NX, y = NY hold the values of x = 0, y = 0
-          const uint2 pre = t.ca_cq[8 * (size_t)c + k];
-          cq[j] = (pre.y & 0x80000000u)
-                      ? atp_point(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz)
-                      : atp_point_pre(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz, pre);
+          cq[j] = atp_point_pre(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz, pre[j]);
         } else if (recompute) {
           // plane iz_bound is never updated inside a solve; its copies are
           // written beside this launch, so read the value they copy
@@ -550,6 +552,8 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
     for (int j = 0; j < NQ; j++) s_g[grp][lane + j * CG] = cq[j];
     wave_sync();
     if (lane == 0) {
+      const double exi = t.ex_inert[c], avg = t.ca2p_avg[c], _th = t.ca_th[c], _Kpa = t.ca_kpa[c],
+                   IAGv = t.ca_iag[c];
       const double ca2p_du = t.ca_du;
       const double* q = s_g[grp];
       const double Bavg = 0.125 * (q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7]);

@@ -69,7 +69,9 @@ void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx,
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
                        int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
-// n_sub sub-steps (fused when tu.ca_fuse), flipping cp and ap after each
+// n_sub sub-steps (fused when tu.ca_fuse), flipping cp and ap after each.
+// The cell kernels take one cell per lane group: max_n (here and in
+// launch_ca_substep) must be at least the cells past the membrane and dermis.
 void launch_ca_substeps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int& cp, int& ap, int bp,
                         int n_sub, int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
 void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
