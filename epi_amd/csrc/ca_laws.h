@@ -36,8 +36,13 @@ __device__ __forceinline__ int ca_iz_bound(const Tis& t) {   // :386
   return (int)((zzmax + FAC_MAP * R_max) / t.dz);
 }
 
-__device__ __forceinline__ double fw_term(double diff, double w) {   // fw (:138-144)
-  return (1. - w) + (-1. + tanh((0.1 - diff) / 0.1)) / 2.;
+// fw (:138-144): (1 - w) + (-1 + tanh(y)) / 2 with y = (0.1 - diff) / 0.1.
+// (-1 + tanh y) / 2 equals -1 / (1 + e^(2y)); that form costs one exp and one
+// division (a double tanh costs about three times as much) and does not
+// cancel where tanh y is near 1.  A large y makes e^(2y) infinite and the term
+// -0, the limit.
+__device__ __forceinline__ double fw_term(double diff, double w) {
+  return (1. - w) - 1. / (1. + exp(2. * ((0.1 - diff) / 0.1)));
 }
 
 }  // namespace epi
