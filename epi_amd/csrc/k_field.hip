@@ -387,10 +387,10 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
   constexpr int CCAP = EPL * CG;    // list entries per chunk
   constexpr int NQ = 16 / CG;       // grid corners per lane
   const int n = t.st->n;
-  __shared__ double s_t1[C_GROUPS][CCAP];
-  __shared__ double s_t2[C_GROUPS][CCAP];
-  __shared__ uint8_t s_f[C_GROUPS][CCAP];
-  __shared__ double s_g[C_GROUPS][16];
+  // per-entry terms for the leader; rows padded by 16 bytes so that the
+  // leaders of a wave's groups read different LDS banks
+  __shared__ double2 s_a[C_GROUPS][CCAP + 1];
+  __shared__ double s_g[C_GROUPS][17];
   const int lane = threadIdx.x & (CG - 1);
   const int grp = threadIdx.x / CG;
   const double2* __restrict__ up = t.ca_up[cp];
@@ -447,22 +447,21 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
 #pragma unroll
         for (int i = 0; i < EPL; i++) {
           const int e = lane + i * CG;
-          if (e < m) { s_f[grp][e] = (en[i] & 3) == 3; s_t1[grp][e] = po[i]; }
+          if (e < m) s_a[grp][e] = make_double2(po[i], (en[i] & 3) == 3 ? 1.0 : 0.0);
         }
         wave_sync();
+        // adding the +0.0 of an entry that is not ALIVE leaves tmp as it is
+        // (tmp starts at +0.0, so it is never -0.0)
         if (lane == 0) {
           int e = 0;
           for (; e + 4 <= m; e += 4) {
-            uint8_t f[4];
-            double a[4];
+            double2 a[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) { f[j] = s_f[grp][e + j]; a[j] = s_t1[grp][e + j]; }
+            for (int j = 0; j < 4; j++) a[j] = s_a[grp][e + j];
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-              if (f[j]) { tmp += a[j]; count++; }
+            for (int j = 0; j < 4; j++) { tmp += a[j].x; count += a[j].y != 0.0; }
           }
-          for (; e < m; e++)
-            if (s_f[grp][e]) { tmp += s_t1[grp][e]; count++; }
+          for (; e < m; e++) { const double2 a = s_a[grp][e]; tmp += a.x; count += a.y != 0.0; }
         }
         wave_sync();
       }
@@ -524,27 +523,23 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
             a2 = gv[i] * (w[i].y - p);
             if (kd == 3) g[e0 + e] = gv[i] + DT_Ca * fw_term(fabs(w[i].x - u), gv[i]);
           }
-          s_f[grp][e] = kd;
-          s_t1[grp][e] = a1;
-          s_t2[grp][e] = a2;
+          s_a[grp][e] = make_double2(a1, a2);
         }
       }
       wave_sync();
-      // the leader adds in list order; four entries' LDS reads are issued
-      // before their adds
+      // the leader adds in list order, four entries' LDS reads ahead of their
+      // adds.  A skipped entry adds +0.0, which leaves the sums as they are
+      // (they start at +0.0, so neither is ever -0.0).
       if (lane == 0) {
         int e = 0;
         for (; e + 4 <= m; e += 4) {
-          uint8_t f[4];
-          double a[4], b[4];
+          double2 a[4];
 #pragma unroll
-          for (int j = 0; j < 4; j++) { f[j] = s_f[grp][e + j]; a[j] = s_t1[grp][e + j]; b[j] = s_t2[grp][e + j]; }
+          for (int j = 0; j < 4; j++) a[j] = s_a[grp][e + j];
 #pragma unroll
-          for (int j = 0; j < 4; j++)
-            if (f[j]) { tmp_diffu += a[j]; tmp_IP3 += b[j]; }
+          for (int j = 0; j < 4; j++) { tmp_diffu += a[j].x; tmp_IP3 += a[j].y; }
         }
-        for (; e < m; e++)
-          if (s_f[grp][e]) { tmp_diffu += s_t1[grp][e]; tmp_IP3 += s_t2[grp][e]; }
+        for (; e < m; e++) { const double2 a = s_a[grp][e]; tmp_diffu += a.x; tmp_IP3 += a.y; }
       }
       wave_sync();
     }
