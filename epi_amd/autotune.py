@@ -7,7 +7,8 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
   k_ca_cells   lanes per cell (4, 8, 16)
   k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
   Ca solve     ATP of a sub-step and cells of the next in one launch (1; 2 =
-               only solves of a few tissues, when the whole set runs faster unfused)
+               only solves of a few tissues, when the whole set runs faster
+               unfused), with its own lanes per cell and z planes per thread
   k_map_stamp  lanes per cell (1, 4)
 
 With CA_FUSE 2 the fused small solves use CA_GROUP_SMALL lanes per cell
@@ -111,16 +112,17 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
     best["xcd"] = int(min(report["xcd"], key=report["xcd"].get)[3:])
     ts.set_tuning(best)
 
-    # per sub-step: the two kernels against one fused launch (z planes per
-    # thread of its grid part from the candidates above)
+    # per sub-step: the two kernels against one fused launch, whose lanes per
+    # cell and z planes per thread of its grid part are chosen for it
     report["ca_fuse"] = {"unfused": report["xcd"][f"xcd{best['xcd']}"]}
-    for zb in (1, 2, 4):
-        ts.set_tuning({"ca_fuse": 1, "atp_zb": zb})
-        report["ca_fuse"][f"fused_zb{zb}"] = _time(ts, "k_ca_fused", run_ca)
+    for g in CA_GROUPS:
+        for zb in (1, 2, 4):
+            ts.set_tuning({"ca_fuse": 1, "ca_group": g, "atp_zb": zb})
+            report["ca_fuse"][f"fused_group{g}_zb{zb}"] = _time(ts, "k_ca_fused", run_ca)
     key = min(report["ca_fuse"], key=report["ca_fuse"].get)
     if key != "unfused":
         best["ca_fuse"] = 1
-        best["atp_zb"] = int(key[8:])
+        best["ca_group"], best["atp_zb"] = (int(v) for v in key[len("fused_group"):].split("_zb"))
     else:
         # the whole set solves faster unfused; a gated solve of a few
         # tissues is latency-bound and still fuses (ca_fuse 2)
