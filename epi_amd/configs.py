@@ -256,6 +256,18 @@ def config_large(seed=2):
                        dead_top=True, n_hot_alive=60)
 
 
+def config_dense_sheet(seed=11):
+    """Small tissue on a dense membrane sheet: 26x26 particles on a 14x14
+    box (pitch 0.54), 300 free cells in 3 layers, grid 16x16x32.  A basal
+    cell's list then holds the many sheet particles within its reach: the
+    first build has 25 lists longer than the 40 entries that the
+    cooperative force kernel holds (the longest 46, about 60 after 100
+    steps), without the strong interactions that make the reference abort."""
+    lx = 14.0
+    return make_tissue(seed, nmx=26, nmy=26, lx=lx, ly=lx, nx=16, ny=16, nz=32,
+                       n_fix=4, n_free=300, n_layers=3, n_hot_alive=4, lz=32 * lx / 16)
+
+
 def config_tiny(seed=7, n_free=160, side=16, grid=16, n_layers=2, dead_top=False, nz=None):
     """Small tissue for fast parity tests (same generator); nz > grid makes
     the box taller (dz = dx)."""

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -125,6 +126,7 @@ struct epi_ctx {
   char* arena = nullptr;
   size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
+  bool len_known = false;   // h_st maxlen read back since the last list build
   // a step builds the voxel maps on side_stream beside the connection build
   // (both only need the positions that bucket insertion fixes); the fork and
   // join events order it with the main stream
@@ -258,6 +260,7 @@ int sync_status(epi_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!g_prof.pending.empty()) prof_resolve();
   for (int t = 0; t < c->T; t++) c->th[t].n_est = c->h_st[t].n;
+  c->len_known = true;
   for (int t = 0; t < c->T; t++)
     if (c->h_st[t].err) {
       char buf[128];
@@ -271,6 +274,15 @@ int put_status(epi_ctx* c, int t) {
   HIPCHK(c, hipMemcpyAsync(c->d_st + t, c->h_st + t, sizeof(Status), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return EPI_OK;
+}
+
+// the longest neighbour list any build has made (maxlen only grows), or
+// INT_MAX when a build has run since the last status read
+int max_list_len(const epi_ctx* c) {
+  if (!c->len_known) return INT_MAX;
+  int m = 0;
+  for (int t = 0; t < c->T; t++) m = std::max(m, c->h_st[t].maxlen);
+  return m;
 }
 
 // launch bounds over all tissues
@@ -750,7 +762,7 @@ int epi_rng_mt19937_64(uint64_t seed, double* out, int64_t n) {
 
 int epi_cell_interaction(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
-  launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), max_n(c));
+  launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), max_n(c), max_list_len(c));
   HIPCHK(c, hipGetLastError());
   c->pos_pending = true;
   return EPI_OK;
@@ -784,6 +796,7 @@ int epi_connect_cell(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
   const int nb = 1 - c->cb;
   launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, max_n(c), max_nb(c), 0);
+  c->len_known = false;
   HIPCHK(c, hipGetLastError());
   c->cb = nb;
   return EPI_OK;
@@ -886,7 +899,7 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
   for (int64_t it = 0; it < nsteps; it++) {
     const int mn = max_n(c);
     // cell_dynamics (proc.cpp:22-32)
-    launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), mn);
+    launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), mn, max_list_len(c));
     c->pb ^= 1;
     launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
     // periodic fix fused into the bucket insertion; calc_zzmax fused into
@@ -895,6 +908,7 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     // build leaves alone, so they run beside it on the side stream.
     const int nb = 1 - c->cb;
     launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
+    c->len_known = false;
     HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
     launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
@@ -1518,7 +1532,7 @@ int mech_params(epi_ctx* c, const epi_mech_params* prm) {
   return EPI_OK;
 }
 int mech_step(epi_ctx* c) {
-  launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), max_n(c));
+  launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), max_n(c), max_list_len(c));
   c->pb ^= 1;
   return EPI_OK;
 }

@@ -332,16 +332,35 @@ __device__ void forces_serial(const Tis& t, int pb, int cb, int X, Status* st) {
 // higher-index handlers ascending with a higher-index partner's spring
 // after that partner's list.  Lane 0..11 evaluate the 12 dihedral terms of a
 // membrane particle.
+//
+// An absent "own" term is stored as -0.0 and an absent "oth" term as +0.0:
+// x + (-0.0) and x - (+0.0) are x for every x, so the leader adds and
+// subtracts every slot without flags or branches, four slots' LDS reads
+// ahead of their adds, in the same order and with the same results.
+//
+// Each group takes one cell, so the grid must cover all cells (a launch that
+// does not is reported as ERR_INVALID); a loop over cells would keep the
+// math's loop-invariant constants in registers and cost occupancy.
 constexpr int FG = 16;
-constexpr int FCAP = 48;
+constexpr int FCAP = FORCES_ROW_CAP;   // 40: an 8-cell block fits 8 per CU in LDS
 constexpr int F_BLOCK = 128;
 constexpr int F_GROUPS = F_BLOCK / FG;
+
+// a group's LDS slots, padded by 8 bytes so that the leaders of a wave's
+// groups read different banks
+struct FRow {
+  P3 v[FCAP];
+  double pad;
+};
+
+__device__ __forceinline__ void p3_add(P3& a, const P3& d) { a.x += d.x; a.y += d.y; a.z += d.z; }
+__device__ __forceinline__ void p3_sub(P3& a, const P3& d) { a.x -= d.x; a.y -= d.y; a.z -= d.z; }
 
 __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, int pb, int cb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
-  __shared__ P3 s_own[F_GROUPS][FCAP];
-  __shared__ P3 s_oth[F_GROUPS][FCAP];
+  __shared__ FRow s_own[F_GROUPS];
+  __shared__ FRow s_oth[F_GROUPS];
   __shared__ int s_idx[F_GROUPS][FCAP];
   __shared__ uint8_t s_flag[F_GROUPS][FCAP];
   __shared__ uint8_t s_perm[F_GROUPS][FCAP];
@@ -349,125 +368,155 @@ __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, i
   const int lane = threadIdx.x & (FG - 1);
   const int grp = threadIdx.x / FG;
   const int n = st->n;
-  const int stride = gridDim.x * F_GROUPS;
-  for (int X = blockIdx.x * F_GROUPS + grp; X < n; X += stride) {
-    const int len = t.nlen[cb][X];
-    if (len > FCAP) continue;   // k_forces_long
-    const CV cx = load_cv(t, pb, X);
-    const bool memb = X < t.nmemb;
-    const int nA = memb ? 0 : t.nlo[X];
-    const int* row = t.col[cb] + (size_t)X * t.ns;
-    for (int e = lane; e < len; e += FG) {
-      const int q = row[e];
-      const CV cq = load_cv(t, pb, q);
-      uint8_t f = 0;
-      P3 own = P3{0, 0, 0}, oth = P3{0, 0, 0};
-      if (memb) {
-        if (e < MEMB_ADJ) {
-          if (cq.s == MEMB) {
-            if (X > q) { own = delta(t, LAW_MEMB_MEMB, cx, cq, st); f |= 1; }
-            else if (q > X) { oth = delta(t, LAW_MEMB_MEMB, cq, cx, st); f |= 2; }
-          }
-        } else {
-          const int law2 = proc_law(cq, q, cx, X);
-          if (law2 != LAW_NONE) { oth = delta(t, law2, cq, cx, st); f |= 2; }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (long long)gridDim.x * F_GROUPS < n) set_err(st, ERR_INVALID);
+  const int X = blockIdx.x * F_GROUPS + grp;
+  if (X >= n) return;
+  const int len = t.nlen[cb][X];
+  if (len > FCAP) return;   // k_forces_long
+  const CV cx = load_cv(t, pb, X);
+  const bool memb = X < t.nmemb;
+  const int nA = memb ? 0 : t.nlo[X];
+  const int* row = t.col[cb] + (size_t)X * t.ns;
+  for (int e = lane; e < len; e += FG) {
+    const int q = row[e];
+    const CV cq = load_cv(t, pb, q);
+    uint8_t f = 0;
+    P3 own = P3{-0.0, -0.0, -0.0}, oth = P3{0, 0, 0};
+    if (memb) {
+      if (e < MEMB_ADJ) {
+        if (cq.s == MEMB) {
+          if (X > q) { own = delta(t, LAW_MEMB_MEMB, cx, cq, st); f |= 1; }
+          else if (q > X) { oth = delta(t, LAW_MEMB_MEMB, cq, cx, st); f |= 2; }
         }
       } else {
-        const int law = proc_law(cx, X, cq, q);
-        if (law != LAW_NONE) { own = delta(t, law, cx, cq, st); f |= 1; }
-        if (e >= nA || cq.s != MEMB) {
-          const int law2 = proc_law(cq, q, cx, X);
-          if (law2 != LAW_NONE) { oth = delta(t, law2, cq, cx, st); f |= 2; }
-        }
+        const int law2 = proc_law(cq, q, cx, X);
+        if (law2 != LAW_NONE) { oth = delta(t, law2, cq, cx, st); f |= 2; }
       }
-      s_idx[grp][e] = q;
-      s_flag[grp][e] = f;
-      s_own[grp][e] = own;
-      s_oth[grp][e] = oth;
-    }
-    if (memb && lane < 12) {
-      const int* md = t.memb6 + 6 * X;
-      const int i = lane < 6 ? lane : lane - 6;
-      const P3 a = ld_pos(t, pb, md[i]);
-      const P3 b = ld_pos(t, pb, md[(i + 1) % 6]);
-      if (lane < 6)
-        s_bend[grp][lane] = tri_bend_1(t, cx.p, a, b, ld_pos(t, pb, t.memb6[6 * md[(i + 1) % 6] + i]));
-      else
-        s_bend[grp][lane] = tri_bend_2(t, cx.p, a, b, ld_pos(t, pb, md[(i + 2) % 6]));
-    }
-    wave_sync();
-    // rank of each lower-part entry by cell index
-    for (int e = lane; e < nA; e += FG) {
-      const int v = s_idx[grp][e];
-      int r = 0;
-      for (int k = 0; k < nA; k++) r += s_idx[grp][k] < v;
-      s_perm[grp][r] = (uint8_t)e;
-    }
-    wave_sync();
-    if (lane == 0) {
-      P3 acc = cx.p;
-      if (memb) {
-        P3 dr = P3{0, 0, 0};
-        for (int i = 0; i < 12; i++) dr = v_add(dr, s_bend[grp][i]);
-        acc.x += DT_Cell * t.m_kbend * dr.x;
-        acc.y += DT_Cell * t.m_kbend * dr.y;
-        acc.z += DT_Cell * t.m_kbend * dr.z;
-        if (cx.p.z < cx.r) acc.z += wall_dz(t, cx);
-        int hi[6], nhi = 0;
-        for (int e = 0; e < MEMB_ADJ; e++) {
-          const uint8_t f = s_flag[grp][e];
-          if (f & 1) { const P3 d = s_own[grp][e]; acc.x += d.x; acc.y += d.y; acc.z += d.z; }
-          if (f & 2) {
-            int k = nhi++;
-            const int q = s_idx[grp][e];
-            while (k > 0 && s_idx[grp][hi[k - 1]] > q) { hi[k] = hi[k - 1]; k--; }
-            hi[k] = e;
-          }
-        }
-        for (int i = 0; i < nhi; i++) {
-          const P3 d = s_oth[grp][hi[i]];
-          acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
-        }
-        for (int e = MEMB_ADJ; e < len; e++)
-          if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
-      } else {
-        for (int r = 0; r < nA; r++) {
-          const int e = s_perm[grp][r];
-          if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
-        }
-        if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(t, cx);
-        for (int e = 0; e < len; e++)
-          if (s_flag[grp][e] & 1) { const P3 d = s_own[grp][e]; acc.x += d.x; acc.y += d.y; acc.z += d.z; }
-        if (cx.pair >= 0 && X > cx.pair) {
-          const P3 d = delta(t, LAW_PAIR, cx, load_cv(t, pb, cx.pair), st);
-          acc.x += d.x; acc.y += d.y; acc.z += d.z;
-        }
-        int P = cx.pair > X ? cx.pair : -1;
-        for (int e = nA; e < len; e++) {
-          const int p = s_idx[grp][e];
-          if (P >= 0 && P < p) {
-            const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
-            acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
-            P = -1;
-          }
-          if (s_flag[grp][e] & 2) { const P3 d = s_oth[grp][e]; acc.x -= d.x; acc.y -= d.y; acc.z -= d.z; }
-          if (p == P) {
-            const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
-            acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
-            P = -1;
-          }
-        }
-        if (P >= 0) {
-          const P3 d = delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st);
-          acc.x -= d.x; acc.y -= d.y; acc.z -= d.z;
-        }
+    } else {
+      const int law = proc_law(cx, X, cq, q);
+      if (law != LAW_NONE) { own = delta(t, law, cx, cq, st); f |= 1; }
+      if (e >= nA || cq.s != MEMB) {
+        const int law2 = proc_law(cq, q, cx, X);
+        if (law2 != LAW_NONE) { oth = delta(t, law2, cq, cx, st); f |= 2; }
       }
-      t.pos[1 - pb][0][X] = acc.x;
-      t.pos[1 - pb][1][X] = acc.y;
-      t.pos[1 - pb][2][X] = acc.z;
     }
-    wave_sync();
+    s_idx[grp][e] = q;
+    s_flag[grp][e] = f;
+    s_own[grp].v[e] = own;
+    s_oth[grp].v[e] = oth;
   }
+  if (memb && lane < 12) {
+    const int* md = t.memb6 + 6 * X;
+    const int i = lane < 6 ? lane : lane - 6;
+    const P3 a = ld_pos(t, pb, md[i]);
+    const P3 b = ld_pos(t, pb, md[(i + 1) % 6]);
+    if (lane < 6)
+      s_bend[grp][lane] = tri_bend_1(t, cx.p, a, b, ld_pos(t, pb, t.memb6[6 * md[(i + 1) % 6] + i]));
+    else
+      s_bend[grp][lane] = tri_bend_2(t, cx.p, a, b, ld_pos(t, pb, md[(i + 2) % 6]));
+  }
+  wave_sync();
+  // rank of each lower-part entry by cell index
+  for (int e = lane; e < nA; e += FG) {
+    const int v = s_idx[grp][e];
+    int r = 0;
+    for (int k = 0; k < nA; k++) r += s_idx[grp][k] < v;
+    s_perm[grp][r] = (uint8_t)e;
+  }
+  wave_sync();
+  if (lane != 0) return;
+  const P3* own = s_own[grp].v;
+  const P3* oth = s_oth[grp].v;
+  P3 acc = cx.p;
+  if (memb) {
+    P3 dr = P3{0, 0, 0};
+    for (int i = 0; i < 12; i++) dr = v_add(dr, s_bend[grp][i]);
+    acc.x += DT_Cell * t.m_kbend * dr.x;
+    acc.y += DT_Cell * t.m_kbend * dr.y;
+    acc.z += DT_Cell * t.m_kbend * dr.z;
+    if (cx.p.z < cx.r) acc.z += wall_dz(t, cx);
+    int hi[6], nhi = 0;
+    for (int e = 0; e < MEMB_ADJ; e++) {
+      p3_add(acc, own[e]);
+      if (s_flag[grp][e] & 2) {
+        int k = nhi++;
+        const int q = s_idx[grp][e];
+        while (k > 0 && s_idx[grp][hi[k - 1]] > q) { hi[k] = hi[k - 1]; k--; }
+        hi[k] = e;
+      }
+    }
+    for (int i = 0; i < nhi; i++) p3_sub(acc, oth[hi[i]]);
+    int e = MEMB_ADJ;
+#pragma unroll 1
+    for (; e + 4 <= len; e += 4) {
+      P3 d[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) d[j] = oth[e + j];
+#pragma unroll
+      for (int j = 0; j < 4; j++) p3_sub(acc, d[j]);
+    }
+    for (; e < len; e++) p3_sub(acc, oth[e]);
+  } else {
+    {
+      int r = 0;
+#pragma unroll 1
+      for (; r + 4 <= nA; r += 4) {
+        int k[4];
+        P3 d[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) k[j] = s_perm[grp][r + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = oth[k[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) p3_sub(acc, d[j]);
+      }
+      for (; r < nA; r++) p3_sub(acc, oth[s_perm[grp][r]]);
+    }
+    if (cx.s == DER && cx.p.z < cx.r) acc.z += wall_dz(t, cx);
+    {
+      int e = 0;
+#pragma unroll 1
+      for (; e + 4 <= len; e += 4) {
+        P3 d[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = own[e + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) p3_add(acc, d[j]);
+      }
+      for (; e < len; e++) p3_add(acc, own[e]);
+    }
+    if (cx.pair >= 0 && X > cx.pair) p3_add(acc, delta(t, LAW_PAIR, cx, load_cv(t, pb, cx.pair), st));
+    int P = cx.pair > X ? cx.pair : -1;
+    if (P < 0) {
+      int e = nA;
+#pragma unroll 1
+      for (; e + 4 <= len; e += 4) {
+        P3 d[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = oth[e + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) p3_sub(acc, d[j]);
+      }
+      for (; e < len; e++) p3_sub(acc, oth[e]);
+    } else {
+      for (int e = nA; e < len; e++) {
+        const int p = s_idx[grp][e];
+        if (P >= 0 && P < p) {
+          p3_sub(acc, delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st));
+          P = -1;
+        }
+        p3_sub(acc, oth[e]);
+        if (p == P) {
+          p3_sub(acc, delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st));
+          P = -1;
+        }
+      }
+      if (P >= 0) p3_sub(acc, delta(t, LAW_PAIR, load_cv(t, pb, P), cx, st));
+    }
+  }
+  t.pos[1 - pb][0][X] = acc.x;
+  t.pos[1 - pb][1][X] = acc.y;
+  t.pos[1 - pb][2][X] = acc.z;
 }
 
 // cells whose rows exceed the LDS slots of k_forces, one lane per cell
@@ -478,12 +527,15 @@ __global__ void __launch_bounds__(256) k_forces_long(const Tis* __restrict__ T, 
     if (t.nlen[cb][X] > FCAP) forces_serial(t, pb, cb, X, t.st);
 }
 
-void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other) {
+void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other,
+                   int max_len) {
   (void)max_memb;
   dim3 g((max_other + F_GROUPS - 1) / F_GROUPS, ntis);
   EPI_LAUNCH(K_FORCES, k_forces, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
-  dim3 gl((max_other + 255) / 256, ntis);
-  EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb);
+  if (max_len > FCAP) {
+    dim3 gl((max_other + 255) / 256, ntis);
+    EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb);
+  }
 }
 
 // ==== frozen mechanics window: the reverse pass ==============================

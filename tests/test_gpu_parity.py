@@ -87,6 +87,24 @@ def test_mechanics_only_is_bit_exact(oracle_mod, seed):
         assert d["x"] == 0.0 and d["gj"] == 0.0 and d["agek"] == 0.0
 
 
+def test_long_rows_are_bit_exact(oracle_mod):
+    """Lists longer than the cooperative force kernel's 40 slots go to the
+    one-lane-per-cell kernel: both must give the oracle's positions."""
+    spec = configs.config_dense_sheet(11)
+    spec["cells"]["agek"] = np.minimum(spec["cells"]["agek"], 10.0)   # no cornification, no Ca
+    ot, gt = pair(oracle_mod, spec)
+    from epi_amd import native
+    do, dg = oracle_mod.Driver(0, 0, 0, 0, 0), native.Driver(0, 0, 0, 0, 0)
+    for k in range(3):
+        gt.step(dg, 40)
+        for _ in range(40):
+            ot.step(do)
+        d = check(gt, ot, tag=f"chunk {k}")
+        assert d["x"] == 0.0 and d["y"] == 0.0 and d["z"] == 0.0
+        assert int(np.diff(gt.get_cells()["conn_ptr"]).max()) > 40
+        assert gt.status().maxlen > 40
+
+
 def test_driver_small_config(oracle_mod):
     """Plain FP64 small config (64x64 sheet, 2,000 free cells, 64^3 grid,
     seed 1) for 1000 steps with divisions and a Ca solve every 100 steps:

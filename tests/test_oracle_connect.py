@@ -125,3 +125,17 @@ def test_dermis_is_nearest_membrane_in_list(oracle_mod):
         d = [pdiff(c["x"][i], c["x"][o], L) ** 2 + pdiff(c["y"][i], c["y"][o], L) ** 2
              + (c["z"][i] - c["z"][o]) ** 2 for o in mem]
         assert c["dermis"][i] == mem[int(np.argmin(d))]
+
+
+def test_dense_sheet_has_long_rows(oracle_mod):
+    """The dense-sheet tissue that the long-row force test runs has lists
+    beyond the force kernel's 40 slots from its first build on."""
+    from epi_amd import configs
+    spec = configs.config_dense_sheet(11)
+    ot = oracle_mod.OracleTissue(spec["geometry"])
+    ot.load(spec)
+    ot.set_rng(oracle_mod.mt19937_64_uniform(1, 100000))
+    ot.connect_cell()
+    ln = np.diff(ot.get_cells()["conn_ptr"])
+    assert ot.error() == 0
+    assert (ln > 40).sum() >= 10
