@@ -341,13 +341,18 @@ __device__ void forces_serial(const Tis& t, int pb, int cb, int X, Status* st) {
 // Each group takes one cell, so the grid must cover all cells (a launch that
 // does not is reported as ERR_INVALID); a loop over cells would keep the
 // math's loop-invariant constants in registers and cost occupancy.
+//
+// FCAP, the LDS slots per cell, is chosen per launch from the longest list
+// built so far: 40 keeps a 128-lane block's LDS small enough for 4 waves per
+// SIMD, 48 and 64 take longer lists at lower occupancy, and rows beyond 64 go
+// to k_forces_long.
 constexpr int FG = 16;
-constexpr int FCAP = FORCES_ROW_CAP;   // 40: an 8-cell block fits 8 per CU in LDS
 constexpr int F_BLOCK = 128;
 constexpr int F_GROUPS = F_BLOCK / FG;
 
 // a group's LDS slots, padded by 8 bytes so that the leaders of a wave's
 // groups read different banks
+template <int FCAP>
 struct FRow {
   P3 v[FCAP];
   double pad;
@@ -356,11 +361,12 @@ struct FRow {
 __device__ __forceinline__ void p3_add(P3& a, const P3& d) { a.x += d.x; a.y += d.y; a.z += d.z; }
 __device__ __forceinline__ void p3_sub(P3& a, const P3& d) { a.x -= d.x; a.y -= d.y; a.z -= d.z; }
 
+template <int FCAP>
 __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, int pb, int cb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
-  __shared__ FRow s_own[F_GROUPS];
-  __shared__ FRow s_oth[F_GROUPS];
+  __shared__ FRow<FCAP> s_own[F_GROUPS];
+  __shared__ FRow<FCAP> s_oth[F_GROUPS];
   __shared__ int s_idx[F_GROUPS][FCAP];
   __shared__ uint8_t s_flag[F_GROUPS][FCAP];
   __shared__ uint8_t s_perm[F_GROUPS][FCAP];
@@ -519,22 +525,32 @@ __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, i
   t.pos[1 - pb][2][X] = acc.z;
 }
 
-// cells whose rows exceed the LDS slots of k_forces, one lane per cell
-__global__ void __launch_bounds__(256) k_forces_long(const Tis* __restrict__ T, int pb, int cb) {
+// cells whose rows exceed the LDS slots (cap) of k_forces, one lane per cell
+__global__ void __launch_bounds__(256) k_forces_long(const Tis* __restrict__ T, int pb, int cb, int cap) {
   const Tis& t = T[blockIdx.y];
   const int n = t.st->n;
   for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x)
-    if (t.nlen[cb][X] > FCAP) forces_serial(t, pb, cb, X, t.st);
+    if (t.nlen[cb][X] > cap) forces_serial(t, pb, cb, X, t.st);
 }
 
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other,
                    int max_len) {
   (void)max_memb;
   dim3 g((max_other + F_GROUPS - 1) / F_GROUPS, ntis);
-  EPI_LAUNCH(K_FORCES, k_forces, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
-  if (max_len > FCAP) {
+  int cap;
+  if (max_len <= 40) {
+    cap = 40;
+    EPI_LAUNCH(K_FORCES, k_forces<40>, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
+  } else if (max_len <= 48) {
+    cap = 48;
+    EPI_LAUNCH(K_FORCES, k_forces<48>, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
+  } else {
+    cap = FORCES_ROW_CAP;
+    EPI_LAUNCH(K_FORCES, k_forces<FORCES_ROW_CAP>, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
+  }
+  if (max_len > cap) {
     dim3 gl((max_other + 255) / 256, ntis);
-    EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb);
+    EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb, cap);
   }
 }
 

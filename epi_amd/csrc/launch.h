@@ -51,9 +51,10 @@ void prof_add(int k, hipEvent_t a, hipEvent_t b, bool counted);
     }                                                                                         \
   } while (0)
 
-// Rows of up to FORCES_ROW_CAP entries go to the cooperative kernel, longer
-// ones to a one-lane-per-cell kernel, launched only when max_len exceeds the cap.
-constexpr int FORCES_ROW_CAP = 40;
+// Rows of up to FORCES_ROW_CAP entries go to the cooperative kernel (whose
+// LDS slots per cell, 40, 48 or 64, follow max_len), longer ones to a
+// one-lane-per-cell kernel, launched only when max_len exceeds the slots.
+constexpr int FORCES_ROW_CAP = 64;
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other, int max_len);
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n);
 void launch_initialize_sc(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int pb, int max_n);
