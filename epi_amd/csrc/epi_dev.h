@@ -106,6 +106,10 @@ struct Tis {
   // per list entry during a solve: neighbour << 2 | kind (0 skip, 1 coupled,
   // 3 coupled ALIVE), one word for the index and the kind
   int* ca_ent;
+  // the coupled entries alone (kind 1 or 3), in list order: (word, slot of
+  // the entry in the row, where its gj lives), and their number per cell
+  int2* ca_ce;
+  int* ca_cn;
   // (ca2p, IP3) of buffer b beside the two arrays during a solve, so that a
   // list entry reads its neighbour's pair with one 16-byte gather; every
   // kernel that writes ca2p or IP3 inside a solve or window writes it too

@@ -218,6 +218,8 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.ca_kpa = cv.take<double>(cap);
   t.ca_iag = cv.take<double>(cap);
   t.ca_ent = cv.take<int>(cap * ns);
+  t.ca_ce = cv.take<int2>(cap * ns);
+  t.ca_cn = cv.take<int>(cap);
   for (int b = 0; b < 2; b++) t.ca_up[b] = cv.take<double2>(cap);
   t.ca_dfu = cv.take<double>(cap);
   t.ca_cq = cv.take<uint2>(cap * 8);

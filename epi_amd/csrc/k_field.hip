@@ -224,11 +224,16 @@ __global__ void __launch_bounds__(256) k_ca_begin(const Tis* __restrict__ T, con
       // which entries take part in the gap-junction sums (supra_calc
       // :201-209; dead_IP3_calc :160-165 counts the ALIVE ones)
       int* ent = t.ca_ent + (size_t)c * t.ns;
+      int2* ce = t.ca_ce + (size_t)c * t.ns;
+      int k = 0;
       for (int e = 0; e < len; e++) {
         const int o = row[e];
         const uint8_t so = t.state[o];
-        ent[e] = (o << 2) | ((so == ALIVE) ? 3 : (so == DEAD || so == FIX || so == MUSUME) ? 1 : 0);
+        const int w = (o << 2) | ((so == ALIVE) ? 3 : (so == DEAD || so == FIX || so == MUSUME) ? 1 : 0);
+        ent[e] = w;
+        if (w & 3) ce[k++] = make_int2(w, e);
       }
+      t.ca_cn[c] = k;
     }
     if (c >= start && is_afm(s)) {
       if (zero_avg) t.ca2p_avg[c] = 0;
@@ -404,20 +409,18 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
   if (c >= n) return;
   {
     const uint8_t s = t.state[c];
-    const int len = t.nlen[cb][c];
+    // the coupled entries only, in list order: an entry that is skipped
+    // would add +0.0 to the sums, which leaves them as they are
+    const int len = t.ca_cn[c];
     const double2 uc = up[c];
     const double u = uc.x, p = uc.y;
-    // the first chunk's list words and gj do not wait for len: a row holds ns
-    // >= CCAP slots, and slots past len are masked once len has arrived
-    const int* ent = t.ca_ent + (size_t)c * t.ns;
+    // the first chunk's entries do not wait for len: a row holds ns >= CCAP
+    // slots, and slots past len are masked once len has arrived
+    const int2* ent = t.ca_ce + (size_t)c * t.ns;
     double* g = t.gj[cb] + (size_t)c * t.ns;
-    int en0[EPL];
-    double gv0[EPL];
+    int2 en0[EPL];
 #pragma unroll
-    for (int i = 0; i < EPL; i++) {
-      en0[i] = ent[lane + i * CG];
-      gv0[i] = g[lane + i * CG];
-    }
+    for (int i = 0; i < EPL; i++) en0[i] = ent[lane + i * CG];
     // the lattice position is fetched now too, so the corner gathers start
     // one round trip earlier
     const int lx = t.lat[3 * c], ly = t.lat[3 * c + 1], lz = t.lat[3 * c + 2];
@@ -440,7 +443,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
 #pragma unroll
         for (int i = 0; i < EPL; i++) {
           const int e = lane + i * CG;
-          en[i] = e < m ? (e0 == 0 ? en0[i] : ent[e0 + e]) : 0;
+          en[i] = e < m ? (e0 == 0 ? en0[i] : ent[e0 + e]).x : 0;
         }
 #pragma unroll
         for (int i = 0; i < EPL; i++) po[i] = (en[i] & 3) == 3 ? up[en[i] >> 2].y : 0.0;
@@ -500,18 +503,21 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
     double tmp_diffu = 0, tmp_IP3 = 0;
     for (int e0 = 0; e0 < len; e0 += CCAP) {
       const int m = min(CCAP, len - e0);
-      int en[EPL];
+      int en[EPL], sl[EPL];
       double gv[EPL];
       double2 w[EPL];
 #pragma unroll
       for (int i = 0; i < EPL; i++) {
         const int e = lane + i * CG;
-        const bool in = e < m;
-        en[i] = in ? (e0 == 0 ? en0[i] : ent[e0 + e]) : 0;
-        gv[i] = in ? (e0 == 0 ? gv0[i] : g[e0 + e]) : 0.0;
+        const int2 x = e < m ? (e0 == 0 ? en0[i] : ent[e0 + e]) : make_int2(0, 0);
+        en[i] = x.x;
+        sl[i] = x.y;
       }
 #pragma unroll
-      for (int i = 0; i < EPL; i++) w[i] = (en[i] & 3) ? up[en[i] >> 2] : make_double2(0.0, 0.0);
+      for (int i = 0; i < EPL; i++) {
+        gv[i] = (en[i] & 3) ? g[sl[i]] : 0.0;
+        w[i] = (en[i] & 3) ? up[en[i] >> 2] : make_double2(0.0, 0.0);
+      }
 #pragma unroll
       for (int i = 0; i < EPL; i++) {
         const int e = lane + i * CG;
@@ -521,7 +527,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           if (kd) {
             a1 = gv[i] * (w[i].x - u);
             a2 = gv[i] * (w[i].y - p);
-            if (kd == 3) g[e0 + e] = gv[i] + DT_Ca * fw_term(fabs(w[i].x - u), gv[i]);
+            if (kd == 3) g[sl[i]] = gv[i] + DT_Ca * fw_term(fabs(w[i].x - u), gv[i]);
           }
           s_a[grp][e] = make_double2(a1, a2);
         }
