@@ -654,8 +654,8 @@ __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const in
 
 // one launch for the ATP update of sub-step i (parities cp, ap) and the cells
 // of sub-step i + 1: blocks [0, ncb) take the cells, the rest the grid
-// (gx x-blocks per z-block, 4 rows of 64 points per block)
-constexpr int F_BLOCK = 256;
+// (gx x-blocks per z-block, 2 rows of 64 points per block)
+constexpr int F_BLOCK = 128;
 
 template <int CG, int ZB>
 __global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
@@ -777,7 +777,8 @@ template <int CG>
 void launch_fused_cg(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
                      int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
   const int ncb = (max_n + F_BLOCK / CG - 1) / (F_BLOCK / CG);
-  const int gx = (max_nx + 63) / 64, nky = (max_ny + 3) / 4;
+  constexpr int F_ROWS = F_BLOCK / 64;
+  const int gx = (max_nx + 63) / 64, nky = (max_ny + F_ROWS - 1) / F_ROWS;
   by_zb(tu.a_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
     const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
