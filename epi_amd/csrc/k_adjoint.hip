@@ -164,7 +164,8 @@ __global__ void __launch_bounds__(256) k_rd_seed(const Tis* __restrict__ T, RdTi
 
 // reverse of dead_IP3_calc and supra_calc for one sub-step, the state before
 // the sub-step from slot `slot`.  A group of RG lanes (8 or 16, a launch
-// shape) owns one cell: lanes take the list entries, their partial sums are
+// shape) owns one cell: lanes take the coupled list entries (the per-solve
+// compact (word, slot) list; gj and its adjoint live at the slot), their partial sums are
 // reduced over the group (a gradient need not follow the forward's summation
 // order), and each lane scatters its entries' contributions.  A lane keeps
 // its first 32 / RG entries (and their neighbour values) in registers for
@@ -193,9 +194,11 @@ __device__ __forceinline__ void rd_entry_bwd(const RdEntry& k, int kd, int o, do
   double to_u = k.abar_du_iag * gv;
   double lge = lg_in;
   if (kd == 3) {
+    // d fw / d |d| = -(1 - tanh^2 y) / (2 * 0.1) with y = (0.1 - |d|) / 0.1,
+    // and 1 - tanh^2 y = 4 E / (1 + E)^2 with E = e^(2y) (<= e^2, as |d| >= 0)
     const double d = uo - k.uc;
-    const double tz = tanh((0.1 - fabs(d)) / 0.1);
-    const double fwd = -5.0 * (1.0 - tz * tz);
+    const double E = exp(2.0 * ((0.1 - fabs(d)) / 0.1));
+    const double fwd = -5.0 * (4.0 * E / ((1.0 + E) * (1.0 + E)));
     const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
     to_u += lg_in * DT_Ca * fwd * sg;
     own_u -= lg_in * DT_Ca * fwd * sg;
@@ -232,15 +235,16 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
   double* const lh_w = r.lh[q];
   for (int c = start + bid * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
     const uint8_t s = t.state[c];
-    const int len = t.nlen[cb][c];
+    // the coupled entries only, (word, slot of the entry in the row)
+    const int len = t.ca_cn[c];
     const double lpc = lp[c];
     const double uc = u[c], pc = p[c];
-    const int* ent = t.ca_ent + (size_t)c * ns;
+    const int2* ent = t.ca_ce + (size_t)c * ns;
     if (s == DEAD) {
       double cnt = 0, sum = 0;
       for (int e = lane; e < len; e += RG)
-        if ((ent[e] & 3) == 3) {
-          const int o = ent[e] >> 2;
+        if ((ent[e].x & 3) == 3) {
+          const int o = ent[e].x >> 2;
           sum += p[o];
           cnt += 1;
           atomicAdd(&np[o], lpc * DT_Ca * dp);
@@ -261,21 +265,21 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
     // the lane's first REP entries: packed neighbour and kind, gj, edge
     // adjoint, then the neighbours' state, Ca2+ and IP3, all in flight together
     const int m0 = min(len, RG * REP);
-    int o[REP];
+    int o[REP], sl[REP];
     uint8_t kd[REP], so[REP];
     double gv[REP], lgv[REP], uo[REP], po[REP];
 #pragma unroll
     for (int i = 0; i < REP; i++) {
       const int e = lane + i * RG;
-      const bool in = e < m0;
-      const int en = in ? ent[e] : 0;
-      o[i] = en >> 2;
-      kd[i] = (uint8_t)(en & 3);
-      gv[i] = in ? gr[e] : 0.0;
-      lgv[i] = in ? lgr[e] : 0.0;
+      const int2 en = e < m0 ? ent[e] : make_int2(0, 0);
+      o[i] = en.x >> 2;
+      kd[i] = (uint8_t)(en.x & 3);
+      sl[i] = en.y;
     }
 #pragma unroll
     for (int i = 0; i < REP; i++) {
+      gv[i] = kd[i] ? gr[sl[i]] : 0.0;
+      lgv[i] = kd[i] ? lgr[sl[i]] : 0.0;
       uo[i] = kd[i] ? u[o[i]] : 0.0;
       po[i] = kd[i] ? p[o[i]] : 0.0;
       so[i] = kd[i] ? t.state[o[i]] : (uint8_t)0;
@@ -299,11 +303,12 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
         gsum += gv[i];
       }
     for (int e = RG * REP + lane; e < len; e += RG) {
-      if (!(ent[e] & 3)) continue;
-      const int oe = ent[e] >> 2;
-      su += gr[e] * (u[oe] - uc);
-      sp += gr[e] * (p[oe] - pc);
-      gsum += gr[e];
+      const int2 en = ent[e];
+      const int oe = en.x >> 2;
+      const double ge = gr[en.y];
+      su += ge * (u[oe] - uc);
+      sp += ge * (p[oe] - pc);
+      gsum += ge;
     }
     su = group_sum<RG>(su);
     sp = group_sum<RG>(sp);
@@ -319,13 +324,13 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
     double own_u = 0;
 #pragma unroll
     for (int i = 0; i < REP; i++)
-      if (kd[i]) rd_entry_bwd(k, kd[i], o[i], gv[i], uo[i], po[i], lgv[i], so[i], &ng[(size_t)c * ns + lane + i * RG],
+      if (kd[i]) rd_entry_bwd(k, kd[i], o[i], gv[i], uo[i], po[i], lgv[i], so[i], &ng[(size_t)c * ns + sl[i]],
                               nu, np, own_u);
     for (int e = RG * REP + lane; e < len; e += RG) {
-      const int kde = ent[e] & 3;
-      if (!kde) continue;
-      const int oe = ent[e] >> 2;
-      rd_entry_bwd(k, kde, oe, gr[e], u[oe], p[oe], lgr[e], t.state[oe], &ng[(size_t)c * ns + e], nu, np, own_u);
+      const int2 en = ent[e];
+      const int oe = en.x >> 2;
+      rd_entry_bwd(k, en.x & 3, oe, gr[en.y], u[oe], p[oe], lgr[en.y], t.state[oe], &ng[(size_t)c * ns + en.y], nu,
+                   np, own_u);
     }
     own_u = group_sum<RG>(own_u);
     if (lane < 8) {   // the eight grid_avg8 corners of the IP3 production term

@@ -826,6 +826,12 @@ __global__ void __launch_bounds__(256) k_mech_bwd_begin(const Tis* __restrict__ 
     }
 }
 
+// The reverse kernels give each cell a group of MG lanes: the lanes take the
+// cell's list entries (or its membrane slots and dihedral terms) side by
+// side.  Their scatters are atomic and the parameter sums are reduced at the
+// end, so no order is kept (a gradient need not follow the forward's).
+constexpr int MG = 16;
+
 // every interaction a cell's handler applies (cell_interaction.cpp:529-641)
 __global__ void __launch_bounds__(256) k_mech_bwd_pairs(const Tis* __restrict__ T, MechTis* __restrict__ M, int pb,
                                                          int cb, int q) {
@@ -835,25 +841,28 @@ __global__ void __launch_bounds__(256) k_mech_bwd_pairs(const Tis* __restrict__ 
   const long long cap = t.cap;
   const double* lam = m.lam[q];
   double* nl = m.lam[1 - q];
+  const int lane = threadIdx.x & (MG - 1);
+  const int groups = gridDim.x * (blockDim.x / MG);
   ParAcc pa{0, 0, 0};
-  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
+  for (int X = blockIdx.x * (blockDim.x / MG) + threadIdx.x / MG; X < n; X += groups) {
     const CV cx = load_cv(t, pb, X);
-    const int* row = t.col[cb] + (size_t)X * t.ns;
-    const int len = t.nlen[cb][X];
     if (X < t.nmemb) {
-      for (int i = 0; i < MEMB_ADJ; i++) {
-        const int o = t.memb6[6 * X + i];
+      if (lane < MEMB_ADJ) {
+        const int o = t.memb6[6 * X + lane];
         if (t.state[o] == MEMB && X > o) pair_bwd(t, LAW_MEMB_MEMB, cx, X, load_cv(t, pb, o), o, lam, nl, cap, pa);
       }
       continue;
     }
-    for (int e = 0; e < len; e++) {
+    const int* row = t.col[cb] + (size_t)X * t.ns;
+    const int len = t.nlen[cb][X];
+    for (int e = lane; e < len; e += MG) {
       const int o = row[e];
       const CV co = load_cv(t, pb, o);
       const int law = proc_law(cx, X, co, o);
       if (law != LAW_NONE) pair_bwd(t, law, cx, X, co, o, lam, nl, cap, pa);
     }
-    if (cx.pair >= 0 && X > cx.pair) pair_bwd(t, LAW_PAIR, cx, X, load_cv(t, pb, cx.pair), cx.pair, lam, nl, cap, pa);
+    if (lane == MG - 1 && cx.pair >= 0 && X > cx.pair)
+      pair_bwd(t, LAW_PAIR, cx, X, load_cv(t, pb, cx.pair), cx.pair, lam, nl, cap, pa);
   }
   for (int o = 32; o > 0; o >>= 1) {
     pa.eps += __shfl_down(pa.eps, o, 64);
@@ -878,39 +887,38 @@ __global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T
   const double* lam = m.lam[q];
   double* nl = m.lam[1 - q];
   double peps = 0, pbend = 0;
-  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & (MG - 1);
+  const int groups = gridDim.x * (blockDim.x / MG);
+  for (int X = blockIdx.x * (blockDim.x / MG) + threadIdx.x / MG; X < n; X += groups) {
     const bool memb = X < t.nmemb;
     if (!memb && t.state[X] != DER) continue;
     const CV cx = load_cv(t, pb, X);
-    if (cx.p.z < cx.r) {
+    if (lane == MG - 1 && cx.p.z < cx.r) {
       const double z = cx.p.z;
       const double L = POW6(cx.r) / POW6(z);
       atomicAdd(&nl[2 * cap + X], lam[2 * cap + X] * DT_Cell * 2.0 * t.m_eps * (7.0 * L - 13.0 * L * L) / (z * z));
       peps += lam[2 * cap + X] * DT_Cell * 2.0 * L * (L - 1.0) / z;
     }
-    if (!memb) continue;
+    if (!memb || lane >= 12) continue;
+    // lanes 0-5: tri_bend_1 term i = lane; lanes 6-11: tri_bend_2 term lane - 6
     const P3 lm = ld3(lam, cap, X);
     const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
     const int* md = t.memb6 + 6 * X;
-    P3 dr{0, 0, 0};
-    for (int i = 0; i < 6; i++) {
-      const int i1 = (i + 1) % 6;
-      const int idx[4] = {X, md[i], md[i1], t.memb6[6 * md[i1] + i]};
-      const P3 pr = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
-      dr = v_add(dr, tri_bend_1(t, cx.p, pr, pbv, pc));
-      P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      tri_bend_1_bar(t, cx.p, pr, pbv, pc, ob, bar);
-      for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
-    }
-    for (int i = 0; i < 6; i++) {
-      const int idx[4] = {X, md[i], md[(i + 1) % 6], md[(i + 2) % 6]};
-      const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
-      dr = v_add(dr, tri_bend_2(t, cx.p, pa, pbv, pc));
-      P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const int i = lane < 6 ? lane : lane - 6;
+    const int i1 = (i + 1) % 6;
+    const int idx[4] = {X, md[i], md[i1], lane < 6 ? t.memb6[6 * md[i1] + i] : md[(i + 2) % 6]};
+    const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
+    P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    P3 term;
+    if (lane < 6) {
+      term = tri_bend_1(t, cx.p, pa, pbv, pc);
+      tri_bend_1_bar(t, cx.p, pa, pbv, pc, ob, bar);
+    } else {
+      term = tri_bend_2(t, cx.p, pa, pbv, pc);
       tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
-      for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
     }
-    pbend += DT_Cell * v_dot(lm, dr);
+    for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
+    pbend += DT_Cell * v_dot(lm, term);
   }
   for (int o = 32; o > 0; o >>= 1) {
     peps += __shfl_down(peps, o, 64);
@@ -932,9 +940,10 @@ void launch_mech_seed(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int p
 void launch_mech_backward_step(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int slot, int pb, int cb, int q,
                                int max_n) {
   const dim3 g((max_n + 255) / 256, ntis);
+  const dim3 gg((max_n + 256 / MG - 1) / (256 / MG), ntis);
   EPI_LAUNCH(K_MECH_BWD_BEGIN, k_mech_bwd_begin, g, dim3(256), 0, s, dT, dM, slot, pb, q);
-  EPI_LAUNCH(K_MECH_BWD_PAIRS, k_mech_bwd_pairs, g, dim3(256), 0, s, dT, dM, pb, cb, q);
-  EPI_LAUNCH(K_MECH_BWD_MEMB, k_mech_bwd_memb, g, dim3(256), 0, s, dT, dM, pb, q);
+  EPI_LAUNCH(K_MECH_BWD_PAIRS, k_mech_bwd_pairs, gg, dim3(256), 0, s, dT, dM, pb, cb, q);
+  EPI_LAUNCH(K_MECH_BWD_MEMB, k_mech_bwd_memb, gg, dim3(256), 0, s, dT, dM, pb, q);
 }
 
 }  // namespace epi
