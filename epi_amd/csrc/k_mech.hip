@@ -677,8 +677,10 @@ __device__ __forceinline__ P3 ld3(const double* lam, long long cap, int c) {
 
 struct ParAcc { double eps, adh, spr; };
 
-__device__ __forceinline__ void pair_bwd(const Tis& t, int law, const CV& a, int ia, const CV& b, int ib,
-                                         const double* lam, double* nl, long long cap, ParAcc& pa) {
+// scatters the pair's term to ib and returns the one for ia (the caller's
+// cell, whose lanes reduce their terms before one atomic)
+__device__ __forceinline__ P3 pair_bwd(const Tis& t, int law, const CV& a, int ia, const CV& b, int ib,
+                                       const double* lam, double* nl, long long cap, ParAcc& pa) {
   const LawD ld = coef_law_d(t, law, a, b);
   const P3 r{p_diff_x(t, a.p.x, b.p.x), p_diff_y(t, a.p.y, b.p.y), a.p.z - b.p.z};
   const P3 la = ld3(lam, cap, ia), lb = ld3(lam, cap, ib);
@@ -687,11 +689,23 @@ __device__ __forceinline__ void pair_bwd(const Tis& t, int law, const CV& a, int
   const double k2 = 2 * ld.ds * ar;
   const P3 g{DT_Cell * (ld.coef * av.x + k2 * r.x), DT_Cell * (ld.coef * av.y + k2 * r.y),
              DT_Cell * (ld.coef * av.z + k2 * r.z)};
-  atomic_add3(nl, cap, ia, g);
   atomic_add3(nl, cap, ib, P3{-g.x, -g.y, -g.z});
   pa.eps += DT_Cell * ld.deps * ar;
   pa.adh += DT_Cell * ld.dadh * ar;
   pa.spr += DT_Cell * ld.dspr * ar;
+  return g;
+}
+
+// sum over the MG lanes of a group (every lane gets it)
+template <int G>
+__device__ __forceinline__ P3 group_sum3(P3 v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    v.x += __shfl_xor(v.x, o, G);
+    v.y += __shfl_xor(v.y, o, G);
+    v.z += __shfl_xor(v.z, o, G);
+  }
+  return v;
 }
 
 // adjoint of n = N / |N|
@@ -828,8 +842,10 @@ __global__ void __launch_bounds__(256) k_mech_bwd_begin(const Tis* __restrict__ 
 
 // The reverse kernels give each cell a group of MG lanes: the lanes take the
 // cell's list entries (or its membrane slots and dihedral terms) side by
-// side.  Their scatters are atomic and the parameter sums are reduced at the
-// end, so no order is kept (a gradient need not follow the forward's).
+// side.  The cell's own term is summed over the group and added by one lane;
+// the terms of other cells are scattered with atomics and the parameter sums
+// are reduced at the end, so no order is kept (a gradient need not follow the
+// forward's).
 constexpr int MG = 16;
 
 // every interaction a cell's handler applies (cell_interaction.cpp:529-641)
@@ -846,23 +862,27 @@ __global__ void __launch_bounds__(256) k_mech_bwd_pairs(const Tis* __restrict__ 
   ParAcc pa{0, 0, 0};
   for (int X = blockIdx.x * (blockDim.x / MG) + threadIdx.x / MG; X < n; X += groups) {
     const CV cx = load_cv(t, pb, X);
+    P3 gx{0, 0, 0};
     if (X < t.nmemb) {
       if (lane < MEMB_ADJ) {
         const int o = t.memb6[6 * X + lane];
-        if (t.state[o] == MEMB && X > o) pair_bwd(t, LAW_MEMB_MEMB, cx, X, load_cv(t, pb, o), o, lam, nl, cap, pa);
+        if (t.state[o] == MEMB && X > o)
+          gx = pair_bwd(t, LAW_MEMB_MEMB, cx, X, load_cv(t, pb, o), o, lam, nl, cap, pa);
       }
-      continue;
+    } else {
+      const int* row = t.col[cb] + (size_t)X * t.ns;
+      const int len = t.nlen[cb][X];
+      for (int e = lane; e < len; e += MG) {
+        const int o = row[e];
+        const CV co = load_cv(t, pb, o);
+        const int law = proc_law(cx, X, co, o);
+        if (law != LAW_NONE) gx = v_add(gx, pair_bwd(t, law, cx, X, co, o, lam, nl, cap, pa));
+      }
+      if (lane == MG - 1 && cx.pair >= 0 && X > cx.pair)
+        gx = v_add(gx, pair_bwd(t, LAW_PAIR, cx, X, load_cv(t, pb, cx.pair), cx.pair, lam, nl, cap, pa));
     }
-    const int* row = t.col[cb] + (size_t)X * t.ns;
-    const int len = t.nlen[cb][X];
-    for (int e = lane; e < len; e += MG) {
-      const int o = row[e];
-      const CV co = load_cv(t, pb, o);
-      const int law = proc_law(cx, X, co, o);
-      if (law != LAW_NONE) pair_bwd(t, law, cx, X, co, o, lam, nl, cap, pa);
-    }
-    if (lane == MG - 1 && cx.pair >= 0 && X > cx.pair)
-      pair_bwd(t, LAW_PAIR, cx, X, load_cv(t, pb, cx.pair), cx.pair, lam, nl, cap, pa);
+    gx = group_sum3<MG>(gx);
+    if (lane == 0) atomic_add3(nl, cap, X, gx);
   }
   for (int o = 32; o > 0; o >>= 1) {
     pa.eps += __shfl_down(pa.eps, o, 64);
@@ -899,26 +919,30 @@ __global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T
       atomicAdd(&nl[2 * cap + X], lam[2 * cap + X] * DT_Cell * 2.0 * t.m_eps * (7.0 * L - 13.0 * L * L) / (z * z));
       peps += lam[2 * cap + X] * DT_Cell * 2.0 * L * (L - 1.0) / z;
     }
-    if (!memb || lane >= 12) continue;
+    if (!memb) continue;
     // lanes 0-5: tri_bend_1 term i = lane; lanes 6-11: tri_bend_2 term lane - 6
-    const P3 lm = ld3(lam, cap, X);
-    const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
-    const int* md = t.memb6 + 6 * X;
-    const int i = lane < 6 ? lane : lane - 6;
-    const int i1 = (i + 1) % 6;
-    const int idx[4] = {X, md[i], md[i1], lane < 6 ? t.memb6[6 * md[i1] + i] : md[(i + 2) % 6]};
-    const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
     P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    P3 term;
-    if (lane < 6) {
-      term = tri_bend_1(t, cx.p, pa, pbv, pc);
-      tri_bend_1_bar(t, cx.p, pa, pbv, pc, ob, bar);
-    } else {
-      term = tri_bend_2(t, cx.p, pa, pbv, pc);
-      tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
+    if (lane < 12) {
+      const P3 lm = ld3(lam, cap, X);
+      const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
+      const int* md = t.memb6 + 6 * X;
+      const int i = lane < 6 ? lane : lane - 6;
+      const int i1 = (i + 1) % 6;
+      const int idx[4] = {X, md[i], md[i1], lane < 6 ? t.memb6[6 * md[i1] + i] : md[(i + 2) % 6]};
+      const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
+      P3 term;
+      if (lane < 6) {
+        term = tri_bend_1(t, cx.p, pa, pbv, pc);
+        tri_bend_1_bar(t, cx.p, pa, pbv, pc, ob, bar);
+      } else {
+        term = tri_bend_2(t, cx.p, pa, pbv, pc);
+        tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
+      }
+      for (int k = 1; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
+      pbend += DT_Cell * v_dot(lm, term);
     }
-    for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
-    pbend += DT_Cell * v_dot(lm, term);
+    bar[0] = group_sum3<MG>(bar[0]);
+    if (lane == 0) atomic_add3(nl, cap, X, bar[0]);
   }
   for (int o = 32; o > 0; o >>= 1) {
     peps += __shfl_down(peps, o, 64);
