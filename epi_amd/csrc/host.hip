@@ -1302,11 +1302,13 @@ int rd_params(epi_ctx* c, const epi_rd_params* prm) {
   return EPI_OK;
 }
 
-int rd_substep(epi_ctx* c) {
+// one window sub-step; with sv the state before it is saved into segment
+// slot `slot` by the cell kernel (cells_only: only that, no ATP update)
+int rd_substep(epi_ctx* c, const RdTis* sv = nullptr, int slot = 0, bool cells_only = false) {
   int izb = 1;
   for (int v : c->rd.izb) izb = std::max(izb, v + 1);
   launch_ca_substep(c->stream, c->d_tis, nullptr, c->T, c->cb, c->cp, c->ap, c->bp, max_n(c), max_nx(c), max_ny(c),
-                    izb, c->tu);
+                    izb, c->tu, sv, slot, cells_only);
   c->cp ^= 1;
   c->ap ^= 1;
   return EPI_OK;
@@ -1436,7 +1438,7 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
     RdTis& r = rd.h[t];
     const size_t cap = c->th[t].cap, ns = c->th[t].ns, box = (size_t)r.box;
     r.su = cv.take<double>(K * cap); r.sp = cv.take<double>(K * cap); r.sh = cv.take<double>(K * cap);
-    r.sg = cv.take<double>(K * cap * ns); r.sA = cv.take<double>(K * box);
+    r.sg = cv.take<double>(K * cap * ns); r.sAv = cv.take<double>(K * cap);
     r.cu = cv.take<double>(2 * nck * cap); r.cp_ = cv.take<double>(2 * nck * cap); r.ch = cv.take<double>(nck * cap);
     r.cg = cv.take<double>(nck * cap * ns); r.cA = cv.take<double>(2 * nck * box);
     HIPCHK(c, hipMemcpyAsync(r.target, targets[t], 8 * (size_t)c->h_st[t].n, hipMemcpyHostToDevice, c->stream));
@@ -1452,8 +1454,10 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
   const int last0 = (nck - 1) * K;
   for (int s = 0; s < n_sub; s++) {
     if (s >= last0) {
-      launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s - last0, 0, c->cb, c->cp, c->ap, items);
-    } else if (s % K == 0) {
+      RCHK(rd_substep(c, rd.d, s - last0));
+      continue;
+    }
+    if (s % K == 0) {
       ck_cp[s / K] = c->cp;
       ck_ap[s / K] = c->ap;
       launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s / K, 2, c->cb, c->cp, c->ap, items);
@@ -1470,10 +1474,10 @@ int epi_rd_adjoint(epi_ctx* c, const epi_rd_params* prm, int32_t n_sub, int32_t 
       c->cp = ck_cp[k];
       c->ap = ck_ap[k];
       launch_rd_state(c->stream, c->d_tis, rd.d, c->T, k, 3, c->cb, c->cp, c->ap, items);
-      for (int s = s0; s < s1; s++) {
-        launch_rd_state(c->stream, c->d_tis, rd.d, c->T, s - s0, 0, c->cb, c->cp, c->ap, items);
-        RCHK(rd_substep(c));
-      }
+      // the states before sub-steps s0 .. s1 - 1, saved by the sub-steps'
+      // cell kernels; the state after s1 - 1 is the next checkpoint, so the
+      // last sub-step runs its cell kernel only, for the save
+      for (int s = s0; s < s1; s++) RCHK(rd_substep(c, rd.d, s - s0, s + 1 == s1));
     }
     for (int s = s1 - 1; s >= s0; s--) {
       launch_rd_backward_substep(c->stream, c->d_tis, rd.d, c->T, s - s0, c->cb, c->bp, q, qa, mn, mb, c->tu.rd_group);
@@ -1640,9 +1644,11 @@ int epi_mech_adjoint(epi_ctx* c, const epi_mech_params* prm, int32_t n_steps, in
     if (k < nck - 1) {
       c->pb = ck_pb[k];
       launch_mech_state(c->stream, c->d_tis, m.d, c->T, k, 3, c->pb, mn);
+      // the states before steps s0 .. s1 - 1; the one after s1 - 1 is the
+      // next checkpoint and is not recomputed
       for (int s = s0; s < s1; s++) {
         launch_mech_state(c->stream, c->d_tis, m.d, c->T, s - s0, 0, c->pb, mn);
-        RCHK(mech_step(c));
+        if (s + 1 < s1) RCHK(mech_step(c));
       }
     }
     for (int s = s1 - 1; s >= s0; s--) {

@@ -96,7 +96,9 @@ __global__ void __launch_bounds__(256) k_rd_put_gj(const Tis* __restrict__ T, Rd
 }
 
 // copy the state between the tissue and a segment slot (dir 0: save,
-// 1: restore) or a checkpoint (both buffers; dir 2: save, 3: restore)
+// 1: restore; of the ATP grid a slot keeps only the cells' grid_avg8, so a
+// restore from it leaves the grid as it is) or a checkpoint (both buffers;
+// dir 2: save, 3: restore)
 __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdTis* __restrict__ R, int slot,
                                                    int dir, int cb, int cp, int ap) {
   const Tis& t = T[blockIdx.y];
@@ -111,10 +113,22 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
     else t.gj[cb][i] = gs[i];
   }
   const int nb = ck ? 2 : 1;
+  const int start = t.nmemb + t.nder;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
     double* hs = (ck ? r.ch : r.sh) + slot * cap;
     if (save) hs[c] = t.ex_inert[c];
     else t.ex_inert[c] = hs[c];
+    if (!ck && save) {
+      // grid_avg8 of the ATP buffer at the cell's lattice corner
+      double av = 0;
+      if (c >= start && is_afm(t.state[c])) {
+        const double* A = t.atp[ap];
+        const long long b0 = gidx(t, t.lat[3 * c], t.lat[3 * c + 1], t.lat[3 * c + 2]);
+        av = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
+                      + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
+      }
+      r.sAv[slot * cap + c] = av;
+    }
     for (int b = 0; b < nb; b++) {
       const int buf = b ? 1 - cp : cp;
       double* us = (ck ? r.cu + (2 * slot + b) * cap : r.su + slot * cap);
@@ -123,10 +137,11 @@ __global__ void __launch_bounds__(256) k_rd_state(const Tis* __restrict__ T, RdT
       else { t.ca2p[buf][c] = us[c]; t.ip3[buf][c] = ps[c]; t.ca_up[buf][c] = make_double2(us[c], ps[c]); }
     }
   }
+  if (!ck) return;   // a segment state keeps grid_avg8 per cell, not the grid
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < r.box; v += stride) {
     for (int b = 0; b < nb; b++) {
       const int buf = b ? 1 - ap : ap;
-      double* as = (ck ? r.cA + (2 * slot + b) * r.box : r.sA + slot * r.box);
+      double* as = r.cA + (2 * slot + b) * r.box;
       if (save) as[v] = t.atp[buf][v];
       else t.atp[buf][v] = as[v];
     }
@@ -223,7 +238,6 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
   const double* p = r.sp + slot * cap;
   const double* h = r.sh + slot * cap;
   const double* g = r.sg + slot * cap * ns;
-  const double* A = r.sA + slot * r.box;
   const double *lu = r.lu[q], *lp = r.lp[q], *lh = r.lh[q], *lg = r.lg[q];
   double *nu = r.lu[1 - q], *np = r.lp[1 - q], *nh = r.lh[1 - q], *ng = r.lg[1 - q];
   const double* lA = r.lA[qa];
@@ -233,7 +247,12 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
   double* const lu_w = r.lu[q];
   double* const lp_w = r.lp[q];
   double* const lh_w = r.lh[q];
-  for (int c = start + bid * (blockDim.x / RG) + threadIdx.x / RG; c < n; c += groups) {
+  // one cell per group (the grid covers every cell past the membrane and
+  // dermis; a launch that does not is reported as ERR_INVALID): a loop over
+  // cells would hold the math's loop-invariant constants in registers
+  if (bid == 0 && threadIdx.x == 0 && start + (long long)groups < n) set_err(t.st, ERR_INVALID);
+  const int c = start + bid * (blockDim.x / RG) + threadIdx.x / RG;
+  if (c < n) do {   // "continue" below leaves the cell
     const uint8_t s = t.state[c];
     // the coupled entries only, (word, slot of the entry in the row)
     const int len = t.ca_cn[c];
@@ -268,31 +287,49 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
     int o[REP], sl[REP];
     uint8_t kd[REP], so[REP];
     double gv[REP], lgv[REP], uo[REP], po[REP];
+    // every load unconditional (a row holds ns >= 32 slots; a masked entry
+    // reads slot 0 and cell 0), masked with integer masks afterwards, so that
+    // no load sits in a branch that waits for it
+    int2 x[REP];
+#pragma unroll
+    for (int i = 0; i < REP; i++) x[i] = ent[lane + i * RG];
 #pragma unroll
     for (int i = 0; i < REP; i++) {
-      const int e = lane + i * RG;
-      const int2 en = e < m0 ? ent[e] : make_int2(0, 0);
-      o[i] = en.x >> 2;
-      kd[i] = (uint8_t)(en.x & 3);
-      sl[i] = en.y;
+      const int msk = -(int)(lane + i * RG < m0);
+      o[i] = (x[i].x & msk) >> 2;
+      kd[i] = (uint8_t)(x[i].x & msk & 3);
+      sl[i] = x[i].y & msk;
     }
 #pragma unroll
     for (int i = 0; i < REP; i++) {
-      gv[i] = kd[i] ? gr[sl[i]] : 0.0;
-      lgv[i] = kd[i] ? lgr[sl[i]] : 0.0;
-      uo[i] = kd[i] ? u[o[i]] : 0.0;
-      po[i] = kd[i] ? p[o[i]] : 0.0;
-      so[i] = kd[i] ? t.state[o[i]] : (uint8_t)0;
+      gv[i] = gr[sl[i]];
+      lgv[i] = lgr[sl[i]];
+      uo[i] = u[o[i]];
+      po[i] = p[o[i]];
+      so[i] = t.state[o[i]];
     }
     const int ix = t.lat[3 * c], iy = t.lat[3 * c + 1], iz = t.lat[3 * c + 2];
     const double hc = h[c], luc = lu[c], lhc = lh[c];
     const double iag = t.ca_iag[c], kpa = t.ca_kpa[c], th = t.ca_th[c];
     // ATP adjoint of the voxels the cell owns (after the sub-step)
+    // four voxels per lane at a time: their indices, then their values
     double acc_own = 0;
     {
       const int no = r.own_n[c];
       const int* ov = r.own + (size_t)c * r.oc;
-      for (int i = lane; i < no; i += RG) acc_own += lA[ov[i]];
+      for (int i0 = 0; i0 < no; i0 += 4 * RG) {
+        int v[4];
+        double a[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int i = i0 + lane + k * RG;
+          v[k] = i < no ? ov[i] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = v[k] >= 0 ? lA[v[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc_own += a[k];
+      }
     }
     double su = 0, sp = 0, gsum = 0;
 #pragma unroll
@@ -315,9 +352,7 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
     gsum = group_sum<RG>(gsum);
     acc_own = group_sum<RG>(acc_own);
     const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
-    const long long b0 = gidx(t, ix, iy, iz);
-    const double Aavg = 0.125 * (A[b0] + A[b0 + 1] + A[b0 + t.XP] + A[b0 + t.PL] + A[b0 + t.XP + 1] + A[b0 + t.PL + 1]
-                                 + A[b0 + t.PL + t.XP] + A[b0 + t.PL + t.XP + 1]);
+    const double Aavg = r.sAv[slot * cap + c];
     const double diffu = ca2p_reaction_factor(uc, hc, pc, Bavg) + du * iag * su;
     const double abar = DT_Ca * luc + (diffu > 0 ? acc_own * DT_Ca * STIM11 : 0.0);
     const RdEntry k{abar * du * iag, lpc * DT_Ca * dp * iag, uc, pc};
@@ -355,7 +390,7 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
       r.gkpa[c] += lpc * DT_Ca * Aavg / (H0 + Aavg);
       lu_w[c] = 0; lp_w[c] = 0; lh_w[c] = 0;
     }
-  }
+  } while (false);
   for (int o = 32; o > 0; o >>= 1) {
     pdu += __shfl_down(pdu, o, 64);
     pdp += __shfl_down(pdp, o, 64);
@@ -493,7 +528,7 @@ void launch_rd_seed(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cp, i
 }
 void launch_rd_backward_substep(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int slot, int cb, int bp, int q,
                                 int qa, int max_n, long long max_box, int rg) {
-  const int ncb = (int)cells_grid((long long)max_n * rg, ntis).x;
+  const int ncb = (int)(((long long)max_n * rg + 255) / 256);   // one cell per group: no cap
   const dim3 g(ncb + cells_grid(max_box, ntis).x, ntis);
   if (rg == 16)
     EPI_LAUNCH(K_RD_STEP, k_rd_step<16>, g, dim3(256), 0, s, dT, dR, slot, cb, bp, q, qa, ncb);

@@ -14,6 +14,7 @@
 #include "launch.h"
 #include "epi_group.h"
 #include "ca_laws.h"
+#include "rd_dev.h"
 
 namespace epi {
 
@@ -381,12 +382,17 @@ __device__ __forceinline__ double atp_point_pre(const Tis& t, const double* __re
 // Ca2+ and IP3 pair) before it uses any of them.  With carr_prev set, the ATP
 // corners are not read but computed from the previous sub-step's buffer
 // (atp_point_pre), the ATP update of that sub-step running beside this one.
+// With sv set (a reaction-diffusion window's reverse pass), the cell also
+// writes what the reverse sub-step reads of the state before it into segment
+// slot `slot`: ca2p and IP3 (AFM and DEAD cells), ex_inert, the ATP
+// grid_avg8 and the gj of its coupled entries (AFM cells).
 constexpr int C_BLOCK = 128;
 
 template <int CG, int NT>
 __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigned nbx, int cb, int cp, int ap, int bp,
                                               const double* __restrict__ carr_prev,
-                                              const double* __restrict__ dfu_prev) {
+                                              const double* __restrict__ dfu_prev,
+                                              const RdTis* __restrict__ sv = nullptr, int slot = 0) {
   constexpr int C_GROUPS = NT / CG;
   constexpr int EPL = CG >= 8 ? 2 : 4;   // entries per lane per chunk
   constexpr int CCAP = EPL * CG;    // list entries per chunk
@@ -469,6 +475,10 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
         wave_sync();
       }
       if (lane == 0) {
+        if (sv) {
+          sv->su[slot * (size_t)t.cap + c] = u;
+          sv->sp[slot * (size_t)t.cap + c] = p;
+        }
         const double np = p + DT_Ca * (t.ca_dp * (tmp - count * p) - Kpp * p);
         t.ip3[1 - cp][c] = np;
         reinterpret_cast<double*>(up_next + c)[1] = np;   // ca2p of a DEAD cell stays as it is
@@ -525,6 +535,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           const int kd = en[i] & 3;
           double a1 = 0, a2 = 0;
           if (kd) {
+            if (sv) sv->sg[(slot * (size_t)t.cap + c) * t.ns + sl[i]] = gv[i];
             a1 = gv[i] * (w[i].x - u);
             a2 = gv[i] * (w[i].y - p);
             if (kd == 3) g[sl[i]] = gv[i] + DT_Ca * fw_term(fabs(w[i].x - u), gv[i]);
@@ -559,6 +570,13 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
       const double* q = s_g[grp];
       const double Bavg = 0.125 * (q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7]);
       const double Aavg = 0.125 * (q[8] + q[9] + q[10] + q[11] + q[12] + q[13] + q[14] + q[15]);
+      if (sv) {
+        const size_t k = slot * (size_t)t.cap + c;
+        sv->su[k] = u;
+        sv->sp[k] = p;
+        sv->sh[k] = exi;
+        sv->sAv[k] = Aavg;
+      }
       const double diff_u = ca2p_reaction_factor(u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
       dfu_out[c] = diff_u;
       const double H0 = 0.5;
@@ -577,11 +595,13 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
 
 template <int CG>
 __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
-                                                      int ap, int bp, int xcd) {
+                                                      int ap, int bp, int xcd, const RdTis* __restrict__ sv,
+                                                      int slot) {
   const unsigned lid = xcd_order(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, xcd);
   const unsigned bx = lid % gridDim.x, by = lid / gridDim.x;
-  const Tis& t = T[tlist ? tlist[by] : by];
-  ca_cells_part<CG, C_BLOCK>(t, bx, gridDim.x, cb, cp, ap, bp, nullptr, nullptr);
+  const int ti = tlist ? tlist[by] : by;
+  const Tis& t = T[ti];
+  ca_cells_part<CG, C_BLOCK>(t, bx, gridDim.x, cb, cp, ap, bp, nullptr, nullptr, sv ? sv + ti : nullptr, slot);
 }
 
 // ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound.
@@ -750,15 +770,15 @@ void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int nt
 
 namespace {
 void launch_cells(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp, int max_n,
-                  const Tuning& tu) {
+                  const Tuning& tu, const RdTis* sv = nullptr, int slot = 0) {
   const int cg = tu.ca_group;
   const dim3 gc((max_n + C_BLOCK / cg - 1) / (C_BLOCK / cg), ntis);
   if (cg == 4)
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd, sv, slot);
   else if (cg == 8)
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd, sv, slot);
   else
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd, sv, slot);
 }
 
 void launch_atp(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cp, int ap, int max_nx, int max_ny,
@@ -790,9 +810,10 @@ void launch_fused_cg(hipStream_t s, const Tis* dT, const int* tlist, int ntis, i
 }  // namespace
 
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
-  launch_cells(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, tu);
-  launch_atp(s, dT, tlist, ntis, cp, ap, max_nx, max_ny, izb_est, tu);
+                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu, const RdTis* sv, int slot,
+                       bool cells_only) {
+  launch_cells(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, tu, sv, slot);
+  if (!cells_only) launch_atp(s, dT, tlist, ntis, cp, ap, max_nx, max_ny, izb_est, tu);
 }
 
 void launch_ca_substeps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int& cp, int& ap, int bp,

@@ -696,18 +696,6 @@ __device__ __forceinline__ P3 pair_bwd(const Tis& t, int law, const CV& a, int i
   return g;
 }
 
-// sum over the MG lanes of a group (every lane gets it)
-template <int G>
-__device__ __forceinline__ P3 group_sum3(P3 v) {
-#pragma unroll
-  for (int o = 1; o < G; o <<= 1) {
-    v.x += __shfl_xor(v.x, o, G);
-    v.y += __shfl_xor(v.y, o, G);
-    v.z += __shfl_xor(v.z, o, G);
-  }
-  return v;
-}
-
 // adjoint of n = N / |N|
 __device__ __forceinline__ P3 unit_bar(P3 N, P3 nb) {
   const double nn = v_norm(N);
@@ -840,13 +828,12 @@ __global__ void __launch_bounds__(256) k_mech_bwd_begin(const Tis* __restrict__ 
     }
 }
 
-// The reverse kernels give each cell a group of MG lanes: the lanes take the
-// cell's list entries (or its membrane slots and dihedral terms) side by
-// side.  The cell's own term is summed over the group and added by one lane;
-// the terms of other cells are scattered with atomics and the parameter sums
-// are reduced at the end, so no order is kept (a gradient need not follow the
-// forward's).
-constexpr int MG = 16;
+// The reverse kernels take one lane per cell.  A cell's own terms are summed
+// by its lane and added with one atomic; the terms of other cells are
+// scattered with atomics and the parameter sums are reduced at the end, so
+// no order is kept (a gradient need not follow the forward's).  (Lane groups
+// per cell measured slower: the laws' branches and the dihedral terms' two
+// code paths then run one after the other within a group.)
 
 // every interaction a cell's handler applies (cell_interaction.cpp:529-641)
 __global__ void __launch_bounds__(256) k_mech_bwd_pairs(const Tis* __restrict__ T, MechTis* __restrict__ M, int pb,
@@ -857,32 +844,29 @@ __global__ void __launch_bounds__(256) k_mech_bwd_pairs(const Tis* __restrict__ 
   const long long cap = t.cap;
   const double* lam = m.lam[q];
   double* nl = m.lam[1 - q];
-  const int lane = threadIdx.x & (MG - 1);
-  const int groups = gridDim.x * (blockDim.x / MG);
   ParAcc pa{0, 0, 0};
-  for (int X = blockIdx.x * (blockDim.x / MG) + threadIdx.x / MG; X < n; X += groups) {
+  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
     const CV cx = load_cv(t, pb, X);
     P3 gx{0, 0, 0};
     if (X < t.nmemb) {
-      if (lane < MEMB_ADJ) {
-        const int o = t.memb6[6 * X + lane];
+      for (int i = 0; i < MEMB_ADJ; i++) {
+        const int o = t.memb6[6 * X + i];
         if (t.state[o] == MEMB && X > o)
-          gx = pair_bwd(t, LAW_MEMB_MEMB, cx, X, load_cv(t, pb, o), o, lam, nl, cap, pa);
+          gx = v_add(gx, pair_bwd(t, LAW_MEMB_MEMB, cx, X, load_cv(t, pb, o), o, lam, nl, cap, pa));
       }
     } else {
       const int* row = t.col[cb] + (size_t)X * t.ns;
       const int len = t.nlen[cb][X];
-      for (int e = lane; e < len; e += MG) {
+      for (int e = 0; e < len; e++) {
         const int o = row[e];
         const CV co = load_cv(t, pb, o);
         const int law = proc_law(cx, X, co, o);
         if (law != LAW_NONE) gx = v_add(gx, pair_bwd(t, law, cx, X, co, o, lam, nl, cap, pa));
       }
-      if (lane == MG - 1 && cx.pair >= 0 && X > cx.pair)
+      if (cx.pair >= 0 && X > cx.pair)
         gx = v_add(gx, pair_bwd(t, LAW_PAIR, cx, X, load_cv(t, pb, cx.pair), cx.pair, lam, nl, cap, pa));
     }
-    gx = group_sum3<MG>(gx);
-    if (lane == 0) atomic_add3(nl, cap, X, gx);
+    atomic_add3(nl, cap, X, gx);
   }
   for (int o = 32; o > 0; o >>= 1) {
     pa.eps += __shfl_down(pa.eps, o, 64);
@@ -907,42 +891,39 @@ __global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T
   const double* lam = m.lam[q];
   double* nl = m.lam[1 - q];
   double peps = 0, pbend = 0;
-  const int lane = threadIdx.x & (MG - 1);
-  const int groups = gridDim.x * (blockDim.x / MG);
-  for (int X = blockIdx.x * (blockDim.x / MG) + threadIdx.x / MG; X < n; X += groups) {
+  for (int X = blockIdx.x * blockDim.x + threadIdx.x; X < n; X += gridDim.x * blockDim.x) {
     const bool memb = X < t.nmemb;
     if (!memb && t.state[X] != DER) continue;
     const CV cx = load_cv(t, pb, X);
-    if (lane == MG - 1 && cx.p.z < cx.r) {
+    if (cx.p.z < cx.r) {
       const double z = cx.p.z;
       const double L = POW6(cx.r) / POW6(z);
       atomicAdd(&nl[2 * cap + X], lam[2 * cap + X] * DT_Cell * 2.0 * t.m_eps * (7.0 * L - 13.0 * L * L) / (z * z));
       peps += lam[2 * cap + X] * DT_Cell * 2.0 * L * (L - 1.0) / z;
     }
     if (!memb) continue;
-    // lanes 0-5: tri_bend_1 term i = lane; lanes 6-11: tri_bend_2 term lane - 6
-    P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if (lane < 12) {
-      const P3 lm = ld3(lam, cap, X);
-      const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
-      const int* md = t.memb6 + 6 * X;
-      const int i = lane < 6 ? lane : lane - 6;
+    const P3 lm = ld3(lam, cap, X);
+    const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
+    const int* md = t.memb6 + 6 * X;
+    P3 dr{0, 0, 0};
+    for (int i = 0; i < 6; i++) {
       const int i1 = (i + 1) % 6;
-      const int idx[4] = {X, md[i], md[i1], lane < 6 ? t.memb6[6 * md[i1] + i] : md[(i + 2) % 6]};
-      const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
-      P3 term;
-      if (lane < 6) {
-        term = tri_bend_1(t, cx.p, pa, pbv, pc);
-        tri_bend_1_bar(t, cx.p, pa, pbv, pc, ob, bar);
-      } else {
-        term = tri_bend_2(t, cx.p, pa, pbv, pc);
-        tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
-      }
-      for (int k = 1; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
-      pbend += DT_Cell * v_dot(lm, term);
+      const int idx[4] = {X, md[i], md[i1], t.memb6[6 * md[i1] + i]};
+      const P3 pr = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
+      dr = v_add(dr, tri_bend_1(t, cx.p, pr, pbv, pc));
+      P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      tri_bend_1_bar(t, cx.p, pr, pbv, pc, ob, bar);
+      for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
     }
-    bar[0] = group_sum3<MG>(bar[0]);
-    if (lane == 0) atomic_add3(nl, cap, X, bar[0]);
+    for (int i = 0; i < 6; i++) {
+      const int idx[4] = {X, md[i], md[(i + 1) % 6], md[(i + 2) % 6]};
+      const P3 pa = ld_pos(t, pb, idx[1]), pbv = ld_pos(t, pb, idx[2]), pc = ld_pos(t, pb, idx[3]);
+      dr = v_add(dr, tri_bend_2(t, cx.p, pa, pbv, pc));
+      P3 bar[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
+      for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
+    }
+    pbend += DT_Cell * v_dot(lm, dr);
   }
   for (int o = 32; o > 0; o >>= 1) {
     peps += __shfl_down(peps, o, 64);
@@ -964,10 +945,9 @@ void launch_mech_seed(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int p
 void launch_mech_backward_step(hipStream_t s, const Tis* dT, MechTis* dM, int ntis, int slot, int pb, int cb, int q,
                                int max_n) {
   const dim3 g((max_n + 255) / 256, ntis);
-  const dim3 gg((max_n + 256 / MG - 1) / (256 / MG), ntis);
   EPI_LAUNCH(K_MECH_BWD_BEGIN, k_mech_bwd_begin, g, dim3(256), 0, s, dT, dM, slot, pb, q);
-  EPI_LAUNCH(K_MECH_BWD_PAIRS, k_mech_bwd_pairs, gg, dim3(256), 0, s, dT, dM, pb, cb, q);
-  EPI_LAUNCH(K_MECH_BWD_MEMB, k_mech_bwd_memb, gg, dim3(256), 0, s, dT, dM, pb, q);
+  EPI_LAUNCH(K_MECH_BWD_PAIRS, k_mech_bwd_pairs, g, dim3(256), 0, s, dT, dM, pb, cb, q);
+  EPI_LAUNCH(K_MECH_BWD_MEMB, k_mech_bwd_memb, g, dim3(256), 0, s, dT, dM, pb, q);
 }
 
 }  // namespace epi

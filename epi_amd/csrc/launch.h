@@ -71,8 +71,13 @@ void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long
 void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est,
                      const Tuning& tu);
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
+struct RdTis;
+// sv: a reaction-diffusion window's reverse-pass storage; the cell kernel
+// then also saves the state before the sub-step into its segment slot
+// `slot`; cells_only skips the ATP update (the last state of a segment)
 void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
+                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu, const RdTis* sv = nullptr,
+                       int slot = 0, bool cells_only = false);
 // n_sub sub-steps (fused when tu.ca_fuse), flipping cp and ap after each.
 // The cell kernels take one cell per lane group: max_n (here and in
 // launch_ca_substep) must be at least the cells past the membrane and dermis.
@@ -82,7 +87,6 @@ void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, i
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 // frozen reaction-diffusion window (k_adjoint.hip)
-struct RdTis;
 void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items);
 void launch_rd_own(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int max_n);
 void launch_rd_restore(hipStream_t s, const Tis* dT, RdTis* dR, const double* gj0, int ntis, int cb, int cp, int ap,

@@ -18,8 +18,10 @@ struct RdTis {
   // snapshot: [0] = the current buffer, [1] = the other one
   double *u0[2], *p0[2], *A0[2];
   double *h0, *avg0, *kpa0, *g0;
-  // saved states of one segment (slot-major) and the checkpoints
-  double *su, *sp, *sh, *sg, *sA;
+  // saved states of one segment (slot-major) and the checkpoints.  Of the
+  // ATP grid a segment state keeps only each AFM cell's grid_avg8 (sAv), the
+  // one value the reverse sub-step reads from it
+  double *su, *sp, *sh, *sg, *sAv;
   double *cu, *cp_, *ch, *cg, *cA;   // checkpoint-major, two buffers each for u, p, A
   // adjoint: [q] = the state after the sub-step, [1-q] = before it
   double *lu[2], *lp[2], *lh[2], *lg[2];
