@@ -35,7 +35,11 @@ PEAK_HBM_GBS = 8000.0
 # algorithmic bytes per unit of work: per grid point of a stencil sweep, and
 # per AFM cell of a Ca sub-step 96 + 24 bytes per list entry (k_ca_cells)
 BYTES_PER_POINT = {"k_atp": 22.0, "k_ext_stim": 21.0}
-CA_CELL_BYTES, CA_ENTRY_BYTES = 96.0, 24.0
+# per AFM cell: its own state read and written; per coupled list entry (a
+# neighbour that is ALIVE, DEAD, FIX or MUSUME; the sub-step reads no other):
+# the neighbour index 4, gj 8, the neighbour's (ca2p, IP3) 16; plus the gj
+# write-back of an ALIVE neighbour, 8
+CA_CELL_BYTES, CA_ENTRY_BYTES, CA_ALIVE_ENTRY_BYTES = 96.0, 28.0, 8.0
 MODELED = ("k_ca_cells", "k_atp", "k_ext_stim", "k_ca_fused")
 
 
@@ -124,21 +128,28 @@ def roofline(ts, prof, s0, s1, rk, launched, workload):
         # tissue (a fused launch also the ATP grid of the sub-step before):
         # weight each tissue by its share of the launches
         per_solve = 1000 if rk == "k_ca_cells" else 999
-        alg, units, entries, points = 0.0, 0, 0.0, 0.0
+        from epi_amd import configs as cf
+        alg, units, entries, alive, points = 0.0, 0, 0.0, 0.0, 0.0
         for t in range(ts.T):
             solves = s1[t].ca_solves - s0[t].ca_solves
             if not solves:
                 continue
             c = ts.get_cells(t)
             st, lens = c["state"], np.diff(c["conn_ptr"])
-            afm = (st == 0) | (st == 4) | (st == 7)
+            afm = (st == cf.ALIVE) | (st == cf.FIX) | (st == cf.MUSUME)
             afm[:c["nmemb"] + c["nder"]] = False
+            owner = np.repeat(np.arange(st.size), lens)
+            nbs = st[c["conn"][:owner.size]]
+            sel = afm[owner]
+            coupled = sel & np.isin(nbs, (cf.ALIVE, cf.DEAD, cf.FIX, cf.MUSUME))
             w = solves * per_solve / cnt
             units += int(afm.sum() * w)
-            entries += float(lens[afm].sum()) * w
+            entries += float(coupled.sum()) * w
+            alive += float((coupled & (nbs == cf.ALIVE)).sum()) * w
             points += (s1[t].grid_points_ca - s0[t].grid_points_ca) / (solves * 1000) * w
-        alg = CA_CELL_BYTES * units + CA_ENTRY_BYTES * entries
-        unit = f"{units} AFM cells, mean list {entries / max(units, 1):.2f}"
+        alg = CA_CELL_BYTES * units + CA_ENTRY_BYTES * entries + CA_ALIVE_ENTRY_BYTES * alive
+        unit = (f"{units} AFM cells, {entries / max(units, 1):.2f} coupled list entries per cell "
+                f"({alive / max(units, 1):.2f} ALIVE)")
         if rk == "k_ca_fused":
             alg += BYTES_PER_POINT["k_atp"] * points
             unit += f", {points:.0f} ATP grid points"
