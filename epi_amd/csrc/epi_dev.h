@@ -128,6 +128,10 @@ struct Tis {
   int* memb6;                 // triangular adjacency, 6 per membrane cell
   // buckets
   int* bucket_of; int* bslot; int* bucket_n; int* bucket_cells;
+  // (x, y, z, radius) of every cell after the periodic fix, written by the
+  // bucket insertion: the sweep and the row assembly gather a candidate's
+  // position and radius with one 32-byte load instead of four
+  double4* pk;
   // state renewal
   int* div_slot; double* div_dir;  // per division: parent, direction
   int* div_parent; int* rm_list;

@@ -239,6 +239,7 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.bslot = cv.take<int>(cap);
   t.bucket_n = cv.take<int>(h.nbuckets);
   t.bucket_cells = cv.take<int>((size_t)h.nbuckets * bucket_stride(h.bcap));
+  t.pk = cv.take<double4>(cap);
   t.div_slot = cv.take<int>(cap);
   t.div_dir = cv.take<double>(3 * cap);
   t.div_parent = cv.take<int>(cap);

@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(256) k_bucket_insert(const Tis* __restrict__ T
     const long aix = (long)((0.5 * t.LX - p_diff_x(t, 0.5 * t.LX, x)) / AREA_GRID);
     const long aiy = (long)((0.5 * t.LY - p_diff_y(t, 0.5 * t.LY, y)) / AREA_GRID);
     const long aiz = (long)((min0(z)) / AREA_GRID);
+    t.pk[c] = make_double4(x, y, z, t.radius[c]);
     t.bcnt[c] = 0;
     t.nlo[c] = 0;
     t.bslot[c] = -1;
@@ -86,12 +87,12 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
 constexpr int SWEEP_G = 32;
 constexpr int SWEEP_H = 8;
 
-__device__ __forceinline__ bool sweep_hit(const Tis& t, int pb, const P3& pc, double rc, int o) {
-  const P3 po = ld_pos(t, pb, o);
+__device__ __forceinline__ bool sweep_hit(const Tis& t, const P3& pc, double rc, int o) {
+  const double4 po = t.pk[o];
   const double diffx = p_diff_x(t, pc.x, po.x);
   const double diffy = p_diff_y(t, pc.y, po.y);
   const double diffz = pc.z - po.z;
-  const double rad_sum = rc + t.radius[o];
+  const double rad_sum = rc + po.w;
   return diffx * diffx + diffy * diffy + diffz * diffz <= LJ_THRESH * LJ_THRESH * rad_sum * rad_sum;
 }
 
@@ -106,8 +107,9 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
   int* const hits = s_hit[threadIdx.x];
   int* const col = t.col[nb];
   for (int c = t.nmemb + blockIdx.x * (blockDim.x / SWEEP_G) + threadIdx.x / SWEEP_G; c < n; c += groups) {
-    const P3 pc = ld_pos(t, pb, c);
-    const double rc = t.radius[c];
+    const double4 pq = t.pk[c];
+    const P3 pc{pq.x, pq.y, pq.z};
+    const double rc = pq.w;
     const long anx = (long)(pc.x / AREA_GRID);
     const long any = (long)(pc.y / AREA_GRID);
     const long anz = (long)(pc.z / AREA_GRID);
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
           const int o4[4] = {q.x, q.y, q.z, q.w};
           bool hit[4];
 #pragma unroll
-          for (int u = 0; u < 4; u++) hit[u] = m0 + u < sz[i] && o4[u] < c && sweep_hit(t, pb, pc, rc, o4[u]);
+          for (int u = 0; u < 4; u++) hit[u] = m0 + u < sz[i] && o4[u] < c && sweep_hit(t, pc, rc, o4[u]);
 #pragma unroll
           for (int u = 0; u < 4; u++) {
             if (done || m0 + u >= sz[i]) continue;
@@ -170,7 +172,7 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
           for (int m = 0; m < sz; ++m) {
             const int o = cells[m];
             if (c <= o) break;
-            if (sweep_hit(t, pb, pc, rc, o)) {
+            if (sweep_hit(t, pc, rc, o)) {
               if (off < t.ns) row[off] = o;
               off++;
               const int slot = atomicAdd(&t.bcnt[o], 1);
