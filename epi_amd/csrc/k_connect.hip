@@ -5,8 +5,8 @@
 //   [6 membrane neighbours if MEMB] ++ [partners o < X found by X's own
 //   5x5x5 bucket sweep, in sweep order] ++ [partners c > X, ascending c].
 // k_bucket_insert  places cells in fixed-capacity buckets (grid_init :38-62),
-// k_bucket_sort    orders each bucket by cell index (the serial insertion
-//                  order),
+// k_bucket_sort    ranks each cell in its bucket by cell index (the serial
+// k_bucket_place   insertion order) and writes it to that slot,
 // k_sweep          runs connect_proc (:70-129) for every non-membrane cell:
 //                  the lower part of its own row is written directly, the
 //                  upper parts of its partners are appended with atomics,
@@ -56,23 +56,35 @@ __global__ void __launch_bounds__(256) k_bucket_insert(const Tis* __restrict__ T
   }
 }
 
-// the cell that took slot 0 of a bucket orders that bucket by cell index
-// (the serial insertion order of grid_init)
+// each bucket in ascending cell index, in two launches: every cell counts
+// the cells of its bucket with a smaller index (its rank, kept in bslot),
+// then writes itself to that slot.  A bucket holds distinct cells, so the
+// ranks of its m cells are 0 .. m - 1.
 __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) {
   const Tis& t = T[blockIdx.y];
   const int n = t.st->n;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    if (t.bslot[c] != 0) continue;
+    if (t.bslot[c] < 0) continue;
     const int b = t.bucket_of[c];
     const int m = min(t.bucket_n[b], t.bcap);
-    if (m < 2) continue;
-    int* a = t.bucket_cells + (size_t)b * bucket_stride(t.bcap);
-    for (int i = 1; i < m; i++) {
-      const int v = a[i];
-      int j = i - 1;
-      while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-      a[j + 1] = v;
+    const int* a = t.bucket_cells + (size_t)b * bucket_stride(t.bcap);
+    int rank = 0;
+    // four slots per 16-byte load (a bucket's slots start 16-byte aligned and
+    // its stride is a multiple of 4); slots from m on are not counted
+    for (int i = 0; i < m; i += 4) {
+      const int4 q = *reinterpret_cast<const int4*>(a + i);
+      rank += (q.x < c) + (i + 1 < m && q.y < c) + (i + 2 < m && q.z < c) + (i + 3 < m && q.w < c);
     }
+    t.bslot[c] = rank;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_bucket_place(const Tis* __restrict__ T) {
+  const Tis& t = T[blockIdx.y];
+  const int n = t.st->n;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int r = t.bslot[c];
+    if (r >= 0) t.bucket_cells[(size_t)t.bucket_of[c] * bucket_stride(t.bcap) + r] = c;
   }
 }
 
@@ -366,6 +378,7 @@ void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int m
 void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n) {
   dim3 gc((max_n + 255) / 256, ntis);
   EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gc, dim3(256), 0, s, dT);
+  EPI_LAUNCH(K_BUCKET_PLACE, k_bucket_place, gc, dim3(256), 0, s, dT);
   dim3 gs((max_n + 7) / 8, ntis);
   EPI_LAUNCH(K_SWEEP, k_sweep, gs, dim3(256), 0, s, dT, pb, nb);
   dim3 gk((max_n + K_GROUPS - 1) / K_GROUPS, ntis);

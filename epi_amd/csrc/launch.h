@@ -14,7 +14,7 @@ enum KernelId {
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
   K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
   K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB,
-  K_CA_FUSED, K_CA_CORNERS, K_COUNT
+  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_COUNT
 };
 
 // Launch shapes that the autotuner may change (values checked by the host).
