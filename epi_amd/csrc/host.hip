@@ -909,15 +909,17 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     // the snapshot after the build.  The voxel maps (proc.cpp:155-157) read
     // only the fixed positions, states and radii, which the rest of the
     // build leaves alone, so they run beside it on the side stream.
+    // The build, the longer chain, is issued first: the host issues the
+    // side stream's launches while the GPU already runs it.
     const int nb = 1 - c->cb;
     launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
     c->len_known = false;
     HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
+    c->cb = nb;
     HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
     launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
     HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
-    launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
-    c->cb = nb;
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
     c->bp ^= 1;
