@@ -86,8 +86,16 @@ const char* kKernelNames[K_COUNT] = {
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
     "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place"};
 
-void prof_resolve() {
-  for (auto& r : g_prof.pending) {
+// with ready_only, only the pairs whose kernels have finished (the rest stay
+// pending for a later read-back)
+void prof_resolve(bool ready_only = false) {
+  size_t keep = 0;
+  for (size_t i = 0; i < g_prof.pending.size(); i++) {
+    auto& r = g_prof.pending[i];
+    if (ready_only && hipEventQuery(r.b) != hipSuccess) {
+      g_prof.pending[keep++] = r;
+      continue;
+    }
     float ms = 0;
     if (r.counted && hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
       g_prof.ms[r.k] += ms;
@@ -96,7 +104,7 @@ void prof_resolve() {
     g_prof.pool.push_back(r.a);
     g_prof.pool.push_back(r.b);
   }
-  g_prof.pending.clear();
+  g_prof.pending.resize(keep);
 }
 
 struct TH {
@@ -132,6 +140,7 @@ struct epi_ctx {
   // join events order it with the main stream
   hipStream_t side_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_st = nullptr;   // the status copy of a step, waited on alone
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
   Tuning tu;
@@ -258,10 +267,29 @@ int check_tissue(epi_ctx* c, int t) {
   return EPI_OK;
 }
 
+// the status read-back, split so that work issued after the copy can run
+// while the host waits for the copy alone (sync_status_end with the stream
+// still busy leaves the timing events for a later read-back to resolve)
+int sync_status_begin(epi_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(Status) * c->T, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev_st, c->stream));
+  return EPI_OK;
+}
+int status_check(epi_ctx* c);
+int sync_status_end(epi_ctx* c) {
+  HIPCHK(c, hipEventSynchronize(c->ev_st));
+  if (!g_prof.pending.empty()) prof_resolve(true);
+  return status_check(c);
+}
+
 int sync_status(epi_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(Status) * c->T, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!g_prof.pending.empty()) prof_resolve();
+  return status_check(c);
+}
+
+int status_check(epi_ctx* c) {
   for (int t = 0; t < c->T; t++) c->th[t].n_est = c->h_st[t].n;
   c->len_known = true;
   for (int t = 0; t < c->T; t++)
@@ -521,7 +549,8 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return EPI_ERR_HIP; }
   if (hipStreamCreate(&c->side_stream) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_st, hipEventDisableTiming) != hipSuccess) {
     epi_destroy(c);
     return EPI_ERR_HIP;
   }
@@ -587,6 +616,7 @@ void epi_destroy(epi_ctx* c) {
   if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_st) (void)hipEventDestroy(c->ev_st);
   delete c;
 }
 
@@ -899,11 +929,22 @@ int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
 
 int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
   if (!c || !drv) return EPI_ERR_INVALID;
+  // The next step's cell_interaction reads positions, radii, ages, states,
+  // pairs, dermis and the lists; a calc_ca2p solve writes none of them (and
+  // reads no position). So, unless initialize_sc (which changes states)
+  // fires, the next step's force kernel is issued before this step's status
+  // read-back, over a grid that covers every cell a tissue can hold, and runs
+  // while the host waits and decides on the solve; the solve follows it.
+  bool forces_issued = false;
+  int forces_cap = 0, forces_pb = 0;
   for (int64_t it = 0; it < nsteps; it++) {
     const int mn = max_n(c);
     // cell_dynamics (proc.cpp:22-32)
-    launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), mn, max_list_len(c));
-    c->pb ^= 1;
+    if (!forces_issued) {
+      launch_forces(c->stream, c->d_tis, c->T, c->pb, c->cb, max_memb(c), mn, max_list_len(c));
+      c->pb ^= 1;
+    }
+    forces_issued = false;
     launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
     // periodic fix fused into the bucket insertion; calc_zzmax fused into
     // the snapshot after the build.  The voxel maps (proc.cpp:155-157) read
@@ -923,8 +964,31 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
     c->bp ^= 1;
-    HIPCHK(c, hipGetLastError());
-    RCHK(sync_status(c));
+    bool sc_due = false;   // proc.cpp:158-163, decided from host state alone
+    for (int t = 0; t < c->T; t++)
+      if ((double)drv[t].step * DT_Cell > 6000.0 && drv[t].force_cornif) sc_due = true;
+    if (it + 1 < nsteps && !sc_due) {
+      // the rows are those of this step's build, whose longest list is known
+      // only after the read-back: the slots follow the last known maxlen,
+      // and rows beyond them go to k_forces_long once it is known
+      int cap_n = 1, known_len = 0;
+      for (int t = 0; t < c->T; t++) {
+        cap_n = std::max(cap_n, c->th[t].cap);
+        known_len = std::max(known_len, c->h_st[t].maxlen);
+      }
+      RCHK(sync_status_begin(c));
+      forces_pb = c->pb;
+      forces_cap = launch_forces_rows(c->stream, c->d_tis, c->T, c->pb, c->cb, cap_n, known_len);
+      c->pb ^= 1;
+      forces_issued = true;
+      HIPCHK(c, hipGetLastError());
+      RCHK(sync_status_end(c));
+    } else {
+      HIPCHK(c, hipGetLastError());
+      RCHK(sync_status(c));
+    }
+    if (forces_issued && max_list_len(c) > forces_cap)
+      launch_forces_long(c->stream, c->d_tis, c->T, forces_pb, c->cb, max_n(c), forces_cap);
     int nsc = 0, nca = 0;
     std::vector<int> ca_list;
     for (int t = 0; t < c->T; t++) {

@@ -533,9 +533,7 @@ __global__ void __launch_bounds__(256) k_forces_long(const Tis* __restrict__ T, 
     if (t.nlen[cb][X] > cap) forces_serial(t, pb, cb, X, t.st);
 }
 
-void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other,
-                   int max_len) {
-  (void)max_memb;
+int launch_forces_rows(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_other, int max_len) {
   dim3 g((max_other + F_GROUPS - 1) / F_GROUPS, ntis);
   int cap;
   if (max_len <= 40) {
@@ -548,10 +546,19 @@ void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int m
     cap = FORCES_ROW_CAP;
     EPI_LAUNCH(K_FORCES, k_forces<FORCES_ROW_CAP>, g, dim3(F_BLOCK), 0, s, dT, pb, cb);
   }
-  if (max_len > cap) {
-    dim3 gl((max_other + 255) / 256, ntis);
-    EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb, cap);
-  }
+  return cap;
+}
+
+void launch_forces_long(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_other, int cap) {
+  dim3 gl((max_other + 255) / 256, ntis);
+  EPI_LAUNCH(K_FORCES_LONG, k_forces_long, gl, dim3(256), 0, s, dT, pb, cb, cap);
+}
+
+void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other,
+                   int max_len) {
+  (void)max_memb;
+  const int cap = launch_forces_rows(s, dT, ntis, pb, cb, max_other, max_len);
+  if (max_len > cap) launch_forces_long(s, dT, ntis, pb, cb, max_other, cap);
 }
 
 // ==== frozen mechanics window: the reverse pass ==============================

@@ -56,6 +56,10 @@ void prof_add(int k, hipEvent_t a, hipEvent_t b, bool counted);
 // one-lane-per-cell kernel, launched only when max_len exceeds the slots.
 constexpr int FORCES_ROW_CAP = 64;
 void launch_forces(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_memb, int max_other, int max_len);
+// the two halves of launch_forces: the cooperative kernel for rows of up to
+// the returned number of slots (chosen from max_len), and the long rows
+int launch_forces_rows(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_other, int max_len);
+void launch_forces_long(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int max_other, int cap);
 void launch_renew(hipStream_t s, const Tis* dT, int ntis, int pb, int cb, int cpar, int max_n);
 void launch_initialize_sc(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int pb, int max_n);
 void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
