@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
 // cell.  Rows of at most KCAP entries are assembled in LDS: the lower part
 // (written by k_sweep in sweep order) and the upper partners (gathered in
 // k_sweep in arbitrary order, placed here by rank since they are distinct).
-// The previous row's uids are staged in LDS too and each lane resolves the
+// The previous row's uids and gj values are staged in LDS too and each lane resolves the
 // gap junction of its entries there (gj_refresh :133-152: a directed edge
 // keeps its value while it stays in the list, a new edge starts at gj_init).
 // The dermis (find_dermis :158-172: the first MEMB entry of minimal distance
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
 // minimum over the group.  Longer rows take the same steps through global
 // memory.
 constexpr int KG = 16;
-constexpr int KCAP = 128;
+constexpr int KCAP = 64;   // rows (and previous rows) longer than this take the global-memory path
 constexpr int K_BLOCK = 128;
 constexpr int K_GROUPS = K_BLOCK / KG;
 
@@ -222,6 +222,7 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
   __shared__ int s_row[K_GROUPS][KCAP];
   __shared__ int s_bt[K_GROUPS][KCAP];
   __shared__ long long s_ou[K_GROUPS][KCAP];
+  __shared__ double s_og[K_GROUPS][KCAP];
   const int lane = threadIdx.x & (KG - 1);
   const int grp = threadIdx.x / KG;
   int mylen = 0;
@@ -255,7 +256,10 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
       for (int i = lane; i < nbh; i += KG) s_bt[grp][i] = bt[i];
     }
     if (staged)
-      for (int f = lane; f < olen; f += KG) s_ou[grp][f] = t.old_uid[orow[f]];
+      for (int f = lane; f < olen; f += KG) {
+        s_ou[grp][f] = t.old_uid[orow[f]];
+        s_og[grp][f] = og[f];
+      }
     wave_sync();
     const int* const b = fits ? s_bt[grp] : bt;
     for (int i = lane; i < nbh; i += KG) {
@@ -280,7 +284,7 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
         int q = e + f;
         if (q >= olen) q -= olen;
         const long long w = staged ? s_ou[grp][q] : t.old_uid[orow[q]];
-        if (w == u) { v = og[q]; break; }
+        if (w == u) { v = staged ? s_og[grp][q] : og[q]; break; }
       }
       g[e] = v;
       if (want_dermis && t.state[o] == MEMB) {
