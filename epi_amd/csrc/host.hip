@@ -946,12 +946,12 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     }
     forces_issued = false;
     launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
-    // periodic fix fused into the bucket insertion; calc_zzmax fused into
-    // the snapshot after the build.  The voxel maps (proc.cpp:155-157) read
-    // only the fixed positions, states and radii, which the rest of the
-    // build leaves alone, so they run beside it on the side stream.
-    // The build, the longer chain, is issued first: the host issues the
-    // side stream's launches while the GPU already runs it.
+    // periodic fix fused into the bucket insertion.  calc_zzmax, the voxel
+    // maps and calc_ext_stim (proc.cpp:154-157) read only the fixed
+    // positions, states and radii (and the B field), which the rest of the
+    // build leaves alone, so they run beside it on the side stream.  The
+    // build, the longer chain, is issued first: the host issues the side
+    // stream's launches while the GPU already runs it.
     const int nb = 1 - c->cb;
     launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
     c->len_known = false;
@@ -959,11 +959,12 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
     c->cb = nb;
     HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
+    launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
     launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
+    launch_ext_stim(c->side_stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
+    c->bp ^= 1;
     HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
-    c->bp ^= 1;
     bool sc_due = false;   // proc.cpp:158-163, decided from host state alone
     for (int t = 0; t < c->T; t++)
       if ((double)drv[t].step * DT_Cell > 6000.0 && drv[t].force_cornif) sc_due = true;

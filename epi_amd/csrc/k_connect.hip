@@ -226,7 +226,6 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
   const int lane = threadIdx.x & (KG - 1);
   const int grp = threadIdx.x / KG;
   int mylen = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->zzmax_bits = 0ull;   // reduced by k_snapshot
   const int stride = gridDim.x * K_GROUPS;
   for (int c = blockIdx.x * K_GROUPS + grp; c < n; c += stride) {
     const bool memb = c < t.nmemb;
@@ -311,37 +310,17 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
 }
 
 // after a build: every cell's row is its own, uids snapshot the numbering,
-// the buckets are emptied for the next build, and calc_zzmax (proc.cpp:34-43)
-// is reduced (z values that matter are positive, where the IEEE bit order is
-// the numeric order)
+// and the buckets are emptied for the next build
 __global__ void __launch_bounds__(256) k_snapshot(const Tis* __restrict__ T, int pb) {
+  (void)pb;
   const Tis& t = T[blockIdx.y];
-  Status* st = t.st;
-  const int n = st->n;
-  const int start = t.nmemb + t.nder;
-  double m = 0;
+  const int n = t.st->n;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     t.src_row[c] = c;
     t.old_uid[c] = t.uid[c];
     const int b = t.bucket_of[c];
     if (b >= 0) t.bucket_n[b] = 0;
-    if (c >= start) {
-      const uint8_t s = t.state[c];
-      if (s == DEAD || s == ALIVE || s == MUSUME || s == FIX) {
-        const double z = t.pos[pb][2][c];
-        if (m < z) m = z;
-      }
-    }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    const double v = __shfl_down(m, o, 64);
-    if (m < v) m = v;
-  }
-  // the value was cleared by k_finish and only grows during this launch
-  const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
-  if ((threadIdx.x & 63) == 0 && m > 0 &&
-      mb > __hip_atomic_load(&st->zzmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    atomicMax(&st->zzmax_bits, mb);
 }
 
 // calc_zzmax (proc.cpp:34-43): z values that matter are positive, where the
@@ -362,7 +341,7 @@ __global__ void __launch_bounds__(256) k_zzmax(const Tis* __restrict__ T, int pb
     const double v = __shfl_down(m, o, 64);
     if (m < v) m = v;
   }
-  // the value was cleared by k_finish and only grows during this launch
+  // the value was cleared by k_zz_set and only grows during this launch
   const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
   if ((threadIdx.x & 63) == 0 && m > 0 &&
       mb > __hip_atomic_load(&st->zzmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
