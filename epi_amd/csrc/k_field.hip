@@ -397,7 +397,6 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
   constexpr int EPL = CG >= 8 ? 2 : 4;   // entries per lane per chunk
   constexpr int CCAP = EPL * CG;    // list entries per chunk
   constexpr int NQ = 16 / CG;       // grid corners per lane
-  const int n = t.st->n;
   // per-entry terms for the leader; rows padded by 16 bytes so that the
   // leaders of a wave's groups read different LDS banks
   __shared__ double2 s_a[C_GROUPS][CCAP + 1];
@@ -407,13 +406,13 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
   const double2* __restrict__ up = t.ca_up[cp];
   double2* __restrict__ up_next = t.ca_up[1 - cp];
   double* __restrict__ dfu_out = dfu_buf(t, cp);
-  const int izb = carr_prev ? ca_iz_bound(t) : 0;
-  const bool recompute = carr_prev && izb <= t.NZ;
   const int start = t.nmemb + t.nder;
-  if (bx == 0 && threadIdx.x == 0 && start + (long long)nbx * C_GROUPS < n) set_err(t.st, ERR_INVALID);
   const int c = start + bx * C_GROUPS + grp;
-  if (c >= n) return;
+  if (c >= t.cap) return;
   {
+    // The cell's own words are loaded before the cell count and iz_bound
+    // (device values) arrive, in one round trip with them: a cell past the
+    // count reads allocated, stale slots and leaves once the count is in.
     const uint8_t s = t.state[c];
     // the coupled entries only, in list order: an entry that is skipped
     // would add +0.0 to the sums, which leaves them as they are
@@ -435,8 +434,13 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
 #pragma unroll
     for (int j = 0; j < NQ; j++) {
       const int q = lane + j * CG;
-      pre[j] = recompute && q >= 8 ? t.ca_cq[8 * (size_t)c + (q & 7)] : make_uint2(0u, 0u);
+      pre[j] = q >= 8 ? t.ca_cq[8 * (size_t)c + (q & 7)] : make_uint2(0u, 0u);
     }
+    const int n = t.st->n;
+    const int izb = carr_prev ? ca_iz_bound(t) : 0;
+    const bool recompute = carr_prev && izb <= t.NZ;
+    if (bx == 0 && threadIdx.x == 0 && start + (long long)nbx * C_GROUPS < n) set_err(t.st, ERR_INVALID);
+    if (c >= n) return;
     const bool dead = s == DEAD;
     if (!dead && !is_afm(s)) return;
     if (dead) {
