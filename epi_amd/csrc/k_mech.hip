@@ -373,18 +373,23 @@ __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, i
   __shared__ P3 s_bend[F_GROUPS][12];
   const int lane = threadIdx.x & (FG - 1);
   const int grp = threadIdx.x / FG;
-  const int n = st->n;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && (long long)gridDim.x * F_GROUPS < n) set_err(st, ERR_INVALID);
   const int X = blockIdx.x * F_GROUPS + grp;
-  if (X >= n) return;
+  if (X >= t.cap) return;
+  // the cell's own words (and its lane's first list word) are loaded before
+  // the cell count arrives, in one round trip with it: a cell past the count
+  // reads allocated, stale slots and leaves once the count is in
   const int len = t.nlen[cb][X];
-  if (len > FCAP) return;   // k_forces_long
   const CV cx = load_cv(t, pb, X);
   const bool memb = X < t.nmemb;
   const int nA = memb ? 0 : t.nlo[X];
   const int* row = t.col[cb] + (size_t)X * t.ns;
+  const int q0 = row[lane];   // a row holds ns >= 64 > FG slots
+  const int n = st->n;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (long long)gridDim.x * F_GROUPS < n) set_err(st, ERR_INVALID);
+  if (X >= n) return;
+  if (len > FCAP) return;   // k_forces_long
   for (int e = lane; e < len; e += FG) {
-    const int q = row[e];
+    const int q = e == lane ? q0 : row[e];
     const CV cq = load_cv(t, pb, q);
     uint8_t f = 0;
     P3 own = P3{-0.0, -0.0, -0.0}, oth = P3{0, 0, 0};
