@@ -536,7 +536,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(tissues shard across GPUs with no per-step exchange; see DESIGN.md)",
+              "(a batch of tissues would shard across GPUs by whole tissues, with no per-step exchange)",
               file=sys.stderr)
         return 2
     if args.copy_calibration:
