@@ -229,7 +229,6 @@ template <int RG>
 __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int bid, int nblk, int slot, int cb,
                                               int bp, int q, int qa) {
   constexpr int REP = 32 / RG;
-  const int n = t.st->n;
   const int start = t.nmemb + t.nder;
   const long long cap = t.cap, ns = t.ns;
   const int lane = threadIdx.x & (RG - 1);
@@ -250,14 +249,21 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
   // one cell per group (the grid covers every cell past the membrane and
   // dermis; a launch that does not is reported as ERR_INVALID): a loop over
   // cells would hold the math's loop-invariant constants in registers
-  if (bid == 0 && threadIdx.x == 0 && start + (long long)groups < n) set_err(t.st, ERR_INVALID);
   const int c = start + bid * (blockDim.x / RG) + threadIdx.x / RG;
+  // the cell's own words are loaded in one round trip with the cell count (a
+  // cell past the count reads allocated, stale slots and leaves)
+  const int cc = c < cap ? c : (int)cap - 1;
+  const uint8_t s0 = t.state[cc];
+  const int len0 = t.ca_cn[cc];
+  const double lpc0 = lp[cc], uc0 = u[cc], pc0 = p[cc];
+  const int n = t.st->n;
+  if (bid == 0 && threadIdx.x == 0 && start + (long long)groups < n) set_err(t.st, ERR_INVALID);
   if (c < n) do {   // "continue" below leaves the cell
-    const uint8_t s = t.state[c];
+    const uint8_t s = s0;
     // the coupled entries only, (word, slot of the entry in the row)
-    const int len = t.ca_cn[c];
-    const double lpc = lp[c];
-    const double uc = u[c], pc = p[c];
+    const int len = len0;
+    const double lpc = lpc0;
+    const double uc = uc0, pc = pc0;
     const int2* ent = t.ca_ce + (size_t)c * ns;
     if (s == DEAD) {
       double cnt = 0, sum = 0;
