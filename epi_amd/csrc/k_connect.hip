@@ -279,11 +279,14 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
       if (fits && (e < base || e >= base + na)) row[e] = o;
       const long long u = t.uid[o];
       double v = gj_init;
+      // the search starts at the entry's own position (where a kept edge
+      // usually sits) and wraps, staying inside the previous row: an entry
+      // past the previous row's end starts at its position modulo olen
+      int q = olen > 0 ? e % olen : 0;
       for (int f = 0; f < olen; f++) {
-        int q = e + f;
-        if (q >= olen) q -= olen;
         const long long w = staged ? s_ou[grp][q] : t.old_uid[orow[q]];
         if (w == u) { v = staged ? s_og[grp][q] : og[q]; break; }
+        if (++q == olen) q = 0;
       }
       g[e] = v;
       if (want_dermis && t.state[o] == MEMB) {
