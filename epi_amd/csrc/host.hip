@@ -42,6 +42,7 @@ struct Prof {
   double ms[K_COUNT] = {};
   long long cnt[K_COUNT] = {};
   long long launched[K_COUNT] = {};   // every launch of an enabled kernel, timed or not
+  bool capturing = false;             // launches captured into a graph carry no events
 };
 Prof g_prof;
 hipEvent_t take_event() {
@@ -56,7 +57,7 @@ hipEvent_t take_event() {
 }
 }  // namespace
 bool prof_on(int k, bool* counted) {
-  if (!g_prof.mask) return false;
+  if (!g_prof.mask || g_prof.capturing) return false;
   const bool want = (g_prof.mask >> k) & 1ull;
   if (want) g_prof.launched[k]++;
   if (g_prof.period <= 1) {
@@ -144,6 +145,15 @@ struct epi_ctx {
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
   Tuning tu;
+  // a step's renewal and build as instantiated graphs, one per launch key
+  // (parities, launch sizes, launch shapes), least recently used evicted
+  struct StepGraph {
+    std::vector<long long> key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+  };
+  std::vector<StepGraph> graphs;
+  unsigned long long graph_tick = 0;
   // frozen reaction-diffusion window (epi_rd_*)
   struct Rd {
     bool active = false;
@@ -617,6 +627,7 @@ void epi_destroy(epi_ctx* c) {
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->ev_st) (void)hipEventDestroy(c->ev_st);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   delete c;
 }
 
@@ -927,6 +938,83 @@ int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
   return sync_status(c);
 }
 
+// cell_state_renew (proc.cpp:153), the bucket insertion with the periodic
+// fix and the rest of connect_cell (:154), and, on the side stream forked
+// after the insertion, calc_zzmax, setup_map_lat and calc_ext_stim
+// (:155-157), which read only the fixed positions, states and radii (and the
+// B field) that the rest of the build leaves alone.  The build, the longer
+// chain, is issued first.
+hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb) {
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+  launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
+  launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
+  chk(hipEventRecord(c->ev_fork, c->stream));
+  launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
+  chk(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
+  launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
+  launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
+  launch_ext_stim(c->side_stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb, c->tu);
+  chk(hipEventRecord(c->ev_join, c->side_stream));
+  chk(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  chk(hipGetLastError());
+  return e;
+}
+
+// issue_build launched one kernel at a time, or replayed from a graph
+// captured once per launch key.  A replay's grids cover the cells up to the
+// next multiple of 1024 (at most the largest capacity): every kernel of the
+// chain skips cells past the tissue's count, and the rounding lets a key
+// outlive the divisions of many steps.  Launches whose kernels are timed go
+// one by one, so that their events exist.
+int run_build(epi_ctx* c, int mn, int nb, int izb) {
+  constexpr unsigned long long seg =
+      (1ull << K_RENEW_FLAGS) | (1ull << K_RENEW_SCAN) | (1ull << K_RENEW_APPLY) | (1ull << K_REMOVE) |
+      (1ull << K_PAIR_DISPERSE) | (1ull << K_BUCKET_INSERT) | (1ull << K_BUCKET_SORT) | (1ull << K_BUCKET_PLACE) |
+      (1ull << K_SWEEP) | (1ull << K_FINISH) | (1ull << K_SNAPSHOT) | (1ull << K_ZZ_SET) | (1ull << K_ZZMAX) |
+      (1ull << K_MAP_CLEAR) | (1ull << K_MAP_STAMP) | (1ull << K_MAP_HALO) | (1ull << K_EXT_STIM);
+  if (!c->tu.step_graph || (g_prof.mask & seg)) {
+    HIPCHK(c, issue_build(c, mn, nb, izb));
+    return EPI_OK;
+  }
+  int mcap = 1;
+  for (auto& h : c->th) mcap = std::max(mcap, h.cap);
+  const int mr = std::max(mn, std::min((mn + 1023) / 1024 * 1024, mcap));
+  const Tuning& tu = c->tu;
+  const std::vector<long long> key = {c->pb, c->cb, c->cp, c->bp, nb, mr, izb, max_gsize(c), max_halo(c),
+                                      max_nx(c), max_ny(c), tu.b_zb, tu.b_rows, tu.map_lanes};
+  for (auto& g : c->graphs)
+    if (g.key == key) {
+      g.used = ++c->graph_tick;
+      HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+      return EPI_OK;
+    }
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  g_prof.capturing = true;
+  const hipError_t e1 = issue_build(c, mr, nb, izb);
+  g_prof.capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(c, EPI_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e3 != hipSuccess) return fail(c, EPI_ERR_HIP, std::string("step graph instantiation: ") + hipGetErrorString(e3));
+  constexpr size_t MAX_GRAPHS = 8;
+  if (c->graphs.size() >= MAX_GRAPHS) {
+    auto old = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                [](const epi_ctx::StepGraph& a, const epi_ctx::StepGraph& b) { return a.used < b.used; });
+    (void)hipGraphExecDestroy(old->exec);
+    c->graphs.erase(old);
+  }
+  c->graphs.push_back({key, exec, ++c->graph_tick});
+  HIPCHK(c, hipGraphLaunch(exec, c->stream));
+  return EPI_OK;
+}
+
 int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
   if (!c || !drv) return EPI_ERR_INVALID;
   // The next step's cell_interaction reads positions, radii, ages, states,
@@ -945,26 +1033,11 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
       c->pb ^= 1;
     }
     forces_issued = false;
-    launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
-    // periodic fix fused into the bucket insertion.  calc_zzmax, the voxel
-    // maps and calc_ext_stim (proc.cpp:154-157) read only the fixed
-    // positions, states and radii (and the B field), which the rest of the
-    // build leaves alone, so they run beside it on the side stream.  The
-    // build, the longer chain, is issued first: the host issues the side
-    // stream's launches while the GPU already runs it.
     const int nb = 1 - c->cb;
-    launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
+    RCHK(run_build(c, mn, nb, izb_est(c, false)));
     c->len_known = false;
-    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-    launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
     c->cb = nb;
-    HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
-    launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
-    launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
-    launch_ext_stim(c->side_stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
     c->bp ^= 1;
-    HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     bool sc_due = false;   // proc.cpp:158-163, decided from host state alone
     for (int t = 0; t < c->T; t++)
       if ((double)drv[t].step * DT_Cell > 6000.0 && drv[t].force_cornif) sc_due = true;
@@ -1117,6 +1190,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_CA_FUSE: return &tu.ca_fuse;
     case EPI_TUNE_MAP_LANES: return &tu.map_lanes;
     case EPI_TUNE_CA_GROUP_SMALL: return &tu.ca_group_small;
+    case EPI_TUNE_STEP_GRAPH: return &tu.step_graph;
     default: return nullptr;
   }
 }
@@ -1136,6 +1210,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_CA_FUSE: ok = value >= 0 && value <= 2; break;
     case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4; break;
     case EPI_TUNE_CA_GROUP_SMALL: ok = value == 4 || value == 8 || value == 16; break;
+    case EPI_TUNE_STEP_GRAPH: ok = value == 0 || value == 1; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

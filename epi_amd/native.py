@@ -103,7 +103,8 @@ SYMBOLS = [
 MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
 
 # launch-shape keys of epi_set_tuning (include/epi_amd.h)
-TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4, "xcd": 5, "rd_group": 6, "ca_fuse": 7, "map_lanes": 8, "ca_group_small": 9}
+TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4, "xcd": 5, "rd_group": 6, "ca_fuse": 7, "map_lanes": 8, "ca_group_small": 9,
+               "step_graph": 10}
 
 class RdParams(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", C.POINTER(C.c_double))]

@@ -257,11 +257,15 @@ int epi_rd_end(epi_ctx* ctx);
  * solves of at most 131072 cells in total), MAP_LANES in
  * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes),
  * CA_GROUP_SMALL in {4, 8, 16} (lanes per cell in the solves CA_FUSE 2
- * fuses).  EPI_ERR_INVALID for other keys or values. */
+ * fuses), STEP_GRAPH in {0, 1} (1, the default: epi_step replays the
+ * renewal and neighbour-build launches of a step, side stream included, from
+ * a HIP graph captured once per launch key; 0: it launches them one by one).
+ * EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
     EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
-    EPI_TUNE_MAP_LANES = 8, EPI_TUNE_CA_GROUP_SMALL = 9, EPI_TUNE_COUNT = 10
+    EPI_TUNE_MAP_LANES = 8, EPI_TUNE_CA_GROUP_SMALL = 9, EPI_TUNE_STEP_GRAPH = 10,
+    EPI_TUNE_COUNT = 11
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);

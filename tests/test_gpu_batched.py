@@ -152,3 +152,43 @@ def test_launch_shapes_do_not_change_results():
             assert "n" not in d and all(v == 0 for v in d.values()), (tu, t, d)
             for k in fa:
                 assert np.array_equal(fa[k], fb[k]), (tu, t, k)
+
+
+def test_step_graph_equals_one_by_one_launches():
+    """A step's renewal and build replayed from captured graphs (the default)
+    gives the tissues that launching them one kernel at a time gives.  The run
+    replays each graph many times, with force launches, Ca solves and status
+    read-backs between the replays, and the cell counts change."""
+    from epi_amd import native
+    specs = batch_specs()
+    steps, ca_every = 60, 20
+    draws = [native.mt19937_64_uniform(500 + i, 300000) for i in range(len(specs))]
+
+    def run(graph):
+        ts = native.TissueSet([s["geometry"] for s in specs])
+        for t, s in enumerate(specs):
+            ts.load(t, s)
+            ts.set_rng(t, draws[t])
+        ts.set_tuning({"step_graph": graph})
+        assert ts.get_tuning()["step_graph"] == graph
+        n0 = [ts.status(t)[0].n for t in range(len(specs))]
+        drv = [native.Driver(0, 0, 0, ca_every, 0) for _ in specs]
+        for _ in range(3):
+            ts.step(drv, steps // 3)
+        out = [(n0[t], ts.status(t)[0], ts.get_cells(t), ts.get_fields(t)) for t in range(len(specs))]
+        ts.close()
+        return out
+
+    eager = run(0)
+    graph = run(1)
+    for t in range(len(specs)):
+        (_, sa, ca, fa), (n0, sb, cb, fb) = eager[t], graph[t]
+        print(f"tissue {t}: n {n0} -> {sb.n}, solves {sb.ca_solves}", flush=True)
+        assert sa.err == 0 and sb.err == 0
+        assert (sa.n, sa.rng_pos, sa.ca_solves, sa.next_uid) == (sb.n, sb.rng_pos, sb.ca_solves, sb.next_uid)
+        d = compare_cells(ca, cb)
+        assert "n" not in d and all(v == 0 for v in d.values()), (t, d)
+        for k in fa:
+            assert np.array_equal(fa[k], fb[k]), (t, k)
+    # divisions moved the cell counts (and with them the rounded launch sizes)
+    assert any(s.n != n0 for n0, s, _, _ in graph)
