@@ -283,10 +283,12 @@ def config_tiny(seed=7, n_free=160, side=16, grid=16, n_layers=2, dead_top=False
 
 def batched_capacity(side_n, lz, dz, n_fix):
     """Free cells a box holds at the layer pitch of make_tissue while the top
-    layer keeps calc_ca2p's iz_bound = (zzmax + 2.8) / dz below nz (with a
-    margin of 2 for the jitter and the first steps' motion)."""
+    layer keeps calc_ca2p's iz_bound = (zzmax + 2.8) / dz below nz, with a
+    margin of 6.4: the jitter, plus two layer pitches for the tissue's upward
+    drift (with a margin of 2.0, the 32 x 32 x 64 box of tissue 91 left its
+    grid within the batched benchmark's 700-odd steps)."""
     z0 = R_MEMB + R_MEMB + R_MAX
-    layers = int((lz - 2.8 - 2.0 * dz - 2.0 - z0) / 2.2) + 1
+    layers = int((lz - 2.8 - 2.0 * dz - 6.4 - z0) / 2.2) + 1
     return side_n * side_n * max(layers, 1) - n_fix
 
 

@@ -28,7 +28,7 @@ struct Tuning {
                               // (0 never, 1 always, 2 for solves of at most 131072 cells)
   int ca_group_small = 8;     // k_ca_cells lanes per cell in the solves that ca_fuse 2 fuses
   int map_lanes = 1;          // k_map_stamp: lanes per cell (z planes split between them)
-  int step_graph = 1;         // epi_step: renewal and build replayed from a captured graph
+  int step_graph = 0;         // epi_step: 1 replays the renewal and build from a captured graph
 };
 
 // Event timer: when the bit of a kernel id is set, its launches carry a

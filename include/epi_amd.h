@@ -257,9 +257,9 @@ int epi_rd_end(epi_ctx* ctx);
  * solves of at most 131072 cells in total), MAP_LANES in
  * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes),
  * CA_GROUP_SMALL in {4, 8, 16} (lanes per cell in the solves CA_FUSE 2
- * fuses), STEP_GRAPH in {0, 1} (1, the default: epi_step replays the
- * renewal and neighbour-build launches of a step, side stream included, from
- * a HIP graph captured once per launch key; 0: it launches them one by one).
+ * fuses), STEP_GRAPH in {0, 1} (0, the default: epi_step launches the
+ * renewal and neighbour-build kernels of a step one by one; 1: it replays
+ * them, side stream included, from a HIP graph captured once per launch key).
  * EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,

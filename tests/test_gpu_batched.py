@@ -155,7 +155,7 @@ def test_launch_shapes_do_not_change_results():
 
 
 def test_step_graph_equals_one_by_one_launches():
-    """A step's renewal and build replayed from captured graphs (the default)
+    """A step's renewal and build replayed from captured graphs (STEP_GRAPH 1)
     gives the tissues that launching them one kernel at a time gives.  The run
     replays each graph many times, with force launches, Ca solves and status
     read-backs between the replays, and the cell counts change."""
