@@ -16,6 +16,7 @@
 #include "epi_dev.h"
 #include "launch.h"
 #include "epi_group.h"
+#include "gj_carry.h"
 
 namespace epi {
 
@@ -278,17 +279,9 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
       const int o = r[e];
       if (fits && (e < base || e >= base + na)) row[e] = o;
       const long long u = t.uid[o];
-      double v = gj_init;
-      // the search starts at the entry's own position (where a kept edge
-      // usually sits) and wraps, staying inside the previous row: an entry
-      // past the previous row's end starts at its position modulo olen
-      int q = olen > 0 ? e % olen : 0;
-      for (int f = 0; f < olen; f++) {
-        const long long w = staged ? s_ou[grp][q] : t.old_uid[orow[q]];
-        if (w == u) { v = staged ? s_og[grp][q] : og[q]; break; }
-        if (++q == olen) q = 0;
-      }
-      g[e] = v;
+      const long long* const ou = s_ou[grp];
+      const int q = gj_carry_slot(e, olen, u, [&](int k) { return staged ? ou[k] : t.old_uid[orow[k]]; });
+      g[e] = q < 0 ? gj_init : staged ? s_og[grp][q] : og[q];
       if (want_dermis && t.state[o] == MEMB) {
         const double d = dist_sq(t, pc, ld_pos(t, pb, o));
         if (d < bd) { bd = d; be = e; }   // entries of a lane ascend

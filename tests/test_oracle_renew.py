@@ -140,3 +140,32 @@ def test_musume_without_dermis_becomes_alive(oracle_mod):
     t.cell_state_renew()
     c = t.get_cells()
     assert c["state"][spec["nmemb"]] == 0 and t.rng_cursor() == 0
+
+
+def test_lipid_laws_at_special_points(oracle_mod):
+    # k_lipid_release = 0.25 * 0.2 * (1 + tanh((avg - ubar) / 0.1)) * (1 + tanh((agek - 3) / 0.1))
+    # k_lipid         = 0.25 * 0.6 * (1 + tanh((ubar - avg) / 0.1)) * (1 + tanh((agek - 3) / 0.1))
+    # (cell_state_renew.cpp:102-119, ubar = 0.45, THRESH_SP = 3).  At agek = 3 and
+    # avg = ubar both tanh are 0: release 0.05, lipid 0.15.  Five units past
+    # either switch the tanh are +-1 in double: release 0.2 and lipid 0 above
+    # ubar, release 0 and lipid 0.6 below.  in_fat = 0.4, ex_fat = 0.3:
+    #   in' = in + DT (lipid (1 - in) - release in), ex' = ex + DT release in,
+    #   agek' = agek + DT 0.05 (0.02 + 2 min0(avg - 0.122)) (:66-72, :347-358)
+    cases = [  # agek, avg, release, lipid
+        (3.0, 0.45, 0.05, 0.15),
+        (13.0, 5.45, 0.2, 0.0),
+        (13.0, -4.55, 0.0, 0.6),
+    ]
+    free = [dict(state=0, x=2.0 + 5.0 * i, y=8.0, z=9.0, agek=a, ca2p_avg=u, in_fat=0.4, ex_fat=0.3)
+            for i, (a, u, _, _) in enumerate(cases)]
+    spec = sheet_spec(free, L=16.0, grid=(16, 16, 16))
+    t = oracle_mod.OracleTissue(spec["geometry"])
+    t.load(spec)
+    m = spec["nmemb"]
+    t.cell_state_renew()
+    t.check()
+    c = t.get_cells()
+    for i, (a, u, rel, lip) in enumerate(cases):
+        assert c["in_fat"][m + i] == pytest.approx(0.4 + DT * (lip * 0.6 - rel * 0.4), rel=1e-14), i
+        assert c["ex_fat"][m + i] == pytest.approx(0.3 + DT * rel * 0.4, rel=1e-14), i
+        assert c["agek"][m + i] == pytest.approx(a + DT * 0.05 * (0.02 + 2 * max(u - 0.122, 0.0)), rel=1e-14), i
