@@ -16,6 +16,45 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* EO_OMP builds the multithreaded form used as the CPU throughput baseline:
+ * the loops that epi5_MT runs as tbb::parallel_for / parallel_for_each
+ * (cellmanager.h:28-35, cell_connection.cpp:41,93,135,179, cell_interaction.cpp:
+ * 503-515,652-656, cell_state_renew.cpp:402,422, map.cpp:35-42,156,
+ * ext_stim.cpp:26,63, ca2p.cpp:158,186,241,311) run over OpenMP threads, with
+ * the reference's atomics: CAS-added position updates (atomics.h:42-65),
+ * fetch-add list pushes, a locked cell creation and replay-stream cursor.
+ * Like the reference under TBB, its list order, accumulation order and
+ * daughter numbering then depend on the thread schedule, so only the
+ * default (serial) build is the parity checker. */
+#ifdef EO_OMP
+#include <omp.h>
+#define EO_PAR _Pragma("omp parallel for schedule(static)")
+#define EO_PAR_DYN _Pragma("omp parallel for schedule(dynamic, 64)")
+#define EO_PAR2 _Pragma("omp parallel for collapse(2) schedule(static)")
+#define EO_CRITICAL _Pragma("omp critical(eo_create)")
+#else
+#define EO_PAR
+#define EO_PAR_DYN
+#define EO_PAR2
+#define EO_CRITICAL
+#endif
+
+/* d += v: a compare-and-swap loop when threads share d (atomic_double's
+ * operator+=, atomics.h:42-65), plain addition in the serial build */
+static inline void add_d(double* d, double v) {
+#ifdef EO_OMP
+    uint64_t old = __atomic_load_n((uint64_t*)d, __ATOMIC_RELAXED), nw;
+    do {
+        double x;
+        memcpy(&x, &old, 8);
+        x += v;
+        memcpy(&nw, &x, 8);
+    } while (!__atomic_compare_exchange_n((uint64_t*)d, &old, nw, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+#else
+    *d += v;
+#endif
+}
+
 /* ---- constants: epi5_MT/define.h:69-197 and the physics files ---------- */
 #define DT_Cell 0.01                  /* define.h:86 */
 #define DT_Ca 0.01                    /* define.h:89 */
@@ -405,17 +444,26 @@ void eo_mt19937_64_uniform(uint64_t seed, double* out, int64_t n) {
 static inline int32_t* conn_row(eo_tissue* t, int c) { return &t->conn[(size_t)c * MAX_CONNECT_CELL_NUM]; }
 
 static void conn_push(eo_tissue* t, int c, int o) {
+#ifdef EO_OMP
+    /* Lockfree_push_stack::push_back_with_index (atomics.h:95-149) */
+    const int k = __atomic_fetch_add(&t->nconn[c], 1, __ATOMIC_RELAXED);
+    if (k >= MAX_CONNECT_CELL_NUM) { set_err(t, EO_ERR_CAPACITY); __atomic_fetch_sub(&t->nconn[c], 1, __ATOMIC_RELAXED); return; }
+    conn_row(t, c)[k] = o;
+    if (k + 1 >= MAX_CONNECT_CELL_NUM) set_err(t, EO_ERR_CAPACITY);
+#else
     if (t->nconn[c] >= MAX_CONNECT_CELL_NUM) { set_err(t, EO_ERR_CAPACITY); return; }
     conn_row(t, c)[t->nconn[c]++] = o;
     /* the reference asserts size < N2 after a push (cell_connection.cpp:122);
      * this restatement treats a full list as a capacity error for both ends */
     if (t->nconn[c] >= MAX_CONNECT_CELL_NUM) set_err(t, EO_ERR_CAPACITY);
+#endif
 }
 
 /* grid_init, cell_connection.cpp:38-62 */
 static void grid_init(eo_tissue* t) {
     const double LX = t->g.lx, LY = t->g.ly;
     memset(t->aindx, 0, sizeof(int32_t) * (size_t)t->ANX * t->ANY * t->ANZ);
+    EO_PAR
     for (int c = 0; c < t->n; c++) {
         long aix = (long)((0.5 * LX - p_diff_x(t, 0.5 * LX, t->x1[c])) / AREA_GRID);
         long aiy = (long)((0.5 * LY - p_diff_y(t, 0.5 * LY, t->y1[c])) / AREA_GRID);
@@ -426,8 +474,14 @@ static void grid_init(eo_tissue* t) {
             continue;
         }
         size_t b = ((size_t)aix * t->ANY + aiy) * t->ANZ + aiz;
+#ifdef EO_OMP
+        const int slot = __atomic_fetch_add(&t->aindx[b], 1, __ATOMIC_RELAXED);
+        if (slot >= N3 - 1) { set_err(t, EO_ERR_CAPACITY); __atomic_fetch_sub(&t->aindx[b], 1, __ATOMIC_RELAXED); }
+        else t->area[b * N3 + slot] = c;
+#else
         t->area[b * N3 + t->aindx[b]++] = c;
         if (t->aindx[b] >= N3) { set_err(t, EO_ERR_CAPACITY); t->aindx[b] = N3 - 1; }
+#endif
         t->nconn[c] = t->state[c] == EO_MEMB ? MEMB_ADJ_CONN_NUM : 0;
     }
 }
@@ -436,6 +490,7 @@ static void grid_init(eo_tissue* t) {
 static void connect_proc(eo_tissue* t) {
     const int ii = 2;
     const int ANX = t->ANX, ANY = t->ANY, ANZ = t->ANZ;
+    EO_PAR_DYN
     for (int c = t->nmemb; c < t->n; c++) {
         const long anx = (long)(t->x1[c] / AREA_GRID);
         const long any = (long)(t->y1[c] / AREA_GRID);
@@ -473,6 +528,7 @@ static void connect_proc(eo_tissue* t) {
 
 /* gj_refresh, cell_connection.cpp:133-152 */
 static void gj_refresh(eo_tissue* t) {
+    EO_PAR_DYN
     for (int c = 0; c < t->n; c++) {
         gj_map* m = &t->gj[c];
         for (int i = 0; i < m->n; i++) m->e[i].checked = 0;
@@ -484,6 +540,7 @@ static void gj_refresh(eo_tissue* t) {
 
 /* find_dermis / set_dermis, cell_connection.cpp:158-185 */
 static void set_dermis(eo_tissue* t) {
+    EO_PAR
     for (int c = t->nmemb + t->nder; c < t->n; c++) {
         if (t->state[c] == EO_FIX || t->state[c] == EO_MUSUME) {
             double d1Sq = t->g.lx * t->g.lx;
@@ -617,8 +674,8 @@ static void apply(eo_tissue* t, int law, int c1, int c2) {
     const double dumx = DT_Cell * coef * p_diff_x(t, t->x1[c1], t->x1[c2]);
     const double dumy = DT_Cell * coef * p_diff_y(t, t->y1[c1], t->y1[c2]);
     const double dumz = DT_Cell * coef * (t->z1[c1] - t->z1[c2]);
-    t->x2[c1] += dumx; t->y2[c1] += dumy; t->z2[c1] += dumz;
-    t->x2[c2] -= dumx; t->y2[c2] -= dumy; t->z2[c2] -= dumz;
+    add_d(&t->x2[c1], dumx); add_d(&t->y2[c1], dumy); add_d(&t->z2[c1], dumz);
+    add_d(&t->x2[c2], -dumx); add_d(&t->y2[c2], -dumy); add_d(&t->z2[c2], -dumz);
 }
 
 /* wall_interaction, cell_interaction.cpp:287-294 */
@@ -626,7 +683,7 @@ static void wall_interaction(eo_tissue* t, int c) {
     const double distlj = 2.0 * t->z1[c];
     const double LJ6 = POW6(t->radius[c]) / POW6(t->z1[c]);
     const double ljm = 4.0 * eps_m * LJ6 * (LJ6 - 1.0) / (distlj * distlj);
-    t->z2[c] += DT_Cell * ljm * 2.0 * t->z1[c];
+    add_d(&t->z2[c], DT_Cell * ljm * 2.0 * t->z1[c]);
 }
 
 /* vec3 (vec3.h:5-55) */
@@ -703,6 +760,7 @@ static inline int collide_with_wall(const eo_tissue* t, int c) { return t->z1[c]
 
 void eo_cell_interaction(eo_tissue* t) {
     /* memb_bend, :501-519 */
+    EO_PAR
     for (int m = 0; m < t->nmemb; m++) {
         v3 dr = tri_memb_dr(t, m);
         t->x2[m] += DT_Cell * KBEND * dr.x;
@@ -710,6 +768,7 @@ void eo_cell_interaction(eo_tissue* t) {
         t->z2[m] += DT_Cell * KBEND * dr.z;
     }
     /* _MEMB_interaction, :529-547 */
+    EO_PAR
     for (int m = 0; m < t->nmemb; m++) {
         if (collide_with_wall(t, m)) wall_interaction(t, m);
         int32_t* row = conn_row(t, m);
@@ -719,6 +778,7 @@ void eo_cell_interaction(eo_tissue* t) {
         }
     }
     /* non-membrane loop, :656-679 */
+    EO_PAR_DYN
     for (int c = t->nmemb; c < t->n; c++) {
         int32_t* row = conn_row(t, c);
         const int nc = t->nconn[c];
@@ -787,6 +847,7 @@ void eo_cell_interaction(eo_tissue* t) {
 
 /* pos_copy, cellmanager.cpp:19-26 */
 void eo_pos_copy(eo_tissue* t) {
+    EO_PAR
     for (int c = 0; c < t->n; c++) { t->x1[c] = t->x2[c]; t->y1[c] = t->y2[c]; t->z1[c] = t->z2[c]; }
 }
 
@@ -833,6 +894,23 @@ static int is_divide_ready(eo_tissue* t, int c) {
                stoch_corr_coef * DT_Cell * weighted_eps_kb(t, c) * S2;
     }
     return 0;
+}
+
+/* is_divide_ready, then cell_divide when it holds; 1 if the cell divided.
+ * The draw and the creation touch the shared replay cursor and cell count,
+ * so the multithreaded build takes them under one lock, entered only by
+ * cells old enough to draw */
+static void cell_divide(eo_tissue* t, int div);
+static int try_divide(eo_tissue* t, int c) {
+#ifdef EO_OMP
+    if (!(t->pair[c] < 0 && (t->ageb[c] >= t->div_age_thresh[c] * (1.0 - stoch_div_time_ratio)))) return 0;
+#endif
+    int divided = 0;
+    EO_CRITICAL
+    {
+        if (is_divide_ready(t, c)) { cell_divide(t, c); divided = 1; }
+    }
+    return divided;
 }
 
 /* calc_cell_uvec + div_direction, :168-217 */
@@ -939,12 +1017,17 @@ static void pair_disperse(eo_tissue* t, int c) {
 
 void eo_cell_state_renew(eo_tissue* t) {
     const int start = t->nmemb + t->nder, end = t->n;  /* bounds captured at call */
+    EO_PAR_DYN
     for (int c = start; c < end; c++) {
         switch (t->state[c]) {
         case EO_ALIVE: /* :347-358 */
             if (t->agek[c] >= THRESH_DEAD) {
                 t->state[c] = EO_DEAD;   /* cornificate, cellmanager.cpp:48-53 */
+#ifdef EO_OMP
+                __atomic_fetch_add(&t->sw, 1, __ATOMIC_RELAXED);
+#else
                 t->sw = (uint8_t)(t->sw + 1);
+#endif
             } else {
                 const double tmp = k_lipid_release(t, c) * t->in_fat[c];
                 t->in_fat[c] += DT_Cell * (k_lipid(t, c) * (1.0 - t->in_fat[c]) - tmp);
@@ -953,23 +1036,27 @@ void eo_cell_state_renew(eo_tissue* t) {
             }
             break;
         case EO_DEAD: case EO_AIR: /* :330-341 */
-            if (t->agek[c] >= 31.3 && t->nconn[c] <= 11) t->rq[t->nrq++] = c;
-            else t->agek[c] += DT_Cell * agek_DEAD_AIR_const();
+            if (t->agek[c] >= 31.3 && t->nconn[c] <= 11) {
+#ifdef EO_OMP
+                t->rq[__atomic_fetch_add(&t->nrq, 1, __ATOMIC_RELAXED)] = c;
+#else
+                t->rq[t->nrq++] = c;
+#endif
+            } else t->agek[c] += DT_Cell * agek_DEAD_AIR_const();
             break;
         case EO_FIX: /* :308-324 */
             if (t->dermis[c] < 0) { set_err(t, EO_ERR_NO_DERMIS); break; }
-            if (is_divide_ready(t, c)) cell_divide(t, c);
-            else t->ageb[c] += DT_Cell * ageb_const(t, c);
+            if (try_divide(t, c) == 0) t->ageb[c] += DT_Cell * ageb_const(t, c);
             break;
         case EO_MUSUME: /* :282-302 (SYSTEM == WHOLE) */
             if (t->dermis[c] < 0 && t->pair[c] < 0) { t->state[c] = EO_ALIVE; break; }
-            if (t->rest_div_times[c] > 0 && is_divide_ready(t, c)) cell_divide(t, c);
-            else t->ageb[c] += DT_Cell * ageb_const(t, c);
+            if (!(t->rest_div_times[c] > 0 && try_divide(t, c))) t->ageb[c] += DT_Cell * ageb_const(t, c);
             break;
         default: break;
         }
     }
     remove_exec(t);
+    EO_PAR
     for (int c = t->nmemb + t->nder; c < t->n; c++)
         if (t->pair[c] >= 0 && c > t->pair[c]) pair_disperse(t, c);
 }
@@ -987,6 +1074,7 @@ void eo_initialize_sc(eo_tissue* t, double zzmax) {
 /* cell_pos_periodic_fix, cellmanager.cpp:395-414 */
 void eo_cell_pos_periodic_fix(eo_tissue* t) {
     const double LX = t->g.lx, LY = t->g.ly;
+    EO_PAR
     for (int c = 0; c < t->n; c++) {
         if (t->x2[c] > LX) { double v = t->x2[c] - LX; t->x1[c] = t->x2[c] = v; }
         else if (t->x2[c] < 0) { double v = t->x2[c] + LX; t->x1[c] = t->x2[c] = v; }
@@ -1007,11 +1095,68 @@ double eo_calc_zzmax(eo_tissue* t) {
 }
 
 /* ---- setup_map_lat: map.cpp:18-167 ------------------------------------- */
+/* cmap1 voxel v = cell c; threads race on it as in the reference, so the
+ * multithreaded build stores atomically (the last writer wins either way) */
+#ifdef EO_OMP
+#define STAMP1(t, v, c) __atomic_store_n(&(t)->cmap1[v], (c), __ATOMIC_RELAXED)
+#else
+#define STAMP1(t, v, c) ((t)->cmap1[v] = (c))
+#endif
 static int lat_wrap(eo_tissue* t, long v, int N) {
     /* precalc_lat_* tables hold i%N for i in [0, 3N) (utils.cpp:18-30) */
     long i = v + N;
     if (i < 0 || i >= 3L * N) { set_err(t, EO_ERR_OUT_OF_GRID); i = ((i % N) + N) % N; }
     return (int)(i % N);
+}
+
+/* the stamps of one cell (map.cpp:42-154); dySq, ipy, dzSq are scratch of
+ * 4 iry + 1, 4 iry + 1 and 4 irz + 1 entries */
+static void stamp_cell(eo_tissue* t, int c, int irx, int iry, int irz, double* dySq, int* ipy, double* dzSq) {
+    const int NX = t->NX, NY = t->NY, NZ = t->NZ;
+    const int* clat = &t->lat[3 * c];
+    const int st = t->state[c];
+    const int afm = st == EO_ALIVE || st == EO_FIX || st == EO_MUSUME;
+    const int _irx = afm ? 2 * irx : irx, _iry = afm ? 2 * iry : iry, _irz = afm ? 2 * irz : irz;
+    const double normal_radSq = t->radius[c] * t->radius[c];
+    const double crad = FAC_MAP * t->radius[c];
+    const double cradSq = crad * crad;
+    const int ymax = clat[1] + _iry, ymin = clat[1] - _iry;
+    for (int l = ymin, yc = 0; l <= ymax; l++) {
+        const double my = l * t->dy;
+        ipy[yc] = lat_wrap(t, l, NY);
+        const double diffy = p_diff_y(t, my, t->y1[c]);
+        dySq[yc] = diffy * diffy;
+        yc++;
+    }
+    const int z_b = clat[2] - _irz, z_u = clat[2] + _irz;
+    const int zmin = z_b >= 1 ? z_b : 1;
+    const int zmax = z_u < NZ ? z_u : NZ - 1;
+    for (int m = zmin, zc = 0; m <= zmax; m++) {
+        const double mz = m * t->dz;
+        const double diffz = mz - t->z1[c];
+        dzSq[zc] = diffz * diffz;
+        zc++;
+    }
+    const int xmax = clat[0] + _irx, xmin = clat[0] - _irx;
+    for (int k = xmin; k <= xmax; k++) {
+        const double mx = k * t->dx;
+        const int ipx = lat_wrap(t, k, NX);
+        const double diffx = p_diff_x(t, mx, t->x1[c]);
+        const double diffxSq = diffx * diffx;
+        for (int yc = 0; yc < _iry * 2 + 1; yc++) {
+            const double tmps = diffxSq + dySq[yc];
+            for (int zc = 0, ipz = zmin; ipz <= zmax; ipz++, zc++) {
+                const double distSq = tmps + dzSq[zc];
+                size_t v = IDX3(t, ipx, ipy[yc], ipz);
+                if (!afm) {
+                    if (distSq < normal_radSq) STAMP1(t, v, c);
+                } else if (distSq < cradSq) {
+                    t->cmap2[v] = 1;
+                    if (distSq < normal_radSq) STAMP1(t, v, c);
+                }
+            }
+        }
+    }
 }
 
 void eo_setup_map_lat(eo_tissue* t) {
@@ -1020,62 +1165,28 @@ void eo_setup_map_lat(eo_tissue* t) {
     const int irx = (int)(Rad * NX / t->g.lx);
     const int iry = (int)(Rad * NY / t->g.ly);
     const int irz = (int)(Rad * NZ / t->g.lz);
+    EO_PAR
     for (size_t i = 0; i < t->gsz; i++) { t->cmap1[i] = -1; t->cmap2[i] = 0; }
+    EO_PAR
     for (int c = 0; c < t->n; c++) {  /* set_lattice :18-22 */
         t->lat[3 * c + 0] = lat_wrap(t, (long)(int)(t->x1[c] * t->inv_dx), NX);
         t->lat[3 * c + 1] = lat_wrap(t, (long)(int)(t->y1[c] * t->inv_dy), NY);
         t->lat[3 * c + 2] = lat_wrap(t, (long)(int)(t->z1[c] * t->inv_dz), NZ);
     }
-    double* dySq = (double*)malloc(sizeof(double) * (4 * iry + 1));
-    int* ipy = (int*)malloc(sizeof(int) * (4 * iry + 1));
-    double* dzSq = (double*)malloc(sizeof(double) * (4 * irz + 1));
-    for (int c = 0; c < t->n; c++) {
-        const int* clat = &t->lat[3 * c];
-        const int st = t->state[c];
-        const int afm = st == EO_ALIVE || st == EO_FIX || st == EO_MUSUME;
-        const int _irx = afm ? 2 * irx : irx, _iry = afm ? 2 * iry : iry, _irz = afm ? 2 * irz : irz;
-        const double normal_radSq = t->radius[c] * t->radius[c];
-        const double crad = FAC_MAP * t->radius[c];
-        const double cradSq = crad * crad;
-        const int ymax = clat[1] + _iry, ymin = clat[1] - _iry;
-        for (int l = ymin, yc = 0; l <= ymax; l++) {
-            const double my = l * t->dy;
-            ipy[yc] = lat_wrap(t, l, NY);
-            const double diffy = p_diff_y(t, my, t->y1[c]);
-            dySq[yc] = diffy * diffy;
-            yc++;
-        }
-        const int z_b = clat[2] - _irz, z_u = clat[2] + _irz;
-        const int zmin = z_b >= 1 ? z_b : 1;
-        const int zmax = z_u < NZ ? z_u : NZ - 1;
-        for (int m = zmin, zc = 0; m <= zmax; m++) {
-            const double mz = m * t->dz;
-            const double diffz = mz - t->z1[c];
-            dzSq[zc] = diffz * diffz;
-            zc++;
-        }
-        const int xmax = clat[0] + _irx, xmin = clat[0] - _irx;
-        for (int k = xmin; k <= xmax; k++) {
-            const double mx = k * t->dx;
-            const int ipx = lat_wrap(t, k, NX);
-            const double diffx = p_diff_x(t, mx, t->x1[c]);
-            const double diffxSq = diffx * diffx;
-            for (int yc = 0; yc < _iry * 2 + 1; yc++) {
-                const double tmps = diffxSq + dySq[yc];
-                for (int zc = 0, ipz = zmin; ipz <= zmax; ipz++, zc++) {
-                    const double distSq = tmps + dzSq[zc];
-                    size_t v = IDX3(t, ipx, ipy[yc], ipz);
-                    if (!afm) {
-                        if (distSq < normal_radSq) t->cmap1[v] = c;
-                    } else if (distSq < cradSq) {
-                        t->cmap2[v] = 1;
-                        if (distSq < normal_radSq) t->cmap1[v] = c;
-                    }
-                }
-            }
-        }
+#ifdef EO_OMP
+#pragma omp parallel
+#endif
+    {
+        double* dySq = (double*)malloc(sizeof(double) * (4 * iry + 1));
+        int* ipy = (int*)malloc(sizeof(int) * (4 * iry + 1));
+        double* dzSq = (double*)malloc(sizeof(double) * (4 * irz + 1));
+#ifdef EO_OMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int c = 0; c < t->n; c++) stamp_cell(t, c, irx, iry, irz, dySq, ipy, dzSq);
+        free(dySq); free(ipy); free(dzSq);
     }
-    free(dySq); free(ipy); free(dzSq);
+    EO_PAR
     for (int l = 0; l < NZ; l++) {  /* :156-166 */
         for (int j = 0; j <= NX; j++) {
             t->cmap2[IDX3(t, j, NY, l)] = t->cmap2[IDX3(t, j, 0, l)];
@@ -1102,9 +1213,10 @@ void eo_calc_ext_stim(eo_tissue* t, double zzmax) {
     const double* carr = t->bst[t->b_first];
     double* narr = t->bst[1 - t->b_first];
     const double inv_dz = t->inv_dz;
+    EO_PAR2
     for (int j = 0; j < NX; j++) {
-        const int prev_x = t->px_prev[j], next_x = t->px_next[j];
         for (int k = 0; k < NY; k++) {
+            const int prev_x = t->px_prev[j], next_x = t->px_next[j];
             const int prev_y = t->py_prev[k], next_y = t->py_next[k];
             for (int l = 0; l < iz_bound; l++) {
                 const int prev_z = t->pz_prev[l], next_z = t->pz_next[l];
@@ -1129,6 +1241,7 @@ void eo_calc_ext_stim(eo_tissue* t, double zzmax) {
             }
         }
     }
+    EO_PAR
     for (int l = 0; l < iz_bound; l++) {
         for (int j = 0; j < NX; j++) narr[IDX3(t, j, NY, l)] = narr[IDX3(t, j, 0, l)];
         for (int k = 0; k <= NY; k++) narr[IDX3(t, NX, k, l)] = narr[IDX3(t, 0, k, l)];
@@ -1203,7 +1316,9 @@ static inline int is_afm(int st) { return st == EO_ALIVE || st == EO_FIX || st =
 typedef struct { int32_t* diffu_cell; uint8_t* air; int iz_bound; } ca_maps;
 
 static void init_ca2p_map(eo_tissue* t, ca_maps* m) {
+    EO_PAR
     for (size_t i = 0; i < t->gsz; i++) { m->diffu_cell[i] = -1; m->air[i] = 0; }
+    EO_PAR
     for (int j = 0; j < t->NX; j++)
         for (int k = 0; k < t->NY; k++)
             for (int l = 0; l < m->iz_bound; l++) {
@@ -1223,6 +1338,7 @@ static void init_ca2p_map(eo_tissue* t, ca_maps* m) {
 static void dead_IP3_calc(eo_tissue* t) {
     const double* ip3 = t->ip3_buf[t->ca_first];
     double* ip3n = t->ip3_buf[1 - t->ca_first];
+    EO_PAR
     for (int c = t->nmemb + t->nder; c < t->n; c++) {
         if (t->state[c] == EO_DEAD) {
             int count = 0;
@@ -1247,6 +1363,7 @@ static void supra_calc(eo_tissue* t) {
     double* ip3n = t->ip3_buf[1 - t->ca_first];
     const double* atp = t->atp[t->atp_first];
     const double* B = t->bst[t->b_first];
+    EO_PAR_DYN
     for (int c = t->nmemb + t->nder; c < t->n; c++) {
         const int st = t->state[c];
         if (!is_afm(st)) continue;
@@ -1286,9 +1403,10 @@ static void ATP_refresh(eo_tissue* t, const ca_maps* m) {
     double* narr = t->atp[1 - t->atp_first];
     const uint8_t* cm2 = t->cmap2;
     const double inv_dx = t->inv_dx;
+    EO_PAR2
     for (int j = 0; j < NX; j++) {
-        const int prev_x = t->px_prev[j], next_x = t->px_next[j];
         for (int k = 0; k < NY; k++) {
+            const int prev_x = t->px_prev[j], next_x = t->px_next[j];
             const int prev_y = t->py_prev[k], next_y = t->py_next[k];
             for (int l = 0; l < iz_bound; l++) {
                 const int prev_z = t->pz_prev[l], next_z = t->pz_next[l];
@@ -1307,6 +1425,7 @@ static void ATP_refresh(eo_tissue* t, const ca_maps* m) {
             }
         }
     }
+    EO_PAR
     for (int l = 0; l <= iz_bound; l++) {
         for (int j = 0; j <= NX; j++) narr[IDX3(t, j, NY, l)] = narr[IDX3(t, j, 0, l)];
         for (int k = 0; k <= NY; k++) narr[IDX3(t, NX, k, l)] = narr[IDX3(t, 0, k, l)];

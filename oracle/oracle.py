@@ -13,6 +13,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libepi_oracle.so")
+# the same restatement with OpenMP threads over the loops epi5_MT runs in
+# parallel: the CPU throughput baseline only (its list and accumulation
+# orders follow the thread schedule, like the reference's under TBB)
+LIB_PATH_MT = os.path.join(HERE, "libepi_oracle_mt.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -62,7 +66,7 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
 class RdParams(C.Structure):
@@ -80,12 +84,12 @@ class RdGrad(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", _dp)]
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(mt=False):
+    path = LIB_PATH_MT if mt else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         L.eo_create.restype = C.c_void_p
         L.eo_create.argtypes = [C.POINTER(Geo)]
         L.eo_destroy.argtypes = [C.c_void_p]
@@ -130,8 +134,8 @@ def lib():
         L.eo_coef.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.eo_coef.restype = C.c_double
         L.eo_tri_bend.argtypes = [C.c_void_p, C.c_int, _dp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def ptr(a, t):
@@ -153,8 +157,8 @@ def mt19937_64_raw(seed, n):
 class OracleTissue:
     """One tissue in the CPU restatement, with the reference call surface."""
 
-    def __init__(self, geometry):
-        self.L = lib()
+    def __init__(self, geometry, mt=False):
+        self.L = lib(mt)
         g = geometry
         self.geo = g
         self._g = Geo(g.nmx, g.nmy, g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.max_cells)
