@@ -5,8 +5,11 @@
 // order of the additions.  Here every cell gathers its own contributions and
 // adds them in the order of the serial loop: bending, wall, its own handler
 // over its list, lower-index handlers before and higher-index handlers after
-// (cell_interaction.cpp:650-679).  One thread owns one cell: no atomics, the
-// FP64 result is the serial one bit for bit.
+// (cell_interaction.cpp:650-679).  In k_forces a group of FG = 16 lanes owns
+// one cell: the lanes compute the cell's terms (neighbour deltas and the 12
+// bend dihedrals) into LDS and the group's leader adds them in that order;
+// k_forces_long, for rows longer than the LDS slots, gives one lane one
+// cell.  No atomics: the FP64 result is the serial one bit for bit.
 #include "epi_dev.h"
 #include "launch.h"
 #include "epi_group.h"

@@ -4,7 +4,8 @@
 //
 //   proc <deck> <steps> [--seed S] [--force-cornif] [--ca-every K]
 //        [--geometry nmx nmy nx ny nz lx ly lz max_cells] [--dump FILE]
-//        [--output FILE] [--snapshots DIR [--cut N]]
+//        [--output FILE] [--output-binary FILE] [--snapshots DIR [--cut N]]
+//        [--save-last DIR] [--use-last DIR]
 //
 // Defaults: define.h's geometry (NMX 100, NMY_tri 114, 100x100x200 grid on
 // 50x50x100), seed 1, the reference's Ca gating only.  --dump writes the
@@ -13,6 +14,13 @@
 // DIR/<i / N> before every step i with i % N == 0 (output_cell_data,
 // proc.cpp:45-51, 139-141; N defaults to CUT = 1000): the numbered series the
 // reference's viewers read (epi_vis_lightweight/main.cpp:493-520).
+// --output-binary writes CellManager::output's binary form.  --save-last
+// writes output_field_data's five files (last_data_cell, last_data_uvp,
+// last_data_w_alt, last_data_a_alt, last_data_B_alt) to DIR every 10 N steps
+// (proc.cpp:142-152) and after the last step; --use-last starts from such a
+// set, with DIR/last_data_cell given as the deck: load_uvp / load_w and both
+// buffers of ATP and ext_stim from the binaries (cman_init with use_last,
+// proc.cpp:98-110).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -38,7 +46,8 @@ int usage() {
   std::fprintf(stderr,
                "usage: proc <deck> <steps> [--seed S] [--force-cornif] [--ca-every K]\n"
                "            [--geometry nmx nmy nx ny nz lx ly lz max_cells] [--dump FILE] [--output FILE]\n"
-               "            [--snapshots DIR [--cut N]]\n");
+               "            [--output-binary FILE] [--snapshots DIR [--cut N]] [--save-last DIR]\n"
+               "            [--use-last DIR]\n");
   return 2;
 }
 
@@ -52,7 +61,7 @@ int main(int argc, char** argv) {
   unsigned long long seed = 1;
   bool force_cornif = false;
   long ca_every = 0;
-  std::string dump, output, snapdir;
+  std::string dump, output, output_bin, snapdir, save_dir, last_dir;
   long cut = 1000;   // CUT (define.h)
   for (int i = 3; i < argc; i++) {
     const std::string a = argv[i];
@@ -61,6 +70,9 @@ int main(int argc, char** argv) {
     else if (a == "--ca-every" && i + 1 < argc) ca_every = std::atol(argv[++i]);
     else if (a == "--dump" && i + 1 < argc) dump = argv[++i];
     else if (a == "--output" && i + 1 < argc) output = argv[++i];
+    else if (a == "--output-binary" && i + 1 < argc) output_bin = argv[++i];
+    else if (a == "--save-last" && i + 1 < argc) save_dir = argv[++i];
+    else if (a == "--use-last" && i + 1 < argc) last_dir = argv[++i];
     else if (a == "--snapshots" && i + 1 < argc) snapdir = argv[++i];
     else if (a == "--cut" && i + 1 < argc) cut = std::atol(argv[++i]);
     else if (a == "--geometry" && i + 9 < argc) {
@@ -72,7 +84,13 @@ int main(int argc, char** argv) {
   }
   try {
     CellManager cman(g);
-    cman_init(cman, deck);
+    if (last_dir.empty()) {
+      cman_init(cman, deck);
+    } else {
+      cman_init(cman, deck, true, last_dir + "/" + last_data_uvp_name, last_dir + "/" + last_data_w_name);
+      cman.atp().read_binary(last_dir + "/" + last_data_a_name);
+      cman.ext_stim().read_binary(last_dir + "/" + last_data_B_name);
+    }
     // genrand_real replay: a generous stream for the run
     cman.seed_random(seed, (long long)(steps + 64) * ((long long)cman.size() / 4 + 64) + 100000);
     auto ATP = cman.atp();
@@ -93,6 +111,10 @@ int main(int argc, char** argv) {
         std::fflush(stdout);
       }
       if (!snapdir.empty() && cut > 0 && i % cut == 0) cman.output(snapdir + "/" + std::to_string(i / cut));
+      if (!save_dir.empty() && cut > 0 && i % (cut * 10) == 0 && i > 0) {   // proc.cpp:142-152
+        cman.clean_up();
+        output_field_data(cman, ATP.first(), ext_stim.first(), save_dir);
+      }
       cell_dynamics(cman);
       zzmax = calc_zzmax(cman);
       setup_map_lat(cman, cell_map1, cell_map2);
@@ -114,6 +136,8 @@ int main(int argc, char** argv) {
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     std::printf("done: %ld steps, %zu cells, %ld Ca solves, %.3f s\n", steps, cman.size(), ca_solves, s);
     if (!output.empty()) cman.output(output);
+    if (!output_bin.empty()) cman.output(output_bin, true);
+    if (!save_dir.empty()) output_field_data(cman, ATP.first(), ext_stim.first(), save_dir);
     if (!dump.empty()) {
       const CellData d = cman.download();
       FILE* f = std::fopen(dump.c_str(), "wb");

@@ -192,17 +192,40 @@ class CellManager {
     return d;
   }
 
-  // CellManager::output text form (cellmanager.cpp:451-538): the 17 deck
-  // columns, states written as integers
-  void output(const std::string& filename) const {
+  // CellManager::output (cellmanager.cpp:451-538): the 17 deck columns per
+  // cell, is_touch from check_localization (:430-445: an ALIVE cell with a
+  // DEAD neighbour).  Text: doubles in fixed notation with 15 decimals, as
+  // the reference's `fixed << setprecision(15)`; the state is written as an
+  // integer where the reference streams the raw state byte, which its own
+  // loader cannot read back.  Binary: per cell, in this order and without
+  // padding, int32 index, int32 state, radius, ageb, agek, ca2p, x, y, z,
+  // ca2p_avg (doubles), int32 rest_div_times, ex_fat, in_fat (doubles), char
+  // is_touch, double spr_nat_len, int32 pair (-1 = none), int32 fix_origin
+  // (109 bytes; :496-513).
+  void output(const std::string& filename, bool binary_mode = false) const {
     const CellData d = download();
-    FILE* f = std::fopen(filename.c_str(), "w");
-    if (!f) throw Error(EPI_ERR_INVALID, "output open error: " + filename);
+    std::vector<char> touch((size_t)d.n, 0);
     for (int i = 0; i < d.n; i++)
-      std::fprintf(f, "%d %d %1.14e %1.14e %1.14e %1.14e %1.14e %1.14e %1.14e %1.14e %d %1.14e %1.14e %d %1.14e %d %d\n",
-                   i, (int)d.state[i], d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i],
-                   d.ca2p_avg[i], d.rest_div_times[i], d.ex_fat[i], d.in_fat[i], 0, d.spr_nat_len[i],
-                   d.pair[i], d.fix_origin[i]);
+      if (d.state[i] == ALIVE)
+        for (int e = d.conn_ptr[i]; e < d.conn_ptr[i + 1]; e++)
+          if (d.state[d.conn[e]] == DEAD) { touch[i] = 1; break; }
+    FILE* f = std::fopen(filename.c_str(), binary_mode ? "wb" : "w");
+    if (!f) throw Error(EPI_ERR_INVALID, "Output file creation error. Filename:" + filename);
+    for (int i = 0; i < d.n; i++) {
+      if (binary_mode) {
+        const int32_t idx = i, st = d.state[i], rdt = d.rest_div_times[i], pr = d.pair[i], fo = d.fix_origin[i];
+        const double a[8] = {d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i], d.ca2p_avg[i]};
+        const double b[2] = {d.ex_fat[i], d.in_fat[i]};
+        std::fwrite(&idx, 4, 1, f); std::fwrite(&st, 4, 1, f); std::fwrite(a, 8, 8, f);
+        std::fwrite(&rdt, 4, 1, f); std::fwrite(b, 8, 2, f); std::fwrite(&touch[i], 1, 1, f);
+        std::fwrite(&d.spr_nat_len[i], 8, 1, f); std::fwrite(&pr, 4, 1, f); std::fwrite(&fo, 4, 1, f);
+      } else {
+        std::fprintf(f, "%d %d %.15f %.15f %.15f %.15f %.15f %.15f %.15f %.15f %d %.15f %.15f %d %.15f %d %d\n",
+                     i, (int)d.state[i], d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i],
+                     d.ca2p_avg[i], d.rest_div_times[i], d.ex_fat[i], d.in_fat[i], (int)touch[i], d.spr_nat_len[i],
+                     d.pair[i], d.fix_origin[i]);
+      }
+    }
     std::fclose(f);
   }
 
@@ -404,6 +427,56 @@ inline void cman_init(CellManager& cells, const std::string& init_data_path, boo
     load_last(cells, ld_uvp, ld_w);
   else
     check(cells.ctx(), epi_value_init(cells.ctx(), 0), "epi_value_init");
+}
+
+// ---- saving the last state (proc.cpp:53-84) ---------------------------------
+// the file names of define.h:219-224
+constexpr const char* last_data_uvp_name = "last_data_uvp";
+constexpr const char* last_data_cell_name = "last_data_cell";
+constexpr const char* last_data_w_name = "last_data_w_alt";
+constexpr const char* last_data_a_name = "last_data_a_alt";
+constexpr const char* last_data_B_name = "last_data_B_alt";
+
+// output_uvp (proc.cpp:53-60): "index ca2p ex_inert IP3" per cell, the rows
+// load_last reads
+inline void output_uvp(const CellManager& cman, const std::string& filename) {
+  const CellData d = cman.download();
+  FILE* f = std::fopen(filename.c_str(), "w");
+  if (!f) throw Error(EPI_ERR_INVALID, "uvp output error. Filename:" + filename);
+  for (int i = 0; i < d.n; i++) std::fprintf(f, "%d %1.14e %1.14e %1.14e\n", i, d.ca2p[i], d.ex_inert[i], d.ip3[i]);
+  std::fclose(f);
+}
+
+// output_w (proc.cpp:63-72): per cell "index count", then count rows
+// "neighbour gj".  The reference walks its per-cell gj map, which also holds
+// entries of past neighbours (reset to gj_init at every rebuild until
+// clean_up erases them); here the entries are the current neighbours, in
+// list order.  A past neighbour's entry would be read back as gj_init and
+// reset to it at the next rebuild, so the two files load to the same state.
+inline void output_w(const CellManager& cman, const std::string& filename) {
+  const CellData d = cman.download();
+  FILE* f = std::fopen(filename.c_str(), "w");
+  if (!f) throw Error(EPI_ERR_INVALID, "w output error. Filename:" + filename);
+  for (int i = 0; i < d.n; i++) {
+    std::fprintf(f, "%d %d\n", i, d.conn_ptr[i + 1] - d.conn_ptr[i]);
+    for (int e = d.conn_ptr[i]; e < d.conn_ptr[i + 1]; e++) std::fprintf(f, "%d %1.14e\n", d.conn[e], d.gj[e]);
+  }
+  std::fclose(f);
+}
+
+// output_field_data (proc.cpp:74-84): the cells as text, uvp, w and the first
+// buffers of ATP and ext_stim as raw [x][y][z] doubles, under dir (the
+// reference writes them to the working directory)
+inline void output_field_data(CellManager& cman, const FArr3D<double>& ATP_first,
+                              const FArr3D<double>& ext_stim_first, const std::string& dir = ".") {
+  detail::same_owner(cman, ATP_first.owner(), "output_field_data");
+  detail::same_owner(cman, ext_stim_first.owner(), "output_field_data");
+  const std::string p = dir.empty() ? std::string() : dir + "/";
+  cman.output(p + last_data_cell_name);
+  output_uvp(cman, p + last_data_uvp_name);
+  output_w(cman, p + last_data_w_name);
+  ATP_first.output_binary(p + last_data_a_name);
+  ext_stim_first.output_binary(p + last_data_B_name);
 }
 
 }  // namespace epi_amd

@@ -32,7 +32,8 @@ def deck_spec():
 # GPU tests run from the most basic to the most composite: single-tissue
 # parity first, then the windows and their adjoints, then batches.  Within a
 # module the file order is kept.
-_GPU_ORDER = ("test_gpu_parity", "test_gpu_rd_adjoint", "test_gpu_mech_adjoint", "test_gpu_batched")
+_GPU_ORDER = ("test_gpu_parity", "test_gpu_rd_adjoint", "test_gpu_mech_adjoint", "test_gpu_batched",
+              "test_gpu_configs")
 
 
 def pytest_collection_modifyitems(config, items):

@@ -77,8 +77,8 @@ def test_driver_matches_python_driver_and_oracle(tmp_path, oracle_mod):
     state = np.array([int(r_[1]) for r_ in rows])
     xyz0 = np.array([[float(r_[6]), float(r_[7]), float(r_[8])] for r_ in rows])
     assert np.array_equal(state, spec0["cells"]["state"])
-    # CellManager::output prints %1.14e (15 significant digits)
-    assert np.allclose(xyz0, np.stack([spec0["cells"][k] for k in ("x", "y", "z")], 1), rtol=1e-13, atol=0)
+    # CellManager::output prints doubles with 15 decimals (fixed << setprecision(15))
+    assert np.allclose(xyz0, np.stack([spec0["cells"][k] for k in ("x", "y", "z")], 1), rtol=0, atol=1e-14)
     raw = np.fromfile(dump, dtype=np.float64)
     n = int(raw[0])
     x, y, z, ca2p, agek = raw[1:].reshape(5, n)
@@ -109,3 +109,81 @@ def test_driver_matches_python_driver_and_oracle(tmp_path, oracle_mod):
     # Ca values carry ulp differences of tanh between glibc and the GPU
     rel = np.max(np.abs(oc["ca2p"] - ca2p) / np.maximum(np.abs(oc["ca2p"]), 1e-300))
     assert rel <= 1e-10, rel
+
+
+REC = np.dtype([("index", "<i4"), ("state", "<i4"), ("radius", "<f8"), ("ageb", "<f8"), ("agek", "<f8"),
+                ("ca2p", "<f8"), ("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("ca2p_avg", "<f8"),
+                ("rest_div_times", "<i4"), ("ex_fat", "<f8"), ("in_fat", "<f8"), ("is_touch", "i1"),
+                ("spr_nat_len", "<f8"), ("pair", "<i4"), ("fix_origin", "<i4")])
+
+
+def test_binary_record_layout():
+    """CellManager::output's binary record (cellmanager.cpp:496-513): the
+    fields back to back, no padding, 109 bytes"""
+    assert REC.itemsize == 4 + 4 + 8 * 8 + 4 + 8 + 8 + 1 + 8 + 4 + 4 == 109
+
+
+def _read_w(path):
+    out, lines = {}, open(path).read().splitlines()
+    i = 0
+    while i < len(lines):
+        c, k = map(int, lines[i].split())
+        out[c] = [(int(a), b) for a, b in (ln.split() for ln in lines[i + 1:i + 1 + k])]
+        i += 1 + k
+    return out
+
+
+@pytest.mark.gpu
+def test_last_data_round_trip(tmp_path):
+    """output_field_data's five files (proc.cpp:74-84) and the binary form of
+    CellManager::output, written by the driver after a run, read back by
+    cman_init with use_last and Field::read_binary (proc.cpp:98-110), and
+    written again: the field binaries and the uvp / w text come back
+    unchanged, the cells within the text's 15 decimals, and the binary
+    record equals the run's final state bit for bit."""
+    assert os.path.exists(DRIVER), "epi_amd/host/epi_proc is not built"
+    spec0 = configs.config_tiny(7, 160)
+    g = spec0["geometry"]
+    deck_p = tmp_path / "deck.txt"
+    deck.write_deck(spec0, deck_p)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    dump, binp = tmp_path / "dump.bin", tmp_path / "cells.bin"
+    print("last data round trip: 30 steps, Ca every 15, then a restart", flush=True)
+    r = subprocess.run([DRIVER, str(deck_p), "30", "--seed", "5", "--ca-every", "15", "--save-last", str(a),
+                        "--output-binary", str(binp), "--dump", str(dump), *_geometry_args(g)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    names = ["last_data_cell", "last_data_uvp", "last_data_w_alt", "last_data_a_alt", "last_data_B_alt"]
+    assert sorted(os.listdir(a)) == sorted(names)
+    raw = np.fromfile(dump, dtype=np.float64)
+    n = int(raw[0])
+    x, y, z, ca2p, agek = raw[1:].reshape(5, n)
+    rec = np.fromfile(binp, dtype=REC)
+    assert rec.size == n and np.array_equal(rec["index"], np.arange(n))
+    for k, v in (("x", x), ("y", y), ("z", z), ("ca2p", ca2p), ("agek", agek)):
+        assert np.array_equal(rec[k], v), k
+    # is_touch: ALIVE with a DEAD neighbour (check_localization); the text
+    # columns agree with the binary record
+    txt = np.loadtxt(a / "last_data_cell", ndmin=2)
+    assert txt.shape == (n, 17)
+    for col, k in ((1, "state"), (10, "rest_div_times"), (13, "is_touch"), (15, "pair"), (16, "fix_origin")):
+        assert np.array_equal(txt[:, col].astype(np.int64), rec[k].astype(np.int64)), k
+    for col, k in ((2, "radius"), (3, "ageb"), (4, "agek"), (5, "ca2p"), (6, "x"), (7, "y"), (8, "z"),
+                   (9, "ca2p_avg"), (11, "ex_fat"), (12, "in_fat"), (14, "spr_nat_len")):
+        assert np.allclose(txt[:, col], rec[k], rtol=0, atol=1e-14), k
+    # restart from the saved set and save again at once
+    r = subprocess.run([DRIVER, str(a / "last_data_cell"), "0", "--use-last", str(a), "--save-last", str(b),
+                        *_geometry_args(g)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for k in ("last_data_a_alt", "last_data_B_alt"):
+        fa, fb = (a / k).read_bytes(), (b / k).read_bytes()
+        assert len(fa) == 8 * (g.nx + 1) * (g.ny + 1) * (g.nz + 1) and fa == fb, k
+    # 15 significant digits read and written again give the same text
+    assert (a / "last_data_uvp").read_text() == (b / "last_data_uvp").read_text()
+    wa, wb = _read_w(a / "last_data_w_alt"), _read_w(b / "last_data_w_alt")
+    assert len(wa) == n and all(sorted(wa[c]) == sorted(wb[c]) for c in wa)
+    ta, tb = np.loadtxt(a / "last_data_cell", ndmin=2), np.loadtxt(b / "last_data_cell", ndmin=2)
+    assert np.array_equal(ta[:, [0, 1, 10, 15, 16]], tb[:, [0, 1, 10, 15, 16]])
+    assert np.allclose(ta, tb, rtol=0, atol=1e-14)
