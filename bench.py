@@ -60,6 +60,11 @@ MAP_KERNELS = ("k_zz_set", "k_zzmax", "k_map_clear", "k_map_stamp", "k_map_halo"
 MAP_BYTES_PER_VOXEL = 5.0
 
 
+def progress(msg):
+    """a phase of the run, on stderr (the JSON line alone goes to stdout)"""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,6 +393,7 @@ def run_batched(args):
 
     tuning, tune_s = None, 0.0
     if not args.no_tune:
+        progress(f"autotuning on {len(specs)} tissues")
         tt = make(range(len(specs)))
         tuning, rep = autotune.tune(tt, drivers(len(specs)), warm_steps=1, b_reps=5, ca_sub=10,
                                     log=sys.stderr)
@@ -401,6 +407,7 @@ def run_batched(args):
         ts.step(drv, args.warmup)
     ts.synchronize()
     s0 = [ts.status(t)[0] for t in range(len(specs))]
+    progress(f"timing {args.steps} batch steps")
     timed_region_profile(ts, None)
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
@@ -424,6 +431,7 @@ def run_batched(args):
     idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
                      for i in range(args.loop_sample)))
     loop_cells, loop_wall = 0, 0.0
+    progress(f"per-instance loop over {len(idx)} tissues")
     for i in idx:
         one = make([i])
         if tuning:
@@ -719,6 +727,7 @@ def run_plain(args):
         tt = native.TissueSet([geo])
         tt.load(0, spec)
         tt.set_rng(0, draws)
+        progress("autotuning the launch shapes")
         tuning, rep = autotune.tune(tt, [native.Driver(0, 0, 0, 0, 0)], log=sys.stderr)
         tune_s = rep["seconds"]
         tt.close()
@@ -735,6 +744,7 @@ def run_plain(args):
         ts.step(drv, args.warmup)
     ts.synchronize()
     s0, _ = ts.status(0)
+    progress(f"timing {args.steps} steps")
     timed_region_profile(ts, None, sample=not args.profile_all)
     t0 = time.perf_counter()
     ts.step(drv, args.steps)
@@ -772,6 +782,7 @@ def run_plain(args):
                   f"{100*ms/max(tot,1e-9):5.1f}%", file=sys.stderr)
     cpu = None
     if not args.no_cpu:
+        progress("CPU baseline sample")
         cpu = cpu_baseline(spec, seed, cell_steps, ca_solves, args.steps, args.cpu_seconds)
     out = {
         "metric": "cell-steps/s + grid-cell-steps/s (FP64, 1e-10 vs ref); batched & adjoint vs plain",

@@ -2,8 +2,9 @@
 
 The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
 
-  k_ext_stim   z planes per thread (1, 2, 4, 8) x y rows per block (1, 2, 4)
-  k_atp        the same
+  k_ext_stim   z planes per tile (1, 2, 4, 8) x 64-lane rows per block (1, 2, 4),
+               then its tile list in one run per XCD or not (b_xcd 0, 1)
+  k_atp        the same shapes
   k_ca_cells   lanes per cell (4, 8, 16)
   k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
   Ca solve     ATP of a sub-step and cells of the next in one launch (1; 2 =
@@ -28,7 +29,8 @@ import time
 
 STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
-DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0, "map_lanes": 1}
+DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0, "map_lanes": 1,
+           "b_xcd": 0}
 
 
 def _avg_us(ts, kernel):
@@ -86,6 +88,12 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
         report["k_ext_stim"][f"zb{zb}_rows{rows}"] = _time(ts, "k_ext_stim", run_b)
     key = min(report["k_ext_stim"], key=report["k_ext_stim"].get)
     best["b_zb"], best["b_rows"] = (int(v) for v in key[2:].split("_rows"))
+    ts.set_tuning(best)
+    report["b_xcd"] = {}
+    for x in (0, 1):
+        ts.set_tuning({"b_xcd": x})
+        report["b_xcd"][f"xcd{x}"] = _time(ts, "k_ext_stim", run_b)
+    best["b_xcd"] = int(min(report["b_xcd"], key=report["b_xcd"].get)[3:])
     ts.set_tuning(best)
 
     def run_ca():

@@ -80,6 +80,9 @@ struct Status {
   long long next_uid;
   unsigned long long zzmax_bits;  // calc_zzmax result (non-negative double bits)
   double zzmax;
+  long long timestep;     // CellManager::current_timestep; k_remove advances it once per renewal
+  int nstat;              // records in cst_store
+  int pad_st;
 };
 
 // ---- per-tissue descriptor (constant between allocations) ---------------
@@ -121,6 +124,10 @@ struct Tis {
   // when a mask is not 0/1 and the corner reads the map instead)
   uint2* ca_cq;
   long long* uid; long long* old_uid; int* src_row;
+  // cell_stat (define.h:198-202): per cell k_aging_start, k_cornified,
+  // k_disap (-1 = never set); cst_store (cellmanager.h:106): the records of
+  // removed DEAD cells, up to cap of them, Status::nstat in use
+  long long* cst; long long* cst_store;
   // neighbour rows (two buffers: current and previous build)
   int* nlen[2]; int* col[2]; double* gj[2];
   int* nlo;                   // entries of the row below the cell index

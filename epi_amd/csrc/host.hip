@@ -85,7 +85,7 @@ const char* kKernelNames[K_COUNT] = {
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
-    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place"};
+    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place", "k_wl"};
 
 // with ready_only, only the pairs whose kernels have finished (the rest stay
 // pending for a later read-back)
@@ -132,6 +132,18 @@ struct epi_ctx {
   double* h_zz = nullptr;
   int* d_tlist = nullptr;
   int* h_tlist = nullptr;
+  // flattened work lists of the launches over a batch (offsets and per-tissue
+  // iz_bound, written on the device): WL_INTS(T) ints
+  int* d_wl = nullptr;
+  // a Ca solve's list (CaList): built in pinned h_cal, uploaded to d_cal
+  // (cal_ints ints) on the stream, ev_cal recorded after the upload; kept
+  // while its key (tissues, cell counts, iz_bound, shapes) holds
+  int* d_cal = nullptr;
+  int* h_cal = nullptr;
+  long long cal_ints = 0;
+  hipEvent_t ev_cal = nullptr;
+  std::vector<long long> cal_key;
+  CaList cal{};
   char* arena = nullptr;
   size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
@@ -176,6 +188,9 @@ struct epi_ctx {
     size_t work_bytes = 0;
   } mech;
   std::vector<long long> ca_solves;
+  // each tissue's device timestep (Status::timestep) as the host last set or
+  // advanced it
+  std::vector<long long> dev_ts;
   // throughput accounting of epi_step
   std::vector<long long> steps, cell_steps, grid_b, grid_ca;
   std::string err;
@@ -244,6 +259,8 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.ca_cq = cv.take<uint2>(cap * 8);
   t.uid = cv.take<long long>(cap);
   t.old_uid = cv.take<long long>(cap);
+  t.cst = cv.take<long long>(3 * cap);
+  t.cst_store = cv.take<long long>(3 * cap);
   t.src_row = cv.take<int>(cap);
   for (int b = 0; b < 2; b++) {
     t.nlen[b] = cv.take<int>(cap);
@@ -374,6 +391,20 @@ int izb_est(const epi_ctx* c, bool ca) {
   return m;
 }
 
+// tiles of the B stencil's work list (k_ext_stim) from the last known
+// zzmax, one plane more than it gives, for sizing the grid
+long long b_tiles_est(const epi_ctx* c) {
+  const int bs = 64 * c->tu.b_rows, zb = c->tu.b_zb;
+  long long n = 0;
+  for (int t = 0; t < c->T; t++) {
+    const Tis& ti = c->tis[t];
+    const double zz = c->h_st ? c->h_st[t].zzmax : 0.0;
+    const int izb = std::min((int)((zz + FAC_MAP * R_max) * ti.inv_dz) + 1, c->th[t].NZ);
+    n += (long long)((ti.NX * ti.NY + bs - 1) / bs) * ((std::max(izb, 1) + zb - 1) / zb);
+  }
+  return n;
+}
+
 long long dev_index(const TH& h, int j, int k, int l) { return (long long)l * h.PL + (long long)k * h.XP + j; }
 
 // [x][y][z] (z fastest) <-> device layout
@@ -424,12 +455,68 @@ int set_zz(epi_ctx* c, const double* zzmax) {
   return EPI_OK;
 }
 
+// The list of a Ca solve (or window) over tissues ids[0..ntis) (nullptr:
+// all of them), for cg lanes per cell and ATP tiles of bs points by zb
+// planes, from what the host holds: the cell counts of the last status
+// read-back (with a margin of 64 cells; a launch that falls short reports
+// ERR_INVALID) and the zzmax of set_zz or of the read-back, whose iz_bound
+// (ca2p.cpp:386) is the device's.  Reused while its key holds.
+int ca_list(epi_ctx* c, const int* ids, int ntis, int cg, int bs, int zb, CaList* out) {
+  const int cpb = 128 / cg;   // cells per block of the Ca cell kernels
+  std::vector<long long> key = {ntis, cg, bs, zb};
+  std::vector<int> nb(ntis), izb(ntis);
+  long long ncb = 0, tiles = 0;
+  for (int i = 0; i < ntis; i++) {
+    const int t = ids ? ids[i] : i;
+    const TH& h = c->th[t];
+    const int cells = std::max(h.n_est + 64 - h.nmemb - h.nder, 1);
+    nb[i] = (cells + cpb - 1) / cpb;
+    izb[i] = (int)((c->h_st[t].zzmax + FAC_MAP * R_max) / c->tis[t].dz);
+    if (izb[i] < 0 || izb[i] > h.NZ) izb[i] = 0;   // k_ca_vox reports it
+    ncb += nb[i];
+    tiles += (long long)((h.NX * h.NY + bs - 1) / bs) * ((izb[i] + zb - 1) / zb);
+    key.insert(key.end(), {t, nb[i], izb[i]});
+  }
+  const long long hdr = (3LL * ntis + 1 + 3) / 4 * 4;
+  if (hdr + 4 * ncb > c->cal_ints || tiles > INT32_MAX) return fail(c, EPI_ERR_INVALID, "Ca work list capacity");
+  if (key != c->cal_key) {
+    HIPCHK(c, hipEventSynchronize(c->ev_cal));   // the last upload has read h_cal
+    int* aoff = c->h_cal;
+    int* aizb = aoff + ntis + 1;
+    int* atis = aizb + ntis;
+    int* cm = c->h_cal + hdr;
+    int o = 0, b = 0;
+    for (int i = 0; i < ntis; i++) {
+      const int t = ids ? ids[i] : i;
+      const TH& h = c->th[t];
+      aoff[i] = o;
+      aizb[i] = izb[i];
+      atis[i] = t;
+      o += ((h.NX * h.NY + bs - 1) / bs) * ((izb[i] + zb - 1) / zb);
+      for (int k = 0; k < nb[i]; k++, b++) {
+        cm[4 * b] = t;
+        cm[4 * b + 1] = k;
+        cm[4 * b + 2] = nb[i];
+        cm[4 * b + 3] = i;
+      }
+    }
+    aoff[ntis] = o;
+    HIPCHK(c, hipMemcpyAsync(c->d_cal, c->h_cal, sizeof(int) * (hdr + 4 * ncb), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_cal, c->stream));
+    c->cal_key = key;
+    c->cal = CaList{c->d_cal, c->d_cal + ntis + 1, c->d_cal + 2 * ntis + 1,
+                    reinterpret_cast<const int4*>(c->d_cal + hdr), ntis, (int)ncb, (int)tiles, bs};
+  }
+  *out = c->cal;
+  return EPI_OK;
+}
+
 // calc_ca2p (ca2p.cpp:380-402) for the tissues in tlist (nullptr = all)
-// (with d_tlist, c->h_tlist holds the same ntis ids).  Launch grids are
-// sized by the solving tissues only: a gated solve of one small tissue in a
-// large batch launches that tissue's blocks, not the batch's largest.
+// (with d_tlist, c->h_tlist holds the same ntis ids).  The list covers the
+// solving tissues only: a gated solve of one small tissue in a large batch
+// launches that tissue's blocks, not the batch's largest.
 int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
-  int mn = 1, mcell = 1, mx = 1, my = 1, izb = 1;
+  int mn = 1;
   long long box = 1;
   for (int i = 0; i < ntis; i++) {
     const int t = d_tlist ? c->h_tlist[i] : i;
@@ -437,10 +524,6 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
     const double zz = c->h_st[t].zzmax;
     const int iz = std::min((int)((zz + FAC_MAP * R_max) / c->tis[t].dz) + 2, h.NZ + 1);
     mn = std::max(mn, h.n_est + 64);
-    mcell = std::max(mcell, h.n_est + 64 - h.nmemb - h.nder);   // cells the sub-step kernels visit
-    mx = std::max(mx, h.NX);
-    my = std::max(my, h.NY);
-    izb = std::max(izb, iz);
     box = std::max(box, (long long)h.NX * h.NY * iz);
   }
   // ca_fuse 2: fuse the solves of at most FUSE_AUTO_CELLS cells (a gated solve
@@ -457,8 +540,11 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
   } else {
     tu.ca_fuse = tu.ca_fuse == 1;
   }
+  CaList cl;
+  RCHK(ca_list(c, d_tlist ? c->h_tlist : nullptr, ntis, tu.ca_group, ca_list_bs(tu, tu.ca_fuse && n_sub >= 2),
+               tu.a_zb, &cl));
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
-  launch_ca_substeps(c->stream, c->d_tis, d_tlist, ntis, c->cb, c->cp, c->ap, c->bp, n_sub, mcell, mx, my, izb, tu);
+  launch_ca_substeps(c->stream, c->d_tis, cl, c->cb, c->cp, c->ap, c->bp, n_sub, tu);
   launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
   HIPCHK(c, hipGetLastError());
   return EPI_OK;
@@ -501,6 +587,7 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
   c->th.resize(ntissues);
   c->tis.resize(ntissues);
   c->ca_solves.assign(ntissues, 0);
+  c->dev_ts.assign(ntissues, 0);
   c->steps.assign(ntissues, 0);
   c->cell_steps.assign(ntissues, 0);
   c->grid_b.assign(ntissues, 0);
@@ -552,6 +639,13 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
     ti.m_kspring = Kspring_ref;
     ti.m_kbend = KBEND_ref;
   }
+  // the Ca list: 3 ints per tissue + 1, padding, and one int4 per cell block
+  // (8 cells per block at the most lanes per cell)
+  {
+    long long blocks = 0;
+    for (int t = 0; t < ntissues; t++) blocks += (c->th[t].cap + 64 + 7) / 8;
+    c->cal_ints = 3LL * ntissues + 8 + 4 * blocks;
+  }
   // arena
   Carver dry{nullptr, 0, true};
   for (int t = 0; t < ntissues; t++) carve_tissue(dry, c->th[t], c->tis[t]);
@@ -560,7 +654,8 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
   if (hipStreamCreate(&c->side_stream) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_st, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_st, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_cal, hipEventDisableTiming) != hipSuccess) {
     epi_destroy(c);
     return EPI_ERR_HIP;
   }
@@ -568,6 +663,9 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
       hipMalloc(&c->d_tis, sizeof(Tis) * ntissues) != hipSuccess ||
       hipMalloc(&c->d_st, sizeof(Status) * ntissues) != hipSuccess ||
       hipMalloc(&c->d_tlist, sizeof(int) * ntissues) != hipSuccess ||
+      hipMalloc(&c->d_wl, sizeof(int) * WL_INTS(ntissues)) != hipSuccess ||
+      hipMalloc(&c->d_cal, sizeof(int) * c->cal_ints) != hipSuccess ||
+      hipHostMalloc(&c->h_cal, sizeof(int) * c->cal_ints, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->h_st, sizeof(Status) * ntissues, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->h_zz, sizeof(double) * ntissues, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->h_tlist, sizeof(int) * ntissues, hipHostMallocDefault) != hipSuccess) {
@@ -619,6 +717,10 @@ void epi_destroy(epi_ctx* c) {
   if (c->d_tis) (void)hipFree(c->d_tis);
   if (c->d_st) (void)hipFree(c->d_st);
   if (c->d_tlist) (void)hipFree(c->d_tlist);
+  if (c->d_wl) (void)hipFree(c->d_wl);
+  if (c->d_cal) (void)hipFree(c->d_cal);
+  if (c->h_cal) (void)hipHostFree(c->h_cal);
+  if (c->ev_cal) (void)hipEventDestroy(c->ev_cal);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->h_zz) (void)hipHostFree(c->h_zz);
   if (c->h_tlist) (void)hipHostFree(c->h_tlist);
@@ -706,12 +808,15 @@ int epi_load_cells(epi_ctx* c, int32_t t, const epi_cell_input* in) {
   RCHK(upload(c, ti.in_fat, in->in_fat, n));
   RCHK(upload(c, ti.spr_nat_len, in->spr_nat_len, n));
   RCHK(upload(c, ti.memb6, m6.data(), m6.size()));
+  const std::vector<long long> unset(3 * (size_t)n, -1);   // cell_stat fields never set
+  RCHK(upload(c, ti.cst, unset.data(), unset.size()));
   Status s{};
   s.n = n;
   s.next_uid = n;
   s.stamp_z = h.NZ;   // the first map build clears every plane
   c->h_st[t] = s;
   RCHK(put_status(c, t));
+  c->dev_ts[t] = 0;
   h.n_est = n;
   h.loaded = true;
   return EPI_OK;
@@ -826,6 +931,7 @@ int epi_cell_state_renew(epi_ctx* c) {
   if (c->pos_pending) return fail(c, EPI_ERR_INVALID, "pos_copy must follow cell_interaction");
   launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, max_n(c));
   HIPCHK(c, hipGetLastError());
+  for (auto& v : c->dev_ts) v++;   // k_remove advances each tissue's timestep
   return EPI_OK;
 }
 
@@ -871,7 +977,7 @@ int epi_setup_map_lat(epi_ctx* c) {
 int epi_calc_ext_stim(epi_ctx* c, const double* zzmax) {
   if (!c) return EPI_ERR_INVALID;
   RCHK(set_zz(c, zzmax));
-  launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb_est(c, false), c->tu);
+  launch_ext_stim(c->stream, c->d_tis, c->T, c->bp, b_tiles_est(c), c->d_wl, c->tu);
   HIPCHK(c, hipGetLastError());
   c->bp ^= 1;
   return EPI_OK;
@@ -917,6 +1023,35 @@ int epi_ca_calc_condition_reset(epi_ctx* c) {
   return EPI_OK;
 }
 
+int epi_set_timestep(epi_ctx* c, int32_t t, int64_t step) {
+  RCHK(check_tissue(c, t));
+  RCHK(sync_status(c));
+  c->h_st[t].timestep = step;
+  HIPCHK(c, hipMemcpyAsync(&c->d_st[t].timestep, &c->h_st[t].timestep, sizeof(long long), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dev_ts[t] = step;
+  return EPI_OK;
+}
+
+int epi_take_cell_stats(epi_ctx* c, int32_t t, int64_t* records, int32_t max_records, int32_t* n_records) {
+  RCHK(check_tissue(c, t));
+  if (!n_records || max_records < 0 || (max_records > 0 && !records)) return EPI_ERR_INVALID;
+  RCHK(sync_status(c));
+  const int k = c->h_st[t].nstat;
+  *n_records = k;
+  if (k > max_records) return fail(c, EPI_ERR_CAPACITY, "more cell_stat records than max_records");
+  if (k) {
+    std::vector<long long> r(3 * (size_t)k);
+    RCHK(download(c, r.data(), c->tis[t].cst_store, r.size()));
+    for (size_t i = 0; i < r.size(); i++) records[i] = r[i];
+  }
+  c->h_st[t].nstat = 0;
+  HIPCHK(c, hipMemcpyAsync(&c->d_st[t].nstat, &c->h_st[t].nstat, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EPI_OK;
+}
+
 int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
   if (!c) return EPI_ERR_INVALID;
   // a zzmax whose calc_ca2p bound is iz_bound
@@ -928,9 +1063,10 @@ int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
   const int mn = max_n(c);
   long long box = 1;
   for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * (iz_bound + 1));
+  CaList cl;
+  RCHK(ca_list(c, nullptr, c->T, c->tu.ca_group, ca_list_bs(c->tu, false), c->tu.a_zb, &cl));
   launch_ca_touch(c->stream, c->d_tis, nullptr, c->T, c->cb, mn, box);
-  launch_ca_substep(c->stream, c->d_tis, nullptr, c->T, c->cb, c->cp, c->ap, c->bp, mn, max_nx(c), max_ny(c),
-                    iz_bound + 1, c->tu);
+  launch_ca_substep(c->stream, c->d_tis, cl, c->cb, c->cp, c->ap, c->bp, c->tu);
   c->cp ^= 1;
   c->ap ^= 1;
   launch_ca_diffu_zero(c->stream, c->d_tis, nullptr, c->T, mn);
@@ -954,7 +1090,7 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb) {
   chk(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
   launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
   launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
-  launch_ext_stim(c->side_stream, c->d_tis, c->T, c->bp, max_nx(c), max_ny(c), izb, c->tu);
+  launch_ext_stim(c->side_stream, c->d_tis, c->T, c->bp, b_tiles_est(c), c->d_wl, c->tu);
   chk(hipEventRecord(c->ev_join, c->side_stream));
   chk(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   chk(hipGetLastError());
@@ -1017,6 +1153,15 @@ int run_build(epi_ctx* c, int mn, int nb, int izb) {
 
 int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
   if (!c || !drv) return EPI_ERR_INVALID;
+  // current_timestep (proc.cpp:132) of each tissue's renewals: the driver's
+  // step; the device advances it once per step from here
+  for (int t = 0; t < c->T; t++)
+    if (c->dev_ts[t] != drv[t].step) {
+      c->h_st[t].timestep = drv[t].step;
+      HIPCHK(c, hipMemcpyAsync(&c->d_st[t].timestep, &c->h_st[t].timestep, sizeof(long long), hipMemcpyHostToDevice,
+                               c->stream));
+      c->dev_ts[t] = drv[t].step;
+    }
   // The next step's cell_interaction reads positions, radii, ages, states,
   // pairs, dermis and the lists; a calc_ca2p solve writes none of them (and
   // reads no position). So, unless initialize_sc (which changes states)
@@ -1097,6 +1242,7 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
         c->grid_ca[t] += (long long)CA_ITR * ti.NX * ti.NY * (int)((c->h_st[t].zzmax + FAC_MAP * R_max) / ti.dz);
       }
       d.step++;
+      c->dev_ts[t]++;
     }
     nca = (int)ca_list.size();
     if (nca) {
@@ -1191,6 +1337,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_MAP_LANES: return &tu.map_lanes;
     case EPI_TUNE_CA_GROUP_SMALL: return &tu.ca_group_small;
     case EPI_TUNE_STEP_GRAPH: return &tu.step_graph;
+    case EPI_TUNE_B_XCD: return &tu.b_xcd;
     default: return nullptr;
   }
 }
@@ -1211,6 +1358,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4; break;
     case EPI_TUNE_CA_GROUP_SMALL: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_STEP_GRAPH: ok = value == 0 || value == 1; break;
+    case EPI_TUNE_B_XCD: ok = value == 0 || value == 1; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;
@@ -1266,6 +1414,11 @@ int epi_get_cells(epi_ctx* c, int32_t t, epi_cells* o) {
     for (int i = 0; i < n; i++) o->uid[i] = u[i];
   }
   if (o->memb_adj) RCHK(download(c, o->memb_adj, ti.memb6, 6 * (size_t)h.nmemb));
+  if (o->cst) {
+    std::vector<long long> v(3 * (size_t)n);
+    RCHK(download(c, v.data(), ti.cst, v.size()));
+    for (size_t i = 0; i < v.size(); i++) o->cst[i] = v[i];
+  }
   if (o->conn_ptr || o->conn || o->gj) {
     std::vector<int> nl(n);
     RCHK(download(c, nl.data(), ti.nlen[c->cb], n));
@@ -1448,10 +1601,9 @@ int rd_params(epi_ctx* c, const epi_rd_params* prm) {
 // one window sub-step; with sv the state before it is saved into segment
 // slot `slot` by the cell kernel (cells_only: only that, no ATP update)
 int rd_substep(epi_ctx* c, const RdTis* sv = nullptr, int slot = 0, bool cells_only = false) {
-  int izb = 1;
-  for (int v : c->rd.izb) izb = std::max(izb, v + 1);
-  launch_ca_substep(c->stream, c->d_tis, nullptr, c->T, c->cb, c->cp, c->ap, c->bp, max_n(c), max_nx(c), max_ny(c),
-                    izb, c->tu, sv, slot, cells_only);
+  CaList cl;
+  RCHK(ca_list(c, nullptr, c->T, c->tu.ca_group, ca_list_bs(c->tu, false), c->tu.a_zb, &cl));
+  launch_ca_substep(c->stream, c->d_tis, cl, c->cb, c->cp, c->ap, c->bp, c->tu, sv, slot, cells_only);
   c->cp ^= 1;
   c->ap ^= 1;
   return EPI_OK;

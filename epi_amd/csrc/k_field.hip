@@ -133,37 +133,146 @@ __device__ __forceinline__ double fB(double age, double B, bool cornif) {   // :
   return (cornif && age > THRESH_DEAD - DUR_ALIVE && age <= THRESH_DEAD + DUR_DEAD ? 1 : 0) - kb * B;
 }
 
-// threads: x fastest (64 per wave), blockDim.x / 64 y rows per block;
-// blockIdx.z packs (y block, z chunk); z chunks are grid-strided up to
-// iz_bound
-template <int ZB>
-__global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, int bp, int nky, int nzc) {
-  const Tis& t = T[blockIdx.y];
-  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
-  const int rows = blockDim.x >> 6;
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int k = (blockIdx.z % nky) * rows + (threadIdx.x >> 6);
-  const int zc = blockIdx.z / nky;
-  const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
-  const int iz_bound = (int)((zzmax + FAC_MAP * R_max) * t.inv_dz);
-  if (iz_bound > NZ || iz_bound < 0) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.z == 0) set_err(t.st, ERR_OUT_OF_GRID);
-    return;
+// ---- flattened work lists ------------------------------------------------
+// A launch over tissues of different sizes numbers the tiles of all of them
+// consecutively, tissue by tissue: off[i] is the first tile of the launch's
+// i-th tissue and off[nt] the total.  A tile is a chunk of `bs` consecutive
+// points of the tissue's (x, y) plane (x fastest, so a wave's lanes read
+// consecutive x of one or two rows) times a chunk of ZB z planes, plane
+// chunks fastest.  No lane covers a point outside its tissue except in the
+// last plane chunk, and no block is launched for a tissue that has no work.
+
+// exclusive prefix sum of v over a block of blockDim.x (a multiple of 64,
+// at most 1024) threads; returns the block total in *total
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int s_w[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  if (j >= NX || k >= NY) return;
-  const double* __restrict__ carr = t.bf[bp];
-  double* __restrict__ narr = t.bf[1 - bp];
-  const uint8_t* __restrict__ cm2 = t.cmap2;
-  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
-  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  if (lane == 63) s_w[wv] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int i = 0; i < nw; i++) {
+    const int w = s_w[i];
+    if (i < wv) before += w;
+    all += w;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+// The B stencil's list, once zzmax is on the device: per tissue
+// iz_bound = (int)((zzmax + 2.8) * inv_dz) (ext_stim.cpp:22) into izb[i], and
+// ceil(NX NY / bs) ceil(iz_bound / zb) tiles.  A tissue whose iz_bound lies
+// outside [0, NZ] gets none and reports ERR_OUT_OF_GRID.  One block.
+__global__ void __launch_bounds__(256) k_wl_b(const Tis* __restrict__ T, int ntis, int bs, int zb,
+                                              int* __restrict__ off, int* __restrict__ izb_out) {
+  int carry = 0;
+  for (int base = 0; base < ntis; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    int cnt = 0;
+    if (i < ntis) {
+      const Tis& t = T[i];
+      const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
+      int izb = (int)((zzmax + FAC_MAP * R_max) * t.inv_dz);
+      if (izb > t.NZ || izb < 0) {
+        set_err(t.st, ERR_OUT_OF_GRID);
+        izb = 0;
+      }
+      izb_out[i] = izb;
+      cnt = ((t.NX * t.NY + bs - 1) / bs) * ((izb + zb - 1) / zb);
+    }
+    int tot;
+    const int ex = block_exclusive_scan(cnt, &tot);
+    if (i < ntis) off[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) off[ntis] = carry;
+}
+
+// Tiles of a work list for this block: a grid-stride loop.  With `xcd` (and
+// a grid that is a multiple of 8) the list is cut into 8 contiguous runs,
+// one per XCD (workgroups go to the XCDs round-robin by id), so that tiles
+// that share halo rows share an L2.
+struct TileWalk {
+  int w, end, stride;
+  // block b of G walking the list (b = blockIdx.x, G = gridDim.x unless the
+  // launch shares its grid with other work)
+  __device__ TileWalk(int total, int xcd, int b, int G) {
+    if (xcd && G >= 8 && (G & 7) == 0) {
+      const int seg = (total + 7) / 8, x = b & 7;
+      w = x * seg + (b >> 3);
+      end = min(total, (x + 1) * seg);
+      stride = G >> 3;
+    } else {
+      w = b;
+      end = total;
+      stride = G;
+    }
+  }
+};
+
+// the launch's tissue of tile w: the last i in [lo, nt) with off[i] <= w
+template <class F>
+__device__ __forceinline__ int wl_find(F off, int lo, int nt, int w) {
+  int hi = nt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off(mid) <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+constexpr int WL_LDS_T = 1024;   // tissues whose list a block stages in LDS
+
+// ---- ext_stim B: calc_ext_stim (ext_stim.cpp:18-69) ------------------------
+// One thread per (x, y) point of a tile; the z column rolls through
+// registers, ZB planes per tile.
+template <int ZB>
+__global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, const int* __restrict__ woff,
+                                                  const int* __restrict__ wizb, int ntis, int bp, int xcd) {
+  __shared__ int s_off[WL_LDS_T + 1];
+  __shared__ int s_izb[WL_LDS_T];
+  const bool lds = ntis <= WL_LDS_T;
+  if (lds) {
+    for (int i = threadIdx.x; i <= ntis; i += blockDim.x) {
+      s_off[i] = woff[i];
+      if (i < ntis) s_izb[i] = wizb[i];
+    }
+    __syncthreads();
+  }
+  auto off = [&](int i) { return lds ? s_off[i] : woff[i]; };
+  const int bs = blockDim.x;
   const double DB = 0.0009;
-  const double inv_dz = t.inv_dz;
-  const long long cxy = gidx(t, j, k, 0);
-  for (int l0 = zc * ZB; l0 < iz_bound; l0 += nzc * ZB) {
+  int ti = 0;
+  for (TileWalk tw(off(ntis), xcd, blockIdx.x, gridDim.x); tw.w < tw.end; tw.w += tw.stride) {
+    ti = wl_find(off, ti, ntis, tw.w);
+    const Tis& t = T[ti];
+    const int iz_bound = lds ? s_izb[ti] : wizb[ti];
+    const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+    const int plane = NX * NY;
+    const int npt = (plane + bs - 1) / bs;
+    const int local = tw.w - off(ti);
+    const int p = (local % npt) * bs + (int)threadIdx.x;
+    if (p >= plane) continue;
+    const int j = p % NX, k = p / NX;
+    const int l0 = (local / npt) * ZB;
     const int l1 = min(l0 + ZB, iz_bound);
+    const double* __restrict__ carr = t.bf[bp];
+    double* __restrict__ narr = t.bf[1 - bp];
+    const uint8_t* __restrict__ cm2 = t.cmap2;
+    const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+    const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+    const double inv_dz = t.inv_dz;
+    const long long cxy = gidx(t, j, k, 0);
     // the z column of (j, k) rolls through registers; mirror z boundary
     // prev_z(0) = 1, next_z(NZ - 1) = NZ - 1 (utils.cpp:32-72)
-    int pz = l0 == 0 ? 1 : l0 - 1;
+    const int pz = l0 == 0 ? 1 : l0 - 1;
     double czm = carr[cxy + pz * t.PL];
     double mzm = (double)cm2[cxy + pz * t.PL];
     double cc = carr[cxy + l0 * t.PL];
@@ -598,28 +707,28 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
 }
 
 template <int CG>
-__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
+__global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, CaList cl, int cb, int cp,
                                                       int ap, int bp, int xcd, const RdTis* __restrict__ sv,
                                                       int slot) {
-  const unsigned lid = xcd_order(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, xcd);
-  const unsigned bx = lid % gridDim.x, by = lid / gridDim.x;
-  const int ti = tlist ? tlist[by] : by;
-  const Tis& t = T[ti];
-  ca_cells_part<CG, C_BLOCK>(t, bx, gridDim.x, cb, cp, ap, bp, nullptr, nullptr, sv ? sv + ti : nullptr, slot);
+  const int4 e = cl.cmap[xcd_order(blockIdx.x, gridDim.x, xcd)];
+  const Tis& t = T[e.x];
+  ca_cells_part<CG, C_BLOCK>(t, e.y, e.z, cb, cp, ap, bp, nullptr, nullptr, sv ? sv + e.x : nullptr, slot);
 }
 
-// ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound.
-// Threads cover [0, NX) x [0, NY); the z column rolls through registers.
+// ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound,
+// one tile of a solve's list: blockDim.x consecutive points of the (x, y)
+// plane times ZB planes; the z column rolls through registers.
 template <int ZB>
-__device__ __forceinline__ void atp_part(const Tis& t, unsigned bx, unsigned bz, int nky, int nzc, int ap, int cp) {
+__device__ __forceinline__ void atp_tile(const Tis& t, int local, int izb, int ap, int cp) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
-  const int rows = blockDim.x >> 6;
-  const int j = bx * 64 + (threadIdx.x & 63);
-  const int k = (bz % nky) * rows + (threadIdx.x >> 6);
-  const int zc = bz / nky;
-  const int izb = ca_iz_bound(t);
-  if (izb > NZ || izb < 0) return;
-  if (j >= NX || k >= NY) return;
+  const int bs = blockDim.x;
+  const int plane = NX * NY;
+  const int npt = (plane + bs - 1) / bs;
+  const int p = (local % npt) * bs + (int)threadIdx.x;
+  if (p >= plane) return;
+  const int j = p % NX, k = p / NX;
+  const int l0 = (local / npt) * ZB;
+  const int l1 = min(l0 + ZB, izb);
   const double* __restrict__ carr = t.atp[ap];
   double* __restrict__ narr = t.atp[1 - ap];
   const double* __restrict__ dfu = dfu_buf(t, cp);
@@ -628,68 +737,88 @@ __device__ __forceinline__ void atp_part(const Tis& t, unsigned bx, unsigned bz,
   const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
   const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
   const long long cxy = gidx(t, j, k, 0);
-  for (int l0 = zc * ZB; l0 < izb; l0 += nzc * ZB) {
-    const int l1 = min(l0 + ZB, izb);
-    const int pz = l0 == 0 ? 1 : l0 - 1;   // mirror z boundary (utils.cpp:32-72)
-    double czm = carr[cxy + pz * t.PL];
-    double mzm = (double)cm2[cxy + pz * t.PL];
-    double cc = carr[cxy + l0 * t.PL];
-    double mc = (double)cm2[cxy + l0 * t.PL];
-    for (int l = l0; l < l1; l++) {
-      const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
-      const long long v = cxy + l * t.PL;
-      const long long vzp = cxy + next_z * t.PL;
-      const double czp = carr[vzp];
-      const double mzp = (double)cm2[vzp];
-      const unsigned w = t.vox[v];
-      const unsigned cell1 = w & 0x7fffffffu;
-      const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
-      const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
-      const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
-      const double nv = atp_new(cc, carr[vxm], carr[vxp], carr[vym], carr[vyp], czm, czp, (double)cm2[vxm],
-                                (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], mzm, mzp, diffu,
-                                (double)(w >> 31), inv_dx);
-      narr[v] = nv;
-      if (k == 0) narr[gidx(t, j, NY, l)] = nv;
-      if (j == 0) narr[gidx(t, NX, k, l)] = nv;
-      if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
-      czm = cc; mzm = mc;
-      cc = czp; mc = mzp;
-    }
+  const int pz = l0 == 0 ? 1 : l0 - 1;   // mirror z boundary (utils.cpp:32-72)
+  double czm = carr[cxy + pz * t.PL];
+  double mzm = (double)cm2[cxy + pz * t.PL];
+  double cc = carr[cxy + l0 * t.PL];
+  double mc = (double)cm2[cxy + l0 * t.PL];
+  for (int l = l0; l < l1; l++) {
+    const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+    const long long v = cxy + l * t.PL;
+    const long long vzp = cxy + next_z * t.PL;
+    const double czp = carr[vzp];
+    const double mzp = (double)cm2[vzp];
+    const unsigned w = t.vox[v];
+    const unsigned cell1 = w & 0x7fffffffu;
+    const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
+    const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
+    const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
+    const double nv = atp_new(cc, carr[vxm], carr[vxp], carr[vym], carr[vyp], czm, czp, (double)cm2[vxm],
+                              (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], mzm, mzp, diffu,
+                              (double)(w >> 31), inv_dx);
+    narr[v] = nv;
+    if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+    if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+    if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+    czm = cc; mzm = mc;
+    cc = czp; mc = mzp;
   }
   // plane l = iz_bound: copies of values left in the next buffer (:334-337);
   // the corner (NX, NY) ends as a copy of (0, 0)
-  if (zc == 0) {
+  if (l0 == 0) {
     if (k == 0) narr[gidx(t, j, NY, izb)] = narr[gidx(t, j, 0, izb)];
     if (j == 0) narr[gidx(t, NX, k, izb)] = narr[gidx(t, 0, k, izb)];
     if (j == 0 && k == 0) narr[gidx(t, NX, NY, izb)] = narr[gidx(t, 0, 0, izb)];
   }
 }
 
+// the ATP tiles of a solve's list, walked grid-stride by `nb` blocks of
+// which this is block `b` (XCD runs with xcd): the tile offsets, iz_bound
+// and tissue ids of up to CA_LDS_T tissues are staged in LDS (kept small: the
+// fused kernel's cell blocks hold LDS of their own)
+constexpr int CA_LDS_T = 256;
 template <int ZB>
-__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, const int* tlist, int ap, int cp, int nky,
-                                             int nzc, int xcd) {
-  const unsigned gxy = gridDim.x * gridDim.y;
-  const unsigned lid = xcd_order((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, gxy * gridDim.z, xcd);
-  const unsigned bx = lid % gridDim.x, by = (lid / gridDim.x) % gridDim.y, bz = lid / gxy;
-  const Tis& t = T[tlist ? tlist[by] : by];
-  atp_part<ZB>(t, bx, bz, nky, nzc, ap, cp);
+__device__ __forceinline__ void atp_walk(const Tis* __restrict__ T, const CaList& cl, int b, int nb, int xcd,
+                                         int ap, int cp) {
+  __shared__ int s_off[CA_LDS_T + 1];
+  __shared__ int2 s_ti[CA_LDS_T];
+  const int nt = cl.ntis;
+  const bool lds = nt <= CA_LDS_T;
+  if (lds) {
+    for (int i = threadIdx.x; i <= nt; i += blockDim.x) {
+      s_off[i] = cl.aoff[i];
+      if (i < nt) s_ti[i] = make_int2(cl.atis[i], cl.aizb[i]);
+    }
+    __syncthreads();
+  }
+  auto off = [&](int i) { return lds ? s_off[i] : cl.aoff[i]; };
+  int ti = 0;
+  for (TileWalk tw(off(nt), xcd, b, nb); tw.w < tw.end; tw.w += tw.stride) {
+    ti = wl_find(off, ti, nt, tw.w);
+    const int2 q = lds ? s_ti[ti] : make_int2(cl.atis[ti], cl.aizb[ti]);
+    atp_tile<ZB>(T[q.x], tw.w - off(ti), q.y, ap, cp);
+  }
+}
+
+template <int ZB>
+__global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, CaList cl, int ap, int cp, int xcd) {
+  atp_walk<ZB>(T, cl, blockIdx.x, gridDim.x, xcd, ap, cp);
 }
 
 // one launch for the ATP update of sub-step i (parities cp, ap) and the cells
-// of sub-step i + 1: blocks [0, ncb) take the cells, the rest the grid
-// (gx x-blocks per z-block, 2 rows of 64 points per block)
+// of sub-step i + 1: blocks [0, ncb) take the cells (one entry of the list's
+// cell-block map each), the rest walk the ATP tiles
 constexpr int F_BLOCK = 128;
 
 template <int CG, int ZB>
-__global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T, const int* tlist, int cb, int cp,
-                                                      int ap, int bp, int ncb, int gx, int nky, int nzc) {
-  const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
+__global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T, CaList cl, int cb, int cp,
+                                                      int ap, int bp, int ncb) {
   if ((int)blockIdx.x < ncb) {
-    ca_cells_part<CG, F_BLOCK>(t, blockIdx.x, ncb, cb, 1 - cp, 1 - ap, bp, t.atp[ap], dfu_buf(t, cp));
+    const int4 e = cl.cmap[blockIdx.x];
+    const Tis& t = T[e.x];
+    ca_cells_part<CG, F_BLOCK>(t, e.y, e.z, cb, 1 - cp, 1 - ap, bp, t.atp[ap], dfu_buf(t, cp));
   } else {
-    const unsigned b = blockIdx.x - ncb;
-    atp_part<ZB>(t, b % gx, b / gx, nky, nzc, ap, cp);
+    atp_walk<ZB>(T, cl, blockIdx.x - ncb, gridDim.x - ncb, 0, ap, cp);
   }
 }
 
@@ -738,15 +867,20 @@ void by_zb(int zb, F f) {
 }
 }  // namespace
 
-void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est,
+void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, long long tiles_est, int* d_wl,
                      const Tuning& tu) {
-  const int rows = tu.b_rows;
-  const int nky = (max_ny + rows - 1) / rows;
+  const int bs = 64 * tu.b_rows;
+  // a grid-stride walk over the list: the grid is sized from the host's
+  // estimate (the last known zzmax), capped at 8 blocks of 256 lanes per CU,
+  // and a multiple of 8 so that the XCD split applies
+  const long long cap = 2048LL * 256 / bs;
+  long long g = std::min(std::max(tiles_est, 1LL), cap);
+  g = (g + 7) / 8 * 8;
   by_zb(tu.b_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
-    const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
-    dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
-    EPI_LAUNCH(K_EXT_STIM, k_ext_stim<zb>, g, dim3(64 * rows), 0, s, dT, bp, nky, nzc);
+    EPI_LAUNCH(K_WL, k_wl_b, dim3(1), dim3(256), 0, s, dT, ntis, bs, zb, d_wl, d_wl + ntis + 1);
+    EPI_LAUNCH(K_EXT_STIM, k_ext_stim<zb>, dim3((unsigned)g), dim3(bs), 0, s, dT, d_wl, d_wl + ntis + 1, ntis, bp,
+               tu.b_xcd);
   });
 }
 
@@ -773,76 +907,76 @@ void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int nt
 }
 
 namespace {
-void launch_cells(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp, int max_n,
-                  const Tuning& tu, const RdTis* sv = nullptr, int slot = 0) {
+void launch_cells(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int cp, int ap, int bp, const Tuning& tu,
+                  const RdTis* sv = nullptr, int slot = 0) {
   const int cg = tu.ca_group;
-  const dim3 gc((max_n + C_BLOCK / cg - 1) / (C_BLOCK / cg), ntis);
+  const dim3 gc((unsigned)std::max(cl.ncb, 1));
   if (cg == 4)
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd, sv, slot);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<4>, gc, dim3(C_BLOCK), 0, s, dT, cl, cb, cp, ap, bp, tu.xcd, sv, slot);
   else if (cg == 8)
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd, sv, slot);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<8>, gc, dim3(C_BLOCK), 0, s, dT, cl, cb, cp, ap, bp, tu.xcd, sv, slot);
   else
-    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, tu.xcd, sv, slot);
+    EPI_LAUNCH(K_CA_CELLS, k_ca_cells<16>, gc, dim3(C_BLOCK), 0, s, dT, cl, cb, cp, ap, bp, tu.xcd, sv, slot);
 }
 
-void launch_atp(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cp, int ap, int max_nx, int max_ny,
-                int izb_est, const Tuning& tu) {
-  const int rows = tu.a_rows;
-  const int nky = (max_ny + rows - 1) / rows;
+// blocks that walk a list of `tiles` tiles: at most 8 blocks of 256 lanes
+// per CU, a multiple of 8 (the XCD split)
+unsigned walk_blocks(long long tiles, int bs) {
+  const long long cap = 2048LL * 256 / bs;
+  long long g = std::min(std::max(tiles, 1LL), cap);
+  return (unsigned)((g + 7) / 8 * 8);
+}
+
+void launch_atp(hipStream_t s, const Tis* dT, const CaList& cl, int cp, int ap, const Tuning& tu) {
   by_zb(tu.a_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
-    const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
-    dim3 g((max_nx + 63) / 64, ntis, nky * nzc);
-    EPI_LAUNCH(K_ATP, k_atp<zb>, g, dim3(64 * rows), 0, s, dT, tlist, ap, cp, nky, nzc, tu.xcd);
+    EPI_LAUNCH(K_ATP, k_atp<zb>, dim3(walk_blocks(cl.atiles, cl.abs)), dim3(cl.abs), 0, s, dT, cl, ap, cp, tu.xcd);
   });
 }
 
 template <int CG>
-void launch_fused_cg(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                     int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
-  const int ncb = (max_n + F_BLOCK / CG - 1) / (F_BLOCK / CG);
-  constexpr int F_ROWS = F_BLOCK / 64;
-  const int gx = (max_nx + 63) / 64, nky = (max_ny + F_ROWS - 1) / F_ROWS;
+void launch_fused_cg(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int cp, int ap, int bp,
+                     const Tuning& tu) {
   by_zb(tu.a_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
-    const int nzc = izb_est > 0 ? (izb_est + zb - 1) / zb : 1;
-    const dim3 g((unsigned)(ncb + gx * nky * nzc), ntis);
+    const dim3 g((unsigned)cl.ncb + walk_blocks(cl.atiles, F_BLOCK));
     const auto kfn = k_ca_fused<CG, zb>;
-    EPI_LAUNCH(K_CA_FUSED, kfn, g, dim3(F_BLOCK), 0, s, dT, tlist, cb, cp, ap, bp, ncb, gx, nky, nzc);
+    EPI_LAUNCH(K_CA_FUSED, kfn, g, dim3(F_BLOCK), 0, s, dT, cl, cb, cp, ap, bp, cl.ncb);
   });
 }
 }  // namespace
 
-void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu, const RdTis* sv, int slot,
-                       bool cells_only) {
-  launch_cells(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, tu, sv, slot);
-  if (!cells_only) launch_atp(s, dT, tlist, ntis, cp, ap, max_nx, max_ny, izb_est, tu);
+int ca_list_bs(const Tuning& tu, bool fused) { return fused ? F_BLOCK : 64 * tu.a_rows; }
+
+void launch_ca_substep(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int cp, int ap, int bp,
+                       const Tuning& tu, const RdTis* sv, int slot, bool cells_only) {
+  launch_cells(s, dT, cl, cb, cp, ap, bp, tu, sv, slot);
+  if (!cells_only) launch_atp(s, dT, cl, cp, ap, tu);
 }
 
-void launch_ca_substeps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int& cp, int& ap, int bp,
-                        int n_sub, int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu) {
+void launch_ca_substeps(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int& cp, int& ap, int bp,
+                        int n_sub, const Tuning& tu) {
   if (!tu.ca_fuse || n_sub < 2) {
     for (int i = 0; i < n_sub; i++) {
-      launch_ca_substep(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+      launch_ca_substep(s, dT, cl, cb, cp, ap, bp, tu);
       cp ^= 1;
       ap ^= 1;
     }
     return;
   }
   // cells of sub-step 0; then (ATP of i, cells of i + 1); then ATP of the last
-  launch_cells(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, tu);
+  launch_cells(s, dT, cl, cb, cp, ap, bp, tu);
   for (int i = 0; i + 1 < n_sub; i++) {
     if (tu.ca_group == 4)
-      launch_fused_cg<4>(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+      launch_fused_cg<4>(s, dT, cl, cb, cp, ap, bp, tu);
     else if (tu.ca_group == 8)
-      launch_fused_cg<8>(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+      launch_fused_cg<8>(s, dT, cl, cb, cp, ap, bp, tu);
     else
-      launch_fused_cg<16>(s, dT, tlist, ntis, cb, cp, ap, bp, max_n, max_nx, max_ny, izb_est, tu);
+      launch_fused_cg<16>(s, dT, cl, cb, cp, ap, bp, tu);
     cp ^= 1;
     ap ^= 1;
   }
-  launch_atp(s, dT, tlist, ntis, cp, ap, max_nx, max_ny, izb_est, tu);
+  launch_atp(s, dT, cl, cp, ap, tu);
   cp ^= 1;
   ap ^= 1;
 }

@@ -256,6 +256,7 @@ __device__ void divide(const Tis& t, int pb, int cpar, int c, int k, int n_old, 
   t.dermis[d] = -1;
   t.diff_u[d] = 0;
   t.uid[d] = next_uid + k;
+  t.cst[3 * d] = -1; t.cst[3 * d + 1] = -1; t.cst[3 * d + 2] = -1;
   t.src_row[d] = -1;
   t.touch_air[d] = 0;
   t.lat[3 * d] = 0; t.lat[3 * d + 1] = 0; t.lat[3 * d + 2] = 0;
@@ -277,6 +278,7 @@ __global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, 
   Status* st = t.st;
   const int n_old = st->n_old;
   const long long next_uid = st->next_uid;
+  const long long ts = st->timestep;
   int ncorn = 0;
   for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n_old;
        c += gridDim.x * blockDim.x) {
@@ -285,6 +287,7 @@ __global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, 
         const double agek = t.agek[c];
         if (agek >= THRESH_DEAD) {
           t.state[c] = DEAD;   // cornificate (cellmanager.cpp:48-53)
+          t.cst[3 * c + 1] = ts;
           ncorn++;
         } else {
           const double cav = t.ca2p_avg[c];
@@ -308,7 +311,11 @@ __global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, 
         break;
       }
       case MUSUME: {   // _MUSUME_state_renew :282-302 (SYSTEM == WHOLE)
-        if (t.dermis[c] < 0 && t.pair[c] < 0) { t.state[c] = ALIVE; break; }
+        if (t.dermis[c] < 0 && t.pair[c] < 0) {
+          t.state[c] = ALIVE;
+          t.cst[3 * c] = ts;   // :287
+          break;
+        }
         const bool rd = needs_draw(t, c);
         const int k = rd ? t.div_slot[c] : -1;
         if (k >= 0) divide(t, pb, cpar, c, k, n_old, next_uid);
@@ -325,12 +332,28 @@ __global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, 
 // CellManager::remove_exec (cellmanager.cpp:338-352): queued slot q_i takes
 // the cell at the current tail N-1-i.  A queued slot at or past the current
 // end means a queued cell had already been moved (the reference deletes a
-// live cell there): ERR_REMOVE_ALIAS.
+// live cell there): ERR_REMOVE_ALIAS.  A removed DEAD cell's cell_stat goes
+// to cst_store with k_disap = the current timestep (:343-346), in queue
+// order; the timestep then advances (one renewal per step).
 __global__ void __launch_bounds__(256) k_remove(const Tis* __restrict__ T, int pb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
   const int ndiv = st->ndiv, nrm = st->nrm;
   const int N = st->n_old + ndiv;
+  if (threadIdx.x == 0) {
+    const long long ts = st->timestep;
+    int ns = st->nstat;
+    for (int i = 0; i < nrm; i++) {
+      const int q = t.rm_list[i];
+      if (q >= N - i || t.state[q] != DEAD) continue;
+      if (ns >= t.cap) { set_err(st, ERR_CAPACITY); break; }
+      long long* o = t.cst_store + 3 * (size_t)ns++;
+      o[0] = t.cst[3 * q]; o[1] = t.cst[3 * q + 1]; o[2] = ts;
+    }
+    st->nstat = ns;
+    st->timestep = ts + 1;
+  }
+  __syncthreads();
   for (int i = threadIdx.x; i < nrm; i += blockDim.x) {
     const int q = t.rm_list[i];
     if (q >= N - i) { set_err(st, ERR_REMOVE_ALIAS); continue; }
@@ -340,7 +363,11 @@ __global__ void __launch_bounds__(256) k_remove(const Tis* __restrict__ T, int p
     t.state[q] = t.state[s]; t.is_malignant[q] = t.is_malignant[s]; t.touch_air[q] = t.touch_air[s];
     t.fix_origin[q] = t.fix_origin[s]; t.rest_div_times[q] = t.rest_div_times[s];
     t.pair[q] = t.pair[s]; t.dermis[q] = t.dermis[s];
-    for (int a = 0; a < 3; a++) { t.lat[3 * q + a] = t.lat[3 * s + a]; t.pos[pb][a][q] = t.pos[pb][a][s]; }
+    for (int a = 0; a < 3; a++) {
+      t.lat[3 * q + a] = t.lat[3 * s + a];
+      t.pos[pb][a][q] = t.pos[pb][a][s];
+      t.cst[3 * q + a] = t.cst[3 * s + a];
+    }
     t.radius[q] = t.radius[s]; t.div_age_thresh[q] = t.div_age_thresh[s];
     t.ca2p[0][q] = t.ca2p[0][s]; t.ca2p[1][q] = t.ca2p[1][s];
     t.ip3[0][q] = t.ip3[0][s]; t.ip3[1][q] = t.ip3[1][s];

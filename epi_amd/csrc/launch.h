@@ -14,15 +14,20 @@ enum KernelId {
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
   K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
   K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB,
-  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_COUNT
+  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_WL, K_COUNT
 };
+
+// ints of a context's work-list buffer: the offsets and per-tissue iz_bound
+// of the B stencil's list, the rest for the Ca solve's lists
+constexpr long long WL_INTS(int ntis) { return 8LL * (ntis + 1); }
 
 // Launch shapes that the autotuner may change (values checked by the host).
 struct Tuning {
-  int b_zb = 4, b_rows = 4;   // k_ext_stim: z planes per chunk, y rows per block
-  int a_zb = 4, a_rows = 4;   // k_atp: the same
+  int b_zb = 4, b_rows = 4;   // k_ext_stim: z planes per tile, 64-lane rows per block (points per tile)
+  int a_zb = 4, a_rows = 4;   // k_atp: the same (a fused solve's tiles are 128 points)
   int ca_group = 16;          // k_ca_cells: lanes per cell
   int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
+  int b_xcd = 0;              // k_ext_stim: its tile list in one contiguous run per XCD
   int rd_group = 8;           // k_rd_step: lanes per cell
   int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
                               // (0 never, 1 always, 2 for solves of at most 131072 cells)
@@ -73,21 +78,37 @@ void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v);
 void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo,
                  const Tuning& tu);
-void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, int max_nx, int max_ny, int izb_est,
+// the B stencil over a work list of (tissue, tile) built on the device from
+// zzmax (d_wl: 2 ntis + 1 ints); tiles_est sizes the grid (any value covers
+// the list: the blocks walk it grid-stride)
+void launch_ext_stim(hipStream_t s, const Tis* dT, int ntis, int bp, long long tiles_est, int* d_wl,
                      const Tuning& tu);
 void launch_ca_begin(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 struct RdTis;
+// A Ca solve's work, flattened over its tissues (built on the host once per
+// solve, device pointers): blocks of the cell kernels map to (tissue, block
+// of that tissue, blocks of that tissue) through cmap (ncb entries); the ATP
+// grid is a list of tiles of `abs` plane points by a_zb planes, aoff[i]
+// being the first tile of the list's i-th tissue (aoff[ntis] = atiles), with
+// its iz_bound aizb[i] and tissue id atis[i].
+struct CaList {
+  const int* aoff;
+  const int* aizb;
+  const int* atis;
+  const int4* cmap;
+  int ntis, ncb, atiles, abs;
+};
+// points per ATP tile of a solve: the fused kernel's block, or the ATP
+// kernel's 64 a_rows
+int ca_list_bs(const Tuning& tu, bool fused);
 // sv: a reaction-diffusion window's reverse-pass storage; the cell kernel
 // then also saves the state before the sub-step into its segment slot
 // `slot`; cells_only skips the ATP update (the last state of a segment)
-void launch_ca_substep(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int cp, int ap, int bp,
-                       int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu, const RdTis* sv = nullptr,
-                       int slot = 0, bool cells_only = false);
-// n_sub sub-steps (fused when tu.ca_fuse), flipping cp and ap after each.
-// The cell kernels take one cell per lane group: max_n (here and in
-// launch_ca_substep) must be at least the cells past the membrane and dermis.
-void launch_ca_substeps(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int& cp, int& ap, int bp,
-                        int n_sub, int max_n, int max_nx, int max_ny, int izb_est, const Tuning& tu);
+void launch_ca_substep(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int cp, int ap, int bp,
+                       const Tuning& tu, const RdTis* sv = nullptr, int slot = 0, bool cells_only = false);
+// n_sub sub-steps (fused when tu.ca_fuse), flipping cp and ap after each
+void launch_ca_substeps(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int& cp, int& ap, int bp,
+                        int n_sub, const Tuning& tu);
 void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);

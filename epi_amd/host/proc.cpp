@@ -16,7 +16,8 @@
 // reference's viewers read (epi_vis_lightweight/main.cpp:493-520).
 // --output-binary writes CellManager::output's binary form.  --save-last
 // writes output_field_data's five files (last_data_cell, last_data_uvp,
-// last_data_w_alt, last_data_a_alt, last_data_B_alt) to DIR every 10 N steps
+// last_data_w_alt, last_data_a_alt, last_data_B_alt) and appends the
+// removed cells' records to DIR/cell_stat (append_stat_data) every 10 N steps
 // (proc.cpp:142-152) and after the last step; --use-last starts from such a
 // set, with DIR/last_data_cell given as the deck: load_uvp / load_w and both
 // buffers of ATP and ext_stim from the binaries (cman_init with use_last,
@@ -114,6 +115,7 @@ int main(int argc, char** argv) {
       if (!save_dir.empty() && cut > 0 && i % (cut * 10) == 0 && i > 0) {   // proc.cpp:142-152
         cman.clean_up();
         output_field_data(cman, ATP.first(), ext_stim.first(), save_dir);
+        cman.append_stat_data(save_dir + "/cell_stat");
       }
       cell_dynamics(cman);
       zzmax = calc_zzmax(cman);
@@ -137,7 +139,10 @@ int main(int argc, char** argv) {
     std::printf("done: %ld steps, %zu cells, %ld Ca solves, %.3f s\n", steps, cman.size(), ca_solves, s);
     if (!output.empty()) cman.output(output);
     if (!output_bin.empty()) cman.output(output_bin, true);
-    if (!save_dir.empty()) output_field_data(cman, ATP.first(), ext_stim.first(), save_dir);
+    if (!save_dir.empty()) {
+      output_field_data(cman, ATP.first(), ext_stim.first(), save_dir);
+      cman.append_stat_data(save_dir + "/cell_stat");
+    }
     if (!dump.empty()) {
       const CellData d = cman.download();
       FILE* f = std::fopen(dump.c_str(), "wb");

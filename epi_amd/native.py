@@ -58,7 +58,7 @@ class Cells(C.Structure):
                 ("rest_div_times", _ip), ("pair", _ip), ("dermis", _ip), ("lat", _ip),
                 ("is_malignant", _u8p), ("uid", _i64p),
                 ("conn_ptr", _ip), ("conn", _ip), ("gj", _dp), ("conn_cap", C.c_int32),
-                ("memb_adj", _ip)]
+                ("memb_adj", _ip), ("cst", _i64p)]
 
 
 class Driver(C.Structure):
@@ -98,13 +98,14 @@ SYMBOLS = [
     "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n", "epi_profile_sample", "epi_profile_launches",
     "epi_rd_begin", "epi_rd_forward", "epi_rd_get_ca2p", "epi_rd_adjoint", "epi_rd_end",
     "epi_mech_begin", "epi_mech_forward", "epi_mech_get_positions", "epi_mech_adjoint", "epi_mech_end",
+    "epi_set_timestep", "epi_take_cell_stats",
 ]
 
 MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
 
 # launch-shape keys of epi_set_tuning (include/epi_amd.h)
-TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4, "xcd": 5, "rd_group": 6, "ca_fuse": 7, "map_lanes": 8, "ca_group_small": 9,
-               "step_graph": 10}
+TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4, "xcd": 5, "rd_group": 6, "ca_fuse": 7,
+               "map_lanes": 8, "ca_group_small": 9, "step_graph": 10, "b_xcd": 11}
 
 class RdParams(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", C.POINTER(C.c_double))]
@@ -180,6 +181,8 @@ def load_library():
                                    C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_double), C.POINTER(MechParams)]
     L.epi_mech_end.argtypes = [vp]
     L.epi_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.epi_set_timestep.argtypes = [vp, C.c_int32, C.c_int64]
+    L.epi_take_cell_stats.argtypes = [vp, C.c_int32, _i64p, C.c_int32, C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -473,6 +476,7 @@ class TissueSet:
         out["conn"] = np.zeros(max(nnz, 1), dtype=np.int32)
         out["gj"] = np.zeros(max(nnz, 1), dtype=np.float64)
         out["memb_adj"] = np.zeros(6 * probe.nmemb, dtype=np.int32)
+        out["cst"] = np.zeros((n, 3), dtype=np.int64)
         cs = Cells()
         for k, dt in CELL_DTYPES.items():
             pt = {np.uint8: _u8p, np.int32: _ip, np.float64: _dp, np.int64: _i64p}[dt]
@@ -483,11 +487,25 @@ class TissueSet:
         cs.gj = _ptr(out["gj"], _dp)
         cs.conn_cap = out["conn"].size
         cs.memb_adj = _ptr(out["memb_adj"], _ip)
+        cs.cst = _ptr(out["cst"], _i64p)
         self._chk(self.L.epi_get_cells(self.h, t, C.byref(cs)), "get_cells")
         out["conn"] = out["conn"][:nnz]
         out["gj"] = out["gj"][:nnz]
         out["n"], out["nmemb"], out["nder"], out["sw"] = cs.n, cs.nmemb, cs.nder, cs.sw
         return out
+
+    # --- cell_stat (cellmanager.cpp:338-352, 559-567) ---
+    def set_timestep(self, t, step):
+        self._chk(self.L.epi_set_timestep(self.h, t, int(step)), "set_timestep")
+
+    def take_cell_stats(self, t):
+        """the records (k_aging_start, k_cornified, k_disap) of the DEAD
+        cells removed since the last call, in removal order"""
+        k = C.c_int32(0)
+        cap = self.geos[t].max_cells
+        out = np.zeros((cap, 3), dtype=np.int64)
+        self._chk(self.L.epi_take_cell_stats(self.h, t, _ptr(out, _i64p), cap, C.byref(k)), "take_cell_stats")
+        return out[:k.value].copy()
 
     def get_fields(self, t):
         g = self.geos[t]
@@ -552,6 +570,8 @@ class Tissue:
     def rng_cursor(self): return int(self.status().rng_pos)
     def error(self): return int(self.set.status(0)[0].err)
     def get_cells(self): return self.set.get_cells(0)
+    def set_timestep(self, step): self.set.set_timestep(0, step)
+    def take_cell_stats(self): return self.set.take_cell_stats(0)
     def get_fields(self): return self.set.get_fields(0)
     def set_positions(self, x, y, z): self.set.set_positions(0, x, y, z)
     def set_state(self, state=None, agek=None): self.set.set_state(0, state, agek)

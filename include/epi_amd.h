@@ -78,6 +78,9 @@ typedef struct epi_cells {
     int64_t* uid;
     int32_t* conn_ptr; int32_t* conn; double* gj; int32_t conn_cap;
     int32_t* memb_adj;
+    /* cell_stat per cell (define.h:198-202): k_aging_start, k_cornified,
+     * k_disap timesteps, 3 per cell, -1 = never set (download only) */
+    int64_t* cst;
 } epi_cells;
 
 /* Step driver of proc.cpp:131-173 per tissue, plus a forced Ca cadence:
@@ -143,6 +146,18 @@ int epi_calc_ca2p(epi_ctx* ctx, const double* zzmax);     /* ca2p.h:7, ca2p.cpp:
 int epi_calc_ca2p_n(epi_ctx* ctx, const double* zzmax, int32_t n_sub);
 int epi_should_calc_ca(epi_ctx* ctx, int32_t* flags);     /* cellmanager.h:97 */
 int epi_ca_calc_condition_reset(epi_ctx* ctx);            /* cellmanager.h:101 */
+/* CellManager::current_timestep (cellmanager.h:105, set per step at
+ * proc.cpp:132): the timestep the next state renewal records in cell_stat;
+ * each renewal (and each step of epi_step) advances it by one, and epi_step
+ * sets it from its drivers' step counters */
+int epi_set_timestep(epi_ctx* ctx, int32_t tissue, int64_t step);
+/* the cst_store of CellManager::append_stat_data (cellmanager.cpp:338-352,
+ * 559-567): the cell_stat records (k_aging_start, k_cornified, k_disap; -1 =
+ * never set) of the DEAD cells removed since the last call, in removal
+ * order, 3 int64 per record; the store is then emptied.  EPI_ERR_CAPACITY
+ * (store kept) when more than max_records are held; *n_records gets the
+ * count either way.  The store holds up to max_cells records. */
+int epi_take_cell_stats(epi_ctx* ctx, int32_t tissue, int64_t* records, int32_t max_records, int32_t* n_records);
 /* one sub-step of calc_ca2p's loop (ca2p.cpp:392-399) with a given iz_bound,
  * for kernel-level checks; maps from the last setup_map_lat */
 int epi_ca_substep(epi_ctx* ctx, int32_t iz_bound);
@@ -246,8 +261,10 @@ int epi_rd_end(epi_ctx* ctx);
 /* ---- launch shapes -------------------------------------------------------
  * No counterpart in the reference.  Launch-shape knobs of the grid kernels,
  * chosen per workload by epi_amd/autotune.py; they change timing only, never
- * results.  Values: *_ZB in {1, 2, 4, 8} (z planes per thread per chunk),
- * *_ROWS in {1, 2, 4} (y rows of 64 x points per block), CA_GROUP in {4, 8, 16} (lanes per
+ * results.  The stencils walk a list of tiles over every tissue of a launch;
+ * a tile is 64 *_ROWS consecutive points of a tissue's x-y plane (the block)
+ * by *_ZB z planes.  Values: *_ZB in {1, 2, 4, 8} (z planes per tile),
+ * *_ROWS in {1, 2, 4} (64-lane rows per block), CA_GROUP in {4, 8, 16} (lanes per
  * cell in the Ca cell kernel), XCD in {0, 1} (1: the Ca cell and ATP kernels
  * number their workgroups so that each of the 8 XCDs runs one contiguous run
  * of cells / grid rows), RD_GROUP in {8, 16} (lanes per cell in the reverse
@@ -259,13 +276,14 @@ int epi_rd_end(epi_ctx* ctx);
  * CA_GROUP_SMALL in {4, 8, 16} (lanes per cell in the solves CA_FUSE 2
  * fuses), STEP_GRAPH in {0, 1} (0, the default: epi_step launches the
  * renewal and neighbour-build kernels of a step one by one; 1: it replays
- * them, side stream included, from a HIP graph captured once per launch key).
- * EPI_ERR_INVALID for other keys or values. */
+ * them, side stream included, from a HIP graph captured once per launch key),
+ * B_XCD in {0, 1} (1: the B stencil's tile list is cut into one contiguous
+ * run per XCD).  EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
     EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
     EPI_TUNE_MAP_LANES = 8, EPI_TUNE_CA_GROUP_SMALL = 9, EPI_TUNE_STEP_GRAPH = 10,
-    EPI_TUNE_COUNT = 11
+    EPI_TUNE_B_XCD = 11, EPI_TUNE_COUNT = 12
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);

@@ -145,7 +145,34 @@ class CellManager {
   // clean_up (cellmanager.cpp:543-557) erases stale gj entries; stale
   // entries are never kept here, so there is nothing to do
   void clean_up() {}
+  // the step the driver is in (proc.cpp:132): cell_state_renew records it in
+  // the cells' cell_stat
   unsigned long long current_timestep = 0;
+  // push current_timestep to the device when the device's count differs
+  void sync_timestep() {
+    if ((long long)current_timestep != device_ts_) {
+      check(ctx_, epi_set_timestep(ctx_, 0, (int64_t)current_timestep), "epi_set_timestep");
+      device_ts_ = (long long)current_timestep;
+    }
+  }
+  void renewed() { device_ts_++; }   // a renewal advanced the device's timestep
+
+  // append_stat_data (cellmanager.cpp:559-567): one line "k_aging_start
+  // k_cornified k_disap" per DEAD cell removed since the last call, appended
+  // to filename; the store is then emptied.  A field the run never set is
+  // written as -1 (the reference leaves it uninitialised).
+  void append_stat_data(const std::string& filename) {
+    const size_t cap = (size_t)geo_.max_cells;
+    std::vector<int64_t> r(3 * cap);
+    int32_t k = 0;
+    check(ctx_, epi_take_cell_stats(ctx_, 0, r.data(), (int32_t)cap, &k), "epi_take_cell_stats");
+    if (k == 0) return;
+    FILE* f = std::fopen(filename.c_str(), "a");
+    if (!f) throw Error(EPI_ERR_INVALID, "stat data append error: " + filename);
+    for (int32_t i = 0; i < k; i++)
+      std::fprintf(f, "%lld %lld %lld\n", (long long)r[3 * i], (long long)r[3 * i + 1], (long long)r[3 * i + 2]);
+    std::fclose(f);
+  }
 
   // grids owned by this manager
   SwapData<FArr3D<double>> atp() { return SwapData<FArr3D<double>>(this, 0); }
@@ -232,6 +259,7 @@ class CellManager {
  private:
   epi_geometry geo_;
   epi_ctx* ctx_ = nullptr;
+  long long device_ts_ = 0;
 };
 
 template <class T>

@@ -147,6 +147,12 @@ struct eo_tissue {
     /* remove queue */
     int32_t* rq; int nrq;
     uint8_t sw;                     /* std::atomic<uint_fast8_t> (cellmanager.h:56) */
+    /* cell_stat (define.h:198-202) per cell: k_aging_start, k_cornified,
+     * k_disap timesteps, -1 = never set (the reference leaves the fields of a
+     * new Cell uninitialised); cst_store (cellmanager.h:106) of removed DEAD
+     * cells, nstat records; current_timestep (cellmanager.h:105) */
+    int64_t* cst; int64_t* stat; int nstat;
+    int64_t timestep;
     const double* rng; int64_t rng_n, rng_pos;
     /* periodic index tables, utils.cpp:32-72 */
     int *px_prev, *px_next, *py_prev, *py_next, *pz_prev, *pz_next;
@@ -224,6 +230,7 @@ eo_tissue* eo_create(const eo_geometry* g) {
     t->aindx = xcalloc((size_t)t->ANX * t->ANY * t->ANZ, 4);
     t->area = xcalloc((size_t)t->ANX * t->ANY * t->ANZ * N3, 4);
     t->rq = xcalloc(cap, 4);
+    t->cst = xcalloc((size_t)cap * 3, 8); t->stat = xcalloc((size_t)cap * 3, 8);
     /* utils.cpp:32-72 */
     int NX = t->NX, NY = t->NY, NZ = t->NZ;
     t->px_prev = xcalloc(NX, sizeof(int)); t->px_next = xcalloc(NX, sizeof(int));
@@ -260,7 +267,7 @@ void eo_destroy(eo_tissue* t) {
                     t->ca_buf[0], t->ca_buf[1], t->ip3_buf[0], t->ip3_buf[1],
                     t->atp[0], t->atp[1], t->bst[0], t->bst[1], t->cmap1, t->cmap2,
                     t->aindx, t->area, t->rq, t->px_prev, t->px_next, t->py_prev,
-                    t->py_next, t->pz_prev, t->pz_next};
+                    t->py_next, t->pz_prev, t->pz_next, t->cst, t->stat};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int i = 0; i < t->cap; i++) free(t->gj[i].e);
     free(t->gj);
@@ -297,6 +304,7 @@ int eo_create_cell(eo_tissue* t, int state, int fix_origin, double x, double y,
     t->lat[3 * i] = t->lat[3 * i + 1] = t->lat[3 * i + 2] = 0;
     t->gj[i].n = 0;
     t->uid[i] = t->next_uid++;
+    t->cst[3 * i] = t->cst[3 * i + 1] = t->cst[3 * i + 2] = -1;
     t->ca_buf[0][i] = t->ca_buf[1][i] = ca2p;      /* ca2p._set */
     t->ip3_buf[0][i] = t->ip3_buf[1][i] = ip3;     /* IP3._set */
     return i;
@@ -975,6 +983,7 @@ static void move_cell(eo_tissue* t, int dst, int src) {
     MV(rest_div_times); MV(pair); MV(dermis); MV(is_malignant); MV(uid); MV(nconn);
 #undef MV
     for (int k = 0; k < 3; k++) t->lat[3 * dst + k] = t->lat[3 * src + k];
+    for (int k = 0; k < 3; k++) t->cst[3 * dst + k] = t->cst[3 * src + k];
     memcpy(conn_row(t, dst), conn_row(t, src), sizeof(int32_t) * (size_t)t->nconn[src]);
     for (int b = 0; b < 2; b++) { t->ca_buf[b][dst] = t->ca_buf[b][src]; t->ip3_buf[b][dst] = t->ip3_buf[b][src]; }
     gj_map tmp = t->gj[dst]; t->gj[dst] = t->gj[src]; t->gj[src] = tmp; t->gj[src].n = 0;
@@ -990,6 +999,13 @@ static void remove_exec(eo_tissue* t) {
         int r = t->rq[i];
         if (r >= t->n) { set_err(t, EO_ERR_REMOVE_ALIAS); continue; }
         if (t->pair[r] >= 0) set_err(t, EO_ERR_PAIR_REMOVED);
+        if (t->state[r] == EO_DEAD) {   /* :343-346: the record goes to cst_store */
+            if (t->nstat >= t->cap) set_err(t, EO_ERR_CAPACITY);
+            else {
+                int64_t* o = &t->stat[3 * (size_t)t->nstat++];
+                o[0] = t->cst[3 * r]; o[1] = t->cst[3 * r + 1]; o[2] = t->timestep;
+            }
+        }
         int tail = --t->n;
         if (tail != r) move_cell(t, r, tail);
         else t->gj[r].n = 0;
@@ -1023,6 +1039,7 @@ void eo_cell_state_renew(eo_tissue* t) {
         case EO_ALIVE: /* :347-358 */
             if (t->agek[c] >= THRESH_DEAD) {
                 t->state[c] = EO_DEAD;   /* cornificate, cellmanager.cpp:48-53 */
+                t->cst[3 * c + 1] = t->timestep;
 #ifdef EO_OMP
                 __atomic_fetch_add(&t->sw, 1, __ATOMIC_RELAXED);
 #else
@@ -1049,7 +1066,11 @@ void eo_cell_state_renew(eo_tissue* t) {
             if (try_divide(t, c) == 0) t->ageb[c] += DT_Cell * ageb_const(t, c);
             break;
         case EO_MUSUME: /* :282-302 (SYSTEM == WHOLE) */
-            if (t->dermis[c] < 0 && t->pair[c] < 0) { t->state[c] = EO_ALIVE; break; }
+            if (t->dermis[c] < 0 && t->pair[c] < 0) {
+                t->state[c] = EO_ALIVE;
+                t->cst[3 * c] = t->timestep;   /* :287 */
+                break;
+            }
             if (!(t->rest_div_times[c] > 0 && try_divide(t, c))) t->ageb[c] += DT_Cell * ageb_const(t, c);
             break;
         default: break;
@@ -1508,6 +1529,7 @@ void eo_clean_up(eo_tissue* t) {
 
 /* proc.cpp:131-173 loop body; cell_dynamics :22-32 */
 int eo_step(eo_tissue* t, eo_driver* d) {
+    t->timestep = d->step;   /* cman.current_timestep = i (proc.cpp:132) */
     if (d->clean_every > 0 && d->step % d->clean_every == 0 && d->step > 0) eo_clean_up(t);
     eo_cell_interaction(t);
     eo_pos_copy(t);
@@ -1532,6 +1554,21 @@ int eo_step(eo_tissue* t, eo_driver* d) {
     }
     d->step++;
     return ran;
+}
+
+/* ---- cell_stat ----------------------------------------------------------- */
+void eo_set_timestep(eo_tissue* t, int64_t step) { t->timestep = step; }
+int64_t eo_timestep(const eo_tissue* t) { return t->timestep; }
+/* the per-cell records, 3 per cell */
+void eo_get_cst(const eo_tissue* t, int64_t* out) { memcpy(out, t->cst, sizeof(int64_t) * 3 * (size_t)t->n); }
+/* append_stat_data (cellmanager.cpp:559-567): the records of the DEAD cells
+ * removed since the last call, in removal order, then the store is cleared */
+int eo_take_cell_stats(eo_tissue* t, int64_t* out, int max_records, int* n_records) {
+    if (t->nstat > max_records) return EO_ERR_CAPACITY;
+    memcpy(out, t->stat, sizeof(int64_t) * 3 * (size_t)t->nstat);
+    *n_records = t->nstat;
+    t->nstat = 0;
+    return EO_OK;
 }
 
 /* ---- state exchange ---------------------------------------------------- */

@@ -81,6 +81,14 @@ void eo_calc_ca2p(eo_tissue* t, double zzmax);     /* ca2p.cpp:380 */
 int eo_should_calc_ca(const eo_tissue* t);     /* cellmanager.h:97 */
 void eo_ca_calc_condition_reset(eo_tissue* t); /* cellmanager.h:101 */
 void eo_clean_up(eo_tissue* t);                /* cellmanager.cpp:543 */
+/* cell_stat accounting: CellManager::current_timestep (cellmanager.h:105;
+ * eo_step sets it from the driver), per-cell records (k_aging_start,
+ * k_cornified, k_disap; -1 = never set) and append_stat_data's store
+ * (cellmanager.cpp:338-352, 559-567) */
+void eo_set_timestep(eo_tissue* t, int64_t step);
+int64_t eo_timestep(const eo_tissue* t);
+void eo_get_cst(const eo_tissue* t, int64_t* out);
+int eo_take_cell_stats(eo_tissue* t, int64_t* out, int max_records, int* n_records);
 
 /* Sub-steps of calc_ca2p exposed for kernel-level tests. */
 void eo_ca_substep(eo_tissue* t, int iz_bound); /* ca2p.cpp:392-399 body */

@@ -134,6 +134,11 @@ def lib(mt=False):
         L.eo_coef.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.eo_coef.restype = C.c_double
         L.eo_tri_bend.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.eo_set_timestep.argtypes = [C.c_void_p, C.c_int64]
+        L.eo_timestep.argtypes = [C.c_void_p]
+        L.eo_timestep.restype = C.c_int64
+        L.eo_get_cst.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.eo_take_cell_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int)]
         _libs[path] = L
     return _libs[path]
 
@@ -242,6 +247,29 @@ class OracleTissue:
     def should_calc_ca(self): return bool(self.L.eo_should_calc_ca(self.h))
     def ca_calc_condition_reset(self): self.L.eo_ca_calc_condition_reset(self.h)
     def clean_up(self): self.L.eo_clean_up(self.h)
+
+    # --- cell_stat (cellmanager.cpp:338-352, 559-567) ---
+    def set_timestep(self, step): self.L.eo_set_timestep(self.h, int(step))
+
+    def timestep(self): return int(self.L.eo_timestep(self.h))
+
+    def get_cst(self):
+        """per cell (k_aging_start, k_cornified, k_disap), -1 = never set"""
+        n = self.get_cells()["n"]
+        out = np.empty((n, 3), dtype=np.int64)
+        self.L.eo_get_cst(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out
+
+    def take_cell_stats(self):
+        """the records of the DEAD cells removed since the last call, in
+        removal order; the store is cleared"""
+        cap = self.geo.max_cells
+        out = np.empty((cap, 3), dtype=np.int64)
+        k = C.c_int(0)
+        rc = self.L.eo_take_cell_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)), cap, C.byref(k))
+        if rc:
+            raise RuntimeError(f"take_cell_stats: {rc}")
+        return out[:k.value].copy()
 
     def step(self, driver):
         return int(self.L.eo_step(self.h, C.byref(driver)))

@@ -53,7 +53,7 @@ def cost_fused_wins(k, tu):
     if k == "k_atp":
         return 6.0 + tu["atp_zb"] * 0.1 + tu["atp_rows"] * 0.01
     if k == "k_ext_stim":
-        return 5.0 + tu["b_zb"] * 0.1 + abs(tu["b_rows"] - 2) * 0.01
+        return 5.0 + tu["b_zb"] * 0.1 + abs(tu["b_rows"] - 2) * 0.01 - tu["b_xcd"] * 0.5
     if k == "k_map_stamp":
         return 30.0 if tu["map_lanes"] == 1 else 20.0
     return 1.0
@@ -66,7 +66,7 @@ def test_fused_shape_is_tuned_in_fused_mode():
     # 2 planes per thread: the fused launch's own optimum is taken
     assert rep["k_ca_cells"]["group16"] < rep["k_ca_cells"]["group8"]
     assert best["ca_fuse"] == 1 and best["ca_group"] == 8 and best["atp_zb"] == 2
-    assert best["b_zb"] == 1 and best["b_rows"] == 2 and best["map_lanes"] == 4
+    assert best["b_zb"] == 1 and best["b_rows"] == 2 and best["map_lanes"] == 4 and best["b_xcd"] == 1
     assert ts.tu == best
     assert len([k for k in rep["ca_fuse"] if k.startswith("fused_")]) == 9
 

@@ -3,7 +3,7 @@ against the oracle on the same seeded inputs:
 
 - the large single tissue (128x128 sheet, 20,000 free cells, 128x128x256
   grid, seed 2) for 1000 steps with a forced calc_ca2p every 500 and the
-  reference's sw gating;
+  reference's sw gating, checked every 100 steps;
 - the shipped deck (epi5_MT/input_2layer_nfix16.dat, 13,728 cells) for 200
   steps with a forced solve every 100;
 - the batch of 256 tissues (sheet side U[32,96], free cells U[500,8000],
@@ -44,10 +44,13 @@ def _check(a_cells, a_fields, a_cursor, o, tag):
     assert a_cursor == o.rng_cursor(), tag
 
 
-@pytest.mark.timeout(900)
-def test_large_config_1000_steps(oracle_mod):
-    """1000 steps of the large tissue, a forced solve after steps 499 and
-    999 (plus whatever the sw gating fires), checked after each solve."""
+LARGE_CHUNK = 100
+
+
+@pytest.fixture(scope="module")
+def large_pair(oracle_mod):
+    """The large tissue in the oracle and on the GPU, advanced chunk by chunk
+    by the tests below (each test is the next 100 steps of one run)."""
     from epi_amd import native
     O = oracle_mod
     spec = configs.config_large(2)
@@ -59,17 +62,31 @@ def test_large_config_1000_steps(oracle_mod):
     t = native.Tissue(g)
     t.load(spec)
     t.set_rng(draws)
-    do, dg = O.Driver(0, 0, 0, 500, 0), native.Driver(0, 0, 0, 500, 0)
-    for k in range(2):
-        print(f"large config 1000 steps: steps {500 * k}-{500 * k + 499}", flush=True)
-        t.step(dg, 500)
-        for _ in range(500):
-            o.step(do)
-        assert t.error() == 0
-        _check(t.get_cells(), t.get_fields(), t.rng_cursor(), o, f"large step {500 * k + 499}")
-    st = t.status()
-    assert st.ca_solves >= 2 and st.steps == 1000
-    assert st.ndiv > 0 or t.get_cells()["n"] != spec["cells"]["x"].size
+    state = {"o": o, "t": t, "do": O.Driver(0, 0, 0, 500, 0), "dg": native.Driver(0, 0, 0, 500, 0), "done": 0,
+             "n0": spec["cells"]["x"].size}
+    yield state
+    t.close()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("chunk", range(10))
+def test_large_config_1000_steps(large_pair, chunk):
+    """1000 steps of the large tissue with a forced solve after steps 499 and
+    999 (plus whatever the sw gating fires), in chunks of 100 steps, each
+    checked against the oracle: the chunks are one run, in order."""
+    s = large_pair
+    assert s["done"] == chunk * LARGE_CHUNK, "the chunks run in order"
+    print(f"large config: steps {s['done']}-{s['done'] + LARGE_CHUNK - 1}", flush=True)
+    s["t"].step(s["dg"], LARGE_CHUNK)
+    for _ in range(LARGE_CHUNK):
+        s["o"].step(s["do"])
+    s["done"] += LARGE_CHUNK
+    assert s["t"].error() == 0
+    _check(s["t"].get_cells(), s["t"].get_fields(), s["t"].rng_cursor(), s["o"], f"large step {s['done'] - 1}")
+    st = s["t"].status()
+    assert st.steps == s["done"] and st.ca_solves >= s["done"] // 500
+    if s["done"] == 1000:
+        assert st.ndiv > 0 or s["t"].get_cells()["n"] != s["n0"]
 
 
 @pytest.mark.timeout(600)
@@ -121,11 +138,15 @@ def batch256():
     return specs, draws, out
 
 
-@pytest.mark.timeout(900)
-def test_batch256_equals_single_runs(batch256):
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("part", range(8))
+def test_batch256_equals_single_runs(batch256, part):
+    """tissues 32 part .. 32 part + 31 of the batch, each against its own
+    single-tissue context over the same steps"""
     from epi_amd import native
     specs, draws, got = batch256
-    for t, s in enumerate(specs):
+    for t in range(32 * part, 32 * part + 32):
+        s = specs[t]
         st, cells, fields = got[t]
         assert st.err == 0, (t, st.err)
         assert st.ca_solves >= 1, t
@@ -144,8 +165,6 @@ def test_batch256_equals_single_runs(batch256):
         s1 = one.status()
         assert (st.rng_pos, st.ca_solves, st.n, st.next_uid) == (s1.rng_pos, s1.ca_solves, s1.n, s1.next_uid), t
         one.close()
-        if t % 32 == 31:
-            print(f"batch256: tissues 0-{t} equal their single runs", flush=True)
 
 
 @pytest.mark.timeout(900)
@@ -294,7 +313,10 @@ def test_mech_adjoint_full_batch(oracle_mod):
             assert grads[i][k] == pytest.approx(go[k], rel=1e-8), (i, k)
 
     # central differences of the GPU window, every tissue at once (each
-    # tissue's loss depends on its own parameters only)
+    # tissue's loss depends on its own parameters only).  The loss is only
+    # piecewise smooth (adhesion has kinks at contact and at 1.2 sigma), so
+    # the step is small: with 1e-5 a few of the 64 tissues had a pair cross a
+    # kink inside the step
     def losses(name, scale):
         p = [dict(q, **{name: q[name] * scale}) for q in params]
         ts.mech_forward(p, n_steps)
@@ -305,7 +327,7 @@ def test_mech_adjoint_full_batch(oracle_mod):
         return np.array(out)
 
     for name in MECH_BASE:
-        eps = 1e-5
+        eps = 1e-7
         lp, lm = losses(name, 1 + eps), losses(name, 1 - eps)
         fd = (lp - lm) / (2 * eps * np.array([q[name] for q in params]))
         g = np.array([gr[name] for gr in grads])
