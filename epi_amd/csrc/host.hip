@@ -317,7 +317,12 @@ int sync_status(epi_ctx* c) {
 }
 
 int status_check(epi_ctx* c) {
-  for (int t = 0; t < c->T; t++) c->th[t].n_est = c->h_st[t].n;
+  for (int t = 0; t < c->T; t++) {
+    c->th[t].n_est = c->h_st[t].n;
+    // the device holds zzmax as bits (k_zzmax, set_zz); the host's double
+    // follows them, so the Ca lists' iz_bound is the device's
+    memcpy(&c->h_st[t].zzmax, &c->h_st[t].zzmax_bits, sizeof(double));
+  }
   c->len_known = true;
   for (int t = 0; t < c->T; t++)
     if (c->h_st[t].err) {
@@ -505,7 +510,8 @@ int ca_list(epi_ctx* c, const int* ids, int ntis, int cg, int bs, int zb, CaList
     HIPCHK(c, hipEventRecord(c->ev_cal, c->stream));
     c->cal_key = key;
     c->cal = CaList{c->d_cal, c->d_cal + ntis + 1, c->d_cal + 2 * ntis + 1,
-                    reinterpret_cast<const int4*>(c->d_cal + hdr), ntis, (int)ncb, (int)tiles, bs};
+                    reinterpret_cast<const int4*>(c->d_cal + hdr), ntis, (int)ncb, (int)tiles, bs,
+                    ids ? ids[0] : 0, izb[0]};
   }
   *out = c->cal;
   return EPI_OK;

@@ -230,9 +230,87 @@ __device__ __forceinline__ int wl_find(F off, int lo, int nt, int w) {
 
 constexpr int WL_LDS_T = 1024;   // tissues whose list a block stages in LDS
 
-// ---- ext_stim B: calc_ext_stim (ext_stim.cpp:18-69) ------------------------
-// One thread per (x, y) point of a tile; the z column rolls through
-// registers, ZB planes per tile.
+// One (x, y) column of a tile: planes l0 .. l1 - 1, at most ZB of them.
+// Every load of the column is issued before any value is used: the column's
+// own planes l0 - 1 .. l1 (mirror prev_z(0) = 1, self next_z(NZ - 1) =
+// NZ - 1, utils.cpp:32-72), then each plane's four lateral neighbours, masks
+// and voxel cell, then the cells' states and ages.  So a tile waits for three
+// memory round trips, not three per plane.  The arithmetic is
+// ext_stim.cpp:36-57's, in its order.
+template <int ZB>
+__device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int l1, int bp) {
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const double* __restrict__ carr = t.bf[bp];
+  double* __restrict__ narr = t.bf[1 - bp];
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  const long long PL = t.PL;
+  const long long cxy = gidx(t, j, k, 0);
+  const int nl = l1 - l0;
+  double cz[ZB + 2];
+  uint8_t mz[ZB + 2];
+#pragma unroll
+  for (int q = 0; q < ZB + 2; q++) {
+    int l = q == 0 ? (l0 == 0 ? 1 : l0 - 1) : l0 + q - 1;
+    if (l > NZ - 1) l = NZ - 1;
+    cz[q] = 0;
+    mz[q] = 0;
+    if (q <= nl + 1) {
+      cz[q] = carr[cxy + l * PL];
+      mz[q] = cm2[cxy + l * PL];
+    }
+  }
+  double xm[ZB], xp[ZB], ym[ZB], yp[ZB];
+  uint8_t mxm[ZB], mxp[ZB], mym[ZB], myp[ZB];
+  int cell[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; q++) {
+    xm[q] = xp[q] = ym[q] = yp[q] = 0;
+    mxm[q] = mxp[q] = mym[q] = myp[q] = 0;
+    cell[q] = -1;
+    if (q < nl) {
+      const int l = l0 + q;
+      const long long vxm = gidx(t, prev_x, k, l), vym = gidx(t, j, prev_y, l);
+      const long long vyp = gidx(t, j, next_y, l), vxp = gidx(t, next_x, k, l);
+      xm[q] = carr[vxm]; ym[q] = carr[vym]; yp[q] = carr[vyp]; xp[q] = carr[vxp];
+      mxm[q] = cm2[vxm]; mym[q] = cm2[vym]; myp[q] = cm2[vyp]; mxp[q] = cm2[vxp];
+      cell[q] = t.cmap1[cxy + l * PL];
+    }
+  }
+  uint8_t cs[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; q++) cs[q] = cell[q] >= 0 ? t.state[cell[q]] : (uint8_t)MEMB;
+  double age[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; q++) age[q] = (cs[q] == DEAD || cs[q] == ALIVE) ? t.agek[cell[q]] : 0.0;
+  const double DB = 0.0009;
+  const double inv_dz = t.inv_dz;
+#pragma unroll
+  for (int q = 0; q < ZB; q++) {
+    if (q < nl) {
+      const int l = l0 + q;
+      const bool flg = cs[q] == DEAD || cs[q] == ALIVE;
+      const double cext = cz[q + 1];
+      const double nv = cext + DT_Ca * (DB *
+          ((double)mxm[q] * (xm[q] - cext)
+           + (double)mym[q] * (ym[q] - cext)
+           + (double)mz[q] * (cz[q] - cext)
+           + (double)mz[q + 2] * (cz[q + 2] - cext)
+           + (double)myp[q] * (yp[q] - cext)
+           + (double)mxp[q] * (xp[q] - cext))
+          * inv_dz * inv_dz + fB(age[q], cext, flg));
+      narr[cxy + l * PL] = nv;
+      // periodic copies (:63-66)
+      if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+      if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+      if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+    }
+  }
+}
+
+// The B stencil over its tile list.
+// One thread per (x, y) point of a tile (b_column).
 template <int ZB>
 __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, const int* __restrict__ woff,
                                                   const int* __restrict__ wizb, int ntis, int bp, int xcd) {
@@ -248,7 +326,6 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, con
   }
   auto off = [&](int i) { return lds ? s_off[i] : woff[i]; };
   const int bs = blockDim.x;
-  const double DB = 0.0009;
   int ti = 0;
   for (TileWalk tw(off(ntis), xcd, blockIdx.x, gridDim.x); tw.w < tw.end; tw.w += tw.stride) {
     ti = wl_find(off, ti, ntis, tw.w);
@@ -263,53 +340,7 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, con
     const int j = p % NX, k = p / NX;
     const int l0 = (local / npt) * ZB;
     const int l1 = min(l0 + ZB, iz_bound);
-    const double* __restrict__ carr = t.bf[bp];
-    double* __restrict__ narr = t.bf[1 - bp];
-    const uint8_t* __restrict__ cm2 = t.cmap2;
-    const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
-    const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
-    const double inv_dz = t.inv_dz;
-    const long long cxy = gidx(t, j, k, 0);
-    // the z column of (j, k) rolls through registers; mirror z boundary
-    // prev_z(0) = 1, next_z(NZ - 1) = NZ - 1 (utils.cpp:32-72)
-    const int pz = l0 == 0 ? 1 : l0 - 1;
-    double czm = carr[cxy + pz * t.PL];
-    double mzm = (double)cm2[cxy + pz * t.PL];
-    double cc = carr[cxy + l0 * t.PL];
-    double mc = (double)cm2[cxy + l0 * t.PL];
-    for (int l = l0; l < l1; l++) {
-      const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
-      const long long v = cxy + l * t.PL;
-      const long long vzp = cxy + next_z * t.PL;
-      const double czp = carr[vzp];
-      const double mzp = (double)cm2[vzp];
-      double dum_age = 0;
-      bool flg = false;
-      const int cell = t.cmap1[v];
-      if (cell >= 0) {
-        const uint8_t s = t.state[cell];
-        if (s == DEAD || s == ALIVE) { dum_age = t.agek[cell]; flg = true; }
-      }
-      const double cext = cc;
-      const long long vxm = gidx(t, prev_x, k, l), vym = gidx(t, j, prev_y, l);
-      const long long vyp = gidx(t, j, next_y, l), vxp = gidx(t, next_x, k, l);
-      const double nv = cext + DT_Ca * (DB *
-          ((double)cm2[vxm] * (carr[vxm] - cext)
-           + (double)cm2[vym] * (carr[vym] - cext)
-           + mzm * (czm - cext)
-           + mzp * (czp - cext)
-           + (double)cm2[vyp] * (carr[vyp] - cext)
-           + (double)cm2[vxp] * (carr[vxp] - cext))
-          * inv_dz * inv_dz + fB(dum_age, cext, flg));
-      narr[v] = nv;
-      // periodic copies (:63-66)
-      if (k == 0) narr[gidx(t, j, NY, l)] = nv;
-      if (j == 0) narr[gidx(t, NX, k, l)] = nv;
-      if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
-      // plane l + 1 sees plane l below it; at l = 0 the plane below 1 is 0
-      czm = cc; mzm = mc;
-      cc = czp; mc = mzp;
-    }
+    b_column<ZB>(t, j, k, l0, l1, bp);
   }
 }
 
@@ -710,7 +741,8 @@ template <int CG>
 __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T, CaList cl, int cb, int cp,
                                                       int ap, int bp, int xcd, const RdTis* __restrict__ sv,
                                                       int slot) {
-  const int4 e = cl.cmap[xcd_order(blockIdx.x, gridDim.x, xcd)];
+  const unsigned b = xcd_order(blockIdx.x, gridDim.x, xcd);
+  const int4 e = cl.ntis == 1 ? make_int4(cl.t0, (int)b, cl.ncb, 0) : cl.cmap[b];
   const Tis& t = T[e.x];
   ca_cells_part<CG, C_BLOCK>(t, e.y, e.z, cb, cp, ap, bp, nullptr, nullptr, sv ? sv + e.x : nullptr, slot);
 }
@@ -718,9 +750,80 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
 // ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound,
 // one tile of a solve's list: blockDim.x consecutive points of the (x, y)
 // plane times ZB planes; the z column rolls through registers.
+// One (x, y) column of an ATP tile, planes l0 .. l1 - 1 (at most ZB): as
+// b_column, every load first (the column, the lateral neighbours and masks,
+// each plane's voxel word), then the owners' diff_u, then ATP_refresh's
+// point update (atp_new) per plane.
+template <int ZB>
+__device__ __forceinline__ void atp_column(const Tis& t, int j, int k, int l0, int l1, int ap, int cp) {
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const double* __restrict__ carr = t.atp[ap];
+  double* __restrict__ narr = t.atp[1 - ap];
+  const double* __restrict__ dfu = dfu_buf(t, cp);
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  const long long PL = t.PL;
+  const long long cxy = gidx(t, j, k, 0);
+  const int nl = l1 - l0;
+  double cz[ZB + 2];
+  uint8_t mz[ZB + 2];
+#pragma unroll
+  for (int q = 0; q < ZB + 2; q++) {
+    int l = q == 0 ? (l0 == 0 ? 1 : l0 - 1) : l0 + q - 1;   // mirror z boundary (utils.cpp:32-72)
+    if (l > NZ - 1) l = NZ - 1;
+    cz[q] = 0;
+    mz[q] = 0;
+    if (q <= nl + 1) {
+      cz[q] = carr[cxy + l * PL];
+      mz[q] = cm2[cxy + l * PL];
+    }
+  }
+  double xm[ZB], xp[ZB], ym[ZB], yp[ZB];
+  uint8_t mxm[ZB], mxp[ZB], mym[ZB], myp[ZB];
+  unsigned w[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; q++) {
+    xm[q] = xp[q] = ym[q] = yp[q] = 0;
+    mxm[q] = mxp[q] = mym[q] = myp[q] = 0;
+    w[q] = 0;
+    if (q < nl) {
+      const int l = l0 + q;
+      const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
+      const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
+      xm[q] = carr[vxm]; xp[q] = carr[vxp]; ym[q] = carr[vym]; yp[q] = carr[vyp];
+      mxm[q] = cm2[vxm]; mxp[q] = cm2[vxp]; mym[q] = cm2[vym]; myp[q] = cm2[vyp];
+      w[q] = t.vox[cxy + l * PL];
+    }
+  }
+  double diffu[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; q++) {
+    const unsigned cell1 = w[q] & 0x7fffffffu;
+    diffu[q] = cell1 ? dfu[cell1 - 1] : 0.0;
+  }
+  const double inv_dx = t.inv_dx;
+#pragma unroll
+  for (int q = 0; q < ZB; q++) {
+    if (q < nl) {
+      const int l = l0 + q;
+      const double nv = atp_new(cz[q + 1], xm[q], xp[q], ym[q], yp[q], cz[q], cz[q + 2], (double)mxm[q],
+                                (double)mxp[q], (double)mym[q], (double)myp[q], (double)mz[q], (double)mz[q + 2],
+                                diffu[q], (double)(w[q] >> 31), inv_dx);
+      narr[cxy + l * PL] = nv;
+      if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+      if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+      if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+    }
+  }
+}
+
+// ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound,
+// one tile of a solve's list: blockDim.x consecutive points of the (x, y)
+// plane times ZB planes (atp_column).
 template <int ZB>
 __device__ __forceinline__ void atp_tile(const Tis& t, int local, int izb, int ap, int cp) {
-  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const int NX = t.NX, NY = t.NY;
   const int bs = blockDim.x;
   const int plane = NX * NY;
   const int npt = (plane + bs - 1) / bs;
@@ -728,44 +831,11 @@ __device__ __forceinline__ void atp_tile(const Tis& t, int local, int izb, int a
   if (p >= plane) return;
   const int j = p % NX, k = p / NX;
   const int l0 = (local / npt) * ZB;
-  const int l1 = min(l0 + ZB, izb);
-  const double* __restrict__ carr = t.atp[ap];
-  double* __restrict__ narr = t.atp[1 - ap];
-  const double* __restrict__ dfu = dfu_buf(t, cp);
-  const double inv_dx = t.inv_dx;
-  const uint8_t* __restrict__ cm2 = t.cmap2;
-  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
-  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
-  const long long cxy = gidx(t, j, k, 0);
-  const int pz = l0 == 0 ? 1 : l0 - 1;   // mirror z boundary (utils.cpp:32-72)
-  double czm = carr[cxy + pz * t.PL];
-  double mzm = (double)cm2[cxy + pz * t.PL];
-  double cc = carr[cxy + l0 * t.PL];
-  double mc = (double)cm2[cxy + l0 * t.PL];
-  for (int l = l0; l < l1; l++) {
-    const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
-    const long long v = cxy + l * t.PL;
-    const long long vzp = cxy + next_z * t.PL;
-    const double czp = carr[vzp];
-    const double mzp = (double)cm2[vzp];
-    const unsigned w = t.vox[v];
-    const unsigned cell1 = w & 0x7fffffffu;
-    const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
-    const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
-    const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
-    const double nv = atp_new(cc, carr[vxm], carr[vxp], carr[vym], carr[vyp], czm, czp, (double)cm2[vxm],
-                              (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], mzm, mzp, diffu,
-                              (double)(w >> 31), inv_dx);
-    narr[v] = nv;
-    if (k == 0) narr[gidx(t, j, NY, l)] = nv;
-    if (j == 0) narr[gidx(t, NX, k, l)] = nv;
-    if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
-    czm = cc; mzm = mc;
-    cc = czp; mc = mzp;
-  }
+  atp_column<ZB>(t, j, k, l0, min(l0 + ZB, izb), ap, cp);
   // plane l = iz_bound: copies of values left in the next buffer (:334-337);
   // the corner (NX, NY) ends as a copy of (0, 0)
   if (l0 == 0) {
+    double* __restrict__ narr = t.atp[1 - ap];
     if (k == 0) narr[gidx(t, j, NY, izb)] = narr[gidx(t, j, 0, izb)];
     if (j == 0) narr[gidx(t, NX, k, izb)] = narr[gidx(t, 0, k, izb)];
     if (j == 0 && k == 0) narr[gidx(t, NX, NY, izb)] = narr[gidx(t, 0, 0, izb)];
@@ -780,9 +850,14 @@ constexpr int CA_LDS_T = 256;
 template <int ZB>
 __device__ __forceinline__ void atp_walk(const Tis* __restrict__ T, const CaList& cl, int b, int nb, int xcd,
                                          int ap, int cp) {
+  const int nt = cl.ntis;
+  if (nt == 1) {   // one tissue: its id and iz_bound come with the list
+    const Tis& t = T[cl.t0];
+    for (TileWalk tw(cl.atiles, xcd, b, nb); tw.w < tw.end; tw.w += tw.stride) atp_tile<ZB>(t, tw.w, cl.izb0, ap, cp);
+    return;
+  }
   __shared__ int s_off[CA_LDS_T + 1];
   __shared__ int2 s_ti[CA_LDS_T];
-  const int nt = cl.ntis;
   const bool lds = nt <= CA_LDS_T;
   if (lds) {
     for (int i = threadIdx.x; i <= nt; i += blockDim.x) {
@@ -814,7 +889,7 @@ template <int CG, int ZB>
 __global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T, CaList cl, int cb, int cp,
                                                       int ap, int bp, int ncb) {
   if ((int)blockIdx.x < ncb) {
-    const int4 e = cl.cmap[blockIdx.x];
+    const int4 e = cl.ntis == 1 ? make_int4(cl.t0, (int)blockIdx.x, ncb, 0) : cl.cmap[blockIdx.x];
     const Tis& t = T[e.x];
     ca_cells_part<CG, F_BLOCK>(t, e.y, e.z, cb, 1 - cp, 1 - ap, bp, t.atp[ap], dfu_buf(t, cp));
   } else {

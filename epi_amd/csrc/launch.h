@@ -91,12 +91,15 @@ struct RdTis;
 // grid is a list of tiles of `abs` plane points by a_zb planes, aoff[i]
 // being the first tile of the list's i-th tissue (aoff[ntis] = atiles), with
 // its iz_bound aizb[i] and tissue id atis[i].
+// A list of one tissue (a single-tissue solve) also carries that tissue id
+// and iz_bound by value (t0, izb0): the kernels then read no list memory.
 struct CaList {
   const int* aoff;
   const int* aizb;
   const int* atis;
   const int4* cmap;
   int ntis, ncb, atiles, abs;
+  int t0, izb0;
 };
 // points per ATP tile of a solve: the fused kernel's block, or the ATP
 // kernel's 64 a_rows

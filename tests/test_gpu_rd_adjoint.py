@@ -147,3 +147,27 @@ def test_adjoint_against_oracle_and_differences(oracle_mod, n_sub, every):
         fd = (L(kpa=kp) - L(kpa=km)) / (2 * h)
         assert abs(g["kpa"][i] - fd) <= 1e-6 * abs(fd), (i, g["kpa"][i], fd)
     t.set.rd_end()
+
+
+def test_window_after_a_status_read_back(oracle_mod):
+    """A window's list of ATP tiles takes iz_bound from the zzmax the host
+    holds: a forward window (which ends with a status read-back) followed by
+    an adjoint and another forward must still give the oracle's loss and
+    values."""
+    o, t = _pair(oracle_mod, _spec())
+    zz = t.calc_zzmax()
+    st = t.get_cells()["state"]
+    afm = np.isin(st, AFM)
+    pert = dict(du=0.0105, dp=0.095, gj0=0.95)
+    n_sub = 30
+    tgt = o.rd_forward(zz, n_sub, **pert)
+    lo, _ = o.rd_adjoint(zz, n_sub, tgt)
+    t.set.rd_begin(zz)
+    t.set.rd_forward([pert], n_sub)
+    t.set.synchronize()
+    loss, _ = t.set.rd_adjoint([{}], n_sub, 10, [tgt])
+    assert loss[0] == pytest.approx(lo, rel=1e-10)
+    t.set.rd_forward([pert], n_sub)
+    u = t.set.rd_get_ca2p(0)
+    assert np.max(np.abs(u[afm] - tgt[afm]) / np.abs(tgt[afm])) <= 1e-10
+    t.set.rd_end()
