@@ -258,6 +258,47 @@ int epi_rd_adjoint(epi_ctx* ctx, const epi_rd_params* params, int32_t n_sub, int
                    const double* const* targets, double* loss, epi_rd_grad* grads);
 int epi_rd_end(epi_ctx* ctx);
 
+/* ---- calc_ext_stim on caller-held grids, allocating nothing ---------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): calc_ext_stim (ext_stim.cpp:18-69) over grids the caller
+ * holds, with no context, through the same kernels as epi_calc_ext_stim.
+ * The sweep's grid descriptors, per-grid status and tile work list live in
+ * one static device array per GPU of EPI_NOALLOC_SCRATCH_BYTES, held until
+ * the process ends, so a call takes at most EPI_NOALLOC_MAX_GRIDS grids.
+ * Calls that overlap on one GPU (on different streams, or from different
+ * threads) race on that array.  The call allocates nothing (no hipMalloc,
+ * hipFree or allocating library routine): it queues four kernels on `stream`
+ * (a hipStream_t; NULL = the null stream) and returns.
+ *
+ * Layout: grid point (j, k, l), 0 <= j <= nx, 0 <= k <= ny, 0 <= l <= nz, is
+ * element l*pl + k*xp + j of b_in, b_out, cmap1 and cmap2 (x fastest, xp >=
+ * nx + 1, pl >= xp*(ny + 1)); planes x = nx and y = ny hold the periodic
+ * copies of x = 0 and y = 0, as in Field<T, NX+1, NY+1, NZ+1>.  b_out is the
+ * reference's ext_stim.second(): the caller swaps the two afterwards
+ * (ext_stim.cpp:68). */
+typedef struct epi_bfield_view {
+    int32_t nx, ny, nz, xp;
+    int64_t pl;
+    double lz;              /* dz = lz / nz */
+    double zzmax;           /* calc_zzmax: iz_bound = (int)((zzmax + 2.8) * (nz / lz)) */
+    const double* b_in;     /* device: B before the sweep */
+    double* b_out;          /* device: B after it, on [0,nx) x [0,ny) x [0,iz_bound) and the copies */
+    const int32_t* cmap1;   /* device: cell index per point, -1 = none (setup_map_lat) */
+    const uint8_t* cmap2;   /* device: 0 / 1 per point */
+    const uint8_t* state;   /* device: per cell */
+    const double* agek;     /* device: per cell */
+} epi_bfield_view;
+enum { EPI_NOALLOC_MAX_GRIDS = 1024 };
+/* bytes of the static scratch array of epi_calc_ext_stim_noalloc per GPU */
+int64_t epi_noalloc_scratch_bytes(void);
+/* views: device array of ngrids views (read by the first kernel); err: device
+ * array of ngrids codes, EPI_OK or EPI_ERR_OUT_OF_GRID (iz_bound outside
+ * [0, nz]: that grid is left as it was).  Returns EPI_ERR_INVALID for ngrids
+ * outside [1, EPI_NOALLOC_MAX_GRIDS] or null pointers, EPI_ERR_HIP when the
+ * scratch address or a launch fails. */
+int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t ngrids, int32_t* err, void* stream);
+
 /* ---- launch shapes -------------------------------------------------------
  * No counterpart in the reference.  Launch-shape knobs of the grid kernels,
  * chosen per workload by epi_amd/autotune.py; they change timing only, never

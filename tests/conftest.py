@@ -33,7 +33,7 @@ def deck_spec():
 # parity first, then the windows and their adjoints, then batches.  Within a
 # module the file order is kept.
 _GPU_ORDER = ("test_gpu_parity", "test_gpu_rd_adjoint", "test_gpu_mech_adjoint", "test_gpu_batched",
-              "test_gpu_configs")
+              "test_gpu_configs", "test_zz_no_alloc")
 
 
 def pytest_collection_modifyitems(config, items):
