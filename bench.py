@@ -392,7 +392,9 @@ def run_batched(args):
         return ts
 
     tuning, tune_s = None, 0.0
-    if not args.no_tune:
+    if args.tuning:
+        tuning = json.loads(args.tuning)
+    elif not args.no_tune:
         progress(f"autotuning on {len(specs)} tissues")
         tt = make(range(len(specs)))
         tuning, rep = autotune.tune(tt, drivers(len(specs)), warm_steps=1, b_reps=5, ca_sub=10,
@@ -544,7 +546,9 @@ def run_ca_solve(args):
     ts = native.TissueSet([spec["geometry"]])
     ts.load(0, spec)
     ts.set_rng(0, native.mt19937_64_uniform(spec["seed"], n_draws_needed(spec, 64)))
-    if not args.no_tune:
+    if args.tuning:
+        ts.set_tuning(json.loads(args.tuning))
+    elif not args.no_tune:
         from epi_amd import autotune
         tuning, _ = autotune.tune(ts, [native.Driver(0, 0, 0, 0, 0)], log=sys.stderr)
     ts.step([native.Driver(0, 0, 0, 0, 0)], 5)

@@ -122,9 +122,11 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
 
     # per sub-step: the two kernels against one fused launch, whose lanes per
     # cell and z planes per thread of its grid part are chosen for it
+    # (1 or 2 planes: the fused grid part is latency-bound, and 4 or more
+    # planes per thread take the column-rolling form meant for large tiles)
     report["ca_fuse"] = {"unfused": report["xcd"][f"xcd{best['xcd']}"]}
     for g in CA_GROUPS:
-        for zb in (1, 2, 4):
+        for zb in (1, 2):
             ts.set_tuning({"ca_fuse": 1, "ca_group": g, "atp_zb": zb})
             report["ca_fuse"][f"fused_group{g}_zb{zb}"] = _time(ts, "k_ca_fused", run_ca)
     key = min(report["ca_fuse"], key=report["ca_fuse"].get)

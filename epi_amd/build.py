@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libepi_amd.so")
-SOURCES = ["k_mech.hip", "k_renew.hip", "k_connect.hip", "k_field.hip", "k_adjoint.hip", "host.hip", "noalloc.hip"]
+SOURCES = ["k_mech.hip", "k_renew.hip", "k_connect.hip", "k_field.hip", "k_adjoint.hip", "host.hip", "noalloc.hip", "params.hip", "init_gen.hip"]
 ARCH = os.environ.get("EPI_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

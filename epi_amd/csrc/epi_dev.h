@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "params.h"
+
 namespace epi {
 
 // cell states, epi5_MT/define.h:36-47
@@ -24,43 +26,16 @@ enum { OK = 0, ERR_CAPACITY = 1, ERR_STRONG_INTERACTION = 2, ERR_NO_DERMIS = 3,
        ERR_OUT_OF_GRID = 4, ERR_REMOVE_ALIAS = 5, ERR_RNG_EXHAUSTED = 6,
        ERR_INVALID = 7, ERR_HIP = 8, ERR_PAIR_REMOVED = 9 };
 
-// ---- constants: define.h:69-197 and the physics files -------------------
-constexpr double DT_Cell = 0.01;           // define.h:86
-constexpr double DT_Ca = 0.01;             // define.h:89
-constexpr double R_max = 1.4;              // define.h:116
-constexpr double R_der = 1.4;              // define.h:119
-constexpr double COMPRESS_FACTOR = 4;      // define.h:125
-constexpr double THRESH_SP = 3.0;          // define.h:132
-constexpr double LJ_THRESH = 1.2;          // define.h:135
-constexpr int MEMB_ADJ = 6;                // define.h:147-153
-constexpr double THRESH_DEAD = 22.0;       // define.h:156
-constexpr double ca2p_init = 0.122;        // define.h:165
-constexpr double gj_init = 0.99;           // define.h:174
-constexpr double FAC_MAP = 2.0;            // define.h:184
-constexpr int SW_THRESH = 20;              // define.h:187
+// ---- constants -----------------------------------------------------------
+// The physics constants are runtime values, one record per tissue (Tis::prm,
+// params.h).  What stays here fixes the shape of the build.
+constexpr int MEMB_ADJ = 6;                // define.h:147-153 (TRI_MEMB)
 constexpr int MAX_CONN = 400;              // define.h:80, cell_connection.cpp:30
 constexpr int N3 = 200;                    // cell_connection.cpp:28
 // slots between two buckets: the capacity rounded up to a multiple of 4, so
 // the sweep can read a bucket's indices four at a time (16-byte aligned)
 __host__ __device__ constexpr int bucket_stride(int bcap) { return (bcap + 3) & ~3; }
-constexpr double AREA_GRID = 2.0 + 1e-7;   // cell_connection.cpp:16-19
-constexpr int CA_ITR = (int)(10.0 / DT_Ca);  // ca2p.cpp:14
-
-// cell_interaction.cpp:21-45
-// eps_m, K_TOTAL, Kspring and KBEND are per-tissue values (Tis::m_eps,
-// m_kadh, m_kspring, m_kbend) so that a mechanics window can vary them;
-// these are the reference's values
-constexpr double eps_m_ref = 0.01;
-constexpr double P_MEMB = 1. / COMPRESS_FACTOR;
-constexpr double DER_DER_CONST = 0.2;
-constexpr double K_TOTAL_ref = 3.0;
-constexpr double K_DESMOSOME_RATIO = 0.01;
-constexpr double Kspring_ref = 25.0;
-constexpr double Kspring_d = 5.0;
-constexpr double KBEND_ref = 0.5;
-constexpr double delta_R = 0.4 * R_der;
-constexpr double para_ljp2 = 0.005;
-constexpr double Kspring_division = 5.0;
+constexpr double AREA_GRID = 2.0 + 1e-7;   // cell_connection.cpp:16-19 (the bucket edge)
 
 // force laws (cell_interaction.cpp:68-76)
 enum : int { LAW_NONE = 0, LAW_OTHER, LAW_DER_DER, LAW_FIXMU, LAW_FIX_MEMB,
@@ -83,6 +58,7 @@ struct Status {
   long long timestep;     // CellManager::current_timestep; k_remove advances it once per renewal
   int nstat;              // records in cst_store
   int pad_st;
+  long long grid_b;       // sum of nx*ny*iz_bound over the B sweeps (k_wl_b)
 };
 
 // ---- per-tissue descriptor (constant between allocations) ---------------
@@ -95,8 +71,11 @@ struct Tis {
   double LX, LY, LZ, dx, dy, dz, inv_dx, inv_dy, inv_dz;
   int ANX, ANY, ANZ, bcap;
   int irx, iry, irz, pad1;
-  double ca_du, ca_dp;        // ca2p_du (ca2p.cpp:183) and dp (ca2p.cpp:17)
-  double m_eps, m_kadh, m_kspring, m_kbend;   // cell_interaction.cpp:21-28
+  // ca2p_du (ca2p.cpp:183), dp (ca2p.cpp:17), eps_m, K_TOTAL, Kspring and
+  // KBEND (cell_interaction.cpp:21-28) as the kernels use them: prm's
+  // values, or a window's while one is open
+  double ca_du, ca_dp;
+  double m_eps, m_kadh, m_kspring, m_kbend;
   Status* st;
   // cells
   uint8_t* state; uint8_t* is_malignant; uint8_t* touch_air;
@@ -148,6 +127,8 @@ struct Tis {
   int* cmap1; uint8_t* cmap2; unsigned* vox;
   // replay stream
   const double* rng; long long rng_n;
+  // the tissue's runtime parameters (epi_set_params)
+  Params prm;
 };
 
 __device__ __forceinline__ void set_err(Status* st, int e) { atomicCAS(&st->err, 0, e); }

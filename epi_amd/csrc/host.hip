@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -85,7 +86,7 @@ const char* kKernelNames[K_COUNT] = {
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
-    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place", "k_wl"};
+    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place", "k_wl", "k_ca_swap"};
 
 // with ready_only, only the pairs whose kernels have finished (the rest stay
 // pending for a later read-back)
@@ -192,7 +193,7 @@ struct epi_ctx {
   // advanced it
   std::vector<long long> dev_ts;
   // throughput accounting of epi_step
-  std::vector<long long> steps, cell_steps, grid_b, grid_ca;
+  std::vector<long long> steps, cell_steps, grid_ca;
   std::string err;
 };
 
@@ -211,6 +212,21 @@ int fail(epi_ctx* c, int code, const std::string& msg) {
   } while (0)
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// the coefficients a window may vary, back to the tissue's parameters
+void window_values_from_params(Tis& t) {
+  t.ca_du = t.prm.ca2p_du;
+  t.ca_dp = t.prm.dp;
+  t.m_eps = t.prm.eps_m;
+  t.m_kadh = t.prm.K_TOTAL;
+  t.m_kspring = t.prm.Kspring;
+  t.m_kbend = t.prm.KBEND;
+}
+
+// proc.cpp:158: i * DT_Cell > T_TURNOVER && flg_forced_sc
+bool sc_turnover(const Params& p, const epi_driver& d) {
+  return (double)d.step * p.DT_Cell > p.T_TURNOVER && d.force_cornif;
+}
 
 struct Carver {
   char* base;
@@ -390,7 +406,7 @@ int izb_est(const epi_ctx* c, bool ca) {
   for (int t = 0; t < c->T; t++) {
     const Tis& ti = c->tis[t];
     double zz = c->h_st ? c->h_st[t].zzmax : 0.0;
-    int v = ca ? (int)((zz + FAC_MAP * R_max) / ti.dz) : (int)((zz + FAC_MAP * R_max) * ti.inv_dz);
+    int v = ca ? (int)((zz + ti.prm.iz_margin) / ti.dz) : (int)((zz + ti.prm.iz_margin) * ti.inv_dz);
     m = std::max(m, std::min(v + 2, c->th[t].NZ + 1));
   }
   return m;
@@ -404,7 +420,7 @@ long long b_tiles_est(const epi_ctx* c) {
   for (int t = 0; t < c->T; t++) {
     const Tis& ti = c->tis[t];
     const double zz = c->h_st ? c->h_st[t].zzmax : 0.0;
-    const int izb = std::min((int)((zz + FAC_MAP * R_max) * ti.inv_dz) + 1, c->th[t].NZ);
+    const int izb = std::min((int)((zz + ti.prm.iz_margin) * ti.inv_dz) + 1, c->th[t].NZ);
     n += (long long)((ti.NX * ti.NY + bs - 1) / bs) * ((std::max(izb, 1) + zb - 1) / zb);
   }
   return n;
@@ -476,7 +492,7 @@ int ca_list(epi_ctx* c, const int* ids, int ntis, int cg, int bs, int zb, CaList
     const TH& h = c->th[t];
     const int cells = std::max(h.n_est + 64 - h.nmemb - h.nder, 1);
     nb[i] = (cells + cpb - 1) / cpb;
-    izb[i] = (int)((c->h_st[t].zzmax + FAC_MAP * R_max) / c->tis[t].dz);
+    izb[i] = (int)((c->h_st[t].zzmax + c->tis[t].prm.iz_margin) / c->tis[t].dz);
     if (izb[i] < 0 || izb[i] > h.NZ) izb[i] = 0;   // k_ca_vox reports it
     ncb += nb[i];
     tiles += (long long)((h.NX * h.NY + bs - 1) / bs) * ((izb[i] + zb - 1) / zb);
@@ -518,17 +534,17 @@ int ca_list(epi_ctx* c, const int* ids, int ntis, int cg, int bs, int zb, CaList
 }
 
 // calc_ca2p (ca2p.cpp:380-402) for the tissues in tlist (nullptr = all)
-// (with d_tlist, c->h_tlist holds the same ntis ids).  The list covers the
-// solving tissues only: a gated solve of one small tissue in a large batch
-// launches that tissue's blocks, not the batch's largest.
-int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
+// (with d_tlist, c->h_tlist holds the same ntis ids), n_sub sub-steps.  The
+// list covers the solving tissues only: a gated solve of one small tissue in
+// a large batch launches that tissue's blocks, not the batch's largest.
+int run_ca_n(epi_ctx* c, const int* d_tlist, int ntis, int n_sub) {
   int mn = 1;
   long long box = 1;
   for (int i = 0; i < ntis; i++) {
     const int t = d_tlist ? c->h_tlist[i] : i;
     const TH& h = c->th[t];
     const double zz = c->h_st[t].zzmax;
-    const int iz = std::min((int)((zz + FAC_MAP * R_max) / c->tis[t].dz) + 2, h.NZ + 1);
+    const int iz = std::min((int)((zz + c->tis[t].prm.iz_margin) / c->tis[t].dz) + 2, h.NZ + 1);
     mn = std::max(mn, h.n_est + 64);
     box = std::max(box, (long long)h.NX * h.NY * iz);
   }
@@ -552,7 +568,38 @@ int run_ca(epi_ctx* c, const int* d_tlist, int ntis, int n_sub = CA_ITR) {
   launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
   launch_ca_substeps(c->stream, c->d_tis, cl, c->cb, c->cp, c->ap, c->bp, n_sub, tu);
   launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
+  if ((n_sub & 1) && d_tlist && ntis < c->T) {
+    // the parities name the same buffer for every tissue: an odd count
+    // flipped them for all, but advanced only the listed tissues.  They go
+    // back, and the listed tissues' buffers trade contents instead.
+    c->cp ^= 1;
+    c->ap ^= 1;
+    long long gmax = 1;
+    for (int i = 0; i < ntis; i++) gmax = std::max(gmax, c->th[c->h_tlist[i]].gsize);
+    launch_ca_swap(c->stream, c->d_tis, d_tlist, ntis, mn, gmax);
+  }
   HIPCHK(c, hipGetLastError());
+  return EPI_OK;
+}
+
+// calc_ca2p with each tissue's own sub-step count, Ca_ITR (ca2p.cpp:14,
+// 392).  Tissues that share it solve together; a list that mixes counts
+// (tissues given different Ca_avg_time / DT_Ca) solves one count at a time.
+int run_ca(epi_ctx* c, const int* d_tlist, int ntis) {
+  auto id = [&](int i) { return d_tlist ? c->h_tlist[i] : i; };
+  const int first = c->tis[id(0)].prm.Ca_ITR;
+  bool same = true;
+  for (int i = 1; i < ntis; i++) same = same && c->tis[id(i)].prm.Ca_ITR == first;
+  if (same) return run_ca_n(c, d_tlist, ntis, first);
+  std::map<int, std::vector<int>> groups;
+  for (int i = 0; i < ntis; i++) groups[c->tis[id(i)].prm.Ca_ITR].push_back(id(i));
+  for (const auto& g : groups) {
+    // h_tlist may still feed an earlier copy: wait for the stream first
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::copy(g.second.begin(), g.second.end(), c->h_tlist);
+    HIPCHK(c, hipMemcpy(c->d_tlist, c->h_tlist, sizeof(int) * g.second.size(), hipMemcpyHostToDevice));
+    RCHK(run_ca_n(c, c->d_tlist, (int)g.second.size(), g.first));
+  }
   return EPI_OK;
 }
 
@@ -596,7 +643,6 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
   c->dev_ts.assign(ntissues, 0);
   c->steps.assign(ntissues, 0);
   c->cell_steps.assign(ntissues, 0);
-  c->grid_b.assign(ntissues, 0);
   c->grid_ca.assign(ntissues, 0);
   for (int t = 0; t < ntissues; t++) {
     const epi_geometry& g = geos[t];
@@ -638,12 +684,13 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
     ti.irx = (int)(Rad * g.nx / g.lx);
     ti.iry = (int)(Rad * g.ny / g.ly);
     ti.irz = (int)(Rad * g.nz / g.lz);
-    ti.ca_du = 0.01;
-    ti.ca_dp = 0.1;
-    ti.m_eps = eps_m_ref;
-    ti.m_kadh = K_TOTAL_ref;
-    ti.m_kspring = Kspring_ref;
-    ti.m_kbend = KBEND_ref;
+    // epi5_MT's parameters with this geometry's grid and box
+    epi_params dp;
+    epi_params_default(&dp);
+    dp.LX = g.lx; dp.LY = g.ly; dp.LZ = g.lz;
+    dp.NX = g.nx; dp.NY = g.ny; dp.NZ = g.nz;
+    derive_params(dp, ti.prm);
+    window_values_from_params(ti);
   }
   // the Ca list: 3 ints per tissue + 1, padding, and one int4 per cell block
   // (8 cells per block at the most lanes per cell)
@@ -755,6 +802,7 @@ int epi_load_cells(epi_ctx* c, int32_t t, const epi_cell_input* in) {
     const bool want_memb = i < in->nmemb, want_der = !want_memb && i < in->nmemb + in->nder;
     if ((s == MEMB) != want_memb || (s == DER) != want_der) return fail(c, EPI_ERR_INVALID, "rows must be MEMB, then DER, then the rest");
   }
+  const Params& pm = ti.prm;
   std::vector<uint8_t> st(n), mal(n);
   std::vector<int> fo(n), rdt(n), pair(n, -1), derm(n, -1), src(n, -1), nl(n, 0);
   std::vector<double> dat(n), ca(n), ip(n);
@@ -763,8 +811,8 @@ int epi_load_cells(epi_ctx* c, int32_t t, const epi_cell_input* in) {
     st[i] = in->state[i];
     mal[i] = in->is_malignant ? in->is_malignant[i] : 0;
     fo[i] = (st[i] == DER || st[i] == MEMB) ? -1 : in->fix_origin[i];           // cell.cpp:67
-    rdt[i] = st[i] == FIX ? 15 : in->rest_div_times[i];                         // cell.cpp:18-20
-    dat[i] = st[i] == FIX ? 6.0 : st[i] == MUSUME ? 6.0 : 0.0;                    // cell.cpp:11-15
+    rdt[i] = st[i] == FIX ? (int)pm.div_max : in->rest_div_times[i];            // cell.cpp:18-20
+    dat[i] = st[i] == FIX ? pm.agki_max_fix : st[i] == MUSUME ? pm.agki_max : 0.0;  // cell.cpp:11-15
     uid[i] = i;
     if (in->pair && in->pair[i] >= 0) {
       if (in->pair[i] >= n) return fail(c, EPI_ERR_INVALID, "pair out of range");
@@ -836,7 +884,8 @@ int epi_value_init(epi_ctx* c, int32_t t) {
   const int n = c->h_st[t].n;
   std::vector<uint8_t> st(n);
   RCHK(download(c, st.data(), ti.state, n));
-  std::vector<double> ca(n, ca2p_init), avg(n, ca2p_init), exi(n, 0.97), ip(n, 0.0);
+  const Params& pm = ti.prm;   // cell_init.cpp:23-28
+  std::vector<double> ca(n, pm.ca2p_init), avg(n, pm.ca2p_init), exi(n, pm.ex_inert_init), ip(n, pm.IP3_init);
   for (int i = h.nmemb + h.nder; i < n; i++)
     if (st[i] == DEAD) { avg[i] = 0; ca[i] = 0; }
   for (int b = 0; b < 2; b++) {
@@ -1007,14 +1056,14 @@ int epi_calc_ca2p(epi_ctx* c, const double* zzmax) {
 int epi_calc_ca2p_n(epi_ctx* c, const double* zzmax, int32_t n_sub) {
   if (!c || n_sub < 0) return EPI_ERR_INVALID;
   RCHK(set_zz(c, zzmax));
-  RCHK(run_ca(c, nullptr, c->T, n_sub));
+  RCHK(run_ca_n(c, nullptr, c->T, n_sub));
   return sync_status(c);
 }
 
 int epi_should_calc_ca(epi_ctx* c, int32_t* flags) {
   if (!c) return EPI_ERR_INVALID;
   RCHK(sync_status(c));
-  for (int t = 0; t < c->T; t++) flags[t] = (c->h_st[t].sw & 255) >= SW_THRESH;
+  for (int t = 0; t < c->T; t++) flags[t] = (c->h_st[t].sw & 255) >= c->tis[t].prm.SW_THRESH;
   return EPI_OK;
 }
 
@@ -1062,10 +1111,11 @@ int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
   if (!c) return EPI_ERR_INVALID;
   // a zzmax whose calc_ca2p bound is iz_bound
   std::vector<double> zz(c->T);
-  for (int t = 0; t < c->T; t++) zz[t] = iz_bound * c->tis[t].dz - FAC_MAP * R_max + 0.5 * c->tis[t].dz;
+  for (int t = 0; t < c->T; t++) zz[t] = iz_bound * c->tis[t].dz - c->tis[t].prm.iz_margin + 0.5 * c->tis[t].dz;
   RCHK(set_zz(c, zz.data()));
   for (int t = 0; t < c->T; t++)
-    if ((int)((zz[t] + FAC_MAP * R_max) / c->tis[t].dz) != iz_bound) return fail(c, EPI_ERR_INVALID, "iz_bound");
+    if ((int)((zz[t] + c->tis[t].prm.iz_margin) / c->tis[t].dz) != iz_bound)
+      return fail(c, EPI_ERR_INVALID, "iz_bound");
   const int mn = max_n(c);
   long long box = 1;
   for (auto& h : c->th) box = std::max(box, (long long)h.NX * h.NY * (iz_bound + 1));
@@ -1086,7 +1136,10 @@ int epi_ca_substep(epi_ctx* c, int32_t iz_bound) {
 // (:155-157), which read only the fixed positions, states and radii (and the
 // B field) that the rest of the build leaves alone.  The build, the longer
 // chain, is issued first.
-hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb) {
+// With join false the main stream does not wait for the side stream's maps
+// and B sweep: epi_step joins it after issuing the next step's forces, which
+// read neither.
+hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb, bool join = true) {
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
@@ -1098,7 +1151,7 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb) {
   launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
   launch_ext_stim(c->side_stream, c->d_tis, c->T, c->bp, b_tiles_est(c), c->d_wl, c->tu);
   chk(hipEventRecord(c->ev_join, c->side_stream));
-  chk(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  if (join) chk(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   chk(hipGetLastError());
   return e;
 }
@@ -1109,16 +1162,19 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb) {
 // chain skips cells past the tissue's count, and the rounding lets a key
 // outlive the divisions of many steps.  Launches whose kernels are timed go
 // one by one, so that their events exist.
-int run_build(epi_ctx* c, int mn, int nb, int izb) {
+// *joined tells whether the main stream already waits for the side stream.
+int run_build(epi_ctx* c, int mn, int nb, int izb, bool* joined) {
   constexpr unsigned long long seg =
       (1ull << K_RENEW_FLAGS) | (1ull << K_RENEW_SCAN) | (1ull << K_RENEW_APPLY) | (1ull << K_REMOVE) |
       (1ull << K_PAIR_DISPERSE) | (1ull << K_BUCKET_INSERT) | (1ull << K_BUCKET_SORT) | (1ull << K_BUCKET_PLACE) |
       (1ull << K_SWEEP) | (1ull << K_FINISH) | (1ull << K_SNAPSHOT) | (1ull << K_ZZ_SET) | (1ull << K_ZZMAX) |
       (1ull << K_MAP_CLEAR) | (1ull << K_MAP_STAMP) | (1ull << K_MAP_HALO) | (1ull << K_EXT_STIM);
   if (!c->tu.step_graph || (g_prof.mask & seg)) {
-    HIPCHK(c, issue_build(c, mn, nb, izb));
+    *joined = false;
+    HIPCHK(c, issue_build(c, mn, nb, izb, false));
     return EPI_OK;
   }
+  *joined = true;   // a captured graph rejoins its forked stream
   int mcap = 1;
   for (auto& h : c->th) mcap = std::max(mcap, h.cap);
   const int mr = std::max(mn, std::min((mn + 1023) / 1024 * 1024, mcap));
@@ -1185,13 +1241,14 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     }
     forces_issued = false;
     const int nb = 1 - c->cb;
-    RCHK(run_build(c, mn, nb, izb_est(c, false)));
+    bool joined = true;
+    RCHK(run_build(c, mn, nb, izb_est(c, false), &joined));
     c->len_known = false;
     c->cb = nb;
     c->bp ^= 1;
     bool sc_due = false;   // proc.cpp:158-163, decided from host state alone
     for (int t = 0; t < c->T; t++)
-      if ((double)drv[t].step * DT_Cell > 6000.0 && drv[t].force_cornif) sc_due = true;
+      if (sc_turnover(c->tis[t].prm, drv[t])) sc_due = true;
     if (it + 1 < nsteps && !sc_due) {
       // the rows are those of this step's build, whose longest list is known
       // only after the read-back: the slots follow the last known maxlen,
@@ -1201,14 +1258,21 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
         cap_n = std::max(cap_n, c->th[t].cap);
         known_len = std::max(known_len, c->h_st[t].maxlen);
       }
+      // the read-back follows this step's main chain only: the maps and the
+      // B sweep (side stream) run on beside the next forces, which read
+      // neither, and what the main stream issues after those forces (the
+      // next renewal, a solve) waits for them.  Their zzmax reaches the host
+      // with a later read-back; a solve reads it again first.
       RCHK(sync_status_begin(c));
       forces_pb = c->pb;
       forces_cap = launch_forces_rows(c->stream, c->d_tis, c->T, c->pb, c->cb, cap_n, known_len);
       c->pb ^= 1;
       forces_issued = true;
+      if (!joined) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
       HIPCHK(c, hipGetLastError());
       RCHK(sync_status_end(c));
     } else {
+      if (!joined) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
       HIPCHK(c, hipGetLastError());
       RCHK(sync_status(c));
     }
@@ -1217,20 +1281,15 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     int nsc = 0, nca = 0;
     std::vector<int> ca_list;
     for (int t = 0; t < c->T; t++) {
-      double v;
-      unsigned long long b = c->h_st[t].zzmax_bits;
-      memcpy(&v, &b, 8);
-      c->h_st[t].zzmax = v;
       const Tis& ti = c->tis[t];
       c->steps[t]++;
       c->cell_steps[t] += c->h_st[t].n;
-      c->grid_b[t] += (long long)ti.NX * ti.NY * (int)((v + FAC_MAP * R_max) * ti.inv_dz);
       epi_driver& d = drv[t];
       // proc.cpp:158-163
-      if ((double)d.step * DT_Cell > 6000.0 && d.force_cornif) {
+      if (sc_turnover(ti.prm, d)) {
         d.force_cornif = 0;
         c->h_tlist[nsc++] = t;
-        d.num_sc = 1;
+        d.num_sc = (int32_t)ti.prm.NUM_SC_INIT;
       }
     }
     if (nsc) {
@@ -1240,18 +1299,22 @@ int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
     for (int t = 0; t < c->T; t++) {
       epi_driver& d = drv[t];
       const bool forced = d.ca_every > 0 && ((d.step + 1) % d.ca_every) == 0;
-      if ((c->h_st[t].sw & 255) >= SW_THRESH || d.num_sc > 0 || forced) {
+      const Tis& ti = c->tis[t];
+      if ((c->h_st[t].sw & 255) >= ti.prm.SW_THRESH || d.num_sc > 0 || forced) {
         ca_list.push_back(t);
         if (d.num_sc > 0) d.num_sc--;
         c->ca_solves[t]++;
-        const Tis& ti = c->tis[t];
-        c->grid_ca[t] += (long long)CA_ITR * ti.NX * ti.NY * (int)((c->h_st[t].zzmax + FAC_MAP * R_max) / ti.dz);
+        c->grid_ca[t] += (long long)ti.prm.Ca_ITR * ti.NX * ti.NY *
+                         (int)((c->h_st[t].zzmax + ti.prm.iz_margin) / ti.dz);
       }
       d.step++;
       c->dev_ts[t]++;
     }
     nca = (int)ca_list.size();
     if (nca) {
+      // the solve's work lists need this step's zzmax, which an early
+      // read-back may have missed
+      if (forces_issued) RCHK(sync_status(c));
       if (nsc) HIPCHK(c, hipStreamSynchronize(c->stream));
       for (int i = 0; i < nca; i++) c->h_tlist[i] = ca_list[i];
       HIPCHK(c, hipMemcpyAsync(c->d_tlist, c->h_tlist, sizeof(int) * nca, hipMemcpyHostToDevice, c->stream));
@@ -1267,6 +1330,26 @@ int epi_synchronize(epi_ctx* c) {
   return sync_status(c);
 }
 
+int epi_set_params(epi_ctx* c, int32_t t, const epi_params* p) {
+  RCHK(check_tissue(c, t));
+  if (!p) return fail(c, EPI_ERR_INVALID, "null parameters");
+  if (c->rd.active || c->mech.active) return fail(c, EPI_ERR_INVALID, "parameters are fixed while a window is open");
+  if (const char* why = params_refusal(*p, c->tis[t].prm)) return fail(c, EPI_ERR_INVALID, why);
+  Tis& ti = c->tis[t];
+  derive_params(*p, ti.prm);
+  ti.prm.outputdir[sizeof(ti.prm.outputdir) - 1] = 0;
+  window_values_from_params(ti);
+  HIPCHK(c, hipMemcpy(c->d_tis + t, &ti, sizeof(Tis), hipMemcpyHostToDevice));
+  return EPI_OK;
+}
+
+int epi_get_params(epi_ctx* c, int32_t t, epi_params* p) {
+  RCHK(check_tissue(c, t));
+  if (!p) return fail(c, EPI_ERR_INVALID, "null parameters");
+  *p = static_cast<const epi_params&>(c->tis[t].prm);
+  return EPI_OK;
+}
+
 int epi_get_status(epi_ctx* c, int32_t t, epi_status* out) {
   RCHK(check_tissue(c, t));
   int rc = sync_status(c);
@@ -1278,7 +1361,7 @@ int epi_get_status(epi_ctx* c, int32_t t, epi_status* out) {
   out->ca_solves = c->ca_solves[t];
   out->steps = c->steps[t];
   out->cell_steps = c->cell_steps[t];
-  out->grid_points_b = c->grid_b[t];
+  out->grid_points_b = s.grid_b;
   out->grid_points_ca = c->grid_ca[t];
   return rc;
 }
@@ -1509,7 +1592,7 @@ int epi_set_lists(epi_ctx* c, int32_t t, const int32_t* ptr, const int32_t* conn
   const TH& h = c->th[t];
   const int n = c->h_st[t].n;
   std::vector<int> col((size_t)n * h.ns, 0), nl(n), nlo(n, 0), srow(n);
-  std::vector<double> g((size_t)n * h.ns, gj_init);
+  std::vector<double> g((size_t)n * h.ns, c->tis[t].prm.gj_init);
   std::vector<long long> u(n);
   RCHK(download(c, u.data(), ti.uid, n));
   for (int i = 0; i < n; i++) {
@@ -1625,7 +1708,7 @@ int epi_rd_begin(epi_ctx* c, const double* zzmax) {
   rd.izb.assign(c->T, 0);
   rd.zz.assign(zzmax, zzmax + c->T);
   for (int t = 0; t < c->T; t++) {
-    rd.izb[t] = (int)((zzmax[t] + FAC_MAP * R_max) / c->tis[t].dz);   // ca2p.cpp:386
+    rd.izb[t] = (int)((zzmax[t] + c->tis[t].prm.iz_margin) / c->tis[t].dz);   // ca2p.cpp:386
     if (rd.izb[t] < 0 || rd.izb[t] > c->tis[t].NZ) return fail(c, EPI_ERR_OUT_OF_GRID, "iz_bound outside the grid");
     // the reverse pass gathers a cell's voxels from its stamp box, which must
     // not wrap onto itself in x or y
@@ -1808,9 +1891,12 @@ int epi_rd_end(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
   auto& rd = c->rd;
   if (!rd.active) return EPI_OK;
-  for (int t = 0; t < c->T; t++) { c->tis[t].ca_du = 0.01; c->tis[t].ca_dp = 0.1; }
+  std::vector<double> g0(c->T);
+  for (int t = 0; t < c->T; t++) {
+    window_values_from_params(c->tis[t]);
+    g0[t] = c->tis[t].prm.gj_init;
+  }
   HIPCHK(c, hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * c->T, hipMemcpyHostToDevice, c->stream));
-  std::vector<double> g0(c->T, gj_init);
   HIPCHK(c, hipMemcpyAsync(rd.d_gj0, g0.data(), 8 * c->T, hipMemcpyHostToDevice, c->stream));
   const long long items = rd_items(c);
   launch_rd_restore(c->stream, c->d_tis, rd.d, rd.d_gj0, c->T, c->cb, c->cp, c->ap, 0, items);
@@ -1972,10 +2058,7 @@ int epi_mech_end(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
   auto& m = c->mech;
   if (!m.active) return EPI_OK;
-  for (int t = 0; t < c->T; t++) {
-    c->tis[t].m_eps = eps_m_ref; c->tis[t].m_kadh = K_TOTAL_ref;
-    c->tis[t].m_kspring = Kspring_ref; c->tis[t].m_kbend = KBEND_ref;
-  }
+  for (int t = 0; t < c->T; t++) window_values_from_params(c->tis[t]);
   HIPCHK(c, hipMemcpyAsync(c->d_tis, c->tis.data(), sizeof(Tis) * c->T, hipMemcpyHostToDevice, c->stream));
   launch_mech_state(c->stream, c->d_tis, m.d, c->T, 0, 4, c->pb, max_n(c));
   HIPCHK(c, hipGetLastError());

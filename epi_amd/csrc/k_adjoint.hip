@@ -18,7 +18,6 @@
 namespace epi {
 
 namespace {
-constexpr double STIM11 = 0.002, Kaa = 0.5, Da = 1.0, H0 = 0.5;
 constexpr int RL = 16;   // lanes per cell for row-wise copies
 
 // every (cell, list entry) of the rows of buffer cb, RL lanes per cell
@@ -201,6 +200,7 @@ struct RdEntry {
   double abar_du_iag;   // abar * du * IAG
   double lp_dp_iag;     // lp * DT_Ca * dp * IAG
   double uc, pc;
+  double DT_Ca, wd;
 };
 
 __device__ __forceinline__ void rd_entry_bwd(const RdEntry& k, int kd, int o, double gv, double uo, double po,
@@ -209,10 +209,11 @@ __device__ __forceinline__ void rd_entry_bwd(const RdEntry& k, int kd, int o, do
   double to_u = k.abar_du_iag * gv;
   double lge = lg_in;
   if (kd == 3) {
-    // d fw / d |d| = -(1 - tanh^2 y) / (2 * 0.1) with y = (0.1 - |d|) / 0.1,
-    // and 1 - tanh^2 y = 4 E / (1 + E)^2 with E = e^(2y) (<= e^2, as |d| >= 0)
+    // d fw / d |d| = -(1 - tanh^2 y) / (2 * 0.1) with y = (wd - |d|) / 0.1,
+    // and 1 - tanh^2 y = 4 E / (1 + E)^2 with E = e^(2y)
     const double d = uo - k.uc;
-    const double E = exp(2.0 * ((0.1 - fabs(d)) / 0.1));
+    const double DT_Ca = k.DT_Ca;
+    const double E = exp(2.0 * ((k.wd - fabs(d)) / 0.1));
     const double fwd = -5.0 * (4.0 * E / ((1.0 + E) * (1.0 + E)));
     const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
     to_u += lg_in * DT_Ca * fwd * sg;
@@ -242,6 +243,9 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
   const double* lA = r.lA[qa];
   double* nA = r.lA[qa == 2 ? 0 : qa + 1];
   const double du = t.ca_du, dp = t.ca_dp;
+  const Params& pm = t.prm;
+  const CaK ck = ca_k(pm);
+  const double DT_Ca = ck.DT_Ca, Kpp = ck.Kpp, STIM11 = pm.STIM11;
   double pdu = 0, pdp = 0;
   double* const lu_w = r.lu[q];
   double* const lp_w = r.lp[q];
@@ -359,9 +363,9 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
     acc_own = group_sum<RG>(acc_own);
     const double Bavg = grid_avg8(t, t.bf[bp], ix, iy, iz);
     const double Aavg = r.sAv[slot * cap + c];
-    const double diffu = ca2p_reaction_factor(uc, hc, pc, Bavg) + du * iag * su;
+    const double diffu = ca2p_reaction_factor(ck, uc, hc, pc, Bavg) + du * iag * su;
     const double abar = DT_Ca * luc + (diffu > 0 ? acc_own * DT_Ca * STIM11 : 0.0);
-    const RdEntry k{abar * du * iag, lpc * DT_Ca * dp * iag, uc, pc};
+    const RdEntry k{abar * du * iag, lpc * DT_Ca * dp * iag, uc, pc, DT_Ca, ck.wd};
     double own_u = 0;
 #pragma unroll
     for (int i = 0; i < REP; i++)
@@ -375,13 +379,15 @@ __device__ __forceinline__ void rd_cells_part(const Tis& t, const RdTis& r, int 
     }
     own_u = group_sum<RG>(own_u);
     if (lane < 8) {   // the eight grid_avg8 corners of the IP3 production term
+      const double H0 = ck.H0;
       const double wA = lpc * DT_Ca * kpa * H0 / ((H0 + Aavg) * (H0 + Aavg)) * 0.125;
       const long long vv = corner_var(t, r, ix + (lane & 1), iy + ((lane >> 1) & 1), iz + ((lane >> 2) & 1));
       if (vv >= 0) atomicAdd(&nA[vv], wA);
     }
     if (lane == 0) {
-      const double kflux = 8.1, mu0 = 0.567, mu1 = 0.1, kmu = 0.05, pb = 0.11, pbb = 0.89, pk1 = 0.7;
-      const double kg = 0.1, gam = 2.0, k2 = 0.7;
+      const double kflux = ck.k_flux, mu0 = ck.mu0, mu1 = ck.mu1, kmu = ck.kmu, pb = ck.para_b, pbb = ck.para_bb,
+                   pk1 = ck.para_k1;
+      const double kg = ck.kg, gam = ck.gamma, k2 = ck.para_k2, H0 = ck.H0;
       const double e1 = mu0 + mu1 * pc / (pc + kmu), e2 = pb + pbb * uc / (pk1 + uc);
       const double dRdu = kflux * e1 * (pbb * pk1 / ((pk1 + uc) * (pk1 + uc))) * hc - gam * kg / ((kg + uc) * (kg + uc));
       const double dRdh = kflux * e1 * e2;
@@ -420,7 +426,8 @@ __device__ __forceinline__ void rd_grid_part(const Tis& t, const RdTis& r, int b
   double* dead = r.lA[qa == 0 ? 2 : qa - 1];
   const uint8_t* m = t.cmap2;
   const int NX = t.NX, NY = t.NY, NZ = t.NZ, izb = r.izb;
-  const double cD = DT_Ca * Da * t.inv_dx * t.inv_dx;
+  const double DT_Ca = t.prm.DT_Ca, Kaa = t.prm.Kaa;
+  const double cD = DT_Ca * t.prm.Da * t.inv_dx * t.inv_dx;
   const long long stride = (long long)nblk * blockDim.x;
   for (long long v = (long long)bid * blockDim.x + threadIdx.x; v < r.box; v += stride) {
     dead[v] = 0;

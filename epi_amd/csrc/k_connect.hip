@@ -106,7 +106,8 @@ __device__ __forceinline__ bool sweep_hit(const Tis& t, const P3& pc, double rc,
   const double diffy = p_diff_y(t, pc.y, po.y);
   const double diffz = pc.z - po.z;
   const double rad_sum = rc + po.w;
-  return diffx * diffx + diffy * diffy + diffz * diffz <= LJ_THRESH * LJ_THRESH * rad_sum * rad_sum;
+  const double lj = t.prm.LJ_THRESH;
+  return diffx * diffx + diffy * diffy + diffz * diffz <= lj * lj * rad_sum * rad_sum;
 }
 
 __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb, int nb) {
@@ -226,6 +227,7 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
   __shared__ double s_og[K_GROUPS][KCAP];
   const int lane = threadIdx.x & (KG - 1);
   const int grp = threadIdx.x / KG;
+  const double gj_init = t.prm.gj_init;   // read before any store (a store could alias the descriptor)
   int mylen = 0;
   const int stride = gridDim.x * K_GROUPS;
   for (int c = blockIdx.x * K_GROUPS + grp; c < n; c += stride) {

@@ -8,6 +8,7 @@
 // by the autotuner); the number of active planes (iz_bound) is derived on the
 // device from zzmax and the z chunks are grid-strided, so no host round trip
 // is needed.
+#include <algorithm>
 #include <type_traits>
 
 #include "epi_dev.h"
@@ -56,6 +57,7 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
   Status* st = t.st;
   const int n = st->n;
   const int zl = threadIdx.x & (MS - 1);
+  const double FAC_MAP = t.prm.FAC_MAP;
   int ztop = 0;
   for (int c = (blockIdx.x * blockDim.x + threadIdx.x) / MS; c < n; c += gridDim.x * blockDim.x / MS) {
     const P3 p = ld_pos(t, pb, c);
@@ -128,9 +130,12 @@ __global__ void __launch_bounds__(256) k_map_halo(const Tis* __restrict__ T) {
 }
 
 // ---- ext_stim B: calc_ext_stim (ext_stim.cpp:18-69) ------------------------
-__device__ __forceinline__ double fB(double age, double B, bool cornif) {   // :9-16
-  const double kb = 0.025, DUR_ALIVE = 0.5, DUR_DEAD = 2.0;
-  return (cornif && age > THRESH_DEAD - DUR_ALIVE && age <= THRESH_DEAD + DUR_DEAD ? 1 : 0) - kb * B;
+struct BK {   // fB's constants, copied out of the record before any store (see CaK)
+  double THRESH_DEAD, DUR_ALIVE, DUR_DEAD, kb;
+};
+__device__ __forceinline__ BK b_k(const Params& pm) { return BK{pm.THRESH_DEAD, pm.DUR_ALIVE, pm.DUR_DEAD, pm.kb}; }
+__device__ __forceinline__ double fB(const BK& k, double age, double B, bool cornif) {   // :9-16
+  return (cornif && age > k.THRESH_DEAD - k.DUR_ALIVE && age <= k.THRESH_DEAD + k.DUR_DEAD ? 1 : 0) - k.kb * B;
 }
 
 // ---- flattened work lists ------------------------------------------------
@@ -167,9 +172,10 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
 }
 
 // The B stencil's list, once zzmax is on the device: per tissue
-// iz_bound = (int)((zzmax + 2.8) * inv_dz) (ext_stim.cpp:22) into izb[i], and
-// ceil(NX NY / bs) ceil(iz_bound / zb) tiles.  A tissue whose iz_bound lies
-// outside [0, NZ] gets none and reports ERR_OUT_OF_GRID.  One block.
+// iz_bound = (int)((zzmax + FAC_MAP * R_max) * inv_dz) (ext_stim.cpp:22) into
+// izb[i], and ceil(NX NY / bs) ceil(iz_bound / zb) tiles.  A tissue whose
+// iz_bound lies outside [0, NZ] gets none and reports ERR_OUT_OF_GRID.  The
+// sweep's NX NY iz_bound points are added to the tissue's count.  One block.
 __global__ void __launch_bounds__(256) k_wl_b(const Tis* __restrict__ T, int ntis, int bs, int zb,
                                               int* __restrict__ off, int* __restrict__ izb_out) {
   int carry = 0;
@@ -179,12 +185,13 @@ __global__ void __launch_bounds__(256) k_wl_b(const Tis* __restrict__ T, int nti
     if (i < ntis) {
       const Tis& t = T[i];
       const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
-      int izb = (int)((zzmax + FAC_MAP * R_max) * t.inv_dz);
+      int izb = (int)((zzmax + t.prm.iz_margin) * t.inv_dz);
       if (izb > t.NZ || izb < 0) {
         set_err(t.st, ERR_OUT_OF_GRID);
         izb = 0;
       }
       izb_out[i] = izb;
+      t.st->grid_b += (long long)t.NX * t.NY * izb;
       cnt = ((t.NX * t.NY + bs - 1) / bs) * ((izb + zb - 1) / zb);
     }
     int tot;
@@ -240,6 +247,8 @@ constexpr int WL_LDS_T = 1024;   // tissues whose list a block stages in LDS
 template <int ZB>
 __device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int l1, int bp) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const BK bk = b_k(t.prm);
+  const double DB = t.prm.DB, DT_Ca = t.prm.DT_Ca;
   const double* __restrict__ carr = t.bf[bp];
   double* __restrict__ narr = t.bf[1 - bp];
   const uint8_t* __restrict__ cm2 = t.cmap2;
@@ -284,7 +293,6 @@ __device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int
   double age[ZB];
 #pragma unroll
   for (int q = 0; q < ZB; q++) age[q] = (cs[q] == DEAD || cs[q] == ALIVE) ? t.agek[cell[q]] : 0.0;
-  const double DB = 0.0009;
   const double inv_dz = t.inv_dz;
 #pragma unroll
   for (int q = 0; q < ZB; q++) {
@@ -299,7 +307,7 @@ __device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int
            + (double)mz[q + 2] * (cz[q + 2] - cext)
            + (double)myp[q] * (yp[q] - cext)
            + (double)mxp[q] * (xp[q] - cext))
-          * inv_dz * inv_dz + fB(age[q], cext, flg));
+          * inv_dz * inv_dz + fB(bk, age[q], cext, flg));
       narr[cxy + l * PL] = nv;
       // periodic copies (:63-66)
       if (k == 0) narr[gidx(t, j, NY, l)] = nv;
@@ -309,8 +317,63 @@ __device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int
   }
 }
 
+// The same column with the z column rolling through registers, one plane
+// after the other: three values and masks live at a time, so a tile of many
+// planes keeps the registers (and occupancy) of one.  Large tiles, where the
+// sweep is bandwidth- and not latency-bound, take this form.
+__device__ __forceinline__ void b_column_roll(const Tis& t, int j, int k, int l0, int l1, int bp) {
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const BK bk = b_k(t.prm);
+  const double DB = t.prm.DB, DT_Ca = t.prm.DT_Ca;
+  const double* __restrict__ carr = t.bf[bp];
+  double* __restrict__ narr = t.bf[1 - bp];
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  const double inv_dz = t.inv_dz;
+  const long long PL = t.PL;
+  const long long cxy = gidx(t, j, k, 0);
+  const int pz = l0 == 0 ? 1 : l0 - 1;   // mirror z (utils.cpp:32-72)
+  double czm = carr[cxy + pz * PL];
+  double mzm = (double)cm2[cxy + pz * PL];
+  double cc = carr[cxy + l0 * PL];
+  double mc = (double)cm2[cxy + l0 * PL];
+  for (int l = l0; l < l1; l++) {
+    const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+    const long long v = cxy + l * PL;
+    const long long vzp = cxy + next_z * PL;
+    const double czp = carr[vzp];
+    const double mzp = (double)cm2[vzp];
+    double dum_age = 0;
+    bool flg = false;
+    const int cell = t.cmap1[v];
+    if (cell >= 0) {
+      const uint8_t s = t.state[cell];
+      if (s == DEAD || s == ALIVE) { dum_age = t.agek[cell]; flg = true; }
+    }
+    const double cext = cc;
+    const long long vxm = gidx(t, prev_x, k, l), vym = gidx(t, j, prev_y, l);
+    const long long vyp = gidx(t, j, next_y, l), vxp = gidx(t, next_x, k, l);
+    const double nv = cext + DT_Ca * (DB *
+        ((double)cm2[vxm] * (carr[vxm] - cext)
+         + (double)cm2[vym] * (carr[vym] - cext)
+         + mzm * (czm - cext)
+         + mzp * (czp - cext)
+         + (double)cm2[vyp] * (carr[vyp] - cext)
+         + (double)cm2[vxp] * (carr[vxp] - cext))
+        * inv_dz * inv_dz + fB(bk, dum_age, cext, flg));
+    narr[v] = nv;
+    if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+    if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+    if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+    czm = cc; mzm = mc;
+    cc = czp; mc = mzp;
+  }
+}
+
 // The B stencil over its tile list.
-// One thread per (x, y) point of a tile (b_column).
+// One thread per (x, y) point of a tile: tiles of up to 2 planes load the
+// column first (b_column), taller ones roll it (b_column_roll).
 template <int ZB>
 __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, const int* __restrict__ woff,
                                                   const int* __restrict__ wizb, int ntis, int bp, int xcd) {
@@ -340,7 +403,8 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, con
     const int j = p % NX, k = p / NX;
     const int l0 = (local / npt) * ZB;
     const int l1 = min(l0 + ZB, iz_bound);
-    b_column<ZB>(t, j, k, l0, l1, bp);
+    if (ZB >= 4) b_column_roll(t, j, k, l0, l1, bp);
+    else b_column<ZB>(t, j, k, l0, l1, bp);
   }
 }
 
@@ -377,14 +441,18 @@ __global__ void __launch_bounds__(256) k_ca_begin(const Tis* __restrict__ T, con
       t.ca_cn[c] = k;
     }
     if (c >= start && is_afm(s)) {
-      if (zero_avg) t.ca2p_avg[c] = 0;
       // calc_th, calc_Kpa, calc_IAG (ca2p.cpp:90-126): agek and state do
       // not change during a solve
       const bool is_alive = s == ALIVE;
       const double agek = t.agek[c];
-      t.ca_th[c] = is_alive ? 0.2 + ((1.0 - 0.2) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 1.0;
-      t.ca_kpa[c] = is_alive ? 4.0 + ((6.0 - 4.0) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / 1.0)) : 6.0;
-      t.ca_iag[c] = is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / 1.5)) : 0.0;
+      // the constants are read before this cell's stores (see CaK)
+      const Params& pm = t.prm;
+      const double thgra = pm.thgra, thpri = pm.thpri, delta_th = pm.delta_th, TSP = pm.THRESH_SP;
+      const double kpa = pm.kpa, Kpri = pm.Kpri, delta_K = pm.delta_K, delta_I = pm.delta_I, ik = pm.iage_kitei;
+      if (zero_avg) t.ca2p_avg[c] = 0;
+      t.ca_th[c] = is_alive ? thgra + ((thpri - thgra) * 0.5) * (1.0 + tanh((TSP - agek) / delta_th)) : thpri;
+      t.ca_kpa[c] = is_alive ? kpa + ((Kpri - kpa) * 0.5) * (1.0 + tanh((TSP - agek) / delta_K)) : Kpri;
+      t.ca_iag[c] = is_alive ? 0.5 * (1.0 + tanh((agek - TSP) / delta_I)) : ik;
     }
   }
 }
@@ -424,14 +492,13 @@ __device__ __forceinline__ unsigned xcd_order(unsigned L, unsigned G, int on) {
 // ATP_refresh's update of one point (ca2p.cpp:302-338) from its value, its
 // six neighbours (mirror z, periodic x and y) and their masks.  Every path
 // that advances an ATP point goes through this one expression.
-__device__ __forceinline__ double atp_new(double catp, double cxm, double cxp, double cym, double cyp, double czm,
-                                          double czp, double mxm, double mxp, double mym, double myp, double mzm,
-                                          double mzp, double diffu, double air, double inv_dx) {
-  const double Da = 1.0, AIR_STIM = 0.1, STIM11 = 0.002, Kaa = 0.5;
-  const double fa = STIM11 * min0(diffu) - catp * Kaa;
-  return catp + DT_Ca * (Da * (mxm * (cxm - catp) + mxp * (cxp - catp) + mym * (cym - catp) + myp * (cyp - catp)
-                               + mzm * (czm - catp) + mzp * (czp - catp))
-                         * inv_dx * inv_dx + fa + air * AIR_STIM);
+__device__ __forceinline__ double atp_new(const AtpK& k, double catp, double cxm, double cxp, double cym,
+                                          double cyp, double czm, double czp, double mxm, double mxp, double mym,
+                                          double myp, double mzm, double mzp, double diffu, double air, double inv_dx) {
+  const double fa = k.STIM11 * min0(diffu) - catp * k.Kaa;
+  return catp + k.DT_Ca * (k.Da * (mxm * (cxm - catp) + mxp * (cxp - catp) + mym * (cym - catp) + myp * (cyp - catp)
+                                   + mzm * (czm - catp) + mzp * (czp - catp))
+                           * inv_dx * inv_dx + fa + air * k.AIR_STIM);
 }
 
 // diff_u of a sub-step by Ca parity: the sub-step at parity cp writes it and
@@ -480,7 +547,7 @@ __global__ void __launch_bounds__(256) k_ca_corners(const Tis* __restrict__ T, c
 // and the sub-step's diff_u, with the point's voxel word and neighbour masks
 // from k_ca_corners.  The seven ATP loads do not depend on them; a point whose
 // top bit is set (a mask byte above 1) reads the word and masks itself.
-__device__ __forceinline__ double atp_point_pre(const Tis& t, const double* __restrict__ carr,
+__device__ __forceinline__ double atp_point_pre(const Tis& t, const AtpK& ak, const double* __restrict__ carr,
                                                 const double* __restrict__ dfu, int j, int k, int l, uint2 pre) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
   const int px = j == 0 ? NX - 1 : j - 1, nx = j == NX - 1 ? 0 : j + 1;
@@ -505,7 +572,7 @@ __device__ __forceinline__ double atp_point_pre(const Tis& t, const double* __re
   }
   const unsigned cell1 = w & 0x7fffffffu;
   const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
-  return atp_new(c0, cxm, cxp, cym, cyp, czm, czp, m[0], m[1], m[2], m[3], m[4], m[5], diffu, (double)(w >> 31),
+  return atp_new(ak, c0, cxm, cxp, cym, cyp, czm, czp, m[0], m[1], m[2], m[3], m[4], m[5], diffu, (double)(w >> 31),
                  t.inv_dx);
 }
 
@@ -549,6 +616,8 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
   const int start = t.nmemb + t.nder;
   const int c = start + bx * C_GROUPS + grp;
   if (c >= t.cap) return;
+  const CaK ck = ca_k(t.prm);
+  const AtpK ak = atp_k(t.prm);
   {
     // The cell's own words are loaded before the cell count and iz_bound
     // (device values) arrive, in one round trip with them: a cell past the
@@ -623,7 +692,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           sv->su[slot * (size_t)t.cap + c] = u;
           sv->sp[slot * (size_t)t.cap + c] = p;
         }
-        const double np = p + DT_Ca * (t.ca_dp * (tmp - count * p) - Kpp * p);
+        const double np = p + ck.DT_Ca * (t.ca_dp * (tmp - count * p) - ck.Kpp * p);
         t.ip3[1 - cp][c] = np;
         reinterpret_cast<double*>(up_next + c)[1] = np;   // ca2p of a DEAD cell stays as it is
       }
@@ -644,7 +713,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
           cq[j] = t.bf[bp][gidx(t, ix, iy, iz)];
         } else if (recompute && iz < izb) {
           // the periodic copies x = NX, y = NY hold the values of x = 0, y = 0
-          cq[j] = atp_point_pre(t, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz, pre[j]);
+          cq[j] = atp_point_pre(t, ak, carr_prev, dfu_prev, ix == t.NX ? 0 : ix, iy == t.NY ? 0 : iy, iz, pre[j]);
         } else if (recompute) {
           // plane iz_bound is never updated inside a solve; its copies are
           // written beside this launch, so read the value they copy
@@ -682,7 +751,7 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
             if (sv) sv->sg[(slot * (size_t)t.cap + c) * t.ns + sl[i]] = gv[i];
             a1 = gv[i] * (w[i].x - u);
             a2 = gv[i] * (w[i].y - p);
-            if (kd == 3) g[sl[i]] = gv[i] + DT_Ca * fw_term(fabs(w[i].x - u), gv[i]);
+            if (kd == 3) g[sl[i]] = gv[i] + ck.DT_Ca * fw_term(ck.wd, fabs(w[i].x - u), gv[i]);
           }
           s_a[grp][e] = make_double2(a1, a2);
         }
@@ -721,16 +790,16 @@ __device__ __forceinline__ void ca_cells_part(const Tis& t, unsigned bx, unsigne
         sv->sh[k] = exi;
         sv->sAv[k] = Aavg;
       }
-      const double diff_u = ca2p_reaction_factor(u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
+      const double diff_u = ca2p_reaction_factor(ck, u, exi, p, Bavg) + ca2p_du * IAGv * tmp_diffu;
       dfu_out[c] = diff_u;
-      const double H0 = 0.5;
-      const double np = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - Kpp * p) + t.ca_dp * IAGv * tmp_IP3);
+      const double H0 = ck.H0, DT_Ca = ck.DT_Ca;
+      const double np = p + DT_Ca * ((_Kpa * Aavg / (H0 + Aavg) - ck.Kpp * p) + t.ca_dp * IAGv * tmp_IP3);
       t.ip3[1 - cp][c] = np;
       const double cs = u + DT_Ca * diff_u;
       t.ca2p[1 - cp][c] = cs;
       up_next[c] = make_double2(cs, np);
       t.ca2p_avg[c] = avg + cs;
-      const double para_k2 = 0.7;
+      const double para_k2 = ck.para_k2;
       t.ex_inert[c] = exi + DT_Ca * ((para_k2 * para_k2 / (para_k2 * para_k2 + u * u) - exi) / _th);
     }
     wave_sync();
@@ -757,6 +826,7 @@ __global__ void __launch_bounds__(C_BLOCK) k_ca_cells(const Tis* __restrict__ T,
 template <int ZB>
 __device__ __forceinline__ void atp_column(const Tis& t, int j, int k, int l0, int l1, int ap, int cp) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const AtpK ak = atp_k(t.prm);
   const double* __restrict__ carr = t.atp[ap];
   double* __restrict__ narr = t.atp[1 - ap];
   const double* __restrict__ dfu = dfu_buf(t, cp);
@@ -807,7 +877,7 @@ __device__ __forceinline__ void atp_column(const Tis& t, int j, int k, int l0, i
   for (int q = 0; q < ZB; q++) {
     if (q < nl) {
       const int l = l0 + q;
-      const double nv = atp_new(cz[q + 1], xm[q], xp[q], ym[q], yp[q], cz[q], cz[q + 2], (double)mxm[q],
+      const double nv = atp_new(ak, cz[q + 1], xm[q], xp[q], ym[q], yp[q], cz[q], cz[q + 2], (double)mxm[q],
                                 (double)mxp[q], (double)mym[q], (double)myp[q], (double)mz[q], (double)mz[q + 2],
                                 diffu[q], (double)(w[q] >> 31), inv_dx);
       narr[cxy + l * PL] = nv;
@@ -818,9 +888,51 @@ __device__ __forceinline__ void atp_column(const Tis& t, int j, int k, int l0, i
   }
 }
 
+// The same column rolling through registers plane by plane (as
+// b_column_roll), for tall tiles.
+__device__ __forceinline__ void atp_column_roll(const Tis& t, int j, int k, int l0, int l1, int ap, int cp) {
+  const int NX = t.NX, NY = t.NY, NZ = t.NZ;
+  const AtpK ak = atp_k(t.prm);
+  const double* __restrict__ carr = t.atp[ap];
+  double* __restrict__ narr = t.atp[1 - ap];
+  const double* __restrict__ dfu = dfu_buf(t, cp);
+  const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
+  const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
+  const double inv_dx = t.inv_dx;
+  const long long PL = t.PL;
+  const long long cxy = gidx(t, j, k, 0);
+  const int pz = l0 == 0 ? 1 : l0 - 1;   // mirror z (utils.cpp:32-72)
+  double czm = carr[cxy + pz * PL];
+  double mzm = (double)cm2[cxy + pz * PL];
+  double cc = carr[cxy + l0 * PL];
+  double mc = (double)cm2[cxy + l0 * PL];
+  for (int l = l0; l < l1; l++) {
+    const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
+    const long long v = cxy + l * PL;
+    const long long vzp = cxy + next_z * PL;
+    const double czp = carr[vzp];
+    const double mzp = (double)cm2[vzp];
+    const unsigned w = t.vox[v];
+    const unsigned cell1 = w & 0x7fffffffu;
+    const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
+    const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
+    const long long vym = gidx(t, j, prev_y, l), vyp = gidx(t, j, next_y, l);
+    const double nv = atp_new(ak, cc, carr[vxm], carr[vxp], carr[vym], carr[vyp], czm, czp, (double)cm2[vxm],
+                              (double)cm2[vxp], (double)cm2[vym], (double)cm2[vyp], mzm, mzp, diffu,
+                              (double)(w >> 31), inv_dx);
+    narr[v] = nv;
+    if (k == 0) narr[gidx(t, j, NY, l)] = nv;
+    if (j == 0) narr[gidx(t, NX, k, l)] = nv;
+    if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
+    czm = cc; mzm = mc;
+    cc = czp; mc = mzp;
+  }
+}
+
 // ATP_refresh (:302-338) with the periodic copies of planes 0..iz_bound,
 // one tile of a solve's list: blockDim.x consecutive points of the (x, y)
-// plane times ZB planes (atp_column).
+// plane times ZB planes (atp_column up to 2 planes, atp_column_roll above).
 template <int ZB>
 __device__ __forceinline__ void atp_tile(const Tis& t, int local, int izb, int ap, int cp) {
   const int NX = t.NX, NY = t.NY;
@@ -831,7 +943,8 @@ __device__ __forceinline__ void atp_tile(const Tis& t, int local, int izb, int a
   if (p >= plane) return;
   const int j = p % NX, k = p / NX;
   const int l0 = (local / npt) * ZB;
-  atp_column<ZB>(t, j, k, l0, min(l0 + ZB, izb), ap, cp);
+  if (ZB >= 4) atp_column_roll(t, j, k, l0, min(l0 + ZB, izb), ap, cp);
+  else atp_column<ZB>(t, j, k, l0, min(l0 + ZB, izb), ap, cp);
   // plane l = iz_bound: copies of values left in the next buffer (:334-337);
   // the corner (NX, NY) ends as a copy of (0, 0)
   if (l0 == 0) {
@@ -900,10 +1013,11 @@ __global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T,
 __global__ void __launch_bounds__(256) k_ca_end(const Tis* __restrict__ T, const int* tlist, int full) {
   const Tis& t = T[tlist ? tlist[blockIdx.y] : blockIdx.y];
   const int n = t.st->n;
+  const int Ca_ITR = t.prm.Ca_ITR;
   for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     if (!full) {
     } else if (is_afm(t.state[c])) {
-      t.ca2p_avg[c] = t.ca2p_avg[c] / CA_ITR;
+      t.ca2p_avg[c] = t.ca2p_avg[c] / Ca_ITR;
     } else {
       t.ca2p_avg[c] = 0;
       t.ca2p[0][c] = 0;
@@ -913,9 +1027,37 @@ __global__ void __launch_bounds__(256) k_ca_end(const Tis* __restrict__ T, const
   }
 }
 
+__global__ void __launch_bounds__(256) k_ca_swap(const Tis* __restrict__ T, const int* __restrict__ tlist) {
+  const Tis& t = T[tlist[blockIdx.y]];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long i = i0; i < t.gsize; i += stride) {
+    const double a = t.atp[0][i];
+    t.atp[0][i] = t.atp[1][i];
+    t.atp[1][i] = a;
+  }
+  const int n = t.st->n;
+  for (long long c = i0; c < n; c += stride) {
+    const double u = t.ca2p[0][c], p = t.ip3[0][c];
+    const double2 w = t.ca_up[0][c];
+    t.ca2p[0][c] = t.ca2p[1][c];
+    t.ca2p[1][c] = u;
+    t.ip3[0][c] = t.ip3[1][c];
+    t.ip3[1][c] = p;
+    t.ca_up[0][c] = t.ca_up[1][c];
+    t.ca_up[1][c] = w;
+  }
+}
+
 __global__ void k_sw_reset(const Tis* __restrict__ T, const int* tlist) {
   const Tis& t = T[tlist ? tlist[blockIdx.x] : blockIdx.x];
   if (threadIdx.x == 0) t.st->sw = 0;
+}
+
+void launch_ca_swap(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n, long long max_gsize) {
+  const long long items = std::max<long long>(max_gsize, max_n);
+  const long long g = std::min<long long>((items + 255) / 256, 4096);
+  EPI_LAUNCH(K_CA_SWAP, k_ca_swap, dim3((unsigned)g, ntis), dim3(256), 0, s, dT, tlist);
 }
 
 void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo,

@@ -74,9 +74,9 @@ __device__ __forceinline__ double adhesion(const Tis& t, const CV& a, const CV& 
   const double distlj = sqrt(dist_sq(t, a.p, b.p));
   const double rad_sum = a.r + b.r;
   const double LJ2_m1 = (distlj / rad_sum) - 1;
-  return (LJ2_m1 + 1 > LJ_THRESH ? 0.0
+  return (LJ2_m1 + 1 > t.prm.LJ_THRESH ? 0.0
           : -(spring_const / distlj) * LJ2_m1 *
-                (1 - LJ2_m1 * LJ2_m1 / ((LJ_THRESH - 1.0) * (LJ_THRESH - 1.0))));
+                (1 - LJ2_m1 * LJ2_m1 / ((t.prm.LJ_THRESH - 1.0) * (t.prm.LJ_THRESH - 1.0))));
 }
 
 // Coefficient of law `law` with c1 = a, c2 = b (cell_interaction.cpp:77-285).
@@ -86,36 +86,36 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
       return ljmain(t, a, b);
     case LAW_DER_DER: {
       const double dist = sqrt(dist_sq(t, a.p, b.p));
-      if (dist + delta_R < a.r + b.r) {
-        const double dist2 = dist + delta_R;
+      if (dist + t.prm.delta_R < a.r + b.r) {
+        const double dist2 = dist + t.prm.delta_R;
         const double rad_sum = a.r + b.r;
         const double LJ6 = POW6(rad_sum) / POW6(dist2);
-        return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / (dist * dist2) + para_ljp2;
+        return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / (dist * dist2) + t.prm.para_ljp2;
       } else if (is_near(t, a, b)) {
-        return para_ljp2;
+        return t.prm.para_ljp2;
       } else {
-        const double dist2 = dist + delta_R;
+        const double dist2 = dist + t.prm.delta_R;
         const double rad_sum = a.r + b.r;
         const double LJ6 = POW6(rad_sum) / POW6(dist2);
-        return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / (dist * dist) + para_ljp2;
+        return 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / (dist * dist) + t.prm.para_ljp2;
       }
     }
     case LAW_ALAIRDE:
       if (is_near(t, a, b)) return ljmain(t, a, b);
       else {
-        double spf = (t.m_kadh * K_DESMOSOME_RATIO);
-        if (a.agek > THRESH_SP && b.agek > THRESH_SP) spf = t.m_kadh;
+        double spf = (t.m_kadh * t.prm.K_DESMOSOME_RATIO);
+        if (a.agek > t.prm.THRESH_SP && b.agek > t.prm.THRESH_SP) spf = t.m_kadh;
         return adhesion(t, a, b, spf);
       }
     case LAW_FIXMU:
       if (is_near(t, a, b)) return ljmain(t, a, b);
-      return adhesion(t, a, b, (t.m_kadh * K_DESMOSOME_RATIO));
+      return adhesion(t, a, b, (t.m_kadh * t.prm.K_DESMOSOME_RATIO));
     case LAW_MU_MEMB: {
       if (is_near(t, a, b)) return ljmain(t, a, b);
       // spring_force_to_memb, set at the top of _MUSUME_interaction (:613-620)
       double spring = 0;
       if (a.pair >= 0 && t.state[a.pair] == FIX) spring = t.m_kspring;
-      else if (a.rdt > 0) spring = Kspring_d;
+      else if (a.rdt > 0) spring = t.prm.Kspring_d;
       return adhesion(t, a, b, spring);
     }
     case LAW_FIX_MEMB:
@@ -131,19 +131,19 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
       double cr_dist_sq;
       const double distSq = dist_sq(t, a.p, b.p);
       double tmp;
-      if (distSq < (cr_dist_sq = rad_sum_sq * P_MEMB * P_MEMB)) {
+      if (distSq < (cr_dist_sq = rad_sum_sq * t.prm.P_MEMB * t.prm.P_MEMB)) {
         const double LJ6 = POW3(cr_dist_sq) / POW3(distSq);
         tmp = 4.0 * t.m_eps * LJ6 * (LJ6 - 1) / distSq;
       } else if (distSq < rad_sum_sq) {
         const double distlj = sqrt(distSq);
-        const double cr_dist = rad_sum * P_MEMB;
-        tmp = -(DER_DER_CONST / distlj) * (distlj / cr_dist - 1.0);
+        const double cr_dist = rad_sum * t.prm.P_MEMB;
+        tmp = -(t.prm.DER_DER_CONST / distlj) * (distlj / cr_dist - 1.0);
       } else {
         const double distlj = sqrt(distSq);
-        const double lambda_dist = (1.0 + P_MEMB) * rad_sum;
-        const double cr_dist = rad_sum * P_MEMB;
+        const double lambda_dist = (1.0 + t.prm.P_MEMB) * rad_sum;
+        const double cr_dist = rad_sum * t.prm.P_MEMB;
         const double LJ6 = POW6(cr_dist) / POW6(lambda_dist - distlj);
-        tmp = -(DER_DER_CONST / rad_sum) * ((1.0 - P_MEMB) / P_MEMB)
+        tmp = -(t.prm.DER_DER_CONST / rad_sum) * ((1.0 - t.prm.P_MEMB) / t.prm.P_MEMB)
               - 4 * t.m_eps * (LJ6 * (LJ6 - 1.0)) / ((lambda_dist - distlj) * distlj);
       }
       if (fabs(tmp) > 100) set_err(st, ERR_STRONG_INTERACTION);
@@ -151,7 +151,7 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
     }
     case LAW_PAIR: {
       const double dist = sqrt(dist_sq(t, a.p, b.p));
-      const double force = Kspring_division * (dist - a.snl);
+      const double force = t.prm.Kspring_division * (dist - a.snl);
       return -force / dist;
     }
     default:
@@ -163,7 +163,7 @@ __device__ double coef_law(const Tis& t, int law, const CV& a, const CV& b, Stat
 __device__ __forceinline__ P3 delta(const Tis& t, int law, const CV& a, const CV& b, Status* st) {
   const double coef = coef_law(t, law, a, b, st);
   if (fabs(coef) > 1000) set_err(st, ERR_STRONG_INTERACTION);
-  const double d = DT_Cell * coef;
+  const double d = t.prm.DT_Cell * coef;
   return P3{d * p_diff_x(t, a.p.x, b.p.x), d * p_diff_y(t, a.p.y, b.p.y), d * (a.p.z - b.p.z)};
 }
 
@@ -172,7 +172,7 @@ __device__ __forceinline__ double wall_dz(const Tis& t, const CV& c) {
   const double distlj = 2.0 * c.p.z;
   const double LJ6 = POW6(c.r) / POW6(c.p.z);
   const double ljm = 4.0 * t.m_eps * LJ6 * (LJ6 - 1.0) / (distlj * distlj);
-  return DT_Cell * ljm * 2.0 * c.p.z;
+  return t.prm.DT_Cell * ljm * 2.0 * c.p.z;
 }
 
 // ---- triangular bending (:451-499) ---------------------------------------
@@ -235,9 +235,9 @@ __device__ void forces_serial(const Tis& t, int pb, int cb, int X, Status* st) {
     for (int i = 0; i < 6; i++)
       dr = v_add(dr, tri_bend_2(t, cx.p, ld_pos(t, pb, md[i]), ld_pos(t, pb, md[(i + 1) % 6]),
                                 ld_pos(t, pb, md[(i + 2) % 6])));
-    acc.x += DT_Cell * t.m_kbend * dr.x;
-    acc.y += DT_Cell * t.m_kbend * dr.y;
-    acc.z += DT_Cell * t.m_kbend * dr.z;
+    acc.x += t.prm.DT_Cell * t.m_kbend * dr.x;
+    acc.y += t.prm.DT_Cell * t.m_kbend * dr.y;
+    acc.z += t.prm.DT_Cell * t.m_kbend * dr.z;
     if (cx.p.z < cx.r) acc.z += wall_dz(t, cx);
     int hi[6], nhi = 0;
     for (int i = 0; i < 6; i++) {
@@ -445,9 +445,9 @@ __global__ void __launch_bounds__(F_BLOCK) k_forces(const Tis* __restrict__ T, i
   if (memb) {
     P3 dr = P3{0, 0, 0};
     for (int i = 0; i < 12; i++) dr = v_add(dr, s_bend[grp][i]);
-    acc.x += DT_Cell * t.m_kbend * dr.x;
-    acc.y += DT_Cell * t.m_kbend * dr.y;
-    acc.z += DT_Cell * t.m_kbend * dr.z;
+    acc.x += t.prm.DT_Cell * t.m_kbend * dr.x;
+    acc.y += t.prm.DT_Cell * t.m_kbend * dr.y;
+    acc.z += t.prm.DT_Cell * t.m_kbend * dr.z;
     if (cx.p.z < cx.r) acc.z += wall_dz(t, cx);
     int hi[6], nhi = 0;
     for (int e = 0; e < MEMB_ADJ; e++) {
@@ -584,12 +584,12 @@ __device__ LawD ljmain_d(double s, double R, double eps) {
   const double L = POW6(R) / POW3(s);
   return LawD{4.0 * eps * L * (L - 1) / s, 4.0 * eps * (4.0 * L - 7.0 * L * L) / (s * s), 4.0 * L * (L - 1) / s, 0, 0};
 }
-__device__ LawD adhesion_d(double s, double R, double K) {
+__device__ LawD adhesion_d(double s, double R, double K, double lj) {
   LawD r{0, 0, 0, 0, 0};
   const double d = sqrt(s);
   const double x = d / R - 1;
-  if (x + 1 > LJ_THRESH) return r;
-  const double c2 = (LJ_THRESH - 1.0) * (LJ_THRESH - 1.0);
+  if (x + 1 > lj) return r;
+  const double c2 = (lj - 1.0) * (lj - 1.0);
   const double g = x * (1 - x * x / c2);
   r.coef = -(K / d) * g;
   const double dg = (1 - 3 * x * x / c2) / R;
@@ -608,7 +608,7 @@ __device__ LawD coef_law_d(const Tis& t, int law, const CV& a, const CV& b) {
     case LAW_OTHER:
       return ljmain_d(s, R, eps);
     case LAW_DER_DER: {
-      const double d = sqrt(s), D = d + delta_R;
+      const double d = sqrt(s), D = d + t.prm.delta_R;
       const double L = POW6(R) / POW6(D);
       const double dL = -6.0 * L / D;
       double g, dgd;
@@ -621,26 +621,26 @@ __device__ LawD coef_law_d(const Tis& t, int law, const CV& a, const CV& b) {
         g = L * (L - 1) / (d * d);
         dgd = (2 * L - 1) * dL / (d * d) - 2 * L * (L - 1) / (d * d * d);
       }
-      return LawD{4.0 * eps * g + para_ljp2, 4.0 * eps * dgd / (2 * d), 4.0 * g, 0, 0};
+      return LawD{4.0 * eps * g + t.prm.para_ljp2, 4.0 * eps * dgd / (2 * d), 4.0 * g, 0, 0};
     }
     case LAW_ALAIRDE:
       if (s < R * R) return ljmain_d(s, R, eps);
       {
-        const bool strong = a.agek > THRESH_SP && b.agek > THRESH_SP;
-        r = adhesion_d(s, R, strong ? t.m_kadh : t.m_kadh * K_DESMOSOME_RATIO);
+        const bool strong = a.agek > t.prm.THRESH_SP && b.agek > t.prm.THRESH_SP;
+        r = adhesion_d(s, R, strong ? t.m_kadh : t.m_kadh * t.prm.K_DESMOSOME_RATIO, t.prm.LJ_THRESH);
         r.dadh = r.coef / t.m_kadh;
         return r;
       }
     case LAW_FIXMU:
       if (s < R * R) return ljmain_d(s, R, eps);
-      r = adhesion_d(s, R, t.m_kadh * K_DESMOSOME_RATIO);
+      r = adhesion_d(s, R, t.m_kadh * t.prm.K_DESMOSOME_RATIO, t.prm.LJ_THRESH);
       r.dadh = r.coef / t.m_kadh;
       return r;
     case LAW_MU_MEMB: {
       if (s < R * R) return ljmain_d(s, R, eps);
       const bool fixpair = a.pair >= 0 && t.state[a.pair] == FIX;
-      const double spring = fixpair ? t.m_kspring : (a.rdt > 0 ? Kspring_d : 0.0);
-      r = adhesion_d(s, R, spring);
+      const double spring = fixpair ? t.m_kspring : (a.rdt > 0 ? t.prm.Kspring_d : 0.0);
+      r = adhesion_d(s, R, spring, t.prm.LJ_THRESH);
       if (fixpair) r.dspr = r.coef / t.m_kspring;
       return r;
     }
@@ -652,29 +652,29 @@ __device__ LawD coef_law_d(const Tis& t, int law, const CV& a, const CV& b) {
                     -(1.0 / d) * (d / R - 1.0)};
       }
     case LAW_MEMB_MEMB: {
-      const double cr = R * P_MEMB, cr2 = cr * cr;
+      const double cr = R * t.prm.P_MEMB, cr2 = cr * cr;
       if (s < cr2) {
         const double L = POW3(cr2) / POW3(s);
         return LawD{4.0 * eps * L * (L - 1) / s, 4.0 * eps * (4.0 * L - 7.0 * L * L) / (s * s),
                     4.0 * L * (L - 1) / s, 0, 0};
       } else if (s < R * R) {
         const double d = sqrt(s);
-        return LawD{-(DER_DER_CONST / d) * (d / cr - 1.0), -DER_DER_CONST / (2 * d * d * d), 0, 0, 0};
+        return LawD{-(t.prm.DER_DER_CONST / d) * (d / cr - 1.0), -t.prm.DER_DER_CONST / (2 * d * d * d), 0, 0, 0};
       } else {
         const double d = sqrt(s);
-        const double lam = (1.0 + P_MEMB) * R;
+        const double lam = (1.0 + t.prm.P_MEMB) * R;
         const double m = lam - d;
         const double L = POW6(cr) / POW6(m);
         const double dL = 6.0 * L / m;
         const double q = L * (L - 1.0) / (m * d);
         const double dq = (2 * L - 1) * dL / (m * d) + L * (L - 1.0) * (d - m) / ((m * d) * (m * d));
-        return LawD{-(DER_DER_CONST / R) * ((1.0 - P_MEMB) / P_MEMB) - 4 * eps * q, -4 * eps * dq / (2 * d), -4 * q,
+        return LawD{-(t.prm.DER_DER_CONST / R) * ((1.0 - t.prm.P_MEMB) / t.prm.P_MEMB) - 4 * eps * q, -4 * eps * dq / (2 * d), -4 * q,
                     0, 0};
       }
     }
     case LAW_PAIR: {
       const double d = sqrt(s);
-      return LawD{-(Kspring_division * (d - a.snl)) / d, -Kspring_division * a.snl / (2 * d * d * d), 0, 0, 0};
+      return LawD{-(t.prm.Kspring_division * (d - a.snl)) / d, -t.prm.Kspring_division * a.snl / (2 * d * d * d), 0, 0, 0};
     }
     default:
       return r;
@@ -702,12 +702,12 @@ __device__ __forceinline__ P3 pair_bwd(const Tis& t, int law, const CV& a, int i
   const P3 av{la.x - lb.x, la.y - lb.y, la.z - lb.z};
   const double ar = v_dot(av, r);
   const double k2 = 2 * ld.ds * ar;
-  const P3 g{DT_Cell * (ld.coef * av.x + k2 * r.x), DT_Cell * (ld.coef * av.y + k2 * r.y),
-             DT_Cell * (ld.coef * av.z + k2 * r.z)};
+  const P3 g{t.prm.DT_Cell * (ld.coef * av.x + k2 * r.x), t.prm.DT_Cell * (ld.coef * av.y + k2 * r.y),
+             t.prm.DT_Cell * (ld.coef * av.z + k2 * r.z)};
   atomic_add3(nl, cap, ib, P3{-g.x, -g.y, -g.z});
-  pa.eps += DT_Cell * ld.deps * ar;
-  pa.adh += DT_Cell * ld.dadh * ar;
-  pa.spr += DT_Cell * ld.dspr * ar;
+  pa.eps += t.prm.DT_Cell * ld.deps * ar;
+  pa.adh += t.prm.DT_Cell * ld.dadh * ar;
+  pa.spr += t.prm.DT_Cell * ld.dspr * ar;
   return g;
 }
 
@@ -913,12 +913,12 @@ __global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T
     if (cx.p.z < cx.r) {
       const double z = cx.p.z;
       const double L = POW6(cx.r) / POW6(z);
-      atomicAdd(&nl[2 * cap + X], lam[2 * cap + X] * DT_Cell * 2.0 * t.m_eps * (7.0 * L - 13.0 * L * L) / (z * z));
-      peps += lam[2 * cap + X] * DT_Cell * 2.0 * L * (L - 1.0) / z;
+      atomicAdd(&nl[2 * cap + X], lam[2 * cap + X] * t.prm.DT_Cell * 2.0 * t.m_eps * (7.0 * L - 13.0 * L * L) / (z * z));
+      peps += lam[2 * cap + X] * t.prm.DT_Cell * 2.0 * L * (L - 1.0) / z;
     }
     if (!memb) continue;
     const P3 lm = ld3(lam, cap, X);
-    const P3 ob = v_scale(DT_Cell * t.m_kbend, lm);
+    const P3 ob = v_scale(t.prm.DT_Cell * t.m_kbend, lm);
     const int* md = t.memb6 + 6 * X;
     P3 dr{0, 0, 0};
     for (int i = 0; i < 6; i++) {
@@ -938,7 +938,7 @@ __global__ void __launch_bounds__(256) k_mech_bwd_memb(const Tis* __restrict__ T
       tri_bend_2_bar(t, cx.p, pa, pbv, pc, ob, bar);
       for (int k = 0; k < 4; k++) atomic_add3(nl, cap, idx[k], bar[k]);
     }
-    pbend += DT_Cell * v_dot(lm, dr);
+    pbend += t.prm.DT_Cell * v_dot(lm, dr);
   }
   for (int o = 32; o > 0; o >>= 1) {
     peps += __shfl_down(peps, o, 64);

@@ -18,35 +18,32 @@
 
 namespace epi {
 
-// cell_state_renew.cpp:17-119
-constexpr double stoch_div_time_ratio = 0.25;
-constexpr double S2 = 0.1;
-constexpr double eps_kb = 0.12;
-constexpr double accel_div = 1.0;
-constexpr double alpha_b = 5.0;
-constexpr double S0 = 0.1 * 0.2;
-constexpr double eps_ks = 0.10 * 0.5;
-constexpr double accel_diff = 1.0;
-constexpr double alpha_k = 2.0;
-constexpr double eps_kk = 0.10 * 0.5;
-constexpr double S1 = 0.1;
-constexpr double ubar = 0.45;
-constexpr double delta_lipid = 0.1;
-constexpr double delta_sig_r1 = 0.1;
-constexpr double lipid_rel = 0.05 * 4.0;
-constexpr double lipid_c = 0.6;
+// cell_state_renew.cpp:17-119: the rates and thresholds are runtime values
+// (Tis::prm); these two are fixed by the reference's code
 constexpr double stoch_corr_coef = 1;
-constexpr double agki_max = 6.0;
-constexpr double delta_L = 0.01 * R_max;      // :231
-constexpr double ADHE_CONST = 31.3;           // :331
-constexpr unsigned DISA_conn_num_thresh = 11;  // :332
 constexpr double PI_D = 3.14159265358979323846;
 
-__device__ __forceinline__ double weighted_eps_kb(uint8_t malig) { return malig ? accel_div * eps_kb : eps_kb; }
+// The constants of the renewal laws, copied out of the tissue's record once,
+// before a kernel's first store: the compiler cannot tell a cell store from
+// the descriptor, so a record field read after a store would be loaded from
+// memory again.
+struct RenewK {
+  double DT_Cell, THRESH_DEAD, THRESH_SP, ca2p_init, stoch_div_time_ratio, S2, accel_div, eps_kb, alpha_b, S0, eps_ks,
+      accel_diff, alpha_k, eps_kk, S1, ubar, delta_lipid, delta_sig_r1, lipid_rel, lipid, ADHE_CONST, DISA;
+};
+__device__ __forceinline__ RenewK renew_k(const Params& pm) {
+  return RenewK{pm.DT_Cell, pm.THRESH_DEAD, pm.THRESH_SP, pm.ca2p_init, pm.stoch_div_time_ratio, pm.S2, pm.accel_div,
+                pm.eps_kb, pm.alpha_b, pm.S0, pm.eps_ks, pm.accel_diff, pm.alpha_k, pm.eps_kk, pm.S1, pm.ubar,
+                pm.delta_lipid, pm.delta_sig_r1, pm.lipid_rel, pm.lipid, pm.ADHE_CONST, pm.DISA_conn_num_thresh};
+}
+
+__device__ __forceinline__ double weighted_eps_kb(const RenewK& k, uint8_t malig) {
+  return malig ? k.accel_div * k.eps_kb : k.eps_kb;
+}
 
 // is_divide_ready without the draw (:147-155), with the MUSUME/FIX guards of
 // _MUSUME_state_renew (:282-298) and _FIX_state_renew (:308-318)
-__device__ __forceinline__ bool needs_draw(const Tis& t, int c) {
+__device__ __forceinline__ bool needs_draw(const Tis& t, const RenewK& k, int c) {
   const uint8_t s = t.state[c];
   if (s == FIX) {
     if (t.dermis[c] < 0) return false;
@@ -56,13 +53,12 @@ __device__ __forceinline__ bool needs_draw(const Tis& t, int c) {
   } else {
     return false;
   }
-  return t.pair[c] < 0 && (t.ageb[c] >= t.div_age_thresh[c] * (1.0 - stoch_div_time_ratio));
+  return t.pair[c] < 0 && (t.ageb[c] >= t.div_age_thresh[c] * (1.0 - k.stoch_div_time_ratio));
 }
 
-__device__ __forceinline__ bool to_remove(const Tis& t, int cb, int c) {
+__device__ __forceinline__ bool to_remove(const Tis& t, const RenewK& k, int cb, int c) {
   const uint8_t s = t.state[c];
-  return (s == DEAD || s == AIR) && t.agek[c] >= ADHE_CONST &&
-         (unsigned)t.nlen[cb][c] <= DISA_conn_num_thresh;
+  return (s == DEAD || s == AIR) && t.agek[c] >= k.ADHE_CONST && (double)t.nlen[cb][c] <= k.DISA;
 }
 
 // Flags of every non-membrane cell (bit 0: needs a draw, bit 1: removal)
@@ -72,6 +68,7 @@ constexpr int RCHUNK = 1024;
 __global__ void __launch_bounds__(256) k_renew_flags(const Tis* __restrict__ T, int cb) {
   const Tis& t = T[blockIdx.y];
   Status* st = t.st;
+  const RenewK rk = renew_k(t.prm);
   const int start = t.nmemb + t.nder;
   const int n = st->n;
   // blocks are 256 cells and chunks 1024, so a wave never spans two chunks
@@ -79,7 +76,7 @@ __global__ void __launch_bounds__(256) k_renew_flags(const Tis* __restrict__ T, 
        c += gridDim.x * blockDim.x) {
     uint8_t f = 0;
     if (c < n) {
-      f = (needs_draw(t, c) ? 1 : 0) | (to_remove(t, cb, c) ? 2 : 0);
+      f = (needs_draw(t, rk, c) ? 1 : 0) | (to_remove(t, rk, cb, c) ? 2 : 0);
       t.rflag[c] = f;
     }
     const unsigned long long any = __ballot(f != 0);
@@ -99,6 +96,7 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int start = t.nmemb + t.nder;
   const int n_old = st->n;
+  const RenewK rk = renew_k(t.prm);
   if (tid == 0) { s_cursor = st->rng_pos; s_ndiv = 0; s_nrm_base = 0; st->n_old = n_old; }
   __syncthreads();
   for (int base = start; base < n_old; base += RCHUNK) {
@@ -125,8 +123,8 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
     if (rd) {
       const int k = off_r + __popcll(mr & below);
       s_ready[k] = c;
-      s_thr[k] = t.div_age_thresh[c] * stoch_div_time_ratio;
-      s_rhs[k] = stoch_corr_coef * DT_Cell * weighted_eps_kb(t.is_malignant[c]) * S2;
+      s_thr[k] = t.div_age_thresh[c] * rk.stoch_div_time_ratio;
+      s_rhs[k] = stoch_corr_coef * rk.DT_Cell * weighted_eps_kb(rk, t.is_malignant[c]) * rk.S2;
     }
     if (rm) t.rm_list[s_nrm_base + off_m + __popcll(mm & below)] = c;
     __syncthreads();
@@ -214,19 +212,19 @@ __global__ void __launch_bounds__(1024) k_renew_scan(const Tis* __restrict__ T, 
 }
 
 // ageing and lipid laws (:31-119)
-__device__ __forceinline__ double ageb_const(double ca2p_avg, uint8_t malig) {
-  return weighted_eps_kb(malig) * (S2 + alpha_b * min0(ca2p_avg - ca2p_init));
+__device__ __forceinline__ double ageb_const(const RenewK& k, double ca2p_avg, uint8_t malig) {
+  return weighted_eps_kb(k, malig) * (k.S2 + k.alpha_b * min0(ca2p_avg - k.ca2p_init));
 }
-__device__ __forceinline__ double agek_ALIVE_const(double ca2p_avg, uint8_t malig) {
-  return (malig ? accel_diff * eps_ks : eps_ks) * (S0 + alpha_k * min0(ca2p_avg - ca2p_init));
+__device__ __forceinline__ double agek_ALIVE_const(const RenewK& k, double ca2p_avg, uint8_t malig) {
+  return (malig ? k.accel_diff * k.eps_ks : k.eps_ks) * (k.S0 + k.alpha_k * min0(ca2p_avg - k.ca2p_init));
 }
-__device__ __forceinline__ double k_lipid_release(double ca2p_avg, double agek) {
-  return 0.25 * lipid_rel * (1 + tanh((ca2p_avg - ubar) / delta_sig_r1)) *
-         (1 + tanh((agek - THRESH_SP) / delta_lipid));
+__device__ __forceinline__ double k_lipid_release(const RenewK& k, double ca2p_avg, double agek) {
+  return 0.25 * k.lipid_rel * (1 + tanh((ca2p_avg - k.ubar) / k.delta_sig_r1)) *
+         (1 + tanh((agek - k.THRESH_SP) / k.delta_lipid));
 }
-__device__ __forceinline__ double k_lipid(double ca2p_avg, double agek) {
-  return 0.25 * lipid_c * (1 + tanh((ubar - ca2p_avg) / delta_sig_r1)) *
-         (1 + tanh((agek - THRESH_SP) / delta_lipid));
+__device__ __forceinline__ double k_lipid(const RenewK& k, double ca2p_avg, double agek) {
+  return 0.25 * k.lipid * (1 + tanh((k.ubar - ca2p_avg) / k.delta_sig_r1)) *
+         (1 + tanh((agek - k.THRESH_SP) / k.delta_lipid));
 }
 
 // cell_divide (:227-270) for division rank k of parent c
@@ -243,12 +241,13 @@ __device__ void divide(const Tis& t, int pb, int cpar, int c, int k, int n_old, 
   t.state[d] = MUSUME;
   t.fix_origin[d] = t.fix_origin[c];
   t.radius[d] = t.radius[c];
-  t.div_age_thresh[d] = agki_max;
+  t.div_age_thresh[d] = t.prm.agki_max;
   t.ca2p[0][d] = cav; t.ca2p[1][d] = cav;
   t.ca2p_avg[d] = cav;
   t.ip3[0][d] = ipv; t.ip3[1][d] = ipv;
   t.ex_inert[d] = t.ex_inert[c];
   t.agek[d] = 0; t.ageb[d] = 0; t.ex_fat[d] = 0; t.in_fat[d] = 0;
+  const double delta_L = t.prm.delta_L;   // :231
   t.spr_nat_len[d] = delta_L;
   t.rest_div_times[d] = mus ? rdt - 1 : rdt;
   t.is_malignant[d] = t.is_malignant[c];
@@ -279,35 +278,37 @@ __global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, 
   const int n_old = st->n_old;
   const long long next_uid = st->next_uid;
   const long long ts = st->timestep;
+  const RenewK pm = renew_k(t.prm);
+  const double DT_Cell = pm.DT_Cell;
   int ncorn = 0;
   for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n_old;
        c += gridDim.x * blockDim.x) {
     switch (t.state[c]) {
       case ALIVE: {   // _ALIVE_state_renew :347-358
         const double agek = t.agek[c];
-        if (agek >= THRESH_DEAD) {
+        if (agek >= pm.THRESH_DEAD) {
           t.state[c] = DEAD;   // cornificate (cellmanager.cpp:48-53)
           t.cst[3 * c + 1] = ts;
           ncorn++;
         } else {
           const double cav = t.ca2p_avg[c];
           const double in_fat = t.in_fat[c];
-          const double tmp = k_lipid_release(cav, agek) * in_fat;
-          t.in_fat[c] = in_fat + DT_Cell * (k_lipid(cav, agek) * (1.0 - in_fat) - tmp);
+          const double tmp = k_lipid_release(pm, cav, agek) * in_fat;
+          t.in_fat[c] = in_fat + DT_Cell * (k_lipid(pm, cav, agek) * (1.0 - in_fat) - tmp);
           t.ex_fat[c] = t.ex_fat[c] + DT_Cell * tmp;
-          t.agek[c] = agek + DT_Cell * agek_ALIVE_const(cav, t.is_malignant[c]);
+          t.agek[c] = agek + DT_Cell * agek_ALIVE_const(pm, cav, t.is_malignant[c]);
         }
         break;
       }
       case DEAD: case AIR:   // _DEAD_AIR_state_renew :330-341
-        if (!to_remove(t, cb, c)) t.agek[c] = t.agek[c] + DT_Cell * (eps_kk * S1);
+        if (!to_remove(t, pm, cb, c)) t.agek[c] = t.agek[c] + DT_Cell * (pm.eps_kk * pm.S1);
         break;
       case FIX: {   // _FIX_state_renew :308-324
         if (t.dermis[c] < 0) { set_err(st, ERR_NO_DERMIS); break; }
-        const bool rd = needs_draw(t, c);
+        const bool rd = needs_draw(t, pm, c);
         const int k = rd ? t.div_slot[c] : -1;
         if (k >= 0) divide(t, pb, cpar, c, k, n_old, next_uid);
-        else t.ageb[c] = t.ageb[c] + DT_Cell * ageb_const(t.ca2p_avg[c], t.is_malignant[c]);
+        else t.ageb[c] = t.ageb[c] + DT_Cell * ageb_const(pm, t.ca2p_avg[c], t.is_malignant[c]);
         break;
       }
       case MUSUME: {   // _MUSUME_state_renew :282-302 (SYSTEM == WHOLE)
@@ -316,10 +317,10 @@ __global__ void __launch_bounds__(256) k_renew_apply(const Tis* __restrict__ T, 
           t.cst[3 * c] = ts;   // :287
           break;
         }
-        const bool rd = needs_draw(t, c);
+        const bool rd = needs_draw(t, pm, c);
         const int k = rd ? t.div_slot[c] : -1;
         if (k >= 0) divide(t, pb, cpar, c, k, n_old, next_uid);
-        else t.ageb[c] = t.ageb[c] + DT_Cell * ageb_const(t.ca2p_avg[c], t.is_malignant[c]);
+        else t.ageb[c] = t.ageb[c] + DT_Cell * ageb_const(pm, t.ca2p_avg[c], t.is_malignant[c]);
         break;
       }
       default:
@@ -407,12 +408,12 @@ __global__ void __launch_bounds__(256) k_pair_disperse(const Tis* __restrict__ T
   for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     const int p = t.pair[c];
     if (p < 0 || !(c > p)) continue;
-    const double eps_L = 0.14, unpair_dist_coef = 0.9;
+    const double eps_L = t.prm.eps_L, unpair_dist_coef = t.prm.unpair_dist_coef;
     const double rad_sum = t.radius[c] + t.radius[p];
     const double unpair_th = unpair_dist_coef * rad_sum;
     const double snl = t.spr_nat_len[c];
     if (snl < 2.0 * t.radius[c]) {
-      const double v = snl + DT_Cell * eps_L;
+      const double v = snl + t.prm.DT_Cell * eps_L;
       t.spr_nat_len[c] = v;
       t.spr_nat_len[p] = v;
     } else if (dist_sq(t, ld_pos(t, pb, c), ld_pos(t, pb, p)) > unpair_th * unpair_th) {
@@ -430,7 +431,8 @@ __global__ void __launch_bounds__(256) k_initialize_sc(const Tis* __restrict__ T
   const int n = t.st->n;
   const double zzmax = __longlong_as_double((long long)t.st->zzmax_bits);
   for (int c = t.nmemb + t.nder + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    if (t.state[c] == ALIVE && zzmax - t.pos[pb][2][c] < 8 * R_max) {
+    const double THRESH_DEAD = t.prm.THRESH_DEAD;
+    if (t.state[c] == ALIVE && zzmax - t.pos[pb][2][c] < 8 * t.prm.R_max) {
       t.agek[c] = t.agek[c] > THRESH_DEAD ? t.agek[c] : THRESH_DEAD;
       t.state[c] = AIR;
     }

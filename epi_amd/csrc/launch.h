@@ -14,7 +14,7 @@ enum KernelId {
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
   K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
   K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB,
-  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_WL, K_COUNT
+  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_WL, K_CA_SWAP, K_COUNT
 };
 
 // ints of a context's work-list buffer: the offsets and per-tissue iz_bound
@@ -115,6 +115,10 @@ void launch_ca_substeps(hipStream_t s, const Tis* dT, const CaList& cl, int cb, 
 void launch_ca_touch(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int cb, int max_n, long long max_box);
 void launch_ca_diffu_zero(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
 void launch_ca_end(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n);
+// the listed tissues' two Ca, IP3 and ATP buffers trade contents (after a
+// solve of part of a context with an odd sub-step count, whose buffer
+// parities are shared by every tissue)
+void launch_ca_swap(hipStream_t s, const Tis* dT, const int* tlist, int ntis, int max_n, long long max_gsize);
 // frozen reaction-diffusion window (k_adjoint.hip)
 void launch_rd_snapshot(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int cb, int cp, int ap, long long items);
 void launch_rd_own(hipStream_t s, const Tis* dT, RdTis* dR, int ntis, int max_n);

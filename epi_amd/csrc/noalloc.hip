@@ -56,9 +56,10 @@ int na_scratch(unsigned char** out) {
 }
 
 // one descriptor and status per view: the fields k_wl_b and k_ext_stim read
-// (the same expressions as epi_create's for dz and inv_dz)
+// (the same expressions as epi_create's for dz and inv_dz), with epi5_MT's
+// parameters
 __global__ void __launch_bounds__(256) k_na_setup(const epi_bfield_view* __restrict__ views, int n,
-                                                  Tis* __restrict__ T, Status* __restrict__ S) {
+                                                  Tis* __restrict__ T, Status* __restrict__ S, const Params prm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const epi_bfield_view v = views[i];
@@ -80,6 +81,7 @@ __global__ void __launch_bounds__(256) k_na_setup(const epi_bfield_view* __restr
   t.cmap2 = const_cast<uint8_t*>(v.cmap2);
   t.state = const_cast<uint8_t*>(v.state);
   t.agek = const_cast<double*>(v.agek);
+  t.prm = prm;
   T[i] = t;
 }
 
@@ -104,7 +106,14 @@ extern "C" int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t n
   int* wl = reinterpret_cast<int*>(base + NA_TIS + NA_ST);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const unsigned g = (unsigned)((ngrids + 255) / 256);
-  hipLaunchKernelGGL(k_na_setup, dim3(g), dim3(256), 0, s, views, ngrids, T, S);
+  static const Params prm = [] {
+    epi_params d;
+    epi_params_default(&d);
+    Params p;
+    derive_params(d, p);
+    return p;
+  }();
+  hipLaunchKernelGGL(k_na_setup, dim3(g), dim3(256), 0, s, views, ngrids, T, S, prm);
   // the grid sizes are in device memory: the blocks walk the list grid-stride
   // from a grid sized for 256 tiles per view
   launch_ext_stim(s, T, ngrids, 0, 256LL * ngrids, wl, Tuning{});

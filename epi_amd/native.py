@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libepi_amd.so")
+# EPI_AMD_LIB names another build of the same library (to compare builds)
+LIB_PATH = os.environ.get("EPI_AMD_LIB") or os.path.join(HERE, "libepi_amd.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -98,7 +99,8 @@ SYMBOLS = [
     "epi_set_tuning", "epi_get_tuning", "epi_calc_ca2p_n", "epi_profile_sample", "epi_profile_launches",
     "epi_rd_begin", "epi_rd_forward", "epi_rd_get_ca2p", "epi_rd_adjoint", "epi_rd_end",
     "epi_mech_begin", "epi_mech_forward", "epi_mech_get_positions", "epi_mech_adjoint", "epi_mech_end",
-    "epi_set_timestep", "epi_take_cell_stats",
+    "epi_set_timestep", "epi_take_cell_stats", "epi_params_default", "epi_params_load", "epi_params_save",
+    "epi_set_params", "epi_get_params", "epi_sheet_size", "epi_init_gen",
 ]
 
 MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
@@ -106,6 +108,124 @@ MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
 # launch-shape keys of epi_set_tuning (include/epi_amd.h)
 TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4, "xcd": 5, "rd_group": 6, "ca_fuse": 7,
                "map_lanes": 8, "ca_group_small": 9, "step_graph": 10, "b_xcd": 11}
+
+class Params(C.Structure):
+    """epi_params (include/epi_amd.h): the keys of epi_intg's CalcParams
+    (epi_intg/calc/CalcParams.cpp:51-162) in the header's order."""
+    _fields_ = [
+        ("MAX_CONNECT_CELL_NUM", C.c_int32), ("DT_Cell", C.c_double), ("DT_Ca", C.c_double),
+        ("MALIG_NUM", C.c_int32), ("LX", C.c_double), ("LY", C.c_double), ("LZ", C.c_double), ("NX", C.c_int32),
+        ("NY", C.c_int32), ("NZ", C.c_int32), ("R_max", C.c_double), ("R_der", C.c_double), ("R_memb", C.c_double),
+        ("COMPRESS_FACTOR", C.c_double), ("MEMB_ADHE_RANGE", C.c_int32), ("THRESH_SP", C.c_double),
+        ("LJ_THRESH", C.c_double), ("THRESH_DEAD", C.c_double), ("NUM_ITR", C.c_int32), ("Ca_avg_time", C.c_double),
+        ("ca2p_init", C.c_double), ("ex_inert_init", C.c_double), ("IP3_init", C.c_double), ("gj_init", C.c_double),
+        ("ATP_init", C.c_double), ("ext_stim_init", C.c_double), ("FAC_MAP", C.c_double), ("SW_THRESH", C.c_int32),
+        ("T_TURNOVER", C.c_double), ("NUM_SC_INIT", C.c_int32), ("USE_TRI_MEMB", C.c_int32),
+        ("agki_max", C.c_double), ("fac", C.c_double), ("div_max", C.c_int32), ("Kpp", C.c_double),
+        ("dp", C.c_double), ("kbc", C.c_double), ("Hb", C.c_double), ("Cout", C.c_double), ("kg", C.c_double),
+        ("gamma", C.c_double), ("mu0", C.c_double), ("mu1", C.c_double), ("kmu", C.c_double), ("para_b", C.c_double),
+        ("para_bb", C.c_double), ("para_k1", C.c_double), ("k_flux", C.c_double), ("beta_zero", C.c_double),
+        ("CA_OUT", C.c_double), ("thgra", C.c_double), ("delta_th", C.c_double), ("thpri", C.c_double),
+        ("kpa", C.c_double), ("delta_K", C.c_double), ("Kpri", C.c_double), ("delta_I", C.c_double),
+        ("iage_kitei", C.c_double), ("para_k2", C.c_double), ("H0", C.c_double), ("wd", C.c_double),
+        ("ca2p_du", C.c_double), ("STIM11", C.c_double), ("Kaa", C.c_double), ("Da", C.c_double),
+        ("AIR_STIM", C.c_double), ("AREA_GRID_ORIGINAL", C.c_double), ("N2", C.c_int32), ("N3", C.c_int32),
+        ("eps_m", C.c_double), ("DER_DER_CONST", C.c_double), ("K_TOTAL", C.c_double),
+        ("K_DESMOSOME_RATIO", C.c_double), ("Kspring", C.c_double), ("Kspring_d", C.c_double), ("KBEND", C.c_double),
+        ("delta_R_coef", C.c_double), ("para_ljp2", C.c_double), ("Kspring_division", C.c_double),
+        ("stoch_div_time_ratio", C.c_double), ("S2", C.c_double), ("accel_div", C.c_double), ("eps_kb", C.c_double),
+        ("alpha_b", C.c_double), ("S0", C.c_double), ("eps_ks", C.c_double), ("accel_diff", C.c_double),
+        ("alpha_k", C.c_double), ("eps_kk", C.c_double), ("S1", C.c_double), ("ubar", C.c_double),
+        ("delta_lipid", C.c_double), ("delta_sig_r1", C.c_double), ("lipid_rel", C.c_double), ("lipid", C.c_double),
+        ("delta_L_coef", C.c_double), ("ADHE_CONST", C.c_double), ("DISA_conn_num_thresh", C.c_double),
+        ("eps_L", C.c_double), ("unpair_dist_coef", C.c_double), ("kb", C.c_double), ("DUR_ALIVE", C.c_double),
+        ("DUR_DEAD", C.c_double), ("DB", C.c_double), ("SYSTEM", C.c_int32), ("CUT", C.c_int32),
+        ("NEW_BEND_POT", C.c_int32), ("STOCHASTIC", C.c_int32), ("force_cornification", C.c_int32),
+        ("outputdir", C.c_char * 256),
+    ]
+
+    def to_dict(self):
+        return {k: (getattr(self, k).decode() if k == "outputdir" else getattr(self, k)) for k, _ in self._fields_}
+
+    def replace(self, **kv):
+        """a copy with the given keys changed"""
+        q = Params()
+        C.pointer(q)[0] = self
+        for k, v in kv.items():
+            if k not in PARAM_KEYS:
+                raise KeyError(k)
+            setattr(q, k, v.encode() if k == "outputdir" else v)
+        return q
+
+
+PARAM_KEYS = [k for k, _ in Params._fields_]
+
+
+def params_default():
+    """epi5_MT's values (epi_params_default)"""
+    p = Params()
+    load_library().epi_params_default(C.byref(p))
+    return p
+
+
+def params_load(path, require_all=True, base=None):
+    """a ppsss-style key=value file (Params::load over parse_paramtext,
+    epi_intg/Params.cpp:91-96, parser.cpp:30-58) read over `base` (default:
+    params_default()); raises EpiError(INVALID) with the reader's message"""
+    p = Params()
+    C.pointer(p)[0] = base if base is not None else params_default()
+    err = C.create_string_buffer(512)
+    rc = load_library().epi_params_load(os.fsencode(path), int(bool(require_all)), C.byref(p), err, 512)
+    if rc:
+        raise EpiError(rc, err.value.decode())
+    return p
+
+
+def params_save(path, p):
+    """one key=value line per key (Params::generate_paramfile, Params.cpp:98-110)"""
+    rc = load_library().epi_params_save(os.fsencode(path), C.byref(p))
+    if rc:
+        raise EpiError(rc, f"cannot write {path}")
+
+
+class GenCells(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nmemb", C.c_int32), ("nder", C.c_int32), ("nmx", C.c_int32),
+                ("nmy", C.c_int32), ("state", _u8p), ("fix_origin", _ip), ("x", _dp), ("y", _dp), ("z", _dp),
+                ("radius", _dp)]
+
+
+def init_gen(params=None, nfix=16, der=8):
+    """init_gen(nfix, der) of epi_intg (epi_intg/calc/init_gen.cpp:23-106)
+    under `params` (default: epi5_MT's values): deck columns of the sheet,
+    the dermis layers and the relaxed FIX ring, plus the sheet's (nmx, nmy)"""
+    L = load_library()
+    p = params if params is not None else params_default()
+    g = GenCells()
+    err = C.create_string_buffer(512)
+    rc = L.epi_init_gen(C.byref(p), nfix, der, 0, C.byref(g), err, 512)
+    if rc not in (0, 1):
+        raise EpiError(rc, err.value.decode())
+    n = g.n
+    cols = {"state": np.zeros(n, np.uint8), "fix_origin": np.zeros(n, np.int32)}
+    for k in ("x", "y", "z", "radius"):
+        cols[k] = np.zeros(n)
+    g = GenCells(0, 0, 0, 0, 0, _ptr(cols["state"], _u8p), _ptr(cols["fix_origin"], _ip), _ptr(cols["x"], _dp),
+                 _ptr(cols["y"], _dp), _ptr(cols["z"], _dp), _ptr(cols["radius"], _dp))
+    rc = L.epi_init_gen(C.byref(p), nfix, der, n, C.byref(g), err, 512)
+    if rc:
+        raise EpiError(rc, err.value.decode())
+    return cols, (g.nmx, g.nmy)
+
+
+def sheet_size(params=None):
+    """(MEMB_NUM_X, MEMB_NUM_Y) of CalcParams::calculate_param (CalcParams.cpp:20-26)"""
+    p = params if params is not None else params_default()
+    x, y = C.c_int32(), C.c_int32()
+    rc = load_library().epi_sheet_size(C.byref(p), C.byref(x), C.byref(y))
+    if rc:
+        raise EpiError(rc, "sheet size")
+    return x.value, y.value
+
 
 class RdParams(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", C.POINTER(C.c_double))]
@@ -182,6 +302,15 @@ def load_library():
     L.epi_mech_end.argtypes = [vp]
     L.epi_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.epi_set_timestep.argtypes = [vp, C.c_int32, C.c_int64]
+    L.epi_params_default.argtypes = [C.POINTER(Params)]
+    L.epi_params_default.restype = None
+    L.epi_params_load.argtypes = [C.c_char_p, C.c_int32, C.POINTER(Params), C.c_char_p, C.c_int32]
+    L.epi_params_save.argtypes = [C.c_char_p, C.POINTER(Params)]
+    L.epi_set_params.argtypes = [vp, C.c_int32, C.POINTER(Params)]
+    L.epi_get_params.argtypes = [vp, C.c_int32, C.POINTER(Params)]
+    L.epi_sheet_size.argtypes = [C.POINTER(Params), _ip, _ip]
+    L.epi_init_gen.argtypes = [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.POINTER(GenCells), C.c_char_p,
+                               C.c_int32]
     L.epi_take_cell_stats.argtypes = [vp, C.c_int32, _i64p, C.c_int32, C.POINTER(C.c_int32)]
     _lib = L
     return L
@@ -494,6 +623,19 @@ class TissueSet:
         out["n"], out["nmemb"], out["nder"], out["sw"] = cs.n, cs.nmemb, cs.nder, cs.sw
         return out
 
+    # --- runtime parameters (epi_intg/calc/CalcParams.cpp:16-279) ---
+    def set_params(self, t, p):
+        """tissue t's parameters, read from the next call on (a Params, or a
+        dict of keys changed from epi5_MT's values)"""
+        if isinstance(p, dict):
+            p = self.get_params(t).replace(**p)
+        self._chk(self.L.epi_set_params(self.h, t, C.byref(p)), "set_params")
+
+    def get_params(self, t):
+        p = Params()
+        self._chk(self.L.epi_get_params(self.h, t, C.byref(p)), "get_params")
+        return p
+
     # --- cell_stat (cellmanager.cpp:338-352, 559-567) ---
     def set_timestep(self, t, step):
         self._chk(self.L.epi_set_timestep(self.h, t, int(step)), "set_timestep")
@@ -572,6 +714,8 @@ class Tissue:
     def get_cells(self): return self.set.get_cells(0)
     def set_timestep(self, step): self.set.set_timestep(0, step)
     def take_cell_stats(self): return self.set.take_cell_stats(0)
+    def set_params(self, p): self.set.set_params(0, p)
+    def get_params(self): return self.set.get_params(0)
     def get_fields(self): return self.set.get_fields(0)
     def set_positions(self, x, y, z): self.set.set_positions(0, x, y, z)
     def set_state(self, state=None, agek=None): self.set.set_state(0, state, agek)

@@ -112,6 +112,101 @@ void epi_destroy(epi_ctx* ctx);
 const char* epi_last_error(const epi_ctx* ctx);
 int32_t epi_num_tissues(const epi_ctx* ctx);
 
+/* ---- runtime parameters ----------------------------------------------------
+ * The physics constants of the path as runtime values: the key list of
+ * epi_intg's CalcParams (epi_intg/calc/CalcParams.cpp:51-162) with the values
+ * of epi5_MT as defaults (define.h:80-230, cell_interaction.cpp:21-45,
+ * cell_state_renew.cpp:17-119, ca2p.cpp:14-144, ext_stim.cpp:9-20) and the
+ * derived quantities of CalcParams::calculate_param (:16-45).  Unsigned and
+ * bool keys are int32 (a bool reads as 1 when its text reads as the integer
+ * 1, else 0; an unsigned key refuses a negative value).  A context starts
+ * with the defaults, its geometry's grid and box filled in.
+ * Keys that fix the shape of the build rather than a value must keep the
+ * values the context was created with, and epi_set_params refuses a different
+ * one with EPI_ERR_INVALID: LX, LY, LZ, NX, NY, NZ (the geometry),
+ * MAX_CONNECT_CELL_NUM (400), N2 (400), N3 (200), AREA_GRID_ORIGINAL (2.0,
+ * the bucket edge), USE_TRI_MEMB, NEW_BEND_POT and STOCHASTIC (1), SYSTEM
+ * (0).  MALIG_NUM, R_memb, MEMB_ADHE_RANGE, ATP_init and ext_stim_init
+ * describe an initial state, which the caller builds; NUM_ITR, CUT,
+ * force_cornification and outputdir are the driver's.  ca2p_init,
+ * ex_inert_init, IP3_init and gj_init are read by epi_value_init (and
+ * ca2p_init by the renewal laws, gj_init by new list entries), agki_max,
+ * fac and div_max by epi_load_cells.  Every other key is a value the kernels
+ * read, per tissue, from the next call on; epi_step also takes DT_Cell,
+ * T_TURNOVER, NUM_SC_INIT and SW_THRESH from it, and a Ca solve runs
+ * (int)(Ca_avg_time / DT_Ca) sub-steps.  DT_Cell and DT_Ca must be positive
+ * and Ca_avg_time / DT_Ca at least 1. */
+typedef struct epi_params {
+    int32_t MAX_CONNECT_CELL_NUM;
+    double DT_Cell, DT_Ca;
+    int32_t MALIG_NUM;
+    double LX, LY, LZ;
+    int32_t NX, NY, NZ;
+    double R_max, R_der, R_memb, COMPRESS_FACTOR;
+    int32_t MEMB_ADHE_RANGE;
+    double THRESH_SP, LJ_THRESH, THRESH_DEAD;
+    int32_t NUM_ITR;
+    double Ca_avg_time, ca2p_init, ex_inert_init, IP3_init, gj_init, ATP_init, ext_stim_init, FAC_MAP;
+    int32_t SW_THRESH;
+    double T_TURNOVER;
+    int32_t NUM_SC_INIT, USE_TRI_MEMB;
+    double agki_max, fac;
+    int32_t div_max;
+    double Kpp, dp, kbc, Hb, Cout, kg, gamma, mu0, mu1, kmu, para_b, para_bb, para_k1, k_flux, beta_zero, CA_OUT;
+    double thgra, delta_th, thpri, kpa, delta_K, Kpri, delta_I, iage_kitei, para_k2, H0, wd, ca2p_du;
+    double STIM11, Kaa, Da, AIR_STIM, AREA_GRID_ORIGINAL;
+    int32_t N2, N3;
+    double eps_m, DER_DER_CONST, K_TOTAL, K_DESMOSOME_RATIO, Kspring, Kspring_d, KBEND, delta_R_coef, para_ljp2,
+        Kspring_division;
+    double stoch_div_time_ratio, S2, accel_div, eps_kb, alpha_b, S0, eps_ks, accel_diff, alpha_k, eps_kk, S1;
+    double ubar, delta_lipid, delta_sig_r1, lipid_rel, lipid, delta_L_coef, ADHE_CONST, DISA_conn_num_thresh;
+    double eps_L, unpair_dist_coef, kb, DUR_ALIVE, DUR_DEAD, DB;
+    int32_t SYSTEM, CUT, NEW_BEND_POT, STOCHASTIC, force_cornification;
+    char outputdir[256];
+} epi_params;
+/* epi5_MT's values (the grid and box of a given geometry are filled in by
+ * epi_get_params) */
+void epi_params_default(epi_params* p);
+/* Params::load (epi_intg/Params.cpp:91-96) over parse_paramtext
+ * (epi_intg/parser.cpp:30-58): "key=value" lines; a line that starts with
+ * '#' or '//', or holds no '=', is skipped; spaces around the key and the
+ * value are dropped; the first line of a key wins.  Every key of the struct
+ * (or its alias: MALIGNANT, LOOP, Ca2P_avg_time, TURNOVER) that appears sets
+ * its field; other keys are ignored.  With require_all a key the file lacks is
+ * an error, as in the reference; without it the field keeps its value.  A
+ * value that does not read as its type is an error.  err (err_len bytes)
+ * gets the message. */
+int epi_params_load(const char* path, int32_t require_all, epi_params* p, char* err, int32_t err_len);
+/* Params::generate_paramfile (epi_intg/Params.cpp:98-110): one "key=value"
+ * line per key in the reference's order */
+int epi_params_save(const char* path, const epi_params* p);
+/* the parameters of one tissue of a context, read from the next call on;
+ * refused (EPI_ERR_INVALID) inside a mechanics or reaction-diffusion window */
+int epi_set_params(epi_ctx* ctx, int32_t tissue, const epi_params* p);
+int epi_get_params(epi_ctx* ctx, int32_t tissue, epi_params* p);
+
+/* ---- initial state: init_gen (epi_intg/calc/init_gen.cpp:23-106) ---------
+ * Host code (no device, no context).  epi_sheet_size gives the membrane
+ * sheet's columns and rows of a parameter record (MEMB_NUM_X, MEMB_NUM_Y of
+ * CalcParams::calculate_param, CalcParams.cpp:20-26): a geometry's nmx, nmy.
+ * epi_init_gen writes the cells of init_gen(nfix, der) into caller arrays of
+ * cap entries: the sheet (triangular when USE_TRI_MEMB), der layers of
+ * dermis, nfix FIX cells on a ring (fix_origin 0..nfix-1), each FIX then
+ * lowered by 0.001 until a membrane cell is within 1.1 (radius + R_der) of
+ * it (relaxation, :23-46).  Rows come MEMB, DER, FIX, as epi_load_cells
+ * wants them; every other cell field is 0.  out->n, nmemb, nder, nmx, nmy
+ * are set even when cap is short (EPI_ERR_CAPACITY).  EPI_ERR_INVALID, with
+ * err's message, when a FIX sinks below Z_LEVEL - R_max as the reference
+ * throws. */
+typedef struct epi_gen_cells {
+    int32_t n, nmemb, nder, nmx, nmy;     /* out */
+    uint8_t* state; int32_t* fix_origin;  /* caller arrays of cap entries */
+    double *x, *y, *z, *radius;
+} epi_gen_cells;
+int epi_sheet_size(const epi_params* p, int32_t* nmx, int32_t* nmy);
+int epi_init_gen(const epi_params* p, int32_t nfix, int32_t der, int32_t cap, epi_gen_cells* out, char* err,
+                 int32_t err_len);
+
 /* ---- initialisation: cman_init (cell_init.cpp:132-142) ------------------ */
 /* CellManager::_load_from_file + _memb_init (cellmanager.cpp:198-323, 66-113) */
 int epi_load_cells(epi_ctx* ctx, int32_t tissue, const epi_cell_input* in);
@@ -262,7 +357,8 @@ int epi_rd_end(epi_ctx* ctx);
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): calc_ext_stim (ext_stim.cpp:18-69) over grids the caller
- * holds, with no context, through the same kernels as epi_calc_ext_stim.
+ * holds, with no context, through the same kernels as epi_calc_ext_stim,
+ * with epi5_MT's parameters (epi_params_default).
  * The sweep's grid descriptors, per-grid status and tile work list live in
  * one static device array per GPU of EPI_NOALLOC_SCRATCH_BYTES, held until
  * the process ends, so a call takes at most EPI_NOALLOC_MAX_GRIDS grids.

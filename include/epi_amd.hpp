@@ -174,6 +174,31 @@ class CellManager {
     std::fclose(f);
   }
 
+  // runtime parameters: the keys of epi_intg's CalcParams
+  // (epi_intg/calc/CalcParams.cpp:51-162), epi5_MT's values by default.
+  // Set them before cman_init for the load to use agki_max, fac and div_max.
+  epi_params params() const {
+    epi_params p;
+    check(ctx_, epi_get_params(ctx_, 0, &p), "epi_get_params");
+    return p;
+  }
+  void set_params(const epi_params& p) { check(ctx_, epi_set_params(ctx_, 0, &p), "epi_set_params"); }
+  // Params::load over parse_paramtext (epi_intg/Params.cpp:91-96,
+  // parser.cpp:30-58): every key must be in the file unless require_all is
+  // false; the grid and box are this manager's
+  void load_params(const std::string& path, bool require_all = true) {
+    epi_params p = params();
+    char err[512];
+    const int rc = epi_params_load(path.c_str(), require_all ? 1 : 0, &p, err, (int32_t)sizeof err);
+    if (rc != EPI_OK) throw Error(rc, err);
+    set_params(p);
+  }
+  // Params::generate_paramfile (Params.cpp:98-110)
+  void save_params(const std::string& path) const {
+    const epi_params p = params();
+    if (epi_params_save(path.c_str(), &p) != EPI_OK) throw Error(EPI_ERR_INVALID, "cannot write " + path);
+  }
+
   // grids owned by this manager
   SwapData<FArr3D<double>> atp() { return SwapData<FArr3D<double>>(this, 0); }
   SwapData<FArr3D<double>> ext_stim() { return SwapData<FArr3D<double>>(this, 1); }

@@ -12,6 +12,7 @@
  * that follows the cell through remove_exec's migration.
  */
 #include "epi_oracle.h"
+#include "../include/epi_amd.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -55,51 +56,166 @@ static inline void add_d(double* d, double v) {
 #endif
 }
 
-/* ---- constants: epi5_MT/define.h:69-197 and the physics files ---------- */
-#define DT_Cell 0.01                  /* define.h:86 */
-#define DT_Ca 0.01                    /* define.h:89 */
-#define R_max 1.4                     /* define.h:116 */
-#define R_der 1.4                     /* define.h:119 */
-#define R_memb 1.0                    /* define.h:122 */
-#define COMPRESS_FACTOR 4.0           /* define.h:125 */
-#define THRESH_SP 3.0                 /* define.h:132 */
-#define LJ_THRESH 1.2                 /* define.h:135 */
-#define MEMB_ADJ_CONN_NUM 6           /* define.h:147-153 (TRI_MEMB) */
-#define THRESH_DEAD 22.0              /* define.h:156 */
-#define Ca_avg_time 10.0              /* define.h:162 */
-#define ca2p_init 0.122               /* define.h:165 */
-#define gj_init 0.99                  /* define.h:174 */
-#define FAC_MAP 2.0                   /* define.h:184 */
-#define SW_THRESH 20                  /* define.h:187 */
-#define T_TURNOVER 6000.0             /* define.h:190 */
-#define NUM_SC_INIT 1                 /* define.h:193 */
+/* ---- the parameter record ------------------------------------------------
+ * These functions name the record's fields, so they sit before the names
+ * below that stand for them. */
+typedef struct eo_prm {
+    epi_params in;
+    double P_MEMB, agki_max_fix, leak_from_storage, delta_R, delta_L, iz_margin;
+    int Ca_ITR;
+} eo_prm;
+
+/* CalcParams::calculate_param (CalcParams.cpp:16-45) */
+static void derive(eo_prm* P, const epi_params* in) {
+    P->in = *in;
+    P->P_MEMB = 1.0 / in->COMPRESS_FACTOR;
+    P->agki_max_fix = in->fac * in->agki_max;
+    P->leak_from_storage = in->CA_OUT * in->beta_zero;
+    P->delta_R = in->delta_R_coef * in->R_der;
+    P->delta_L = in->delta_L_coef * in->R_max;
+    P->iz_margin = in->FAC_MAP * in->R_max;
+    P->Ca_ITR = (int)(in->Ca_avg_time / in->DT_Ca);
+}
+
+/* epi5_MT's constants: define.h:80-230, cell.h:23-32, ca2p.cpp:16-144,
+ * cell_connection.cpp:16-30, cell_interaction.cpp:21-45,
+ * cell_state_renew.cpp:17-119, ext_stim.cpp:9-20 */
+void eo_params_default(epi_params* p) {
+    memset(p, 0, sizeof *p);
+    p->MAX_CONNECT_CELL_NUM = 400; p->DT_Cell = 0.01; p->DT_Ca = 0.01; p->MALIG_NUM = 0;
+    p->LX = 50.0; p->LY = 50.0; p->LZ = 100.0; p->NX = 100; p->NY = 100; p->NZ = 200;
+    p->R_max = 1.4; p->R_der = 1.4; p->R_memb = 1.0; p->COMPRESS_FACTOR = 4; p->MEMB_ADHE_RANGE = 0;
+    p->THRESH_SP = 3.0; p->LJ_THRESH = 1.2; p->THRESH_DEAD = 22.0; p->NUM_ITR = 4000000;
+    p->Ca_avg_time = 10.0; p->ca2p_init = 0.122; p->ex_inert_init = 0.97; p->IP3_init = 0; p->gj_init = 0.99;
+    p->ATP_init = 0; p->ext_stim_init = 0; p->FAC_MAP = 2.0; p->SW_THRESH = 20; p->T_TURNOVER = 6000.0;
+    p->NUM_SC_INIT = 1; p->USE_TRI_MEMB = 1;
+    p->agki_max = 6.0; p->fac = 1; p->div_max = 15;
+    p->Kpp = 0.3; p->dp = 0.1; p->kbc = 0.4 * 1.2; p->Hb = 0.01; p->Cout = 1.0; p->kg = 0.1; p->gamma = 2.0;
+    p->mu0 = 0.567; p->mu1 = 0.1; p->kmu = 0.05; p->para_b = 0.11; p->para_bb = 0.89; p->para_k1 = 0.7;
+    p->k_flux = 8.1; p->beta_zero = 0.02; p->CA_OUT = 1.0; p->thgra = 0.2; p->delta_th = 1.0; p->thpri = 1.0;
+    p->kpa = 4.0; p->delta_K = 1.0; p->Kpri = 6.0; p->delta_I = 1.5; p->iage_kitei = 0; p->para_k2 = 0.7;
+    p->H0 = 0.5; p->wd = 0.1; p->ca2p_du = 0.01; p->STIM11 = 0.002; p->Kaa = 0.5; p->Da = 1.0; p->AIR_STIM = 0.1;
+    p->AREA_GRID_ORIGINAL = 2.0; p->N2 = 400; p->N3 = 200;
+    p->eps_m = 0.01; p->DER_DER_CONST = 0.2; p->K_TOTAL = 3.0; p->K_DESMOSOME_RATIO = 0.01; p->Kspring = 25.0;
+    p->Kspring_d = 5.0; p->KBEND = 0.5; p->delta_R_coef = 0.4; p->para_ljp2 = 0.005; p->Kspring_division = 5.0;
+    p->stoch_div_time_ratio = 0.25; p->S2 = 0.1; p->accel_div = 1.0; p->eps_kb = 0.12; p->alpha_b = 5.0;
+    p->S0 = 0.1 * 0.2; p->eps_ks = 0.10 * 0.5; p->accel_diff = 1.0; p->alpha_k = 2.0; p->eps_kk = 0.10 * 0.5;
+    p->S1 = 0.1; p->ubar = 0.45; p->delta_lipid = 0.1; p->delta_sig_r1 = 0.1; p->lipid_rel = 0.05 * 4.0;
+    p->lipid = 0.6; p->delta_L_coef = 0.01; p->ADHE_CONST = 31.3; p->DISA_conn_num_thresh = 11;
+    p->eps_L = 0.14; p->unpair_dist_coef = 0.9;
+    p->kb = 0.025; p->DUR_ALIVE = 0.5; p->DUR_DEAD = 2.0; p->DB = 0.0009;
+    p->SYSTEM = 0; p->CUT = 1000; p->NEW_BEND_POT = 1; p->STOCHASTIC = 1; p->force_cornification = 1;
+    strcpy(p->outputdir, "output");
+}
+
+/* init_gen (epi_intg/calc/init_gen.cpp:48-106) with relaxation (:23-46):
+ * the sheet, der dermis layers and a ring of nfix FIX cells, each FIX
+ * lowered by 0.001 until within 1.1 (radius + R_der) of a membrane cell.
+ * MEMB_NUM_X / MEMB_NUM_Y from calculate_param (CalcParams.cpp:20-26). */
+static double gen_pdiff(double a, double b, double L) {
+    double d = a - b;
+    if (d > 0.5 * L) return d - L;
+    if (d <= -0.5 * L) return d + L;
+    return d;
+}
+int eo_init_gen(const epi_params* p, int nfix, int der, int cap, epi_gen_cells* o) {
+    const double P_MEMB_ = 1.0 / p->COMPRESS_FACTOR, pratio = 2.0 * P_MEMB_;
+    const double ytri = 2.0 / sqrt(3.0);
+    unsigned mx = (unsigned)(p->LX / (p->R_memb * pratio));
+    unsigned my = (unsigned)(p->LY / (p->R_memb * pratio));
+    if (p->USE_TRI_MEMB) my = ((uint32_t)(my * ytri)) & 0xfffffffeu;
+    const double zl = der * 2.0 * p->R_der;
+    const unsigned ndx = (unsigned)(p->LX / (2.0 * p->R_der)), ndy = (unsigned)(p->LY / (2.0 * p->R_der));
+    const int n = (int)(mx * my + (unsigned)der * ndy * ndx + (unsigned)nfix);
+    o->nmx = (int)mx; o->nmy = (int)my; o->nmemb = (int)(mx * my); o->nder = (int)((unsigned)der * ndy * ndx);
+    o->n = n;
+    if (n > cap) return EO_ERR_CAPACITY;
+    int c = 0;
+    for (unsigned k = 0; k < my; k++)
+        for (unsigned j = 0; j < mx; j++, c++) {
+            const double shift = (p->USE_TRI_MEMB && k % 2 == 1) ? p->R_memb * pratio * 0.5 : 0.0;
+            o->state[c] = EO_MEMB; o->fix_origin[c] = 0; o->radius[c] = p->R_memb;
+            o->x[c] = 0.5 * p->R_memb + j * p->R_memb * pratio + shift;
+            o->y[c] = p->USE_TRI_MEMB ? 0.5 * p->R_memb + k * p->R_memb * pratio / ytri
+                                      : 0.5 * p->R_memb + k * p->R_memb * pratio;
+            o->z[c] = zl + p->R_memb;
+        }
+    for (int l = 0; l < der; l++)
+        for (unsigned k = 0; k < ndy; k++)
+            for (unsigned j = 0; j < ndx; j++, c++) {
+                o->state[c] = EO_DER; o->fix_origin[c] = 0; o->radius[c] = p->R_der;
+                o->x[c] = p->R_der + 2.0 * j * p->R_der;
+                o->y[c] = p->R_der + 2.0 * k * p->R_der;
+                o->z[c] = p->R_der + 2.0 * l * p->R_der;
+            }
+    const int nb = c;
+    for (int l = 0; l < nfix; l++, c++) {
+        o->state[c] = EO_FIX; o->fix_origin[c] = l; o->radius[c] = p->R_max;
+        o->x[c] = p->LX / 2.0 + (p->LX / 4.0) * cos(2.0 * 3.14159265358979323846 * l / nfix);
+        o->y[c] = p->LY / 2.0 + (p->LY / 4.0) * sin(2.0 * 3.14159265358979323846 * l / nfix);
+        o->z[c] = p->R_max + p->R_memb + zl + p->R_memb;
+    }
+    for (int f = nb; f < n; f++) {
+        double dmin = 10000.0;
+        while (1) {
+            for (int l = 0; l < nb && o->state[l] == EO_MEMB; l++) {
+                const double dx = gen_pdiff(o->x[f], o->x[l], p->LX), dy = gen_pdiff(o->y[f], o->y[l], p->LY);
+                const double dz = o->z[f] - o->z[l];
+                const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                if (d < dmin) dmin = d;
+            }
+            if (dmin < 1.1 * (o->radius[f] + p->R_der)) break;
+            if (o->z[f] < zl - p->R_max) return EO_ERR_INVALID;
+            o->z[f] = o->z[f] - 0.001;
+        }
+    }
+    return EO_OK;
+}
+
+/* the mechanics coefficients of a record (cell_interaction.cpp:21-28) */
+static void mp_from(const eo_prm* P, eo_mech_params* mp) {
+    mp->eps = P->in.eps_m; mp->k_adh = P->in.K_TOTAL; mp->k_spring = P->in.Kspring; mp->k_bend = P->in.KBEND;
+}
+static unsigned prm_sw_thresh(const eo_prm* P) { return (unsigned)P->in.SW_THRESH; }
+
+/* ---- parameters --------------------------------------------------------
+ * The physics constants are runtime values, one record per tissue (the
+ * epi_params struct of include/epi_amd.h, keys of epi_intg's CalcParams),
+ * with the derived quantities of CalcParams::calculate_param
+ * (epi_intg/calc/CalcParams.cpp:16-45).  Every exported function points g_P
+ * at its tissue's record first (use()); the names below read it. */
+static const eo_prm* g_P;
+#define DT_Cell (g_P->in.DT_Cell)
+#define DT_Ca (g_P->in.DT_Ca)
+#define R_max (g_P->in.R_max)
+#define R_der (g_P->in.R_der)
+#define THRESH_SP (g_P->in.THRESH_SP)
+#define LJ_THRESH (g_P->in.LJ_THRESH)
+#define THRESH_DEAD (g_P->in.THRESH_DEAD)
+#define ca2p_init (g_P->in.ca2p_init)
+#define gj_init (g_P->in.gj_init)
+#define FAC_MAP (g_P->in.FAC_MAP)
+#define SW_THRESH (g_P->in.SW_THRESH)
+#define T_TURNOVER (g_P->in.T_TURNOVER)
+#define NUM_SC_INIT ((int)g_P->in.NUM_SC_INIT)
+#define P_MEMB (g_P->P_MEMB)
+#define DER_DER_CONST (g_P->in.DER_DER_CONST)
+#define K_DESMOSOME_RATIO (g_P->in.K_DESMOSOME_RATIO)
+#define K_DESMOSOME (g_mp.k_adh * K_DESMOSOME_RATIO)
+#define Kspring_d (g_P->in.Kspring_d)
+#define delta_R (g_P->delta_R)
+#define para_ljp2 (g_P->in.para_ljp2)
+#define Kspring_division (g_P->in.Kspring_division)
+#define agki_max (g_P->in.agki_max)
+#define agki_max_fix (g_P->agki_max_fix)
+#define div_max ((int)g_P->in.div_max)
+/* the shape of the build */
 #define stoch_corr_coef 1.0           /* define.h:195 */
+#define MEMB_ADJ_CONN_NUM 6           /* define.h:147-153 (TRI_MEMB) */
 #define MAX_CONNECT_CELL_NUM 400      /* define.h:80, cell_connection.cpp:30 */
 #define N3 200                        /* cell_connection.cpp:28 */
 #define AREA_GRID_ORIGINAL 2.0        /* cell_connection.cpp:16 */
 #define AREA_GRID (AREA_GRID_ORIGINAL + 1e-7) /* cell_connection.cpp:19 */
-
-/* cell_interaction.cpp:21-45 */
-/* eps_m, K_TOTAL, Kspring and KBEND are runtime values (g_mp) so that a
- * mechanics window can vary them; these are their reference values */
-#define eps_m_ref 0.01
-#define P_MEMB (1. / COMPRESS_FACTOR)
-#define DER_DER_CONST 0.2
-#define K_TOTAL_ref 3.0
-#define K_DESMOSOME_RATIO 0.01
-#define K_DESMOSOME (g_mp.k_adh * K_DESMOSOME_RATIO)
-#define Kspring_ref 25.0
-#define Kspring_d 5.0
-#define KBEND_ref 0.5
-#define delta_R (0.4 * R_der)
-#define para_ljp2 0.005
-#define Kspring_division 5.0
-
-/* cell.h:23-32 */
-#define agki_max 6.0
-#define fac_fix 1.0
-#define agki_max_fix (fac_fix * agki_max)
-#define div_max 15
 
 #define POW6(x) ((x) * (x) * (x) * (x) * (x) * (x))
 #define POW3(x) ((x) * (x) * (x))
@@ -109,7 +225,10 @@ static inline void add_d(double* d, double v) {
 enum { LAW_NONE = 0, LAW_OTHER, LAW_DER_DER, LAW_FIXMU, LAW_FIX_MEMB,
        LAW_MU_MEMB, LAW_ALAIRDE, LAW_MEMB_MEMB, LAW_PAIR };
 
-static eo_mech_params g_mp = {eps_m_ref, K_TOTAL_ref, Kspring_ref, KBEND_ref};
+/* eps_m, K_TOTAL, Kspring and KBEND as the force laws use them: the
+ * tissue's parameters, or a mechanics window's while one runs (g_window) */
+static eo_mech_params g_mp;
+static int g_window;
 #define eps_m (g_mp.eps)
 #define K_TOTAL (g_mp.k_adh)
 #define Kspring (g_mp.k_spring)
@@ -153,10 +272,21 @@ struct eo_tissue {
      * cells, nstat records; current_timestep (cellmanager.h:105) */
     int64_t* cst; int64_t* stat; int nstat;
     int64_t timestep;
+    eo_prm P;                       /* runtime parameters */
     const double* rng; int64_t rng_n, rng_pos;
     /* periodic index tables, utils.cpp:32-72 */
     int *px_prev, *px_next, *py_prev, *py_next, *pz_prev, *pz_next;
 };
+
+/* the record the parameter names read, and the mechanics coefficients of
+ * the tissue unless a window runs */
+static void use(const eo_tissue* t) {
+    g_P = &t->P;
+    if (!g_window) mp_from(&t->P, &g_mp);
+}
+
+void eo_set_params(eo_tissue* t, const epi_params* p) { derive(&t->P, p); }
+void eo_get_params(const eo_tissue* t, epi_params* p) { *p = t->P.in; }
 
 #define IDX3(t, j, k, l) (((size_t)(j) * ((t)->NY + 1) + (size_t)(k)) * ((t)->NZ + 1) + (size_t)(l))
 
@@ -196,6 +326,12 @@ static void* xcalloc(size_t n, size_t s) { void* p = calloc(n ? n : 1, s); retur
 eo_tissue* eo_create(const eo_geometry* g) {
     eo_tissue* t = (eo_tissue*)xcalloc(1, sizeof(eo_tissue));
     t->g = *g;
+    {
+        epi_params d;
+        eo_params_default(&d);
+        d.LX = g->lx; d.LY = g->ly; d.LZ = g->lz; d.NX = g->nx; d.NY = g->ny; d.NZ = g->nz;
+        derive(&t->P, &d);
+    }
     t->NX = g->nx; t->NY = g->ny; t->NZ = g->nz;
     /* define.h:111-113 */
     t->dx = g->lx / g->nx; t->inv_dx = g->nx / g->lx;
@@ -288,6 +424,7 @@ int eo_create_cell(eo_tissue* t, int state, int fix_origin, double x, double y,
                    double ip3, double ex_inert, double agek, double ageb,
                    double ex_fat, double in_fat, double spr_nat_len,
                    int rest_div_times, int is_malignant) {
+    use(t);
     if (t->n >= t->cap) { set_err(t, EO_ERR_CAPACITY); return -1; }
     int i = t->n++;
     t->state[i] = (uint8_t)state;
@@ -311,6 +448,7 @@ int eo_create_cell(eo_tissue* t, int state, int fix_origin, double x, double y,
 }
 
 int eo_link_pair(eo_tissue* t, int a, int b) {
+    use(t);
     if (a < 0 || b < 0 || a >= t->n || b >= t->n) return EO_ERR_INVALID;
     t->pair[a] = b; t->pair[b] = a;
     return EO_OK;
@@ -318,6 +456,7 @@ int eo_link_pair(eo_tissue* t, int a, int b) {
 
 /* _memb_init, cellmanager.cpp:66-113 (TRI_MEMB branch) */
 int eo_finish_load(eo_tissue* t, int nmemb, int nder) {
+    use(t);
     const int NMX = t->g.nmx;
     if (nmemb != t->g.nmx * t->g.nmy || (t->g.nmy % 2) != 0) return EO_ERR_INVALID;
     t->nmemb = nmemb; t->nder = nder;
@@ -368,11 +507,12 @@ static gj_ent* gj_get(gj_map* m, int64_t uid) {
 
 /* _cman_value_init, cell_init.cpp:18-40 */
 void eo_value_init(eo_tissue* t) {
+    use(t);
     for (int i = 0; i < t->n; i++) {
         t->ca_buf[0][i] = t->ca_buf[1][i] = ca2p_init;
         t->ca2p_avg[i] = t->ca_buf[t->ca_first][i];
-        t->ex_inert[i] = 0.97; /* ex_inert_init, define.h:168 */
-        t->ip3_buf[0][i] = t->ip3_buf[1][i] = 0.0; /* IP3_init */
+        t->ex_inert[i] = g_P->in.ex_inert_init;
+        t->ip3_buf[0][i] = t->ip3_buf[1][i] = g_P->in.IP3_init;
         for (int k = 0; k < t->nconn[i]; k++) {
             int64_t u = t->uid[t->conn[(size_t)i * MAX_CONNECT_CELL_NUM + k]];
             if (!gj_find(&t->gj[i], u)) { gj_ent* e = gj_get(&t->gj[i], u); e->v = gj_init; }
@@ -568,6 +708,7 @@ static void set_dermis(eo_tissue* t) {
 }
 
 void eo_connect_cell(eo_tissue* t) {
+    use(t);
     grid_init(t);
     connect_proc(t);
     gj_refresh(t);
@@ -670,7 +811,10 @@ static double coef_law(eo_tissue* t, int law, int c1, int c2) {
     }
 }
 
-double eo_coef(eo_tissue* t, int law, int c1, int c2) { return coef_law(t, law, c1, c2); }
+double eo_coef(eo_tissue* t, int law, int c1, int c2) {
+    use(t);
+    return coef_law(t, law, c1, c2);
+}
 
 /* cell_interaction_apply, cell_interaction.cpp:48-66 */
 static int g_tape_on;
@@ -760,6 +904,7 @@ static v3 tri_memb_dr(const eo_tissue* t, int m) {
     return dr;
 }
 void eo_tri_bend(const eo_tissue* t, int m, double* out3) {
+    use(t);
     v3 dr = tri_memb_dr(t, m);
     out3[0] = dr.x; out3[1] = dr.y; out3[2] = dr.z;
 }
@@ -767,6 +912,7 @@ void eo_tri_bend(const eo_tissue* t, int m, double* out3) {
 static inline int collide_with_wall(const eo_tissue* t, int c) { return t->z1[c] < t->radius[c]; }
 
 void eo_cell_interaction(eo_tissue* t) {
+    use(t);
     /* memb_bend, :501-519 */
     EO_PAR
     for (int m = 0; m < t->nmemb; m++) {
@@ -860,22 +1006,22 @@ void eo_pos_copy(eo_tissue* t) {
 }
 
 /* ---- cell_state_renew: cell_state_renew.cpp:17-443 --------------------- */
-#define stoch_div_time_ratio 0.25
-#define S2 0.1
-#define eps_kb 0.12
-#define accel_div 1.0
-#define alpha_b 5.0
-#define S0 (0.1 * 0.2)
-#define eps_ks (0.10 * 0.5)
-#define accel_diff 1.0
-#define alpha_k 2.0
-#define eps_kk (0.10 * 0.5)
-#define S1 0.1
-#define ubar 0.45
-#define delta_lipid 0.1
-#define delta_sig_r1 0.1
-#define lipid_rel (0.05 * 4.0)
-#define lipid_c 0.6
+#define stoch_div_time_ratio (g_P->in.stoch_div_time_ratio)
+#define S2 (g_P->in.S2)
+#define eps_kb (g_P->in.eps_kb)
+#define accel_div (g_P->in.accel_div)
+#define alpha_b (g_P->in.alpha_b)
+#define S0 (g_P->in.S0)
+#define eps_ks (g_P->in.eps_ks)
+#define accel_diff (g_P->in.accel_diff)
+#define alpha_k (g_P->in.alpha_k)
+#define eps_kk (g_P->in.eps_kk)
+#define S1 (g_P->in.S1)
+#define ubar (g_P->in.ubar)
+#define delta_lipid (g_P->in.delta_lipid)
+#define delta_sig_r1 (g_P->in.delta_sig_r1)
+#define lipid_rel (g_P->in.lipid_rel)
+#define lipid_c (g_P->in.lipid)
 
 static inline double weighted_eps_kb(const eo_tissue* t, int c) { return t->is_malignant[c] ? accel_div * eps_kb : eps_kb; }
 static inline double ageb_const(const eo_tissue* t, int c) {
@@ -950,7 +1096,7 @@ static void div_direction(eo_tissue* t, int me, int dermis, double* outx, double
 
 /* cell_divide, :227-270 */
 static void cell_divide(eo_tissue* t, int div) {
-    const double delta_L = 0.01 * R_max;
+    const double delta_L = g_P->delta_L;   /* :231, delta_L_coef * R_max (CalcParams.cpp:41) */
     int d = eo_create_cell(t, EO_MUSUME, t->fix_origin[div], t->x1[div], t->y1[div], t->z1[div],
                            t->radius[div], t->ca_buf[t->ca_first][div], t->ca_buf[t->ca_first][div],
                            t->ip3_buf[t->ca_first][div], t->ex_inert[div], 0, 0, 0, 0, delta_L,
@@ -1015,7 +1161,7 @@ static void remove_exec(eo_tissue* t) {
 
 /* pair_disperse, :364-385 */
 static void pair_disperse(eo_tissue* t, int c) {
-    const double eps_L = 0.14, unpair_dist_coef = 0.9;
+    const double eps_L = g_P->in.eps_L, unpair_dist_coef = g_P->in.unpair_dist_coef;
     int p = t->pair[c];
     double rad_sum = t->radius[c] + t->radius[p];
     double unpair_th = unpair_dist_coef * rad_sum;
@@ -1032,6 +1178,7 @@ static void pair_disperse(eo_tissue* t, int c) {
 }
 
 void eo_cell_state_renew(eo_tissue* t) {
+    use(t);
     const int start = t->nmemb + t->nder, end = t->n;  /* bounds captured at call */
     EO_PAR_DYN
     for (int c = start; c < end; c++) {
@@ -1053,7 +1200,7 @@ void eo_cell_state_renew(eo_tissue* t) {
             }
             break;
         case EO_DEAD: case EO_AIR: /* :330-341 */
-            if (t->agek[c] >= 31.3 && t->nconn[c] <= 11) {
+            if (t->agek[c] >= g_P->in.ADHE_CONST && (double)t->nconn[c] <= g_P->in.DISA_conn_num_thresh) {
 #ifdef EO_OMP
                 t->rq[__atomic_fetch_add(&t->nrq, 1, __ATOMIC_RELAXED)] = c;
 #else
@@ -1084,6 +1231,7 @@ void eo_cell_state_renew(eo_tissue* t) {
 
 /* initialize_sc, :435-443 */
 void eo_initialize_sc(eo_tissue* t, double zzmax) {
+    use(t);
     for (int c = t->nmemb + t->nder; c < t->n; c++) {
         if (t->state[c] == EO_ALIVE && zzmax - t->z1[c] < 8 * R_max) {
             t->agek[c] = t->agek[c] > THRESH_DEAD ? t->agek[c] : THRESH_DEAD;
@@ -1094,6 +1242,7 @@ void eo_initialize_sc(eo_tissue* t, double zzmax) {
 
 /* cell_pos_periodic_fix, cellmanager.cpp:395-414 */
 void eo_cell_pos_periodic_fix(eo_tissue* t) {
+    use(t);
     const double LX = t->g.lx, LY = t->g.ly;
     EO_PAR
     for (int c = 0; c < t->n; c++) {
@@ -1106,6 +1255,7 @@ void eo_cell_pos_periodic_fix(eo_tissue* t) {
 
 /* calc_zzmax, proc.cpp:34-43 */
 double eo_calc_zzmax(eo_tissue* t) {
+    use(t);
     double zmax = 0;
     for (int c = t->nmemb + t->nder; c < t->n; c++) {
         int st = t->state[c];
@@ -1181,6 +1331,7 @@ static void stamp_cell(eo_tissue* t, int c, int irx, int iry, int irz, double* d
 }
 
 void eo_setup_map_lat(eo_tissue* t) {
+    use(t);
     const double Rad = 1.4;
     const int NX = t->NX, NY = t->NY, NZ = t->NZ;
     const int irx = (int)(Rad * NX / t->g.lx);
@@ -1222,12 +1373,13 @@ void eo_setup_map_lat(eo_tissue* t) {
 
 /* ---- calc_ext_stim: ext_stim.cpp:9-69 ---------------------------------- */
 static inline double fB(double age, double B, int cornif) {
-    const double kb = 0.025, DUR_ALIVE = 0.5, DUR_DEAD = 2.0;
+    const double kb = g_P->in.kb, DUR_ALIVE = g_P->in.DUR_ALIVE, DUR_DEAD = g_P->in.DUR_DEAD;
     return (cornif && age > THRESH_DEAD - DUR_ALIVE && age <= THRESH_DEAD + DUR_DEAD ? 1 : 0) - kb * B;
 }
 
 void eo_calc_ext_stim(eo_tissue* t, double zzmax) {
-    const double DB = 0.0009;
+    use(t);
+    const double DB = g_P->in.DB;
     const int NX = t->NX, NY = t->NY, NZ = t->NZ;
     const int iz_bound = (int)((zzmax + FAC_MAP * R_max) * t->inv_dz);
     if (iz_bound > NZ || iz_bound < 0) { set_err(t, EO_ERR_OUT_OF_GRID); return; }
@@ -1271,53 +1423,53 @@ void eo_calc_ext_stim(eo_tissue* t, double zzmax) {
 }
 
 /* ---- calc_ca2p: ca2p.cpp:14-402 ---------------------------------------- */
-#define Kpp 0.3
-#define dp_c 0.1
+#define Kpp (g_P->in.Kpp)
+#define dp_c (g_P->in.dp)
 
 static inline double ca2p_by_ext_stim(double ext_stim) {
-    const double kbc = 0.4 * 1.2, Hb = 0.01, Cout = 1.0;
+    const double kbc = g_P->in.kbc, Hb = g_P->in.Hb, Cout = g_P->in.Cout;
     return kbc * ext_stim * ext_stim * Cout / (Hb + ext_stim * ext_stim);
 }
 static inline double ca2p_into_storage(double ca2p_in_cell) {
-    const double kg = 0.1, gamma = 2.0;
-    return gamma * ca2p_in_cell / (kg + ca2p_in_cell);
+    const double kg = g_P->in.kg, gam = g_P->in.gamma;
+    return gam * ca2p_in_cell / (kg + ca2p_in_cell);
 }
 static inline double ER_domain1_active(double ip3) {
-    const double mu0 = 0.567, mu1 = 0.1, kmu = 0.05;
+    const double mu0 = g_P->in.mu0, mu1 = g_P->in.mu1, kmu = g_P->in.kmu;
     return (mu0 + mu1 * ip3 / (ip3 + kmu));
 }
 static inline double ER_domain2_active(double ca2p) {
-    const double para_b = 0.11, para_bb = 0.89, para_k1 = 0.7;
+    const double para_b = g_P->in.para_b, para_bb = g_P->in.para_bb, para_k1 = g_P->in.para_k1;
     return (para_b + para_bb * ca2p / (para_k1 + ca2p));
 }
 static inline double ca2p_reaction_factor(double u, double ER_domain3_deactive, double p, double B) {
-    const double k_flux = 8.1, beta_zero = 0.02, CA_OUT = 1.0;
-    const double leak_from_storage = CA_OUT * beta_zero;
+    const double k_flux = g_P->in.k_flux;
+    const double leak_from_storage = g_P->leak_from_storage;
     double ca2p_from_storage = k_flux * ER_domain1_active(p) * ER_domain2_active(u) * ER_domain3_deactive;
     return ca2p_from_storage - ca2p_into_storage(u) + leak_from_storage + ca2p_by_ext_stim(B);
 }
 static inline double calc_th(int is_alive, double agek) {
-    const double thgra = 0.2, delta_th = 1.0, thpri = 1.0;
+    const double thgra = g_P->in.thgra, delta_th = g_P->in.delta_th, thpri = g_P->in.thpri;
     return is_alive ? thgra + ((thpri - thgra) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / delta_th)) : thpri;
 }
 static inline double calc_Kpa(int is_alive, double agek) {
-    const double kpa = 4.0, Kgra = kpa, delta_K = 1.0, Kpri = 6.0;
+    const double kpa = g_P->in.kpa, Kgra = kpa, delta_K = g_P->in.delta_K, Kpri = g_P->in.Kpri;
     return is_alive ? Kgra + ((Kpri - Kgra) * 0.5) * (1.0 + tanh((THRESH_SP - agek) / delta_K)) : Kpri;
 }
 static inline double calc_IAG(int is_alive, double agek) {
-    const double delta_I = 1.5, iage_kitei = 0;
+    const double delta_I = g_P->in.delta_I, iage_kitei = g_P->in.iage_kitei;
     return is_alive ? 0.5 * (1.0 + tanh((agek - THRESH_SP) / delta_I)) : iage_kitei;
 }
 static inline double ex_inert_diff(double ca2p, double current_ex_inert, double _th) {
-    const double para_k2 = 0.7;
+    const double para_k2 = g_P->in.para_k2;
     return ((para_k2 * para_k2 / (para_k2 * para_k2 + ca2p * ca2p) - current_ex_inert) / _th);
 }
 static inline double IP3_default_diff(double _Kpa, double a_avg, double current_IP3) {
-    const double H0 = 0.5;
+    const double H0 = g_P->in.H0;
     return (_Kpa * a_avg / (H0 + a_avg) - Kpp * current_IP3);
 }
 static inline double fw(double diff, double w) {
-    const double wd = 0.1;
+    const double wd = g_P->in.wd;
     return (1. - w) + (-1. + tanh((wd - diff) / 0.1)) / 2.;
 }
 static inline double grid_avg8(const eo_tissue* t, const double* g, int ix, int iy, int iz) {
@@ -1326,7 +1478,7 @@ static inline double grid_avg8(const eo_tissue* t, const double* g, int ix, int 
                     + g[IDX3(t, ix, iy + 1, iz + 1)] + g[IDX3(t, ix + 1, iy + 1, iz + 1)]);
 }
 static inline double fa(double diffu, double A) {
-    const double STIM11 = 0.002, Kaa = 0.5;
+    const double STIM11 = g_P->in.STIM11, Kaa = g_P->in.Kaa;
     return STIM11 * min0(diffu) - A * Kaa;
 }
 static inline int is_afm(int st) { return st == EO_ALIVE || st == EO_FIX || st == EO_MUSUME; }
@@ -1377,7 +1529,7 @@ static void dead_IP3_calc(eo_tissue* t) {
 
 /* supra_calc :182-222 */
 static void supra_calc(eo_tissue* t) {
-    const double ca2p_du = 0.01;
+    const double ca2p_du = g_P->in.ca2p_du;
     const double* ca = t->ca_buf[t->ca_first];
     double* can = t->ca_buf[1 - t->ca_first];
     const double* ip3 = t->ip3_buf[t->ca_first];
@@ -1418,7 +1570,7 @@ static void supra_calc(eo_tissue* t) {
 
 /* ATP_refresh :302-338 */
 static void ATP_refresh(eo_tissue* t, const ca_maps* m) {
-    const double Da = 1.0, AIR_STIM = 0.1;
+    const double Da = g_P->in.Da, AIR_STIM = g_P->in.AIR_STIM;
     const int NX = t->NX, NY = t->NY, iz_bound = m->iz_bound;
     const double* carr = t->atp[t->atp_first];
     double* narr = t->atp[1 - t->atp_first];
@@ -1467,6 +1619,7 @@ static int ca_iz_bound(const eo_tissue* t, double zzmax) {
 static ca_maps* g_maps_cache = NULL;  /* not used: maps are per call */
 
 void eo_ca_substep(eo_tissue* t, int iz_bound) {
+    use(t);
     if (iz_bound > t->NZ || iz_bound < 0) { set_err(t, EO_ERR_OUT_OF_GRID); return; }
     ca_maps m;
     m.iz_bound = iz_bound;
@@ -1483,15 +1636,19 @@ void eo_ca_substep(eo_tissue* t, int iz_bound) {
 static void calc_ca2p_n(eo_tissue* t, double zzmax, unsigned n_sub);
 
 void eo_calc_ca2p(eo_tissue* t, double zzmax) {
-    calc_ca2p_n(t, zzmax, (int)(Ca_avg_time / DT_Ca));
+    use(t);
+    calc_ca2p_n(t, zzmax, (unsigned)g_P->Ca_ITR);
 }
 
 /* calc_ca2p with a given number of sub-steps (timing samples only) */
-void eo_calc_ca2p_n(eo_tissue* t, double zzmax, int n_sub) { calc_ca2p_n(t, zzmax, (unsigned)n_sub); }
+void eo_calc_ca2p_n(eo_tissue* t, double zzmax, int n_sub) {
+    use(t);
+    calc_ca2p_n(t, zzmax, (unsigned)n_sub);
+}
 
 static void calc_ca2p_n(eo_tissue* t, double zzmax, unsigned n_sub) {
     (void)g_maps_cache;
-    const unsigned Ca_ITR = (int)(Ca_avg_time / DT_Ca);
+    const unsigned Ca_ITR = (unsigned)g_P->Ca_ITR;
     /* init_ca2p_avg :20-27 */
     for (int c = t->nmemb + t->nder; c < t->n; c++) if (is_afm(t->state[c])) t->ca2p_avg[c] = 0;
     ca_maps m;
@@ -1514,11 +1671,12 @@ static void calc_ca2p_n(eo_tissue* t, double zzmax, unsigned n_sub) {
     free(m.diffu_cell); free(m.air);
 }
 
-int eo_should_calc_ca(const eo_tissue* t) { return t->sw >= SW_THRESH; }
+int eo_should_calc_ca(const eo_tissue* t) { return t->sw >= prm_sw_thresh(&t->P); }
 void eo_ca_calc_condition_reset(eo_tissue* t) { t->sw = 0; }
 
 /* CellManager::clean_up, cellmanager.cpp:543-557 */
 void eo_clean_up(eo_tissue* t) {
+    use(t);
     for (int c = 0; c < t->n; c++) {
         gj_map* m = &t->gj[c];
         int w = 0;
@@ -1529,6 +1687,7 @@ void eo_clean_up(eo_tissue* t) {
 
 /* proc.cpp:131-173 loop body; cell_dynamics :22-32 */
 int eo_step(eo_tissue* t, eo_driver* d) {
+    use(t);
     t->timestep = d->step;   /* cman.current_timestep = i (proc.cpp:132) */
     if (d->clean_every > 0 && d->step % d->clean_every == 0 && d->step > 0) eo_clean_up(t);
     eo_cell_interaction(t);
@@ -1614,6 +1773,7 @@ int eo_get_cells(const eo_tissue* t, eo_cells* o) {
 
 int eo_set_cell_values(eo_tissue* t, const double* ca2p, const double* ip3,
                        const double* ex_inert, const double* ca2p_avg, const double* gj) {
+    use(t);
     for (int c = 0; c < t->n; c++) {
         if (ca2p) t->ca_buf[0][c] = t->ca_buf[1][c] = ca2p[c];
         if (ip3) t->ip3_buf[0][c] = t->ip3_buf[1][c] = ip3[c];
@@ -1643,6 +1803,7 @@ int eo_get_fields(const eo_tissue* t, double* atp, double* atp_next, double* b,
 
 int eo_set_fields(eo_tissue* t, const double* atp, const double* atp_next,
                   const double* b, const double* b_next) {
+    use(t);
     size_t s = t->gsz;
     if (atp) memcpy(t->atp[t->atp_first], atp, 8 * s);
     if (atp_next) memcpy(t->atp[1 - t->atp_first], atp_next, 8 * s);
@@ -1652,6 +1813,7 @@ int eo_set_fields(eo_tissue* t, const double* atp, const double* atp_next,
 }
 
 int eo_set_maps(eo_tissue* t, const int32_t* cmap1, const uint8_t* cmap2, const int32_t* lat) {
+    use(t);
     if (cmap1) memcpy(t->cmap1, cmap1, 4 * t->gsz);
     if (cmap2) memcpy(t->cmap2, cmap2, t->gsz);
     if (lat) memcpy(t->lat, lat, sizeof(int32_t) * 3 * (size_t)t->n);
@@ -1659,6 +1821,7 @@ int eo_set_maps(eo_tissue* t, const int32_t* cmap1, const uint8_t* cmap2, const 
 }
 
 int eo_set_positions(eo_tissue* t, const double* x, const double* y, const double* z) {
+    use(t);
     for (int c = 0; c < t->n; c++) {
         t->x1[c] = t->x2[c] = x[c]; t->y1[c] = t->y2[c] = y[c]; t->z1[c] = t->z2[c] = z[c];
     }
@@ -1666,6 +1829,7 @@ int eo_set_positions(eo_tissue* t, const double* x, const double* y, const doubl
 }
 
 int eo_set_state(eo_tissue* t, const uint8_t* state, const double* agek) {
+    use(t);
     for (int c = 0; c < t->n; c++) {
         if (state) t->state[c] = state[c];
         if (agek) t->agek[c] = agek[c];
@@ -1674,6 +1838,7 @@ int eo_set_state(eo_tissue* t, const uint8_t* state, const double* agek) {
 }
 
 int eo_set_lists(eo_tissue* t, const int32_t* conn_ptr, const int32_t* conn, const double* gj) {
+    use(t);
     for (int c = 0; c < t->n; c++) {
         int len = conn_ptr[c + 1] - conn_ptr[c];
         if (len > MAX_CONNECT_CELL_NUM) return EO_ERR_CAPACITY;
@@ -1819,7 +1984,7 @@ static void rd_substep(eo_tissue* t, const rd_win* w, const eo_rd_params* P, rd_
         s->h[c] += DT_Ca * ex_inert_diff(u[c], s->h[c], w->th[c]);
     }
     /* ATP_refresh */
-    const double Da = 1.0, AIR_STIM = 0.1;
+    const double Da = g_P->in.Da, AIR_STIM = g_P->in.AIR_STIM;
     const uint8_t* cm2 = t->cmap2;
     const double inv_dx = t->inv_dx;
     for (int j = 0; j < w->NX; j++) {
@@ -1850,6 +2015,7 @@ static void rd_substep(eo_tissue* t, const rd_win* w, const eo_rd_params* P, rd_
 }
 
 int eo_rd_forward(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, double* ca2p_out) {
+    use(t);
     rd_win w; rd_state s;
     int rc = rd_setup(t, zzmax, &w);
     if (rc) { rd_free(&w); return rc; }
@@ -1872,6 +2038,7 @@ static inline long rd_corner_var(const eo_tissue* t, const rd_win* w, int j, int
 
 int eo_rd_adjoint(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, const double* target,
                   double* loss, eo_rd_grad* G) {
+    use(t);
     rd_win w; rd_state s;
     int rc = rd_setup(t, zzmax, &w);
     if (rc) { rd_free(&w); return rc; }
@@ -1908,9 +2075,10 @@ int eo_rd_adjoint(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, 
     *loss = L / nafm;
     G->du = G->dp = G->gj0 = 0;
     if (G->kpa) for (int c = 0; c < n; c++) G->kpa[c] = 0;
-    const double Da = 1.0, Kaa = 0.5, STIM11 = 0.002, H0 = 0.5;
-    const double kflux = 8.1, mu0 = 0.567, mu1 = 0.1, kmu = 0.05, pb = 0.11, pbb = 0.89, pk1 = 0.7;
-    const double kg = 0.1, gam = 2.0, k2 = 0.7;
+    const double Da = g_P->in.Da, Kaa = g_P->in.Kaa, STIM11 = g_P->in.STIM11, H0 = g_P->in.H0;
+    const double kflux = g_P->in.k_flux, mu0 = g_P->in.mu0, mu1 = g_P->in.mu1, kmu = g_P->in.kmu,
+                 pb = g_P->in.para_b, pbb = g_P->in.para_bb, pk1 = g_P->in.para_k1;
+    const double kg = g_P->in.kg, gam = g_P->in.gamma, k2 = g_P->in.para_k2, wd = g_P->in.wd;
     const double cD = DT_Ca * Da * t->inv_dx * t->inv_dx;
     const uint8_t* cm2 = t->cmap2;
     for (int i = n_sub - 1; i >= 0; i--) {
@@ -1964,7 +2132,7 @@ int eo_rd_adjoint(eo_tissue* t, double zzmax, const eo_rd_params* P, int n_sub, 
                 double lge = lg[e];
                 if (w.kind[e] == 3) {
                     const double d = u[o] - u[c];
-                    const double th_ = tanh((0.1 - fabs(d)) / 0.1);
+                    const double th_ = tanh((wd - fabs(d)) / 0.1);
                     const double fwd = -5.0 * (1.0 - th_ * th_);
                     const double sg = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
                     to_u += lg[e] * DT_Ca * fwd * sg;
@@ -2245,8 +2413,10 @@ static void tri_bend_2_bar(const eo_tissue* t, int r1, int a1, int b1, int c1, v
 }
 
 static int mech_window(eo_tissue* t, const eo_mech_params* p, int n_steps, double** keep) {
+    use(t);
     const eo_mech_params saved = g_mp;
     g_mp = *p;
+    g_window = 1;
     for (int s = 0; s < n_steps; s++) {
         if (keep) {
             memcpy(keep[s], t->x1, 8 * (size_t)t->n);
@@ -2257,6 +2427,7 @@ static int mech_window(eo_tissue* t, const eo_mech_params* p, int n_steps, doubl
         eo_pos_copy(t);
     }
     g_mp = saved;
+    g_window = 0;
     return t->err;
 }
 
@@ -2284,8 +2455,10 @@ int eo_mech_adjoint(eo_tissue* t, const eo_mech_params* p, int n_steps, const do
     /* the interactions of one step (the same every step) */
     g_tape_n = 0;
     g_tape_on = 1;
+    use(t);
     const eo_mech_params saved = g_mp;
     g_mp = *p;
+    g_window = 1;
     int rc = mech_window(t, p, n_steps, keep);
     g_tape_on = 0;
     memcpy(keep[n_steps], t->x1, 8 * (size_t)n); memcpy(keep[n_steps] + n, t->y1, 8 * (size_t)n);
@@ -2369,6 +2542,7 @@ int eo_mech_adjoint(eo_tissue* t, const eo_mech_params* p, int n_steps, const do
         double* tmp = lam; lam = nl; nl = tmp;
     }
     g_mp = saved;
+    g_window = 0;
     memcpy(t->x1, save, 8 * (size_t)n); memcpy(t->y1, save + n, 8 * (size_t)n); memcpy(t->z1, save + 2 * n, 8 * (size_t)n);
     memcpy(t->x2, save + 3 * n, 8 * (size_t)n); memcpy(t->y2, save + 4 * n, 8 * (size_t)n);
     memcpy(t->z2, save + 5 * n, 8 * (size_t)n);

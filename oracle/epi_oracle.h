@@ -20,6 +20,8 @@
 #define EPI_ORACLE_H
 #include <stdint.h>
 
+#include "../include/epi_amd.h"   /* epi_params, the runtime parameter record */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -42,7 +44,17 @@ typedef struct eo_geometry {
 
 typedef struct eo_tissue eo_tissue;
 
+/* a new tissue starts with epi5_MT's parameters and its geometry's grid and box */
 eo_tissue* eo_create(const eo_geometry* g);
+/* the runtime parameters: epi5_MT's values (define.h:80-230 and the physics
+ * files) and a tissue's record, with the derived quantities of
+ * CalcParams::calculate_param (epi_intg/calc/CalcParams.cpp:16-45) */
+void eo_params_default(epi_params* p);
+void eo_set_params(eo_tissue* t, const epi_params* p);
+void eo_get_params(const eo_tissue* t, epi_params* p);
+/* init_gen (epi_intg/calc/init_gen.cpp:23-106): the cells into o's arrays
+ * of cap entries; EO_ERR_CAPACITY when they do not fit (o->n set) */
+int eo_init_gen(const epi_params* p, int nfix, int der, int cap, epi_gen_cells* o);
 void eo_destroy(eo_tissue* t);
 int eo_error(const eo_tissue* t);
 void eo_clear_error(eo_tissue* t);

@@ -66,6 +66,69 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+HEADER = os.path.join(os.path.dirname(HERE), "include", "epi_amd.h")
+
+
+def header_param_fields(path=HEADER):
+    """the fields of `typedef struct epi_params` as the C header declares
+    them (the oracle's C code includes the same header): (name, ctypes type)"""
+    text = open(path).read()
+    a = text.index("typedef struct epi_params {") + len("typedef struct epi_params {")
+    body = text[a:text.index("} epi_params;")]
+    kinds = {"int32_t": C.c_int32, "double": C.c_double}
+    out = []
+    for stmt in body.split(";"):
+        stmt = " ".join(stmt.split())
+        if not stmt:
+            continue
+        ty, names = stmt.split(" ", 1)
+        for nm in names.split(","):
+            nm = nm.strip()
+            if "[" in nm:
+                base, n = nm.split("[")
+                out.append((base, C.c_char * int(n.rstrip("]"))))
+            else:
+                out.append((nm, kinds[ty]))
+    return out
+
+
+class Params(C.Structure):
+    _fields_ = header_param_fields()
+
+    def to_dict(self):
+        return {k: (getattr(self, k).decode() if k == "outputdir" else getattr(self, k)) for k, _ in self._fields_}
+
+
+PARAM_KEYS = [k for k, _ in Params._fields_]
+
+
+class GenCells(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nmemb", C.c_int32), ("nder", C.c_int32), ("nmx", C.c_int32),
+                ("nmy", C.c_int32), ("state", _u8p), ("fix_origin", _ip), ("x", _dp), ("y", _dp), ("z", _dp),
+                ("radius", _dp)]
+
+
+def init_gen(params, nfix=16, der=8):
+    """the oracle's init_gen (epi_intg/calc/init_gen.cpp:23-106): deck
+    columns and the sheet's (nmx, nmy)"""
+    L = lib()
+    p = Params()
+    C.memmove(C.byref(p), C.byref(params), C.sizeof(Params))
+    g = GenCells()
+    rc = L.eo_init_gen(C.byref(p), nfix, der, 0, C.byref(g))
+    n = g.n
+    cols = {"state": np.zeros(n, np.uint8), "fix_origin": np.zeros(n, np.int32)}
+    for k in ("x", "y", "z", "radius"):
+        cols[k] = np.zeros(n)
+    g = GenCells(0, 0, 0, 0, 0, cols["state"].ctypes.data_as(_u8p), cols["fix_origin"].ctypes.data_as(_ip),
+                 cols["x"].ctypes.data_as(_dp), cols["y"].ctypes.data_as(_dp), cols["z"].ctypes.data_as(_dp),
+                 cols["radius"].ctypes.data_as(_dp))
+    rc = L.eo_init_gen(C.byref(p), nfix, der, n, C.byref(g))
+    if rc:
+        raise RuntimeError(f"eo_init_gen: {rc}")
+    return cols, (g.nmx, g.nmy)
+
+
 _libs = {}
 
 
@@ -84,6 +147,13 @@ class RdGrad(C.Structure):
     _fields_ = [("du", C.c_double), ("dp", C.c_double), ("gj0", C.c_double), ("kpa", _dp)]
 
 
+def params_default():
+    """epi5_MT's values as the oracle restates them"""
+    p = Params()
+    lib().eo_params_default(C.byref(p))
+    return p
+
+
 def lib(mt=False):
     path = LIB_PATH_MT if mt else LIB_PATH
     if path not in _libs:
@@ -91,6 +161,13 @@ def lib(mt=False):
             build()
         L = C.CDLL(path)
         L.eo_create.restype = C.c_void_p
+        L.eo_params_default.argtypes = [C.POINTER(Params)]
+        L.eo_params_default.restype = None
+        L.eo_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        L.eo_set_params.restype = None
+        L.eo_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        L.eo_get_params.restype = None
+        L.eo_init_gen.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.eo_create.argtypes = [C.POINTER(Geo)]
         L.eo_destroy.argtypes = [C.c_void_p]
         L.eo_error.argtypes = [C.c_void_p]
@@ -250,6 +327,22 @@ class OracleTissue:
 
     # --- cell_stat (cellmanager.cpp:338-352, 559-567) ---
     def set_timestep(self, step): self.L.eo_set_timestep(self.h, int(step))
+
+    def get_params(self):
+        p = Params()
+        self.L.eo_get_params(self.h, C.byref(p))
+        return p
+
+    def set_params(self, p):
+        """a Params, or a dict of keys changed from the current ones"""
+        if isinstance(p, dict):
+            q = self.get_params()
+            for k, v in p.items():
+                if k not in PARAM_KEYS:
+                    raise KeyError(k)
+                setattr(q, k, v.encode() if k == "outputdir" else v)
+            p = q
+        self.L.eo_set_params(self.h, C.byref(p))
 
     def timestep(self): return int(self.L.eo_timestep(self.h))
 

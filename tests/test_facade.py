@@ -187,3 +187,42 @@ def test_last_data_round_trip(tmp_path):
     ta, tb = np.loadtxt(a / "last_data_cell", ndmin=2), np.loadtxt(b / "last_data_cell", ndmin=2)
     assert np.array_equal(ta[:, [0, 1, 10, 15, 16]], tb[:, [0, 1, 10, 15, 16]])
     assert np.allclose(ta, tb, rtol=0, atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_driver_runs_under_a_parameter_file(tmp_path):
+    """`epi_proc --params FILE` (a CalcParams key=value file, every key
+    present as Params::load requires) runs the loop under those values, the
+    same bits as the Python driver given the same record; `--params-out`
+    writes the record back"""
+    from epi_amd import native
+    assert os.path.exists(DRIVER), "epi_amd/host/epi_proc is not built"
+    spec0 = configs.config_tiny(7, 160)
+    g = spec0["geometry"]
+    p = tmp_path / "deck.txt"
+    deck.write_deck(spec0, p)
+    prm = native.params_default().replace(LX=g.lx, LY=g.ly, LZ=g.lz, NX=g.nx, NY=g.ny, NZ=g.nz, DT_Cell=0.0105,
+                                          K_TOTAL=3.2, kpa=4.3, SW_THRESH=10, Ca_avg_time=3.09, eps_kb=0.13)
+    pf, pf2 = tmp_path / "prm", tmp_path / "prm_out"
+    native.params_save(str(pf), prm)
+    steps, ca_every, seed = 40, 20, 11
+    dump = tmp_path / "dump.bin"
+    print(f"facade driver under a parameter file: tiny deck, {steps} steps", flush=True)
+    r = subprocess.run([DRIVER, str(p), str(steps), "--seed", str(seed), "--ca-every", str(ca_every), "--dump", str(dump),
+                        "--params", str(pf), "--params-out", str(pf2), *_geometry_args(g)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert native.params_load(str(pf2)).to_dict() == prm.to_dict()
+    raw = np.fromfile(dump, dtype=np.float64)
+    n = int(raw[0])
+    x, y, z, ca2p, agek = raw[1:].reshape(5, n)
+    spec = deck.spec_from_columns(deck.parse_deck(p), g)
+    t = native.Tissue(g)
+    t.set_params(prm)
+    t.load(spec, value_init=True)
+    t.set_rng(native.mt19937_64_uniform(seed, 200000))
+    t.step(native.Driver(0, 0, 0, ca_every, 0), steps)
+    assert t.error() == 0
+    c = t.get_cells()
+    for k, v in (("x", x), ("y", y), ("z", z), ("ca2p", ca2p), ("agek", agek)):
+        assert np.array_equal(c[k], v), k
