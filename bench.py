@@ -268,6 +268,43 @@ def roofline(ts, prof, s0, s1, rk, launched, workload):
     return roof
 
 
+def stencil_alone(ts, reps=20):
+    """The B stencil (k_ext_stim) over the whole context on its own, after the
+    timed region: in a step it runs on the side stream beside the next step's
+    forces, which share the GPU with it, so its event span there is not its
+    own speed.  Algorithmic bytes (21 per point below iz_bound) over the mean
+    launch duration."""
+    zz = ts.calc_zzmax()
+    ts.calc_ext_stim(zz)
+    ts.synchronize()
+    a = [ts.status(t)[0] for t in range(ts.T)]
+    ts.profile_reset()
+    ts.profile_sample(1)
+    ts.profile_enable(["k_ext_stim"])
+    for _ in range(reps):
+        ts.calc_ext_stim(zz)
+    ts.synchronize()
+    prof = ts.profile()
+    ts.profile_enable([])
+    b = [ts.status(t)[0] for t in range(ts.T)]
+    if "k_ext_stim" not in prof:
+        return None
+    ms, cnt = prof["k_ext_stim"]
+    avg_s = ms / cnt / 1e3
+    pts = sum(y.grid_points_b - x.grid_points_b for x, y in zip(a, b)) / reps
+    alg = BYTES_PER_POINT["k_ext_stim"] * pts
+    traffic = None
+    pmc = os.path.join(HERE, "profiles", "k_ext_stim_batched_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    achieved = alg / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": "k_ext_stim",
+            "avg_us": round(avg_s * 1e6, 3), "launches": reps, "units_per_launch": f"{pts:.0f} grid points",
+            "algorithmic_bytes_per_launch": round(alg, 1)}
+
+
 def make_spec(name):
     from epi_amd import configs
     return configs.config_large(2) if name == "large" else configs.config_small(1)
@@ -428,6 +465,7 @@ def run_batched(args):
     dominant, kernels = dominant_kernel(prof, launched)
     rk = dominant if args.roofline_kernel == "auto" else args.roofline_kernel
     roof = roofline(ts, prof, s0, s1, rk, launched.get(rk, 0), "batched")
+    b_alone = stencil_alone(ts)
     ts.close()
     # per-instance loop over an evenly spaced sample of the same tissues
     idx = sorted(set(int(round(i * (len(specs) - 1) / max(args.loop_sample - 1, 1)))
@@ -474,6 +512,7 @@ def run_batched(args):
                    "autotune_seconds": round(tune_s, 2)},
         "grid_cell_steps_per_s": round(grid_pts / wall, 1),
         "roofline": roof,
+        "b_stencil_alone": b_alone,
         "kernel_mix": kernels,
         "per_instance_loop": {"value": round(loop_value, 1), "unit": "cell-steps/s",
                               "tissues_timed": len(idx), "steps": args.steps,
@@ -499,10 +538,13 @@ def copy_calibration(mib):
     return 0
 
 
-def _pmc_per_dispatch(d, counter, match):
+def _pmc_per_dispatch(d, counter, match, largest_grid=False):
+    """per-dispatch values of `counter` for kernels whose name contains
+    `match`, in dispatch order; with largest_grid only the dispatches of the
+    largest grid (a batch's launches, not a single tissue's)"""
     import csv
     import glob
-    vals = {}
+    vals, grid = {}, {}
     for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         with open(f) as fh:
             for row in csv.DictReader(fh):
@@ -510,19 +552,24 @@ def _pmc_per_dispatch(d, counter, match):
                     continue
                 key = (f, int(row.get("Dispatch_Id", 0)))
                 vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+                grid[key] = int(row.get("Grid_Size") or 0)
+    if largest_grid and grid:
+        top = max(grid.values())
+        vals = {k: v for k, v in vals.items() if grid[k] == top}
     return [v for _, v in sorted(vals.items(), key=lambda kv: kv[0][1])]
 
 
 def traffic_summary(kernel, fdir, wdir, cfdir, cwdir, copy_mib=512):
     """HBM bytes per launch of `kernel` from the FETCH_SIZE / WRITE_SIZE runs
-    (KiB per dispatch), each scaled by what the counter reads for a copy of
-    copy_mib MiB (the largest dispatches of the calibration run)."""
+    (KiB per dispatch, over the kernel's dispatches of the largest grid), each
+    scaled by what the counter reads for a copy of copy_mib MiB (the largest
+    dispatches of the calibration run)."""
     import statistics
     copy_bytes = copy_mib * 1024 * 1024
     cf = statistics.median(sorted(_pmc_per_dispatch(cfdir, "FETCH_SIZE", ""))[-5:]) * 1024
     cw = statistics.median(sorted(_pmc_per_dispatch(cwdir, "WRITE_SIZE", ""))[-5:]) * 1024
-    fetch = _pmc_per_dispatch(fdir, "FETCH_SIZE", kernel)
-    write = _pmc_per_dispatch(wdir, "WRITE_SIZE", kernel)
+    fetch = _pmc_per_dispatch(fdir, "FETCH_SIZE", kernel, largest_grid=True)
+    write = _pmc_per_dispatch(wdir, "WRITE_SIZE", kernel, largest_grid=True)
     n = min(len(fetch), len(write), 1000)
     fk, wk = statistics.mean(fetch[-n:]), statistics.mean(write[-n:])
     return {"kernel": kernel, "dispatches": n, "fetch_kib_per_dispatch": fk, "write_kib_per_dispatch": wk,

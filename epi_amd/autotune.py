@@ -27,7 +27,7 @@ import itertools
 import json
 import time
 
-STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8), (1, 2, 4)))
+STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8, 16), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
 DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0, "map_lanes": 1,
            "b_xcd": 0}
@@ -72,7 +72,7 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
         for _ in range(b_reps):
             ts.setup_map_lat()
 
-    for ml in (1, 4):
+    for ml in (1, 4, 16):
         ts.set_tuning({"map_lanes": ml})
         report["k_map_stamp"][f"lanes{ml}"] = _time(ts, "k_map_stamp", run_map)
     best["map_lanes"] = int(min(report["k_map_stamp"], key=report["k_map_stamp"].get)[5:])

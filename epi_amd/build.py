@@ -55,7 +55,7 @@ DRIVER = os.path.join(HERE, "host", "epi_proc")
 
 def build_driver():
     """The C++ driver of proc.cpp's loop over the facade (host code only)."""
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-pthread",
            f"-I{os.path.join(os.path.dirname(HERE), 'include')}", DRIVER_SRC, "-o", DRIVER + ".tmp",
            f"-L{HERE}", "-l:libepi_amd.so", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)

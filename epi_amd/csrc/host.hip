@@ -86,7 +86,8 @@ const char* kKernelNames[K_COUNT] = {
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
-    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place", "k_wl", "k_ca_swap"};
+    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place", "k_wl", "k_ca_swap",
+    "k_out_pack"};
 
 // with ready_only, only the pairs whose kernels have finished (the rest stay
 // pending for a later read-back)
@@ -158,8 +159,9 @@ struct epi_ctx {
   int pb = 0, cb = 0, cp = 0, ap = 0, bp = 0;
   bool pos_pending = false;
   Tuning tu;
-  // a step's renewal and build as instantiated graphs, one per launch key
-  // (parities, launch sizes, launch shapes), least recently used evicted
+  // with STEP_GRAPH: a step's renewal and build, and whole Ca solves, as
+  // instantiated graphs, one per launch key (parities, launch sizes, launch
+  // shapes, the solve's list), least recently used evicted
   struct StepGraph {
     std::vector<long long> key;
     hipGraphExec_t exec = nullptr;
@@ -188,6 +190,18 @@ struct epi_ctx {
     char* work = nullptr;
     size_t work_bytes = 0;
   } mech;
+  // asynchronous snapshots (epi_snapshot_*): per slot the packed record on
+  // the device, its pinned host copy (downloaded on copy_stream) and events
+  struct Snap {
+    char* dev = nullptr;
+    char* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev_pack = nullptr, ev_done = nullptr;
+    int tissue = -1, n = 0;
+    long long m = 0;
+    bool pending = false, landed = false;
+  } snap[EPI_SNAPSHOT_SLOTS];
+  hipStream_t copy_stream = nullptr;
   std::vector<long long> ca_solves;
   // each tissue's device timestep (Status::timestep) as the host last set or
   // advanced it
@@ -490,7 +504,10 @@ int ca_list(epi_ctx* c, const int* ids, int ntis, int cg, int bs, int zb, CaList
   for (int i = 0; i < ntis; i++) {
     const int t = ids ? ids[i] : i;
     const TH& h = c->th[t];
-    const int cells = std::max(h.n_est + 64 - h.nmemb - h.nder, 1);
+    // with step graphs, blocks for the cells up to the next multiple of 1024:
+    // the list (and a captured solve) then outlives the divisions of many steps
+    const int upto = c->tu.step_graph ? std::min((h.n_est + 64 + 1023) / 1024 * 1024, h.cap) : h.n_est + 64;
+    const int cells = std::max(upto - h.nmemb - h.nder, 1);
     nb[i] = (cells + cpb - 1) / cpb;
     izb[i] = (int)((c->h_st[t].zzmax + c->tis[t].prm.iz_margin) / c->tis[t].dz);
     if (izb[i] < 0 || izb[i] > h.NZ) izb[i] = 0;   // k_ca_vox reports it
@@ -533,6 +550,43 @@ int ca_list(epi_ctx* c, const int* ids, int ntis, int cg, int bs, int zb, CaList
   return EPI_OK;
 }
 
+// Replays the graph of `key`, or captures what `issue` launches on the
+// stream into a new one (kept with the others, least recently used evicted)
+// and launches it.  Captured launches carry no timing events.
+template <class F>
+int graph_run(epi_ctx* c, const std::vector<long long>& key, F issue) {
+  for (auto& g : c->graphs)
+    if (g.key == key) {
+      g.used = ++c->graph_tick;
+      HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+      return EPI_OK;
+    }
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  g_prof.capturing = true;
+  const hipError_t e1 = issue();
+  g_prof.capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(c, EPI_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e3 != hipSuccess) return fail(c, EPI_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(e3));
+  constexpr size_t MAX_GRAPHS = 12;
+  if (c->graphs.size() >= MAX_GRAPHS) {
+    auto old = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                [](const epi_ctx::StepGraph& a, const epi_ctx::StepGraph& b) { return a.used < b.used; });
+    (void)hipGraphExecDestroy(old->exec);
+    c->graphs.erase(old);
+  }
+  c->graphs.push_back({key, exec, ++c->graph_tick});
+  HIPCHK(c, hipGraphLaunch(exec, c->stream));
+  return EPI_OK;
+}
+
 // calc_ca2p (ca2p.cpp:380-402) for the tissues in tlist (nullptr = all)
 // (with d_tlist, c->h_tlist holds the same ntis ids), n_sub sub-steps.  The
 // list covers the solving tissues only: a gated solve of one small tissue in
@@ -565,9 +619,32 @@ int run_ca_n(epi_ctx* c, const int* d_tlist, int ntis, int n_sub) {
   CaList cl;
   RCHK(ca_list(c, d_tlist ? c->h_tlist : nullptr, ntis, tu.ca_group, ca_list_bs(tu, tu.ca_fuse && n_sub >= 2),
                tu.a_zb, &cl));
-  launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
-  launch_ca_substeps(c->stream, c->d_tis, cl, c->cb, c->cp, c->ap, c->bp, n_sub, tu);
-  launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
+  constexpr unsigned long long seg = (1ull << K_CA_BEGIN) | (1ull << K_CA_VOX) | (1ull << K_CA_CORNERS) |
+                                     (1ull << K_CA_CELLS) | (1ull << K_ATP) | (1ull << K_CA_FUSED) |
+                                     (1ull << K_CA_END) | (1ull << K_SW_RESET);
+  if (c->tu.step_graph && !(g_prof.mask & seg)) {
+    // the whole solve (set-up, n_sub sub-steps, averages) as one graph per
+    // launch key; its grids cover the cells up to the next multiple of 1024
+    int mcap = 1;
+    for (int i = 0; i < ntis; i++) mcap = std::max(mcap, c->th[d_tlist ? c->h_tlist[i] : i].cap);
+    const int mr = std::max(mn, std::min((mn + 1023) / 1024 * 1024, mcap));
+    std::vector<long long> key = {2, d_tlist != nullptr, ntis, n_sub, mr, box, c->cb, c->cp, c->ap, c->bp,
+                                  tu.ca_fuse, tu.ca_group, tu.a_zb, tu.a_rows, tu.xcd};
+    key.insert(key.end(), c->cal_key.begin(), c->cal_key.end());
+    RCHK(graph_run(c, key, [&] {
+      int cp = c->cp, ap = c->ap;
+      launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mr, box);
+      launch_ca_substeps(c->stream, c->d_tis, cl, c->cb, cp, ap, c->bp, n_sub, tu);
+      launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mr);
+      return hipGetLastError();
+    }));
+    c->cp ^= n_sub & 1;
+    c->ap ^= n_sub & 1;
+  } else {
+    launch_ca_begin(c->stream, c->d_tis, d_tlist, ntis, c->cb, mn, box);
+    launch_ca_substeps(c->stream, c->d_tis, cl, c->cb, c->cp, c->ap, c->bp, n_sub, tu);
+    launch_ca_end(c->stream, c->d_tis, d_tlist, ntis, mn);
+  }
   if ((n_sub & 1) && d_tlist && ntis < c->T) {
     // the parities name the same buffer for every tissue: an odd count
     // flipped them for all, but advanced only the listed tissues.  They go
@@ -757,6 +834,14 @@ int epi_create(int32_t ntissues, const epi_geometry* geos, epi_ctx** out) {
 void epi_destroy(epi_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  for (auto& S : c->snap) {
+    if (S.dev) (void)hipFree(S.dev);
+    if (S.host) (void)hipHostFree(S.host);
+    if (S.ev_pack) (void)hipEventDestroy(S.ev_pack);
+    if (S.ev_done) (void)hipEventDestroy(S.ev_done);
+  }
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->rd.arena) (void)hipFree(c->rd.arena);
   if (c->mech.arena) (void)hipFree(c->mech.arena);
   if (c->mech.work) (void)hipFree(c->mech.work);
@@ -1179,38 +1264,9 @@ int run_build(epi_ctx* c, int mn, int nb, int izb, bool* joined) {
   for (auto& h : c->th) mcap = std::max(mcap, h.cap);
   const int mr = std::max(mn, std::min((mn + 1023) / 1024 * 1024, mcap));
   const Tuning& tu = c->tu;
-  const std::vector<long long> key = {c->pb, c->cb, c->cp, c->bp, nb, mr, izb, max_gsize(c), max_halo(c),
+  const std::vector<long long> key = {1, c->pb, c->cb, c->cp, c->bp, nb, mr, izb, max_gsize(c), max_halo(c),
                                       max_nx(c), max_ny(c), tu.b_zb, tu.b_rows, tu.map_lanes};
-  for (auto& g : c->graphs)
-    if (g.key == key) {
-      g.used = ++c->graph_tick;
-      HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
-      return EPI_OK;
-    }
-  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  g_prof.capturing = true;
-  const hipError_t e1 = issue_build(c, mr, nb, izb);
-  g_prof.capturing = false;
-  hipGraph_t graph = nullptr;
-  const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-  if (e1 != hipSuccess || e2 != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return fail(c, EPI_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
-  }
-  hipGraphExec_t exec = nullptr;
-  const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e3 != hipSuccess) return fail(c, EPI_ERR_HIP, std::string("step graph instantiation: ") + hipGetErrorString(e3));
-  constexpr size_t MAX_GRAPHS = 8;
-  if (c->graphs.size() >= MAX_GRAPHS) {
-    auto old = std::min_element(c->graphs.begin(), c->graphs.end(),
-                                [](const epi_ctx::StepGraph& a, const epi_ctx::StepGraph& b) { return a.used < b.used; });
-    (void)hipGraphExecDestroy(old->exec);
-    c->graphs.erase(old);
-  }
-  c->graphs.push_back({key, exec, ++c->graph_tick});
-  HIPCHK(c, hipGraphLaunch(exec, c->stream));
-  return EPI_OK;
+  return graph_run(c, key, [&] { return issue_build(c, mr, nb, izb); });
 }
 
 int epi_step(epi_ctx* c, int64_t nsteps, epi_driver* drv) {
@@ -1438,13 +1494,13 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
   if (!slot) return fail(c, EPI_ERR_INVALID, "unknown tuning key");
   bool ok = false;
   switch (key) {
-    case EPI_TUNE_B_ZB: case EPI_TUNE_ATP_ZB: ok = value == 1 || value == 2 || value == 4 || value == 8; break;
+    case EPI_TUNE_B_ZB: case EPI_TUNE_ATP_ZB: ok = value == 1 || value == 2 || value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_B_ROWS: case EPI_TUNE_ATP_ROWS: ok = value == 1 || value == 2 || value == 4; break;
     case EPI_TUNE_CA_GROUP: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_XCD: ok = value == 0 || value == 1; break;
     case EPI_TUNE_RD_GROUP: ok = value == 8 || value == 16; break;
     case EPI_TUNE_CA_FUSE: ok = value >= 0 && value <= 2; break;
-    case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4; break;
+    case EPI_TUNE_MAP_LANES: ok = value == 1 || value == 4 || value == 16; break;
     case EPI_TUNE_CA_GROUP_SMALL: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_STEP_GRAPH: ok = value == 0 || value == 1; break;
     case EPI_TUNE_B_XCD: ok = value == 0 || value == 1; break;
@@ -1532,6 +1588,72 @@ int epi_get_cells(epi_ctx* c, int32_t t, epi_cells* o) {
         }
     }
   }
+  return EPI_OK;
+}
+
+int epi_snapshot_begin(epi_ctx* c, int32_t t, int32_t slot) {
+  RCHK(check_tissue(c, t));
+  if (slot < 0 || slot >= EPI_SNAPSHOT_SLOTS) return fail(c, EPI_ERR_INVALID, "snapshot slot out of range");
+  if (c->rd.active || c->mech.active) return fail(c, EPI_ERR_INVALID, "no snapshot while a window is active");
+  auto& S = c->snap[slot];
+  if (S.pending) {   // the slot's last download still reads its buffers
+    HIPCHK(c, hipEventSynchronize(S.ev_done));
+    S.pending = false;
+  }
+  S.landed = false;
+  const int n = c->h_st[t].n;   // current after epi_step's status read-back
+  const long long m = std::max(32LL, ((long long)n + 31) / 32 * 32);
+  const size_t bytes = (size_t)out_record_bytes(m);
+  if (bytes > S.bytes) {
+    if (S.dev) HIPCHK(c, hipFree(S.dev));
+    if (S.host) HIPCHK(c, hipHostFree(S.host));
+    S.dev = nullptr;
+    S.host = nullptr;
+    S.bytes = 0;
+    HIPCHK(c, hipMalloc(&S.dev, bytes));
+    HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&S.host), bytes, hipHostMallocDefault));
+    S.bytes = bytes;
+  }
+  if (!S.ev_pack) HIPCHK(c, hipEventCreateWithFlags(&S.ev_pack, hipEventDisableTiming));
+  if (!S.ev_done) HIPCHK(c, hipEventCreateWithFlags(&S.ev_done, hipEventDisableTiming));
+  if (!c->copy_stream) HIPCHK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  launch_out_pack(c->stream, c->d_tis, t, c->pb, c->cp, c->cb, n, m, S.dev);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(S.ev_pack, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->copy_stream, S.ev_pack, 0));
+  HIPCHK(c, hipMemcpyAsync(S.host, S.dev, (size_t)out_record_bytes(m), hipMemcpyDeviceToHost, c->copy_stream));
+  HIPCHK(c, hipEventRecord(S.ev_done, c->copy_stream));
+  S.tissue = t;
+  S.n = n;
+  S.m = m;
+  S.pending = true;
+  return EPI_OK;
+}
+
+int epi_snapshot_wait(epi_ctx* c, int32_t slot, epi_snapshot* o) {
+  if (!c || !o) return EPI_ERR_INVALID;
+  if (slot < 0 || slot >= EPI_SNAPSHOT_SLOTS) return fail(c, EPI_ERR_INVALID, "snapshot slot out of range");
+  auto& S = c->snap[slot];
+  if (!S.pending && !S.landed) return fail(c, EPI_ERR_INVALID, "no snapshot in this slot");
+  if (S.pending) {
+    HIPCHK(c, hipEventSynchronize(S.ev_done));
+    S.pending = false;
+    S.landed = true;
+  }
+  if (*reinterpret_cast<const int*>(S.host) != S.n)
+    return fail(c, EPI_ERR_INVALID, "the cell count changed after the host last read it: take snapshots between epi_step calls");
+  const long long m = S.m;
+  const char* base = S.host + OUT_HDR;
+  const double* f = reinterpret_cast<const double*>(base);
+  const int* iv = reinterpret_cast<const int*>(base + 8 * OUT_NF64 * m);
+  const uint8_t* bv = reinterpret_cast<const uint8_t*>(base + (8 * OUT_NF64 + 4 * OUT_NI32) * m);
+  o->tissue = S.tissue;
+  o->n = S.n;
+  o->radius = f; o->ageb = f + m; o->agek = f + 2 * m; o->ca2p = f + 3 * m;
+  o->x = f + 4 * m; o->y = f + 5 * m; o->z = f + 6 * m; o->ca2p_avg = f + 7 * m;
+  o->ex_fat = f + 8 * m; o->in_fat = f + 9 * m; o->spr_nat_len = f + 10 * m;
+  o->rest_div_times = iv; o->pair = iv + m; o->fix_origin = iv + 2 * m;
+  o->state = bv; o->touch = bv + m;
   return EPI_OK;
 }
 

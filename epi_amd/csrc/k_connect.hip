@@ -351,6 +351,47 @@ __global__ void k_zz_set(const Tis* __restrict__ T, int use_value, double v) {
   if (threadIdx.x == 0) t.st->zzmax_bits = use_value ? (unsigned long long)__double_as_longlong(v) : 0ull;
 }
 
+// CellManager::output's per-cell record (cellmanager.cpp:451-538) packed for
+// an asynchronous download: one thread per cell; touch is an ALIVE cell with
+// a DEAD cell in its list, as output computes it
+__global__ void __launch_bounds__(256) k_out_pack(const Tis* __restrict__ T, int tissue, int pb, int cp, int cb,
+                                                  int n, long long m, char* __restrict__ out) {
+  const Tis& t = T[tissue];
+  double* __restrict__ f = reinterpret_cast<double*>(out + OUT_HDR);
+  int* __restrict__ iv = reinterpret_cast<int*>(out + OUT_HDR + 8 * OUT_NF64 * m);
+  uint8_t* __restrict__ bv = reinterpret_cast<uint8_t*>(out + OUT_HDR + (8 * OUT_NF64 + 4 * OUT_NI32) * m);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *reinterpret_cast<int*>(out) = t.st->n;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint8_t s = t.state[c];
+    uint8_t touch = 0;
+    if (s == ALIVE) {
+      const int len = t.nlen[cb][c];
+      const int* row = t.col[cb] + (size_t)c * t.ns;
+      for (int e = 0; e < len; e++)
+        if (t.state[row[e]] == DEAD) {
+          touch = 1;
+          break;
+        }
+    }
+    const double v[OUT_NF64] = {t.radius[c], t.ageb[c], t.agek[c], t.ca2p[cp][c], t.pos[pb][0][c],
+                                t.pos[pb][1][c], t.pos[pb][2][c], t.ca2p_avg[c], t.ex_fat[c], t.in_fat[c],
+                                t.spr_nat_len[c]};
+#pragma unroll
+    for (int k = 0; k < OUT_NF64; k++) f[k * m + c] = v[k];
+    iv[c] = t.rest_div_times[c];
+    iv[m + c] = t.pair[c];
+    iv[2 * m + c] = t.fix_origin[c];
+    bv[c] = s;
+    bv[m + c] = touch;
+  }
+}
+
+void launch_out_pack(hipStream_t s, const Tis* dT, int tissue, int pb, int cp, int cb, int n, long long m,
+                     char* out) {
+  const unsigned g = (unsigned)std::max(1, std::min((n + 255) / 256, 4096));
+  EPI_LAUNCH(K_OUT_PACK, k_out_pack, dim3(g), dim3(256), 0, s, dT, tissue, pb, cp, cb, n, m, out);
+}
+
 void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, int fix) {
   dim3 gc((max_n + 255) / 256, ntis);
   EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb, fix);

@@ -48,9 +48,10 @@ __device__ __forceinline__ int lat_wrap(long v, int N, Status* st) {
 // set_lattice (:18-22) and the sphere stamping (:39-155); the last writer of
 // the serial loop is the highest cell index, hence atomicMax on cmap1.  One
 // lane per cell.
-// MS lanes per cell (1 or 4, a launch shape) split the stamp box by z plane:
-// the lanes of a cell then touch voxels a plane apart, never one cache line.
-// The result does not depend on MS (a max and idempotent stores).
+// MS lanes per cell (1, 4 or 16, a launch shape) split each x slice of the
+// stamp box, its (y, z) voxels taken z fastest, round-robin: the lanes of a
+// cell touch voxels a plane apart.  The result does not depend on MS (a max
+// and idempotent stores).
 template <int MS>
 __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, int pb) {
   const Tis& t = T[blockIdx.y];
@@ -76,17 +77,28 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
     const int zmin = z_b >= 1 ? z_b : 1;
     const int zmax = z_u < t.NZ ? z_u : t.NZ - 1;
     ztop = max(ztop, zmax);
+    const int nzr = zmax - zmin + 1;
+    if (nzr <= 0) {   // no plane to stamp: the box's x and y range is still checked
+      if (zl == 0) {
+        for (int k = lx - irx; k <= lx + irx; k++) lat_wrap(k, t.NX, st);
+        for (int l = ly - iry; l <= ly + iry; l++) lat_wrap(l, t.NY, st);
+      }
+      continue;
+    }
+    // this lane's first (y, z) voxel of a slice: the zl-th, z fastest
+    const int l_first = ly - iry + zl / nzr, m_first = zmin + zl % nzr;
     for (int k = lx - irx; k <= lx + irx; k++) {
       const double mx = k * t.dx;
       const int ipx = lat_wrap(k, t.NX, st);
       const double diffx = p_diff_x(t, mx, p.x);
       const double diffxSq = diffx * diffx;
-      for (int l = ly - iry; l <= ly + iry; l++) {
+      int m = m_first;
+      for (int l = l_first; l <= ly + iry;) {
         const double my = l * t.dy;
         const int ipy = lat_wrap(l, t.NY, st);
         const double diffy = p_diff_y(t, my, p.y);
         const double tmps = diffxSq + diffy * diffy;
-        for (int m = zmin + zl; m <= zmax; m += MS) {
+        for (; m <= zmax; m += MS) {
           const double mz = m * t.dz;
           const double diffz = mz - p.z;
           const double distSq = tmps + diffz * diffz;
@@ -98,6 +110,8 @@ __global__ void __launch_bounds__(128) k_map_stamp(const Tis* __restrict__ T, in
             if (distSq < normal_radSq) atomicMax(&t.cmap1[v], c);
           }
         }
+        // carry the stride into the next rows
+        while (m > zmax) { m -= nzr; l++; }
       }
     }
   }
@@ -287,12 +301,21 @@ __device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int
       cell[q] = t.cmap1[cxy + l * PL];
     }
   }
+  // a voxel's state and age are read together (the age of a cell that is
+  // neither DEAD nor ALIVE is then dropped), one round trip after the map
   uint8_t cs[ZB];
-#pragma unroll
-  for (int q = 0; q < ZB; q++) cs[q] = cell[q] >= 0 ? t.state[cell[q]] : (uint8_t)MEMB;
   double age[ZB];
 #pragma unroll
-  for (int q = 0; q < ZB; q++) age[q] = (cs[q] == DEAD || cs[q] == ALIVE) ? t.agek[cell[q]] : 0.0;
+  for (int q = 0; q < ZB; q++) {
+    cs[q] = MEMB;
+    age[q] = 0.0;
+    if (cell[q] >= 0) {
+      cs[q] = t.state[cell[q]];
+      age[q] = t.agek[cell[q]];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ZB; q++) age[q] = (cs[q] == DEAD || cs[q] == ALIVE) ? age[q] : 0.0;
   const double inv_dz = t.inv_dz;
 #pragma unroll
   for (int q = 0; q < ZB; q++) {
@@ -320,7 +343,9 @@ __device__ __forceinline__ void b_column(const Tis& t, int j, int k, int l0, int
 // The same column with the z column rolling through registers, one plane
 // after the other: three values and masks live at a time, so a tile of many
 // planes keeps the registers (and occupancy) of one.  Large tiles, where the
-// sweep is bandwidth- and not latency-bound, take this form.
+// sweep is bandwidth- and not latency-bound, take this form.  The voxel's
+// cell is read one plane ahead and its state and age together, so a plane
+// waits for one memory round trip, not for the chain map -> state -> age.
 __device__ __forceinline__ void b_column_roll(const Tis& t, int j, int k, int l0, int l1, int bp) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
   const BK bk = b_k(t.prm);
@@ -328,6 +353,9 @@ __device__ __forceinline__ void b_column_roll(const Tis& t, int j, int k, int l0
   const double* __restrict__ carr = t.bf[bp];
   double* __restrict__ narr = t.bf[1 - bp];
   const uint8_t* __restrict__ cm2 = t.cmap2;
+  const int* __restrict__ cm1 = t.cmap1;
+  const uint8_t* __restrict__ cst = t.state;
+  const double* __restrict__ cage = t.agek;
   const int prev_x = j == 0 ? NX - 1 : j - 1, next_x = j == NX - 1 ? 0 : j + 1;
   const int prev_y = k == 0 ? NY - 1 : k - 1, next_y = k == NY - 1 ? 0 : k + 1;
   const double inv_dz = t.inv_dz;
@@ -338,19 +366,22 @@ __device__ __forceinline__ void b_column_roll(const Tis& t, int j, int k, int l0
   double mzm = (double)cm2[cxy + pz * PL];
   double cc = carr[cxy + l0 * PL];
   double mc = (double)cm2[cxy + l0 * PL];
+  int cell = l0 < l1 ? cm1[cxy + l0 * PL] : -1;
   for (int l = l0; l < l1; l++) {
     const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
     const long long v = cxy + l * PL;
     const long long vzp = cxy + next_z * PL;
     const double czp = carr[vzp];
     const double mzp = (double)cm2[vzp];
-    double dum_age = 0;
-    bool flg = false;
-    const int cell = t.cmap1[v];
+    const int cell_next = l + 1 < l1 ? cm1[v + PL] : -1;
+    uint8_t s = MEMB;
+    double a = 0;
     if (cell >= 0) {
-      const uint8_t s = t.state[cell];
-      if (s == DEAD || s == ALIVE) { dum_age = t.agek[cell]; flg = true; }
+      s = cst[cell];
+      a = cage[cell];
     }
+    const bool flg = s == DEAD || s == ALIVE;
+    const double dum_age = flg ? a : 0.0;
     const double cext = cc;
     const long long vxm = gidx(t, prev_x, k, l), vym = gidx(t, j, prev_y, l);
     const long long vyp = gidx(t, j, next_y, l), vxp = gidx(t, next_x, k, l);
@@ -368,6 +399,7 @@ __device__ __forceinline__ void b_column_roll(const Tis& t, int j, int k, int l0
     if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
     czm = cc; mzm = mc;
     cc = czp; mc = mzp;
+    cell = cell_next;
   }
 }
 
@@ -391,13 +423,15 @@ __global__ void __launch_bounds__(256) k_ext_stim(const Tis* __restrict__ T, con
   const int bs = blockDim.x;
   int ti = 0;
   for (TileWalk tw(off(ntis), xcd, blockIdx.x, gridDim.x); tw.w < tw.end; tw.w += tw.stride) {
-    ti = wl_find(off, ti, ntis, tw.w);
+    // the tile, hence its tissue, is the block's: held in scalar registers,
+    // so the tissue's record is read with scalar loads
+    ti = __builtin_amdgcn_readfirstlane(wl_find(off, ti, ntis, tw.w));
     const Tis& t = T[ti];
-    const int iz_bound = lds ? s_izb[ti] : wizb[ti];
+    const int iz_bound = __builtin_amdgcn_readfirstlane(lds ? s_izb[ti] : wizb[ti]);
     const int NX = t.NX, NY = t.NY, NZ = t.NZ;
     const int plane = NX * NY;
     const int npt = (plane + bs - 1) / bs;
-    const int local = tw.w - off(ti);
+    const int local = tw.w - __builtin_amdgcn_readfirstlane(off(ti));
     const int p = (local % npt) * bs + (int)threadIdx.x;
     if (p >= plane) continue;
     const int j = p % NX, k = p / NX;
@@ -889,7 +923,8 @@ __device__ __forceinline__ void atp_column(const Tis& t, int j, int k, int l0, i
 }
 
 // The same column rolling through registers plane by plane (as
-// b_column_roll), for tall tiles.
+// b_column_roll), for tall tiles; the voxel word is read one plane ahead, so
+// a plane waits for one round trip (its owner's diff_u), not two.
 __device__ __forceinline__ void atp_column_roll(const Tis& t, int j, int k, int l0, int l1, int ap, int cp) {
   const int NX = t.NX, NY = t.NY, NZ = t.NZ;
   const AtpK ak = atp_k(t.prm);
@@ -907,13 +942,15 @@ __device__ __forceinline__ void atp_column_roll(const Tis& t, int j, int k, int 
   double mzm = (double)cm2[cxy + pz * PL];
   double cc = carr[cxy + l0 * PL];
   double mc = (double)cm2[cxy + l0 * PL];
+  const unsigned* __restrict__ vox = t.vox;
+  unsigned w = l0 < l1 ? vox[cxy + l0 * PL] : 0u;
   for (int l = l0; l < l1; l++) {
     const int next_z = l + 1 == NZ ? NZ - 1 : l + 1;
     const long long v = cxy + l * PL;
     const long long vzp = cxy + next_z * PL;
     const double czp = carr[vzp];
     const double mzp = (double)cm2[vzp];
-    const unsigned w = t.vox[v];
+    const unsigned w_next = l + 1 < l1 ? vox[v + PL] : 0u;
     const unsigned cell1 = w & 0x7fffffffu;
     const double diffu = cell1 ? dfu[cell1 - 1] : 0.0;
     const long long vxm = gidx(t, prev_x, k, l), vxp = gidx(t, next_x, k, l);
@@ -927,6 +964,7 @@ __device__ __forceinline__ void atp_column_roll(const Tis& t, int j, int k, int 
     if (j == 0 && k == 0) narr[gidx(t, NX, NY, l)] = nv;
     czm = cc; mzm = mc;
     cc = czp; mc = mzp;
+    w = w_next;
   }
 }
 
@@ -982,9 +1020,11 @@ __device__ __forceinline__ void atp_walk(const Tis* __restrict__ T, const CaList
   auto off = [&](int i) { return lds ? s_off[i] : cl.aoff[i]; };
   int ti = 0;
   for (TileWalk tw(off(nt), xcd, b, nb); tw.w < tw.end; tw.w += tw.stride) {
-    ti = wl_find(off, ti, nt, tw.w);
+    // uniform over the block (see k_ext_stim)
+    ti = __builtin_amdgcn_readfirstlane(wl_find(off, ti, nt, tw.w));
     const int2 q = lds ? s_ti[ti] : make_int2(cl.atis[ti], cl.aizb[ti]);
-    atp_tile<ZB>(T[q.x], tw.w - off(ti), q.y, ap, cp);
+    atp_tile<ZB>(T[__builtin_amdgcn_readfirstlane(q.x)], tw.w - __builtin_amdgcn_readfirstlane(off(ti)),
+                 __builtin_amdgcn_readfirstlane(q.y), ap, cp);
   }
 }
 
@@ -995,18 +1035,41 @@ __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, CaList c
 
 // one launch for the ATP update of sub-step i (parities cp, ap) and the cells
 // of sub-step i + 1: blocks [0, ncb) take the cells (one entry of the list's
-// cell-block map each), the rest walk the ATP tiles
+// cell-block map each), the rest walk the ATP tiles.  With xcd (the grid then
+// holds 8 ceil(ncb / 8) cell blocks, the last ones idle, and a multiple of 8
+// ATP blocks) each XCD (block id mod 8) takes one contiguous run of the cell
+// blocks and one of the ATP tiles, so that cells of one layer, their
+// neighbours and their ATP corners, and the z neighbours of ATP tiles, share
+// an L2.  Results do not depend on it.
 constexpr int F_BLOCK = 128;
 
 template <int CG, int ZB>
 __global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T, CaList cl, int cb, int cp,
-                                                      int ap, int bp, int ncb) {
-  if ((int)blockIdx.x < ncb) {
-    const int4 e = cl.ntis == 1 ? make_int4(cl.t0, (int)blockIdx.x, ncb, 0) : cl.cmap[blockIdx.x];
+                                                      int ap, int bp, int ncb, int xcd) {
+  int cbk = -1;   // this block's cell block, or -1 for an ATP block
+  int ab = 0, an = 0;
+  if (xcd) {
+    const int cps = (ncb + 7) >> 3;   // cell blocks per XCD
+    const int x = (int)(blockIdx.x & 7), sl = (int)(blockIdx.x >> 3);
+    if (sl < cps) {
+      cbk = x * cps + sl;
+      if (cbk >= ncb) return;
+    } else {
+      ab = (sl - cps) * 8 + x;
+      an = (int)gridDim.x - 8 * cps;
+    }
+  } else if ((int)blockIdx.x < ncb) {
+    cbk = (int)blockIdx.x;
+  } else {
+    ab = (int)blockIdx.x - ncb;
+    an = (int)gridDim.x - ncb;
+  }
+  if (cbk >= 0) {
+    const int4 e = cl.ntis == 1 ? make_int4(cl.t0, cbk, ncb, 0) : cl.cmap[cbk];
     const Tis& t = T[e.x];
     ca_cells_part<CG, F_BLOCK>(t, e.y, e.z, cb, 1 - cp, 1 - ap, bp, t.atp[ap], dfu_buf(t, cp));
   } else {
-    atp_walk<ZB>(T, cl, blockIdx.x - ncb, gridDim.x - ncb, 0, ap, cp);
+    atp_walk<ZB>(T, cl, ab, an, xcd, ap, cp);
   }
 }
 
@@ -1065,7 +1128,9 @@ void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long
   const long long q = (max_gsize / 4 + 255) / 256;
   dim3 gg((unsigned)(q < 1 ? 1 : q < 65535 ? q : 65535), ntis);
   EPI_LAUNCH(K_MAP_CLEAR, k_map_clear, gg, dim3(256), 0, s, dT);
-  if (tu.map_lanes == 4)
+  if (tu.map_lanes == 16)
+    EPI_LAUNCH(K_MAP_STAMP, k_map_stamp<16>, dim3((16 * max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
+  else if (tu.map_lanes == 4)
     EPI_LAUNCH(K_MAP_STAMP, k_map_stamp<4>, dim3((4 * max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
   else
     EPI_LAUNCH(K_MAP_STAMP, k_map_stamp<1>, dim3((max_n + 127) / 128, ntis), dim3(128), 0, s, dT, pb);
@@ -1079,6 +1144,7 @@ void by_zb(int zb, F f) {
     case 1: f(std::integral_constant<int, 1>()); break;
     case 2: f(std::integral_constant<int, 2>()); break;
     case 8: f(std::integral_constant<int, 8>()); break;
+    case 16: f(std::integral_constant<int, 16>()); break;
     default: f(std::integral_constant<int, 4>()); break;
   }
 }
@@ -1156,9 +1222,10 @@ void launch_fused_cg(hipStream_t s, const Tis* dT, const CaList& cl, int cb, int
                      const Tuning& tu) {
   by_zb(tu.a_zb, [&](auto zbc) {
     constexpr int zb = decltype(zbc)::value;
-    const dim3 g((unsigned)cl.ncb + walk_blocks(cl.atiles, F_BLOCK));
+    const unsigned cblocks = tu.xcd ? (unsigned)(cl.ncb + 7) / 8 * 8 : (unsigned)cl.ncb;
+    const dim3 g(cblocks + walk_blocks(cl.atiles, F_BLOCK));
     const auto kfn = k_ca_fused<CG, zb>;
-    EPI_LAUNCH(K_CA_FUSED, kfn, g, dim3(F_BLOCK), 0, s, dT, cl, cb, cp, ap, bp, cl.ncb);
+    EPI_LAUNCH(K_CA_FUSED, kfn, g, dim3(F_BLOCK), 0, s, dT, cl, cb, cp, ap, bp, cl.ncb, tu.xcd);
   });
 }
 }  // namespace

@@ -14,7 +14,7 @@ enum KernelId {
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
   K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
   K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB,
-  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_WL, K_CA_SWAP, K_COUNT
+  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_WL, K_CA_SWAP, K_OUT_PACK, K_COUNT
 };
 
 // ints of a context's work-list buffer: the offsets and per-tissue iz_bound
@@ -32,7 +32,7 @@ struct Tuning {
   int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
                               // (0 never, 1 always, 2 for solves of at most 131072 cells)
   int ca_group_small = 8;     // k_ca_cells lanes per cell in the solves that ca_fuse 2 fuses
-  int map_lanes = 1;          // k_map_stamp: lanes per cell (z planes split between them)
+  int map_lanes = 1;          // k_map_stamp: lanes per cell (1, 4 or 16; a box slice split between them)
   int step_graph = 0;         // epi_step: 1 replays the renewal and build from a captured graph
 };
 
@@ -75,6 +75,18 @@ void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int m
 void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n);
 void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix);
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
+// A snapshot's output record (CellManager::output's fields): a header of
+// OUT_HDR bytes whose first int is the device's cell count, then m >= n
+// slots per array: the doubles radius, ageb, agek, ca2p, x, y, z, ca2p_avg,
+// ex_fat, in_fat, spr_nat_len, the ints rest_div_times, pair, fix_origin and
+// the bytes state, touch.  m is a multiple of 32, so every array is aligned.
+constexpr int OUT_NF64 = 11, OUT_NI32 = 3, OUT_NU8 = 2;
+constexpr long long OUT_HDR = 256;
+constexpr long long out_record_bytes(long long m) { return OUT_HDR + m * (8 * OUT_NF64 + 4 * OUT_NI32 + OUT_NU8); }
+
+// the output record of one tissue's first n cells into out, positions from
+// buffer pb, Ca2+ from cp, lists from cb
+void launch_out_pack(hipStream_t s, const Tis* dT, int tissue, int pb, int cp, int cb, int n, long long m, char* out);
 void launch_zz_set(hipStream_t s, const Tis* dT, int ntis, double v);
 void launch_maps(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, long long max_gsize, int max_halo,
                  const Tuning& tu);

@@ -14,7 +14,9 @@
 // 17-column text of CellManager::output.  --snapshots writes that text to
 // DIR/<i / N> before every step i with i % N == 0 (output_cell_data,
 // proc.cpp:45-51, 139-141; N defaults to CUT = 1000): the numbered series the
-// reference's viewers read (epi_vis_lightweight/main.cpp:493-520).
+// reference's viewers read (epi_vis_lightweight/main.cpp:493-520).  Those
+// snapshots are asynchronous (CellManager::output_async): the record is
+// packed on the GPU and downloaded and written while later steps run.
 // --output-binary writes CellManager::output's binary form.  --save-last
 // writes output_field_data's five files (last_data_cell, last_data_uvp,
 // last_data_w_alt, last_data_a_alt, last_data_B_alt) and appends the
@@ -198,7 +200,8 @@ int main(int argc, char** argv) {
         std::printf("loop:%ld elapsed[sec]:%lf\n", i, s);
         std::fflush(stdout);
       }
-      if (!snapdir.empty() && cut > 0 && i % cut == 0) cman.output(snapdir + "/" + std::to_string(i / cut));
+      // taken on the device and written by a writer thread while the loop goes on
+      if (!snapdir.empty() && cut > 0 && i % cut == 0) cman.output_async(snapdir + "/" + std::to_string(i / cut));
       if (!save_dir.empty() && cut > 0 && i % (cut * 10) == 0 && i > 0) {   // proc.cpp:142-152
         cman.clean_up();
         output_field_data(cman, ATP.first(), ext_stim.first(), save_dir);
@@ -222,6 +225,7 @@ int main(int argc, char** argv) {
         cman.ca_calc_condition_reset();
       }
     }
+    cman.flush_outputs();
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     std::printf("done: %ld steps, %zu cells, %ld Ca solves, %.3f s\n", steps, cman.size(), ca_solves, s);
     if (!output.empty()) cman.output(output);

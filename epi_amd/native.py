@@ -67,6 +67,18 @@ class Driver(C.Structure):
                 ("ca_every", C.c_int32), ("clean_every", C.c_int32)]
 
 
+SNAPSHOT_F64 = ("radius", "ageb", "agek", "ca2p", "x", "y", "z", "ca2p_avg", "ex_fat", "in_fat", "spr_nat_len")
+
+
+class Snapshot(C.Structure):
+    """epi_snapshot: CellManager::output's per-cell record, downloaded
+    asynchronously (pointers into the context's pinned memory)"""
+    _fields_ = [("tissue", C.c_int32), ("n", C.c_int32), ("state", C.POINTER(C.c_uint8)),
+                ("touch", C.POINTER(C.c_uint8)), ("rest_div_times", C.POINTER(C.c_int32)),
+                ("pair", C.POINTER(C.c_int32)), ("fix_origin", C.POINTER(C.c_int32))] + \
+               [(k, C.POINTER(C.c_double)) for k in SNAPSHOT_F64]
+
+
 class Status(C.Structure):
     _fields_ = [("n", C.c_int32), ("err", C.c_int32), ("sw", C.c_int32), ("ndiv", C.c_int32),
                 ("nrm", C.c_int32), ("maxlen", C.c_int32), ("rng_pos", C.c_int64),
@@ -101,7 +113,10 @@ SYMBOLS = [
     "epi_mech_begin", "epi_mech_forward", "epi_mech_get_positions", "epi_mech_adjoint", "epi_mech_end",
     "epi_set_timestep", "epi_take_cell_stats", "epi_params_default", "epi_params_load", "epi_params_save",
     "epi_set_params", "epi_get_params", "epi_sheet_size", "epi_init_gen",
+    "epi_snapshot_begin", "epi_snapshot_wait",
 ]
+
+SNAPSHOT_SLOTS = 2   # EPI_SNAPSHOT_SLOTS
 
 MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
 
@@ -312,6 +327,8 @@ def load_library():
     L.epi_init_gen.argtypes = [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.POINTER(GenCells), C.c_char_p,
                                C.c_int32]
     L.epi_take_cell_stats.argtypes = [vp, C.c_int32, _i64p, C.c_int32, C.POINTER(C.c_int32)]
+    L.epi_snapshot_begin.argtypes = [vp, C.c_int32, C.c_int32]
+    L.epi_snapshot_wait.argtypes = [vp, C.c_int32, C.POINTER(Snapshot)]
     _lib = L
     return L
 
@@ -583,6 +600,22 @@ class TissueSet:
             v = C.c_int32()
             self._chk(self.L.epi_get_tuning(self.h, key, C.byref(v)), "get_tuning")
             out[k] = v.value
+        return out
+
+    # --- asynchronous snapshots ---
+    def snapshot_begin(self, t, slot=0):
+        """queue tissue t's output record into snapshot slot `slot` behind
+        the work queued so far; returns at once"""
+        self._chk(self.L.epi_snapshot_begin(self.h, int(t), int(slot)), "snapshot_begin")
+
+    def snapshot_wait(self, slot=0):
+        """wait for slot's download; the record as numpy copies"""
+        o = Snapshot()
+        self._chk(self.L.epi_snapshot_wait(self.h, int(slot), C.byref(o)), "snapshot_wait")
+        n = o.n
+        out = {"tissue": o.tissue, "n": n}
+        for k, _ in Snapshot._fields_[2:]:
+            out[k] = np.ctypeslib.as_array(getattr(o, k), shape=(n,)).copy() if n else np.zeros(0)
         return out
 
     # --- state exchange ---

@@ -269,6 +269,31 @@ int epi_get_status(epi_ctx* ctx, int32_t tissue, epi_status* out);
 int epi_get_cells(epi_ctx* ctx, int32_t tissue, epi_cells* out);
 int epi_get_fields(epi_ctx* ctx, int32_t tissue, double* atp, double* atp_next, double* b,
                    double* b_next, int32_t* cmap1, uint8_t* cmap2);
+/* ---- asynchronous snapshots ---------------------------------------------
+ * The per-cell record CellManager::output writes (cellmanager.cpp:451-538),
+ * taken the way proc.cpp:139-141 takes one every CUT steps, but without
+ * stopping the step loop.  epi_snapshot_begin queues, behind the work queued
+ * so far (call it between epi_step calls), one kernel that packs tissue's
+ * record into the context's snapshot buffer `slot` (0 <= slot <
+ * EPI_SNAPSHOT_SLOTS) and its download into pinned host memory on a copy
+ * stream of its own; it returns at once, and later epi_step calls run beside
+ * the download.  The first call on a slot, or one for a tissue of more cells
+ * than the slot has held, allocates the slot's device and pinned buffers.
+ * A call on a slot whose previous snapshot was not waited for waits for it.
+ * epi_snapshot_wait blocks until the slot's download has landed and points
+ * *out at it: the pointers stay valid until the next epi_snapshot_begin on
+ * the slot or epi_destroy.  touch is the ALIVE cell's "has a DEAD neighbour"
+ * flag of the output record. */
+#define EPI_SNAPSHOT_SLOTS 2
+typedef struct epi_snapshot {
+    int32_t tissue, n;
+    const uint8_t *state, *touch;
+    const int32_t *rest_div_times, *pair, *fix_origin;
+    const double *radius, *ageb, *agek, *ca2p, *x, *y, *z, *ca2p_avg, *ex_fat, *in_fat, *spr_nat_len;
+} epi_snapshot;
+int epi_snapshot_begin(epi_ctx* ctx, int32_t tissue, int32_t slot);
+int epi_snapshot_wait(epi_ctx* ctx, int32_t slot, epi_snapshot* out);
+
 /* kernel-level inputs: positions (both buffers), state/agek, lists, maps */
 int epi_set_positions(epi_ctx* ctx, int32_t tissue, const double* x, const double* y, const double* z);
 int epi_set_state(epi_ctx* ctx, int32_t tissue, const uint8_t* state, const double* agek);
@@ -400,7 +425,7 @@ int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t ngrids, int3
  * chosen per workload by epi_amd/autotune.py; they change timing only, never
  * results.  The stencils walk a list of tiles over every tissue of a launch;
  * a tile is 64 *_ROWS consecutive points of a tissue's x-y plane (the block)
- * by *_ZB z planes.  Values: *_ZB in {1, 2, 4, 8} (z planes per tile),
+ * by *_ZB z planes.  Values: *_ZB in {1, 2, 4, 8, 16} (z planes per tile),
  * *_ROWS in {1, 2, 4} (64-lane rows per block), CA_GROUP in {4, 8, 16} (lanes per
  * cell in the Ca cell kernel), XCD in {0, 1} (1: the Ca cell and ATP kernels
  * number their workgroups so that each of the 8 XCDs runs one contiguous run
@@ -409,11 +434,13 @@ int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t ngrids, int3
  * solve launches the ATP update of a sub-step and the cells of the next one
  * together, the cells computing their ATP corners themselves; 2: only for
  * solves of at most 131072 cells in total), MAP_LANES in
- * {1, 4} (lanes per cell in the voxel-map stamp, splitting its z planes),
+ * {1, 4, 16} (lanes per cell in the voxel-map stamp, splitting its voxels),
  * CA_GROUP_SMALL in {4, 8, 16} (lanes per cell in the solves CA_FUSE 2
  * fuses), STEP_GRAPH in {0, 1} (0, the default: epi_step launches the
  * renewal and neighbour-build kernels of a step one by one; 1: it replays
- * them, side stream included, from a HIP graph captured once per launch key),
+ * them, side stream included, and every whole Ca solve (set-up, sub-steps,
+ * averages) from HIP graphs captured once per launch key; whether a step
+ * solves is still decided on the host from the step's status read-back),
  * B_XCD in {0, 1} (1: the B stencil's tile list is cut into one contiguous
  * run per XCD).  EPI_ERR_INVALID for other keys or values. */
 enum {

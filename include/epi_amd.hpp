@@ -28,10 +28,13 @@
 #pragma once
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -116,13 +119,51 @@ struct CellData {
   std::vector<int64_t> uid;
 };
 
+// CellManager::output's per-cell record (cellmanager.cpp:451-538)
+struct OutRecord {
+  int32_t n = 0;
+  std::vector<uint8_t> state, touch;
+  std::vector<int32_t> rest_div_times, pair, fix_origin;
+  std::vector<double> radius, ageb, agek, ca2p, x, y, z, ca2p_avg, ex_fat, in_fat, spr_nat_len;
+};
+
+// the file CellManager::output writes: 17 text columns per cell, or the
+// binary records of cellmanager.cpp:496-513
+inline void write_output(const OutRecord& d, const std::string& filename, bool binary_mode) {
+  FILE* f = std::fopen(filename.c_str(), binary_mode ? "wb" : "w");
+  if (!f) throw Error(EPI_ERR_INVALID, "Output file creation error. Filename:" + filename);
+  for (int i = 0; i < d.n; i++) {
+    if (binary_mode) {
+      const int32_t idx = i, st = d.state[i], rdt = d.rest_div_times[i], pr = d.pair[i], fo = d.fix_origin[i];
+      const double a[8] = {d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i], d.ca2p_avg[i]};
+      const double b[2] = {d.ex_fat[i], d.in_fat[i]};
+      const char touch = (char)d.touch[i];
+      std::fwrite(&idx, 4, 1, f); std::fwrite(&st, 4, 1, f); std::fwrite(a, 8, 8, f);
+      std::fwrite(&rdt, 4, 1, f); std::fwrite(b, 8, 2, f); std::fwrite(&touch, 1, 1, f);
+      std::fwrite(&d.spr_nat_len[i], 8, 1, f); std::fwrite(&pr, 4, 1, f); std::fwrite(&fo, 4, 1, f);
+    } else {
+      std::fprintf(f, "%d %d %.15f %.15f %.15f %.15f %.15f %.15f %.15f %.15f %d %.15f %.15f %d %.15f %d %d\n",
+                   i, (int)d.state[i], d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i],
+                   d.ca2p_avg[i], d.rest_div_times[i], d.ex_fat[i], d.in_fat[i], (int)d.touch[i], d.spr_nat_len[i],
+                   d.pair[i], d.fix_origin[i]);
+    }
+  }
+  std::fclose(f);
+}
+
 class CellManager {
  public:
   // geometry: define.h:72-113 as runtime values
   explicit CellManager(const epi_geometry& g) : geo_(g) {
     check(nullptr, epi_create(1, &geo_, &ctx_), "epi_create");
   }
-  ~CellManager() { epi_destroy(ctx_); }
+  ~CellManager() {
+    try {
+      flush_outputs();
+    } catch (...) {
+    }
+    epi_destroy(ctx_);
+  }
   CellManager(const CellManager&) = delete;
   CellManager& operator=(const CellManager&) = delete;
 
@@ -256,32 +297,84 @@ class CellManager {
   // (109 bytes; :496-513).
   void output(const std::string& filename, bool binary_mode = false) const {
     const CellData d = download();
-    std::vector<char> touch((size_t)d.n, 0);
+    OutRecord r;
+    r.n = d.n;
+    r.touch.assign((size_t)d.n, 0);
     for (int i = 0; i < d.n; i++)
       if (d.state[i] == ALIVE)
         for (int e = d.conn_ptr[i]; e < d.conn_ptr[i + 1]; e++)
-          if (d.state[d.conn[e]] == DEAD) { touch[i] = 1; break; }
-    FILE* f = std::fopen(filename.c_str(), binary_mode ? "wb" : "w");
-    if (!f) throw Error(EPI_ERR_INVALID, "Output file creation error. Filename:" + filename);
-    for (int i = 0; i < d.n; i++) {
-      if (binary_mode) {
-        const int32_t idx = i, st = d.state[i], rdt = d.rest_div_times[i], pr = d.pair[i], fo = d.fix_origin[i];
-        const double a[8] = {d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i], d.ca2p_avg[i]};
-        const double b[2] = {d.ex_fat[i], d.in_fat[i]};
-        std::fwrite(&idx, 4, 1, f); std::fwrite(&st, 4, 1, f); std::fwrite(a, 8, 8, f);
-        std::fwrite(&rdt, 4, 1, f); std::fwrite(b, 8, 2, f); std::fwrite(&touch[i], 1, 1, f);
-        std::fwrite(&d.spr_nat_len[i], 8, 1, f); std::fwrite(&pr, 4, 1, f); std::fwrite(&fo, 4, 1, f);
-      } else {
-        std::fprintf(f, "%d %d %.15f %.15f %.15f %.15f %.15f %.15f %.15f %.15f %d %.15f %.15f %d %.15f %d %d\n",
-                     i, (int)d.state[i], d.radius[i], d.ageb[i], d.agek[i], d.ca2p[i], d.x[i], d.y[i], d.z[i],
-                     d.ca2p_avg[i], d.rest_div_times[i], d.ex_fat[i], d.in_fat[i], (int)touch[i], d.spr_nat_len[i],
-                     d.pair[i], d.fix_origin[i]);
-      }
+          if (d.state[d.conn[e]] == DEAD) { r.touch[i] = 1; break; }
+    r.state = d.state; r.rest_div_times = d.rest_div_times; r.pair = d.pair; r.fix_origin = d.fix_origin;
+    r.radius = d.radius; r.ageb = d.ageb; r.agek = d.agek; r.ca2p = d.ca2p; r.x = d.x; r.y = d.y; r.z = d.z;
+    r.ca2p_avg = d.ca2p_avg; r.ex_fat = d.ex_fat; r.in_fat = d.in_fat; r.spr_nat_len = d.spr_nat_len;
+    write_output(r, filename, binary_mode);
+  }
+
+  // output() without stopping the step loop, for the snapshot every CUT
+  // steps (proc.cpp:139-141): the record is packed on the device behind the
+  // queued work and downloaded on a copy stream (epi_snapshot_begin) while
+  // later steps run; at the next call on the same slot (the slots alternate)
+  // or at flush_outputs() it is copied out and a writer thread writes the
+  // file.  The files are those output() writes at the same point.
+  void output_async(const std::string& filename, bool binary_mode = false) {
+    const int slot = next_slot_;
+    next_slot_ = (next_slot_ + 1) % EPI_SNAPSHOT_SLOTS;
+    collect(slot);
+    check(ctx_, epi_snapshot_begin(ctx_, 0, slot), "epi_snapshot_begin");
+    pending_[slot] = Pending{filename, binary_mode, true};
+  }
+  // every file of output_async written; rethrows a writer's error
+  void flush_outputs() {
+    for (int s = 0; s < EPI_SNAPSHOT_SLOTS; s++) collect(s);
+    for (auto& w : writers_) w.join();
+    writers_.clear();
+    std::exception_ptr e;
+    {
+      std::lock_guard<std::mutex> g(err_mu_);
+      std::swap(e, writer_err_);
     }
-    std::fclose(f);
+    if (e) std::rethrow_exception(e);
   }
 
  private:
+  struct Pending {
+    std::string filename;
+    bool binary = false, active = false;
+  };
+  // the slot's landed snapshot, copied out and handed to a writer thread
+  void collect(int slot) {
+    Pending& p = pending_[slot];
+    if (!p.active) return;
+    p.active = false;
+    epi_snapshot s{};
+    check(ctx_, epi_snapshot_wait(ctx_, slot, &s), "epi_snapshot_wait");
+    OutRecord r;
+    const size_t n = (size_t)s.n;
+    r.n = s.n;
+    r.state.assign(s.state, s.state + n); r.touch.assign(s.touch, s.touch + n);
+    r.rest_div_times.assign(s.rest_div_times, s.rest_div_times + n);
+    r.pair.assign(s.pair, s.pair + n); r.fix_origin.assign(s.fix_origin, s.fix_origin + n);
+    for (auto [dst, src] : {std::make_pair(&r.radius, s.radius), std::make_pair(&r.ageb, s.ageb),
+                            std::make_pair(&r.agek, s.agek), std::make_pair(&r.ca2p, s.ca2p),
+                            std::make_pair(&r.x, s.x), std::make_pair(&r.y, s.y), std::make_pair(&r.z, s.z),
+                            std::make_pair(&r.ca2p_avg, s.ca2p_avg), std::make_pair(&r.ex_fat, s.ex_fat),
+                            std::make_pair(&r.in_fat, s.in_fat), std::make_pair(&r.spr_nat_len, s.spr_nat_len)})
+      dst->assign(src, src + n);
+    writers_.emplace_back([this, r = std::move(r), f = p.filename, b = p.binary]() {
+      try {
+        write_output(r, f, b);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(err_mu_);
+        if (!writer_err_) writer_err_ = std::current_exception();
+      }
+    });
+  }
+  Pending pending_[EPI_SNAPSHOT_SLOTS];
+  int next_slot_ = 0;
+  std::vector<std::thread> writers_;
+  std::mutex err_mu_;
+  std::exception_ptr writer_err_;
+
   epi_geometry geo_;
   epi_ctx* ctx_ = nullptr;
   long long device_ts_ = 0;

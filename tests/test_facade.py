@@ -88,7 +88,20 @@ def test_driver_matches_python_driver_and_oracle(tmp_path, oracle_mod):
     t = native.Tissue(g)
     t.load(spec, value_init=True)
     t.set_rng(draws)
-    t.step(native.Driver(0, 0, 0, ca_every, 0), steps)
+    drv = native.Driver(0, 0, 0, ca_every, 0)
+    t.step(drv, 30)
+    # snapshot 3 (before step 30), taken asynchronously by the driver: the
+    # text output() writes from a synchronous download at that step
+    c30 = t.get_cells()
+    st, ptr, conn = c30["state"], c30["conn_ptr"], c30["conn"]
+    touch = [int(st[i] == configs.ALIVE and np.any(st[conn[ptr[i]:ptr[i + 1]]] == configs.DEAD))
+             for i in range(st.size)]
+    want = "".join("%d %d %.15f %.15f %.15f %.15f %.15f %.15f %.15f %.15f %d %.15f %.15f %d %.15f %d %d\n" % (
+        i, st[i], c30["radius"][i], c30["ageb"][i], c30["agek"][i], c30["ca2p"][i], c30["x"][i], c30["y"][i],
+        c30["z"][i], c30["ca2p_avg"][i], c30["rest_div_times"][i], c30["ex_fat"][i], c30["in_fat"][i], touch[i],
+        c30["spr_nat_len"][i], c30["pair"][i], c30["fix_origin"][i]) for i in range(st.size))
+    assert (snaps / "3").read_text() == want
+    t.step(drv, steps - 30)
     assert t.error() == 0
     c = t.get_cells()
     assert c["x"].size == n

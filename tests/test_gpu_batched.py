@@ -142,7 +142,7 @@ def test_launch_shapes_do_not_change_results():
     shapes = [dict(autotune.DEFAULT, ca_group=g, xcd=x, atp_zb=zb, atp_rows=rows, b_zb=zb, b_rows=rows, ca_fuse=f,
                    map_lanes=ml, b_xcd=1 - x)
               for g, x, (zb, rows), f, ml in [(4, 1, (1, 4), 0, 4), (8, 0, (2, 2), 1, 1), (8, 1, (8, 1), 0, 4),
-                                              (16, 1, (4, 4), 1, 4), (4, 0, (1, 1), 1, 1), (4, 0, (2, 4), 2, 4)]]
+                                              (16, 1, (16, 4), 1, 4), (4, 0, (1, 1), 1, 16), (4, 0, (2, 4), 2, 4)]]
     for tu in shapes:
         print(f"shape {tu}", flush=True)
         got = run(tu)
@@ -192,3 +192,38 @@ def test_step_graph_equals_one_by_one_launches():
             assert np.array_equal(fa[k], fb[k]), (t, k)
     # divisions moved the cell counts (and with them the rounded launch sizes)
     assert any(s.n != n0 for n0, s, _, _ in graph)
+
+
+def test_ca_solve_graph_replays_equal_eager_solves():
+    """Whole Ca solves replayed from one captured graph (STEP_GRAPH 1): a
+    forced solve every 5 steps for 40 steps, the tissues compared with the
+    eager run after every 5 steps, so that each replay follows eager work
+    and read-backs of its own"""
+    from epi_amd import native
+    specs = batch_specs()[:2]
+    draws = [native.mt19937_64_uniform(700 + i, 300000) for i in range(len(specs))]
+
+    def start(graph):
+        ts = native.TissueSet([s["geometry"] for s in specs])
+        for t, s in enumerate(specs):
+            ts.load(t, s)
+            ts.set_rng(t, draws[t])
+        ts.set_tuning({"step_graph": graph})
+        return ts, [native.Driver(0, 0, 0, 5, 0) for _ in specs]
+
+    (ea, da), (gr, dg) = start(0), start(1)
+    for chunk in range(8):
+        ea.step(da, 5)
+        gr.step(dg, 5)
+        for t in range(len(specs)):
+            sa, sb = ea.status(t)[0], gr.status(t)[0]
+            assert sa.err == 0 and sb.err == 0
+            assert (sa.n, sa.ca_solves, sa.rng_pos) == (sb.n, sb.ca_solves, sb.rng_pos), (chunk, t)
+            d = compare_cells(ea.get_cells(t), gr.get_cells(t))
+            assert "n" not in d and all(v == 0 for v in d.values()), (chunk, t, d)
+            fa, fb = ea.get_fields(t), gr.get_fields(t)
+            for k in fa:
+                assert np.array_equal(fa[k], fb[k]), (chunk, t, k)
+    print(f"solves per tissue: {[gr.status(t)[0].ca_solves for t in range(len(specs))]}", flush=True)
+    ea.close()
+    gr.close()
