@@ -2,7 +2,7 @@
 
 The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
 
-  k_ext_stim   z planes per tile (1, 2, 4, 8) x 64-lane rows per block (1, 2, 4),
+  k_ext_stim   z planes per tile (1, 2, 4, 8, 16) x 64-lane rows per block (1, 2, 4),
                then its tile list in one run per XCD or not (b_xcd 0, 1)
   k_atp        the same shapes
   k_ca_cells   lanes per cell (4, 8, 16)
@@ -10,7 +10,8 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
   Ca solve     ATP of a sub-step and cells of the next in one launch (1; 2 =
                only solves of a few tissues, when the whole set runs faster
                unfused), with its own lanes per cell and z planes per thread
-  k_map_stamp  lanes per cell (1, 4)
+  k_map_stamp  lanes per cell (1, 4, 16)
+  k_sweep      32 lanes per cell or one wave per cell (sweep_wave 0, 1)
 
 With CA_FUSE 2 the fused small solves use CA_GROUP_SMALL lanes per cell
 (default 8, the best width on a latency-bound single-tissue solve).
@@ -30,7 +31,7 @@ import time
 STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8, 16), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
 DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0, "map_lanes": 1,
-           "b_xcd": 0}
+           "b_xcd": 0, "sweep_wave": 0}
 
 
 def _avg_us(ts, kernel):
@@ -76,6 +77,17 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
         ts.set_tuning({"map_lanes": ml})
         report["k_map_stamp"][f"lanes{ml}"] = _time(ts, "k_map_stamp", run_map)
     best["map_lanes"] = int(min(report["k_map_stamp"], key=report["k_map_stamp"].get)[5:])
+    ts.set_tuning(best)
+
+    def run_lists():
+        for _ in range(b_reps):
+            ts.connect_cell()
+
+    report["k_sweep"] = {}
+    for w in (0, 1):
+        ts.set_tuning({"sweep_wave": w})
+        report["k_sweep"][f"wave{w}"] = _time(ts, "k_sweep_wave" if w else "k_sweep", run_lists)
+    best["sweep_wave"] = int(min(report["k_sweep"], key=report["k_sweep"].get)[4:])
     ts.set_tuning(best)
 
     def run_b():

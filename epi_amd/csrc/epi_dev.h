@@ -114,6 +114,7 @@ struct Tis {
   int* memb6;                 // triangular adjacency, 6 per membrane cell
   // buckets
   int* bucket_of; int* bslot; int* bucket_n; int* bucket_cells;
+  int* bucket_sorted;   // the buckets in ascending cell index, written by k_bucket_sort for the sweep
   // (x, y, z, radius) of every cell after the periodic fix, written by the
   // bucket insertion: the sweep and the row assembly gather a candidate's
   // position and radius with one 32-byte load instead of four

@@ -86,7 +86,7 @@ const char* kKernelNames[K_COUNT] = {
     "k_ext_stim", "k_ca_begin", "k_ca_vox", "k_ca_cells", "k_atp", "k_ca_end", "k_sw_reset",
     "k_rd_snapshot", "k_rd_restore", "k_rd_put_gj", "k_rd_state", "k_rd_seed",
     "k_rd_step", "k_rd_gj0", "k_rd_own", "k_mech_state", "k_mech_seed", "k_mech_bwd_begin",
-    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_bucket_place", "k_wl", "k_ca_swap",
+    "k_mech_bwd_pairs", "k_mech_bwd_memb", "k_ca_fused", "k_ca_corners", "k_sweep_wave", "k_wl", "k_ca_swap",
     "k_out_pack"};
 
 // with ready_only, only the pairs whose kernels have finished (the rest stay
@@ -305,6 +305,7 @@ void carve_tissue(Carver& cv, TH& h, Tis& t) {
   t.bslot = cv.take<int>(cap);
   t.bucket_n = cv.take<int>(h.nbuckets);
   t.bucket_cells = cv.take<int>((size_t)h.nbuckets * bucket_stride(h.bcap));
+  t.bucket_sorted = cv.take<int>((size_t)h.nbuckets * bucket_stride(h.bcap));
   t.pk = cv.take<double4>(cap);
   t.div_slot = cv.take<int>(cap);
   t.div_dir = cv.take<double>(3 * cap);
@@ -1085,7 +1086,7 @@ int epi_cell_pos_periodic_fix(epi_ctx* c) {
 int epi_connect_cell(epi_ctx* c) {
   if (!c) return EPI_ERR_INVALID;
   const int nb = 1 - c->cb;
-  launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, max_n(c), max_nb(c), 0);
+  launch_connect(c->stream, c->d_tis, c->T, c->pb, nb, max_n(c), max_nb(c), 0, c->tu.sweep_wave);
   c->len_known = false;
   HIPCHK(c, hipGetLastError());
   c->cb = nb;
@@ -1230,7 +1231,7 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb, bool join = true) {
   launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
   launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
   chk(hipEventRecord(c->ev_fork, c->stream));
-  launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn);
+  launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn, c->tu.sweep_wave);
   chk(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
   launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
   launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
@@ -1251,7 +1252,7 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb, bool join = true) {
 int run_build(epi_ctx* c, int mn, int nb, int izb, bool* joined) {
   constexpr unsigned long long seg =
       (1ull << K_RENEW_FLAGS) | (1ull << K_RENEW_SCAN) | (1ull << K_RENEW_APPLY) | (1ull << K_REMOVE) |
-      (1ull << K_PAIR_DISPERSE) | (1ull << K_BUCKET_INSERT) | (1ull << K_BUCKET_SORT) | (1ull << K_BUCKET_PLACE) |
+      (1ull << K_PAIR_DISPERSE) | (1ull << K_BUCKET_INSERT) | (1ull << K_BUCKET_SORT) | (1ull << K_SWEEP_WAVE) |
       (1ull << K_SWEEP) | (1ull << K_FINISH) | (1ull << K_SNAPSHOT) | (1ull << K_ZZ_SET) | (1ull << K_ZZMAX) |
       (1ull << K_MAP_CLEAR) | (1ull << K_MAP_STAMP) | (1ull << K_MAP_HALO) | (1ull << K_EXT_STIM);
   if (!c->tu.step_graph || (g_prof.mask & seg)) {
@@ -1265,7 +1266,7 @@ int run_build(epi_ctx* c, int mn, int nb, int izb, bool* joined) {
   const int mr = std::max(mn, std::min((mn + 1023) / 1024 * 1024, mcap));
   const Tuning& tu = c->tu;
   const std::vector<long long> key = {1, c->pb, c->cb, c->cp, c->bp, nb, mr, izb, max_gsize(c), max_halo(c),
-                                      max_nx(c), max_ny(c), tu.b_zb, tu.b_rows, tu.map_lanes};
+                                      max_nx(c), max_ny(c), tu.b_zb, tu.b_rows, tu.map_lanes, tu.sweep_wave};
   return graph_run(c, key, [&] { return issue_build(c, mr, nb, izb); });
 }
 
@@ -1483,6 +1484,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_CA_GROUP_SMALL: return &tu.ca_group_small;
     case EPI_TUNE_STEP_GRAPH: return &tu.step_graph;
     case EPI_TUNE_B_XCD: return &tu.b_xcd;
+    case EPI_TUNE_SWEEP_WAVE: return &tu.sweep_wave;
     default: return nullptr;
   }
 }
@@ -1504,6 +1506,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_CA_GROUP_SMALL: ok = value == 4 || value == 8 || value == 16; break;
     case EPI_TUNE_STEP_GRAPH: ok = value == 0 || value == 1; break;
     case EPI_TUNE_B_XCD: ok = value == 0 || value == 1; break;
+    case EPI_TUNE_SWEEP_WAVE: ok = value == 0 || value == 1; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

@@ -6,7 +6,8 @@
 //   5x5x5 bucket sweep, in sweep order] ++ [partners c > X, ascending c].
 // k_bucket_insert  places cells in fixed-capacity buckets (grid_init :38-62),
 // k_bucket_sort    ranks each cell in its bucket by cell index (the serial
-// k_bucket_place   insertion order) and writes it to that slot,
+//                  insertion order) and writes it to that slot of a second
+//                  copy of the buckets,
 // k_sweep          runs connect_proc (:70-129) for every non-membrane cell:
 //                  the lower part of its own row is written directly, the
 //                  upper parts of its partners are appended with atomics,
@@ -57,10 +58,10 @@ __global__ void __launch_bounds__(256) k_bucket_insert(const Tis* __restrict__ T
   }
 }
 
-// each bucket in ascending cell index, in two launches: every cell counts
-// the cells of its bucket with a smaller index (its rank, kept in bslot),
-// then writes itself to that slot.  A bucket holds distinct cells, so the
-// ranks of its m cells are 0 .. m - 1.
+// each bucket in ascending cell index: every cell counts the cells of its
+// bucket with a smaller index (its rank, kept in bslot) and writes itself to
+// that slot of bucket_sorted, the copy the sweep reads.  A bucket holds
+// distinct cells, so the ranks of its m cells are 0 .. m - 1.
 __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) {
   const Tis& t = T[blockIdx.y];
   const int n = t.st->n;
@@ -68,7 +69,8 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
     if (t.bslot[c] < 0) continue;
     const int b = t.bucket_of[c];
     const int m = min(t.bucket_n[b], t.bcap);
-    const int* a = t.bucket_cells + (size_t)b * bucket_stride(t.bcap);
+    const size_t base = (size_t)b * bucket_stride(t.bcap);
+    const int* a = t.bucket_cells + base;
     int rank = 0;
     // four slots per 16-byte load (a bucket's slots start 16-byte aligned and
     // its stride is a multiple of 4); slots from m on are not counted
@@ -77,15 +79,7 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const Tis* __restrict__ T) 
       rank += (q.x < c) + (i + 1 < m && q.y < c) + (i + 2 < m && q.z < c) + (i + 3 < m && q.w < c);
     }
     t.bslot[c] = rank;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_bucket_place(const Tis* __restrict__ T) {
-  const Tis& t = T[blockIdx.y];
-  const int n = t.st->n;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const int r = t.bslot[c];
-    if (r >= 0) t.bucket_cells[(size_t)t.bucket_of[c] * bucket_stride(t.bcap) + r] = c;
+    t.bucket_sorted[base + rank] = c;
   }
 }
 
@@ -147,7 +141,7 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
       // >= c (ascending within the bucket)
 #pragma unroll
       for (int i = 0; i < 5; i++) {
-        const int* cells = t.bucket_cells + (size_t)(bxy + (anz - 2 + i + ANZ) % ANZ) * bucket_stride(t.bcap);
+        const int* cells = t.bucket_sorted + (size_t)(bxy + (anz - 2 + i + ANZ) % ANZ) * bucket_stride(t.bcap);
         bool done = false;
         for (int m0 = 0; m0 < sz[i] && !done; m0 += 4) {
           const int4 q = *reinterpret_cast<const int4*>(cells + m0);
@@ -182,8 +176,127 @@ __global__ void __launch_bounds__(256) k_sweep(const Tis* __restrict__ T, int pb
         for (long l = anz - 2 + ANZ; l <= anz + 2 + ANZ; l++) {
           const long b = bxy + l % ANZ;
           const int sz = min(t.bucket_n[b], t.bcap);
-          const int* cells = t.bucket_cells + (size_t)b * bucket_stride(t.bcap);
+          const int* cells = t.bucket_sorted + (size_t)b * bucket_stride(t.bcap);
           for (int m = 0; m < sz; ++m) {
+            const int o = cells[m];
+            if (c <= o) break;
+            if (sweep_hit(t, pc, rc, o)) {
+              if (off < t.ns) row[off] = o;
+              off++;
+              const int slot = atomicAdd(&t.bcnt[o], 1);
+              if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
+            }
+          }
+        }
+      }
+    }
+    if (lane == 0) t.nlo[c] = total;
+  }
+}
+
+// The same sweep with one wave per cell, for small tissues, whose few cells
+// leave the GPU mostly idle: lane g takes the buckets 2g and 2g + 1 of the
+// 125 in sweep order (bucket q is (j, k, l) = (q / 25, q / 5 % 5, q % 5)).
+// A lane issues both bucket counts, then both buckets' first four indices,
+// then those eight candidates' positions, so a sweep waits for three memory
+// round trips; buckets of more than four cells walk on four at a time.  The
+// wave's scan places each lane's partners, so the row is the same.  Twice
+// the lanes per cell cost more where the GPU is full (a launch shape chosen
+// by the tuner).
+constexpr int SWEEP_W = 64;
+
+__global__ void __launch_bounds__(256) k_sweep_wave(const Tis* __restrict__ T, int pb, int nb) {
+  const Tis& t = T[blockIdx.y];
+  Status* st = t.st;
+  const int n = st->n;
+  const int ANX = t.ANX, ANY = t.ANY, ANZ = t.ANZ;
+  const int lane = threadIdx.x & (SWEEP_W - 1);
+  const int groups = gridDim.x * (blockDim.x / SWEEP_W);
+  __shared__ int s_hit[256][SWEEP_H];
+  int* const hits = s_hit[threadIdx.x];
+  int* const col = t.col[nb];
+  const int bstride = bucket_stride(t.bcap);
+  for (int c = t.nmemb + blockIdx.x * (blockDim.x / SWEEP_W) + threadIdx.x / SWEEP_W; c < n; c += groups) {
+    const double4 pq = t.pk[c];
+    const P3 pc{pq.x, pq.y, pq.z};
+    const double rc = pq.w;
+    const long anx = (long)(pc.x / AREA_GRID);
+    const long any = (long)(pc.y / AREA_GRID);
+    const long anz = (long)(pc.z / AREA_GRID);
+    const bool bad = anx >= ANX || any >= ANY || anz >= ANZ || anx < 0 || any < 0 || anz < 0;
+    if (bad) {
+      if (lane == 0) set_err(st, ERR_OUT_OF_GRID);
+      continue;
+    }
+    long bk[2];
+    int sz[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int q = 2 * lane + u;
+      bk[u] = ((((anx - 2 + q / 25) + ANX) % ANX) * ANY + ((any - 2 + q / 5 % 5) + ANY) % ANY) * ANZ +
+              ((anz - 2 + q % 5) + ANZ) % ANZ;
+      sz[u] = q < 125 ? min(t.bucket_n[bk[u]], t.bcap) : 0;
+    }
+    int4 q4[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      q4[u] = make_int4(0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff);
+      if (sz[u] > 0) q4[u] = *reinterpret_cast<const int4*>(t.bucket_sorted + (size_t)bk[u] * bstride);
+    }
+    bool hit[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int o4[4] = {q4[u].x, q4[u].y, q4[u].z, q4[u].w};
+#pragma unroll
+      for (int v = 0; v < 4; v++) hit[u][v] = v < sz[u] && o4[v] < c && sweep_hit(t, pc, rc, o4[v]);
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int o4[4] = {q4[u].x, q4[u].y, q4[u].z, q4[u].w};
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (hit[u][v]) {
+          if (cnt < SWEEP_H) hits[cnt] = o4[v];
+          cnt++;
+        }
+      // the rest of a bucket of more than four cells whose first four lie below c
+      if (sz[u] > 4 && q4[u].w < c) {
+        const int* cells = t.bucket_sorted + (size_t)bk[u] * bstride;
+        bool done = false;
+        for (int m0 = 4; m0 < sz[u] && !done; m0 += 4) {
+          const int4 q = *reinterpret_cast<const int4*>(cells + m0);
+          const int o4b[4] = {q.x, q.y, q.z, q.w};
+          bool h[4];
+#pragma unroll
+          for (int v = 0; v < 4; v++) h[v] = m0 + v < sz[u] && o4b[v] < c && sweep_hit(t, pc, rc, o4b[v]);
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            if (done || m0 + v >= sz[u]) continue;
+            if (c <= o4b[v]) { done = true; continue; }
+            if (h[v]) {
+              if (cnt < SWEEP_H) hits[cnt] = o4b[v];
+              cnt++;
+            }
+          }
+        }
+      }
+    }
+    int total;
+    int off = group_exclusive_scan<SWEEP_W>(cnt, lane, &total);
+    if (cnt) {
+      int* const row = col + (size_t)c * t.ns;
+      if (cnt <= SWEEP_H) {
+        for (int i = 0; i < cnt; i++, off++) {
+          const int o = hits[i];
+          if (off < t.ns) row[off] = o;
+          const int slot = atomicAdd(&t.bcnt[o], 1);
+          if (slot < t.ns) t.btmp[(size_t)o * t.ns + slot] = c;
+        }
+      } else {
+        for (int u = 0; u < 2; u++) {
+          const int* cells = t.bucket_sorted + (size_t)bk[u] * bstride;
+          for (int m = 0; m < sz[u]; ++m) {
             const int o = cells[m];
             if (c <= o) break;
             if (sweep_hit(t, pc, rc, o)) {
@@ -397,21 +510,24 @@ void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int m
   EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb, fix);
 }
 
-void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n) {
+void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int sweep_wave) {
   dim3 gc((max_n + 255) / 256, ntis);
   EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gc, dim3(256), 0, s, dT);
-  EPI_LAUNCH(K_BUCKET_PLACE, k_bucket_place, gc, dim3(256), 0, s, dT);
-  dim3 gs((max_n + 7) / 8, ntis);
-  EPI_LAUNCH(K_SWEEP, k_sweep, gs, dim3(256), 0, s, dT, pb, nb);
+  if (sweep_wave) {
+    EPI_LAUNCH(K_SWEEP_WAVE, k_sweep_wave, dim3((max_n + 3) / 4, ntis), dim3(256), 0, s, dT, pb, nb);
+  } else {
+    EPI_LAUNCH(K_SWEEP, k_sweep, dim3((max_n + 7) / 8, ntis), dim3(256), 0, s, dT, pb, nb);
+  }
   dim3 gk((max_n + K_GROUPS - 1) / K_GROUPS, ntis);
   EPI_LAUNCH(K_FINISH, k_finish, gk, dim3(K_BLOCK), 0, s, dT, pb, nb);
   EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT, pb);
 }
 
-void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix) {
+void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix,
+                    int sweep_wave) {
   (void)max_buckets;
   launch_connect_insert(s, dT, ntis, pb, max_n, fix);
-  launch_connect_rest(s, dT, ntis, pb, nb, max_n);
+  launch_connect_rest(s, dT, ntis, pb, nb, max_n, sweep_wave);
 }
 
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {

@@ -1037,10 +1037,11 @@ __global__ void __launch_bounds__(256) k_atp(const Tis* __restrict__ T, CaList c
 // of sub-step i + 1: blocks [0, ncb) take the cells (one entry of the list's
 // cell-block map each), the rest walk the ATP tiles.  With xcd (the grid then
 // holds 8 ceil(ncb / 8) cell blocks, the last ones idle, and a multiple of 8
-// ATP blocks) each XCD (block id mod 8) takes one contiguous run of the cell
-// blocks and one of the ATP tiles, so that cells of one layer, their
-// neighbours and their ATP corners, and the z neighbours of ATP tiles, share
-// an L2.  Results do not depend on it.
+// ATP blocks) each XCD (block id mod 8) walks one contiguous run of the ATP
+// tiles, so that the tiles above and below a tile, which read its plane,
+// share its L2; the cell blocks stay dealt round-robin (contiguous runs of
+// cells are runs of one layer, whose cells cost alike, and left the XCDs
+// unevenly loaded).  Results do not depend on it.
 constexpr int F_BLOCK = 128;
 
 template <int CG, int ZB>
@@ -1049,14 +1050,13 @@ __global__ void __launch_bounds__(F_BLOCK) k_ca_fused(const Tis* __restrict__ T,
   int cbk = -1;   // this block's cell block, or -1 for an ATP block
   int ab = 0, an = 0;
   if (xcd) {
-    const int cps = (ncb + 7) >> 3;   // cell blocks per XCD
-    const int x = (int)(blockIdx.x & 7), sl = (int)(blockIdx.x >> 3);
-    if (sl < cps) {
-      cbk = x * cps + sl;
+    const int cblocks = (ncb + 7) & ~7;   // a multiple of 8: ATP block ids keep their XCD
+    if ((int)blockIdx.x < cblocks) {
+      cbk = (int)blockIdx.x;
       if (cbk >= ncb) return;
     } else {
-      ab = (sl - cps) * 8 + x;
-      an = (int)gridDim.x - 8 * cps;
+      ab = (int)blockIdx.x - cblocks;
+      an = (int)gridDim.x - cblocks;
     }
   } else if ((int)blockIdx.x < ncb) {
     cbk = (int)blockIdx.x;

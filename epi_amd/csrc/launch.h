@@ -14,7 +14,7 @@ enum KernelId {
   K_EXT_STIM, K_CA_BEGIN, K_CA_VOX, K_CA_CELLS, K_ATP, K_CA_END, K_SW_RESET,
   K_RD_SNAPSHOT, K_RD_RESTORE, K_RD_PUT_GJ, K_RD_STATE, K_RD_SEED, K_RD_STEP,
   K_RD_GJ0, K_RD_OWN, K_MECH_STATE, K_MECH_SEED, K_MECH_BWD_BEGIN, K_MECH_BWD_PAIRS, K_MECH_BWD_MEMB,
-  K_CA_FUSED, K_CA_CORNERS, K_BUCKET_PLACE, K_WL, K_CA_SWAP, K_OUT_PACK, K_COUNT
+  K_CA_FUSED, K_CA_CORNERS, K_SWEEP_WAVE, K_WL, K_CA_SWAP, K_OUT_PACK, K_COUNT
 };
 
 // ints of a context's work-list buffer: the offsets and per-tissue iz_bound
@@ -28,6 +28,7 @@ struct Tuning {
   int ca_group = 16;          // k_ca_cells: lanes per cell
   int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
   int b_xcd = 0;              // k_ext_stim: its tile list in one contiguous run per XCD
+  int sweep_wave = 0;         // k_sweep: 1, one wave per cell (k_sweep_wave)
   int rd_group = 8;           // k_rd_step: lanes per cell
   int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
                               // (0 never, 1 always, 2 for solves of at most 131072 cells)
@@ -72,8 +73,9 @@ void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max
 // the bucket insertion (with the periodic fix) and the rest of the build,
 // for callers that run the voxel maps beside the rest
 void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, int fix);
-void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n);
-void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix);
+void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int sweep_wave);
+void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix,
+                    int sweep_wave);
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
 // A snapshot's output record (CellManager::output's fields): a header of
 // OUT_HDR bytes whose first int is the device's cell count, then m >= n

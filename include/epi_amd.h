@@ -442,12 +442,14 @@ int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t ngrids, int3
  * averages) from HIP graphs captured once per launch key; whether a step
  * solves is still decided on the host from the step's status read-back),
  * B_XCD in {0, 1} (1: the B stencil's tile list is cut into one contiguous
- * run per XCD).  EPI_ERR_INVALID for other keys or values. */
+ * run per XCD), SWEEP_WAVE in {0, 1} (lanes per cell of the neighbour
+ * sweep: 0, 32 lanes of five buckets each; 1, one wave of two buckets per
+ * lane, for small tissues).  EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
     EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
     EPI_TUNE_MAP_LANES = 8, EPI_TUNE_CA_GROUP_SMALL = 9, EPI_TUNE_STEP_GRAPH = 10,
-    EPI_TUNE_B_XCD = 11, EPI_TUNE_COUNT = 12
+    EPI_TUNE_B_XCD = 11, EPI_TUNE_SWEEP_WAVE = 12, EPI_TUNE_COUNT = 13
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);

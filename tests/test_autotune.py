@@ -20,6 +20,9 @@ class FakeSet:
     def setup_map_lat(self):
         pass
 
+    def connect_cell(self):
+        pass
+
     def calc_ext_stim(self, zz):
         pass
 
