@@ -6,7 +6,8 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
                then its tile list in one run per XCD or not (b_xcd 0, 1)
   k_atp        the same shapes
   k_ca_cells   lanes per cell (4, 8, 16)
-  k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1)
+  k_ca_cells + k_atp   XCD-contiguous workgroup order (0, 1); in fused mode,
+               the fused launch's (its ATP tiles in one run per XCD)
   Ca solve     ATP of a sub-step and cells of the next in one launch (1; 2 =
                only solves of a few tissues, when the whole set runs faster
                unfused), with its own lanes per cell and z planes per thread
@@ -145,6 +146,13 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
     if key != "unfused":
         best["ca_fuse"] = 1
         best["ca_group"], best["atp_zb"] = (int(v) for v in key[len("fused_group"):].split("_zb"))
+        # the fused launch's own XCD order (its ATP tiles in one run per XCD)
+        ts.set_tuning(best)
+        report["xcd_fused"] = {}
+        for x in (0, 1):
+            ts.set_tuning({"xcd": x})
+            report["xcd_fused"][f"xcd{x}"] = _time(ts, "k_ca_fused", run_ca)
+        best["xcd"] = int(min(report["xcd_fused"], key=report["xcd_fused"].get)[3:])
     else:
         # the whole set solves faster unfused; a gated solve of a few
         # tissues is latency-bound and still fuses (ca_fuse 2)
