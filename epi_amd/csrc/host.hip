@@ -328,8 +328,8 @@ int check_tissue(epi_ctx* c, int t) {
 // the status read-back, split so that work issued after the copy can run
 // while the host waits for the copy alone (sync_status_end with the stream
 // still busy leaves the timing events for a later read-back to resolve)
+// after issue_build: its k_snapshot has copied the statuses into h_st
 int sync_status_begin(epi_ctx* c) {
-  HIPCHK(c, hipMemcpyAsync(c->h_st, c->d_st, sizeof(Status) * c->T, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->ev_st, c->stream));
   return EPI_OK;
 }
@@ -1231,7 +1231,7 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb, bool join = true) {
   launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
   launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
   chk(hipEventRecord(c->ev_fork, c->stream));
-  launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn, c->tu.sweep_wave);
+  launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn, c->tu.sweep_wave, c->h_st);
   chk(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
   launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
   launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);

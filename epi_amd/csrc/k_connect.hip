@@ -422,9 +422,16 @@ __global__ void __launch_bounds__(K_BLOCK) k_finish(const Tis* __restrict__ T, i
 
 // after a build: every cell's row is its own, uids snapshot the numbering,
 // and the buckets are emptied for the next build
-__global__ void __launch_bounds__(256) k_snapshot(const Tis* __restrict__ T, int pb) {
+// With h_out (pinned host memory), block 0 of each tissue first copies the
+// tissue's status there: the status is final once the build's row assembly
+// is done, so the step's read-back needs no copy launch of its own.
+__global__ void __launch_bounds__(256) k_snapshot(const Tis* __restrict__ T, int pb, Status* h_out) {
   (void)pb;
   const Tis& t = T[blockIdx.y];
+  static_assert(sizeof(Status) % 8 == 0 && sizeof(Status) / 8 <= 256, "status copied in 8-byte words");
+  if (h_out && blockIdx.x == 0 && threadIdx.x < sizeof(Status) / 8)
+    reinterpret_cast<unsigned long long*>(h_out + blockIdx.y)[threadIdx.x] =
+        reinterpret_cast<const unsigned long long*>(t.st)[threadIdx.x];
   const int n = t.st->n;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     t.src_row[c] = c;
@@ -510,7 +517,8 @@ void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int m
   EPI_LAUNCH(K_BUCKET_INSERT, k_bucket_insert, gc, dim3(256), 0, s, dT, pb, fix);
 }
 
-void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int sweep_wave) {
+void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int sweep_wave,
+                         Status* h_status) {
   dim3 gc((max_n + 255) / 256, ntis);
   EPI_LAUNCH(K_BUCKET_SORT, k_bucket_sort, gc, dim3(256), 0, s, dT);
   if (sweep_wave) {
@@ -520,14 +528,14 @@ void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb,
   }
   dim3 gk((max_n + K_GROUPS - 1) / K_GROUPS, ntis);
   EPI_LAUNCH(K_FINISH, k_finish, gk, dim3(K_BLOCK), 0, s, dT, pb, nb);
-  EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT, pb);
+  EPI_LAUNCH(K_SNAPSHOT, k_snapshot, gc, dim3(256), 0, s, dT, pb, h_status);
 }
 
 void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix,
                     int sweep_wave) {
   (void)max_buckets;
   launch_connect_insert(s, dT, ntis, pb, max_n, fix);
-  launch_connect_rest(s, dT, ntis, pb, nb, max_n, sweep_wave);
+  launch_connect_rest(s, dT, ntis, pb, nb, max_n, sweep_wave, nullptr);
 }
 
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n) {

@@ -6,6 +6,7 @@
 
 namespace epi {
 struct Tis;
+struct Status;
 
 enum KernelId {
   K_FORCES, K_FORCES_LONG, K_RENEW_FLAGS, K_RENEW_SCAN, K_RENEW_APPLY, K_REMOVE, K_PAIR_DISPERSE,
@@ -73,7 +74,9 @@ void launch_periodic_fix(hipStream_t s, const Tis* dT, int ntis, int pb, int max
 // the bucket insertion (with the periodic fix) and the rest of the build,
 // for callers that run the voxel maps beside the rest
 void launch_connect_insert(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n, int fix);
-void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int sweep_wave);
+// with h_status, k_snapshot also copies every tissue's status there
+void launch_connect_rest(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int sweep_wave,
+                         Status* h_status);
 void launch_connect(hipStream_t s, const Tis* dT, int ntis, int pb, int nb, int max_n, int max_buckets, int fix,
                     int sweep_wave);
 void launch_zzmax(hipStream_t s, const Tis* dT, int ntis, int pb, int max_n);
