@@ -611,9 +611,13 @@ int run_ca_n(epi_ctx* c, const int* d_tlist, int ntis, int n_sub) {
   for (int i = 0; i < ntis; i++) cells += c->th[d_tlist ? c->h_tlist[i] : i].n_est;
   Tuning tu = c->tu;
   if (tu.ca_fuse == 2 && cells <= FUSE_AUTO_CELLS) {
-    // a small, latency-bound solve: fused, with its own lanes per cell
+    // a small, latency-bound solve: fused, with its own lanes per cell, and
+    // one-plane ATP tiles (the planes per thread chosen for the batch-wide
+    // grid kernel roll 8 or 16 planes through one thread, which a
+    // latency-bound solve waits for)
     tu.ca_fuse = 1;
     tu.ca_group = tu.ca_group_small;
+    if (tu.a_zb > 2) tu.a_zb = 1;
   } else {
     tu.ca_fuse = tu.ca_fuse == 1;
   }

@@ -436,7 +436,7 @@ int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t ngrids, int3
  * solves of at most 131072 cells in total), MAP_LANES in
  * {1, 4, 16} (lanes per cell in the voxel-map stamp, splitting its voxels),
  * CA_GROUP_SMALL in {4, 8, 16} (lanes per cell in the solves CA_FUSE 2
- * fuses), STEP_GRAPH in {0, 1} (0, the default: epi_step launches the
+ * fuses; those solves' ATP tiles are one plane deep when ATP_ZB is above 2), STEP_GRAPH in {0, 1} (0, the default: epi_step launches the
  * renewal and neighbour-build kernels of a step one by one; 1: it replays
  * them, side stream included, and every whole Ca solve (set-up, sub-steps,
  * averages) from HIP graphs captured once per launch key; whether a step
