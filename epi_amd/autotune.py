@@ -13,6 +13,7 @@ The shapes it chooses (see epi_set_tuning in include/epi_amd.h):
                unfused), with its own lanes per cell and z planes per thread
   k_map_stamp  lanes per cell (1, 4, 16)
   k_sweep      32 lanes per cell or one wave per cell (sweep_wave 0, 1)
+  whole steps  where the side stream forks (side_fork 0, 1)
 
 With CA_FUSE 2 the fused small solves use CA_GROUP_SMALL lanes per cell
 (default 8, the best width on a latency-bound single-tissue solve).
@@ -27,12 +28,13 @@ from __future__ import annotations
 
 import itertools
 import json
+import statistics
 import time
 
 STENCIL_SHAPES = list(itertools.product((1, 2, 4, 8, 16), (1, 2, 4)))
 CA_GROUPS = (4, 8, 16)
 DEFAULT = {"b_zb": 4, "b_rows": 4, "atp_zb": 4, "atp_rows": 4, "ca_group": 16, "xcd": 0, "ca_fuse": 0, "map_lanes": 1,
-           "b_xcd": 0, "sweep_wave": 0}
+           "b_xcd": 0, "sweep_wave": 0, "side_fork": 0}
 
 
 def _avg_us(ts, kernel):
@@ -57,7 +59,7 @@ def _time(ts, kernel, run):
     return us
 
 
-def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
+def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None, step_reps=7):
     """Choose launch shapes on `ts`, a loaded TissueSet that may be advanced.
 
     Returns (tuning dict, report dict with every candidate's mean launch time
@@ -157,6 +159,27 @@ def tune(ts, drivers, warm_steps=3, b_reps=20, ca_sub=40, log=None):
         # the whole set solves faster unfused; a gated solve of a few
         # tissues is latency-bound and still fuses (ca_fuse 2)
         best["ca_fuse"] = 2
+    ts.set_tuning(best)
+
+    # where the side stream forks: a property of whole steps (what runs beside
+    # what), so whole steps are timed, the median of single steps (a step
+    # that an sw-gated solve joins is an outlier); the late fork must win by
+    # 2 % to be taken
+    report["side_fork"] = {}
+    if drivers:
+        for f in (0, 1):
+            ts.set_tuning({"side_fork": f})
+            ts.step(drivers, 1)
+            us = []
+            for _ in range(step_reps):
+                ts.synchronize()
+                s0 = time.perf_counter()
+                ts.step(drivers, 1)
+                ts.synchronize()
+                us.append(1e6 * (time.perf_counter() - s0))
+            report["side_fork"][f"fork{f}"] = statistics.median(us)
+        late = report["side_fork"]["fork1"] < 0.98 * report["side_fork"]["fork0"]
+        best["side_fork"] = 1 if late else 0
     ts.set_tuning(best)
     report["seconds"] = time.perf_counter() - t0
     if log is not None:

@@ -1234,8 +1234,12 @@ hipError_t issue_build(epi_ctx* c, int mn, int nb, int izb, bool join = true) {
   auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
   launch_renew(c->stream, c->d_tis, c->T, c->pb, c->cb, c->cp, mn);
   launch_connect_insert(c->stream, c->d_tis, c->T, c->pb, mn, 1);
-  chk(hipEventRecord(c->ev_fork, c->stream));
+  // the side stream forks after the insertion (SIDE_FORK 0: the maps run
+  // beside the sweep) or after the whole build (1: the sweep runs alone and
+  // the maps beside the next forces)
+  if (!c->tu.side_fork) chk(hipEventRecord(c->ev_fork, c->stream));
   launch_connect_rest(c->stream, c->d_tis, c->T, c->pb, nb, mn, c->tu.sweep_wave, c->h_st);
+  if (c->tu.side_fork) chk(hipEventRecord(c->ev_fork, c->stream));
   chk(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
   launch_zzmax(c->side_stream, c->d_tis, c->T, c->pb, mn);
   launch_maps(c->side_stream, c->d_tis, c->T, c->pb, mn, max_gsize(c), max_halo(c), c->tu);
@@ -1270,7 +1274,7 @@ int run_build(epi_ctx* c, int mn, int nb, int izb, bool* joined) {
   const int mr = std::max(mn, std::min((mn + 1023) / 1024 * 1024, mcap));
   const Tuning& tu = c->tu;
   const std::vector<long long> key = {1, c->pb, c->cb, c->cp, c->bp, nb, mr, izb, max_gsize(c), max_halo(c),
-                                      max_nx(c), max_ny(c), tu.b_zb, tu.b_rows, tu.map_lanes, tu.sweep_wave};
+                                      max_nx(c), max_ny(c), tu.b_zb, tu.b_rows, tu.map_lanes, tu.sweep_wave, tu.side_fork};
   return graph_run(c, key, [&] { return issue_build(c, mr, nb, izb); });
 }
 
@@ -1489,6 +1493,7 @@ int* tuning_slot(Tuning& tu, int key) {
     case EPI_TUNE_STEP_GRAPH: return &tu.step_graph;
     case EPI_TUNE_B_XCD: return &tu.b_xcd;
     case EPI_TUNE_SWEEP_WAVE: return &tu.sweep_wave;
+    case EPI_TUNE_SIDE_FORK: return &tu.side_fork;
     default: return nullptr;
   }
 }
@@ -1511,6 +1516,7 @@ int epi_set_tuning(epi_ctx* c, int32_t key, int32_t value) {
     case EPI_TUNE_STEP_GRAPH: ok = value == 0 || value == 1; break;
     case EPI_TUNE_B_XCD: ok = value == 0 || value == 1; break;
     case EPI_TUNE_SWEEP_WAVE: ok = value == 0 || value == 1; break;
+    case EPI_TUNE_SIDE_FORK: ok = value == 0 || value == 1; break;
   }
   if (!ok) return fail(c, EPI_ERR_INVALID, "tuning value out of range");
   *slot = value;

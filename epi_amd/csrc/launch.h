@@ -30,6 +30,7 @@ struct Tuning {
   int xcd = 0;                // k_ca_cells, k_atp: XCD-contiguous workgroup order
   int b_xcd = 0;              // k_ext_stim: its tile list in one contiguous run per XCD
   int sweep_wave = 0;         // k_sweep: 1, one wave per cell (k_sweep_wave)
+  int side_fork = 0;          // 1: the side stream forks after the whole build
   int rd_group = 8;           // k_rd_step: lanes per cell
   int ca_fuse = 1;            // Ca solves: ATP of a sub-step and cells of the next in one launch
                               // (0 never, 1 always, 2 for solves of at most 131072 cells)

@@ -123,7 +123,7 @@ MECH_DEFAULT = {"eps": 0.01, "k_adh": 3.0, "k_spring": 25.0, "k_bend": 0.5}
 # launch-shape keys of epi_set_tuning (include/epi_amd.h)
 TUNING_KEYS = {"b_zb": 0, "b_rows": 1, "atp_zb": 2, "atp_rows": 3, "ca_group": 4, "xcd": 5, "rd_group": 6, "ca_fuse": 7,
                "map_lanes": 8, "ca_group_small": 9, "step_graph": 10, "b_xcd": 11,
-               "sweep_wave": 12}
+               "sweep_wave": 12, "side_fork": 13}
 
 class Params(C.Structure):
     """epi_params (include/epi_amd.h): the keys of epi_intg's CalcParams

@@ -444,12 +444,16 @@ int epi_calc_ext_stim_noalloc(const epi_bfield_view* views, int32_t ngrids, int3
  * B_XCD in {0, 1} (1: the B stencil's tile list is cut into one contiguous
  * run per XCD), SWEEP_WAVE in {0, 1} (lanes per cell of the neighbour
  * sweep: 0, 32 lanes of five buckets each; 1, one wave of two buckets per
- * lane, for small tissues).  EPI_ERR_INVALID for other keys or values. */
+ * lane, for small tissues), SIDE_FORK in {0, 1} (where a step's voxel maps
+ * and B sweep fork onto the side stream: 0, after the bucket insertion, beside
+ * the neighbour sweep; 1, after the whole build, beside the next step's
+ * forces).  EPI_ERR_INVALID for other keys or values. */
 enum {
     EPI_TUNE_B_ZB = 0, EPI_TUNE_B_ROWS = 1, EPI_TUNE_ATP_ZB = 2, EPI_TUNE_ATP_ROWS = 3,
     EPI_TUNE_CA_GROUP = 4, EPI_TUNE_XCD = 5, EPI_TUNE_RD_GROUP = 6, EPI_TUNE_CA_FUSE = 7,
     EPI_TUNE_MAP_LANES = 8, EPI_TUNE_CA_GROUP_SMALL = 9, EPI_TUNE_STEP_GRAPH = 10,
-    EPI_TUNE_B_XCD = 11, EPI_TUNE_SWEEP_WAVE = 12, EPI_TUNE_COUNT = 13
+    EPI_TUNE_B_XCD = 11, EPI_TUNE_SWEEP_WAVE = 12, EPI_TUNE_SIDE_FORK = 13,
+    EPI_TUNE_COUNT = 14
 };
 int epi_set_tuning(epi_ctx* ctx, int32_t key, int32_t value);
 int epi_get_tuning(epi_ctx* ctx, int32_t key, int32_t* value);

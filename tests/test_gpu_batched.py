@@ -140,7 +140,7 @@ def test_launch_shapes_do_not_change_results():
 
     ref = run(autotune.DEFAULT)
     shapes = [dict(autotune.DEFAULT, ca_group=g, xcd=x, atp_zb=zb, atp_rows=rows, b_zb=zb, b_rows=rows, ca_fuse=f,
-                   map_lanes=ml, b_xcd=1 - x, sweep_wave=int(ml == 4))
+                   map_lanes=ml, b_xcd=1 - x, sweep_wave=int(ml == 4), side_fork=x)
               for g, x, (zb, rows), f, ml in [(4, 1, (1, 4), 0, 4), (8, 0, (2, 2), 1, 1), (8, 1, (8, 1), 0, 4),
                                               (16, 1, (16, 4), 1, 4), (4, 0, (1, 1), 1, 16), (4, 0, (2, 4), 2, 4), (8, 1, (8, 4), 2, 1)]]
     for tu in shapes:
